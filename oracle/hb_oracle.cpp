@@ -1,0 +1,920 @@
+// FP64 CPU oracle of HydroBEM's low-order frequency-domain solve (test infrastructure only).
+//
+// This file is the checker of the HIP library in hydrobem_amd/: it restates, in plain C++ and
+// double precision, the algorithm of the reference's diagnostic double mode (HB_DIAG_DOUBLE):
+//   - panel rule            crates/hydrobem/src/geometry.rs:120-199            (make_panel)
+//   - panel geometry        crates/hydrobem/src/panel_integrals.rs:145-217     (make_panel_geom)
+//   - Gauss-Legendre        crates/hydrobem/src/quadrature.rs:17-59
+//   - Rankine closed forms  crates/hydrobem/src/panel_integrals.rs:226-275     (diag: to 1000 radii)
+//   - log singularity       crates/hydrobem/src/panel_integrals.rs:380-450, Adaptive :69-140
+//   - static rows           crates/hydrobem/src/lo_static.rs:47-119            (infinite depth)
+//   - wave kernel           crates/hydrobem/src/green_inf.rs:316-368           (identity form)
+//   - pair kernel / apply   crates/hydrobem/src/low_order.rs:627-648, 717-769
+//   - assembly              crates/hydrobem/src/lo_frequency.rs:399-537        (G = 1, wet panels)
+//   - boundary values       crates/hydrobem/src/lo_frequency.rs:214-273, low_order.rs:1401-1485
+//   - RHS product + LU      crates/hydrobem/src/lo_frequency.rs:823-955, linalg.rs:134-252
+//   - forces, Haskind       crates/hydrobem/src/lo_frequency.rs:1039-1171
+//
+// Where the reference reads generated Chebyshev tables (V, V_X, Yt, Yt' of green_inf.rs), this
+// oracle evaluates the same functions directly from their definitions (scripts/green_reference.py
+// docstring): Yt by its power series or libm's y0/j0, V and V_X by Gauss-Legendre quadrature of
+// the defining integral after the substitution t = X sinh u, H0 and H1 by their integral
+// representations. It is therefore more accurate than the tables (2e-10) and independent of any
+// table layout. Everything is double precision; nothing here is shipped or called by the product.
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "hb_oracle.h"
+
+typedef std::complex<double> cd;
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+const double GAMMA = 0.5772156649015329;  // constants.rs:4
+const double LOG2 = 0.69314718055994530942;
+
+struct V3 {
+  double x[3];
+  double& operator[](int i) { return x[i]; }
+  double operator[](int i) const { return x[i]; }
+};
+inline V3 v3(double a, double b, double c) { V3 r; r.x[0] = a; r.x[1] = b; r.x[2] = c; return r; }
+inline V3 operator+(V3 a, V3 b) { return v3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+inline V3 operator-(V3 a, V3 b) { return v3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline V3 operator*(double s, V3 a) { return v3(s * a[0], s * a[1], s * a[2]); }
+inline double dot(V3 a, V3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline V3 cross(V3 a, V3 b) {
+  return v3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+// ------------------------------------------------------------------ quadrature.rs:17-59
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  double nf = n;
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(PI * (i + 0.75) / (nf + 0.5));
+    double dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = nf * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) <= 1e-16) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; ++k) {
+      double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = nf * (z * p1 - p0) / (z * z - 1.0);
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    double wi = 2.0 / ((1.0 - z * z) * dp * dp);
+    w[i] = wi;
+    w[n - 1 - i] = wi;
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+struct GaussCache {
+  std::vector<double> x[17], w[17];
+  GaussCache() {
+    for (int n = 1; n <= 16; ++n) gauss_legendre(n, x[n], w[n]);
+  }
+};
+const GaussCache& gauss() {
+  static GaussCache c;
+  return c;
+}
+
+// ------------------------------------------------------------------ geometry.rs:120-199
+const double COINCIDENCE_TOL = 1e-6;
+const double CONVEXITY_TOL = 1e-6;
+
+struct Panel {
+  V3 raw[4], proj[4], poly[4], normal, centroid;
+  int distinct = 0;
+  double area = 0, diameter = 0;
+};
+
+int make_panel(const double* v12, Panel& out) {
+  for (int i = 0; i < 12; ++i)
+    if (!std::isfinite(v12[i])) return ORC_ERR_VALUE;
+  for (int i = 0; i < 4; ++i) out.raw[i] = v3(v12[3 * i], v12[3 * i + 1], v12[3 * i + 2]);
+  double diameter = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = i + 1; j < 4; ++j) diameter = std::max(diameter, norm(out.raw[i] - out.raw[j]));
+  out.diameter = diameter;
+  if (diameter == 0.0) return ORC_ERR_VALUE;
+  double tol = COINCIDENCE_TOL * diameter;
+  int keep[4], m = 0;
+  for (int i = 0; i < 4; ++i)
+    if (norm(out.raw[i] - out.raw[(i + 1) % 4]) > tol) keep[m++] = i;
+  if (m < 3) return ORC_ERR_VALUE;
+  for (int a = 0; a < m; ++a)
+    for (int b = a + 1; b < m; ++b)
+      if (norm(out.raw[keep[a]] - out.raw[keep[b]]) <= tol) return ORC_ERR_VALUE;
+  V3 c = 0.25 * (out.raw[0] + out.raw[1] + out.raw[2] + out.raw[3]);
+  V3 n = cross(out.raw[2] - out.raw[0], out.raw[3] - out.raw[1]);
+  double nn = norm(n);
+  if (0.5 * nn <= COINCIDENCE_TOL * diameter * diameter) return ORC_ERR_VALUE;
+  V3 nh = (1.0 / nn) * n;
+  for (int i = 0; i < 4; ++i) out.proj[i] = out.raw[i] - dot(out.raw[i] - c, nh) * nh;
+  out.distinct = m;
+  for (int a = 0; a < m; ++a) out.poly[a] = out.proj[keep[a]];
+  for (int a = 0; a < m; ++a) {
+    V3 e1 = out.poly[(a + 1) % m] - out.poly[a];
+    V3 e2 = out.poly[(a + 2) % m] - out.poly[(a + 1) % m];
+    double s = dot(cross(e1, e2), nh) / (norm(e1) * norm(e2));
+    if (!(s > CONVEXITY_TOL)) return ORC_ERR_VALUE;
+  }
+  double area = 0.0;
+  V3 moment = v3(0, 0, 0);
+  for (int a = 1; a < m - 1; ++a) {
+    double t = 0.5 * dot(cross(out.poly[a] - out.poly[0], out.poly[a + 1] - out.poly[0]), nh);
+    area += t;
+    moment = moment + (t / 3.0) * (out.poly[0] + out.poly[a] + out.poly[a + 1]);
+  }
+  out.area = area;
+  out.centroid = (1.0 / area) * moment;
+  out.normal = nh;
+  return ORC_OK;
+}
+
+// ------------------------------------------------------------------ panel_integrals.rs:22-217
+struct PanelGeom {
+  int nv = 0;
+  V3 v[4], c, n;
+  double area = 0, radius = 0;
+  double elen[4];
+  V3 em[4];
+  V3 gx[9];
+  double gw[9];
+  V3 q4[4];  // the four (projected) corners of the bilinear map of the 3 x 3 rule
+};
+
+// patch_geometry.rs:14-22
+void bilinear(const V3* vt, double u, double v, V3& x, double& jac) {
+  double n0 = 0.25 * (1.0 - u) * (1.0 - v), n1 = 0.25 * (1.0 + u) * (1.0 - v);
+  double n2 = 0.25 * (1.0 + u) * (1.0 + v), n3 = 0.25 * (1.0 - u) * (1.0 + v);
+  x = n0 * vt[0] + n1 * vt[1] + n2 * vt[2] + n3 * vt[3];
+  V3 xu = (0.25 * (1.0 - v)) * (vt[1] - vt[0]) + (0.25 * (1.0 + v)) * (vt[2] - vt[3]);
+  V3 xv = (0.25 * (1.0 - u)) * (vt[3] - vt[0]) + (0.25 * (1.0 + u)) * (vt[2] - vt[1]);
+  jac = norm(cross(xu, xv));
+}
+
+PanelGeom make_panel_geom(const V3* q, V3 normal, V3 centroid, double area) {
+  PanelGeom g;
+  g.n = normal;
+  g.c = centroid;
+  g.area = area;
+  double diam = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = i + 1; j < 4; ++j) diam = std::max(diam, norm(q[i] - q[j]));
+  for (int i = 0; i < 4; ++i)
+    if (norm(q[i] - q[(i + 1) % 4]) > COINCIDENCE_TOL * diam) g.v[g.nv++] = q[i];
+  for (int i = 0; i < g.nv; ++i) g.radius = std::max(g.radius, norm(g.v[i] - centroid));
+  for (int e = 0; e < g.nv; ++e) {
+    V3 edge = g.v[(e + 1) % g.nv] - g.v[e];
+    g.elen[e] = norm(edge);
+    g.em[e] = (1.0 / g.elen[e]) * cross(edge, normal);
+  }
+  for (int i = 0; i < 4; ++i) g.q4[i] = q[i];
+  const GaussCache& G = gauss();
+  double sum = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double jac;
+      bilinear(q, G.x[3][i], G.x[3][j], g.gx[3 * i + j], jac);
+      g.gw[3 * i + j] = G.w[3][i] * G.w[3][j] * jac;
+      sum += g.gw[3 * i + j];
+    }
+  for (int k = 0; k < 9; ++k) g.gw[k] *= area / sum;
+  return g;
+}
+
+// the bilinear map of the distinct vertices (panel_integrals.rs:52-56)
+void panel_map(const PanelGeom& p, V3* m) {
+  m[0] = p.v[0];
+  m[1] = p.v[1];
+  m[2] = p.v[2];
+  m[3] = p.nv == 4 ? p.v[3] : p.v[2];
+}
+
+// panel_integrals.rs:226-275 with the diagnostic-mode far switch at 1000 radii (:231)
+void rankine(const PanelGeom& p, V3 x, double& S, double& D) {
+  V3 rc = x - p.c;
+  double dist = norm(rc);
+  if (dist > 1000.0 * p.radius) {
+    S = 0.0;
+    D = 0.0;
+    for (int k = 0; k < 9; ++k) {
+      V3 r = x - p.gx[k];
+      double ir = 1.0 / norm(r);
+      S += p.gw[k] * ir;
+      D += p.gw[k] * dot(p.n, r) * ir * ir * ir;
+    }
+    return;
+  }
+  double w = dot(rc, p.n);
+  int nv = p.nv;
+  V3 av[4];
+  double r[4];
+  for (int i = 0; i < nv; ++i) {
+    av[i] = p.v[i] - x;
+    r[i] = norm(av[i]);
+  }
+  double omega = 0.0;
+  if (std::fabs(w) > 1e-12 * p.radius) {
+    for (int i = 1; i < nv - 1; ++i) {
+      V3 a = av[0], b = av[i], c = av[i + 1];
+      double num = dot(a, cross(b, c));
+      double den = r[0] * r[i] * r[i + 1] + dot(a, b) * r[i + 1] + dot(a, c) * r[i] + dot(b, c) * r[0];
+      omega -= 2.0 * std::atan2(num, den);
+    }
+  }
+  double s = 0.0;
+  for (int e = 0; e < nv; ++e) {
+    int f = (e + 1) % nv;
+    double d = dot(av[e], p.em[e]);
+    if (std::fabs(d) <= 1e-14 * p.radius) continue;
+    double q = r[e] * r[f] + dot(av[e], av[f]);
+    if (!(q > 0.0)) continue;
+    double sm = r[e] + r[f] + p.elen[e];
+    s += d * std::log(sm * sm / (2.0 * q));
+  }
+  S = s - w * omega;
+  D = omega;
+}
+
+// panel_integrals.rs:281-286
+inline void psi_terms(V3 xi, V3 p, V3 n, double& psi, double& dpsi) {
+  V3 d = xi - p;
+  double R = norm(d);
+  double den = R - d[2];
+  psi = std::log(den);
+  dpsi = (dot(d, n) / R - n[2]) / den;
+}
+
+// panel_integrals.rs:69-140: adaptive quadrisection of the 4 x 4 Gauss rule
+struct AdaptiveLog {
+  const V3* map;
+  V3 image, n;
+  double tol_l, tol_d;
+  void cell(double u0, double u1, double v0, double v1, double* s) const {
+    const GaussCache& G = gauss();
+    s[0] = s[1] = 0.0;
+    double hu = 0.5 * (u1 - u0), hv = 0.5 * (v1 - v0);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        V3 xi;
+        double jac;
+        bilinear(map, u0 + hu * (G.x[4][i] + 1.0), v0 + hv * (G.x[4][j] + 1.0), xi, jac);
+        double ww = G.w[4][i] * G.w[4][j] * hu * hv * jac;
+        double psi, dpsi;
+        psi_terms(xi, image, n, psi, dpsi);
+        s[0] += ww * psi;
+        s[1] += ww * dpsi;
+      }
+  }
+  void adapt(double u0, double u1, double v0, double v1, const double* prev, int depth, double* acc) const {
+    double um = 0.5 * (u0 + u1), vm = 0.5 * (v0 + v1);
+    double c[4][2];
+    cell(u0, um, v0, vm, c[0]);
+    cell(um, u1, v0, vm, c[1]);
+    cell(um, u1, vm, v1, c[2]);
+    cell(u0, um, vm, v1, c[3]);
+    double sum[2];
+    for (int k = 0; k < 2; ++k) sum[k] = c[0][k] + c[1][k] + c[2][k] + c[3][k];
+    bool done = std::fabs(sum[0] - prev[0]) <= tol_l && std::fabs(sum[1] - prev[1]) <= tol_d;
+    if (depth >= 14 || done) {
+      acc[0] += sum[0];
+      acc[1] += sum[1];
+      return;
+    }
+    const double us[4][2] = {{u0, um}, {um, u1}, {um, u1}, {u0, um}};
+    const double vs[4][2] = {{v0, vm}, {v0, vm}, {vm, v1}, {vm, v1}};
+    for (int q = 0; q < 4; ++q) adapt(us[q][0], us[q][1], vs[q][0], vs[q][1], c[q], depth + 1, acc);
+  }
+};
+
+// panel_integrals.rs:380-450 (diagnostic mode: no 3 x 3 far rule, :382)
+void log_singularity(const PanelGeom& p, V3 image, double& L, double& Ld) {
+  double dist = norm(image - p.c);
+  V3 map[4];
+  panel_map(p, map);
+  double top = p.v[0][2];
+  for (int i = 1; i < p.nv; ++i) top = std::max(top, p.v[i][2]);
+  double r = dist / p.radius;
+  double clearance = (image[2] - top) / p.radius;
+  int order = 0;
+  if (clearance >= 0.5 || r >= 2.0) {
+    order = r >= 6.0 ? 4 : r >= 3.0 ? 5 : r >= 2.0 ? 6 : r >= 1.5 ? 8 : r >= 1.0 ? 10 : 0;
+  }
+  const GaussCache& G = gauss();
+  if (order > 0) {
+    L = 0.0;
+    Ld = 0.0;
+    for (int i = 0; i < order; ++i)
+      for (int j = 0; j < order; ++j) {
+        V3 xi;
+        double jac;
+        bilinear(map, G.x[order][i], G.x[order][j], xi, jac);
+        double psi, dpsi;
+        psi_terms(xi, image, p.n, psi, dpsi);
+        double ww = G.w[order][i] * G.w[order][j] * jac;
+        L += ww * psi;
+        Ld += ww * dpsi;
+      }
+    return;
+  }
+  double dmin = std::max(dist - p.radius, 1e-3 * p.radius);
+  double scale = std::fabs(std::log(dist + p.radius)) + 1.0;
+  AdaptiveLog a;
+  a.map = map;
+  a.image = image;
+  a.n = p.n;
+  a.tol_l = 1e-13 * p.area * scale;
+  a.tol_d = 1e-13 * p.area / dmin;
+  double whole[2], acc[2] = {0.0, 0.0};
+  a.cell(-1.0, 1.0, -1.0, 1.0, whole);
+  a.adapt(-1.0, 1.0, -1.0, 1.0, whole, 0, acc);
+  L = acc[0];
+  Ld = acc[1];
+}
+
+// lo_static.rs:47-55
+bool log_pair(const PanelGeom& p, V3 x, double tol) {
+  double dx = x[0] - p.c[0], dy = x[1] - p.c[1];
+  double zz = std::fabs(x[2] + p.c[2]);
+  double d = std::sqrt(dx * dx + dy * dy + zz * zz) + zz;
+  return d * d * tol < 0.6 * p.radius * p.radius;
+}
+
+// lo_static.rs:59-119, infinite depth, wetted panels (no surface panels in z = 0)
+void static_entry(const PanelGeom& p, V3 x, bool logint, double tol, double* e) {
+  double s0, d0, sb, db;
+  rankine(p, x, s0, d0);
+  rankine(p, v3(x[0], x[1], -x[2]), sb, db);
+  e[0] = s0 + sb;
+  e[1] = d0 + db;
+  e[2] = 0.0;
+  e[3] = 0.0;
+  if (logint && log_pair(p, x, tol)) {
+    V3 xb = v3(x[0], x[1], -x[2]);
+    double L, Ld, psi, dpsi;
+    log_singularity(p, xb, L, Ld);
+    psi_terms(p.c, xb, p.n, psi, dpsi);
+    e[2] = L - p.area * psi;
+    e[3] = Ld - p.area * dpsi;
+  }
+}
+
+// ------------------------------------------------------------------ special functions
+// Struve H0, H1 (A&S 12.1.7): H0 = (2/pi) int_0^{pi/2} sin(x cos t) dt,
+// H1 = (2x/pi) int_0^{pi/2} sin(x cos t) sin^2 t dt; composite Gauss-Legendre, exact to rounding.
+void struve01(double x, double& h0, double& h1) {
+  const GaussCache& G = gauss();
+  int panels = 8 + (int)(x / 3.0);
+  double a = 0.5 * PI / panels;
+  double s0 = 0.0, s1 = 0.0;
+  for (int q = 0; q < panels; ++q) {
+    double t0 = q * a;
+    for (int i = 0; i < 16; ++i) {
+      double t = t0 + 0.5 * a * (G.x[16][i] + 1.0);
+      double w = 0.5 * a * G.w[16][i];
+      double sn = std::sin(x * std::cos(t));
+      double st = std::sin(t);
+      s0 += w * sn;
+      s1 += w * sn * st * st;
+    }
+  }
+  h0 = 2.0 / PI * s0;
+  h1 = 2.0 * x / PI * s1;
+}
+
+// 1 - J0(x), without cancellation for small x
+double one_minus_j0(double x) {
+  if (x < 1.0) {
+    double q = 0.25 * x * x, term = 1.0, s = 0.0;
+    for (int k = 1; k < 30; ++k) {
+      term *= -q / ((double)k * k);
+      s -= term;
+      if (std::fabs(term) < 1e-18 * std::fabs(s)) break;
+    }
+    return s;
+  }
+  return 1.0 - ::j0(x);
+}
+
+// Yt = Y0 - (2/pi)(log(x/2) + gamma) J0 and its derivative (green_reference.py Yt)
+void yt_and_derivative(double x, double& yt, double& dyt) {
+  if (x == 0.0) {
+    yt = 0.0;
+    dyt = 0.0;
+    return;
+  }
+  if (x <= 8.0) {
+    // (2/pi) sum_{k>=1} (-1)^{k+1} H_k q^k / (k!)^2, q = x^2/4
+    double q = 0.25 * x * x, pk = 1.0, hk = 0.0, s = 0.0, ds = 0.0;
+    for (int k = 1; k < 80; ++k) {
+      pk *= q / ((double)k * k);  // q^k / (k!)^2
+      hk += 1.0 / k;
+      double sg = (k % 2) ? 1.0 : -1.0;
+      double t = sg * hk * pk;
+      s += t;
+      ds += t * k;  // d/dq q^k = k q^{k-1}; times dq/dx = x/2 below
+      if (pk * hk < 1e-19 * std::fabs(s) && k > 4) break;
+    }
+    yt = 2.0 / PI * s;
+    dyt = 2.0 / PI * ds / q * 0.5 * x;
+    return;
+  }
+  double j0v = ::j0(x), j1v = ::j1(x), y0v = ::y0(x), y1v = ::y1(x);
+  double L = std::log(0.5 * x) + GAMMA;
+  yt = y0v - 2.0 / PI * L * j0v;
+  dyt = -y1v - 2.0 / PI * (j0v / x - L * j1v);
+}
+
+// Ein(y) = int_0^y (e^t - 1)/t dt = sum y^k / (k k!)
+double ein(double y) {
+  double term = 1.0, s = 0.0;
+  for (int k = 1; k < 400; ++k) {
+    term *= y / k;
+    double t = term / k;
+    s += t;
+    if (t < 1e-18 * s) break;
+  }
+  return s;
+}
+
+// N(X, Y) = pi H0(X)/2 + int_0^Y (e^t - J0(X)) / sqrt(X^2 + t^2) dt and dN/dX, X > 0
+// (green_reference.py V_VX: V = exp(-Y) N / R0), by Gauss-Legendre in u, t = X sinh u.
+void n_and_nx(double X, double Y, double& N, double& NX) {
+  double h0, h1;
+  struve01(X, h0, h1);
+  double j1v = ::j1(X);
+  double omj0 = one_minus_j0(X);
+  N = 0.5 * PI * h0;
+  NX = 1.0 - 0.5 * PI * h1;  // (pi/2) H0' with H0' = 2/pi - H1
+  if (Y <= 0.0) return;
+  double U = std::asinh(Y / X);
+  const GaussCache& G = gauss();
+  double i1 = 0.0, i2 = 0.0;  // int expm1(X sinh u) du, int expm1(X sinh u) / (X cosh^2 u) du
+  double a = 0.0;
+  while (a < U) {
+    double w = std::min(1.0, U - a);
+    while (w > 1e-3 && X * std::cosh(a + w) * w > 0.5) w *= 0.5;
+    double b = std::min(U, a + w);
+    double h = 0.5 * (b - a);
+    for (int i = 0; i < 16; ++i) {
+      double u = a + h * (G.x[16][i] + 1.0);
+      double sh = std::sinh(u), ch = std::cosh(u);
+      double e = std::expm1(X * sh);
+      i1 += h * G.w[16][i] * e;
+      i2 += h * G.w[16][i] * e / (X * ch * ch);
+    }
+    a = b;
+  }
+  double R0 = std::sqrt(X * X + Y * Y);
+  N += i1 + U * omj0;
+  NX += j1v * U - (i2 + omj0 / X * (Y / R0));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ wave kernel (green_inf.rs:316-368)
+extern "C" int orc_wave_kernel(double X, double Y, double* out) {
+  if (!(X >= 0.0) || !(Y >= 0.0) || (X == 0.0 && Y == 0.0)) return ORC_ERR_ARGUMENT;
+  double R0 = std::sqrt(X * X + Y * Y);
+  double eY = std::exp(-Y);
+  double j0v = ::j0(X), j1v = ::j1(X);
+  double yt, dyt;
+  yt_and_derivative(X, yt, dyt);
+  double V, VX;
+  if (X == 0.0) {
+    V = eY * ein(Y) / Y;
+    VX = 0.0;
+  } else {
+    double N, NX;
+    n_and_nx(X, Y, N, NX);
+    V = eY * N / R0;
+    VX = eY * (NX / R0 - N * X / (R0 * R0 * R0));
+  }
+  double lg = std::log(Y + R0) + GAMMA - LOG2;
+  double F = -2.0 * eY * j0v * lg - PI * eY * yt - 2.0 * R0 * V;
+  double FX = 2.0 * eY * j1v * lg - 2.0 * eY * j0v * X / (R0 * (Y + R0)) - PI * eY * dyt -
+              2.0 * (X / R0) * V - 2.0 * R0 * VX;
+  out[0] = F;
+  out[1] = FX;
+  out[2] = -F - 2.0 / R0;
+  out[3] = -2.0 * PI * eY * j0v;
+  out[4] = 2.0 * PI * eY * j1v;
+  out[5] = -out[3];
+  out[6] = eY * j0v;
+  return ORC_OK;
+}
+
+extern "C" int orc_v_vx(double X, double Y, double* out) {
+  if (!(X >= 0.0) || !(Y >= 0.0) || (X == 0.0 && Y == 0.0)) return ORC_ERR_ARGUMENT;
+  double R0 = std::sqrt(X * X + Y * Y);
+  double eY = std::exp(-Y);
+  if (X == 0.0) {
+    out[0] = eY * ein(Y) / Y;
+    out[1] = 0.0;
+    return ORC_OK;
+  }
+  double N, NX;
+  n_and_nx(X, Y, N, NX);
+  out[0] = eY * N / R0;
+  out[1] = eY * (NX / R0 - N * X / (R0 * R0 * R0));
+  return ORC_OK;
+}
+
+extern "C" int orc_yt(double X, double* out) {
+  yt_and_derivative(X, out[0], out[1]);
+  return ORC_OK;
+}
+
+extern "C" int orc_struve01(double x, double* out) {
+  struve01(x, out[0], out[1]);
+  return ORC_OK;
+}
+
+extern "C" int orc_gauss_legendre(int n, double* x, double* w) {
+  if (n < 1 || n > 64) return ORC_ERR_ARGUMENT;
+  std::vector<double> xs, ws;
+  gauss_legendre(n, xs, ws);
+  for (int i = 0; i < n; ++i) {
+    x[i] = xs[i];
+    w[i] = ws[i];
+  }
+  return ORC_OK;
+}
+
+extern "C" int orc_make_panel(const double* v12, double* out) {
+  Panel p;
+  int st = make_panel(v12, p);
+  if (st) return st;
+  // out: centroid[3], normal[3], area, distinct, proj[12]
+  for (int i = 0; i < 3; ++i) {
+    out[i] = p.centroid[i];
+    out[3 + i] = p.normal[i];
+  }
+  out[6] = p.area;
+  out[7] = p.distinct;
+  for (int i = 0; i < 4; ++i)
+    for (int d = 0; d < 3; ++d) out[8 + 3 * i + d] = p.proj[i][d];
+  return ORC_OK;
+}
+
+static int geom_of(const double* v12, PanelGeom& g) {
+  Panel p;
+  int st = make_panel(v12, p);
+  if (st) return st;
+  g = make_panel_geom(p.proj, p.normal, p.centroid, p.area);
+  return ORC_OK;
+}
+
+extern "C" int orc_rankine(const double* v12, const double* x, double* out) {
+  PanelGeom g;
+  int st = geom_of(v12, g);
+  if (st) return st;
+  rankine(g, v3(x[0], x[1], x[2]), out[0], out[1]);
+  return ORC_OK;
+}
+
+extern "C" int orc_log_singularity(const double* v12, const double* image, double* out) {
+  PanelGeom g;
+  int st = geom_of(v12, g);
+  if (st) return st;
+  log_singularity(g, v3(image[0], image[1], image[2]), out[0], out[1]);
+  return ORC_OK;
+}
+
+// ------------------------------------------------------------------ the problem
+struct OrcProblem {
+  int N = 0, nb = 0;
+  std::vector<PanelGeom> pg;
+  std::vector<int> body;
+  std::vector<double> nj;  // [N][6]
+  double rho = 1025.0, grav = 9.81, depth = INFINITY;
+  bool logint = true;
+  double logtol = 1e-4;
+  std::vector<double> st;  // statics [k][j][4]: rs, rd, dl, dd
+  bool have_statics = false;
+};
+
+extern "C" void* orc_problem_create(const double* vertices, int n, const int* body, int nbodies,
+                                    const double* rotation_centres, double rho, double grav, double depth,
+                                    int logint, double logtol, int* status) {
+  OrcProblem* p = new OrcProblem();
+  p->N = n;
+  p->nb = nbodies;
+  p->rho = rho;
+  p->grav = grav;
+  p->depth = depth;
+  p->logint = logint != 0;
+  p->logtol = logtol;
+  p->pg.resize(n);
+  p->body.resize(n);
+  p->nj.resize(6 * (size_t)n);
+  for (int j = 0; j < n; ++j) {
+    Panel pn;
+    int st = make_panel(vertices + 12 * (size_t)j, pn);
+    if (st) {
+      *status = st;
+      delete p;
+      return nullptr;
+    }
+    p->pg[j] = make_panel_geom(pn.proj, pn.normal, pn.centroid, pn.area);
+    int b = body ? body[j] : 0;
+    p->body[j] = b;
+    // lo_geometry.rs:799-802: n_j = (n, (c - R) x n)
+    V3 rc = v3(rotation_centres[3 * b], rotation_centres[3 * b + 1], rotation_centres[3 * b + 2]);
+    V3 mom = cross(pn.centroid - rc, pn.normal);
+    for (int d = 0; d < 3; ++d) {
+      p->nj[6 * j + d] = pn.normal[d];
+      p->nj[6 * j + 3 + d] = mom[d];
+    }
+  }
+  *status = ORC_OK;
+  return p;
+}
+
+extern "C" void orc_problem_free(void* h) { delete (OrcProblem*)h; }
+
+static void ensure_statics(OrcProblem* p) {
+  if (p->have_statics) return;
+  int N = p->N;
+  p->st.assign((size_t)N * N * 4, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < N; ++k) {
+    V3 x = p->pg[k].c;
+    for (int j = 0; j < N; ++j) static_entry(p->pg[j], x, p->logint, p->logtol, &p->st[((size_t)k * N + j) * 4]);
+  }
+  p->have_statics = true;
+}
+
+extern "C" int orc_statics(void* h, double* out) {
+  OrcProblem* p = (OrcProblem*)h;
+  ensure_statics(p);
+  std::memcpy(out, p->st.data(), p->st.size() * sizeof(double));
+  return ORC_OK;
+}
+
+namespace {
+
+struct Kern {
+  double Wr = 0, WrR = 0, WrZ = 0, Wrz = 0, Wi = 0, WiR = 0, WiZ = 0, Wiz = 0;
+};
+
+// low_order.rs:627-648 and :717-730 (infinite depth, ordinary frequency)
+Kern pair_kernel(double K, V3 x, V3 xi, double& R) {
+  double dx = x[0] - xi[0], dy = x[1] - xi[1];
+  R = std::sqrt(dx * dx + dy * dy);
+  Kern k;
+  if (R == 0.0 && x[2] == 0.0 && xi[2] == 0.0) {
+    k.Wr = -2.0 * K * (GAMMA - LOG2 + std::log(K));
+    k.Wi = -2.0 * PI * K;
+    return k;
+  }
+  double w[7];
+  orc_wave_kernel(K * R, -K * (x[2] + xi[2]), w);
+  double K2 = K * K;
+  k.Wr = K * w[0];
+  k.WrR = K2 * w[1];
+  k.WrZ = -K2 * w[2];
+  k.Wrz = k.WrZ;
+  k.Wi = K * w[3];
+  k.WiR = K2 * w[4];
+  k.WiZ = -K2 * w[5];
+  k.Wiz = k.WiZ;
+  return k;
+}
+
+// low_order.rs:738-769 (wetted panels: the dipole column is D)
+void apply_kernel(const OrcProblem* p, int row_k, int j, V3 x, double K, double logc, const Kern& k, double R,
+                  bool swapped, cd& S, cd& D) {
+  const PanelGeom& pj = p->pg[j];
+  double a = R > 0.0 ? ((x[0] - pj.c[0]) * pj.n[0] + (x[1] - pj.c[1]) * pj.n[1]) / R : 0.0;
+  const double* e = &p->st[((size_t)row_k * p->N + j) * 4];
+  double A = pj.area;
+  double rz = swapped ? k.Wrz : k.WrZ;
+  double iz = swapped ? k.Wiz : k.WiZ;
+  S = cd(e[0] + A * k.Wr - 2.0 * K * logc * e[2], A * k.Wi);
+  D = cd(e[1] + A * (-k.WrR * a + rz * pj.n[2]) - 2.0 * K * logc * e[3], A * (-k.WiR * a + iz * pj.n[2]));
+}
+
+// lo_frequency.rs:399-537 for G = 1: S(k, n), A(k, n) = 2 pi delta_kn - D(k, n), column-major
+void assemble(OrcProblem* p, double omega, std::vector<cd>& S, std::vector<cd>& Am) {
+  ensure_statics(p);
+  int N = p->N;
+  double K = omega * omega / p->grav;
+  double logc = p->logint ? 1.0 : 0.0;
+  S.assign((size_t)N * N, cd(0, 0));
+  Am.assign((size_t)N * N, cd(0, 0));
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int k = 0; k < N; ++k) {
+    V3 xk = p->pg[k].c;
+    for (int n = k; n < N; ++n) {
+      V3 xn = p->pg[n].c;
+      double R;
+      Kern w = pair_kernel(K, xk, xn, R);
+      cd S1, D1, S2, D2;
+      apply_kernel(p, k, n, xk, K, logc, w, R, false, S1, D1);
+      S[k + (size_t)n * N] = S1;
+      Am[k + (size_t)n * N] = (k == n ? cd(2.0 * PI, 0.0) : cd(0, 0)) - D1;
+      if (n != k) {
+        apply_kernel(p, n, k, xn, K, logc, w, R, true, S2, D2);
+        S[n + (size_t)k * N] = S2;
+        Am[n + (size_t)k * N] = -D2;
+      }
+    }
+  }
+}
+
+// Complex LU with partial pivoting (linalg.rs:134-252 semantics: LAPACK ipiv, unit lower L,
+// zero pivot -> numerical error); the pivot is the entry of largest |re| + |im| (LAPACK izamax).
+int lu_solve(int n, cd* a, int nrhs, cd* b) {
+  for (size_t i = 0; i < (size_t)n * n; ++i)
+    if (!std::isfinite(a[i].real()) || !std::isfinite(a[i].imag())) return ORC_ERR_NUMERICAL;
+  std::vector<int> ipiv(n);
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    double best = -1.0;
+    for (int i = k; i < n; ++i) {
+      cd v = a[i + (size_t)k * n];
+      double m = std::fabs(v.real()) + std::fabs(v.imag());
+      if (m > best) {
+        best = m;
+        piv = i;
+      }
+    }
+    ipiv[k] = piv;
+    if (piv != k)
+      for (int j = 0; j < n; ++j) std::swap(a[k + (size_t)j * n], a[piv + (size_t)j * n]);
+    cd d = a[k + (size_t)k * n];
+    if (d == cd(0, 0)) return ORC_ERR_NUMERICAL;
+    cd inv = 1.0 / d;
+    for (int i = k + 1; i < n; ++i) a[i + (size_t)k * n] *= inv;
+#pragma omp parallel for schedule(static) if (n - k > 256)
+    for (int j = k + 1; j < n; ++j) {
+      cd u = a[k + (size_t)j * n];
+      if (u == cd(0, 0)) continue;
+      cd* cj = a + (size_t)j * n;
+      const cd* ck = a + (size_t)k * n;
+      for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * u;
+    }
+  }
+  for (int c = 0; c < nrhs; ++c) {
+    cd* x = b + (size_t)c * n;
+    for (int k = 0; k < n; ++k)
+      if (ipiv[k] != k) std::swap(x[k], x[ipiv[k]]);
+    for (int k = 0; k < n; ++k) {
+      cd v = x[k];
+      for (int i = k + 1; i < n; ++i) x[i] -= a[i + (size_t)k * n] * v;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+      x[k] /= a[k + (size_t)k * n];
+      cd v = x[k];
+      for (int i = 0; i < k; ++i) x[i] -= a[i + (size_t)k * n] * v;
+    }
+  }
+  for (size_t i = 0; i < (size_t)n * nrhs; ++i)
+    if (!std::isfinite(b[i].real()) || !std::isfinite(b[i].imag())) return ORC_ERR_NUMERICAL;
+  return ORC_OK;
+}
+
+}  // namespace
+
+extern "C" int orc_assemble(void* h, double omega, double* S_out, double* A_out) {
+  OrcProblem* p = (OrcProblem*)h;
+  if (!(omega > 0.0)) return ORC_ERR_ARGUMENT;
+  if (std::isfinite(p->depth)) return ORC_ERR_UNSUPPORTED;
+  std::vector<cd> S, A;
+  assemble(p, omega, S, A);
+  std::memcpy(S_out, S.data(), S.size() * sizeof(cd));
+  std::memcpy(A_out, A.data(), A.size() * sizeof(cd));
+  return ORC_OK;
+}
+
+extern "C" int orc_lu_solve(int n, double* a, int nrhs, double* b) {
+  return lu_solve(n, (cd*)a, nrhs, (cd*)b);
+}
+
+// One solve over F frequencies and H headings (degrees). Outputs (double / interleaved complex):
+// A, B [F][D][K], X, FK, SC [F][H][D], HK [F][K][H], with D = K = 6 nb (rigid modes of each body,
+// lo_frequency.rs:1039-1171; solve_driver.rs:362-387 orders the result arrays so).
+extern "C" int orc_solve(void* h, const double* omegas, int nf, const double* headings_deg, int nh, double* Aout,
+                         double* Bout, double* Xout, double* FKout, double* SCout, double* HKout) {
+  OrcProblem* p = (OrcProblem*)h;
+  if (std::isfinite(p->depth)) return ORC_ERR_UNSUPPORTED;
+  int N = p->N;
+  int D = 6 * p->nb, nrad = D, nrhs = nrad + nh;
+  std::vector<cd> S, Am;
+  for (int fi = 0; fi < nf; ++fi) {
+    double w = omegas[fi];
+    if (!(w > 0.0) || !std::isfinite(w)) return ORC_ERR_ARGUMENT;
+    double K = w * w / p->grav, k = K;
+    assemble(p, w, S, Am);
+    // boundary values (lo_frequency.rs:214-273; low_order.rs:1458-1468 incident_one, frame at the origin)
+    std::vector<cd> phiI((size_t)nh * N), dphiI((size_t)nh * N), u((size_t)nrhs * N, cd(0, 0));
+    for (int hh = 0; hh < nh; ++hh) {
+      double beta = headings_deg[hh] * PI / 180.0;
+      double cb = std::cos(beta), sb = std::sin(beta);
+      for (int j = 0; j < N; ++j) {
+        V3 x = p->pg[j].c;
+        double phase = -k * (x[0] * cb + x[1] * sb);
+        double Z = std::exp(k * x[2]), Zz = k * Z;
+        cd e = cd(0.0, p->grav / w) * cd(std::cos(phase), std::sin(phase));
+        cd phi = e * Z;
+        cd g0 = phi * cd(0.0, -k * cb), g1 = phi * cd(0.0, -k * sb), g2 = e * Zz;
+        V3 n = p->pg[j].n;
+        phiI[(size_t)hh * N + j] = phi;
+        dphiI[(size_t)hh * N + j] = g0 * n[0] + g1 * n[1] + g2 * n[2];
+      }
+    }
+    for (int c = 0; c < nrad; ++c) {
+      int b = c / 6, mode = c % 6;
+      for (int j = 0; j < N; ++j)
+        if (p->body[j] == b) u[(size_t)c * N + j] = cd(p->nj[6 * j + mode], 0.0);
+    }
+    for (int hh = 0; hh < nh; ++hh)
+      for (int j = 0; j < N; ++j) u[(size_t)(nrad + hh) * N + j] = -dphiI[(size_t)hh * N + j];
+    // rhs = S U, U = -u (lo_frequency.rs:855-893 with G = 1, chi = 1)
+    std::vector<cd> rhs((size_t)N * nrhs, cd(0, 0));
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nrhs; ++c)
+      for (int nn = 0; nn < N; ++nn) {
+        cd un = -u[(size_t)c * N + nn];
+        if (un == cd(0, 0)) continue;
+        const cd* col = &S[(size_t)nn * N];
+        cd* r = &rhs[(size_t)c * N];
+        for (int kk = 0; kk < N; ++kk) r[kk] += col[kk] * un;
+      }
+    int st = lu_solve(N, Am.data(), nrhs, rhs.data());
+    if (st) return st;
+    const std::vector<cd>& phi = rhs;  // [nrhs][N]
+    cd iwr(0.0, w * p->rho);
+    for (int i = 0; i < D; ++i) {
+      int bi = i / 6, mi = i % 6;
+      for (int c = 0; c < nrad; ++c) {
+        cd z(0, 0);
+        for (int j = 0; j < N; ++j)
+          if (p->body[j] == bi) z += p->nj[6 * j + mi] * phi[(size_t)c * N + j] * p->pg[j].area;
+        z *= -p->rho;
+        size_t o = ((size_t)fi * D + i) * nrad + c;
+        Aout[o] = z.real();
+        Bout[o] = -w * z.imag();
+      }
+      for (int hh = 0; hh < nh; ++hh) {
+        cd x(0, 0), fk(0, 0);
+        for (int j = 0; j < N; ++j)
+          if (p->body[j] == bi) {
+            double ni = p->nj[6 * j + mi];
+            cd pd = phi[(size_t)(nrad + hh) * N + j] + phiI[(size_t)hh * N + j];
+            x += ni * pd * p->pg[j].area;
+            fk += ni * phiI[(size_t)hh * N + j] * p->pg[j].area;
+          }
+        size_t o = ((size_t)fi * nh + hh) * D + i;
+        cd X = iwr * x, FK = iwr * fk, SC = iwr * (x - fk);
+        Xout[2 * o] = X.real();
+        Xout[2 * o + 1] = X.imag();
+        FKout[2 * o] = FK.real();
+        FKout[2 * o + 1] = FK.imag();
+        SCout[2 * o] = SC.real();
+        SCout[2 * o + 1] = SC.imag();
+      }
+    }
+    for (int c = 0; c < nrad; ++c) {
+      int bi = c / 6, mc = c % 6;
+      for (int hh = 0; hh < nh; ++hh) {
+        cd v(0, 0);
+        for (int j = 0; j < N; ++j) {
+          cd ni = p->body[j] == bi ? cd(p->nj[6 * j + mc], 0.0) : cd(0, 0);
+          v += (ni * phiI[(size_t)hh * N + j] - phi[(size_t)c * N + j] * dphiI[(size_t)hh * N + j]) * p->pg[j].area;
+        }
+        cd HK = iwr * v;
+        size_t o = ((size_t)fi * nrad + c) * nh + hh;
+        HKout[2 * o] = HK.real();
+        HKout[2 * o + 1] = HK.imag();
+      }
+    }
+  }
+  return ORC_OK;
+}
