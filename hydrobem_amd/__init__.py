@@ -1,0 +1,197 @@
+"""hydrobem_amd: HydroBEM's low-order frequency-domain solve on one AMD MI355X (gfx950).
+
+The work happens in the HIP library hydrobem_amd/_build/libhydrobem_amd.so behind the C ABI of
+include/hydrobem_amd.h. This module is its Python host mirror: `Session.solve` returns the
+quantities of the reference's results (solve_driver.rs:336-600 common_results: added_mass and
+radiation_damping [F, D, K], excitation_diffraction / _froude_krylov / _scattering [F, H, D],
+excitation_haskind [F, H, K]) for wetted bodies in infinite depth. There is no CPU fallback: the
+library must be built (`python -m hydrobem_amd.build`) and a GPU present, or the calls fail.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from . import meshes  # noqa: F401
+
+__all__ = ["Session", "HbgError", "library", "LIB_PATH", "wamit"]
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "libhydrobem_amd.so"
+
+STATUS = {0: "OK", 1: "VALUE", 2: "TYPE", 3: "UNSUPPORTED", 4: "STATE", 5: "CANCELLED", 6: "INTERNAL",
+          7: "ARGUMENT", 8: "NUMERICAL", 9: "IO", 10: "EXISTS"}
+
+
+class HbgError(RuntimeError):
+    """A failed library call: `status` is the HbgStatus code, the message the library's own."""
+
+    def __init__(self, status: int, message: str, location: str = ""):
+        super().__init__(f"{STATUS.get(status, status)}: {message}" + (f" ({location})" if location else ""))
+        self.status = status
+        self.message = message
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+PROGRESS = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("logarithmic", ctypes.c_int32), ("log_tolerance", ctypes.c_double)]
+
+
+_lib = None
+
+# the C ABI of include/hydrobem_amd.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "hbg_last_error_message": (ctypes.c_char_p, []),
+    "hbg_last_error_location": (ctypes.c_char_p, []),
+    "hbg_version": (ctypes.c_char_p, []),
+    "hbg_session_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "hbg_session_free": (ctypes.c_int32, [ctypes.c_void_p]),
+    "hbg_set_environment": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "hbg_set_panels": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip, ctypes.c_int32, _dp]),
+    "hbg_solve": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
+                                   ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
+                                   ctypes.c_void_p]),
+    "hbg_wave_kernel": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, ctypes.c_int64, _dp]),
+    "hbg_statics": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp]),
+    "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
+    "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
+    "hbg_write_wamit": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
+                                         _dp, _dp, _dp, _dp, ctypes.c_char_p]),
+}
+
+
+def library():
+    """The loaded HIP library (fails loudly if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hydrobem_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int):
+    if status != 0:
+        L = library()
+        raise HbgError(status, L.hbg_last_error_message().decode(), L.hbg_last_error_location().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _options(logarithmic=True, log_tolerance=1e-4, precision=0) -> Options:
+    return Options(int(precision), int(bool(logarithmic)), float(log_tolerance))
+
+
+class Session:
+    """One geometry on one GPU: panels, environment, statics cached on the device.
+
+    The Python face of the reference's solve (hb.Model.solve, python/hydrobem/engine.py:286-301)
+    for wetted panels in infinite depth without symmetry planes.
+    """
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(library().hbg_session_create(int(device), ctypes.byref(self._h)))
+        self.n_panels = 0
+        self.n_bodies = 0
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            library().hbg_session_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_environment(self, density=1000.0, gravity=9.80665, depth=float("inf")):
+        _check(library().hbg_set_environment(self._h, float(density), float(gravity), float(depth)))
+
+    def set_panels(self, vertices, body=None, rotation_centres=None):
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 4, 3)
+        n = v.shape[0]
+        b = None if body is None else np.ascontiguousarray(body, dtype=np.int32)
+        nb = 1 if b is None else int(b.max()) + 1
+        rc = None if rotation_centres is None else np.ascontiguousarray(rotation_centres, dtype=np.float64)
+        _check(library().hbg_set_panels(self._h, _p(v), n, b.ctypes.data_as(_ip) if b is not None else None, nb,
+                                        _p(rc)))
+        self.n_panels, self.n_bodies = n, nb
+
+    def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None):
+        """The quantities of one solve (the reference's result names and axes)."""
+        w = np.ascontiguousarray(omegas, dtype=np.float64).reshape(-1)
+        h = np.ascontiguousarray(headings_deg, dtype=np.float64).reshape(-1)
+        F, H, D = len(w), len(h), 6 * self.n_bodies
+        A = np.zeros((F, D, D))
+        B = np.zeros((F, D, D))
+        X = np.zeros((F, H, D), np.complex128)
+        FK = np.zeros((F, H, D), np.complex128)
+        SC = np.zeros((F, H, D), np.complex128)
+        HK = np.zeros((F, D, H), np.complex128)
+        cb = PROGRESS(lambda user, stage, done, total: int(progress(stage, done, total)) if progress else 1)
+        opt = _options(logarithmic, log_tolerance)
+        _check(library().hbg_solve(self._h, _p(w), F, _p(h), H, ctypes.byref(opt), _p(A), _p(B), _p(X), _p(FK),
+                                   _p(SC), _p(HK), cb, None))
+        return {"omegas": w, "headings": h, "added_mass": A, "radiation_damping": B,
+                "excitation_diffraction": X, "excitation_froude_krylov": FK, "excitation_scattering": SC,
+                "excitation_haskind": np.transpose(HK, (0, 2, 1)).copy()}
+
+    def wave_kernel(self, X, Y):
+        x = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        y = np.ascontiguousarray(Y, dtype=np.float64).reshape(-1)
+        out = np.zeros((len(x), 7))
+        _check(library().hbg_wave_kernel(self._h, _p(x), _p(y), len(x), _p(out)))
+        return out
+
+    def statics(self, logarithmic=True, log_tolerance=1e-4):
+        out = np.zeros((self.n_panels, self.n_panels, 4))
+        _check(library().hbg_statics(self._h, int(bool(logarithmic)), float(log_tolerance), _p(out)))
+        return out
+
+    def assemble(self, omega, logarithmic=True, log_tolerance=1e-4):
+        N = self.n_panels
+        S = np.zeros((N, N), np.complex128)
+        A = np.zeros((N, N), np.complex128)
+        opt = _options(logarithmic, log_tolerance)
+        _check(library().hbg_assemble(self._h, float(omega), ctypes.byref(opt), _p(S), _p(A)))
+        return S, A
+
+    def lu_solve(self, a, b):
+        """x = a^-1 b on the GPU (complex128); a (n, n), b (n,) or (n, r)."""
+        n = a.shape[0]
+        af = np.array(a, dtype=np.complex128, order="F", copy=True)
+        bb = np.array(b, dtype=np.complex128, order="F", copy=True).reshape(n, -1, order="F")
+        _check(library().hbg_lu_solve(self._h, n, af.ctypes.data_as(_dp), bb.shape[1], bb.ctypes.data_as(_dp)))
+        return bb.reshape(b.shape, order="F")
+
+
+def wamit(stem, result, density, gravity, reference_length=1.0, overwrite=True, stamp="", n_bodies=1):
+    """The WAMIT .1/.2/.3 files of a solve (the C ABI's hbg_write_wamit)."""
+    w = np.ascontiguousarray(result["omegas"], dtype=np.float64)
+    h = np.ascontiguousarray(result["headings"], dtype=np.float64)
+    A = np.ascontiguousarray(result["added_mass"], dtype=np.float64)
+    B = np.ascontiguousarray(result["radiation_damping"], dtype=np.float64)
+    HK = np.ascontiguousarray(np.transpose(result["excitation_haskind"], (0, 2, 1)), dtype=np.complex128)
+    X = np.ascontiguousarray(result["excitation_diffraction"], dtype=np.complex128)
+    _check(library().hbg_write_wamit(str(stem).encode(), int(bool(overwrite)), float(density), float(gravity),
+                                     float(reference_length), _p(w), len(w), _p(h), len(h), int(n_bodies), _p(A),
+                                     _p(B), HK.ctypes.data_as(_dp), X.ctypes.data_as(_dp), stamp.encode()))

@@ -1,0 +1,611 @@
+// The C ABI of include/hydrobem_amd.h: sessions, geometry, the frequency loop on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hydrobem_amd.h"
+#include "hbg_internal.h"
+
+namespace hbg {
+
+// kernels (statics.hip, frequency.hip, lu.hip)
+void launch_statics(const PanelD* P, int N, int logint, double tol, const double* gl, StaticD* st, int2* queue,
+                    unsigned* qcount, unsigned qcap, hipStream_t stream);
+void launch_adaptive(const PanelD* P, int N, const double* gl, StaticD* st, const int2* queue, unsigned count,
+                     hipStream_t stream);
+void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
+                     double2* A, int lda, hipStream_t stream);
+void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream);
+void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
+                     double grav, const double* heads, double2* phiI, double2* dphiI, double2* u,
+                     hipStream_t stream);
+void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, hipStream_t stream);
+void launch_forces(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double rho,
+                   const double2* M, int ldm, const double2* phiI, const double2* dphiI, double* out,
+                   hipStream_t stream);
+int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream);
+int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
+                const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
+                const double* X, const char* stamp, std::string& err);
+
+namespace {
+
+thread_local std::string g_msg, g_loc;
+
+struct V3 {
+  double x[3];
+  double& operator[](int i) { return x[i]; }
+  double operator[](int i) const { return x[i]; }
+};
+V3 v3(double a, double b, double c) { return V3{{a, b, c}}; }
+V3 operator+(V3 a, V3 b) { return v3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+V3 operator-(V3 a, V3 b) { return v3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+V3 operator*(double s, V3 a) { return v3(s * a[0], s * a[1], s * a[2]); }
+double dot(V3 a, V3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+V3 cross(V3 a, V3 b) { return v3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]); }
+double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+// The panel rule of geometry.rs:120-199: projection onto the plane through the side midpoints
+// (normal d1 x d2 of the diagonals), distinct vertices (coincidence 1e-6 of the diameter),
+// strict convexity (sine 1e-6), area and centroid by the fan of triangles. Returns "" or the reason.
+std::string make_panel(const double* v12, V3* proj, V3* poly, int& m, V3& normal, V3& centroid, double& area) {
+  for (int i = 0; i < 12; ++i)
+    if (!std::isfinite(v12[i])) return "non-finite vertex coordinate";
+  V3 raw[4];
+  for (int i = 0; i < 4; ++i) raw[i] = v3(v12[3 * i], v12[3 * i + 1], v12[3 * i + 2]);
+  double diameter = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = i + 1; j < 4; ++j) diameter = std::max(diameter, norm(raw[i] - raw[j]));
+  if (diameter == 0.0) return "fewer than three distinct vertices";
+  const double tol = 1e-6 * diameter;
+  int keep[4];
+  m = 0;
+  for (int i = 0; i < 4; ++i)
+    if (norm(raw[i] - raw[(i + 1) % 4]) > tol) keep[m++] = i;
+  if (m < 3) return "fewer than three distinct vertices";
+  for (int a = 0; a < m; ++a)
+    for (int b = a + 1; b < m; ++b)
+      if (norm(raw[keep[a]] - raw[keep[b]]) <= tol)
+        return "projected polygon is self-intersecting (coincident non-adjacent vertices)";
+  V3 c = 0.25 * (raw[0] + raw[1] + raw[2] + raw[3]);
+  V3 n = cross(raw[2] - raw[0], raw[3] - raw[1]);
+  const double nn = norm(n);
+  if (0.5 * nn <= 1e-6 * diameter * diameter) return "zero area (degenerate or self-intersecting polygon)";
+  V3 nh = (1.0 / nn) * n;
+  for (int i = 0; i < 4; ++i) proj[i] = raw[i] - dot(raw[i] - c, nh) * nh;
+  for (int a = 0; a < m; ++a) poly[a] = proj[keep[a]];
+  for (int a = 0; a < m; ++a) {
+    V3 e1 = poly[(a + 1) % m] - poly[a];
+    V3 e2 = poly[(a + 2) % m] - poly[(a + 1) % m];
+    const double s = dot(cross(e1, e2), nh) / (norm(e1) * norm(e2));
+    if (!(s > 1e-6)) return "projected polygon is not strictly convex or is self-intersecting";
+  }
+  area = 0.0;
+  V3 mom = v3(0, 0, 0);
+  for (int a = 1; a < m - 1; ++a) {
+    const double t = 0.5 * dot(cross(poly[a] - poly[0], poly[a + 1] - poly[0]), nh);
+    area += t;
+    mom = mom + (t / 3.0) * (poly[0] + poly[a] + poly[a + 1]);
+  }
+  centroid = (1.0 / area) * mom;
+  normal = nh;
+  return "";
+}
+
+// The panel record of the kernels (panel_integrals.rs:145-217 make_panel_geom: distinct vertices
+// of the projected corners, radius, edge lengths and in-plane edge normals).
+PanelD panel_record(const V3* q, V3 normal, V3 centroid, double area) {
+  PanelD p;
+  std::memset(&p, 0, sizeof(p));
+  double diam = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = i + 1; j < 4; ++j) diam = std::max(diam, norm(q[i] - q[j]));
+  V3 v[4];
+  int nv = 0;
+  for (int i = 0; i < 4; ++i)
+    if (norm(q[i] - q[(i + 1) % 4]) > 1e-6 * diam) v[nv++] = q[i];
+  double radius = 0.0;
+  for (int i = 0; i < nv; ++i) radius = std::max(radius, norm(v[i] - centroid));
+  for (int e = 0; e < nv; ++e) {
+    V3 edge = v[(e + 1) % nv] - v[e];
+    p.elen[e] = norm(edge);
+    V3 em = (1.0 / p.elen[e]) * cross(edge, normal);
+    for (int d = 0; d < 3; ++d) p.em[e][d] = em[d];
+  }
+  if (nv == 3) v[3] = v[2];
+  double top = v[0][2];
+  for (int i = 0; i < 4; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      p.v[i][d] = v[i][d];
+      p.q4[i][d] = q[i][d];
+    }
+    if (i < nv) top = std::max(top, v[i][2]);
+  }
+  for (int d = 0; d < 3; ++d) {
+    p.c[d] = centroid[d];
+    p.n[d] = normal[d];
+  }
+  p.area = area;
+  p.radius = radius;
+  p.nv = nv;
+  p.top = top;
+  return p;
+}
+
+// Gauss-Legendre nodes and weights of orders 1..16, [order][x 16 | w 16] (quadrature.rs:17-59)
+std::vector<double> gauss_table() {
+  std::vector<double> t(17 * 32, 0.0);
+  for (int n = 1; n <= 16; ++n) {
+    double* x = &t[n * 32];
+    double* w = x + 16;
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+      double z = std::cos(PI * (i + 0.75) / (n + 0.5)), dp = 0.0;
+      for (int it = 0; it < 100; ++it) {
+        double p0 = 1.0, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+          double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+          p0 = p1;
+          p1 = p2;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        double dz = p1 / dp;
+        z -= dz;
+        if (std::fabs(dz) <= 1e-16) break;
+      }
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      x[i] = -z;
+      x[n - 1 - i] = z;
+      w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;
+  }
+  return t;
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    if (p) HBG_HIP(hipFree(p));
+    p = nullptr;
+    HBG_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// process-wide wave-kernel tables of a device
+struct TablesOnDevice {
+  int device = -1;
+  double* t2d = nullptr;
+  double* t1d = nullptr;
+};
+TablesOnDevice& tables_for(int device) {
+  static TablesOnDevice tabs[16];
+  TablesOnDevice& t = tabs[device & 15];
+  if (t.device == device) return t;
+  std::vector<double> h2, h1;
+  build_wave_tables(true, h2, h1);
+  HBG_HIP(hipMalloc(&t.t2d, h2.size() * sizeof(double)));
+  HBG_HIP(hipMalloc(&t.t1d, h1.size() * sizeof(double)));
+  HBG_HIP(hipMemcpy(t.t2d, h2.data(), h2.size() * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(t.t1d, h1.data(), h1.size() * sizeof(double), hipMemcpyHostToDevice));
+  t.device = device;
+  return t;
+}
+
+}  // namespace
+
+void set_error(int status, const std::string& msg, const char* where) {
+  (void)status;
+  g_msg = msg;
+  g_loc = where ? where : "";
+}
+
+}  // namespace hbg
+
+using namespace hbg;
+
+struct HbgSession {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double rho = 1000.0, grav = 9.80665, depth = INFINITY;
+  int N = 0, nb = 0;
+  std::vector<PanelD> panels;
+  std::vector<int> body;
+  std::vector<double> nj;
+  DevBuf<PanelD> d_panels;
+  DevBuf<int> d_body;
+  DevBuf<double> d_nj, d_gl, d_heads, d_out;
+  DevBuf<StaticD> d_st;
+  DevBuf<int2> d_queue;
+  DevBuf<unsigned> d_qcount;
+  DevBuf<double2> d_S, d_M, d_phiI, d_dphiI, d_u;
+  bool statics_valid = false;
+  int st_logint = -1;
+  double st_tol = -1.0;
+  WaveTables T;
+};
+
+namespace {
+
+HbgStatus fail(int status, const std::string& msg, const char* where) {
+  set_error(status, msg, where);
+  return status;
+}
+
+#define GUARD_BEGIN try {
+#define GUARD_END(where)                                             \
+  }                                                                  \
+  catch (const ::hbg::Error& e) {                                    \
+    return fail(e.status, e.msg, where);                             \
+  }                                                                  \
+  catch (const std::exception& e) {                                  \
+    return fail(HBG_ERR_INTERNAL, std::string("internal: ") + e.what(), where); \
+  }                                                                  \
+  catch (...) {                                                      \
+    return fail(HBG_ERR_INTERNAL, "internal error", where);          \
+  }
+
+void use_device(HbgSession* s) { HBG_HIP(hipSetDevice(s->device)); }
+
+void ensure_statics(HbgSession* s, int logint, double tol) {
+  if (s->statics_valid && s->st_logint == logint && s->st_tol == tol) return;
+  const int N = s->N;
+  s->d_st.ensure((size_t)N * N);
+  unsigned qcap = (unsigned)std::min<size_t>((size_t)N * 64, (size_t)1 << 28);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    s->d_queue.ensure(qcap);
+    s->d_qcount.ensure(1);
+    HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
+    launch_statics(s->d_panels.p, N, logint, tol, s->d_gl.p, s->d_st.p, s->d_queue.p, s->d_qcount.p, qcap,
+                   s->stream);
+    unsigned count = 0;
+    HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipStreamSynchronize(s->stream));
+    if (count <= qcap) {
+      launch_adaptive(s->d_panels.p, N, s->d_gl.p, s->d_st.p, s->d_queue.p, count, s->stream);
+      HBG_HIP(hipStreamSynchronize(s->stream));
+      s->statics_valid = true;
+      s->st_logint = logint;
+      s->st_tol = tol;
+      return;
+    }
+    qcap = count;
+  }
+  throw Error{HBG_ERR_INTERNAL, "statics: adaptive queue overflow"};
+}
+
+void check_options(const HbgOptions* o) {
+  if (o && o->precision != HBG_PRECISION_FP64) throw Error{HBG_ERR_UNSUPPORTED, "precision not supported"};
+  if (o && !(o->log_tolerance >= 0.0)) throw Error{HBG_ERR_VALUE, "log_tolerance must be >= 0"};
+}
+
+HbgOptions defaults(const HbgOptions* o) {
+  HbgOptions d;
+  d.precision = HBG_PRECISION_FP64;
+  d.logarithmic = 1;
+  d.log_tolerance = 1e-4;
+  return o ? *o : d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hbg_last_error_message(void) { return g_msg.c_str(); }
+const char* hbg_last_error_location(void) { return g_loc.c_str(); }
+const char* hbg_version(void) { return "hydrobem_amd 0.1.0 (gfx950, FP64 low-order infinite depth)"; }
+
+HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
+  if (!out) return fail(HBG_ERR_ARGUMENT, "out is null", "hbg_session_create");
+  GUARD_BEGIN
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    throw Error{HBG_ERR_STATE, "no GPU device: the HIP library needs an MI355X (gfx950)"};
+  if (device < 0 || device >= count) throw Error{HBG_ERR_ARGUMENT, "device index out of range"};
+  std::unique_ptr<HbgSession> s(new HbgSession());
+  s->device = device;
+  HBG_HIP(hipSetDevice(device));
+  HBG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  TablesOnDevice& t = tables_for(device);
+  s->T.t2d = t.t2d;
+  s->T.t1d = t.t1d;
+  std::vector<double> gl = gauss_table();
+  s->d_gl.ensure(gl.size());
+  HBG_HIP(hipMemcpy(s->d_gl.p, gl.data(), gl.size() * sizeof(double), hipMemcpyHostToDevice));
+  *out = s.release();
+  return HBG_OK;
+  GUARD_END("hbg_session_create")
+}
+
+HbgStatus hbg_session_free(HbgSession* s) {
+  if (!s) return HBG_OK;
+  GUARD_BEGIN
+  hipSetDevice(s->device);
+  hipStreamSynchronize(s->stream);
+  s->d_panels.release();
+  s->d_body.release();
+  s->d_nj.release();
+  s->d_gl.release();
+  s->d_heads.release();
+  s->d_out.release();
+  s->d_st.release();
+  s->d_queue.release();
+  s->d_qcount.release();
+  s->d_S.release();
+  s->d_M.release();
+  s->d_phiI.release();
+  s->d_dphiI.release();
+  s->d_u.release();
+  hipStreamDestroy(s->stream);
+  delete s;
+  return HBG_OK;
+  GUARD_END("hbg_session_free")
+}
+
+HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, double depth) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_environment");
+  if (!(density > 0.0) || !std::isfinite(density)) return fail(HBG_ERR_VALUE, "density must be > 0", "hbg_set_environment");
+  if (!(gravity > 0.0) || !std::isfinite(gravity)) return fail(HBG_ERR_VALUE, "gravity must be > 0", "hbg_set_environment");
+  if (!(depth > 0.0)) return fail(HBG_ERR_VALUE, "depth must be > 0", "hbg_set_environment");
+  if (std::isfinite(depth)) return fail(HBG_ERR_UNSUPPORTED, "finite depth is not supported", "hbg_set_environment");
+  s->rho = density;
+  s->grav = gravity;
+  s->depth = depth;
+  return HBG_OK;
+}
+
+HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
+                         int32_t n_bodies, const double* rotation_centres) {
+  if (!s || !vertices) return fail(HBG_ERR_ARGUMENT, "session or vertices is null", "hbg_set_panels");
+  if (n_panels <= 0 || n_bodies <= 0) return fail(HBG_ERR_ARGUMENT, "n_panels and n_bodies must be > 0", "hbg_set_panels");
+  GUARD_BEGIN
+  std::vector<PanelD> panels(n_panels);
+  std::vector<int> b(n_panels, 0);
+  std::vector<double> nj(6 * (size_t)n_panels);
+  for (int j = 0; j < n_panels; ++j) {
+    V3 proj[4], poly[4], normal, centroid;
+    int m;
+    double area;
+    std::string why = make_panel(vertices + 12 * (size_t)j, proj, poly, m, normal, centroid, area);
+    if (!why.empty()) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": " + why};
+    if (!(centroid[2] < 0.0))
+      throw Error{HBG_ERR_UNSUPPORTED,
+                  "panel " + std::to_string(j + 1) + ": centroid not below z = 0 (surfaces in z = 0 are not supported)"};
+    panels[j] = panel_record(proj, normal, centroid, area);
+    const int bi = body ? body[j] : 0;
+    if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
+    b[j] = bi;
+    V3 rc = rotation_centres ? v3(rotation_centres[3 * bi], rotation_centres[3 * bi + 1], rotation_centres[3 * bi + 2])
+                             : v3(0, 0, 0);
+    V3 mom = cross(centroid - rc, normal);
+    for (int d = 0; d < 3; ++d) {
+      nj[6 * (size_t)j + d] = normal[d];
+      nj[6 * (size_t)j + 3 + d] = mom[d];
+    }
+  }
+  use_device(s);
+  s->panels = panels;
+  s->body = b;
+  s->nj = nj;
+  s->N = n_panels;
+  s->nb = n_bodies;
+  s->d_panels.ensure(panels.size());
+  s->d_body.ensure(b.size());
+  s->d_nj.ensure(nj.size());
+  HBG_HIP(hipMemcpy(s->d_panels.p, panels.data(), panels.size() * sizeof(PanelD), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(s->d_body.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(s->d_nj.p, nj.data(), nj.size() * sizeof(double), hipMemcpyHostToDevice));
+  s->statics_valid = false;
+  return HBG_OK;
+  GUARD_END("hbg_set_panels")
+}
+
+HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64_t n, double* out) {
+  if (!s || !X || !Y || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_wave_kernel");
+  if (n <= 0) return HBG_OK;
+  for (int64_t i = 0; i < n; ++i)
+    if (!(X[i] >= 0.0) || !(Y[i] >= 0.0) || (X[i] == 0.0 && Y[i] == 0.0) || !std::isfinite(X[i]) ||
+        !std::isfinite(Y[i]))
+      return fail(HBG_ERR_VALUE, "wave kernel point " + std::to_string(i) + " outside X, Y >= 0, (X, Y) != 0",
+                  "hbg_wave_kernel");
+  GUARD_BEGIN
+  use_device(s);
+  DevBuf<double> dx, dy, dout;
+  dx.ensure(n);
+  dy.ensure(n);
+  dout.ensure(7 * (size_t)n);
+  HBG_HIP(hipMemcpyAsync(dx.p, X, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HBG_HIP(hipMemcpyAsync(dy.p, Y, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  launch_wave_points(s->T, dx.p, dy.p, (long)n, dout.p, s->stream);
+  HBG_HIP(hipMemcpyAsync(out, dout.p, 7 * n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  dx.release();
+  dy.release();
+  dout.release();
+  return HBG_OK;
+  GUARD_END("hbg_wave_kernel")
+}
+
+HbgStatus hbg_statics(HbgSession* s, int32_t logarithmic, double log_tolerance, double* out) {
+  if (!s || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_statics");
+  if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_statics");
+  GUARD_BEGIN
+  use_device(s);
+  ensure_statics(s, logarithmic ? 1 : 0, log_tolerance);
+  const int N = s->N;
+  std::vector<StaticD> h((size_t)N * N);
+  HBG_HIP(hipMemcpy(h.data(), s->d_st.p, h.size() * sizeof(StaticD), hipMemcpyDeviceToHost));
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j) {
+      const StaticD& e = h[(size_t)j * N + k];
+      double* o = out + ((size_t)k * N + j) * 4;
+      o[0] = e.rs;
+      o[1] = e.rd;
+      o[2] = e.dl;
+      o[3] = e.dd;
+    }
+  return HBG_OK;
+  GUARD_END("hbg_statics")
+}
+
+HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, double* S, double* A) {
+  if (!s || !S || !A) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_assemble");
+  if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_assemble");
+  if (!(omega > 0.0) || !std::isfinite(omega)) return fail(HBG_ERR_VALUE, "omega must be finite and > 0", "hbg_assemble");
+  GUARD_BEGIN
+  check_options(options);
+  HbgOptions o = defaults(options);
+  use_device(s);
+  ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
+  const int N = s->N;
+  s->d_S.ensure((size_t)N * N);
+  s->d_M.ensure((size_t)N * N);
+  const double K = omega * omega / s->grav;
+  launch_assemble(s->d_panels.p, N, K, o.logarithmic ? 1.0 : 0.0, s->T, s->d_st.p, s->d_S.p, s->d_M.p, N, s->stream);
+  std::vector<double2> hs((size_t)N * N), ha((size_t)N * N);
+  HBG_HIP(hipMemcpyAsync(hs.data(), s->d_S.p, hs.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipMemcpyAsync(ha.data(), s->d_M.p, ha.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < N; ++k)
+    for (int n = 0; n < N; ++n) {
+      const size_t src = (size_t)n * N + k, dst = 2 * ((size_t)k * N + n);
+      S[dst] = hs[src].x;
+      S[dst + 1] = hs[src].y;
+      A[dst] = ha[src].x;
+      A[dst + 1] = ha[src].y;
+    }
+  return HBG_OK;
+  GUARD_END("hbg_assemble")
+}
+
+HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b) {
+  if (!s || !a || (nrhs > 0 && !b)) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_lu_solve");
+  if (n <= 0 || nrhs < 0) return fail(HBG_ERR_ARGUMENT, "n must be > 0 and nrhs >= 0", "hbg_lu_solve");
+  for (size_t i = 0; i < 2 * (size_t)n * n; ++i)
+    if (!std::isfinite(a[i])) return fail(HBG_ERR_NUMERICAL, "non-finite system matrix", "hbg_lu_solve");
+  GUARD_BEGIN
+  use_device(s);
+  DevBuf<double2> m;
+  m.ensure((size_t)n * (n + nrhs));
+  HBG_HIP(hipMemcpyAsync(m.p, a, (size_t)n * n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+  if (nrhs > 0)
+    HBG_HIP(hipMemcpyAsync(m.p + (size_t)n * n, b, (size_t)n * nrhs * sizeof(double2), hipMemcpyHostToDevice,
+                           s->stream));
+  const int zp = lu_solve_device(m.p, n, n, nrhs, s->stream);
+  HBG_HIP(hipMemcpyAsync(a, m.p, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+  if (nrhs > 0)
+    HBG_HIP(hipMemcpyAsync(b, m.p + (size_t)n * n, (size_t)n * nrhs * sizeof(double2), hipMemcpyDeviceToHost,
+                           s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  m.release();
+  if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(zp) + ")"};
+  for (size_t i = 0; i < 2 * (size_t)n * nrhs; ++i)
+    if (!std::isfinite(b[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
+  return HBG_OK;
+  GUARD_END("hbg_lu_solve")
+}
+
+HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const double* headings_deg,
+                    int32_t n_headings, const HbgOptions* options, double* added_mass, double* damping,
+                    double* excitation, double* froude_krylov, double* scattering, double* haskind,
+                    HbgProgressFn progress, void* user) {
+  if (!s || (n_omegas > 0 && !omegas) || (n_headings > 0 && !headings_deg))
+    return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_solve");
+  if (n_omegas < 0 || n_headings < 0) return fail(HBG_ERR_ARGUMENT, "negative count", "hbg_solve");
+  if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_solve");
+  for (int i = 0; i < n_omegas; ++i)
+    if (!(omegas[i] > 0.0) || !std::isfinite(omegas[i]))
+      return fail(HBG_ERR_VALUE, "omega " + std::to_string(i) + " must be finite and > 0", "hbg_solve");
+  GUARD_BEGIN
+  check_options(options);
+  HbgOptions o = defaults(options);
+  use_device(s);
+  const int N = s->N, H = n_headings, nrad = 6 * s->nb, D = nrad, R = nrad + H;
+  if (R > 32) throw Error{HBG_ERR_UNSUPPORTED, "more than 32 right-hand sides"};
+  if (progress && progress(user, HBG_STAGE_STATICS, 0, 1) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
+  ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
+  s->d_S.ensure((size_t)N * N);
+  s->d_M.ensure((size_t)N * (N + R));
+  s->d_phiI.ensure((size_t)std::max(H, 1) * N);
+  s->d_dphiI.ensure((size_t)std::max(H, 1) * N);
+  s->d_u.ensure((size_t)R * N);
+  std::vector<double> heads(std::max(H, 1), 0.0);
+  for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
+  s->d_heads.ensure(heads.size());
+  HBG_HIP(hipMemcpy(s->d_heads.p, heads.data(), heads.size() * sizeof(double), hipMemcpyHostToDevice));
+  const size_t nAB = (size_t)D * nrad, nX = (size_t)H * D, nHK = (size_t)nrad * H;
+  const size_t nout = 2 * nAB + 6 * nX + 2 * nHK;
+  s->d_out.ensure(nout);
+  std::vector<double> hout(nout);
+  for (int fi = 0; fi < n_omegas; ++fi) {
+    if (progress && progress(user, HBG_STAGE_ASSEMBLY, fi, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
+    const double w = omegas[fi], K = w * w / s->grav;
+    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, K, s->grav, s->d_heads.p, s->d_phiI.p,
+                    s->d_dphiI.p, s->d_u.p, s->stream);
+    launch_assemble(s->d_panels.p, N, K, o.logarithmic ? 1.0 : 0.0, s->T, s->d_st.p, s->d_S.p, s->d_M.p, N,
+                    s->stream);
+    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->stream);
+    if (progress && progress(user, HBG_STAGE_LU, fi, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
+    const int zp = lu_solve_device(s->d_M.p, N, N, R, s->stream);
+    if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(zp) + ")"};
+    launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, s->rho, s->d_M.p, N, s->d_phiI.p,
+                  s->d_dphiI.p, s->d_out.p, s->stream);
+    HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p, nout * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipStreamSynchronize(s->stream));
+    for (double v : hout)
+      if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
+    const double* hA = hout.data();
+    const double* hB = hA + nAB;
+    const double* hX = hB + nAB;
+    const double* hFK = hX + 2 * nX;
+    const double* hSC = hFK + 2 * nX;
+    const double* hHK = hSC + 2 * nX;
+    if (added_mass) std::memcpy(added_mass + fi * nAB, hA, nAB * sizeof(double));
+    if (damping) std::memcpy(damping + fi * nAB, hB, nAB * sizeof(double));
+    if (excitation) std::memcpy(excitation + fi * 2 * nX, hX, 2 * nX * sizeof(double));
+    if (froude_krylov) std::memcpy(froude_krylov + fi * 2 * nX, hFK, 2 * nX * sizeof(double));
+    if (scattering) std::memcpy(scattering + fi * 2 * nX, hSC, 2 * nX * sizeof(double));
+    if (haskind) std::memcpy(haskind + fi * 2 * nHK, hHK, 2 * nHK * sizeof(double));
+  }
+  if (progress && progress(user, HBG_STAGE_OUTPUTS, n_omegas, n_omegas) == 0)
+    throw Error{HBG_ERR_CANCELLED, "cancelled"};
+  return HBG_OK;
+  GUARD_END("hbg_solve")
+}
+
+HbgStatus hbg_write_wamit(const char* stem, int32_t overwrite, double density, double gravity,
+                          double reference_length, const double* omegas, int32_t n_omegas,
+                          const double* headings_deg, int32_t n_headings, int32_t n_bodies,
+                          const double* added_mass, const double* damping, const double* haskind,
+                          const double* excitation, const char* stamp) {
+  if (!stem || !omegas || !added_mass || !damping || (n_headings > 0 && (!headings_deg || !haskind || !excitation)))
+    return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_write_wamit");
+  GUARD_BEGIN
+  std::string err;
+  int st = write_wamit(stem, overwrite, density, gravity, reference_length, omegas, n_omegas, headings_deg,
+                       n_headings, n_bodies, added_mass, damping, haskind, excitation, stamp, err);
+  if (st) throw Error{st, err};
+  return HBG_OK;
+  GUARD_END("hbg_write_wamit")
+}
+
+}  // extern "C"

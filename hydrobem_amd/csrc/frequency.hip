@@ -1,0 +1,258 @@
+// One frequency of the low-order method on the GPU, FP64: the boundary values, the influence
+// matrices, the right-hand sides S U and the forces (lo_frequency.rs:214-273, 399-537, 823-955,
+// 1039-1171; low_order.rs:627-648, 717-790, 1401-1485), for wetted panels without symmetry
+// (one character, chi = 1) in infinite depth.
+#include "green_inf_dev.h"
+#include "hbg_internal.h"
+
+namespace hbg {
+
+// The matrices of an ordinary frequency: for the field point k (centroid of panel k) and panel n,
+//   S(k, n) = rs + A_n Wr - 2 K logc dl + i A_n Wi
+//   D(k, n) = rd + A_n (-WrR a + WrZ n_z) - 2 K logc dd + i A_n (-WiR a + WiZ n_z)
+//   A(k, n) = 2 pi delta_kn - D(k, n)
+// with the wave kernel at the centroids (one-point rule, low_order.rs:738-769) and
+// a = ((x_k - c_n)_h . n_h) / R. Column-major outputs with leading dimension lda; one thread per
+// entry, the 64 lanes of a wavefront along k (coalesced loads of the statics and stores of the
+// matrices, the panel record of n a wavefront-uniform load).
+__global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict__ P, int N, double K, double logc,
+                                                       WaveTables T, const StaticD* __restrict__ st,
+                                                       double2* __restrict__ S, double2* __restrict__ A, int lda) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int n = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  if (k >= N || n >= N) return;
+  const PanelD* pn = P + n;
+  const double xk = P[k].c[0], yk = P[k].c[1], zk = P[k].c[2];
+  const double dx = xk - pn->c[0], dy = yk - pn->c[1];
+  const double R = sqrt(dx * dx + dy * dy);
+  const double X = K * R, Y = -K * (zk + pn->c[2]);
+  const WaveK w = wave_kernel(T, X, Y);
+  const double R0 = sqrt(X * X + Y * Y);
+  const double FY = -w.F - 2.0 / R0, IY = -w.I;
+  const double K2 = K * K;
+  const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
+  const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
+  const double nx = pn->n[0], ny = pn->n[1], nz = pn->n[2];
+  const double a = R > 0.0 ? (dx * nx + dy * ny) / R : 0.0;
+  const StaticD e = st[(size_t)n * N + k];
+  const double Ar = pn->area;
+  const double c2 = 2.0 * K * logc;
+  const double2 s = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
+  const double2 d = make_double2(e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, Ar * (-WiR * a + WiZ * nz));
+  S[(size_t)n * lda + k] = s;
+  A[(size_t)n * lda + k] = make_double2((k == n ? 2.0 * PI : 0.0) - d.x, -d.y);
+}
+
+// The wave kernel at points (X, Y) for the tests: out[7 i ..] = Ft, Ft_X, Ft_Y, Gi, Gi_X, Gi_Y, e^-Y J0
+__global__ void wave_points_kernel(WaveTables T, const double* __restrict__ X, const double* __restrict__ Y, long n,
+                                   double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = X[i], y = Y[i];
+  const WaveK w = wave_kernel(T, x, y);
+  const double R0 = sqrt(x * x + y * y);
+  double* o = out + 7 * i;
+  o[0] = w.F;
+  o[1] = w.FX;
+  o[2] = -w.F - 2.0 / R0;
+  o[3] = w.I;
+  o[4] = w.IX;
+  o[5] = -w.I;
+  o[6] = -w.I / (2.0 * PI);
+}
+
+// Incident wave and boundary values (lo_frequency.rs:214-273, low_order.rs:1458-1468, frame at
+// the origin): phi_I = (i g / w) e^{k z} e^{-i k (x cos b + y sin b)}; u = n_j on the body of the
+// radiation column, -dphi_I/dn for each heading (the scattered formulation).
+// phiI, dphiI: [H][N]; u: [R][N], R = nrad + H, nrad = 6 nbodies.
+__global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restrict__ body,
+                                const double* __restrict__ nj, int N, int nrad, int H, double w, double k, double grav,
+                                const double* __restrict__ heads, double2* __restrict__ phiI,
+                                double2* __restrict__ dphiI, double2* __restrict__ u) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const PanelD* p = P + j;
+  const double x = p->c[0], y = p->c[1], z = p->c[2];
+  for (int c = 0; c < nrad; ++c)
+    u[(size_t)c * N + j] = make_double2(body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0, 0.0);
+  const double Z = exp(k * z), Zz = k * Z;
+  for (int h = 0; h < H; ++h) {
+    double sb, cb;
+    sincos(heads[h], &sb, &cb);
+    const double ph = -k * (x * cb + y * sb);
+    double sp, cp;
+    sincos(ph, &sp, &cp);
+    // e = (i g / w) (cos ph + i sin ph)
+    const double gw = grav / w;
+    const double2 e = make_double2(-gw * sp, gw * cp);
+    const double2 phi = make_double2(e.x * Z, e.y * Z);
+    // gradient: phi (-i k cb), phi (-i k sb), e Zz
+    const double2 g0 = make_double2(phi.y * k * cb, -phi.x * k * cb);
+    const double2 g1 = make_double2(phi.y * k * sb, -phi.x * k * sb);
+    const double2 g2 = make_double2(e.x * Zz, e.y * Zz);
+    const double2 dn = make_double2(g0.x * p->n[0] + g1.x * p->n[1] + g2.x * p->n[2],
+                                    g0.y * p->n[0] + g1.y * p->n[1] + g2.y * p->n[2]);
+    phiI[(size_t)h * N + j] = phi;
+    dphiI[(size_t)h * N + j] = dn;
+    u[(size_t)(nrad + h) * N + j] = make_double2(-dn.x, -dn.y);
+  }
+}
+
+// rhs(:, c) = S U(:, c), U = -u (lo_frequency.rs:855-893 with G = 1): one row k per thread,
+// S read column by column (coalesced along k), written into the augmented columns of M.
+constexpr int MAXRHS = 32;
+__global__ void __launch_bounds__(256) rhs_kernel(const double2* __restrict__ S, int N, int lds,
+                                                  const double2* __restrict__ u, int R, double2* __restrict__ M,
+                                                  int ldm) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  double2 acc[MAXRHS];
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c) acc[c] = make_double2(0.0, 0.0);
+  for (int l = 0; l < N; ++l) {
+    const double2 s = S[(size_t)l * lds + k];
+#pragma unroll
+    for (int c = 0; c < MAXRHS; ++c) {
+      if (c < R) {
+        const double2 v = u[(size_t)c * N + l];  // wavefront-uniform
+        acc[c].x -= s.x * v.x - s.y * v.y;
+        acc[c].y -= s.x * v.y + s.y * v.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < MAXRHS; ++c)
+    if (c < R) M[(size_t)(N + c) * ldm + k] = acc[c];
+}
+
+// The forces of one frequency (lo_frequency.rs:1039-1171), each output a sum over the panels by
+// one workgroup (fixed-order tree reduction: results independent of timing).
+// phi: the solved potentials, column c at M[(N + c) * ldm]. Outputs (double, complex interleaved):
+// A, B [D][nrad]; X, FK, SC [H][D]; HK [nrad][H]; D = nrad = 6 nbodies.
+__global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ P, const int* __restrict__ body,
+                                                     const double* __restrict__ nj, int N, int nrad, int H,
+                                                     double w, double rho, const double2* __restrict__ M, int ldm,
+                                                     const double2* __restrict__ phiI,
+                                                     const double2* __restrict__ dphiI, double* __restrict__ out) {
+  __shared__ double2 red[256];
+  __shared__ double2 red2[256];
+  const int D = nrad;
+  const int nAB = D * nrad, nX = H * D, nHK = nrad * H;
+  const int item = blockIdx.x;
+  const int tid = threadIdx.x;
+  double2 s = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+  int kind, i = 0, c = 0, h = 0;
+  if (item < nAB) {
+    kind = 0;
+    i = item / nrad;
+    c = item % nrad;
+  } else if (item < nAB + nX) {
+    kind = 1;
+    h = (item - nAB) / D;
+    i = (item - nAB) % D;
+  } else {
+    kind = 2;
+    c = (item - nAB - nX) / H;
+    h = (item - nAB - nX) % H;
+  }
+  for (int j = tid; j < N; j += blockDim.x) {
+    const double area = P[j].area;
+    if (kind == 0) {
+      if (body[j] == i / 6) {
+        const double ni = nj[6 * (size_t)j + i % 6] * area;
+        const double2 ph = M[(size_t)(N + c) * ldm + j];
+        s.x += ni * ph.x;
+        s.y += ni * ph.y;
+      }
+    } else if (kind == 1) {
+      if (body[j] == i / 6) {
+        const double ni = nj[6 * (size_t)j + i % 6] * area;
+        const double2 ps = M[(size_t)(N + nrad + h) * ldm + j];
+        const double2 pi = phiI[(size_t)h * N + j];
+        s.x += ni * (ps.x + pi.x);
+        s.y += ni * (ps.y + pi.y);
+        s2.x += ni * pi.x;
+        s2.y += ni * pi.y;
+      }
+    } else {
+      const double ncv = body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0;
+      const double2 pi = phiI[(size_t)h * N + j];
+      const double2 ph = M[(size_t)(N + c) * ldm + j];
+      const double2 dn = dphiI[(size_t)h * N + j];
+      // (n_c phi_I - phi_c dphi_I/dn) A
+      s.x += (ncv * pi.x - (ph.x * dn.x - ph.y * dn.y)) * area;
+      s.y += (ncv * pi.y - (ph.x * dn.y + ph.y * dn.x)) * area;
+    }
+  }
+  red[tid] = s;
+  red2[tid] = s2;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) {
+      red[tid].x += red[tid + off].x;
+      red[tid].y += red[tid + off].y;
+      red2[tid].x += red2[tid + off].x;
+      red2[tid].y += red2[tid + off].y;
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  const double2 z = red[0], z2 = red2[0];
+  // i w rho (x + i y) = (-w rho y, w rho x)
+  const double wr = w * rho;
+  if (kind == 0) {
+    // z = -rho sum n_i phi_c A: A = Re z, B = -w Im z
+    out[item] = -rho * z.x;                      // added mass
+    out[nAB + item] = -w * (-rho * z.y);         // damping
+  } else if (kind == 1) {
+    const int o = item - nAB;
+    double* X = out + 2 * nAB;
+    double* FK = X + 2 * nX;
+    double* SC = FK + 2 * nX;
+    X[2 * o] = -wr * z.y;
+    X[2 * o + 1] = wr * z.x;
+    FK[2 * o] = -wr * z2.y;
+    FK[2 * o + 1] = wr * z2.x;
+    SC[2 * o] = -wr * (z.y - z2.y);
+    SC[2 * o + 1] = wr * (z.x - z2.x);
+  } else {
+    const int o = item - nAB - nX;
+    double* HK = out + 2 * nAB + 6 * nX;
+    HK[2 * o] = -wr * z.y;
+    HK[2 * o + 1] = wr * z.x;
+  }
+}
+
+void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
+                     double2* A, int lda, hipStream_t stream) {
+  dim3 block(64, 4), grid((N + 63) / 64, (N + 3) / 4);
+  assemble_kernel<<<grid, block, 0, stream>>>(P, N, K, logc, T, st, S, A, lda);
+  HBG_HIP(hipGetLastError());
+}
+
+void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream) {
+  wave_points_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(T, X, Y, n, out);
+  HBG_HIP(hipGetLastError());
+}
+
+void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
+                     double grav, const double* heads, double2* phiI, double2* dphiI, double2* u,
+                     hipStream_t stream) {
+  boundary_kernel<<<(N + 255) / 256, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, k, grav, heads, phiI, dphiI, u);
+  HBG_HIP(hipGetLastError());
+}
+
+void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, hipStream_t stream) {
+  rhs_kernel<<<(N + 255) / 256, 256, 0, stream>>>(S, N, lds, u, R, M, ldm);
+  HBG_HIP(hipGetLastError());
+}
+
+void launch_forces(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double rho,
+                   const double2* M, int ldm, const double2* phiI, const double2* dphiI, double* out,
+                   hipStream_t stream) {
+  const int items = nrad * nrad + H * nrad + nrad * H;
+  forces_kernel<<<items, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, rho, M, ldm, phiI, dphiI, out);
+  HBG_HIP(hipGetLastError());
+}
+
+}  // namespace hbg

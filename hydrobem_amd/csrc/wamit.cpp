@@ -1,0 +1,207 @@
+// WAMIT numeric files .1, .2, .3 of a solve (wamit_files.rs:794-867 build, :534-550 force_file,
+// :92-97 complex_cols, :1718-1757 fortran_e / fortran_i, :1822-1867 write_wamit; normalization
+// docs/reference/file-formats.md section 1):
+//   .1  period, I, J, A/(rho L^(3+ri+rj)), B/(w rho L^(3+ri+rj))            per frequency, I, J
+//   .2  period, beta, I, |X|, phase, Re, Im of the Haskind force / (rho g L^(2+ri))
+//   .3  the same columns for the pressure-integrated force
+// Period 2 pi / w in E14.6 0P form, values E14.6 1P, integers I6, CRLF line ends, a header line
+// " HydroBEM <version>   <file>   <stamp>"; the files are written to temporary names and renamed
+// once all exist (all or none).
+#include <cerrno>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+namespace hbg {
+
+namespace {
+
+const double PI_W = 3.14159265358979323846;
+
+bool exists(const std::string& p) {
+  struct stat st;
+  return ::stat(p.c_str(), &st) == 0;
+}
+
+// E<width>.<digits> of x, 1P (d.dddddd E+xx) or 0P (0.dddddd E+xx), right-aligned
+bool fortran_e(double x, bool one_p, int width, int digits, std::string& out) {
+  if (!std::isfinite(x)) return false;
+  if (x == 0.0) x = 0.0;
+  std::string body;
+  if (x == 0.0) {
+    body = "0." + std::string(digits, '0') + "E+00";
+  } else {
+    const int p = one_p ? digits : digits - 1;
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.*e", p, std::fabs(x));
+    std::string s(buf);
+    const size_t epos = s.find('e');
+    std::string mant = s.substr(0, epos);
+    int e = std::atoi(s.c_str() + epos + 1);
+    if (!one_p) {
+      std::string m;
+      for (char c : mant)
+        if (c != '.') m += c;
+      mant = "0." + m;
+      e += 1;
+    }
+    if (std::abs(e) > 99) return false;
+    char eb[8];
+    std::snprintf(eb, sizeof eb, "E%c%02d", e < 0 ? '-' : '+', std::abs(e));
+    body = mant + eb;
+  }
+  if (x < 0.0) body = "-" + body;
+  if ((int)body.size() > width) return false;
+  out += std::string(width - body.size(), ' ') + body;
+  return true;
+}
+
+bool fortran_i(long n, int width, std::string& out) {
+  std::string s = std::to_string(n);
+  if ((int)s.size() > width) return false;
+  out += std::string(width - s.size(), ' ') + s;
+  return true;
+}
+
+double degrees(std::complex<double> z) {
+  return (z == std::complex<double>(0.0, 0.0)) ? 0.0 : std::atan2(z.imag(), z.real()) * (180.0 / PI_W);
+}
+
+bool complex_cols(std::complex<double> z, std::string& out) {
+  return fortran_e(std::hypot(z.real(), z.imag()), true, 14, 6, out) && fortran_e(degrees(z), true, 14, 6, out) &&
+         fortran_e(z.real(), true, 14, 6, out) && fortran_e(z.imag(), true, 14, 6, out);
+}
+
+}  // namespace
+
+// A, B [F][D][D]; HK [F][D][H] complex (Output.HK order); X [F][H][D] complex.
+int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
+                const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
+                const double* X, const char* stamp, std::string& err) {
+  const int D = 6 * nb, K = D;
+  if (!(rho > 0.0) || !(g > 0.0) || !(L > 0.0)) {
+    err = "density, gravity and reference_length must be > 0";
+    return 1;
+  }
+  std::string st = stamp && *stamp ? stamp : "";
+  if (st.empty()) {
+    char buf[64];
+    std::time_t t = std::time(nullptr);
+    std::strftime(buf, sizeof buf, "%Y-%m-%d %H:%M:%S", std::localtime(&t));
+    st = buf;
+  }
+  if (st.find('\r') != std::string::npos || st.find('\n') != std::string::npos) {
+    err = "the time stamp must be a single line";
+    return 1;
+  }
+  auto r = [](int i) { return (i % 6) >= 3 ? 1 : 0; };
+  std::vector<std::string> per(F), bcol(H);
+  for (int f = 0; f < F; ++f)
+    if (!fortran_e(2.0 * PI_W / omegas[f], false, 14, 6, per[f])) {
+      err = "frequency does not fit E14.6";
+      return 1;
+    }
+  for (int h = 0; h < H; ++h)
+    if (!fortran_e(heads[h], false, 14, 6, bcol[h])) {
+      err = "heading does not fit E14.6";
+      return 1;
+    }
+  for (int a = 0; a < F; ++a)
+    for (int b = a + 1; b < F; ++b)
+      if (per[a] == per[b]) {
+        err = "distinct frequencies collide at the printed precision of the numeric files";
+        return 1;
+      }
+  std::vector<std::pair<std::string, std::vector<std::string>>> files;
+  {
+    std::vector<std::string> lines;
+    for (int f = 0; f < F; ++f)
+      for (int i = 0; i < D; ++i)
+        for (int k = 0; k < K; ++k) {
+          const double s = rho * std::pow(L, 3 + r(i) + r(k));
+          const size_t o = ((size_t)f * D + i) * K + k;
+          std::string l = per[f];
+          if (!fortran_i(i + 1, 6, l) || !fortran_i(k + 1, 6, l) || !fortran_e(A[o] / s, true, 14, 6, l) ||
+              !fortran_e(B[o] / (omegas[f] * s), true, 14, 6, l)) {
+            err = "value does not fit E14.6";
+            return 1;
+          }
+          lines.push_back(l);
+        }
+    files.push_back({".1", lines});
+  }
+  if (H > 0) {
+    std::vector<std::string> l2, l3;
+    for (int f = 0; f < F; ++f)
+      for (int h = 0; h < H; ++h) {
+        for (int n = 0; n < K; ++n) {
+          const size_t o = ((size_t)f * K + n) * H + h;
+          const double s = rho * g * std::pow(L, 2 + r(n));
+          std::string l = per[f] + bcol[h];
+          if (!fortran_i(n + 1, 6, l) || !complex_cols(std::complex<double>(HK[2 * o], HK[2 * o + 1]) / s, l)) {
+            err = "value does not fit E14.6";
+            return 1;
+          }
+          l2.push_back(l);
+        }
+        for (int i = 0; i < D; ++i) {
+          const size_t o = ((size_t)f * H + h) * D + i;
+          const double s = rho * g * std::pow(L, 2 + r(i));
+          std::string l = per[f] + bcol[h];
+          if (!fortran_i(i + 1, 6, l) || !complex_cols(std::complex<double>(X[2 * o], X[2 * o + 1]) / s, l)) {
+            err = "value does not fit E14.6";
+            return 1;
+          }
+          l3.push_back(l);
+        }
+      }
+    files.push_back({".2", l2});
+    files.push_back({".3", l3});
+  }
+  const std::string sstem(stem);
+  if (!overwrite)
+    for (auto& f : files)
+      if (exists(sstem + f.first)) {
+        err = sstem + f.first + " exists";
+        return 10;
+      }
+  std::string base = sstem;
+  const size_t slash = base.find_last_of("/\\");
+  if (slash != std::string::npos) base = base.substr(slash + 1);
+  std::vector<std::string> temps;
+  auto cleanup = [&]() {
+    for (auto& t : temps) std::remove(t.c_str());
+  };
+  for (size_t q = 0; q < files.size(); ++q) {
+    const std::string tmp = sstem + ".hbg-" + std::to_string((long)std::time(nullptr)) + "-" + std::to_string(q) + ".tmp";
+    FILE* fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) {
+      cleanup();
+      err = "cannot write " + tmp + ": " + std::strerror(errno);
+      return 9;
+    }
+    temps.push_back(tmp);
+    std::string text = " HydroBEM 0.1.0-amd   " + base + files[q].first + "   " + st + "\r\n";
+    for (auto& l : files[q].second) text += l + "\r\n";
+    const bool ok = std::fwrite(text.data(), 1, text.size(), fp) == text.size();
+    if (std::fclose(fp) != 0 || !ok) {
+      cleanup();
+      err = "cannot write " + tmp;
+      return 9;
+    }
+  }
+  for (size_t q = 0; q < files.size(); ++q)
+    if (std::rename(temps[q].c_str(), (sstem + files[q].first).c_str()) != 0) {
+      cleanup();
+      err = "cannot rename to " + sstem + files[q].first;
+      return 9;
+    }
+  return 0;
+}
+
+}  // namespace hbg

@@ -1,0 +1,135 @@
+/* hydrobem_amd: the low-order frequency-domain solve of HydroBEM on one AMD MI355X (gfx950).
+ *
+ * A C ABI (plain pointers and sizes, no exceptions, caller-owned buffers) that a Rust engine binds
+ * through a thin FFI crate. Conventions follow the reference's C ABI
+ * (crates/hydrobem-c/include/hydrobem.h; docs/reference/c-api.md:49-146):
+ *   - every fallible call returns an HbgStatus with the reference's HbStatus values
+ *     (hydrobem.h:42-98: 0 OK, 1 VALUE, 2 TYPE, 3 UNSUPPORTED, 4 STATE, 5 CANCELLED, 6 INTERNAL,
+ *     7 ARGUMENT, 8 NUMERICAL, 9 IO, 10 EXISTS);
+ *   - the message and location of the last failure are kept per thread
+ *     (hb_last_error_message / hb_last_error_location, hydrobem.h);
+ *   - out-pointers are checked before anything changes; inputs are read during the call only;
+ *   - arrays are row-major C arrays unless stated; complex values are interleaved (re, im) doubles.
+ *
+ * What the functions replace: the engine seam solver::run::<R: Real> (crates/hydrobem/src/
+ * solver.rs:265-345) with its statics (lo_static.rs:254-417) and its per-frequency
+ * solve_frequency (lo_frequency.rs:1565-1635) for wetted bodies in infinite depth without
+ * symmetry planes; the public seam hb_model_solve + hb_results_quantity (hydrobem.h:646, 1164)
+ * for the quantities added_mass, radiation_damping, excitation_diffraction,
+ * excitation_froude_krylov, excitation_scattering, excitation_haskind
+ * (solve_driver.rs:336-600); hb_results_write_wamit (hydrobem.h:1297) for the .1/.2/.3 files.
+ */
+#ifndef HYDROBEM_AMD_H
+#define HYDROBEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t HbgStatus;
+enum {
+  HBG_OK = 0,
+  HBG_ERR_VALUE = 1,
+  HBG_ERR_TYPE = 2,
+  HBG_ERR_UNSUPPORTED = 3,
+  HBG_ERR_STATE = 4,
+  HBG_ERR_CANCELLED = 5,
+  HBG_ERR_INTERNAL = 6,
+  HBG_ERR_ARGUMENT = 7,
+  HBG_ERR_NUMERICAL = 8,
+  HBG_ERR_IO = 9,
+  HBG_ERR_EXISTS = 10
+};
+
+/* Stages of the progress callback (progress.rs:425-480 Stage::{Statics, Assembly, Lu, Outputs}). */
+enum { HBG_STAGE_STATICS = 0, HBG_STAGE_ASSEMBLY = 1, HBG_STAGE_LU = 2, HBG_STAGE_OUTPUTS = 3 };
+
+/* Progress callback (hydrobem.h:481-499 HbProgressFn): called on the caller's thread between
+ * stages with the frequency index done of total; returning 0 cancels the solve
+ * (HBG_ERR_CANCELLED), as the reference's callback does. */
+typedef int32_t (*HbgProgressFn)(void* user, int32_t stage, int32_t done, int32_t total);
+
+/* Precision of a solve. HBG_PRECISION_FP64: complex128 assembly and LU on the GPU (the
+ * reference's double mode, env_hooks.rs:51-56). */
+enum { HBG_PRECISION_FP64 = 0 };
+
+typedef struct HbgOptions {
+  int32_t precision;     /* HBG_PRECISION_FP64 */
+  int32_t logarithmic;   /* analytic log-singularity integral (document_schema.rs:218, default 1) */
+  double log_tolerance;  /* document_schema.rs:219, default 1e-4 */
+} HbgOptions;
+
+typedef struct HbgSession HbgSession;
+
+/* Thread-local message and location of the last failure ("" if none). */
+const char* hbg_last_error_message(void);
+const char* hbg_last_error_location(void);
+
+/* Library version string (static storage). */
+const char* hbg_version(void);
+
+/* A session on GPU `device`: owns the device buffers of one geometry and its statics.
+ * Replaces the model handle of hb_model_new / hb_model_free (hydrobem.h). */
+HbgStatus hbg_session_create(int32_t device, HbgSession** out);
+HbgStatus hbg_session_free(HbgSession* s);
+
+/* Environment (problem.rs env; ρ, g, depth); depth must be +inf (infinite depth). */
+HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, double depth);
+
+/* The wetted panels: vertices [n_panels][4][3] as the mesh gives them (a triangle repeats a
+ * vertex), normals into the fluid by the vertex order; body [n_panels] (NULL: all body 0);
+ * rotation_centres [n_bodies][3] (NULL: the origin). Each panel passes the reference's panel rule
+ * (geometry.rs:120-199) or the call fails with HBG_ERR_VALUE naming the panel. Replaces the
+ * geometry build of lo_geometry.rs:1100-1113 (body frame, no symmetry). Invalidates the statics. */
+HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
+                         int32_t n_bodies, const double* rotation_centres);
+
+/* The solve of n_omegas frequencies (rad/s) and n_headings headings (degrees), radiation of the
+ * six rigid modes of every body plus diffraction (solver::run, lo_frequency.rs:1565-1635).
+ * D = 6 n_bodies. Outputs (any may be NULL):
+ *   added_mass, damping         [F][D][D]                      (A_ij, B_ij)
+ *   excitation, froude_krylov,
+ *   scattering                  [F][H][D] complex              (X_i = FK + SC)
+ *   haskind                     [F][D][H] complex              (Haskind exciting force, Output.HK)
+ * HBG_ERR_NUMERICAL on a zero pivot ("singular system matrix (zero pivot k)") or a non-finite
+ * matrix or solution (linalg.rs:221-252); HBG_ERR_CANCELLED when the callback returns 0. */
+HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const double* headings_deg,
+                    int32_t n_headings, const HbgOptions* options, double* added_mass, double* damping,
+                    double* excitation, double* froude_krylov, double* scattering, double* haskind,
+                    HbgProgressFn progress, void* user);
+
+/* ---- kernel-level entry points (the granularity of the reference's component tests,
+ * SURVEY call-stack 3.4): */
+
+/* The infinite-depth wave kernel at n points (green_inf.rs:316-368 wave_kernel):
+ * out [n][7] = Ft, Ft_X, Ft_Y, Gi, Gi_X, Gi_Y, exp(-Y) J0(X), for X = K R >= 0, Y = K|z+zeta| >= 0. */
+HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64_t n, double* out);
+
+/* The statics of the current panels (lo_static.rs:59-119): out [N][N][4] = rs, rd, dl, dd for
+ * field point k (row) and panel j (column), logarithmic / log_tolerance as in HbgOptions. */
+HbgStatus hbg_statics(HbgSession* s, int32_t logarithmic, double log_tolerance, double* out);
+
+/* The influence matrices of one frequency (lo_frequency.rs:399-537, G = 1): S and A = 2πI - D,
+ * complex [N][N] row-major (S[k][n] = influence of panel n at field point k). */
+HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, double* S, double* A);
+
+/* A x = b on the GPU in FP64 (linalg.rs:321-328 lu_solve_c128): a [n][n] and b [nrhs][n]
+ * complex, column-major (a[j][i] = A(i, j)), overwritten by the factors and the solution. */
+HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b);
+
+/* WAMIT .1 (added mass and damping), .2 (Haskind exciting force) and .3 (pressure-integrated
+ * exciting force) files of a solve (wamit_files.rs:794-867, docs/reference/file-formats.md):
+ * stem + ".1" etc., lengths normalised by reference_length; overwrite 0 fails with HBG_ERR_EXISTS
+ * if a file exists. Written all or none. */
+HbgStatus hbg_write_wamit(const char* stem, int32_t overwrite, double density, double gravity,
+                          double reference_length, const double* omegas, int32_t n_omegas,
+                          const double* headings_deg, int32_t n_headings, int32_t n_bodies,
+                          const double* added_mass, const double* damping, const double* haskind,
+                          const double* excitation, const char* stamp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+"""The C-ABI library on the CPU side: it loads, exports every symbol include/hydrobem_amd.h
+declares, refuses bad arguments before touching a GPU, and reports a missing GPU as a state
+error (no compute calls here)."""
+
+import ctypes
+import re
+
+import pytest
+
+import hydrobem_amd as hb
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "hydrobem_amd.h").read_text()
+    return sorted(set(re.findall(r"\b(hbg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = hb.library()
+    names = declared_symbols()
+    assert len(names) >= 12
+    for name in names:
+        assert hasattr(L, name), name
+        assert name in hb.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_version_and_errors():
+    L = hb.library()
+    assert b"gfx950" in L.hbg_version()
+    h = ctypes.c_void_p()
+    assert L.hbg_session_create(0, None) == 7  # ARGUMENT: out-pointer checked first
+    assert b"null" in L.hbg_last_error_message()
+    assert L.hbg_set_environment(None, 1025.0, 9.81, float("inf")) == 7
+    assert L.hbg_session_free(None) == 0
+
+
+@pytest.mark.skipif(__import__("torch").cuda.is_available(), reason="GPU present: covered by the GPU tests")
+def test_session_without_gpu_is_a_state_error():
+    with pytest.raises(hb.HbgError) as e:
+        hb.Session()
+    assert e.value.status in (4,)
+
+
+def test_wamit_files_on_host(tmp_path):
+    import numpy as np
+
+    F, H, D = 2, 1, 6
+    rng = np.random.default_rng(0)
+    res = {"omegas": np.array([0.5, 1.0]), "headings": np.array([0.0]),
+           "added_mass": rng.normal(size=(F, D, D)) * 1e3, "radiation_damping": rng.normal(size=(F, D, D)) * 1e2,
+           "excitation_diffraction": rng.normal(size=(F, H, D)) + 1j * rng.normal(size=(F, H, D)),
+           "excitation_haskind": rng.normal(size=(F, H, D)) + 1j * rng.normal(size=(F, H, D))}
+    stem = tmp_path / "body"
+    hb.wamit(stem, res, density=1025.0, gravity=9.81, stamp="2026-01-01 00:00:00")
+    t1 = (tmp_path / "body.1").read_bytes().decode()
+    lines = t1.split("\r\n")
+    assert lines[0] == " HydroBEM 0.1.0-amd   body.1   2026-01-01 00:00:00"
+    assert len(lines) == 1 + F * D * D + 1 and lines[-1] == ""
+    # PER I J Abar Bbar: E14.6 (0P), I6, I6, E14.6 (1P), E14.6 (1P)
+    row = lines[1]
+    assert len(row) == 14 + 6 + 6 + 14 + 14
+    per = float(row[:14])
+    assert per == pytest.approx(2 * np.pi / 0.5, rel=5e-6)  # 0P: six significant digits
+    assert row[:14].strip().startswith("0.")
+    assert int(row[14:20]) == 1 and int(row[20:26]) == 1
+    assert float(row[26:40]) == pytest.approx(res["added_mass"][0, 0, 0] / 1025.0, rel=1e-6)
+    assert float(row[40:54]) == pytest.approx(res["radiation_damping"][0, 0, 0] / (0.5 * 1025.0), rel=1e-6)
+    t3 = (tmp_path / "body.3").read_bytes().decode().split("\r\n")
+    r3 = t3[1]
+    assert len(r3) == 14 * 2 + 6 + 4 * 14
+    z = res["excitation_diffraction"][0, 0, 0] / (1025.0 * 9.81)
+    assert float(r3[34:48]) == pytest.approx(abs(z), rel=1e-6)
+    assert float(r3[48:62]) == pytest.approx(np.degrees(np.angle(z)), rel=1e-6, abs=1e-5)
+    # overwrite refused
+    with pytest.raises(hb.HbgError) as e:
+        hb.wamit(stem, res, density=1025.0, gravity=9.81, overwrite=False)
+    assert e.value.status == 10
