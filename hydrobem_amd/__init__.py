@@ -61,6 +61,8 @@ SIGNATURES = {
     "hbg_statics": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp]),
     "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
+    "hbg_set_profiling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "hbg_profile_read": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
     "hbg_write_wamit": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
                                          _dp, _dp, _dp, _dp, ctypes.c_char_p]),
@@ -154,6 +156,17 @@ class Session:
         return {"omegas": w, "headings": h, "added_mass": A, "radiation_damping": B,
                 "excitation_diffraction": X, "excitation_froude_krylov": FK, "excitation_scattering": SC,
                 "excitation_haskind": np.transpose(HK, (0, 2, 1)).copy()}
+
+    PROFILE_KEYS = ("statics_ms", "assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "lu_gemm_flops",
+                    "lu_gemm_launches", "forces_ms", "frequencies", "lu_flops", "assembly_bytes", "statics_runs")
+
+    def set_profiling(self, enable: bool):
+        _check(library().hbg_set_profiling(self._h, int(bool(enable))))
+
+    def profile(self) -> dict:
+        out = np.zeros(16)
+        _check(library().hbg_profile_read(self._h, _p(out), 16))
+        return {k: float(out[i]) for i, k in enumerate(self.PROFILE_KEYS)}
 
     def wave_kernel(self, X, Y):
         x = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)

@@ -28,7 +28,11 @@ void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, doubl
 void launch_forces(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double rho,
                    const double2* M, int ldm, const double2* phiI, const double2* dphiI, double* out,
                    hipStream_t stream);
-int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream);
+struct LuStats {
+  double total_ms = 0.0, gemm_ms = 0.0, gemm_flops = 0.0;
+  long gemm_launches = 0;
+};
+int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
                 const double* X, const char* stamp, std::string& err);
@@ -242,6 +246,10 @@ struct HbgSession {
   int st_logint = -1;
   double st_tol = -1.0;
   WaveTables T;
+  // profiling (hbg_set_profiling): GPU milliseconds of the stages, by HIP events on the stream
+  bool profiling = false;
+  double prof[HBG_PROFILE_COUNTERS] = {};
+  hipEvent_t ev[8] = {};
 };
 
 namespace {
@@ -271,6 +279,7 @@ void ensure_statics(HbgSession* s, int logint, double tol) {
   const int N = s->N;
   s->d_st.ensure((size_t)N * N);
   unsigned qcap = (unsigned)std::min<size_t>((size_t)N * 64, (size_t)1 << 28);
+  if (s->profiling) HBG_HIP(hipEventRecord(s->ev[0], s->stream));
   for (int attempt = 0; attempt < 2; ++attempt) {
     s->d_queue.ensure(qcap);
     s->d_qcount.ensure(1);
@@ -282,7 +291,14 @@ void ensure_statics(HbgSession* s, int logint, double tol) {
     HBG_HIP(hipStreamSynchronize(s->stream));
     if (count <= qcap) {
       launch_adaptive(s->d_panels.p, N, s->d_gl.p, s->d_st.p, s->d_queue.p, count, s->stream);
+      if (s->profiling) HBG_HIP(hipEventRecord(s->ev[1], s->stream));
       HBG_HIP(hipStreamSynchronize(s->stream));
+      if (s->profiling) {
+        float ms = 0.0f;
+        HBG_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+        s->prof[0] += ms;
+        s->prof[11] += 1.0;
+      }
       s->statics_valid = true;
       s->st_logint = logint;
       s->st_tol = tol;
@@ -325,6 +341,7 @@ HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
   s->device = device;
   HBG_HIP(hipSetDevice(device));
   HBG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  for (auto& e : s->ev) HBG_HIP(hipEventCreate(&e));
   TablesOnDevice& t = tables_for(device);
   s->T.t2d = t.t2d;
   s->T.t1d = t.t1d;
@@ -355,6 +372,8 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_phiI.release();
   s->d_dphiI.release();
   s->d_u.release();
+  for (auto& e : s->ev)
+    if (e) hipEventDestroy(e);
   hipStreamDestroy(s->stream);
   delete s;
   return HBG_OK;
@@ -510,7 +529,7 @@ HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double
   if (nrhs > 0)
     HBG_HIP(hipMemcpyAsync(m.p + (size_t)n * n, b, (size_t)n * nrhs * sizeof(double2), hipMemcpyHostToDevice,
                            s->stream));
-  const int zp = lu_solve_device(m.p, n, n, nrhs, s->stream);
+  const int zp = lu_solve_device(m.p, n, n, nrhs, s->stream, nullptr);
   HBG_HIP(hipMemcpyAsync(a, m.p, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
   if (nrhs > 0)
     HBG_HIP(hipMemcpyAsync(b, m.p + (size_t)n * n, (size_t)n * nrhs * sizeof(double2), hipMemcpyDeviceToHost,
@@ -559,18 +578,41 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   for (int fi = 0; fi < n_omegas; ++fi) {
     if (progress && progress(user, HBG_STAGE_ASSEMBLY, fi, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     const double w = omegas[fi], K = w * w / s->grav;
+    const bool prof = s->profiling;
+    if (prof) HBG_HIP(hipEventRecord(s->ev[2], s->stream));
     launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, K, s->grav, s->d_heads.p, s->d_phiI.p,
                     s->d_dphiI.p, s->d_u.p, s->stream);
     launch_assemble(s->d_panels.p, N, K, o.logarithmic ? 1.0 : 0.0, s->T, s->d_st.p, s->d_S.p, s->d_M.p, N,
                     s->stream);
+    if (prof) HBG_HIP(hipEventRecord(s->ev[3], s->stream));
     launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->stream);
+    if (prof) HBG_HIP(hipEventRecord(s->ev[4], s->stream));
     if (progress && progress(user, HBG_STAGE_LU, fi, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
-    const int zp = lu_solve_device(s->d_M.p, N, N, R, s->stream);
+    LuStats lus;
+    const int zp = lu_solve_device(s->d_M.p, N, N, R, s->stream, prof ? &lus : nullptr);
     if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(zp) + ")"};
+    if (prof) HBG_HIP(hipEventRecord(s->ev[5], s->stream));
     launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, s->rho, s->d_M.p, N, s->d_phiI.p,
                   s->d_dphiI.p, s->d_out.p, s->stream);
+    if (prof) HBG_HIP(hipEventRecord(s->ev[6], s->stream));
     HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p, nout * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
+    if (prof) {
+      float ms = 0.0f;
+      HBG_HIP(hipEventElapsedTime(&ms, s->ev[2], s->ev[3]));
+      s->prof[1] += ms;
+      HBG_HIP(hipEventElapsedTime(&ms, s->ev[3], s->ev[4]));
+      s->prof[2] += ms;
+      s->prof[3] += lus.total_ms;
+      s->prof[4] += lus.gemm_ms;
+      s->prof[5] += lus.gemm_flops;
+      s->prof[6] += (double)lus.gemm_launches;
+      HBG_HIP(hipEventElapsedTime(&ms, s->ev[5], s->ev[6]));
+      s->prof[7] += ms;
+      s->prof[8] += 1.0;
+      s->prof[9] += 8.0 / 3.0 * (double)N * N * N;
+      s->prof[10] += (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
+    }
     for (double v : hout)
       if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
     const double* hA = hout.data();
@@ -590,6 +632,20 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_CANCELLED, "cancelled"};
   return HBG_OK;
   GUARD_END("hbg_solve")
+}
+
+HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_profiling");
+  s->profiling = enable != 0;
+  for (double& v : s->prof) v = 0.0;
+  return HBG_OK;
+}
+
+HbgStatus hbg_profile_read(HbgSession* s, double* out, int32_t n) {
+  if (!s || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_profile_read");
+  for (int i = 0; i < n && i < HBG_PROFILE_COUNTERS; ++i) out[i] = s->prof[i];
+  for (int i = HBG_PROFILE_COUNTERS; i < n; ++i) out[i] = 0.0;
+  return HBG_OK;
 }
 
 HbgStatus hbg_write_wamit(const char* stem, int32_t overwrite, double density, double gravity,

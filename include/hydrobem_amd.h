@@ -100,8 +100,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                     double* excitation, double* froude_krylov, double* scattering, double* haskind,
                     HbgProgressFn progress, void* user);
 
-/* ---- kernel-level entry points (the granularity of the reference's component tests,
- * SURVEY call-stack 3.4): */
+/* ---- kernel-level entry points, at the granularity of the reference's component tests
+ * (crates/hydrobem/tests/core.rs, lo_a.rs; linalg.rs:422-489): */
 
 /* The infinite-depth wave kernel at n points (green_inf.rs:316-368 wave_kernel):
  * out [n][7] = Ft, Ft_X, Ft_Y, Gi, Gi_X, Gi_Y, exp(-Y) J0(X), for X = K R >= 0, Y = K|z+zeta| >= 0. */
@@ -118,6 +118,17 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
 /* A x = b on the GPU in FP64 (linalg.rs:321-328 lu_solve_c128): a [n][n] and b [nrhs][n]
  * complex, column-major (a[j][i] = A(i, j)), overwritten by the factors and the solution. */
 HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b);
+
+/* Stage timing (the reference's HB_PROFILE phase marks, low_order.rs:184-208), by HIP events on the
+ * session's stream: enable (clears the counters) or disable. hbg_profile_read copies up to n of the
+ * HBG_PROFILE_COUNTERS counters, accumulated over the solves since enabling:
+ *   0 statics ms, 1 assembly ms (boundary values + matrices), 2 right-hand sides ms, 3 LU ms
+ *   (factorization + substitution), 4 LU trailing-update GEMM ms, 5 their algorithmic flops
+ *   (8 m n k each), 6 GEMM launches, 7 forces ms, 8 frequencies, 9 LU algorithmic flops
+ *   (8 N^3 / 3 each), 10 assembly algorithmic bytes (N^2 (32 + 2 x 16) each), 11 statics runs. */
+#define HBG_PROFILE_COUNTERS 16
+HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable);
+HbgStatus hbg_profile_read(HbgSession* s, double* out, int32_t n);
 
 /* WAMIT .1 (added mass and damping), .2 (Haskind exciting force) and .3 (pressure-integrated
  * exciting force) files of a solve (wamit_files.rs:794-867, docs/reference/file-formats.md):
