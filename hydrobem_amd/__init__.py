@@ -58,6 +58,9 @@ SIGNATURES = {
                                    ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
                                    ctypes.c_void_p]),
     "hbg_wave_kernel": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, ctypes.c_int64, _dp]),
+    "hbg_finite_kernel": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _dp,
+                                           _dp, _dp, _dp]),
+    "hbg_dispersion_roots": (ctypes.c_int32, [ctypes.c_double, ctypes.c_int32, _dp]),
     "hbg_statics": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp]),
     "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
@@ -175,6 +178,15 @@ class Session:
         _check(library().hbg_wave_kernel(self._h, _p(x), _p(y), len(x), _p(out)))
         return out
 
+    def finite_kernel(self, h, K, R, z, zeta):
+        """(Wr, WrR, WrZ, Wrz, Wi, WiR, WiZ, Wiz) of the finite-depth kernel at point pairs."""
+        R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
+        z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+        zeta = np.ascontiguousarray(zeta, dtype=np.float64).reshape(-1)
+        out = np.zeros((len(R), 8))
+        _check(library().hbg_finite_kernel(self._h, float(h), float(K), len(R), _p(R), _p(z), _p(zeta), _p(out)))
+        return out
+
     def statics(self, logarithmic=True, log_tolerance=1e-4):
         out = np.zeros((self.n_panels, self.n_panels, 4))
         _check(library().hbg_statics(self._h, int(bool(logarithmic)), float(log_tolerance), _p(out)))
@@ -195,6 +207,13 @@ class Session:
         bb = np.array(b, dtype=np.complex128, order="F", copy=True).reshape(n, -1, order="F")
         _check(library().hbg_lu_solve(self._h, n, af.ctypes.data_as(_dp), bb.shape[1], bb.ctypes.data_as(_dp)))
         return bb.reshape(b.shape, order="F")
+
+
+def dispersion_roots(H, n):
+    """[x0, x1, ..., xn] of x0 tanh x0 = H and xi tan xi = -H."""
+    out = np.zeros(n + 1)
+    _check(library().hbg_dispersion_roots(float(H), int(n), _p(out)))
+    return out
 
 
 def wamit(stem, result, density, gravity, reference_length=1.0, overwrite=True, stamp="", n_bodies=1):

@@ -9,20 +9,31 @@
 #include <vector>
 
 #include "../../include/hydrobem_amd.h"
+#include "green_fin_dev.h"
 #include "hbg_internal.h"
 
 namespace hbg {
 
 // kernels (statics.hip, frequency.hip, lu.hip)
-void launch_statics(const PanelD* P, int N, int logint, double tol, const double* gl, StaticD* st, int2* queue,
-                    unsigned* qcount, unsigned qcap, hipStream_t stream);
+void launch_statics(const PanelD* P, int N, int logint, double tol, double depth, const double* gl, StaticD* st,
+                    int2* queue, unsigned* qcount, unsigned qcap, hipStream_t stream);
+void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
+                         const StaticD* st, double2* S, double2* A, int lda, hipStream_t stream);
+void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
+                            const double* zeta, int n, double* out, hipStream_t stream);
+void launch_fin_tables(const FinFreq& f, WaveTables T, const double* g16, double* phi1, double* phi2,
+                       const double* M1dev, const PanelD* P, int N, double4* pf, double2* pcs, hipStream_t stream);
+FinFreq make_fin_freq(double h, double K);
+double dispersion_root(double H, int n);
+std::vector<double> fit_matrix(int M);
+std::vector<double> build_k_table();
 void launch_adaptive(const PanelD* P, int N, const double* gl, StaticD* st, const int2* queue, unsigned count,
                      hipStream_t stream);
 void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
                      double2* A, int lda, hipStream_t stream);
 void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream);
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
-                     double grav, const double* heads, double2* phiI, double2* dphiI, double2* u,
+                     double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
                      hipStream_t stream);
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, hipStream_t stream);
 void launch_forces(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double rho,
@@ -200,6 +211,8 @@ struct TablesOnDevice {
   int device = -1;
   double* t2d = nullptr;
   double* t1d = nullptr;
+  double* ktab = nullptr;  // scaled K0, K1 tiles of the finite-depth series
+  double* fitm = nullptr;  // node values -> monomial coefficients of the Phi tiles
 };
 TablesOnDevice& tables_for(int device) {
   static TablesOnDevice tabs[16];
@@ -211,6 +224,11 @@ TablesOnDevice& tables_for(int device) {
   HBG_HIP(hipMalloc(&t.t1d, h1.size() * sizeof(double)));
   HBG_HIP(hipMemcpy(t.t2d, h2.data(), h2.size() * sizeof(double), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(t.t1d, h1.data(), h1.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> kt = build_k_table(), fm = fit_matrix(FM);
+  HBG_HIP(hipMalloc(&t.ktab, kt.size() * sizeof(double)));
+  HBG_HIP(hipMalloc(&t.fitm, fm.size() * sizeof(double)));
+  HBG_HIP(hipMemcpy(t.ktab, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(t.fitm, fm.data(), fm.size() * sizeof(double), hipMemcpyHostToDevice));
   t.device = device;
   return t;
 }
@@ -246,6 +264,13 @@ struct HbgSession {
   int st_logint = -1;
   double st_tol = -1.0;
   WaveTables T;
+  // finite depth: per-frequency tiles and per-panel factors
+  DevBuf<double> d_phi1, d_phi2;
+  DevBuf<double4> d_pf;
+  DevBuf<double2> d_pcs;
+  DevBuf<FinFreq> d_ff;
+  const double* ktab = nullptr;
+  const double* fitm = nullptr;
   // profiling (hbg_set_profiling): GPU milliseconds of the stages, by HIP events on the stream
   bool profiling = false;
   double prof[HBG_PROFILE_COUNTERS] = {};
@@ -284,8 +309,8 @@ void ensure_statics(HbgSession* s, int logint, double tol) {
     s->d_queue.ensure(qcap);
     s->d_qcount.ensure(1);
     HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
-    launch_statics(s->d_panels.p, N, logint, tol, s->d_gl.p, s->d_st.p, s->d_queue.p, s->d_qcount.p, qcap,
-                   s->stream);
+    launch_statics(s->d_panels.p, N, logint, tol, std::isfinite(s->depth) ? s->depth : 0.0, s->d_gl.p, s->d_st.p,
+                   s->d_queue.p, s->d_qcount.p, qcap, s->stream);
     unsigned count = 0;
     HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
@@ -307,6 +332,46 @@ void ensure_statics(HbgSession* s, int logint, double tol) {
     qcap = count;
   }
   throw Error{HBG_ERR_INTERNAL, "statics: adaptive queue overflow"};
+}
+
+// the depths of the panels must lie in the water: -h <= z < 0 at the centroids
+void check_depth(HbgSession* s) {
+  if (!std::isfinite(s->depth)) return;
+  for (int j = 0; j < s->N; ++j)
+    if (s->panels[j].c[2] < -s->depth)
+      throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": centroid below the sea bed"};
+}
+
+FinTables fin_tables(HbgSession* s) {
+  FinTables F;
+  F.phi1 = s->d_phi1.p;
+  F.phi2 = s->d_phi2.p;
+  F.ktab = s->ktab;
+  F.pf = s->d_pf.p;
+  F.pcs = s->d_pcs.p;
+  return F;
+}
+
+// S and A of the frequency K into S (ld N) and M (ld N): infinite depth, or finite depth with the
+// per-frequency tables built first; returns the wave number of the incident wave
+double assemble_frequency(HbgSession* s, double K, double logc) {
+  const int N = s->N;
+  if (!std::isfinite(s->depth)) {
+    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, s->d_M.p, N, s->stream);
+    return K;
+  }
+  const FinFreq f = make_fin_freq(s->depth, K);
+  s->d_phi1.ensure((size_t)FTS * FTW * FTILE);
+  s->d_phi2.ensure((size_t)FTS * FTV * FTILE);
+  s->d_pf.ensure(N);
+  s->d_pcs.ensure((size_t)N * FIN_ROOTS);
+  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1.p, s->d_phi2.p, s->fitm, s->d_panels.p, N, s->d_pf.p,
+                    s->d_pcs.p, s->stream);
+  s->d_ff.ensure(1);
+  HBG_HIP(hipMemcpyAsync(s->d_ff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, s->stream));
+  launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, s->d_M.p, N,
+                      s->stream);
+  return f.k0;
 }
 
 void check_options(const HbgOptions* o) {
@@ -345,6 +410,8 @@ HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
   TablesOnDevice& t = tables_for(device);
   s->T.t2d = t.t2d;
   s->T.t1d = t.t1d;
+  s->ktab = t.ktab;
+  s->fitm = t.fitm;
   std::vector<double> gl = gauss_table();
   s->d_gl.ensure(gl.size());
   HBG_HIP(hipMemcpy(s->d_gl.p, gl.data(), gl.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -372,6 +439,11 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_phiI.release();
   s->d_dphiI.release();
   s->d_u.release();
+  s->d_phi1.release();
+  s->d_phi2.release();
+  s->d_pf.release();
+  s->d_pcs.release();
+  s->d_ff.release();
   for (auto& e : s->ev)
     if (e) hipEventDestroy(e);
   hipStreamDestroy(s->stream);
@@ -385,7 +457,7 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
   if (!(density > 0.0) || !std::isfinite(density)) return fail(HBG_ERR_VALUE, "density must be > 0", "hbg_set_environment");
   if (!(gravity > 0.0) || !std::isfinite(gravity)) return fail(HBG_ERR_VALUE, "gravity must be > 0", "hbg_set_environment");
   if (!(depth > 0.0)) return fail(HBG_ERR_VALUE, "depth must be > 0", "hbg_set_environment");
-  if (std::isfinite(depth)) return fail(HBG_ERR_UNSUPPORTED, "finite depth is not supported", "hbg_set_environment");
+  if (depth != s->depth) s->statics_valid = false;
   s->rho = density;
   s->grav = gravity;
   s->depth = depth;
@@ -464,6 +536,78 @@ HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64
   GUARD_END("hbg_wave_kernel")
 }
 
+HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
+                            const double* zeta, double* out) {
+  if (!s || !R || !z || !zeta || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_finite_kernel");
+  if (!(h > 0.0) || !std::isfinite(h) || !(K > 0.0) || !std::isfinite(K) || n < 0)
+    return fail(HBG_ERR_VALUE, "h and K must be finite and > 0", "hbg_finite_kernel");
+  for (int i = 0; i < n; ++i)
+    if (!(R[i] >= 0.0) || !(z[i] <= 0.0) || !(zeta[i] <= 0.0) || z[i] < -h || zeta[i] < -h ||
+        (R[i] == 0.0 && z[i] + zeta[i] == 0.0))
+      return fail(HBG_ERR_VALUE, "point " + std::to_string(i) + " outside R >= 0, -h <= z, zeta <= 0",
+                  "hbg_finite_kernel");
+  if (n == 0) return HBG_OK;
+  GUARD_BEGIN
+  use_device(s);
+  const FinFreq f = make_fin_freq(h, K);
+  // the point depths as panels: z[i] at i, zeta[i] at n + i
+  std::vector<PanelD> pts(2 * (size_t)n);
+  std::memset(pts.data(), 0, pts.size() * sizeof(PanelD));
+  for (int i = 0; i < n; ++i) {
+    pts[i].c[2] = z[i];
+    pts[n + i].c[2] = zeta[i];
+  }
+  DevBuf<PanelD> dp;
+  DevBuf<double> dphi1, dphi2, dR, dz, dzeta, dout;
+  DevBuf<double4> dpf;
+  DevBuf<double2> dpcs;
+  dp.ensure(pts.size());
+  dphi1.ensure((size_t)FTS * FTW * FTILE);
+  dphi2.ensure((size_t)FTS * FTV * FTILE);
+  dpf.ensure(pts.size());
+  dpcs.ensure(pts.size() * FIN_ROOTS);
+  dR.ensure(n);
+  dz.ensure(n);
+  dzeta.ensure(n);
+  dout.ensure(8 * (size_t)n);
+  HBG_HIP(hipMemcpy(dp.p, pts.data(), pts.size() * sizeof(PanelD), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dR.p, R, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dz.p, z, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dzeta.p, zeta, n * sizeof(double), hipMemcpyHostToDevice));
+  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, dphi1.p, dphi2.p, s->fitm, dp.p, 2 * n, dpf.p, dpcs.p, s->stream);
+  FinTables F;
+  F.phi1 = dphi1.p;
+  F.phi2 = dphi2.p;
+  F.ktab = s->ktab;
+  F.pf = dpf.p;
+  F.pcs = dpcs.p;
+  DevBuf<FinFreq> dff;
+  dff.ensure(1);
+  HBG_HIP(hipMemcpy(dff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice));
+  launch_fin_points_eval(dff.p, F, s->T, dR.p, dz.p, dzeta.p, n, dout.p, s->stream);
+  HBG_HIP(hipMemcpyAsync(out, dout.p, 8 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  dp.release();
+  dphi1.release();
+  dphi2.release();
+  dR.release();
+  dz.release();
+  dzeta.release();
+  dout.release();
+  dpf.release();
+  dpcs.release();
+  dff.release();
+  return HBG_OK;
+  GUARD_END("hbg_finite_kernel")
+}
+
+HbgStatus hbg_dispersion_roots(double H, int32_t n, double* out) {
+  if (!out || n < 0) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_dispersion_roots");
+  if (!(H >= 0.0)) return fail(HBG_ERR_VALUE, "H must be >= 0", "hbg_dispersion_roots");
+  for (int i = 0; i <= n; ++i) out[i] = dispersion_root(H, i);
+  return HBG_OK;
+}
+
 HbgStatus hbg_statics(HbgSession* s, int32_t logarithmic, double log_tolerance, double* out) {
   if (!s || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_statics");
   if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_statics");
@@ -499,7 +643,8 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   s->d_S.ensure((size_t)N * N);
   s->d_M.ensure((size_t)N * N);
   const double K = omega * omega / s->grav;
-  launch_assemble(s->d_panels.p, N, K, o.logarithmic ? 1.0 : 0.0, s->T, s->d_st.p, s->d_S.p, s->d_M.p, N, s->stream);
+  check_depth(s);
+  assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0);
   std::vector<double2> hs((size_t)N * N), ha((size_t)N * N);
   HBG_HIP(hipMemcpyAsync(hs.data(), s->d_S.p, hs.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
   HBG_HIP(hipMemcpyAsync(ha.data(), s->d_M.p, ha.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
@@ -560,6 +705,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   use_device(s);
   const int N = s->N, H = n_headings, nrad = 6 * s->nb, D = nrad, R = nrad + H;
   if (R > 32) throw Error{HBG_ERR_UNSUPPORTED, "more than 32 right-hand sides"};
+  check_depth(s);
   if (progress && progress(user, HBG_STAGE_STATICS, 0, 1) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
   ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
   s->d_S.ensure((size_t)N * N);
@@ -580,9 +726,9 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     const double w = omegas[fi], K = w * w / s->grav;
     const bool prof = s->profiling;
     if (prof) HBG_HIP(hipEventRecord(s->ev[2], s->stream));
-    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, K, s->grav, s->d_heads.p, s->d_phiI.p,
-                    s->d_dphiI.p, s->d_u.p, s->stream);
-    launch_assemble(s->d_panels.p, N, K, o.logarithmic ? 1.0 : 0.0, s->T, s->d_st.p, s->d_S.p, s->d_M.p, N,
+    const double kw = assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0);
+    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, kw, s->grav,
+                    std::isfinite(s->depth) ? s->depth : 0.0, s->d_heads.p, s->d_phiI.p, s->d_dphiI.p, s->d_u.p,
                     s->stream);
     if (prof) HBG_HIP(hipEventRecord(s->ev[3], s->stream));
     launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->stream);

@@ -2,6 +2,7 @@
 // matrices, the right-hand sides S U and the forces (lo_frequency.rs:214-273, 399-537, 823-955,
 // 1039-1171; low_order.rs:627-648, 717-790, 1401-1485), for wetted panels without symmetry
 // (one character, chi = 1) in infinite depth.
+#include "green_fin_dev.h"
 #include "green_inf_dev.h"
 #include "hbg_internal.h"
 
@@ -43,6 +44,51 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
   A[(size_t)n * lda + k] = make_double2((k == n ? 2.0 * PI : 0.0) - d.x, -d.y);
 }
 
+// The matrices in finite depth: the same entries with the kernel of green_fin_dev.h; the (k, n)
+// entry takes the source-depth derivative WrZ of the pair kernel evaluated at (x_k, c_n).
+__global__ void __launch_bounds__(256) assemble_fin_kernel(const PanelD* __restrict__ P, int N,
+                                                           const FinFreq* __restrict__ f,
+                                                           FinTables F, WaveTables T, double logc,
+                                                           const StaticD* __restrict__ st, double2* __restrict__ S,
+                                                           double2* __restrict__ A, int lda) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int n = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  if (k >= N || n >= N) return;
+  const PanelD* pn = P + n;
+  const double xk = P[k].c[0], yk = P[k].c[1], zk = P[k].c[2];
+  const double dx = xk - pn->c[0], dy = yk - pn->c[1];
+  const double R = sqrt(dx * dx + dy * dy);
+  const FinK w = fin_kernel(f, F, T, k, n, R, zk, pn->c[2]);
+  const double nx = pn->n[0], ny = pn->n[1], nz = pn->n[2];
+  const double a = R > 0.0 ? (dx * nx + dy * ny) / R : 0.0;
+  const StaticD e = st[(size_t)n * N + k];
+  const double Ar = pn->area, K = f->K;
+  const double c2 = 2.0 * K * logc;
+  const double2 s = make_double2(e.rs + Ar * w.Wr - c2 * e.dl, Ar * w.Wi);
+  const double2 d = make_double2(e.rd + Ar * (-w.WrR * a + w.WrZ * nz) - c2 * e.dd, Ar * (-w.WiR * a + w.WiZ * nz));
+  S[(size_t)n * lda + k] = s;
+  A[(size_t)n * lda + k] = make_double2((k == n ? 2.0 * PI : 0.0) - d.x, -d.y);
+}
+
+// The finite-depth kernel at n point pairs for the tests: the point depths z[i] at "panel" i and
+// zeta[i] at "panel" n + i of the per-point factors; out [n][8] = Wr, WrR, WrZ, Wrz, Wi, WiR, WiZ, Wiz
+__global__ void fin_points_eval_kernel(const FinFreq* __restrict__ f, FinTables F, WaveTables T, const double* __restrict__ R,
+                                       const double* __restrict__ z, const double* __restrict__ zeta, int n,
+                                       double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const FinK w = fin_kernel(f, F, T, i, n + i, R[i], z[i], zeta[i]);
+  double* o = out + 8 * (size_t)i;
+  o[0] = w.Wr;
+  o[1] = w.WrR;
+  o[2] = w.WrZ;
+  o[3] = w.Wrz;
+  o[4] = w.Wi;
+  o[5] = w.WiR;
+  o[6] = w.WiZ;
+  o[7] = w.Wiz;
+}
+
 // The wave kernel at points (X, Y) for the tests: out[7 i ..] = Ft, Ft_X, Ft_Y, Gi, Gi_X, Gi_Y, e^-Y J0
 __global__ void wave_points_kernel(WaveTables T, const double* __restrict__ X, const double* __restrict__ Y, long n,
                                    double* __restrict__ out) {
@@ -67,7 +113,7 @@ __global__ void wave_points_kernel(WaveTables T, const double* __restrict__ X, c
 // phiI, dphiI: [H][N]; u: [R][N], R = nrad + H, nrad = 6 nbodies.
 __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restrict__ body,
                                 const double* __restrict__ nj, int N, int nrad, int H, double w, double k, double grav,
-                                const double* __restrict__ heads, double2* __restrict__ phiI,
+                                double depth, const double* __restrict__ heads, double2* __restrict__ phiI,
                                 double2* __restrict__ dphiI, double2* __restrict__ u) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
@@ -75,7 +121,13 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
   const double x = p->c[0], y = p->c[1], z = p->c[2];
   for (int c = 0; c < nrad; ++c)
     u[(size_t)c * N + j] = make_double2(body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0, 0.0);
-  const double Z = exp(k * z), Zz = k * Z;
+  // depth factor (low_order.rs:1401-1413): Z = cosh k(z + h) / cosh kh, e^{k z} in infinite depth
+  double Z = exp(k * z), Zz = k * Z;
+  if (depth > 0.0) {
+    const double b = exp(-k * (z + 2.0 * depth)), dd = 1.0 + exp(-2.0 * k * depth);
+    Zz = k * (Z - b) / dd;
+    Z = (Z + b) / dd;
+  }
   for (int h = 0; h < H; ++h) {
     double sb, cb;
     sincos(heads[h], &sb, &cb);
@@ -230,15 +282,29 @@ void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T
   HBG_HIP(hipGetLastError());
 }
 
+void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
+                         const StaticD* st, double2* S, double2* A, int lda, hipStream_t stream) {
+  dim3 block(64, 4), grid((N + 63) / 64, (N + 3) / 4);
+  assemble_fin_kernel<<<grid, block, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda);
+  HBG_HIP(hipGetLastError());
+}
+
+void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
+                            const double* zeta, int n, double* out, hipStream_t stream) {
+  fin_points_eval_kernel<<<(n + 127) / 128, 128, 0, stream>>>(f, F, T, R, z, zeta, n, out);
+  HBG_HIP(hipGetLastError());
+}
+
 void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream) {
   wave_points_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(T, X, Y, n, out);
   HBG_HIP(hipGetLastError());
 }
 
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
-                     double grav, const double* heads, double2* phiI, double2* dphiI, double2* u,
+                     double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
                      hipStream_t stream) {
-  boundary_kernel<<<(N + 255) / 256, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, k, grav, heads, phiI, dphiI, u);
+  boundary_kernel<<<(N + 255) / 256, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, k, grav, depth, heads, phiI,
+                                                       dphiI, u);
   HBG_HIP(hipGetLastError());
 }
 

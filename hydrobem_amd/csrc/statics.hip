@@ -138,9 +138,9 @@ __device__ __forceinline__ bool log_pair_d(const PanelD* __restrict__ p, P3 x, d
 
 // gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]
 __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, int N, int logint, double tol,
-                                                      const double* __restrict__ gl, StaticD* __restrict__ st,
-                                                      int2* __restrict__ queue, unsigned* __restrict__ qcount,
-                                                      unsigned qcap) {
+                                                      double depth, const double* __restrict__ gl,
+                                                      StaticD* __restrict__ st, int2* __restrict__ queue,
+                                                      unsigned* __restrict__ qcount, unsigned qcap) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
   if (j >= N || k >= N) return;
@@ -153,6 +153,18 @@ __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__
   StaticD e;
   e.rs = s0 + sb;
   e.rd = d0 + db;
+  if (depth > 0.0) {
+    // finite depth: the bottom images of lo_static.rs:29-38 (x, y, -z - 2h), (x, y, z + 2h),
+    // (x, y, z - 2h), (x, y, -z - 4h), summed in the reference's order
+    const double h = depth;
+    double s2, d2, s4, d4, s5, d5, s6, d6;
+    rankine_d(p, mk(x.x, x.y, -x.z - 2.0 * h), gl, s2, d2);
+    rankine_d(p, mk(x.x, x.y, x.z + 2.0 * h), gl, s4, d4);
+    rankine_d(p, mk(x.x, x.y, x.z - 2.0 * h), gl, s5, d5);
+    rankine_d(p, mk(x.x, x.y, -x.z - 4.0 * h), gl, s6, d6);
+    e.rs += (s2 + s4) + s5 + s6;
+    e.rd += (d2 + d4) + d5 + d6;
+  }
   e.dl = 0.0;
   e.dd = 0.0;
   if (logint && log_pair_d(p, x, tol)) {
@@ -292,11 +304,11 @@ __global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__
   }
 }
 
-void launch_statics(const PanelD* P, int N, int logint, double tol, const double* gl, StaticD* st, int2* queue,
-                    unsigned* qcount, unsigned qcap, hipStream_t stream) {
+void launch_statics(const PanelD* P, int N, int logint, double tol, double depth, const double* gl, StaticD* st,
+                    int2* queue, unsigned* qcount, unsigned qcap, hipStream_t stream) {
   dim3 block(64, 4);
   dim3 grid((N + 63) / 64, (N + 3) / 4);
-  statics_kernel<<<grid, block, 0, stream>>>(P, N, logint, tol, gl, st, queue, qcount, qcap);
+  statics_kernel<<<grid, block, 0, stream>>>(P, N, logint, tol, depth, gl, st, queue, qcount, qcap);
   HBG_HIP(hipGetLastError());
 }
 

@@ -75,7 +75,8 @@ const char* hbg_version(void);
 HbgStatus hbg_session_create(int32_t device, HbgSession** out);
 HbgStatus hbg_session_free(HbgSession* s);
 
-/* Environment (problem.rs env; ρ, g, depth); depth must be +inf (infinite depth). */
+/* Environment (problem.rs env; ρ, g, depth): depth +inf (infinite depth) or finite (the
+ * finite-depth Green function of green_fin.rs; every panel centroid between -depth and 0). */
 HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, double depth);
 
 /* The wetted panels: vertices [n_panels][4][3] as the mesh gives them (a triangle repeats a
@@ -106,6 +107,17 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
 /* The infinite-depth wave kernel at n points (green_inf.rs:316-368 wave_kernel):
  * out [n][7] = Ft, Ft_X, Ft_Y, Gi, Gi_X, Gi_Y, exp(-Y) J0(X), for X = K R >= 0, Y = K|z+zeta| >= 0. */
 HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64_t n, double* out);
+
+/* The finite-depth kernel at n point pairs (green_fin.rs:1007-1095 finite_kernel, ordinary
+ * frequency K = w^2/g, depth h): horizontal distance R[i], field-point depth z[i], source depth
+ * zeta[i]; out [n][8] = Wr, dWr/dR, dWr/dzeta, dWr/dz, Wi, dWi/dR, dWi/dzeta, dWi/dz of the regular
+ * part Wr = Re G - sum_p 1/r_p, Wi = Im G. */
+HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
+                            const double* zeta, double* out);
+
+/* Dispersion roots (dispersion.rs:19-95): out[0] = x0 (x0 tanh x0 = H), out[i] = x_i
+ * (x_i tan x_i = -H, (i - 1/2) pi < x_i < i pi) for i = 1..n. */
+HbgStatus hbg_dispersion_roots(double H, int32_t n, double* out);
 
 /* The statics of the current panels (lo_static.rs:59-119): out [N][N][4] = rs, rd, dl, dd for
  * field point k (row) and panel j (column), logarithmic / log_tolerance as in HbgOptions. */

@@ -365,13 +365,25 @@ bool log_pair(const PanelGeom& p, V3 x, double tol) {
   return d * d * tol < 0.6 * p.radius * p.radius;
 }
 
-// lo_static.rs:59-119, infinite depth, wetted panels (no surface panels in z = 0)
-void static_entry(const PanelGeom& p, V3 x, bool logint, double tol, double* e) {
+// lo_static.rs:59-119, wetted panels (no surface panels in z = 0); finite depth h adds the bottom
+// images of image_points (:29-38): (x, y, -z - 2h) and its image (x, y, z + 2h), (x, y, z - 2h),
+// (x, y, -z - 4h)
+void static_entry(const PanelGeom& p, V3 x, bool logint, double tol, double depth, double* e) {
   double s0, d0, sb, db;
   rankine(p, x, s0, d0);
   rankine(p, v3(x[0], x[1], -x[2]), sb, db);
   e[0] = s0 + sb;
   e[1] = d0 + db;
+  if (std::isfinite(depth)) {
+    const double h = depth;
+    double s2, d2, s4, d4, s5, d5, s6, d6;
+    rankine(p, v3(x[0], x[1], -x[2] - 2.0 * h), s2, d2);
+    rankine(p, v3(x[0], x[1], x[2] + 2.0 * h), s4, d4);
+    rankine(p, v3(x[0], x[1], x[2] - 2.0 * h), s5, d5);
+    rankine(p, v3(x[0], x[1], -x[2] - 4.0 * h), s6, d6);
+    e[0] += (s2 + s4) + s5 + s6;
+    e[1] += (d2 + d4) + d5 + d6;
+  }
   e[2] = 0.0;
   e[3] = 0.0;
   if (logint && log_pair(p, x, tol)) {
@@ -450,8 +462,9 @@ void yt_and_derivative(double x, double& yt, double& dyt) {
   dyt = -y1v - 2.0 / PI * (j0v / x - L * j1v);
 }
 
-// Ein(y) = int_0^y (e^t - 1)/t dt = sum y^k / (k k!)
+// Ein(y) = int_0^y (e^t - 1)/t dt = sum y^k / (k k!) (the series below 40; Ei(y) - log y - gamma above)
 double ein(double y) {
+  if (y > 40.0) return std::expint(y) - std::log(y) - GAMMA;
   double term = 1.0, s = 0.0;
   for (int k = 1; k < 400; ++k) {
     term *= y / k;
@@ -498,10 +511,61 @@ void n_and_nx(double X, double Y, double& N, double& NX) {
 }  // namespace
 
 // ------------------------------------------------------------------ wave kernel (green_inf.rs:316-368)
+// Far from the free surface (Y > 300, where e^Y of the identity form would overflow) Havelock's form (3b) with P = int_0^Y e^{t-Y} / sqrt(X^2 + t^2) dt
+// written as int_0^Y e^{-s} / sqrt(X^2 + (Y - s)^2) ds (no overflow of e^t for large Y):
+// Ft = -pi e^-Y (H0 + Y0) - 2 P, Ft_X = -pi e^-Y (2/pi - H1 - Y1) - 2 P_X; on the axis
+// Ft(0, Y) = -2 e^-Y Ei(Y) by its asymptotic series.
+void wave_far(double X, double Y, double& F, double& FX) {
+  const GaussCache& G = gauss();
+  const double top = std::min(Y, 60.0);
+  double P = 0.0, PX = 0.0;
+  const int panels = (int)std::ceil(top / 2.0);
+  for (int q = 0; q < panels; ++q) {
+    const double a = top * q / panels, b = top * (q + 1) / panels, h = 0.5 * (b - a);
+    for (int i = 0; i < 16; ++i) {
+      const double sv = a + h * (G.x[16][i] + 1.0), t = Y - sv;
+      const double r2 = X * X + t * t, ir = 1.0 / std::sqrt(r2), e = std::exp(-sv) * h * G.w[16][i];
+      P += e * ir;
+      PX -= e * X * ir * ir * ir;
+    }
+  }
+  if (X == 0.0) {
+    // e^-Y Ei(Y) = sum_k k! / Y^(k+1) (asymptotic, Y > 40)
+    double term = 1.0 / Y, s = 0.0;
+    for (int k = 0; k < 40; ++k) {
+      s += term;
+      const double next = term * (k + 1) / Y;
+      if (next > term || next < 1e-18 * s) break;
+      term = next;
+    }
+    F = -2.0 * s;
+    FX = 0.0;
+    return;
+  }
+  double h0, h1;
+  struve01(X, h0, h1);
+  const double eY = std::exp(-Y);
+  F = -PI * eY * (h0 + ::y0(X)) - 2.0 * P;
+  FX = -PI * eY * (2.0 / PI - h1 - ::y1(X)) - 2.0 * PX;
+}
+
 extern "C" int orc_wave_kernel(double X, double Y, double* out) {
   if (!(X >= 0.0) || !(Y >= 0.0) || (X == 0.0 && Y == 0.0)) return ORC_ERR_ARGUMENT;
   double R0 = std::sqrt(X * X + Y * Y);
   double eY = std::exp(-Y);
+  if (Y > 300.0) {
+    double F, FX;
+    wave_far(X, Y, F, FX);
+    const double j0v = ::j0(X), j1v = ::j1(X);
+    out[0] = F;
+    out[1] = FX;
+    out[2] = -F - 2.0 / R0;
+    out[3] = -2.0 * PI * eY * j0v;
+    out[4] = 2.0 * PI * eY * j1v;
+    out[5] = -out[3];
+    out[6] = eY * j0v;
+    return ORC_OK;
+  }
   double j0v = ::j0(X), j1v = ::j1(X);
   double yt, dyt;
   yt_and_derivative(X, yt, dyt);
@@ -606,6 +670,256 @@ extern "C" int orc_log_singularity(const double* v12, const double* image, doubl
   return ORC_OK;
 }
 
+// ------------------------------------------------------------------ finite depth
+// dispersion.rs:19-95: x0 tanh x0 = H (propagating) and xn tan xn = -H (evanescent), safeguarded
+// Newton on the brackets of the reference.
+extern "C" double orc_dispersion_x0(double H) {
+  if (!(H > 0.0)) return 0.0;
+  if (H >= 40.0) return H;
+  double lo = std::max(H, std::sqrt(H)), hi = 0.5 * (H + std::sqrt(H * H + 4.0 * H));
+  double x = H < 1.0 ? std::sqrt(H) * (1.0 + H / 6.0) : H;
+  x = std::min(std::max(x, lo), hi);
+  double prev = -1.0;
+  for (int it = 0; it < 200; ++it) {
+    double t = std::tanh(x), f = x * t - H;
+    if (f < 0.0) lo = x; else hi = x;
+    double fp = t + x * (1.0 - t * t);
+    double xn = x - f / fp;
+    if (!(xn >= lo && xn <= hi)) xn = 0.5 * (lo + hi);
+    if (xn == x || xn == prev) break;
+    prev = x;
+    x = xn;
+  }
+  return x;
+}
+
+extern "C" double orc_dispersion_xn(double H, int n) {
+  double npi = n * PI;
+  if (!(H > 0.0)) return npi;
+  if (std::isinf(H)) return npi - 0.5 * PI;
+  double lo = 0.0, hi = 0.5 * PI, d = std::atan(H / (npi - 0.25 * PI)), prev = -1.0;
+  for (int it = 0; it < 200; ++it) {
+    double sd = std::sin(d), cd_ = std::cos(d), g = (npi - d) * sd - H * cd_;
+    if (g < 0.0) lo = d; else hi = d;
+    double gp = (npi - d) * cd_ + (H - 1.0) * sd;
+    double dn = d - g / gp;
+    if (!(dn > lo && dn < hi)) dn = 0.5 * (lo + hi);
+    if (dn == d || dn == prev) break;
+    prev = d;
+    d = dn;
+  }
+  return npi - d;
+}
+
+namespace {
+
+const int FIN_ROOTS = 24;         // green_fin.rs:113
+const double SERIES_STOP = 18.0;  // green_fin.rs:349
+
+// complex J0, J1 by their power series (|z| <= ~15 here)
+void cj01(cd z, cd& j0, cd& j1) {
+  cd q = -0.25 * z * z, t0 = 1.0, t1 = 1.0, s0 = 1.0, s1 = 1.0;
+  for (int m = 1; m < 80; ++m) {
+    t0 *= q / ((double)m * m);
+    t1 *= q / ((double)m * (m + 1));
+    s0 += t0;
+    s1 += t1;
+    if (std::abs(t0) < 1e-18 * std::abs(s0) && std::abs(t1) < 1e-18 * std::abs(s1) && m > 3) break;
+  }
+  j0 = s0;
+  j1 = 0.5 * z * s1;
+}
+
+// The smooth remainder of Newman's decomposition (green_fin.rs:1-20, 466-497) without tables:
+// Q(xb, vb) = PV int_0^inf (k+H)^2 cosh(k vb) e^{-3k} J0(k xb) / [(k sinh k - H cosh k)(k - H)] dk
+// (k = kappa h dimensionless) with dQ/dxb and dQ/dvb; the principal value as the real part of
+// the integral along 0 -> i d -> T + i d -> T -> inf, above the poles k = x0 and k = H and
+// below the first imaginary pole i x1 (x1 > pi/2).
+void q_direct(double H, double x0, double xb, double vb, double* q) {
+  const GaussCache& G = gauss();
+  cd s0(0, 0), s1(0, 0), s2(0, 0);
+  auto f = [&](cd k, cd dk) {
+    cd num = (k + H) * (k + H) * std::exp(-3.0 * k);
+    cd den = (k * std::sinh(k) - H * std::cosh(k)) * (k - H);
+    cd ch = std::cosh(k * vb), sh = std::sinh(k * vb);
+    cd j0, j1;
+    cj01(k * xb, j0, j1);
+    cd w = num / den * dk;
+    s0 += w * ch * j0;
+    s1 += -w * ch * k * j1;
+    s2 += w * k * sh * j0;
+  };
+  auto seg = [&](cd a, cd b, int panels) {
+    for (int p = 0; p < panels; ++p) {
+      cd pa = a + (b - a) * ((double)p / panels), pb = a + (b - a) * ((double)(p + 1) / panels);
+      cd h2 = 0.5 * (pb - pa);
+      for (int i = 0; i < 16; ++i) f(pa + h2 * (G.x[16][i] + 1.0), h2 * G.w[16][i]);
+    }
+  };
+  if (std::min(x0, H) > 28.0) {
+    seg(cd(0, 0), cd(28.0, 0), 56);
+  } else {
+    const double d = 0.7, T = std::max(x0, H) + 3.0;
+    // the integrand behaves like 1/k for sqrt(H) << |k| << 1 (the log H of Q as H -> 0):
+    // geometric panels up the imaginary axis from far below sqrt(H)
+    double y = std::min(0.05, 1e-3 * std::sqrt(H));
+    seg(cd(0, 0), cd(0, y), 1);
+    while (2.0 * y < d) {
+      seg(cd(0, y), cd(0, 2.0 * y), 1);
+      y *= 2.0;
+    }
+    seg(cd(0, y), cd(0, d), 1);
+    seg(cd(0, d), cd(T, d), (int)std::ceil(T) + 1);
+    seg(cd(T, d), cd(T, 0), 1);
+    // the tail decays at least like e^{-2k}
+    seg(cd(T, 0), cd(T + 24.0, 0), 12);
+  }
+  q[0] = s0.real();
+  q[1] = s1.real();
+  q[2] = s2.real();
+}
+
+// Phi1 (which 1) and Phi2 (which 2) with their xb and vb derivatives (green_fin.rs:466-497):
+// Phi1 = H [Ft(X, 2H - V) + Ft(X, 2H + V)] + Q(xb, vb), Phi2 = H Ft(X, 2H + V) + Q(xb, vb)
+void phi_direct(double H, double x0, int which, double xb, double vb, double* out) {
+  double q[3];
+  q_direct(H, x0, xb, vb, q);
+  out[0] = q[0];
+  out[1] = q[1];
+  out[2] = q[2];
+  const double X = H * xb, H2 = H * H;
+  double w[7];
+  if (which == 1) {
+    orc_wave_kernel(X, H * (2.0 - vb), w);
+    out[0] += H * w[0];
+    out[1] += H2 * w[1];
+    out[2] -= H2 * w[2];
+  }
+  orc_wave_kernel(X, H * (2.0 + vb), w);
+  out[0] += H * w[0];
+  out[1] += H2 * w[1];
+  out[2] += H2 * w[2];
+}
+
+struct FinDepth {
+  double h = 0, K = 0, H = 0, x0 = 0, k0 = 0, A0 = 0;
+  double kn[FIN_ROOTS], cn[FIN_ROOTS];
+};
+
+// make_finite_depth (green_fin.rs:874-934), ordinary frequencies
+FinDepth make_fd(double h, double K) {
+  FinDepth fd;
+  fd.h = h;
+  fd.K = K;
+  fd.H = K * h;
+  fd.x0 = orc_dispersion_x0(fd.H);
+  fd.k0 = fd.x0 / h;
+  fd.A0 = 2.0 * PI * fd.k0 / (-std::expm1(-4.0 * fd.x0) + 4.0 * fd.x0 * std::exp(-2.0 * fd.x0));
+  for (int n = 1; n <= FIN_ROOTS; ++n) {
+    double k = orc_dispersion_xn(fd.H, n) / h;
+    fd.kn[n - 1] = k;
+    fd.cn[n - 1] = 4.0 * (k * k + K * K) / (k * k * h + K * K * h - K);
+  }
+  return fd;
+}
+
+// K0, K1 by the integral int_0^inf e^{-x cosh t} (1, cosh t) dt, trapezoidal rule (exponentially
+// convergent for this integrand)
+void bessel_k01(double x, double& k0v, double& k1v) {
+  const double step = 0.05;
+  double s0 = 0.5 * std::exp(-x), s1 = s0;
+  for (int i = 1; i < 4000; ++i) {
+    double t = i * step, c = std::cosh(t), e = std::exp(-x * c);
+    s0 += e;
+    s1 += e * c;
+    if (e < 1e-300 || e * c < 1e-19 * s1) break;
+  }
+  k0v = s0 * step;
+  k1v = s1 * step;
+}
+
+// finite_kernel (green_fin.rs:1007-1095) for ordinary frequencies: the regular part
+// Wr = Re G - sum_p 1/r_p and Wi = Im G with d/dR, d/dzeta (source), d/dz (field point)
+void finite_kernel(const FinDepth& fd, double R, double z, double zeta, double* o) {
+  const double h = fd.h, K = fd.K, k0 = fd.k0;
+  const double d1 = z - zeta;
+  const double sg = d1 > 0.0 ? 1.0 : d1 < 0.0 ? -1.0 : 0.0;
+  const double v[6] = {std::fabs(d1), z + zeta + 2.0 * h, -(z + zeta), d1 + 2.0 * h, -d1 + 2.0 * h, z + zeta + 4.0 * h};
+  double e3 = std::exp(-k0 * v[2]), e4 = std::exp(-k0 * v[3]), e5 = std::exp(-k0 * v[4]), e6 = std::exp(-k0 * v[5]);
+  double E = e3 + e4 + e5 + e6, Ez = k0 * (e3 + e4 - e5 - e6), Ef = k0 * (e3 - e4 + e5 - e6);
+  const double x = k0 * R;
+  double j0v = ::j0(x), j1v = ::j1(x);
+  double Wr, WrR, WrZ, Wrz;
+  if (R >= 0.5 * h) {
+    double y0v = ::y0(x), y1v = ::y1(x);
+    double re = -fd.A0 * E * y0v, reR = fd.A0 * E * k0 * y1v, reZ = -fd.A0 * Ez * y0v, rez = -fd.A0 * Ef * y0v;
+    for (int n = 0; n < FIN_ROOTS; ++n) {
+      double k = fd.kn[n], xn = k * R;
+      if (xn >= SERIES_STOP) break;
+      double K0, K1;
+      bessel_k01(xn, K0, K1);
+      double cz = std::cos(k * (z + h)), sz = std::sin(k * (z + h));
+      double cs = std::cos(k * (zeta + h)), ss = std::sin(k * (zeta + h));
+      double c = fd.cn[n];
+      re += c * cz * cs * K0;
+      reR -= c * cz * cs * k * K1;
+      reZ -= c * cz * k * ss * K0;
+      rez -= c * k * sz * cs * K0;
+    }
+    const double dvz[6] = {-sg, 1.0, -1.0, -1.0, 1.0, 1.0}, dvf[6] = {sg, 1.0, -1.0, 1.0, -1.0, 1.0};
+    for (int p = 0; p < 6; ++p) {
+      double r2 = R * R + v[p] * v[p], r = std::sqrt(r2), ir3 = 1.0 / (r2 * r);
+      re -= 1.0 / r;
+      reR += R * ir3;
+      reZ += v[p] * dvz[p] * ir3;
+      rez += v[p] * dvf[p] * ir3;
+    }
+    Wr = re;
+    WrR = reR;
+    WrZ = reZ;
+    Wrz = rez;
+  } else {
+    const double ih = 1.0 / h, ih2 = ih * ih;
+    double f1[3], f2[3];
+    phi_direct(fd.H, fd.x0, 1, R * ih, v[0] * ih, f1);
+    phi_direct(fd.H, fd.x0, 2, R * ih, v[1] * ih, f2);
+    Wr = ih * (f1[0] + f2[0]);
+    WrR = ih2 * (f1[1] + f2[1]);
+    double A1z = -sg * ih2 * f1[2];
+    double Bz = ih2 * f2[2];
+    double w[7];
+    orc_wave_kernel(K * R, K * v[2], w);
+    Wr += K * w[0];
+    WrR += K * K * w[1];
+    Bz -= K * K * w[2];
+    WrZ = A1z + Bz;
+    Wrz = -A1z + Bz;
+  }
+  o[0] = Wr;
+  o[1] = WrR;
+  o[2] = WrZ;
+  o[3] = Wrz;
+  o[4] = -fd.A0 * E * j0v;
+  o[5] = fd.A0 * E * k0 * j1v;
+  o[6] = -fd.A0 * Ez * j0v;
+  o[7] = -fd.A0 * Ef * j0v;
+}
+
+}  // namespace
+
+extern "C" int orc_finite_kernel(double h, double K, double R, double z, double zeta, double* out8) {
+  if (!(h > 0.0) || !(K > 0.0) || !(R >= 0.0) || z > 0.0 || zeta > 0.0 || z < -h || zeta < -h)
+    return ORC_ERR_ARGUMENT;
+  FinDepth fd = make_fd(h, K);
+  finite_kernel(fd, R, z, zeta, out8);
+  return ORC_OK;
+}
+
+extern "C" int orc_finite_phi(double H, int which, double xb, double vb, double* out3) {
+  phi_direct(H, orc_dispersion_x0(H), which, xb, vb, out3);
+  return ORC_OK;
+}
+
 // ------------------------------------------------------------------ the problem
 struct OrcProblem {
   int N = 0, nb = 0;
@@ -665,7 +979,8 @@ static void ensure_statics(OrcProblem* p) {
 #pragma omp parallel for schedule(dynamic, 1)
   for (int k = 0; k < N; ++k) {
     V3 x = p->pg[k].c;
-    for (int j = 0; j < N; ++j) static_entry(p->pg[j], x, p->logint, p->logtol, &p->st[((size_t)k * N + j) * 4]);
+    for (int j = 0; j < N; ++j)
+      static_entry(p->pg[j], x, p->logint, p->logtol, p->depth, &p->st[((size_t)k * N + j) * 4]);
   }
   p->have_statics = true;
 }
@@ -720,12 +1035,17 @@ void apply_kernel(const OrcProblem* p, int row_k, int j, V3 x, double K, double 
   D = cd(e[1] + A * (-k.WrR * a + rz * pj.n[2]) - 2.0 * K * logc * e[3], A * (-k.WiR * a + iz * pj.n[2]));
 }
 
-// lo_frequency.rs:399-537 for G = 1: S(k, n), A(k, n) = 2 pi delta_kn - D(k, n), column-major
+// lo_frequency.rs:399-537 for G = 1: S(k, n), A(k, n) = 2 pi delta_kn - D(k, n), column-major;
+// finite depth: the kernel of finite_kernel (green_fin.rs:1007-1095), the (n, k) entry with the
+// derivative in its first point (Wrz, lo_frequency.rs:494-497)
 void assemble(OrcProblem* p, double omega, std::vector<cd>& S, std::vector<cd>& Am) {
   ensure_statics(p);
   int N = p->N;
   double K = omega * omega / p->grav;
   double logc = p->logint ? 1.0 : 0.0;
+  const bool finite = std::isfinite(p->depth);
+  FinDepth fd;
+  if (finite) fd = make_fd(p->depth, K);
   S.assign((size_t)N * N, cd(0, 0));
   Am.assign((size_t)N * N, cd(0, 0));
 #pragma omp parallel for schedule(dynamic, 4)
@@ -734,7 +1054,23 @@ void assemble(OrcProblem* p, double omega, std::vector<cd>& S, std::vector<cd>& 
     for (int n = k; n < N; ++n) {
       V3 xn = p->pg[n].c;
       double R;
-      Kern w = pair_kernel(K, xk, xn, R);
+      Kern w;
+      if (finite) {
+        double dx = xk[0] - xn[0], dy = xk[1] - xn[1];
+        R = std::sqrt(dx * dx + dy * dy);
+        double o[8];
+        finite_kernel(fd, R, xk[2], xn[2], o);
+        w.Wr = o[0];
+        w.WrR = o[1];
+        w.WrZ = o[2];
+        w.Wrz = o[3];
+        w.Wi = o[4];
+        w.WiR = o[5];
+        w.WiZ = o[6];
+        w.Wiz = o[7];
+      } else {
+        w = pair_kernel(K, xk, xn, R);
+      }
       cd S1, D1, S2, D2;
       apply_kernel(p, k, n, xk, K, logc, w, R, false, S1, D1);
       S[k + (size_t)n * N] = S1;
@@ -805,7 +1141,6 @@ int lu_solve(int n, cd* a, int nrhs, cd* b) {
 extern "C" int orc_assemble(void* h, double omega, double* S_out, double* A_out) {
   OrcProblem* p = (OrcProblem*)h;
   if (!(omega > 0.0)) return ORC_ERR_ARGUMENT;
-  if (std::isfinite(p->depth)) return ORC_ERR_UNSUPPORTED;
   std::vector<cd> S, A;
   assemble(p, omega, S, A);
   std::memcpy(S_out, S.data(), S.size() * sizeof(cd));
@@ -823,7 +1158,6 @@ extern "C" int orc_lu_solve(int n, double* a, int nrhs, double* b) {
 extern "C" int orc_solve(void* h, const double* omegas, int nf, const double* headings_deg, int nh, double* Aout,
                          double* Bout, double* Xout, double* FKout, double* SCout, double* HKout) {
   OrcProblem* p = (OrcProblem*)h;
-  if (std::isfinite(p->depth)) return ORC_ERR_UNSUPPORTED;
   int N = p->N;
   int D = 6 * p->nb, nrad = D, nrhs = nrad + nh;
   std::vector<cd> S, Am;
@@ -831,6 +1165,8 @@ extern "C" int orc_solve(void* h, const double* omegas, int nf, const double* he
     double w = omegas[fi];
     if (!(w > 0.0) || !std::isfinite(w)) return ORC_ERR_ARGUMENT;
     double K = w * w / p->grav, k = K;
+    const bool finite = std::isfinite(p->depth);
+    if (finite) k = orc_dispersion_x0(K * p->depth) / p->depth;
     assemble(p, w, S, Am);
     // boundary values (lo_frequency.rs:214-273; low_order.rs:1458-1468 incident_one, frame at the origin)
     std::vector<cd> phiI((size_t)nh * N), dphiI((size_t)nh * N), u((size_t)nrhs * N, cd(0, 0));
@@ -840,7 +1176,13 @@ extern "C" int orc_solve(void* h, const double* omegas, int nf, const double* he
       for (int j = 0; j < N; ++j) {
         V3 x = p->pg[j].c;
         double phase = -k * (x[0] * cb + x[1] * sb);
+        // depth factor (low_order.rs:1401-1413): cosh k(z + h) / cosh kh and its z derivative
         double Z = std::exp(k * x[2]), Zz = k * Z;
+        if (finite) {
+          const double hh = p->depth, b = std::exp(-k * (x[2] + 2.0 * hh)), d = 1.0 + std::exp(-2.0 * k * hh);
+          Zz = k * (Z - b) / d;
+          Z = (Z + b) / d;
+        }
         cd e = cd(0.0, p->grav / w) * cd(std::cos(phase), std::sin(phase));
         cd phi = e * Z;
         cd g0 = phi * cd(0.0, -k * cb), g1 = phi * cd(0.0, -k * sb), g2 = e * Zz;

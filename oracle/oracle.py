@@ -49,11 +49,17 @@ def lib():
             ("orc_statics", [ctypes.c_void_p, _dp]),
             ("orc_assemble", [ctypes.c_void_p, ctypes.c_double, _dp, _dp]),
             ("orc_lu_solve", [ctypes.c_int, _dp, ctypes.c_int, _dp]),
+            ("orc_finite_kernel", [ctypes.c_double] * 5 + [_dp]),
+            ("orc_finite_phi", [ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]),
             ("orc_solve", [ctypes.c_void_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
+        L.orc_dispersion_x0.argtypes = [ctypes.c_double]
+        L.orc_dispersion_x0.restype = ctypes.c_double
+        L.orc_dispersion_xn.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_dispersion_xn.restype = ctypes.c_double
         L.orc_problem_create.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, _ip]
         L.orc_problem_create.restype = ctypes.c_void_p
@@ -94,6 +100,29 @@ def yt(X: float) -> np.ndarray:
 def struve01(x: float) -> np.ndarray:
     out = np.zeros(2)
     _check(lib().orc_struve01(float(x), _p(out)), "struve01")
+    return out
+
+
+def dispersion_x0(H: float) -> float:
+    """x0 tanh x0 = H (dispersion.rs:19-52)."""
+    return lib().orc_dispersion_x0(float(H))
+
+
+def dispersion_xn(H: float, n: int) -> float:
+    """xn tan xn = -H, (n - 1/2) pi < xn < n pi (dispersion.rs:62-95)."""
+    return lib().orc_dispersion_xn(float(H), int(n))
+
+
+def finite_kernel(h, K, R, z, zeta) -> np.ndarray:
+    """(Wr, WrR, WrZ, Wrz, Wi, WiR, WiZ, Wiz) of the finite-depth regular part (green_fin.rs:1007)."""
+    out = np.zeros(8)
+    _check(lib().orc_finite_kernel(float(h), float(K), float(R), float(z), float(zeta), _p(out)), "finite_kernel")
+    return out
+
+
+def finite_phi(H, which, xb, vb) -> np.ndarray:
+    out = np.zeros(3)
+    _check(lib().orc_finite_phi(float(H), int(which), float(xb), float(vb), _p(out)), "finite_phi")
     return out
 
 

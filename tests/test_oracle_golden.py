@@ -181,3 +181,62 @@ def test_hulme_hemisphere_converges(oracle):
         errs.append(np.abs(vals - ref[:, 1:]).max(axis=0) / np.abs(ref[:, 1:]).max(axis=0))
     assert np.all(errs[1] < 0.4 * errs[0]), errs
     assert np.all(errs[1] < 1e-2), errs
+
+
+def test_dispersion_roots_against_40_digit_vectors(oracle):
+    # tests/data/dispersion_reference.txt: 2136 roots H n x (40 digits); core.rs:390-407 allows 2 ulp
+    d = np.loadtxt(GOLDEN / "dispersion_reference.txt")
+    worst = 0.0
+    for H, n, x in d:
+        v = oracle.dispersion_x0(H) if n == 0 else oracle.dispersion_xn(H, int(n))
+        worst = max(worst, abs(v - x) / np.spacing(x))
+    assert worst <= 2.0, worst
+
+
+def finite_reference_rows():
+    rows = []
+    for line in (GOLDEN / "green_fin_reference.txt").read_text().splitlines():
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.split()
+        if f[0] == "0":  # ordinary frequencies (modes 1, 2: the limits omega = 0 and inf)
+            rows.append([float(x) for x in f[1:]])
+    return np.array(rows)
+
+
+def test_finite_kernel_against_mpmath_vectors(oracle):
+    # tests/data/green_fin_reference.txt: h K R z zeta, Wr dWr/dR dWr/dzeta dWr/dz Wi ..., scale,
+    # scaleD (30 digits); core.rs:155-195 allows 2e-6 of max(|v|, scale). The near form (R < h/2)
+    # is evaluated from its integral here (1e-10 up to H = 1e5); the series form keeps the reference's stop at
+    # k_n R >= 18 (green_fin.rs:349), whose truncation is about 1e-7.
+    d = finite_reference_rows()
+    worst_near = worst_far = worst_im = 0.0
+    for h, K, R, z, ze, *r in d:
+        ref, sc, scD = np.array(r[:8]), r[8], r[9]
+        g = oracle.finite_kernel(h, K, R, z, ze)
+        e = [abs(g[i] - ref[i]) / max(abs(ref[i]), scD if i % 4 else sc) for i in range(8)]
+        worst_im = max(worst_im, max(e[4:]))
+        if R < 0.5 * h:
+            worst_near = max(worst_near, max(e[:4]))
+        else:
+            worst_far = max(worst_far, max(e[:4]))
+    assert worst_near < 1e-10, worst_near
+    assert worst_far < 2e-6, worst_far
+    assert worst_im < 1e-13, worst_im
+
+
+def test_finite_depth_tends_to_infinite_depth(oracle):
+    # h -> inf: the regular part plus the bottom images tends to the infinite-depth kernel
+    # K Ft(X, V3) (the bottom's wave response decays algebraically, about h^-3); Im G tends to
+    # the infinite-depth K Gi
+    K = 1.0
+    for R, z, ze in [(0.3, -0.2, -0.5), (1.5, -0.1, -0.1), (3.0, -1.0, -0.3)]:
+        w = oracle.wave_kernel(K * R, -K * (z + ze))
+        diffs = []
+        for h in (50.0, 100.0, 200.0):
+            g = oracle.finite_kernel(h, K, R, z, ze)
+            v = [z - ze + 2 * h, -(z - ze) + 2 * h, z + ze + 2 * h, z + ze + 4 * h]
+            rank = sum(1.0 / math.hypot(R, vv) for vv in v)
+            diffs.append(abs(g[0] + rank - K * w[0]))
+            assert g[4] == pytest.approx(K * w[3], abs=1e-12)
+        assert diffs[1] < 1e-6 and diffs[2] < diffs[1] / 4 < diffs[0] / 16, diffs
