@@ -161,7 +161,8 @@ class Session:
                 "excitation_haskind": np.transpose(HK, (0, 2, 1)).copy()}
 
     PROFILE_KEYS = ("statics_ms", "assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "lu_gemm_flops",
-                    "lu_gemm_launches", "forces_ms", "frequencies", "lu_flops", "assembly_bytes", "statics_runs")
+                    "lu_gemm_launches", "forces_ms", "frequencies", "lu_flops", "assembly_bytes", "statics_runs",
+                    "lu_fallback_panels")
 
     def set_profiling(self, enable: bool):
         _check(library().hbg_set_profiling(self._h, int(bool(enable))))

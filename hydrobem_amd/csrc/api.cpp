@@ -35,14 +35,12 @@ void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, 
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
                      hipStream_t stream);
-void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, hipStream_t stream);
+void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,
+                hipStream_t stream);
+int rhs_chunks(int N);
 void launch_forces(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double rho,
                    const double2* M, int ldm, const double2* phiI, const double2* dphiI, double* out,
                    hipStream_t stream);
-struct LuStats {
-  double total_ms = 0.0, gemm_ms = 0.0, gemm_flops = 0.0;
-  long gemm_launches = 0;
-};
 int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
@@ -259,7 +257,7 @@ struct HbgSession {
   DevBuf<StaticD> d_st;
   DevBuf<int2> d_queue;
   DevBuf<unsigned> d_qcount;
-  DevBuf<double2> d_S, d_M, d_phiI, d_dphiI, d_u;
+  DevBuf<double2> d_S, d_M, d_phiI, d_dphiI, d_u, d_part;
   bool statics_valid = false;
   int st_logint = -1;
   double st_tol = -1.0;
@@ -439,6 +437,7 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_phiI.release();
   s->d_dphiI.release();
   s->d_u.release();
+  s->d_part.release();
   s->d_phi1.release();
   s->d_phi2.release();
   s->d_pf.release();
@@ -713,6 +712,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   s->d_phiI.ensure((size_t)std::max(H, 1) * N);
   s->d_dphiI.ensure((size_t)std::max(H, 1) * N);
   s->d_u.ensure((size_t)R * N);
+  s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
   std::vector<double> heads(std::max(H, 1), 0.0);
   for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
   s->d_heads.ensure(heads.size());
@@ -731,7 +731,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                     std::isfinite(s->depth) ? s->depth : 0.0, s->d_heads.p, s->d_phiI.p, s->d_dphiI.p, s->d_u.p,
                     s->stream);
     if (prof) HBG_HIP(hipEventRecord(s->ev[3], s->stream));
-    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->stream);
+    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->d_part.p, s->stream);
     if (prof) HBG_HIP(hipEventRecord(s->ev[4], s->stream));
     if (progress && progress(user, HBG_STAGE_LU, fi, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     LuStats lus;
@@ -758,6 +758,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       s->prof[8] += 1.0;
       s->prof[9] += 8.0 / 3.0 * (double)N * N * N;
       s->prof[10] += (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
+      s->prof[12] += (double)lus.fallback_panels;
     }
     for (double v : hout)
       if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};

@@ -150,23 +150,29 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
   }
 }
 
-// rhs(:, c) = S U(:, c), U = -u (lo_frequency.rs:855-893 with G = 1): one row k per thread,
-// S read column by column (coalesced along k), written into the augmented columns of M.
+// rhs(:, c) = S U(:, c), U = -u (lo_frequency.rs:855-893 with G = 1), in two passes: every
+// workgroup multiplies a block of 256 rows of S by one chunk of RHS_CHUNK columns (one row per
+// thread, S read column by column, coalesced along the rows, u wavefront-uniform), the partial
+// sums [chunk][c][row] are then added in chunk order (deterministic), into the augmented columns
+// of M.
 constexpr int MAXRHS = 32;
-__global__ void __launch_bounds__(256) rhs_kernel(const double2* __restrict__ S, int N, int lds,
-                                                  const double2* __restrict__ u, int R, double2* __restrict__ M,
-                                                  int ldm) {
+constexpr int RHS_CHUNK = 256;
+__global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restrict__ S, int N, int lds,
+                                                          const double2* __restrict__ u, int R,
+                                                          double2* __restrict__ part) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunk = blockIdx.y;
   if (k >= N) return;
   double2 acc[MAXRHS];
 #pragma unroll
   for (int c = 0; c < MAXRHS; ++c) acc[c] = make_double2(0.0, 0.0);
-  for (int l = 0; l < N; ++l) {
+  const int l0 = chunk * RHS_CHUNK, l1 = min(N, l0 + RHS_CHUNK);
+  for (int l = l0; l < l1; ++l) {
     const double2 s = S[(size_t)l * lds + k];
 #pragma unroll
     for (int c = 0; c < MAXRHS; ++c) {
       if (c < R) {
-        const double2 v = u[(size_t)c * N + l];  // wavefront-uniform
+        const double2 v = u[(size_t)c * N + l];
         acc[c].x -= s.x * v.x - s.y * v.y;
         acc[c].y -= s.x * v.y + s.y * v.x;
       }
@@ -174,7 +180,21 @@ __global__ void __launch_bounds__(256) rhs_kernel(const double2* __restrict__ S,
   }
 #pragma unroll
   for (int c = 0; c < MAXRHS; ++c)
-    if (c < R) M[(size_t)(N + c) * ldm + k] = acc[c];
+    if (c < R) part[((size_t)chunk * R + c) * N + k] = acc[c];
+}
+
+__global__ void rhs_reduce_kernel(const double2* __restrict__ part, int N, int R, int chunks, double2* __restrict__ M,
+                                  int ldm) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)R * N) return;
+  const int c = (int)(e / N), k = (int)(e % N);
+  double2 s = make_double2(0.0, 0.0);
+  for (int q = 0; q < chunks; ++q) {
+    const double2 v = part[((size_t)q * R + c) * N + k];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  M[(size_t)(N + c) * ldm + k] = s;
 }
 
 // The forces of one frequency (lo_frequency.rs:1039-1171), each output a sum over the panels by
@@ -308,8 +328,14 @@ void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, 
   HBG_HIP(hipGetLastError());
 }
 
-void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, hipStream_t stream) {
-  rhs_kernel<<<(N + 255) / 256, 256, 0, stream>>>(S, N, lds, u, R, M, ldm);
+int rhs_chunks(int N) { return (N + RHS_CHUNK - 1) / RHS_CHUNK; }
+
+// part: rhs_chunks(N) * R * N values of scratch
+void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,
+                hipStream_t stream) {
+  const int chunks = rhs_chunks(N);
+  rhs_partial_kernel<<<dim3((N + 255) / 256, chunks), 256, 0, stream>>>(S, N, lds, u, R, part);
+  rhs_reduce_kernel<<<(unsigned)(((size_t)R * N + 255) / 256), 256, 0, stream>>>(part, N, R, chunks, M, ldm);
   HBG_HIP(hipGetLastError());
 }
 

@@ -55,6 +55,15 @@ struct WaveTables {
   const double* t1d = nullptr;  // [XT][TILE1]
 };
 
+// Timing of the LU's kernels by HIP events (lu.hip): the whole factorization and solve, the
+// trailing-update GEMMs with their algorithmic flops (8 m n k each), and the panels that needed
+// interchanges (the pivoting fallback).
+struct LuStats {
+  double total_ms = 0.0, gemm_ms = 0.0, gemm_flops = 0.0;
+  long gemm_launches = 0;
+  long fallback_panels = 0;
+};
+
 // ------------------------------------------------------------------ host helpers
 struct Error {
   int status;

@@ -1,26 +1,24 @@
-// Dense complex LU of the system matrices, FP64, on the GPU (linalg.rs:134-252 semantics).
+// Dense complex LU of the system matrices, FP64, on the GPU (linalg.rs:134-252 semantics:
+// partial pivoting on the largest |re| + |im| as LAPACK's izamax, LAPACK ipiv, unit lower L,
+// "singular system matrix (zero pivot k)").
 //
 // The matrix is stored column-major with the right-hand sides appended as extra columns
 // ([A | B], n x (n + r), leading dimension lda), so that the row interchanges, the block row of U
 // and the trailing updates carry the forward substitution L^-1 P B along; a blocked back
-// substitution then solves U X = L^-1 P B.
+// substitution with the stored inverses of U's diagonal blocks then solves U X = L^-1 P B.
 //
-// Right-looking, in panels of NB columns:
-//   1. pivot rows of the panel by a tournament (tslu_kernel, one launch): every workgroup
-//      takes up to 256 rows of the panel, one row per thread in registers, and eliminates with
-//      partial pivoting (the pivot is the entry of largest |re| + |im|, as LAPACK's izamax; on a
-//      tie the lower row index), the pivot search a wavefront shuffle reduction plus one LDS
-//      exchange across the four wavefronts; its NB pivot rows go up a tree of 4-way merges that
-//      repeat the elimination on the original values of the candidates; the last workgroup to
-//      finish a node merges it (atomic counters, no waiting). The root's elimination gives the
-//      panel's diagonal block L11 \ U11. With one leaf this is partial pivoting exactly; with
-//      more it is tournament pivoting (Grigori, Demmel & Xiang 2011), whose pivots can differ
-//      from partial pivoting's while keeping its stability in practice;
-//   2. the interchanges in LAPACK's ipiv form, applied to every column (swap_kernel);
-//   3. L21 = A21 U11^-1 (one row per thread) and U12 = L11^-1 A12 (one column per thread);
-//   4. the trailing update A22 -= L21 U12 (gemm_update_kernel).
-// A zero pivot leaves the factorization going and records the first such column; the caller
-// reports "singular system matrix (zero pivot k)" as linalg.rs:221-233 does.
+// Right-looking, in panels of NB columns. Each panel is first factored without interchanges
+// (diag_nopiv_kernel on the diagonal block in LDS, L21 = A21 U11^-1 by tri_apply_kernel) while
+// checking, column by column, that the diagonal entry is the largest |re| + |im| of its column at
+// that step of the elimination: then partial pivoting would make no interchange and the result
+// IS the partially pivoted factorization. Where the check fails (flag of the panel), the gated
+// fallback kernels restore the saved panel and pivot it by a tournament (tslu_kernel): every
+// workgroup takes up to 256 rows, one row per thread in registers, and eliminates with partial
+// pivoting (a wavefront shuffle reduction per column), its NB pivot rows go up a tree of 4-way
+// merges that repeat the elimination on the original values of the candidates; the last
+// workgroup to finish a node merges it (atomic counters, no waiting). With one leaf this is
+// partial pivoting exactly; with more it is tournament pivoting (Grigori, Demmel & Xiang 2011).
+// Then U12 = L11^-1 A12 and the trailing update A22 -= L21 U12 (gemm_update_kernel).
 #include <vector>
 
 #include "hbg_internal.h"
@@ -127,7 +125,8 @@ __device__ void eliminate(double2 (&a)[NB], int grow, int nrows, int kb, int* wi
 
 // The tournament of the panel (k0, kb) over rows [k0, n): grid = number of leaves.
 __global__ void __launch_bounds__(LEAF) tslu_kernel(const double2* __restrict__ A, int n, int lda, int k0, int kb,
-                                                    TsluWork w) {
+                                                    TsluWork w, const int* __restrict__ gate) {
+  if (*gate == 0) return;
   __shared__ int win[NB];
   __shared__ double2 urow[NB];
   __shared__ double redv[LEAF / 64];
@@ -197,46 +196,31 @@ __global__ void __launch_bounds__(LEAF) tslu_kernel(const double2* __restrict__ 
   }
 }
 
-// LAPACK-style interchanges of the panel from its pivot rows, then applied to all columns
-// [0, ncols) of the augmented matrix; the root's L11 \ U11 written into the diagonal block.
-__global__ void ipiv_kernel(int k0, int kb, const int* __restrict__ winners, int* __restrict__ ipiv) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  // positions touched so far: (position, original row now there)
-  int pos[2 * NB], who[2 * NB];
-  int t = 0;
-  auto at = [&](int p) {  // original row now at position p
-    for (int i = 0; i < t; ++i)
-      if (pos[i] == p) return who[i];
-    return p;
-  };
-  auto where = [&](int r) {  // position of original row r
-    for (int i = 0; i < t; ++i)
-      if (who[i] == r) return pos[i];
-    // untouched rows sit where they started, unless that position was overwritten
-    return r;
-  };
-  auto set = [&](int p, int r) {
-    for (int i = 0; i < t; ++i)
-      if (pos[i] == p) {
-        who[i] = r;
-        return;
-      }
-    pos[t] = p;
-    who[t] = r;
-    ++t;
-  };
+// The pivot interchanges of the panel in LAPACK's ipiv form from its pivot rows, by one
+// wavefront: lane l tracks the position of the panel's original row k0 + l (pos) and the original
+// row now at position k0 + l (at); a pivot row outside the block sits where it started until it
+// is chosen. Gated by the panel's flag.
+__global__ void __launch_bounds__(64) ipiv_kernel(int k0, int kb, const int* __restrict__ winners,
+                                                  int* __restrict__ ipiv, const int* __restrict__ gate) {
+  if (*gate == 0) return;
+  const int l = threadIdx.x;
+  int pos = k0 + l, at = k0 + l;
   for (int i = 0; i < kb; ++i) {
-    const int r = winners[i];
-    const int p = where(r);
-    ipiv[k0 + i] = p;
-    const int o = at(k0 + i);
-    set(k0 + i, r);
-    set(p, o);
+    const int w = winners[i];
+    const bool inside = w < k0 + kb;
+    const int p = inside ? __shfl(pos, w - k0) : w;  // current position of row w
+    const int o = __shfl(at, i);                        // the original block row at position k0 + i
+    if (l == i) at = w;
+    if (p < k0 + kb && l == p - k0) at = o;
+    if (inside && l == w - k0) pos = k0 + i;
+    if (l == o - k0) pos = p;
+    if (l == 0) ipiv[k0 + i] = p;
   }
 }
 
 __global__ void swap_kernel(double2* __restrict__ A, int lda, int ncols, int k0, int kb, const int* __restrict__ ipiv,
-                            const double2* __restrict__ top) {
+                            const double2* __restrict__ top, const int* __restrict__ gate) {
+  if (*gate == 0) return;
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncols) return;
   double2* c = A + (size_t)col * lda;
@@ -254,74 +238,126 @@ __global__ void swap_kernel(double2* __restrict__ A, int lda, int ncols, int k0,
   }
 }
 
-// The inverses of the diagonal block's factors: Uinv = U11^-1 (upper) and Linv = L11^-1 (unit
-// lower), one column per thread by substitution in LDS; kb <= NB, unused entries zero.
-__global__ void __launch_bounds__(2 * NB) trinv_kernel(const double2* __restrict__ A, int lda, int k0, int kb,
-                                                        double2* __restrict__ Uinv, double2* __restrict__ Linv) {
+// the panel's columns [k0, k0 + kb), rows [k0, n): saved before the attempt without interchanges
+// (dir 0) and restored for the fallback (dir 1, gated)
+__global__ void panel_copy_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb, double2* __restrict__ save,
+                                  int dir, const int* __restrict__ gate) {
+  if (dir == 1 && *gate == 0) return;
+  const int m = n - k0;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)m * kb) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  double2* a = M + (size_t)(k0 + c) * lda + k0 + r;
+  if (dir == 0)
+    save[e] = *a;
+  else
+    *a = save[e];
+}
+
+// The diagonal block (k0, kb): mode 0 factors it without interchanges in LDS, checking at every
+// step j that no entry below the diagonal exceeds the pivot in |re| + |im| (else the panel's flag
+// is set), writes L11 \ U11 back and the identity interchanges; mode 1 (the fallback, gated) takes
+// the block as the tournament wrote it. Both then invert the triangles (Gauss-Jordan in LDS):
+// Uinv = U11^-1 (kept per panel for the back substitution), Linv = L11^-1, and the diagonal of U.
+__global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int lda, int k0, int kb, int mode,
+                                                   int* __restrict__ flag, int* __restrict__ zp_fast,
+                                                   int* __restrict__ ipiv, double2* __restrict__ Uinv,
+                                                   double2* __restrict__ Linv, double2* __restrict__ udiag) {
+  if (mode == 1 && *flag == 0) return;
   __shared__ double2 T[NB][NB + 1];
   __shared__ double2 X[NB][NB + 1];
+  __shared__ int s_fail;
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += blockDim.x) {
+  if (tid == 0) s_fail = 0;
+  for (int e = tid; e < NB * NB; e += 256) {
     const int r = e % NB, c = e / NB;
-    T[r][c] = (r < kb && c < kb) ? A[(size_t)(k0 + c) * lda + k0 + r] : make_double2(0.0, 0.0);
+    T[r][c] = (r < kb && c < kb) ? M[(size_t)(k0 + c) * lda + k0 + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  const int j = tid % NB;
-  const bool upper = tid < NB;
-  if (j < kb) {
-    if (upper) {
-      // U X = e_j: X[i][j] for i = j .. 0
-      for (int i = kb - 1; i > j; --i) X[i][j] = make_double2(0.0, 0.0);
-      const double2 d = T[j][j];
-      X[j][j] = cdiv(make_double2(1.0, 0.0), d);
-      for (int i = j - 1; i >= 0; --i) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int l = i + 1; l <= j; ++l) {
-          const double2 u = T[i][l], x = X[l][j];
-          s.x += u.x * x.x - u.y * x.y;
-          s.y += u.x * x.y + u.y * x.x;
-        }
-        X[i][j] = cdiv(make_double2(-s.x, -s.y), T[i][i]);
+  if (mode == 0) {
+    for (int j = 0; j < kb; ++j) {
+      const double2 piv = T[j][j];
+      const double pm = mag1(piv);
+      for (int i = j + 1 + tid; i < kb; i += 256)
+        if (mag1(T[i][j]) > pm) s_fail = 1;
+      if (tid == 0 && pm == 0.0) atomicMin(zp_fast, k0 + j + 1);
+      __syncthreads();
+      const int rows = kb - j - 1;
+      for (int e = tid; e < rows * (kb - j); e += 256) {
+        const int i = j + 1 + e / (kb - j), c = j + e % (kb - j);
+        if (c == j) continue;
+        const double2 l = pm == 0.0 ? make_double2(0.0, 0.0) : cdiv(T[i][j], piv);
+        const double2 u = T[j][c];
+        T[i][c].x -= l.x * u.x - l.y * u.y;
+        T[i][c].y -= l.x * u.y + l.y * u.x;
       }
-      for (int i = 0; i < NB; ++i) Uinv[i * NB + j] = i < kb ? X[i][j] : make_double2(0.0, 0.0);
+      __syncthreads();
+      for (int i = j + 1 + tid; i < kb; i += 256) T[i][j] = pm == 0.0 ? make_double2(0.0, 0.0) : cdiv(T[i][j], piv);
+      __syncthreads();
     }
+    for (int e = tid; e < kb * kb; e += 256) {
+      const int r = e % kb, c = e / kb;
+      M[(size_t)(k0 + c) * lda + k0 + r] = T[r][c];
+    }
+    for (int i = tid; i < kb; i += 256) ipiv[k0 + i] = k0 + i;
+    __syncthreads();
+    if (tid == 0 && s_fail) *flag = 1;
   }
+  // Uinv: U X = I, rows j = kb-1 .. 0 (X[j][*] /= u_jj, X[i][*] -= u_ij X[j][*] for i < j)
+  for (int e = tid; e < NB * NB; e += 256) X[e / NB][e % NB] = make_double2((e / NB) == (e % NB) ? 1.0 : 0.0, 0.0);
   __syncthreads();
-  if (j < kb && !upper) {
-    // L X = e_j, L unit lower: X[i][j] for i = j .. kb - 1
-    for (int i = 0; i < j; ++i) X[i][j] = make_double2(0.0, 0.0);
-    X[j][j] = make_double2(1.0, 0.0);
-    for (int i = j + 1; i < kb; ++i) {
-      double2 s = make_double2(0.0, 0.0);
-      for (int l = j; l < i; ++l) {
-        const double2 v = T[i][l], x = X[l][j];
-        s.x += v.x * x.x - v.y * x.y;
-        s.y += v.x * x.y + v.y * x.x;
-      }
-      X[i][j] = make_double2(-s.x, -s.y);
+  for (int j = kb - 1; j >= 0; --j) {
+    const double2 d = T[j][j];
+    const bool zero = d.x == 0.0 && d.y == 0.0;
+    if (tid < NB) X[j][tid] = zero ? make_double2(0.0, 0.0) : cdiv(X[j][tid], d);
+    __syncthreads();
+    for (int e = tid; e < j * NB; e += 256) {
+      const int i = e / NB, c = e % NB;
+      const double2 u = T[i][j], x = X[j][c];
+      X[i][c].x -= u.x * x.x - u.y * x.y;
+      X[i][c].y -= u.x * x.y + u.y * x.x;
     }
-    for (int i = 0; i < NB; ++i) Linv[i * NB + j] = i < kb ? X[i][j] : make_double2(0.0, 0.0);
+    __syncthreads();
   }
-  if (j >= kb) {
-    for (int i = 0; i < NB; ++i) (upper ? Uinv : Linv)[i * NB + j] = make_double2(0.0, 0.0);
+  for (int e = tid; e < NB * NB; e += 256) Uinv[e] = X[e / NB][e % NB];
+  if (tid < NB) udiag[tid] = tid < kb ? T[tid][tid] : make_double2(1.0, 0.0);
+  __syncthreads();
+  // Linv: L X = I, L unit lower: X[i][*] -= l_ij X[j][*] for i > j, j = 0 .. kb-1
+  for (int e = tid; e < NB * NB; e += 256) X[e / NB][e % NB] = make_double2((e / NB) == (e % NB) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    for (int e = tid; e < (kb - j - 1) * NB; e += 256) {
+      const int i = j + 1 + e / NB, c = e % NB;
+      const double2 v = T[i][j], x = X[j][c];
+      X[i][c].x -= v.x * x.x - v.y * x.y;
+      X[i][c].y -= v.x * x.y + v.y * x.x;
+    }
+    __syncthreads();
   }
+  for (int e = tid; e < NB * NB; e += 256) Linv[e] = X[e / NB][e % NB];
 }
 
 // In-place products with a kb x kb triangle inverse Tm (row-major NB x NB):
-//   right: X = X Tm for X = rows [r0, n) of columns [k0, k0 + kb)      (L21 = A21 U11^-1)
-//   left:  X = Tm X for X = rows [k0, k0 + kb) of columns [c0, ncols)  (U12 = L11^-1 A12)
-// 64 rows (right) or 64 columns (left) per workgroup, the block and Tm staged in LDS.
+//   right: X = X Tm for X = rows [k0 + kb, lim) of columns [k0, k0 + kb)   (L21 = A21 U11^-1)
+//   left:  X = Tm X for X = rows [k0, k0 + kb) of columns [k0 + kb, lim)   (U12 = L11^-1 A12)
+// 64 rows (right) or 64 columns (left) per workgroup, the block and Tm staged in LDS. The right
+// product also checks that |l_ij u_jj| (|re| + |im|) does not exceed |u_jj|: the entry at step j
+// of the elimination below the pivot (else the panel's flag). gate: run only if the flag is set.
 __global__ void __launch_bounds__(256) tri_apply_kernel(double2* __restrict__ M, int lda, int lim, int k0, int kb,
-                                                        const double2* __restrict__ Tm, int right) {
+                                                        const double2* __restrict__ Tm, int right,
+                                                        const double2* __restrict__ udiag, int* __restrict__ flag,
+                                                        const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   __shared__ double2 Xs[NB][NB + 1];  // right: [row][k]; left: [k][col]
   __shared__ double2 Ts[NB][NB + 1];
+  __shared__ int s_fail;
   const int tid = threadIdx.x;
-  const int base = (right ? k0 + kb : k0 + kb) + blockIdx.x * NB;  // first row (right) / column (left)
+  if (tid == 0) s_fail = 0;
+  const int base = k0 + kb + blockIdx.x * NB;  // first row (right) / column (left)
   for (int e = tid; e < NB * NB; e += 256) {
     const int a = e / NB, b = e % NB;
     Ts[a][b] = Tm[a * NB + b];
     if (right) {
-      // e = k * NB + i: column k0 + k, row base + i (coalesced along rows)
       const int k = e / NB, i = e % NB;
       Xs[i][k] = (base + i < lim && k < kb) ? M[(size_t)(k0 + k) * lda + base + i] : make_double2(0.0, 0.0);
     } else {
@@ -330,7 +366,6 @@ __global__ void __launch_bounds__(256) tri_apply_kernel(double2* __restrict__ M,
     }
   }
   __syncthreads();
-  // thread: 4 x 4 outputs, rows ti + 16 u, columns tj + 16 v of the 64 x 64 result
   const int ti = tid & 15, tj = tid >> 4;
   double2 acc[4][4];
 #pragma unroll
@@ -351,17 +386,33 @@ __global__ void __launch_bounds__(256) tri_apply_kernel(double2* __restrict__ M,
         acc[u][v].y = fma(a[u].x, b[v].y, fma(a[u].y, b[v].x, acc[u][v].y));
       }
   }
+  bool fail = false;
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = ti + 16 * u, c = tj + 16 * v;
       if (right) {
-        if (base + r < lim && c < kb) M[(size_t)(k0 + c) * lda + base + r] = acc[u][v];
+        if (base + r < lim && c < kb) {
+          M[(size_t)(k0 + c) * lda + base + r] = acc[u][v];
+          if (flag) {
+            const double2 d = udiag[c];
+            if (mag1(cmul(acc[u][v], d)) > mag1(d)) fail = true;
+          }
+        }
       } else {
         if (base + c < lim && r < kb) M[(size_t)(base + c) * lda + k0 + r] = acc[u][v];
       }
     }
+  if (fail) s_fail = 1;
+  __syncthreads();
+  if (tid == 0 && s_fail && flag) *flag = 1;
+}
+
+// After the panel: the zero pivot found without interchanges counts when no fallback ran.
+__global__ void panel_finish_kernel(const int* __restrict__ flag, const int* __restrict__ zp_fast,
+                                    int* __restrict__ zero_pivot) {
+  if (threadIdx.x == 0 && *flag == 0 && *zp_fast != 0x7fffffff) atomicMin(zero_pivot, *zp_fast);
 }
 
 // C -= A B on the trailing matrix: rows [r0, n), columns [c0, ncols), K = kb (<= NB); A is
@@ -425,43 +476,64 @@ __global__ void __launch_bounds__(256) gemm_update_kernel(double2* __restrict__ 
   }
 }
 
-// Back substitution U X = Y on the augmented columns [n, n + r): the diagonal block (k0, kb)
-// solved by one workgroup per right-hand side, then the rows above updated.
-__global__ void __launch_bounds__(NB) backsolve_block_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb) {
-  __shared__ double2 y[NB];
-  const int rhs = blockIdx.x;
-  const int t = threadIdx.x;
-  double2* b = M + (size_t)(n + rhs) * lda;
-  if (t < kb) y[t] = b[k0 + t];
-  __syncthreads();
-  for (int r = kb - 1; r >= 0; --r) {
-    if (t == 0) y[r] = cdiv(y[r], M[(size_t)(k0 + r) * lda + k0 + r]);
-    __syncthreads();
-    if (t < r) {
-      const double2 u = M[(size_t)(k0 + r) * lda + k0 + t];
-      const double2 v = y[r];
-      y[t].x -= u.x * v.x - u.y * v.y;
-      y[t].y -= u.x * v.y + u.y * v.x;
-    }
-    __syncthreads();
+// Back substitution U X = Y on the augmented columns [n, n + r), block by block from the bottom:
+// the block of rows (k0, kb) is X_b = Uinv_b Y_b (the stored inverse of its diagonal block), then
+// the rows above are updated, Y[0:k0] -= U[0:k0, b] X_b.
+__global__ void __launch_bounds__(256) backsolve_block_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
+                                                              const double2* __restrict__ Uinv, int nrhs) {
+  __shared__ double2 Y[NB][33];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kb * nrhs; e += 256) {
+    const int r = e % kb, c = e / kb;
+    Y[r][c] = M[(size_t)(n + c) * lda + k0 + r];
   }
-  if (t < kb) b[k0 + t] = y[t];
+  __syncthreads();
+  for (int e = tid; e < kb * nrhs; e += 256) {
+    const int r = e % kb, c = e / kb;
+    double2 s = make_double2(0.0, 0.0);
+    for (int k = r; k < kb; ++k) {
+      const double2 u = Uinv[r * NB + k], y = Y[k][c];
+      s.x += u.x * y.x - u.y * y.y;
+      s.y += u.x * y.y + u.y * y.x;
+    }
+    M[(size_t)(n + c) * lda + k0 + r] = s;
+  }
 }
 
 __global__ void __launch_bounds__(256) backsolve_update_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
                                                                int nrhs) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double2 Xb[NB][33];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kb * nrhs; e += 256) {
+    const int r = e % kb, c = e / kb;
+    Xb[r][c] = M[(size_t)(n + c) * lda + k0 + r];
+  }
+  __syncthreads();
+  const int row = blockIdx.x * blockDim.x + tid;
   if (row >= k0) return;
-  for (int r = 0; r < nrhs; ++r) {
-    double2* b = M + (size_t)(n + r) * lda;
-    double2 s = b[row];
-    for (int c = 0; c < kb; ++c) {
-      const double2 u = M[(size_t)(k0 + c) * lda + row];
-      const double2 x = b[k0 + c];
-      s.x -= u.x * x.x - u.y * x.y;
-      s.y -= u.x * x.y + u.y * x.x;
+  double2 acc[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) acc[r] = make_double2(0.0, 0.0);
+  for (int c = 0; c < kb; ++c) {
+    const double2 u = M[(size_t)(k0 + c) * lda + row];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+      if (r < nrhs) {
+        const double2 x = Xb[c][r];
+        acc[r].x += u.x * x.x - u.y * x.y;
+        acc[r].y += u.x * x.y + u.y * x.x;
+      }
     }
-    b[row] = s;
+  }
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    if (r < nrhs) {
+      double2* b = M + (size_t)(n + r) * lda + row;
+      double2 v = *b;
+      v.x -= acc[r].x;
+      v.y -= acc[r].y;
+      *b = v;
+    }
   }
 }
 
@@ -479,8 +551,12 @@ struct LuWorkspace {
   int n = 0;
   TsluWork w{};
   int* ipiv = nullptr;
-  double2* uinv = nullptr;
+  int* flags = nullptr;      // per panel: 1 if the panel needed interchanges
+  int* zp_fast = nullptr;    // per panel: zero pivot found without interchanges
+  double2* uinv = nullptr;   // per panel: U11^-1 (NB x NB), for the back substitution
   double2* linv = nullptr;
+  double2* udiag = nullptr;
+  double2* save = nullptr;   // the panel before the attempt without interchanges
   void release() {
     hipFree(w.cand);
     hipFree(w.ccount);
@@ -489,8 +565,12 @@ struct LuWorkspace {
     hipFree(w.winners);
     hipFree(w.zero_pivot);
     hipFree(ipiv);
+    hipFree(flags);
+    hipFree(zp_fast);
     hipFree(uinv);
     hipFree(linv);
+    hipFree(udiag);
+    hipFree(save);
     *this = LuWorkspace{};
   }
 };
@@ -504,6 +584,7 @@ static void lu_workspace(LuWorkspace& ws, int n) {
     nodes = (nodes + FAN - 1) / FAN;
     ++levels;
   }
+  const int panels = (n + NB - 1) / NB;
   ws.w.levels = levels;
   ws.w.max_nodes = leaves;
   const size_t slots = (size_t)levels * leaves;
@@ -514,8 +595,12 @@ static void lu_workspace(LuWorkspace& ws, int n) {
   HBG_HIP(hipMalloc(&ws.w.winners, NB * sizeof(int)));
   HBG_HIP(hipMalloc(&ws.w.zero_pivot, sizeof(int)));
   HBG_HIP(hipMalloc(&ws.ipiv, (size_t)n * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.uinv, (size_t)NB * NB * sizeof(double2)));
+  HBG_HIP(hipMalloc(&ws.flags, (size_t)panels * sizeof(int)));
+  HBG_HIP(hipMalloc(&ws.zp_fast, (size_t)panels * sizeof(int)));
+  HBG_HIP(hipMalloc(&ws.uinv, (size_t)panels * NB * NB * sizeof(double2)));
   HBG_HIP(hipMalloc(&ws.linv, (size_t)NB * NB * sizeof(double2)));
+  HBG_HIP(hipMalloc(&ws.udiag, (size_t)NB * sizeof(double2)));
+  HBG_HIP(hipMalloc(&ws.save, (size_t)n * NB * sizeof(double2)));
   ws.n = n;
 }
 
@@ -524,12 +609,6 @@ static LuWorkspace& workspace() {
   return ws;
 }
 
-// Timing of the LU's kernels by HIP events on its stream (when requested): the trailing-update
-// GEMMs (their algorithmic flops 8 m n k each) and the whole factorization and solve.
-struct LuStats {
-  double total_ms = 0.0, gemm_ms = 0.0, gemm_flops = 0.0;
-  long gemm_launches = 0;
-};
 
 namespace {
 struct EventPool {
@@ -564,19 +643,37 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
   LuWorkspace& ws = workspace();
   lu_workspace(ws, n);
   const int ncols = n + nrhs;
+  const int panels = (n + NB - 1) / NB;
   const size_t slots = (size_t)ws.w.levels * ws.w.max_nodes;
   fill_i32<<<1, 64, 0, stream>>>(ws.w.zero_pivot, 1, 0x7fffffff);
-  for (int k0 = 0; k0 < n; k0 += NB) {
+  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
+  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
+  for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
     const int kb = min(NB, n - k0);
     const int m = n - k0;
+    int* flag = ws.flags + p;
+    double2* uinv = ws.uinv + (size_t)p * NB * NB;
+    const unsigned pe = (unsigned)(((size_t)m * kb + 255) / 256);
+    // attempt without interchanges, with the partial-pivoting check
+    panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 0, flag);
+    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    if (m > kb)
+      tri_apply_kernel<<<(m - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, n, k0, kb, uinv, 1, ws.udiag, flag,
+                                                                   nullptr);
+    // the fallback, gated by the panel's flag
+    panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 1, flag);
     fill_u32<<<(unsigned)((slots + 255) / 256), 256, 0, stream>>>(ws.w.ctr, slots, 0u);
-    tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w);
-    ipiv_kernel<<<1, 1, 0, stream>>>(k0, kb, ws.w.winners, ws.ipiv);
-    swap_kernel<<<(ncols + 255) / 256, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.ipiv, ws.w.top);
-    trinv_kernel<<<1, 2 * NB, 0, stream>>>(M, lda, k0, kb, ws.uinv, ws.linv);
-    if (m > kb) tri_apply_kernel<<<(m - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, n, k0, kb, ws.uinv, 1);
+    tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w, flag);
+    ipiv_kernel<<<1, 64, 0, stream>>>(k0, kb, ws.w.winners, ws.ipiv, flag);
+    swap_kernel<<<(ncols + 255) / 256, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.ipiv, ws.w.top, flag);
+    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    if (m > kb)
+      tri_apply_kernel<<<(m - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, n, k0, kb, uinv, 1, ws.udiag, nullptr,
+                                                                   flag);
+    panel_finish_kernel<<<1, 64, 0, stream>>>(flag, ws.zp_fast + p, ws.w.zero_pivot);
     if (ncols > k0 + kb) {
-      tri_apply_kernel<<<(ncols - k0 - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.linv, 0);
+      tri_apply_kernel<<<(ncols - k0 - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.linv, 0,
+                                                                          nullptr, nullptr, nullptr);
       if (m > kb) {
         dim3 grid((m - kb + 63) / 64, (ncols - k0 - kb + 63) / 64);
         hipEvent_t g0 = nullptr, g1 = nullptr;
@@ -596,9 +693,9 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
     }
   }
   HBG_HIP(hipGetLastError());
-  for (int k0 = ((n - 1) / NB) * NB; k0 >= 0 && nrhs > 0; k0 -= NB) {
+  for (int k0 = ((n - 1) / NB) * NB, p = (n - 1) / NB; k0 >= 0 && nrhs > 0; k0 -= NB, --p) {
     const int kb = min(NB, n - k0);
-    backsolve_block_kernel<<<nrhs, NB, 0, stream>>>(M, n, lda, k0, kb);
+    backsolve_block_kernel<<<1, 256, 0, stream>>>(M, n, lda, k0, kb, ws.uinv + (size_t)p * NB * NB, nrhs);
     if (k0 > 0) backsolve_update_kernel<<<(k0 + 255) / 256, 256, 0, stream>>>(M, n, lda, k0, kb, nrhs);
   }
   HBG_HIP(hipGetLastError());
@@ -607,7 +704,9 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
     HBG_HIP(hipEventRecord(t1, stream));
   }
   int zp = 0;
+  std::vector<int> flags(stats ? panels : 0);
   HBG_HIP(hipMemcpyAsync(&zp, ws.w.zero_pivot, sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (stats) HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
   HBG_HIP(hipStreamSynchronize(stream));
   if (stats) {
     float ms = 0.0f;
@@ -617,6 +716,7 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
       HBG_HIP(hipEventElapsedTime(&ms, g.first, g.second));
       stats->gemm_ms += ms;
     }
+    for (int f : flags) stats->fallback_panels += f != 0;
   }
   return zp == 0x7fffffff ? 0 : zp;
 }

@@ -137,7 +137,8 @@ HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double
  *   0 statics ms, 1 assembly ms (boundary values + matrices), 2 right-hand sides ms, 3 LU ms
  *   (factorization + substitution), 4 LU trailing-update GEMM ms, 5 their algorithmic flops
  *   (8 m n k each), 6 GEMM launches, 7 forces ms, 8 frequencies, 9 LU algorithmic flops
- *   (8 N^3 / 3 each), 10 assembly algorithmic bytes (N^2 (32 + 2 x 16) each), 11 statics runs. */
+ *   (8 N^3 / 3 each), 10 assembly algorithmic bytes (N^2 (32 + 2 x 16) each), 11 statics runs,
+ *   12 LU panels that needed row interchanges (the pivoting fallback). */
 #define HBG_PROFILE_COUNTERS 16
 HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable);
 HbgStatus hbg_profile_read(HbgSession* s, double* out, int32_t n);
