@@ -7,18 +7,20 @@
 // and the trailing updates carry the forward substitution L^-1 P B along; a blocked back
 // substitution with the stored inverses of U's diagonal blocks then solves U X = L^-1 P B.
 //
-// Right-looking, in panels of NB columns. Each panel is first factored without interchanges
-// (diag_nopiv_kernel on the diagonal block in LDS, L21 = A21 U11^-1 by tri_apply_kernel) while
-// checking, column by column, that the diagonal entry is the largest |re| + |im| of its column at
-// that step of the elimination: then partial pivoting would make no interchange and the result
-// IS the partially pivoted factorization. Where the check fails (flag of the panel), the gated
-// fallback kernels restore the saved panel and pivot it by a tournament (tslu_kernel): every
-// workgroup takes up to 256 rows, one row per thread in registers, and eliminates with partial
-// pivoting (a wavefront shuffle reduction per column), its NB pivot rows go up a tree of 4-way
-// merges that repeat the elimination on the original values of the candidates; the last
-// workgroup to finish a node merges it (atomic counters, no waiting). With one leaf this is
-// partial pivoting exactly; with more it is tournament pivoting (Grigori, Demmel & Xiang 2011).
+// Right-looking, in panels of NB columns. The whole matrix is first factored without
+// interchanges (diag_kernel on the diagonal block in registers, L21 = A21 U11^-1 by apply_kernel)
+// while checking, column by column, that the diagonal entry is the largest |re| + |im| of its
+// column at that step of the elimination: then partial pivoting would make no interchange and the
+// result IS the partially pivoted factorization. Where the check fails on any panel (its flag),
+// the saved matrix is restored and factored again panel by panel, the panels whose check fails
+// being restored and pivoted by a tournament (tslu_kernel): every workgroup takes up to 256 rows,
+// one row per thread in registers, and eliminates with partial pivoting (a wavefront shuffle
+// reduction per column), its NB pivot rows go up a tree of 4-way merges that repeat the
+// elimination on the original values of the candidates; the last workgroup to finish a node
+// merges it (atomic counters, no waiting). With one leaf this is partial pivoting exactly; with
+// more it is tournament pivoting (Grigori, Demmel & Xiang 2011).
 // Then U12 = L11^-1 A12 and the trailing update A22 -= L21 U12 (gemm_update_kernel).
+#include <algorithm>
 #include <vector>
 
 #include "hbg_internal.h"
@@ -28,6 +30,7 @@ namespace hbg {
 constexpr int NB = 64;      // panel width
 constexpr int LEAF = 256;   // rows per tournament leaf (one per thread)
 constexpr int FAN = 4;      // candidate sets per merge
+constexpr int MAXRHS_LU = 32;  // right-hand sides carried along
 
 struct TsluWork {
   int* cand;       // [levels][nodes][NB] candidate rows (global indices)
@@ -254,159 +257,258 @@ __global__ void panel_copy_kernel(double2* __restrict__ M, int n, int lda, int k
     *a = save[e];
 }
 
-// The diagonal block (k0, kb): mode 0 factors it without interchanges in LDS, checking at every
-// step j that no entry below the diagonal exceeds the pivot in |re| + |im| (else the panel's flag
-// is set), writes L11 \ U11 back and the identity interchanges; mode 1 (the fallback, gated) takes
-// the block as the tournament wrote it. Both then invert the triangles (Gauss-Jordan in LDS):
-// Uinv = U11^-1 (kept per panel for the back substitution), Linv = L11^-1, and the diagonal of U.
+__device__ __forceinline__ double2 crecip(double2 v) {
+  const double d = v.x * v.x + v.y * v.y;
+  if (d == 0.0) return make_double2(0.0, 0.0);
+  const double id = 1.0 / d;
+  return make_double2(v.x * id, -v.y * id);
+}
+
+// The diagonal block (k0, kb): mode 0 factors it without interchanges, checking at every step j
+// that no entry below the diagonal exceeds the pivot in |re| + |im| (else the panel's flag is
+// set), writes L11 \ U11 back and the identity interchanges; mode 1 (gated) takes the block as the
+// tournament wrote it. Both then invert the triangles: Uinv = U11^-1 (kept per panel for the back
+// substitution), Linv = L11^-1, and keep the diagonal of U. Outside kb the block is the identity.
+//
+// The block lives in registers: thread (ti, tj) of the 16 x 16 grid holds the entries
+// (ti + 16 a, tj + 16 b), a, b < 4. At step j the owners of row j and column j publish them in
+// LDS (two alternating buffers, so one barrier per step) and every thread updates its entries.
 __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int lda, int k0, int kb, int mode,
                                                    int* __restrict__ flag, int* __restrict__ zp_fast,
                                                    int* __restrict__ ipiv, double2* __restrict__ Uinv,
                                                    double2* __restrict__ Linv, double2* __restrict__ udiag) {
   if (mode == 1 && *flag == 0) return;
-  __shared__ double2 T[NB][NB + 1];
-  __shared__ double2 X[NB][NB + 1];
+  __shared__ double2 rowb[2][NB], colb[2][NB];
+  __shared__ double2 rowu[2][NB], colu[2][NB];
+  __shared__ double2 dg[NB];
   __shared__ int s_fail;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
   if (tid == 0) s_fail = 0;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int r = e % NB, c = e / NB;
-    T[r][c] = (r < kb && c < kb) ? M[(size_t)(k0 + c) * lda + k0 + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
-  }
-  __syncthreads();
+  double2 t[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int r = ti + 16 * a, c = tj + 16 * b;
+      t[a][b] = (r < kb && c < kb) ? M[(size_t)(k0 + c) * lda + k0 + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+    }
   if (mode == 0) {
-    for (int j = 0; j < kb; ++j) {
-      const double2 piv = T[j][j];
-      const double pm = mag1(piv);
-      for (int i = j + 1 + tid; i < kb; i += 256)
-        if (mag1(T[i][j]) > pm) s_fail = 1;
-      if (tid == 0 && pm == 0.0) atomicMin(zp_fast, k0 + j + 1);
-      __syncthreads();
-      const int rows = kb - j - 1;
-      for (int e = tid; e < rows * (kb - j); e += 256) {
-        const int i = j + 1 + e / (kb - j), c = j + e % (kb - j);
-        if (c == j) continue;
-        const double2 l = pm == 0.0 ? make_double2(0.0, 0.0) : cdiv(T[i][j], piv);
-        const double2 u = T[j][c];
-        T[i][c].x -= l.x * u.x - l.y * u.y;
-        T[i][c].y -= l.x * u.y + l.y * u.x;
-      }
-      __syncthreads();
-      for (int i = j + 1 + tid; i < kb; i += 256) T[i][j] = pm == 0.0 ? make_double2(0.0, 0.0) : cdiv(T[i][j], piv);
-      __syncthreads();
-    }
-    for (int e = tid; e < kb * kb; e += 256) {
-      const int r = e % kb, c = e / kb;
-      M[(size_t)(k0 + c) * lda + k0 + r] = T[r][c];
-    }
-    for (int i = tid; i < kb; i += 256) ipiv[k0 + i] = k0 + i;
-    __syncthreads();
-    if (tid == 0 && s_fail) *flag = 1;
-  }
-  // Uinv: U X = I, rows j = kb-1 .. 0 (X[j][*] /= u_jj, X[i][*] -= u_ij X[j][*] for i < j)
-  for (int e = tid; e < NB * NB; e += 256) X[e / NB][e % NB] = make_double2((e / NB) == (e % NB) ? 1.0 : 0.0, 0.0);
-  __syncthreads();
-  for (int j = kb - 1; j >= 0; --j) {
-    const double2 d = T[j][j];
-    const bool zero = d.x == 0.0 && d.y == 0.0;
-    if (tid < NB) X[j][tid] = zero ? make_double2(0.0, 0.0) : cdiv(X[j][tid], d);
-    __syncthreads();
-    for (int e = tid; e < j * NB; e += 256) {
-      const int i = e / NB, c = e % NB;
-      const double2 u = T[i][j], x = X[j][c];
-      X[i][c].x -= u.x * x.x - u.y * x.y;
-      X[i][c].y -= u.x * x.y + u.y * x.x;
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < NB * NB; e += 256) Uinv[e] = X[e / NB][e % NB];
-  if (tid < NB) udiag[tid] = tid < kb ? T[tid][tid] : make_double2(1.0, 0.0);
-  __syncthreads();
-  // Linv: L X = I, L unit lower: X[i][*] -= l_ij X[j][*] for i > j, j = 0 .. kb-1
-  for (int e = tid; e < NB * NB; e += 256) X[e / NB][e % NB] = make_double2((e / NB) == (e % NB) ? 1.0 : 0.0, 0.0);
-  __syncthreads();
-  for (int j = 0; j < kb; ++j) {
-    for (int e = tid; e < (kb - j - 1) * NB; e += 256) {
-      const int i = j + 1 + e / NB, c = e % NB;
-      const double2 v = T[i][j], x = X[j][c];
-      X[i][c].x -= v.x * x.x - v.y * x.y;
-      X[i][c].y -= v.x * x.y + v.y * x.x;
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < NB * NB; e += 256) Linv[e] = X[e / NB][e % NB];
-}
-
-// In-place products with a kb x kb triangle inverse Tm (row-major NB x NB):
-//   right: X = X Tm for X = rows [k0 + kb, lim) of columns [k0, k0 + kb)   (L21 = A21 U11^-1)
-//   left:  X = Tm X for X = rows [k0, k0 + kb) of columns [k0 + kb, lim)   (U12 = L11^-1 A12)
-// 64 rows (right) or 64 columns (left) per workgroup, the block and Tm staged in LDS. The right
-// product also checks that |l_ij u_jj| (|re| + |im|) does not exceed |u_jj|: the entry at step j
-// of the elimination below the pivot (else the panel's flag). gate: run only if the flag is set.
-__global__ void __launch_bounds__(256) tri_apply_kernel(double2* __restrict__ M, int lda, int lim, int k0, int kb,
-                                                        const double2* __restrict__ Tm, int right,
-                                                        const double2* __restrict__ udiag, int* __restrict__ flag,
-                                                        const int* __restrict__ gate) {
-  if (gate && *gate == 0) return;
-  __shared__ double2 Xs[NB][NB + 1];  // right: [row][k]; left: [k][col]
-  __shared__ double2 Ts[NB][NB + 1];
-  __shared__ int s_fail;
-  const int tid = threadIdx.x;
-  if (tid == 0) s_fail = 0;
-  const int base = k0 + kb + blockIdx.x * NB;  // first row (right) / column (left)
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int a = e / NB, b = e % NB;
-    Ts[a][b] = Tm[a * NB + b];
-    if (right) {
-      const int k = e / NB, i = e % NB;
-      Xs[i][k] = (base + i < lim && k < kb) ? M[(size_t)(k0 + k) * lda + base + i] : make_double2(0.0, 0.0);
-    } else {
-      const int c = e / NB, k = e % NB;
-      Xs[k][c] = (base + c < lim && k < kb) ? M[(size_t)(base + c) * lda + k0 + k] : make_double2(0.0, 0.0);
-    }
-  }
-  __syncthreads();
-  const int ti = tid & 15, tj = tid >> 4;
-  double2 acc[4][4];
+    bool fail = false;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+    for (int jb = 0; jb < 4; ++jb) {
+      for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * jb + jj, buf = jj & 1;
+        if (ti == jj) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = make_double2(0.0, 0.0);
-  for (int k = 0; k < NB; ++k) {
-    double2 a[4], b[4];
+          for (int b = 0; b < 4; ++b) rowb[buf][tj + 16 * b] = t[jb][b];
+        }
+        if (tj == jj) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = right ? Xs[ti + 16 * u][k] : Ts[ti + 16 * u][k];
+          for (int a = 0; a < 4; ++a) colb[buf][ti + 16 * a] = t[a][jb];
+        }
+        __syncthreads();
+        const double2 piv = rowb[buf][j];
+        const double pm = mag1(piv);
+        const double2 rp = crecip(piv);
+        if (tid == 0 && pm == 0.0 && j < kb) atomicMin(zp_fast, k0 + j + 1);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) b[v] = right ? Ts[k][tj + 16 * v] : Xs[k][tj + 16 * v];
+        for (int a = 0; a < 4; ++a) {
+          if (a < jb) continue;
+          if (a == jb && ti <= jj) continue;
+          const double2 v = colb[buf][ti + 16 * a];
+          if (tj == jj && mag1(v) > pm) fail = true;
+          const double2 l = cmul(v, rp);
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        acc[u][v].x = fma(a[u].x, b[v].x, fma(-a[u].y, b[v].y, acc[u][v].x));
-        acc[u][v].y = fma(a[u].x, b[v].y, fma(a[u].y, b[v].x, acc[u][v].y));
-      }
-  }
-  bool fail = false;
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = ti + 16 * u, c = tj + 16 * v;
-      if (right) {
-        if (base + r < lim && c < kb) {
-          M[(size_t)(k0 + c) * lda + base + r] = acc[u][v];
-          if (flag) {
-            const double2 d = udiag[c];
-            if (mag1(cmul(acc[u][v], d)) > mag1(d)) fail = true;
+          for (int b = 0; b < 4; ++b) {
+            if (b < jb) continue;
+            if (b == jb && tj < jj) continue;
+            if (b == jb && tj == jj) {
+              t[a][b] = l;
+            } else {
+              const double2 u = rowb[buf][tj + 16 * b];
+              t[a][b].x -= l.x * u.x - l.y * u.y;
+              t[a][b].y -= l.x * u.y + l.y * u.x;
+            }
           }
         }
-      } else {
-        if (base + c < lim && r < kb) M[(size_t)(base + c) * lda + k0 + r] = acc[u][v];
       }
     }
-  if (fail) s_fail = 1;
+    if (fail) s_fail = 1;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int r = ti + 16 * a, c = tj + 16 * b;
+        if (r < kb && c < kb) M[(size_t)(k0 + c) * lda + k0 + r] = t[a][b];
+      }
+    for (int i = tid; i < kb; i += 256) ipiv[k0 + i] = k0 + i;
+  }
+  if (ti == tj) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) dg[ti + 16 * a] = t[a][a];
+  }
   __syncthreads();
-  if (tid == 0 && s_fail && flag) *flag = 1;
+  if (mode == 0 && tid == 0 && s_fail) *flag = 1;
+  if (tid < NB) udiag[tid] = dg[tid];
+  // Linv: L X = I (unit lower), step j ascending: X[r][*] -= l_rj X[j][*], r > j.
+  // Uinv: U X = I, step j descending: X[j][*] *= 1 / u_jj, then X[r][*] -= u_rj X[j][*], r < j.
+  // Both in one loop, step s doing j = s for Linv and j = 63 - s for Uinv.
+  double2 xl[4][4], xu[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const double v = (a == b && ti == tj) ? 1.0 : 0.0;
+      xl[a][b] = make_double2(v, 0.0);
+      xu[a][b] = make_double2(v, 0.0);
+    }
+#pragma unroll
+  for (int sb = 0; sb < 4; ++sb) {
+    const int ub = 3 - sb;
+    for (int ss = 0; ss < 16; ++ss) {
+      const int buf = ss & 1, uj = 15 - ss;
+      const int ju = 16 * ub + uj;
+      if (ti == ss) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) rowb[buf][tj + 16 * b] = xl[sb][b];
+      }
+      if (tj == ss) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) colb[buf][ti + 16 * a] = t[a][sb];
+      }
+      if (ti == uj) {
+        const double2 rd = crecip(dg[ju]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          xu[ub][b] = cmul(xu[ub][b], rd);
+          rowu[buf][tj + 16 * b] = xu[ub][b];
+        }
+      }
+      if (tj == uj) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) colu[buf][ti + 16 * a] = t[a][ub];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        // Linv rows r > jl; row jl of X is zero right of jl
+        if (a > sb || (a == sb && ti > ss)) {
+          const double2 l = colb[buf][ti + 16 * a];
+#pragma unroll
+          for (int b = 0; b <= sb; ++b) {
+            const double2 x = rowb[buf][tj + 16 * b];
+            xl[a][b].x -= l.x * x.x - l.y * x.y;
+            xl[a][b].y -= l.x * x.y + l.y * x.x;
+          }
+        }
+        // Uinv rows r < ju; row ju of X is zero left of ju
+        if (a < ub || (a == ub && ti < uj)) {
+          const double2 u = colu[buf][ti + 16 * a];
+#pragma unroll
+          for (int b = ub; b < 4; ++b) {
+            const double2 x = rowu[buf][tj + 16 * b];
+            xu[a][b].x -= u.x * x.x - u.y * x.y;
+            xu[a][b].y -= u.x * x.y + u.y * x.x;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int r = ti + 16 * a, c = tj + 16 * b;
+      Uinv[r * NB + c] = xu[a][b];
+      Linv[r * NB + c] = xl[a][b];
+    }
+}
+
+// The panel's block products with the triangle inverses, in place:
+//   blocks [0, rblocks): L21 = A21 Uinv on 16 rows each (rows k0 + kb + 16 i .. < lim), checking
+//     that |l_ij u_jj| (|re| + |im|) does not exceed |u_jj|: the entry at step j of the
+//     elimination below the pivot (else the panel's flag);
+//   blocks [rblocks, gridDim.x): U12 = Linv A12 on 16 columns each (< ncols).
+// gate: run only if the flag is set.
+constexpr int AB = 16;
+__global__ void __launch_bounds__(256) apply_kernel(double2* __restrict__ M, int lda, int lim, int ncols, int k0,
+                                                    int kb, int rblocks, const double2* __restrict__ Uinv,
+                                                    const double2* __restrict__ Linv,
+                                                    const double2* __restrict__ udiag, int* __restrict__ flag,
+                                                    const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double2 Ts[NB][NB + 1];
+  __shared__ double2 Xs[NB][AB + 1];  // right: [k][row]; left: [k][col]
+  __shared__ int s_fail;
+  const int tid = threadIdx.x;
+  const bool right = (int)blockIdx.x < rblocks;
+  const double2* __restrict__ Tm = right ? Uinv : Linv;
+  for (int e = tid; e < NB * NB; e += 256) Ts[e >> 6][e & 63] = Tm[e];
+  if (right) {
+    const int base = k0 + kb + blockIdx.x * AB;
+    if (tid == 0) s_fail = 0;
+    for (int e = tid; e < NB * AB; e += 256) {
+      const int i = e & (AB - 1), k = e >> 4;
+      Xs[k][i] = (base + i < lim && k < kb) ? M[(size_t)(k0 + k) * lda + base + i] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int i = tid & 15, cq = tid >> 4;
+    double2 acc[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = make_double2(0.0, 0.0);
+    for (int k = 0; k < NB; ++k) {
+      const double2 x = Xs[k][i];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double2 u = Ts[k][cq + 16 * v];
+        acc[v].x = fma(x.x, u.x, fma(-x.y, u.y, acc[v].x));
+        acc[v].y = fma(x.x, u.y, fma(x.y, u.x, acc[v].y));
+      }
+    }
+    bool fail = false;
+    if (base + i < lim) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int c = cq + 16 * v;
+        if (c >= kb) continue;
+        M[(size_t)(k0 + c) * lda + base + i] = acc[v];
+        if (flag) {
+          const double2 d = udiag[c];
+          if (mag1(cmul(acc[v], d)) > mag1(d)) fail = true;
+        }
+      }
+    }
+    if (fail) s_fail = 1;
+    __syncthreads();
+    if (tid == 0 && s_fail && flag) *flag = 1;
+  } else {
+    const int base = k0 + kb + (blockIdx.x - rblocks) * AB;
+    for (int e = tid; e < NB * AB; e += 256) {
+      const int k = e & 63, c = e >> 6;
+      Xs[k][c] = (base + c < ncols && k < kb) ? M[(size_t)(base + c) * lda + k0 + k] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int r = tid & 63, cq = tid >> 6;
+    double2 acc[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = make_double2(0.0, 0.0);
+    for (int k = 0; k < NB; ++k) {
+      const double2 l = Ts[r][k];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double2 x = Xs[k][cq + 4 * v];
+        acc[v].x = fma(l.x, x.x, fma(-l.y, x.y, acc[v].x));
+        acc[v].y = fma(l.x, x.y, fma(l.y, x.x, acc[v].y));
+      }
+    }
+    if (r < kb) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int c = base + cq + 4 * v;
+        if (c < ncols) M[(size_t)c * lda + k0 + r] = acc[v];
+      }
+    }
+  }
 }
 
 // After the panel: the zero pivot found without interchanges counts when no fallback ran.
@@ -478,21 +580,24 @@ __global__ void __launch_bounds__(256) gemm_update_kernel(double2* __restrict__ 
 
 // Back substitution U X = Y on the augmented columns [n, n + r), block by block from the bottom:
 // the block of rows (k0, kb) is X_b = Uinv_b Y_b (the stored inverse of its diagonal block), then
-// the rows above are updated, Y[0:k0] -= U[0:k0, b] X_b.
+// the rows above are updated, Y[0:k0] -= U[0:k0, b] X_b (one output per thread).
 __global__ void __launch_bounds__(256) backsolve_block_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
                                                               const double2* __restrict__ Uinv, int nrhs) {
-  __shared__ double2 Y[NB][33];
+  __shared__ double2 U[NB][NB + 1];
+  __shared__ double2 Y[NB][MAXRHS_LU + 1];
   const int tid = threadIdx.x;
-  for (int e = tid; e < kb * nrhs; e += 256) {
-    const int r = e % kb, c = e / kb;
-    Y[r][c] = M[(size_t)(n + c) * lda + k0 + r];
+  for (int e = tid; e < NB * NB; e += 256) U[e >> 6][e & 63] = Uinv[e];
+  for (int e = tid; e < NB * nrhs; e += 256) {
+    const int r = e & (NB - 1), c = e >> 6;
+    Y[r][c] = r < kb ? M[(size_t)(n + c) * lda + k0 + r] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  for (int e = tid; e < kb * nrhs; e += 256) {
-    const int r = e % kb, c = e / kb;
+  for (int e = tid; e < NB * nrhs; e += 256) {
+    const int r = e & (NB - 1), c = e >> 6;
+    if (r >= kb) continue;
     double2 s = make_double2(0.0, 0.0);
     for (int k = r; k < kb; ++k) {
-      const double2 u = Uinv[r * NB + k], y = Y[k][c];
+      const double2 u = U[r][k], y = Y[k][c];
       s.x += u.x * y.x - u.y * y.y;
       s.y += u.x * y.y + u.y * y.x;
     }
@@ -500,38 +605,38 @@ __global__ void __launch_bounds__(256) backsolve_block_kernel(double2* __restric
   }
 }
 
-__global__ void __launch_bounds__(256) backsolve_update_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
-                                                               int nrhs) {
-  __shared__ double2 Xb[NB][33];
+// Y[0:k0] -= U[0:k0, k0:k0+kb] X_b: one row per thread, the right-hand sides in chunks of 8.
+__global__ void __launch_bounds__(64) backsolve_update_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
+                                                              int nrhs) {
+  __shared__ double2 Xb[NB][MAXRHS_LU];
   const int tid = threadIdx.x;
-  for (int e = tid; e < kb * nrhs; e += 256) {
-    const int r = e % kb, c = e / kb;
-    Xb[r][c] = M[(size_t)(n + c) * lda + k0 + r];
+  for (int e = tid; e < NB * nrhs; e += 64) {
+    const int c = e & (NB - 1), r = e >> 6;
+    Xb[c][r] = c < kb ? M[(size_t)(n + r) * lda + k0 + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const int row = blockIdx.x * blockDim.x + tid;
+  const int row = blockIdx.x * 64 + tid;
   if (row >= k0) return;
-  double2 acc[32];
+  for (int r0 = 0; r0 < nrhs; r0 += 8) {
+    double2 acc[8];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) acc[r] = make_double2(0.0, 0.0);
-  for (int c = 0; c < kb; ++c) {
-    const double2 u = M[(size_t)(k0 + c) * lda + row];
+    for (int q = 0; q < 8; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int c = 0; c < kb; ++c) {
+      const double2 u = M[(size_t)(k0 + c) * lda + row];
 #pragma unroll
-    for (int r = 0; r < 32; ++r) {
-      if (r < nrhs) {
-        const double2 x = Xb[c][r];
-        acc[r].x += u.x * x.x - u.y * x.y;
-        acc[r].y += u.x * x.y + u.y * x.x;
+      for (int q = 0; q < 8; ++q) {
+        const double2 x = Xb[c][min(r0 + q, MAXRHS_LU - 1)];
+        acc[q].x = fma(u.x, x.x, fma(-u.y, x.y, acc[q].x));
+        acc[q].y = fma(u.x, x.y, fma(u.y, x.x, acc[q].y));
       }
     }
-  }
 #pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    if (r < nrhs) {
-      double2* b = M + (size_t)(n + r) * lda + row;
+    for (int q = 0; q < 8; ++q) {
+      if (r0 + q >= nrhs) break;
+      double2* b = M + (size_t)(n + r0 + q) * lda + row;
       double2 v = *b;
-      v.x -= acc[r].x;
-      v.y -= acc[r].y;
+      v.x -= acc[q].x;
+      v.y -= acc[q].y;
       *b = v;
     }
   }
@@ -557,6 +662,8 @@ struct LuWorkspace {
   double2* linv = nullptr;
   double2* udiag = nullptr;
   double2* save = nullptr;   // the panel before the attempt without interchanges
+  double2* backup = nullptr; // [A | B] before the factorization without interchanges
+  size_t backup_elems = 0;
   void release() {
     hipFree(w.cand);
     hipFree(w.ccount);
@@ -571,6 +678,7 @@ struct LuWorkspace {
     hipFree(linv);
     hipFree(udiag);
     hipFree(save);
+    hipFree(backup);
     *this = LuWorkspace{};
   }
 };
@@ -629,12 +737,108 @@ EventPool& events() {
 }
 }  // namespace
 
+namespace {
+
+struct GemmTimer {
+  EventPool* ep;
+  LuStats* stats;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+// A22 -= L21 U12 for the panel (k0, kb), timed when stats are kept
+void trailing_update(double2* M, int n, int lda, int ncols, int k0, int kb, hipStream_t stream, GemmTimer& gt) {
+  const int m = n - k0;
+  if (m <= kb || ncols <= k0 + kb) return;
+  dim3 grid((m - kb + 63) / 64, (ncols - k0 - kb + 63) / 64);
+  hipEvent_t g0 = nullptr;
+  if (gt.stats) {
+    g0 = gt.ep->next();
+    HBG_HIP(hipEventRecord(g0, stream));
+  }
+  gemm_update_kernel<<<grid, 256, 0, stream>>>(M, n, lda, ncols, k0, kb);
+  if (gt.stats) {
+    hipEvent_t g1 = gt.ep->next();
+    HBG_HIP(hipEventRecord(g1, stream));
+    gt.ev.push_back({g0, g1});
+    gt.stats->gemm_flops += 8.0 * (double)(m - kb) * (double)(ncols - k0 - kb) * (double)kb;
+    gt.stats->gemm_launches += 1;
+  }
+}
+
+// Every panel without interchanges: the diagonal block, L21 = A21 U11^-1 (both checking that
+// partial pivoting would make no interchange; the panel's flag is set where it would),
+// U12 = L11^-1 A12 and the trailing update. Four launches per panel.
+void factor_optimistic(double2* M, int n, int lda, int ncols, hipStream_t stream, LuWorkspace& ws, GemmTimer& gt) {
+  for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
+    const int kb = min(NB, n - k0);
+    const int m = n - k0;
+    int* flag = ws.flags + p;
+    double2* uinv = ws.uinv + (size_t)p * NB * NB;
+    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
+    if (rb + cb > 0)
+      apply_kernel<<<rb + cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, flag, nullptr);
+    trailing_update(M, n, lda, ncols, k0, kb, stream, gt);
+  }
+}
+
+// Panel by panel: the attempt without interchanges, then, gated by the panel's flag, the saved
+// panel restored and pivoted by the tournament. The result of a panel without a flag is the one
+// partial pivoting gives.
+void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, LuWorkspace& ws, GemmTimer& gt) {
+  const size_t slots = (size_t)ws.w.levels * ws.w.max_nodes;
+  for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
+    const int kb = min(NB, n - k0);
+    const int m = n - k0;
+    int* flag = ws.flags + p;
+    double2* uinv = ws.uinv + (size_t)p * NB * NB;
+    const unsigned pe = (unsigned)(((size_t)m * kb + 255) / 256);
+    panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 0, flag);
+    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
+    if (rb > 0)
+      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, flag, nullptr);
+    panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 1, flag);
+    fill_u32<<<(unsigned)((slots + 255) / 256), 256, 0, stream>>>(ws.w.ctr, slots, 0u);
+    tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w, flag);
+    ipiv_kernel<<<1, 64, 0, stream>>>(k0, kb, ws.w.winners, ws.ipiv, flag);
+    swap_kernel<<<(ncols + 255) / 256, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.ipiv, ws.w.top, flag);
+    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    if (rb > 0)
+      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, nullptr, flag);
+    panel_finish_kernel<<<1, 64, 0, stream>>>(flag, ws.zp_fast + p, ws.w.zero_pivot);
+    if (cb > 0)
+      apply_kernel<<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, ws.linv, ws.udiag, nullptr, nullptr);
+    trailing_update(M, n, lda, ncols, k0, kb, stream, gt);
+  }
+}
+
+// U X = Y on the augmented columns, at most MAXRHS_LU of them at a time (the kernels take the
+// first column of the chunk as their `n`).
+void back_substitute(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuWorkspace& ws) {
+  for (int r0 = 0; r0 < nrhs; r0 += MAXRHS_LU) {
+    const int nr = min(MAXRHS_LU, nrhs - r0);
+    for (int k0 = ((n - 1) / NB) * NB, p = (n - 1) / NB; k0 >= 0; k0 -= NB, --p) {
+      const int kb = min(NB, n - k0);
+      backsolve_block_kernel<<<1, 256, 0, stream>>>(M, n + r0, lda, k0, kb, ws.uinv + (size_t)p * NB * NB, nr);
+      if (k0 > 0) backsolve_update_kernel<<<(k0 + 63) / 64, 64, 0, stream>>>(M, n + r0, lda, k0, kb, nr);
+    }
+  }
+}
+
+}  // namespace
+
 // Factor [A | B] (n x (n + nrhs), leading dimension lda) in place and solve; returns the first
 // zero pivot (1-based) or 0. Everything on `stream`; the call waits for the result.
+//
+// The whole matrix is first factored without interchanges (the partial-pivoting check on every
+// panel), after a copy of [A | B] is kept. When any panel's check fails, the copy is restored and
+// the factorization is redone with the pivoted panels; otherwise the result is the partially
+// pivoted one, with four launches per panel.
 int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats) {
   EventPool& ep = events();
   ep.used = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_ev;
+  GemmTimer gt{&ep, stats, {}};
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (stats) {
     t0 = ep.next();
@@ -644,79 +848,56 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
   lu_workspace(ws, n);
   const int ncols = n + nrhs;
   const int panels = (n + NB - 1) / NB;
-  const size_t slots = (size_t)ws.w.levels * ws.w.max_nodes;
-  fill_i32<<<1, 64, 0, stream>>>(ws.w.zero_pivot, 1, 0x7fffffff);
+  const size_t whole = (size_t)lda * ncols;
+  if (ws.backup_elems < whole) {
+    hipFree(ws.backup);
+    ws.backup = nullptr;
+    ws.backup_elems = 0;
+    HBG_HIP(hipMalloc(&ws.backup, whole * sizeof(double2)));
+    ws.backup_elems = whole;
+  }
+  HBG_HIP(hipMemcpyAsync(ws.backup, M, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
   fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
   fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
-  for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
-    const int kb = min(NB, n - k0);
-    const int m = n - k0;
-    int* flag = ws.flags + p;
-    double2* uinv = ws.uinv + (size_t)p * NB * NB;
-    const unsigned pe = (unsigned)(((size_t)m * kb + 255) / 256);
-    // attempt without interchanges, with the partial-pivoting check
-    panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 0, flag);
-    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
-    if (m > kb)
-      tri_apply_kernel<<<(m - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, n, k0, kb, uinv, 1, ws.udiag, flag,
-                                                                   nullptr);
-    // the fallback, gated by the panel's flag
-    panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 1, flag);
-    fill_u32<<<(unsigned)((slots + 255) / 256), 256, 0, stream>>>(ws.w.ctr, slots, 0u);
-    tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w, flag);
-    ipiv_kernel<<<1, 64, 0, stream>>>(k0, kb, ws.w.winners, ws.ipiv, flag);
-    swap_kernel<<<(ncols + 255) / 256, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.ipiv, ws.w.top, flag);
-    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
-    if (m > kb)
-      tri_apply_kernel<<<(m - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, n, k0, kb, uinv, 1, ws.udiag, nullptr,
-                                                                   flag);
-    panel_finish_kernel<<<1, 64, 0, stream>>>(flag, ws.zp_fast + p, ws.w.zero_pivot);
-    if (ncols > k0 + kb) {
-      tri_apply_kernel<<<(ncols - k0 - kb + NB - 1) / NB, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.linv, 0,
-                                                                          nullptr, nullptr, nullptr);
-      if (m > kb) {
-        dim3 grid((m - kb + 63) / 64, (ncols - k0 - kb + 63) / 64);
-        hipEvent_t g0 = nullptr, g1 = nullptr;
-        if (stats) {
-          g0 = ep.next();
-          HBG_HIP(hipEventRecord(g0, stream));
-        }
-        gemm_update_kernel<<<grid, 256, 0, stream>>>(M, n, lda, ncols, k0, kb);
-        if (stats) {
-          g1 = ep.next();
-          HBG_HIP(hipEventRecord(g1, stream));
-          gemm_ev.push_back({g0, g1});
-          stats->gemm_flops += 8.0 * (double)(m - kb) * (double)(ncols - k0 - kb) * (double)kb;
-          stats->gemm_launches += 1;
-        }
-      }
-    }
-  }
+  factor_optimistic(M, n, lda, ncols, stream, ws, gt);
   HBG_HIP(hipGetLastError());
-  for (int k0 = ((n - 1) / NB) * NB, p = (n - 1) / NB; k0 >= 0 && nrhs > 0; k0 -= NB, --p) {
-    const int kb = min(NB, n - k0);
-    backsolve_block_kernel<<<1, 256, 0, stream>>>(M, n, lda, k0, kb, ws.uinv + (size_t)p * NB * NB, nrhs);
-    if (k0 > 0) backsolve_update_kernel<<<(k0 + 255) / 256, 256, 0, stream>>>(M, n, lda, k0, kb, nrhs);
+  std::vector<int> flags(panels), zps(panels);
+  HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HBG_HIP(hipMemcpyAsync(zps.data(), ws.zp_fast, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HBG_HIP(hipStreamSynchronize(stream));
+  bool any = false;
+  int zp = 0x7fffffff;
+  for (int p = 0; p < panels; ++p) {
+    any = any || flags[p] != 0;
+    zp = std::min(zp, zps[p]);
   }
+  if (any) {
+    HBG_HIP(hipMemcpyAsync(M, ws.backup, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+    fill_i32<<<1, 64, 0, stream>>>(ws.w.zero_pivot, 1, 0x7fffffff);
+    fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
+    fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
+    factor_pivoted(M, n, lda, ncols, stream, ws, gt);
+    HBG_HIP(hipGetLastError());
+    HBG_HIP(hipMemcpyAsync(&zp, ws.w.zero_pivot, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
+  }
+  back_substitute(M, n, lda, nrhs, stream, ws);
   HBG_HIP(hipGetLastError());
   if (stats) {
     t1 = ep.next();
     HBG_HIP(hipEventRecord(t1, stream));
   }
-  int zp = 0;
-  std::vector<int> flags(stats ? panels : 0);
-  HBG_HIP(hipMemcpyAsync(&zp, ws.w.zero_pivot, sizeof(int), hipMemcpyDeviceToHost, stream));
-  if (stats) HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
   HBG_HIP(hipStreamSynchronize(stream));
   if (stats) {
     float ms = 0.0f;
     HBG_HIP(hipEventElapsedTime(&ms, t0, t1));
     stats->total_ms += ms;
-    for (auto& g : gemm_ev) {
+    for (auto& g : gt.ev) {
       HBG_HIP(hipEventElapsedTime(&ms, g.first, g.second));
       stats->gemm_ms += ms;
     }
-    for (int f : flags) stats->fallback_panels += f != 0;
+    if (any)
+      for (int f : flags) stats->fallback_panels += f != 0;
   }
   return zp == 0x7fffffff ? 0 : zp;
 }

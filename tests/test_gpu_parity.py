@@ -109,6 +109,24 @@ def test_lu_known_answer_and_random(session, n):
     assert np.abs(xs - ref).max() < 1e-8 * np.abs(ref).max()
 
 
+def test_lu_many_rhs_and_late_pivoting(session):
+    # diagonally dominant except rows 200..259 (swapped in pairs), so only the fourth panel needs
+    # interchanges; 45 right-hand sides (more than one chunk of the back substitution)
+    say("case: lu n=260, 45 rhs, interchanges in the fourth panel only")
+    n, r = 260, 45
+    rng = np.random.default_rng(7)
+    a = 0.05 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    a[np.arange(n), np.arange(n)] += 4.0 + 1.0j
+    perm = np.arange(n)
+    perm[200:260] = np.arange(200, 260).reshape(-1, 2)[:, ::-1].reshape(-1)
+    a = a[perm]
+    x = rng.normal(size=(n, r)) + 1j * rng.normal(size=(n, r))
+    sol = session.lu_solve(a, a @ x)
+    err = np.abs(sol - x).max() / np.abs(x).max()
+    say(f"  forward error {err:.2e}")
+    assert err < 1e-12
+
+
 def test_lu_singular_reports_zero_pivot(session):
     say("case: lu singular")
     a = np.zeros((3, 3), complex)
