@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cylinder4096")
+    ap.add_argument("--batch", type=int, default=0, help="frequencies in flight (0: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -98,6 +99,7 @@ def main():
     N, F = v.shape[0], len(omegas)
     s = hb.Session(0)
     s.set_environment(w["rho"], w["g"], w.get("depth", float("inf")))
+    s.set_batch(a.batch)
 
     def step():
         s.set_panels(v)

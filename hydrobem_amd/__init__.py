@@ -65,6 +65,7 @@ SIGNATURES = {
     "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "hbg_set_profiling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "hbg_set_batch": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "hbg_profile_read": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
     "hbg_write_wamit": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
@@ -163,6 +164,10 @@ class Session:
     PROFILE_KEYS = ("statics_ms", "assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "lu_gemm_flops",
                     "lu_gemm_launches", "forces_ms", "frequencies", "lu_flops", "assembly_bytes", "statics_runs",
                     "lu_fallback_panels")
+
+    def set_batch(self, max_batch: int):
+        """Frequencies in flight in solve (0: as many as fit, at most 16)."""
+        _check(library().hbg_set_batch(self._h, int(max_batch)))
 
     def set_profiling(self, enable: bool):
         _check(library().hbg_set_profiling(self._h, int(bool(enable))))

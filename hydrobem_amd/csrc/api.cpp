@@ -42,6 +42,9 @@ void launch_forces(const PanelD* P, const int* body, const double* nj, int N, in
                    const double2* M, int ldm, const double2* phiI, const double2* dphiI, double* out,
                    hipStream_t stream);
 int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
+void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
+                      LuStats* stats, int* status);
+int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
                 const double* X, const char* stamp, std::string& err);
@@ -258,6 +261,8 @@ struct HbgSession {
   DevBuf<int2> d_queue;
   DevBuf<unsigned> d_qcount;
   DevBuf<double2> d_S, d_M, d_phiI, d_dphiI, d_u, d_part;
+  int max_batch = 0;  // frequencies in flight (0: as many as fit, at most 16)
+  std::vector<hipEvent_t> bev;
   bool statics_valid = false;
   int st_logint = -1;
   double st_tol = -1.0;
@@ -352,10 +357,10 @@ FinTables fin_tables(HbgSession* s) {
 
 // S and A of the frequency K into S (ld N) and M (ld N): infinite depth, or finite depth with the
 // per-frequency tables built first; returns the wave number of the incident wave
-double assemble_frequency(HbgSession* s, double K, double logc) {
+double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout) {
   const int N = s->N;
   if (!std::isfinite(s->depth)) {
-    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, s->d_M.p, N, s->stream);
+    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, Mout, N, s->stream);
     return K;
   }
   const FinFreq f = make_fin_freq(s->depth, K);
@@ -367,9 +372,22 @@ double assemble_frequency(HbgSession* s, double K, double logc) {
                     s->d_pcs.p, s->stream);
   s->d_ff.ensure(1);
   HBG_HIP(hipMemcpyAsync(s->d_ff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, s->stream));
-  launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, s->d_M.p, N,
+  launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, N,
                       s->stream);
   return f.k0;
+}
+
+// Frequencies in flight: the session's setting, else as many as a quarter of the free device
+// memory holds (each needs its N x (N + R) system), at most 16, at most the number of frequencies.
+int batch_size(HbgSession* s, int N, int R, int F) {
+  const size_t per = (size_t)N * (N + R) * sizeof(double2) + (size_t)N * 64 * sizeof(double2) * 2;
+  int B = s->max_batch;
+  if (B <= 0) {
+    size_t free_b = 0, total_b = 0;
+    HBG_HIP(hipMemGetInfo(&free_b, &total_b));
+    B = (int)std::min<size_t>(16, std::max<size_t>(1, free_b / 4 / per));
+  }
+  return std::max(1, std::min(B, std::max(F, 1)));
 }
 
 void check_options(const HbgOptions* o) {
@@ -643,7 +661,7 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   s->d_M.ensure((size_t)N * N);
   const double K = omega * omega / s->grav;
   check_depth(s);
-  assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0);
+  assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0, s->d_M.p);
   std::vector<double2> hs((size_t)N * N), ha((size_t)N * N);
   HBG_HIP(hipMemcpyAsync(hs.data(), s->d_S.p, hs.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
   HBG_HIP(hipMemcpyAsync(ha.data(), s->d_M.p, ha.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
@@ -707,78 +725,126 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   check_depth(s);
   if (progress && progress(user, HBG_STAGE_STATICS, 0, 1) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
   ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
-  s->d_S.ensure((size_t)N * N);
-  s->d_M.ensure((size_t)N * (N + R));
-  s->d_phiI.ensure((size_t)std::max(H, 1) * N);
-  s->d_dphiI.ensure((size_t)std::max(H, 1) * N);
-  s->d_u.ensure((size_t)R * N);
-  s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
   std::vector<double> heads(std::max(H, 1), 0.0);
   for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
   s->d_heads.ensure(heads.size());
   HBG_HIP(hipMemcpy(s->d_heads.p, heads.data(), heads.size() * sizeof(double), hipMemcpyHostToDevice));
   const size_t nAB = (size_t)D * nrad, nX = (size_t)H * D, nHK = (size_t)nrad * H;
   const size_t nout = 2 * nAB + 6 * nX + 2 * nHK;
-  s->d_out.ensure(nout);
-  std::vector<double> hout(nout);
-  for (int fi = 0; fi < n_omegas; ++fi) {
-    if (progress && progress(user, HBG_STAGE_ASSEMBLY, fi, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
+  // frequencies in flight: each holds its [A | B] (N x (N + R)) and incident-wave values until
+  // its forces are taken; S, the right-hand-side partials and the finite-depth tables are shared
+  const size_t mstride = (size_t)N * (N + R);
+  const size_t hstride = (size_t)std::max(H, 1) * N;
+  int B = batch_size(s, N, R, n_omegas);
+  s->d_S.ensure((size_t)N * N);
+  s->d_M.ensure(mstride * B);
+  s->d_phiI.ensure(hstride * B);
+  s->d_dphiI.ensure(hstride * B);
+  s->d_u.ensure((size_t)R * N);
+  s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
+  s->d_out.ensure(nout * B);
+  std::vector<double> hout(nout * B);
+  const bool prof = s->profiling;
+  if (prof && s->bev.size() < (size_t)(3 * B + 3)) {
+    for (size_t i = s->bev.size(); i < (size_t)(3 * B + 3); ++i) {
+      hipEvent_t e;
+      HBG_HIP(hipEventCreate(&e));
+      s->bev.push_back(e);
+    }
+  }
+  const double logc = o.logarithmic ? 1.0 : 0.0;
+  const double depth = std::isfinite(s->depth) ? s->depth : 0.0;
+  // assembly, incident wave and S U of frequency fi into slot b
+  auto build = [&](int fi, int b) {
     const double w = omegas[fi], K = w * w / s->grav;
-    const bool prof = s->profiling;
-    if (prof) HBG_HIP(hipEventRecord(s->ev[2], s->stream));
-    const double kw = assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0);
-    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, kw, s->grav,
-                    std::isfinite(s->depth) ? s->depth : 0.0, s->d_heads.p, s->d_phiI.p, s->d_dphiI.p, s->d_u.p,
-                    s->stream);
-    if (prof) HBG_HIP(hipEventRecord(s->ev[3], s->stream));
-    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->d_part.p, s->stream);
-    if (prof) HBG_HIP(hipEventRecord(s->ev[4], s->stream));
-    if (progress && progress(user, HBG_STAGE_LU, fi, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
+    double2* Mb = s->d_M.p + mstride * b;
+    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], s->stream));
+    const double kw = assemble_frequency(s, K, logc, Mb);
+    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
+                    s->d_phiI.p + hstride * b, s->d_dphiI.p + hstride * b, s->d_u.p, s->stream);
+    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], s->stream));
+    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, s->stream);
+    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 2], s->stream));
+  };
+  auto stage_ms = [&](hipEvent_t a0, hipEvent_t a1) {
+    float ms = 0.0f;
+    HBG_HIP(hipEventSynchronize(a1));
+    HBG_HIP(hipEventElapsedTime(&ms, a0, a1));
+    return (double)ms;
+  };
+  std::vector<int> status(B);
+  for (int f0 = 0; f0 < n_omegas; f0 += B) {
+    const int nb = std::min(B, n_omegas - f0);
+    for (int b = 0; b < nb; ++b) {
+      if (progress && progress(user, HBG_STAGE_ASSEMBLY, f0 + b, n_omegas) == 0)
+        throw Error{HBG_ERR_CANCELLED, "cancelled"};
+      build(f0 + b, b);
+    }
+    if (prof)
+      for (int b = 0; b < nb; ++b) {
+        s->prof[1] += stage_ms(s->bev[3 * b], s->bev[3 * b + 1]);
+        s->prof[2] += stage_ms(s->bev[3 * b + 1], s->bev[3 * b + 2]);
+      }
+    for (int b = 0; b < nb; ++b)
+      if (progress && progress(user, HBG_STAGE_LU, f0 + b, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     LuStats lus;
-    const int zp = lu_solve_device(s->d_M.p, N, N, R, s->stream, prof ? &lus : nullptr);
-    if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(zp) + ")"};
-    if (prof) HBG_HIP(hipEventRecord(s->ev[5], s->stream));
-    launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, s->rho, s->d_M.p, N, s->d_phiI.p,
-                  s->d_dphiI.p, s->d_out.p, s->stream);
-    if (prof) HBG_HIP(hipEventRecord(s->ev[6], s->stream));
-    HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p, nout * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    lu_solve_batched(s->d_M.p, mstride, nb, N, N, R, s->stream, prof ? &lus : nullptr, status.data());
+    for (int b = 0; b < nb; ++b) {
+      if (status[b] < 0) {  // a panel needs interchanges: rebuild the system, pivoted path
+        build(f0 + b, b);
+        status[b] = lu_solve_pivoted(s->d_M.p + mstride * b, N, N, R, s->stream, prof ? &lus : nullptr);
+      }
+      if (status[b] > 0)
+        throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(status[b]) + ")"};
+    }
+    const int e0 = 3 * B, e1 = 3 * B + 1;
+    if (prof) HBG_HIP(hipEventRecord(s->bev[e0], s->stream));
+    for (int b = 0; b < nb; ++b)
+      launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, omegas[f0 + b], s->rho, s->d_M.p + mstride * b,
+                    N, s->d_phiI.p + hstride * b, s->d_dphiI.p + hstride * b, s->d_out.p + nout * b, s->stream);
+    if (prof) HBG_HIP(hipEventRecord(s->bev[e1], s->stream));
+    HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p, nb * nout * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
     if (prof) {
-      float ms = 0.0f;
-      HBG_HIP(hipEventElapsedTime(&ms, s->ev[2], s->ev[3]));
-      s->prof[1] += ms;
-      HBG_HIP(hipEventElapsedTime(&ms, s->ev[3], s->ev[4]));
-      s->prof[2] += ms;
       s->prof[3] += lus.total_ms;
       s->prof[4] += lus.gemm_ms;
       s->prof[5] += lus.gemm_flops;
       s->prof[6] += (double)lus.gemm_launches;
-      HBG_HIP(hipEventElapsedTime(&ms, s->ev[5], s->ev[6]));
-      s->prof[7] += ms;
-      s->prof[8] += 1.0;
-      s->prof[9] += 8.0 / 3.0 * (double)N * N * N;
-      s->prof[10] += (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
+      s->prof[7] += stage_ms(s->bev[e0], s->bev[e1]);
+      s->prof[8] += nb;
+      s->prof[9] += nb * 8.0 / 3.0 * (double)N * N * N;
+      s->prof[10] += nb * (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
       s->prof[12] += (double)lus.fallback_panels;
     }
-    for (double v : hout)
-      if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
-    const double* hA = hout.data();
-    const double* hB = hA + nAB;
-    const double* hX = hB + nAB;
-    const double* hFK = hX + 2 * nX;
-    const double* hSC = hFK + 2 * nX;
-    const double* hHK = hSC + 2 * nX;
-    if (added_mass) std::memcpy(added_mass + fi * nAB, hA, nAB * sizeof(double));
-    if (damping) std::memcpy(damping + fi * nAB, hB, nAB * sizeof(double));
-    if (excitation) std::memcpy(excitation + fi * 2 * nX, hX, 2 * nX * sizeof(double));
-    if (froude_krylov) std::memcpy(froude_krylov + fi * 2 * nX, hFK, 2 * nX * sizeof(double));
-    if (scattering) std::memcpy(scattering + fi * 2 * nX, hSC, 2 * nX * sizeof(double));
-    if (haskind) std::memcpy(haskind + fi * 2 * nHK, hHK, 2 * nHK * sizeof(double));
+    for (size_t i = 0; i < nb * nout; ++i)
+      if (!std::isfinite(hout[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
+    for (int b = 0; b < nb; ++b) {
+      const int fi = f0 + b;
+      const double* hA = hout.data() + nout * b;
+      const double* hB = hA + nAB;
+      const double* hX = hB + nAB;
+      const double* hFK = hX + 2 * nX;
+      const double* hSC = hFK + 2 * nX;
+      const double* hHK = hSC + 2 * nX;
+      if (added_mass) std::memcpy(added_mass + fi * nAB, hA, nAB * sizeof(double));
+      if (damping) std::memcpy(damping + fi * nAB, hB, nAB * sizeof(double));
+      if (excitation) std::memcpy(excitation + fi * 2 * nX, hX, 2 * nX * sizeof(double));
+      if (froude_krylov) std::memcpy(froude_krylov + fi * 2 * nX, hFK, 2 * nX * sizeof(double));
+      if (scattering) std::memcpy(scattering + fi * 2 * nX, hSC, 2 * nX * sizeof(double));
+      if (haskind) std::memcpy(haskind + fi * 2 * nHK, hHK, 2 * nHK * sizeof(double));
+    }
   }
   if (progress && progress(user, HBG_STAGE_OUTPUTS, n_omegas, n_omegas) == 0)
     throw Error{HBG_ERR_CANCELLED, "cancelled"};
   return HBG_OK;
   GUARD_END("hbg_solve")
+}
+
+HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_batch");
+  if (max_batch < 0 || max_batch > 64) return fail(HBG_ERR_VALUE, "max_batch must be in 0..64", "hbg_set_batch");
+  s->max_batch = max_batch;
+  return HBG_OK;
 }
 
 HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable) {

@@ -43,6 +43,14 @@ struct TsluWork {
   int max_nodes;
 };
 
+// Strides between the matrices of a batch (blockIdx.z) and their workspaces, in elements.
+struct BatchStrides {
+  size_t m;     // [A | B]
+  int flag;     // per-panel flags and zero pivots
+  int ipiv;
+  size_t uinv;  // per-panel U11^-1
+};
+
 namespace {
 
 __device__ __forceinline__ double mag1(double2 v) { return fabs(v.x) + fabs(v.y); }
@@ -276,7 +284,17 @@ __device__ __forceinline__ double2 crecip(double2 v) {
 __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int lda, int k0, int kb, int mode,
                                                    int* __restrict__ flag, int* __restrict__ zp_fast,
                                                    int* __restrict__ ipiv, double2* __restrict__ Uinv,
-                                                   double2* __restrict__ Linv, double2* __restrict__ udiag) {
+                                                   double2* __restrict__ Linv, double2* __restrict__ udiag, BatchStrides bs) {
+  {
+    const int z = blockIdx.z;
+    M += z * bs.m;
+    flag += z * bs.flag;
+    zp_fast += z * bs.flag;
+    ipiv += z * bs.ipiv;
+    Uinv += z * bs.uinv;
+    Linv += z * (NB * NB);
+    udiag += z * NB;
+  }
   if (mode == 1 && *flag == 0) return;
   __shared__ double2 rowb[2][NB], colb[2][NB];
   __shared__ double2 rowu[2][NB], colu[2][NB];
@@ -435,7 +453,15 @@ __global__ void __launch_bounds__(256) apply_kernel(double2* __restrict__ M, int
                                                     int kb, int rblocks, const double2* __restrict__ Uinv,
                                                     const double2* __restrict__ Linv,
                                                     const double2* __restrict__ udiag, int* __restrict__ flag,
-                                                    const int* __restrict__ gate) {
+                                                    const int* __restrict__ gate, BatchStrides bs) {
+  {
+    const int z = blockIdx.z;
+    M += z * bs.m;
+    Uinv += z * bs.uinv;
+    Linv += z * (NB * NB);
+    udiag += z * NB;
+    if (flag) flag += z * bs.flag;
+  }
   if (gate && *gate == 0) return;
   __shared__ double2 Ts[NB][NB + 1];
   __shared__ double2 Xs[NB][AB + 1];  // right: [k][row]; left: [k][col]
@@ -521,7 +547,8 @@ __global__ void panel_finish_kernel(const int* __restrict__ flag, const int* __r
 // the block column L21 (column k0..), B the block row U12 (rows k0..). 64 x 64 tiles of C per
 // workgroup of 256 threads, 4 x 4 complex outputs per thread, operands staged in LDS.
 __global__ void __launch_bounds__(256) gemm_update_kernel(double2* __restrict__ M, int n, int lda, int ncols, int k0,
-                                                          int kb) {
+                                                          int kb, BatchStrides bs) {
+  M += blockIdx.z * bs.m;
   __shared__ double2 As[16][64 + 1];
   __shared__ double2 Bs[16][64 + 1];
   const int r0 = k0 + kb, c0 = k0 + kb;
@@ -582,7 +609,9 @@ __global__ void __launch_bounds__(256) gemm_update_kernel(double2* __restrict__ 
 // the block of rows (k0, kb) is X_b = Uinv_b Y_b (the stored inverse of its diagonal block), then
 // the rows above are updated, Y[0:k0] -= U[0:k0, b] X_b (one output per thread).
 __global__ void __launch_bounds__(256) backsolve_block_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
-                                                              const double2* __restrict__ Uinv, int nrhs) {
+                                                              const double2* __restrict__ Uinv, int nrhs, BatchStrides bs) {
+  M += blockIdx.z * bs.m;
+  Uinv += blockIdx.z * bs.uinv;
   __shared__ double2 U[NB][NB + 1];
   __shared__ double2 Y[NB][MAXRHS_LU + 1];
   const int tid = threadIdx.x;
@@ -607,7 +636,8 @@ __global__ void __launch_bounds__(256) backsolve_block_kernel(double2* __restric
 
 // Y[0:k0] -= U[0:k0, k0:k0+kb] X_b: one row per thread, the right-hand sides in chunks of 8.
 __global__ void __launch_bounds__(64) backsolve_update_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
-                                                              int nrhs) {
+                                                              int nrhs, BatchStrides bs) {
+  M += blockIdx.z * bs.m;
   __shared__ double2 Xb[NB][MAXRHS_LU];
   const int tid = threadIdx.x;
   for (int e = tid; e < NB * nrhs; e += 64) {
@@ -653,16 +683,16 @@ __global__ void fill_i32(int* p, size_t n, int v) {
 
 // ------------------------------------------------------------------ host driver
 struct LuWorkspace {
-  int n = 0;
+  int n = 0, batch = 0;
   TsluWork w{};
-  int* ipiv = nullptr;
-  int* flags = nullptr;      // per panel: 1 if the panel needed interchanges
-  int* zp_fast = nullptr;    // per panel: zero pivot found without interchanges
-  double2* uinv = nullptr;   // per panel: U11^-1 (NB x NB), for the back substitution
-  double2* linv = nullptr;
-  double2* udiag = nullptr;
-  double2* save = nullptr;   // the panel before the attempt without interchanges
-  double2* backup = nullptr; // [A | B] before the factorization without interchanges
+  int* ipiv = nullptr;       // [batch][n]
+  int* flags = nullptr;      // [batch][panels]: 1 if the panel needed interchanges
+  int* zp_fast = nullptr;    // [batch][panels]: zero pivot found without interchanges
+  double2* uinv = nullptr;   // [batch][panels]: U11^-1 (NB x NB), for the back substitution
+  double2* linv = nullptr;   // [batch]
+  double2* udiag = nullptr;  // [batch]
+  double2* save = nullptr;   // the panel before the attempt without interchanges (pivoted path)
+  double2* backup = nullptr; // [A | B] of a single solve, restored for the pivoted path
   size_t backup_elems = 0;
   void release() {
     hipFree(w.cand);
@@ -681,10 +711,16 @@ struct LuWorkspace {
     hipFree(backup);
     *this = LuWorkspace{};
   }
+  BatchStrides strides() const {
+    const int panels = (n + NB - 1) / NB;
+    return BatchStrides{0, panels, n, (size_t)panels * NB * NB};
+  }
 };
 
-static void lu_workspace(LuWorkspace& ws, int n) {
-  if (ws.n >= n && ws.ipiv) return;
+static void lu_workspace(LuWorkspace& ws, int n, int batch) {
+  if (ws.n >= n && ws.batch >= batch && ws.ipiv) return;
+  n = std::max(n, ws.n);
+  batch = std::max(batch, std::max(ws.batch, 1));
   ws.release();
   int leaves = (n + LEAF - 1) / LEAF;
   int levels = 1, nodes = leaves;
@@ -702,21 +738,21 @@ static void lu_workspace(LuWorkspace& ws, int n) {
   HBG_HIP(hipMalloc(&ws.w.top, (size_t)NB * NB * sizeof(double2)));
   HBG_HIP(hipMalloc(&ws.w.winners, NB * sizeof(int)));
   HBG_HIP(hipMalloc(&ws.w.zero_pivot, sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.ipiv, (size_t)n * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.flags, (size_t)panels * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.zp_fast, (size_t)panels * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.uinv, (size_t)panels * NB * NB * sizeof(double2)));
-  HBG_HIP(hipMalloc(&ws.linv, (size_t)NB * NB * sizeof(double2)));
-  HBG_HIP(hipMalloc(&ws.udiag, (size_t)NB * sizeof(double2)));
+  HBG_HIP(hipMalloc(&ws.ipiv, (size_t)batch * n * sizeof(int)));
+  HBG_HIP(hipMalloc(&ws.flags, (size_t)batch * panels * sizeof(int)));
+  HBG_HIP(hipMalloc(&ws.zp_fast, (size_t)batch * panels * sizeof(int)));
+  HBG_HIP(hipMalloc(&ws.uinv, (size_t)batch * panels * NB * NB * sizeof(double2)));
+  HBG_HIP(hipMalloc(&ws.linv, (size_t)batch * NB * NB * sizeof(double2)));
+  HBG_HIP(hipMalloc(&ws.udiag, (size_t)batch * NB * sizeof(double2)));
   HBG_HIP(hipMalloc(&ws.save, (size_t)n * NB * sizeof(double2)));
   ws.n = n;
+  ws.batch = batch;
 }
 
 static LuWorkspace& workspace() {
   static thread_local LuWorkspace ws;
   return ws;
 }
-
 
 namespace {
 struct EventPool {
@@ -735,9 +771,6 @@ EventPool& events() {
   static thread_local EventPool p;
   return p;
 }
-}  // namespace
-
-namespace {
 
 struct GemmTimer {
   EventPool* ep;
@@ -745,48 +778,52 @@ struct GemmTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
-// A22 -= L21 U12 for the panel (k0, kb), timed when stats are kept
-void trailing_update(double2* M, int n, int lda, int ncols, int k0, int kb, hipStream_t stream, GemmTimer& gt) {
+// A22 -= L21 U12 for the panel (k0, kb) of `batch` matrices, timed when stats are kept
+void trailing_update(double2* M, int n, int lda, int ncols, int k0, int kb, int batch, BatchStrides bs,
+                     hipStream_t stream, GemmTimer& gt) {
   const int m = n - k0;
   if (m <= kb || ncols <= k0 + kb) return;
-  dim3 grid((m - kb + 63) / 64, (ncols - k0 - kb + 63) / 64);
+  dim3 grid((m - kb + 63) / 64, (ncols - k0 - kb + 63) / 64, batch);
   hipEvent_t g0 = nullptr;
   if (gt.stats) {
     g0 = gt.ep->next();
     HBG_HIP(hipEventRecord(g0, stream));
   }
-  gemm_update_kernel<<<grid, 256, 0, stream>>>(M, n, lda, ncols, k0, kb);
+  gemm_update_kernel<<<grid, 256, 0, stream>>>(M, n, lda, ncols, k0, kb, bs);
   if (gt.stats) {
     hipEvent_t g1 = gt.ep->next();
     HBG_HIP(hipEventRecord(g1, stream));
     gt.ev.push_back({g0, g1});
-    gt.stats->gemm_flops += 8.0 * (double)(m - kb) * (double)(ncols - k0 - kb) * (double)kb;
+    gt.stats->gemm_flops += 8.0 * batch * (double)(m - kb) * (double)(ncols - k0 - kb) * (double)kb;
     gt.stats->gemm_launches += 1;
   }
 }
 
-// Every panel without interchanges: the diagonal block, L21 = A21 U11^-1 (both checking that
-// partial pivoting would make no interchange; the panel's flag is set where it would),
-// U12 = L11^-1 A12 and the trailing update. Four launches per panel.
-void factor_optimistic(double2* M, int n, int lda, int ncols, hipStream_t stream, LuWorkspace& ws, GemmTimer& gt) {
+// Every panel of every matrix of the batch without interchanges: the diagonal block,
+// L21 = A21 U11^-1 (both checking that partial pivoting would make no interchange; the panel's
+// flag is set where it would), U12 = L11^-1 A12 and the trailing update. Three launches per panel.
+void factor_optimistic(double2* M, int n, int lda, int ncols, int batch, BatchStrides bs, hipStream_t stream,
+                       LuWorkspace& ws, GemmTimer& gt) {
   for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
     const int kb = min(NB, n - k0);
     const int m = n - k0;
-    int* flag = ws.flags + p;
     double2* uinv = ws.uinv + (size_t)p * NB * NB;
-    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    diag_kernel<<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
+                                                       uinv, ws.linv, ws.udiag, bs);
     const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
     if (rb + cb > 0)
-      apply_kernel<<<rb + cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, flag, nullptr);
-    trailing_update(M, n, lda, ncols, k0, kb, stream, gt);
+      apply_kernel<<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv,
+                                                                ws.udiag, ws.flags + p, nullptr, bs);
+    trailing_update(M, n, lda, ncols, k0, kb, batch, bs, stream, gt);
   }
 }
 
-// Panel by panel: the attempt without interchanges, then, gated by the panel's flag, the saved
-// panel restored and pivoted by the tournament. The result of a panel without a flag is the one
-// partial pivoting gives.
+// One matrix, panel by panel: the attempt without interchanges, then, gated by the panel's flag,
+// the saved panel restored and pivoted by the tournament. The result of a panel without a flag is
+// the one partial pivoting gives.
 void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, LuWorkspace& ws, GemmTimer& gt) {
   const size_t slots = (size_t)ws.w.levels * ws.w.max_nodes;
+  const BatchStrides bs = ws.strides();
   for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
     const int kb = min(NB, n - k0);
     const int m = n - k0;
@@ -794,61 +831,140 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
     double2* uinv = ws.uinv + (size_t)p * NB * NB;
     const unsigned pe = (unsigned)(((size_t)m * kb + 255) / 256);
     panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 0, flag);
-    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag, bs);
     const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
     if (rb > 0)
-      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, flag, nullptr);
+      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, flag, nullptr, bs);
     panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 1, flag);
     fill_u32<<<(unsigned)((slots + 255) / 256), 256, 0, stream>>>(ws.w.ctr, slots, 0u);
     tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w, flag);
     ipiv_kernel<<<1, 64, 0, stream>>>(k0, kb, ws.w.winners, ws.ipiv, flag);
     swap_kernel<<<(ncols + 255) / 256, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.ipiv, ws.w.top, flag);
-    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag);
+    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag, bs);
     if (rb > 0)
-      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, nullptr, flag);
+      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, nullptr, flag, bs);
     panel_finish_kernel<<<1, 64, 0, stream>>>(flag, ws.zp_fast + p, ws.w.zero_pivot);
     if (cb > 0)
-      apply_kernel<<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, ws.linv, ws.udiag, nullptr, nullptr);
-    trailing_update(M, n, lda, ncols, k0, kb, stream, gt);
+      apply_kernel<<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, ws.linv, ws.udiag, nullptr, nullptr, bs);
+    trailing_update(M, n, lda, ncols, k0, kb, 1, bs, stream, gt);
   }
 }
 
-// U X = Y on the augmented columns, at most MAXRHS_LU of them at a time (the kernels take the
-// first column of the chunk as their `n`).
-void back_substitute(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuWorkspace& ws) {
+// U X = Y on the augmented columns of every matrix of the batch, at most MAXRHS_LU of them at a
+// time (the kernels take the first column of the chunk as their `n`).
+void back_substitute(double2* M, int n, int lda, int nrhs, int batch, BatchStrides bs, hipStream_t stream,
+                     LuWorkspace& ws) {
   for (int r0 = 0; r0 < nrhs; r0 += MAXRHS_LU) {
     const int nr = min(MAXRHS_LU, nrhs - r0);
     for (int k0 = ((n - 1) / NB) * NB, p = (n - 1) / NB; k0 >= 0; k0 -= NB, --p) {
       const int kb = min(NB, n - k0);
-      backsolve_block_kernel<<<1, 256, 0, stream>>>(M, n + r0, lda, k0, kb, ws.uinv + (size_t)p * NB * NB, nr);
-      if (k0 > 0) backsolve_update_kernel<<<(k0 + 63) / 64, 64, 0, stream>>>(M, n + r0, lda, k0, kb, nr);
+      backsolve_block_kernel<<<dim3(1, 1, batch), 256, 0, stream>>>(M, n + r0, lda, k0, kb,
+                                                                    ws.uinv + (size_t)p * NB * NB, nr, bs);
+      if (k0 > 0)
+        backsolve_update_kernel<<<dim3((k0 + 63) / 64, 1, batch), 64, 0, stream>>>(M, n + r0, lda, k0, kb, nr, bs);
     }
   }
 }
 
+void finish_stats(LuStats* stats, EventPool& ep, hipEvent_t t0, GemmTimer& gt, hipStream_t stream) {
+  if (!stats) return;
+  hipEvent_t t1 = ep.next();
+  HBG_HIP(hipEventRecord(t1, stream));
+  HBG_HIP(hipEventSynchronize(t1));
+  float ms = 0.0f;
+  HBG_HIP(hipEventElapsedTime(&ms, t0, t1));
+  stats->total_ms += ms;
+  for (auto& g : gt.ev) {
+    HBG_HIP(hipEventElapsedTime(&ms, g.first, g.second));
+    stats->gemm_ms += ms;
+  }
+  gt.ev.clear();
+}
+
 }  // namespace
 
-// Factor [A | B] (n x (n + nrhs), leading dimension lda) in place and solve; returns the first
-// zero pivot (1-based) or 0. Everything on `stream`; the call waits for the result.
-//
-// The whole matrix is first factored without interchanges (the partial-pivoting check on every
-// panel), after a copy of [A | B] is kept. When any panel's check fails, the copy is restored and
-// the factorization is redone with the pivoted panels; otherwise the result is the partially
-// pivoted one, with four launches per panel.
-int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats) {
+// Factor and solve `batch` systems [A | B] (n x (n + nrhs), leading dimension lda, matrix b at
+// M + b * mstride) without interchanges, checking on every panel that partial pivoting would make
+// none. status[b]: 0 solved (the partially pivoted result), k > 0 zero pivot k (1-based; the
+// column is zero on and below the diagonal, as partial pivoting would find it), -1 a panel needs
+// interchanges: the matrix is left overwritten and must be rebuilt and given to lu_solve_pivoted.
+void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
+                      LuStats* stats, int* status) {
   EventPool& ep = events();
   ep.used = 0;
   GemmTimer gt{&ep, stats, {}};
-  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipEvent_t t0 = nullptr;
   if (stats) {
     t0 = ep.next();
     HBG_HIP(hipEventRecord(t0, stream));
   }
   LuWorkspace& ws = workspace();
-  lu_workspace(ws, n);
+  lu_workspace(ws, n, batch);
+  BatchStrides bs = ws.strides();
+  bs.m = mstride;
   const int ncols = n + nrhs;
   const int panels = (n + NB - 1) / NB;
-  const size_t whole = (size_t)lda * ncols;
+  const size_t pf = (size_t)bs.flag * batch;
+  fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.flags, pf, 0);
+  fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.zp_fast, pf, 0x7fffffff);
+  factor_optimistic(M, n, lda, ncols, batch, bs, stream, ws, gt);
+  if (nrhs > 0) back_substitute(M, n, lda, nrhs, batch, bs, stream, ws);
+  HBG_HIP(hipGetLastError());
+  std::vector<int> flags(pf), zps(pf);
+  HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, pf * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HBG_HIP(hipMemcpyAsync(zps.data(), ws.zp_fast, pf * sizeof(int), hipMemcpyDeviceToHost, stream));
+  finish_stats(stats, ep, t0, gt, stream);
+  HBG_HIP(hipStreamSynchronize(stream));
+  for (int b = 0; b < batch; ++b) {
+    bool any = false;
+    int zp = 0x7fffffff;
+    for (int p = 0; p < panels; ++p) {
+      any = any || flags[(size_t)b * bs.flag + p] != 0;
+      zp = std::min(zp, zps[(size_t)b * bs.flag + p]);
+    }
+    status[b] = any ? -1 : (zp == 0x7fffffff ? 0 : zp);
+  }
+}
+
+// One system with partial pivoting where a panel needs it (tournament pivoting over several
+// leaves); returns the first zero pivot (1-based) or 0.
+int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats) {
+  EventPool& ep = events();
+  ep.used = 0;
+  GemmTimer gt{&ep, stats, {}};
+  hipEvent_t t0 = nullptr;
+  if (stats) {
+    t0 = ep.next();
+    HBG_HIP(hipEventRecord(t0, stream));
+  }
+  LuWorkspace& ws = workspace();
+  lu_workspace(ws, n, 1);
+  const int ncols = n + nrhs;
+  const int panels = (n + NB - 1) / NB;
+  fill_i32<<<1, 64, 0, stream>>>(ws.w.zero_pivot, 1, 0x7fffffff);
+  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
+  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
+  factor_pivoted(M, n, lda, ncols, stream, ws, gt);
+  if (nrhs > 0) back_substitute(M, n, lda, nrhs, 1, ws.strides(), stream, ws);
+  HBG_HIP(hipGetLastError());
+  int zp = 0;
+  std::vector<int> flags(panels);
+  HBG_HIP(hipMemcpyAsync(&zp, ws.w.zero_pivot, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
+  finish_stats(stats, ep, t0, gt, stream);
+  HBG_HIP(hipStreamSynchronize(stream));
+  if (stats)
+    for (int f : flags) stats->fallback_panels += f != 0;
+  return zp == 0x7fffffff ? 0 : zp;
+}
+
+// One system [A | B] in place: the batched path on a batch of one, a copy of the matrix kept to
+// restore it for the pivoted path when a panel needs interchanges. Returns the first zero pivot
+// (1-based) or 0.
+int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats) {
+  LuWorkspace& ws = workspace();
+  lu_workspace(ws, n, 1);
+  const size_t whole = (size_t)lda * (n + nrhs);
   if (ws.backup_elems < whole) {
     hipFree(ws.backup);
     ws.backup = nullptr;
@@ -857,49 +973,11 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
     ws.backup_elems = whole;
   }
   HBG_HIP(hipMemcpyAsync(ws.backup, M, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
-  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
-  factor_optimistic(M, n, lda, ncols, stream, ws, gt);
-  HBG_HIP(hipGetLastError());
-  std::vector<int> flags(panels), zps(panels);
-  HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
-  HBG_HIP(hipMemcpyAsync(zps.data(), ws.zp_fast, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
-  HBG_HIP(hipStreamSynchronize(stream));
-  bool any = false;
-  int zp = 0x7fffffff;
-  for (int p = 0; p < panels; ++p) {
-    any = any || flags[p] != 0;
-    zp = std::min(zp, zps[p]);
-  }
-  if (any) {
-    HBG_HIP(hipMemcpyAsync(M, ws.backup, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-    fill_i32<<<1, 64, 0, stream>>>(ws.w.zero_pivot, 1, 0x7fffffff);
-    fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
-    fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
-    factor_pivoted(M, n, lda, ncols, stream, ws, gt);
-    HBG_HIP(hipGetLastError());
-    HBG_HIP(hipMemcpyAsync(&zp, ws.w.zero_pivot, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
-  }
-  back_substitute(M, n, lda, nrhs, stream, ws);
-  HBG_HIP(hipGetLastError());
-  if (stats) {
-    t1 = ep.next();
-    HBG_HIP(hipEventRecord(t1, stream));
-  }
-  HBG_HIP(hipStreamSynchronize(stream));
-  if (stats) {
-    float ms = 0.0f;
-    HBG_HIP(hipEventElapsedTime(&ms, t0, t1));
-    stats->total_ms += ms;
-    for (auto& g : gt.ev) {
-      HBG_HIP(hipEventElapsedTime(&ms, g.first, g.second));
-      stats->gemm_ms += ms;
-    }
-    if (any)
-      for (int f : flags) stats->fallback_panels += f != 0;
-  }
-  return zp == 0x7fffffff ? 0 : zp;
+  int st = 0;
+  lu_solve_batched(M, whole, 1, n, lda, nrhs, stream, stats, &st);
+  if (st >= 0) return st;
+  HBG_HIP(hipMemcpyAsync(M, ws.backup, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+  return lu_solve_pivoted(M, n, lda, nrhs, stream, stats);
 }
 
 void lu_release() { workspace().release(); }

@@ -131,6 +131,12 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
  * complex, column-major (a[j][i] = A(i, j)), overwritten by the factors and the solution. */
 HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b);
 
+/* Frequencies in flight in hbg_solve (no counterpart in the reference, whose frequency pools run
+ * one CPU solve per worker, parallel.rs:63-89): each holds its N x (N + R) system on the device
+ * and their LUs advance panel by panel in the same launches. 0 (the default): as many as a quarter
+ * of the free device memory holds, at most 16. */
+HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch);
+
 /* Stage timing (the reference's HB_PROFILE phase marks, low_order.rs:184-208), by HIP events on the
  * session's stream: enable (clears the counters) or disable. hbg_profile_read copies up to n of the
  * HBG_PROFILE_COUNTERS counters, accumulated over the solves since enabling:

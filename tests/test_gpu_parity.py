@@ -184,6 +184,22 @@ def test_solve_multi_leaf_tournament(session, oracle):
         assert e < 1e-10
 
 
+def test_batched_frequencies_match_one_at_a_time(session, oracle):
+    # frequencies in flight share the LU launches; each system's arithmetic is the same
+    say("case: batch 1 vs batch 3 on 5 frequencies, hemisphere 512 panels")
+    v = meshes.hemisphere_panels(1.0, 16, 32, quarter=False)
+    w = np.sqrt(9.81 * np.linspace(0.2, 2.8, 5))
+    session.set_environment(1025.0, 9.81)
+    session.set_panels(v)
+    session.set_batch(1)
+    one = session.solve(w, [0.0, 30.0])
+    session.set_batch(3)
+    three = session.solve(w, [0.0, 30.0])
+    session.set_batch(0)
+    for key in ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_haskind"):
+        assert np.array_equal(one[key], three[key]), key
+
+
 def test_progress_and_cancel(session):
     say("case: progress callback and cancel")
     v = meshes.hemisphere_panels(1.0, 6, 24, quarter=False)
