@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cylinder4096")
     ap.add_argument("--batch", type=int, default=0, help="frequencies in flight (0: automatic)")
+    ap.add_argument("--gemm", type=int, default=-1, help="LU GEMM variant (-1: the library's default)")
+    ap.add_argument("--depth", type=int, default=-1, help="LU panels per trailing update (-1: the library's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -100,6 +102,10 @@ def main():
     s = hb.Session(0)
     s.set_environment(w["rho"], w["g"], w.get("depth", float("inf")))
     s.set_batch(a.batch)
+    if a.gemm >= 0:
+        s.set_tuning(s.TUNE_LU_GEMM, a.gemm)
+    if a.depth >= 0:
+        s.set_tuning(s.TUNE_LU_DEPTH, a.depth)
 
     def step():
         s.set_panels(v)
@@ -145,7 +151,7 @@ def main():
                    "lu_achieved_tflops": round(lu_tf / 1e12, 3),
                    "lu_pivoting_fallback_panels": int(prof["lu_fallback_panels"]),
                    "assembly_achieved_gbs": round(asm_bw / 1e9, 1)},
-        "roofline": {"bound": "mfma", "kernel": "gemm_update_kernel (LU trailing update, complex FP64)",
+        "roofline": {"bound": "mfma", "kernel": "gemm_mfma_kernel (LU trailing updates, complex FP64 on MFMA)",
                      "achieved": round(gemm_tf / 1e12, 3), "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": round(gemm_tf / PEAK_FP64, 4), "traffic": None,
                      "per_launch": {"flops": prof["lu_gemm_flops"] / launches,

@@ -66,6 +66,8 @@ SIGNATURES = {
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "hbg_set_profiling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "hbg_set_batch": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "hbg_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "hbg_get_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "hbg_profile_read": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
     "hbg_write_wamit": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
@@ -168,6 +170,17 @@ class Session:
     def set_batch(self, max_batch: int):
         """Frequencies in flight in solve (0: as many as fit, at most 16)."""
         _check(library().hbg_set_batch(self._h, int(max_batch)))
+
+    TUNE_LU_GEMM = 0
+    TUNE_LU_DEPTH = 1
+
+    def set_tuning(self, key: int, value: int):
+        _check(library().hbg_set_tuning(self._h, int(key), int(value)))
+
+    def get_tuning(self, key: int) -> int:
+        v = ctypes.c_int32(0)
+        _check(library().hbg_get_tuning(self._h, int(key), ctypes.byref(v)))
+        return v.value
 
     def set_profiling(self, enable: bool):
         _check(library().hbg_set_profiling(self._h, int(bool(enable))))

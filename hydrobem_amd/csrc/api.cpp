@@ -45,6 +45,10 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
 void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
                       LuStats* stats, int* status);
 int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
+void lu_set_gemm_variant(int v);
+int lu_gemm_variant();
+void lu_set_depth(int q);
+int lu_depth();
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
                 const double* X, const char* stamp, std::string& err);
@@ -838,6 +842,36 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_CANCELLED, "cancelled"};
   return HBG_OK;
   GUARD_END("hbg_solve")
+}
+
+HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_tuning");
+  switch (key) {
+    case HBG_TUNE_LU_GEMM:
+      if (value < 1 || value > 2) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1 or 2", "hbg_set_tuning");
+      lu_set_gemm_variant(value);
+      return HBG_OK;
+    case HBG_TUNE_LU_DEPTH:
+      if (value < 1 || value > 16) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..16", "hbg_set_tuning");
+      lu_set_depth(value);
+      return HBG_OK;
+    default:
+      return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_set_tuning");
+  }
+}
+
+HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value) {
+  if (!s || !value) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_get_tuning");
+  switch (key) {
+    case HBG_TUNE_LU_GEMM:
+      *value = lu_gemm_variant();
+      return HBG_OK;
+    case HBG_TUNE_LU_DEPTH:
+      *value = lu_depth();
+      return HBG_OK;
+    default:
+      return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_get_tuning");
+  }
 }
 
 HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch) {

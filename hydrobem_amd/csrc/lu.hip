@@ -19,10 +19,12 @@
 // elimination on the original values of the candidates; the last workgroup to finish a node
 // merges it (atomic counters, no waiting). With one leaf this is partial pivoting exactly; with
 // more it is tournament pivoting (Grigori, Demmel & Xiang 2011).
-// Then U12 = L11^-1 A12 and the trailing update A22 -= L21 U12 (gemm_update_kernel).
+// Then U12 = L11^-1 A12 and the trailing update A22 -= L21 U12 (gemm_mfma_kernel, complex FP64
+// on the matrix cores), once per block of panels in the pass without interchanges.
 #include <algorithm>
 #include <vector>
 
+#include "gemm_mfma.h"
 #include "hbg_internal.h"
 
 namespace hbg {
@@ -543,68 +545,6 @@ __global__ void panel_finish_kernel(const int* __restrict__ flag, const int* __r
   if (threadIdx.x == 0 && *flag == 0 && *zp_fast != 0x7fffffff) atomicMin(zero_pivot, *zp_fast);
 }
 
-// C -= A B on the trailing matrix: rows [r0, n), columns [c0, ncols), K = kb (<= NB); A is
-// the block column L21 (column k0..), B the block row U12 (rows k0..). 64 x 64 tiles of C per
-// workgroup of 256 threads, 4 x 4 complex outputs per thread, operands staged in LDS.
-__global__ void __launch_bounds__(256) gemm_update_kernel(double2* __restrict__ M, int n, int lda, int ncols, int k0,
-                                                          int kb, BatchStrides bs) {
-  M += blockIdx.z * bs.m;
-  __shared__ double2 As[16][64 + 1];
-  __shared__ double2 Bs[16][64 + 1];
-  const int r0 = k0 + kb, c0 = k0 + kb;
-  const int tr = blockIdx.x * 64, tc = blockIdx.y * 64;
-  const int tid = threadIdx.x;
-  const int ti = tid & 15, tj = tid >> 4;  // thread's 4 x 4 block: rows ti + 16 u, columns tj + 16 v
-  double2 acc[4][4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = make_double2(0.0, 0.0);
-  for (int kc = 0; kc < kb; kc += 16) {
-    // A tile: 64 rows x 16 k; B tile: 16 k x 64 columns
-    for (int e = tid; e < 64 * 16; e += 256) {
-      const int i = e & 63, kk = e >> 6;
-      const int row = r0 + tr + i, k = kc + kk;
-      As[kk][i] = (row < n && k < kb) ? M[(size_t)(k0 + k) * lda + row] : make_double2(0.0, 0.0);
-      const int jj = e >> 4, kk2 = e & 15;
-      const int col = c0 + tc + jj, k2 = kc + kk2;
-      Bs[kk2][jj] = (col < ncols && k2 < kb) ? M[(size_t)col * lda + k0 + k2] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      double2 a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) a[u] = As[kk][ti + 16 * u];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) b[v] = Bs[kk][tj + 16 * v];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          acc[u][v].x = fma(a[u].x, b[v].x, fma(-a[u].y, b[v].y, acc[u][v].x));
-          acc[u][v].y = fma(a[u].x, b[v].y, fma(a[u].y, b[v].x, acc[u][v].y));
-        }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int col = c0 + tc + tj + 16 * v;
-    if (col >= ncols) continue;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int row = r0 + tr + ti + 16 * u;
-      if (row >= n) continue;
-      double2* p = M + (size_t)col * lda + row;
-      double2 c = *p;
-      c.x -= acc[u][v].x;
-      c.y -= acc[u][v].y;
-      *p = c;
-    }
-  }
-}
-
 // Back substitution U X = Y on the augmented columns [n, n + r), block by block from the bottom:
 // the block of rows (k0, kb) is X_b = Uinv_b Y_b (the stored inverse of its diagonal block), then
 // the rows above are updated, Y[0:k0] -= U[0:k0, b] X_b (one output per thread).
@@ -749,6 +689,13 @@ static void lu_workspace(LuWorkspace& ws, int n, int batch) {
   ws.batch = batch;
 }
 
+static int g_gemm_variant = 1;
+static int g_lu_depth = 4;
+void lu_set_gemm_variant(int v) { g_gemm_variant = v; }
+int lu_gemm_variant() { return g_gemm_variant; }
+void lu_set_depth(int q) { g_lu_depth = q; }
+int lu_depth() { return g_lu_depth; }
+
 static LuWorkspace& workspace() {
   static thread_local LuWorkspace ws;
   return ws;
@@ -778,43 +725,66 @@ struct GemmTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
-// A22 -= L21 U12 for the panel (k0, kb) of `batch` matrices, timed when stats are kept
-void trailing_update(double2* M, int n, int lda, int ncols, int k0, int kb, int batch, BatchStrides bs,
-                     hipStream_t stream, GemmTimer& gt) {
-  const int m = n - k0;
-  if (m <= kb || ncols <= k0 + kb) return;
-  dim3 grid((m - kb + 63) / 64, (ncols - k0 - kb + 63) / 64, batch);
+// M[r0 + i][c0 + j] -= sum_k M[r0 + i][kf + k] M[kf + k][c0 + j] (i < nrows, j < ncolsC, k < kk)
+// on `batch` matrices, on the matrix cores; timed when stats are kept. Variant 1: 64 x 64 tiles,
+// 2: 128 x 64.
+void gemm_sub(double2* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk, int batch, size_t mstride,
+              hipStream_t stream, GemmTimer& gt) {
+  if (nrows <= 0 || ncolsC <= 0 || kk <= 0) return;
   hipEvent_t g0 = nullptr;
   if (gt.stats) {
     g0 = gt.ep->next();
     HBG_HIP(hipEventRecord(g0, stream));
   }
-  gemm_update_kernel<<<grid, 256, 0, stream>>>(M, n, lda, ncols, k0, kb, bs);
+  if (g_gemm_variant == 2)
+    gemm_mfma_kernel<double, 128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+        M, lda, r0, nrows, c0, ncolsC, kf, kk, GemmBatch{mstride});
+  else
+    gemm_mfma_kernel<double, 64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+        M, lda, r0, nrows, c0, ncolsC, kf, kk, GemmBatch{mstride});
   if (gt.stats) {
     hipEvent_t g1 = gt.ep->next();
     HBG_HIP(hipEventRecord(g1, stream));
     gt.ev.push_back({g0, g1});
-    gt.stats->gemm_flops += 8.0 * batch * (double)(m - kb) * (double)(ncols - k0 - kb) * (double)kb;
+    gt.stats->gemm_flops += 8.0 * batch * (double)nrows * (double)ncolsC * (double)kk;
     gt.stats->gemm_launches += 1;
   }
 }
 
+// A22 -= L21 U12 for the panel (k0, kb): rows and columns from k0 + kb
+void trailing_update(double2* M, int n, int lda, int ncols, int k0, int kb, int batch, BatchStrides bs,
+                     hipStream_t stream, GemmTimer& gt) {
+  gemm_sub(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, batch, bs.m, stream, gt);
+}
+
 // Every panel of every matrix of the batch without interchanges: the diagonal block,
 // L21 = A21 U11^-1 (both checking that partial pivoting would make no interchange; the panel's
-// flag is set where it would), U12 = L11^-1 A12 and the trailing update. Three launches per panel.
+// flag is set where it would) and U12 = L11^-1 A12. The panels go in blocks of g_lu_depth: before
+// a panel, the earlier panels of its block update its columns (rows from the panel down) and its
+// rows (columns right of it); after the block one trailing update with K = the block's width, so
+// the trailing matrix is read and written once per block rather than once per panel.
 void factor_optimistic(double2* M, int n, int lda, int ncols, int batch, BatchStrides bs, hipStream_t stream,
                        LuWorkspace& ws, GemmTimer& gt) {
-  for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
-    const int kb = min(NB, n - k0);
-    const int m = n - k0;
-    double2* uinv = ws.uinv + (size_t)p * NB * NB;
-    diag_kernel<<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
-                                                       uinv, ws.linv, ws.udiag, bs);
-    const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
-    if (rb + cb > 0)
-      apply_kernel<<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv,
-                                                                ws.udiag, ws.flags + p, nullptr, bs);
-    trailing_update(M, n, lda, ncols, k0, kb, batch, bs, stream, gt);
+  const int Q = std::max(1, g_lu_depth);
+  for (int K0 = 0; K0 < n; K0 += Q * NB) {
+    const int qend = std::min(n, K0 + Q * NB);
+    for (int k0 = K0; k0 < qend; k0 += NB) {
+      const int p = k0 / NB;
+      const int kb = min(NB, n - k0);
+      const int m = n - k0;
+      if (k0 > K0) {
+        gemm_sub(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, stream, gt);
+        gemm_sub(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, stream, gt);
+      }
+      double2* uinv = ws.uinv + (size_t)p * NB * NB;
+      diag_kernel<<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
+                                                         uinv, ws.linv, ws.udiag, bs);
+      const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
+      if (rb + cb > 0)
+        apply_kernel<<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv,
+                                                                  ws.udiag, ws.flags + p, nullptr, bs);
+    }
+    gemm_sub(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, stream, gt);
   }
 }
 

@@ -137,6 +137,13 @@ HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double
  * of the free device memory holds, at most 16. */
 HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch);
 
+/* Kernel variants (no counterpart in the reference), per process: HBG_TUNE_LU_GEMM selects the
+ * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64;
+ * HBG_TUNE_LU_DEPTH the number of 64-column panels per trailing update (1..16, default 4). */
+enum { HBG_TUNE_LU_GEMM = 0, HBG_TUNE_LU_DEPTH = 1 };
+HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
+HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);
+
 /* Stage timing (the reference's HB_PROFILE phase marks, low_order.rs:184-208), by HIP events on the
  * session's stream: enable (clears the counters) or disable. hbg_profile_read copies up to n of the
  * HBG_PROFILE_COUNTERS counters, accumulated over the solves since enabling:
