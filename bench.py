@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="frequencies in flight (0: automatic)")
     ap.add_argument("--gemm", type=int, default=-1, help="LU GEMM variant (-1: the library's default)")
     ap.add_argument("--depth", type=int, default=-1, help="LU panels per trailing update (-1: the library's default)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -109,7 +110,7 @@ def main():
 
     def step():
         s.set_panels(v)
-        return s.solve(omegas, heads)
+        return s.solve(omegas, heads, precision=a.precision)
 
     print(f"[bench] workload {w['name']}: {N} panels, {F} frequencies; warmup {a.warmup}", file=sys.stderr, flush=True)
     for i in range(a.warmup):

@@ -100,6 +100,9 @@ def _p(a):
     return a.ctypes.data_as(_dp) if a is not None else None
 
 
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
 def _options(logarithmic=True, log_tolerance=1e-4, precision=0) -> Options:
     return Options(int(precision), int(bool(logarithmic)), float(log_tolerance))
 
@@ -144,8 +147,9 @@ class Session:
                                         _p(rc)))
         self.n_panels, self.n_bodies = n, nb
 
-    def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None):
-        """The quantities of one solve (the reference's result names and axes)."""
+    def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None, precision="fp64"):
+        """The quantities of one solve (the reference's result names and axes). precision "fp64"
+        (the plain path) or "mixed" (complex64 LU on the matrix cores + FP64 refinement)."""
         w = np.ascontiguousarray(omegas, dtype=np.float64).reshape(-1)
         h = np.ascontiguousarray(headings_deg, dtype=np.float64).reshape(-1)
         F, H, D = len(w), len(h), 6 * self.n_bodies
@@ -156,7 +160,7 @@ class Session:
         SC = np.zeros((F, H, D), np.complex128)
         HK = np.zeros((F, D, H), np.complex128)
         cb = PROGRESS(lambda user, stage, done, total: int(progress(stage, done, total)) if progress else 1)
-        opt = _options(logarithmic, log_tolerance)
+        opt = _options(logarithmic, log_tolerance, PRECISIONS[precision])
         _check(library().hbg_solve(self._h, _p(w), F, _p(h), H, ctypes.byref(opt), _p(A), _p(B), _p(X), _p(FK),
                                    _p(SC), _p(HK), cb, None))
         return {"omegas": w, "headings": h, "added_mass": A, "radiation_damping": B,
@@ -165,7 +169,8 @@ class Session:
 
     PROFILE_KEYS = ("statics_ms", "assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "lu_gemm_flops",
                     "lu_gemm_launches", "forces_ms", "frequencies", "lu_flops", "assembly_bytes", "statics_runs",
-                    "lu_fallback_panels")
+                    "lu_fallback_panels", "mixed_refinement_steps", "mixed_max_backward_error",
+                    "mixed_fp64_fallbacks")
 
     def set_batch(self, max_batch: int):
         """Frequencies in flight in solve (0: as many as fit, at most 16)."""

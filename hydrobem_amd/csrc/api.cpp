@@ -45,6 +45,10 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
 void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
                       LuStats* stats, int* status);
 int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
+void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, int batch, int n, int R,
+                         double tol, int max_iter, hipStream_t stream, LuStats* stats, int* status, int* iters,
+                         double* eta);
+int residual_blocks(int n);
 void lu_set_gemm_variant(int v);
 int lu_gemm_variant();
 void lu_set_depth(int q);
@@ -265,6 +269,10 @@ struct HbgSession {
   DevBuf<int2> d_queue;
   DevBuf<unsigned> d_qcount;
   DevBuf<double2> d_S, d_M, d_phiI, d_dphiI, d_u, d_part;
+  // mixed precision: the complex64 systems, the FP64 solutions, the complex64 corrections, norms
+  DevBuf<float2> d_M32, d_D32;
+  DevBuf<double2> d_X64;
+  DevBuf<double> d_rpart;
   int max_batch = 0;  // frequencies in flight (0: as many as fit, at most 16)
   std::vector<hipEvent_t> bev;
   bool statics_valid = false;
@@ -381,10 +389,14 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout) {
   return f.k0;
 }
 
+constexpr double MIXED_TOL = 1e-12;  // normwise backward error that ends the refinement
+constexpr int MIXED_MAX_ITER = 10;
+
 // Frequencies in flight: the session's setting, else as many as a quarter of the free device
 // memory holds (each needs its N x (N + R) system), at most 16, at most the number of frequencies.
-int batch_size(HbgSession* s, int N, int R, int F) {
-  const size_t per = (size_t)N * (N + R) * sizeof(double2) + (size_t)N * 64 * sizeof(double2) * 2;
+int batch_size(HbgSession* s, int N, int R, int F, bool mixed) {
+  size_t per = (size_t)N * (N + R) * sizeof(double2) + (size_t)N * 64 * sizeof(double2) * 2;
+  if (mixed) per += (size_t)N * (N + R) * sizeof(float2) + (size_t)N * R * (sizeof(double2) + sizeof(float2));
   int B = s->max_batch;
   if (B <= 0) {
     size_t free_b = 0, total_b = 0;
@@ -395,7 +407,8 @@ int batch_size(HbgSession* s, int N, int R, int F) {
 }
 
 void check_options(const HbgOptions* o) {
-  if (o && o->precision != HBG_PRECISION_FP64) throw Error{HBG_ERR_UNSUPPORTED, "precision not supported"};
+  if (o && o->precision != HBG_PRECISION_FP64 && o->precision != HBG_PRECISION_MIXED)
+    throw Error{HBG_ERR_UNSUPPORTED, "precision not supported"};
   if (o && !(o->log_tolerance >= 0.0)) throw Error{HBG_ERR_VALUE, "log_tolerance must be >= 0"};
 }
 
@@ -739,7 +752,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // its forces are taken; S, the right-hand-side partials and the finite-depth tables are shared
   const size_t mstride = (size_t)N * (N + R);
   const size_t hstride = (size_t)std::max(H, 1) * N;
-  int B = batch_size(s, N, R, n_omegas);
+  const bool mixed = o.precision == HBG_PRECISION_MIXED;
+  int B = batch_size(s, N, R, n_omegas, mixed);
   s->d_S.ensure((size_t)N * N);
   s->d_M.ensure(mstride * B);
   s->d_phiI.ensure(hstride * B);
@@ -747,7 +761,15 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   s->d_u.ensure((size_t)R * N);
   s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
   s->d_out.ensure(nout * B);
+  if (mixed) {
+    s->d_M32.ensure(mstride * B);
+    s->d_D32.ensure((size_t)N * R * B);
+    s->d_X64.ensure((size_t)N * R * B);
+    s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * B);
+  }
   std::vector<double> hout(nout * B);
+  std::vector<int> iters(B);
+  std::vector<double> eta(B);
   const bool prof = s->profiling;
   if (prof && s->bev.size() < (size_t)(3 * B + 3)) {
     for (size_t i = s->bev.size(); i < (size_t)(3 * B + 3); ++i) {
@@ -792,7 +814,22 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     for (int b = 0; b < nb; ++b)
       if (progress && progress(user, HBG_STAGE_LU, f0 + b, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     LuStats lus;
-    lu_solve_batched(s->d_M.p, mstride, nb, N, N, R, s->stream, prof ? &lus : nullptr, status.data());
+    if (mixed) {
+      mixed_solve_batched(s->d_M.p, s->d_M32.p, s->d_X64.p, s->d_D32.p, s->d_rpart.p, nb, N, R, MIXED_TOL,
+                          MIXED_MAX_ITER, s->stream, prof ? &lus : nullptr, status.data(), iters.data(), eta.data());
+      for (int b = 0; b < nb; ++b) {
+        if (status[b] < 0) {  // the plain FP64 path for this frequency
+          build(f0 + b, b);
+          lu_solve_batched(s->d_M.p + mstride * b, mstride, 1, N, N, R, s->stream, prof ? &lus : nullptr, &status[b]);
+          if (prof) s->prof[15] += 1.0;
+        } else if (prof) {
+          s->prof[13] += iters[b];
+          s->prof[14] = std::max(s->prof[14], eta[b]);
+        }
+      }
+    } else {
+      lu_solve_batched(s->d_M.p, mstride, nb, N, N, R, s->stream, prof ? &lus : nullptr, status.data());
+    }
     for (int b = 0; b < nb; ++b) {
       if (status[b] < 0) {  // a panel needs interchanges: rebuild the system, pivoted path
         build(f0 + b, b);

@@ -22,6 +22,7 @@
 // Then U12 = L11^-1 A12 and the trailing update A22 -= L21 U12 (gemm_mfma_kernel, complex FP64
 // on the matrix cores), once per block of panels in the pass without interchanges.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "gemm_mfma.h"
@@ -56,6 +57,10 @@ struct BatchStrides {
 namespace {
 
 __device__ __forceinline__ double mag1(double2 v) { return fabs(v.x) + fabs(v.y); }
+__device__ __forceinline__ float mag1(float2 v) { return fabsf(v.x) + fabsf(v.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -267,11 +272,12 @@ __global__ void panel_copy_kernel(double2* __restrict__ M, int n, int lda, int k
     *a = save[e];
 }
 
-__device__ __forceinline__ double2 crecip(double2 v) {
-  const double d = v.x * v.x + v.y * v.y;
-  if (d == 0.0) return make_double2(0.0, 0.0);
-  const double id = 1.0 / d;
-  return make_double2(v.x * id, -v.y * id);
+template <typename T, typename C2 = typename Cplx<T>::type>
+__device__ __forceinline__ C2 crecip(C2 v) {
+  const T d = v.x * v.x + v.y * v.y;
+  if (d == 0.0) return C2(0.0, 0.0);
+  const T id = T(1) / d;
+  return C2(v.x * id, -v.y * id);
 }
 
 // The diagonal block (k0, kb): mode 0 factors it without interchanges, checking at every step j
@@ -283,10 +289,12 @@ __device__ __forceinline__ double2 crecip(double2 v) {
 // The block lives in registers: thread (ti, tj) of the 16 x 16 grid holds the entries
 // (ti + 16 a, tj + 16 b), a, b < 4. At step j the owners of row j and column j publish them in
 // LDS (two alternating buffers, so one barrier per step) and every thread updates its entries.
-__global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int lda, int k0, int kb, int mode,
+template <typename T>
+__global__ void __launch_bounds__(256) diag_kernel(typename Cplx<T>::type* __restrict__ M, int lda, int k0, int kb, int mode,
                                                    int* __restrict__ flag, int* __restrict__ zp_fast,
-                                                   int* __restrict__ ipiv, double2* __restrict__ Uinv,
-                                                   double2* __restrict__ Linv, double2* __restrict__ udiag, BatchStrides bs) {
+                                                   int* __restrict__ ipiv, typename Cplx<T>::type* __restrict__ Uinv,
+                                                   typename Cplx<T>::type* __restrict__ Linv, typename Cplx<T>::type* __restrict__ udiag, BatchStrides bs) {
+  typedef typename Cplx<T>::type C2;
   {
     const int z = blockIdx.z;
     M += z * bs.m;
@@ -294,23 +302,23 @@ __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int 
     zp_fast += z * bs.flag;
     ipiv += z * bs.ipiv;
     Uinv += z * bs.uinv;
-    Linv += z * (NB * NB);
+    Linv += z * bs.uinv;
     udiag += z * NB;
   }
   if (mode == 1 && *flag == 0) return;
-  __shared__ double2 rowb[2][NB], colb[2][NB];
-  __shared__ double2 rowu[2][NB], colu[2][NB];
-  __shared__ double2 dg[NB];
+  __shared__ C2 rowb[2][NB], colb[2][NB];
+  __shared__ C2 rowu[2][NB], colu[2][NB];
+  __shared__ C2 dg[NB];
   __shared__ int s_fail;
   const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
   if (tid == 0) s_fail = 0;
-  double2 t[4][4];
+  C2 t[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int r = ti + 16 * a, c = tj + 16 * b;
-      t[a][b] = (r < kb && c < kb) ? M[(size_t)(k0 + c) * lda + k0 + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+      t[a][b] = (r < kb && c < kb) ? M[(size_t)(k0 + c) * lda + k0 + r] : C2(r == c ? 1.0 : 0.0, 0.0);
     }
   if (mode == 0) {
     bool fail = false;
@@ -327,17 +335,17 @@ __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int 
           for (int a = 0; a < 4; ++a) colb[buf][ti + 16 * a] = t[a][jb];
         }
         __syncthreads();
-        const double2 piv = rowb[buf][j];
-        const double pm = mag1(piv);
-        const double2 rp = crecip(piv);
+        const C2 piv = rowb[buf][j];
+        const T pm = mag1(piv);
+        const C2 rp = crecip<T>(piv);
         if (tid == 0 && pm == 0.0 && j < kb) atomicMin(zp_fast, k0 + j + 1);
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           if (a < jb) continue;
           if (a == jb && ti <= jj) continue;
-          const double2 v = colb[buf][ti + 16 * a];
+          const C2 v = colb[buf][ti + 16 * a];
           if (tj == jj && mag1(v) > pm) fail = true;
-          const double2 l = cmul(v, rp);
+          const C2 l = cmul(v, rp);
 #pragma unroll
           for (int b = 0; b < 4; ++b) {
             if (b < jb) continue;
@@ -345,7 +353,7 @@ __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int 
             if (b == jb && tj == jj) {
               t[a][b] = l;
             } else {
-              const double2 u = rowb[buf][tj + 16 * b];
+              const C2 u = rowb[buf][tj + 16 * b];
               t[a][b].x -= l.x * u.x - l.y * u.y;
               t[a][b].y -= l.x * u.y + l.y * u.x;
             }
@@ -373,14 +381,14 @@ __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int 
   // Linv: L X = I (unit lower), step j ascending: X[r][*] -= l_rj X[j][*], r > j.
   // Uinv: U X = I, step j descending: X[j][*] *= 1 / u_jj, then X[r][*] -= u_rj X[j][*], r < j.
   // Both in one loop, step s doing j = s for Linv and j = 63 - s for Uinv.
-  double2 xl[4][4], xu[4][4];
+  C2 xl[4][4], xu[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const double v = (a == b && ti == tj) ? 1.0 : 0.0;
-      xl[a][b] = make_double2(v, 0.0);
-      xu[a][b] = make_double2(v, 0.0);
+      const T v = (a == b && ti == tj) ? 1.0 : 0.0;
+      xl[a][b] = C2(v, 0.0);
+      xu[a][b] = C2(v, 0.0);
     }
 #pragma unroll
   for (int sb = 0; sb < 4; ++sb) {
@@ -397,7 +405,7 @@ __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int 
         for (int a = 0; a < 4; ++a) colb[buf][ti + 16 * a] = t[a][sb];
       }
       if (ti == uj) {
-        const double2 rd = crecip(dg[ju]);
+        const C2 rd = crecip<T>(dg[ju]);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           xu[ub][b] = cmul(xu[ub][b], rd);
@@ -413,20 +421,20 @@ __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int 
       for (int a = 0; a < 4; ++a) {
         // Linv rows r > jl; row jl of X is zero right of jl
         if (a > sb || (a == sb && ti > ss)) {
-          const double2 l = colb[buf][ti + 16 * a];
+          const C2 l = colb[buf][ti + 16 * a];
 #pragma unroll
           for (int b = 0; b <= sb; ++b) {
-            const double2 x = rowb[buf][tj + 16 * b];
+            const C2 x = rowb[buf][tj + 16 * b];
             xl[a][b].x -= l.x * x.x - l.y * x.y;
             xl[a][b].y -= l.x * x.y + l.y * x.x;
           }
         }
         // Uinv rows r < ju; row ju of X is zero left of ju
         if (a < ub || (a == ub && ti < uj)) {
-          const double2 u = colu[buf][ti + 16 * a];
+          const C2 u = colu[buf][ti + 16 * a];
 #pragma unroll
           for (int b = ub; b < 4; ++b) {
-            const double2 x = rowu[buf][tj + 16 * b];
+            const C2 x = rowu[buf][tj + 16 * b];
             xu[a][b].x -= u.x * x.x - u.y * x.y;
             xu[a][b].y -= u.x * x.y + u.y * x.x;
           }
@@ -451,44 +459,46 @@ __global__ void __launch_bounds__(256) diag_kernel(double2* __restrict__ M, int 
 //   blocks [rblocks, gridDim.x): U12 = Linv A12 on 16 columns each (< ncols).
 // gate: run only if the flag is set.
 constexpr int AB = 16;
-__global__ void __launch_bounds__(256) apply_kernel(double2* __restrict__ M, int lda, int lim, int ncols, int k0,
-                                                    int kb, int rblocks, const double2* __restrict__ Uinv,
-                                                    const double2* __restrict__ Linv,
-                                                    const double2* __restrict__ udiag, int* __restrict__ flag,
+template <typename T>
+__global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __restrict__ M, int lda, int lim, int ncols, int k0,
+                                                    int kb, int rblocks, const typename Cplx<T>::type* __restrict__ Uinv,
+                                                    const typename Cplx<T>::type* __restrict__ Linv,
+                                                    const typename Cplx<T>::type* __restrict__ udiag, int* __restrict__ flag,
                                                     const int* __restrict__ gate, BatchStrides bs) {
+  typedef typename Cplx<T>::type C2;
   {
     const int z = blockIdx.z;
     M += z * bs.m;
     Uinv += z * bs.uinv;
-    Linv += z * (NB * NB);
+    Linv += z * bs.uinv;
     udiag += z * NB;
     if (flag) flag += z * bs.flag;
   }
   if (gate && *gate == 0) return;
-  __shared__ double2 Ts[NB][NB + 1];
-  __shared__ double2 Xs[NB][AB + 1];  // right: [k][row]; left: [k][col]
+  __shared__ C2 Ts[NB][NB + 1];
+  __shared__ C2 Xs[NB][AB + 1];  // right: [k][row]; left: [k][col]
   __shared__ int s_fail;
   const int tid = threadIdx.x;
   const bool right = (int)blockIdx.x < rblocks;
-  const double2* __restrict__ Tm = right ? Uinv : Linv;
+  const C2* __restrict__ Tm = right ? Uinv : Linv;
   for (int e = tid; e < NB * NB; e += 256) Ts[e >> 6][e & 63] = Tm[e];
   if (right) {
     const int base = k0 + kb + blockIdx.x * AB;
     if (tid == 0) s_fail = 0;
     for (int e = tid; e < NB * AB; e += 256) {
       const int i = e & (AB - 1), k = e >> 4;
-      Xs[k][i] = (base + i < lim && k < kb) ? M[(size_t)(k0 + k) * lda + base + i] : make_double2(0.0, 0.0);
+      Xs[k][i] = (base + i < lim && k < kb) ? M[(size_t)(k0 + k) * lda + base + i] : C2(0.0, 0.0);
     }
     __syncthreads();
     const int i = tid & 15, cq = tid >> 4;
-    double2 acc[4];
+    C2 acc[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] = make_double2(0.0, 0.0);
+    for (int v = 0; v < 4; ++v) acc[v] = C2(0.0, 0.0);
     for (int k = 0; k < NB; ++k) {
-      const double2 x = Xs[k][i];
+      const C2 x = Xs[k][i];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const double2 u = Ts[k][cq + 16 * v];
+        const C2 u = Ts[k][cq + 16 * v];
         acc[v].x = fma(x.x, u.x, fma(-x.y, u.y, acc[v].x));
         acc[v].y = fma(x.x, u.y, fma(x.y, u.x, acc[v].y));
       }
@@ -501,7 +511,7 @@ __global__ void __launch_bounds__(256) apply_kernel(double2* __restrict__ M, int
         if (c >= kb) continue;
         M[(size_t)(k0 + c) * lda + base + i] = acc[v];
         if (flag) {
-          const double2 d = udiag[c];
+          const C2 d = udiag[c];
           if (mag1(cmul(acc[v], d)) > mag1(d)) fail = true;
         }
       }
@@ -513,18 +523,18 @@ __global__ void __launch_bounds__(256) apply_kernel(double2* __restrict__ M, int
     const int base = k0 + kb + (blockIdx.x - rblocks) * AB;
     for (int e = tid; e < NB * AB; e += 256) {
       const int k = e & 63, c = e >> 6;
-      Xs[k][c] = (base + c < ncols && k < kb) ? M[(size_t)(base + c) * lda + k0 + k] : make_double2(0.0, 0.0);
+      Xs[k][c] = (base + c < ncols && k < kb) ? M[(size_t)(base + c) * lda + k0 + k] : C2(0.0, 0.0);
     }
     __syncthreads();
     const int r = tid & 63, cq = tid >> 6;
-    double2 acc[4];
+    C2 acc[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] = make_double2(0.0, 0.0);
+    for (int v = 0; v < 4; ++v) acc[v] = C2(0.0, 0.0);
     for (int k = 0; k < NB; ++k) {
-      const double2 l = Ts[r][k];
+      const C2 l = Ts[r][k];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const double2 x = Xs[k][cq + 4 * v];
+        const C2 x = Xs[k][cq + 4 * v];
         acc[v].x = fma(l.x, x.x, fma(-l.y, x.y, acc[v].x));
         acc[v].y = fma(l.x, x.y, fma(l.y, x.x, acc[v].y));
       }
@@ -545,66 +555,77 @@ __global__ void panel_finish_kernel(const int* __restrict__ flag, const int* __r
   if (threadIdx.x == 0 && *flag == 0 && *zp_fast != 0x7fffffff) atomicMin(zero_pivot, *zp_fast);
 }
 
-// Back substitution U X = Y on the augmented columns [n, n + r), block by block from the bottom:
-// the block of rows (k0, kb) is X_b = Uinv_b Y_b (the stored inverse of its diagonal block), then
-// the rows above are updated, Y[0:k0] -= U[0:k0, b] X_b (one output per thread).
-__global__ void __launch_bounds__(256) backsolve_block_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
-                                                              const double2* __restrict__ Uinv, int nrhs, BatchStrides bs) {
-  M += blockIdx.z * bs.m;
-  Uinv += blockIdx.z * bs.uinv;
-  __shared__ double2 U[NB][NB + 1];
-  __shared__ double2 Y[NB][MAXRHS_LU + 1];
+// Block substitutions with the stored inverses of the diagonal blocks, on right-hand sides Y
+// (column c of system z at Y + z * ystride + c * ldy, rows [0, n)), nrhs <= MAXRHS_LU:
+//   subst_block_kernel: Y_b = Tinv_b Y_b on the rows (k0, kb) (Uinv for U X = Y, Linv for L X = Y);
+//   subst_update_kernel: Y[r0 : r0 + nr] -= M[r0 : r0 + nr, k0 : k0 + kb] Y_b, one row per thread
+//     (rows above the block for U, below it for L).
+template <typename T>
+__global__ void __launch_bounds__(256) subst_block_kernel(const typename Cplx<T>::type* __restrict__ Tinv, size_t tstride,
+                                                          typename Cplx<T>::type* __restrict__ Y, int ldy,
+                                                          size_t ystride, int k0, int kb, int nrhs, int upper) {
+  typedef typename Cplx<T>::type C2;
+  __shared__ C2 U[NB][NB + 1];
+  __shared__ C2 Ys[NB][MAXRHS_LU + 1];
+  Tinv += blockIdx.z * tstride;
+  Y += blockIdx.z * ystride;
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) U[e >> 6][e & 63] = Uinv[e];
+  for (int e = tid; e < NB * NB; e += 256) U[e >> 6][e & 63] = Tinv[e];
   for (int e = tid; e < NB * nrhs; e += 256) {
     const int r = e & (NB - 1), c = e >> 6;
-    Y[r][c] = r < kb ? M[(size_t)(n + c) * lda + k0 + r] : make_double2(0.0, 0.0);
+    Ys[r][c] = r < kb ? Y[(size_t)c * ldy + k0 + r] : C2(0, 0);
   }
   __syncthreads();
   for (int e = tid; e < NB * nrhs; e += 256) {
     const int r = e & (NB - 1), c = e >> 6;
     if (r >= kb) continue;
-    double2 s = make_double2(0.0, 0.0);
-    for (int k = r; k < kb; ++k) {
-      const double2 u = U[r][k], y = Y[k][c];
+    C2 s = C2(0, 0);
+    const int lo = upper ? r : 0, hi = upper ? kb : r + 1;
+    for (int k = lo; k < hi; ++k) {
+      const C2 u = U[r][k], y = Ys[k][c];
       s.x += u.x * y.x - u.y * y.y;
       s.y += u.x * y.y + u.y * y.x;
     }
-    M[(size_t)(n + c) * lda + k0 + r] = s;
+    Y[(size_t)c * ldy + k0 + r] = s;
   }
 }
 
-// Y[0:k0] -= U[0:k0, k0:k0+kb] X_b: one row per thread, the right-hand sides in chunks of 8.
-__global__ void __launch_bounds__(64) backsolve_update_kernel(double2* __restrict__ M, int n, int lda, int k0, int kb,
-                                                              int nrhs, BatchStrides bs) {
-  M += blockIdx.z * bs.m;
-  __shared__ double2 Xb[NB][MAXRHS_LU];
+template <typename T>
+__global__ void __launch_bounds__(64) subst_update_kernel(const typename Cplx<T>::type* __restrict__ M, int lda,
+                                                          size_t mstride, typename Cplx<T>::type* __restrict__ Y,
+                                                          int ldy, size_t ystride, int k0, int kb, int r0, int nr,
+                                                          int nrhs) {
+  typedef typename Cplx<T>::type C2;
+  __shared__ C2 Xb[NB][MAXRHS_LU];
+  M += blockIdx.z * mstride;
+  Y += blockIdx.z * ystride;
   const int tid = threadIdx.x;
   for (int e = tid; e < NB * nrhs; e += 64) {
     const int c = e & (NB - 1), r = e >> 6;
-    Xb[c][r] = c < kb ? M[(size_t)(n + r) * lda + k0 + c] : make_double2(0.0, 0.0);
+    Xb[c][r] = c < kb ? Y[(size_t)r * ldy + k0 + c] : C2(0, 0);
   }
   __syncthreads();
-  const int row = blockIdx.x * 64 + tid;
-  if (row >= k0) return;
-  for (int r0 = 0; r0 < nrhs; r0 += 8) {
-    double2 acc[8];
+  const int i = blockIdx.x * 64 + tid;
+  if (i >= nr) return;
+  const int row = r0 + i;
+  for (int q0 = 0; q0 < nrhs; q0 += 8) {
+    C2 acc[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < 8; ++q) acc[q] = C2(0, 0);
     for (int c = 0; c < kb; ++c) {
-      const double2 u = M[(size_t)(k0 + c) * lda + row];
+      const C2 u = M[(size_t)(k0 + c) * lda + row];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const double2 x = Xb[c][min(r0 + q, MAXRHS_LU - 1)];
+        const C2 x = Xb[c][min(q0 + q, MAXRHS_LU - 1)];
         acc[q].x = fma(u.x, x.x, fma(-u.y, x.y, acc[q].x));
         acc[q].y = fma(u.x, x.y, fma(u.y, x.x, acc[q].y));
       }
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      if (r0 + q >= nrhs) break;
-      double2* b = M + (size_t)(n + r0 + q) * lda + row;
-      double2 v = *b;
+      if (q0 + q >= nrhs) break;
+      C2* b = Y + (size_t)(q0 + q) * ldy + row;
+      C2 v = *b;
       v.x -= acc[q].x;
       v.y -= acc[q].y;
       *b = v;
@@ -622,17 +643,19 @@ __global__ void fill_i32(int* p, size_t n, int v) {
 }
 
 // ------------------------------------------------------------------ host driver
+template <typename T>
 struct LuWorkspace {
+  typedef typename Cplx<T>::type C2;
   int n = 0, batch = 0;
-  TsluWork w{};
+  TsluWork w{};              // the tournament (FP64 only)
   int* ipiv = nullptr;       // [batch][n]
   int* flags = nullptr;      // [batch][panels]: 1 if the panel needed interchanges
   int* zp_fast = nullptr;    // [batch][panels]: zero pivot found without interchanges
-  double2* uinv = nullptr;   // [batch][panels]: U11^-1 (NB x NB), for the back substitution
-  double2* linv = nullptr;   // [batch]
-  double2* udiag = nullptr;  // [batch]
-  double2* save = nullptr;   // the panel before the attempt without interchanges (pivoted path)
-  double2* backup = nullptr; // [A | B] of a single solve, restored for the pivoted path
+  C2* uinv = nullptr;        // [batch][panels]: U11^-1 (NB x NB), for the back substitution
+  C2* linv = nullptr;        // [batch][panels]: L11^-1, for the forward substitution
+  C2* udiag = nullptr;       // [batch]
+  C2* save = nullptr;        // the panel before the attempt without interchanges (pivoted path)
+  C2* backup = nullptr;      // [A | B] of a single solve, restored for the pivoted path
   size_t backup_elems = 0;
   void release() {
     hipFree(w.cand);
@@ -655,39 +678,40 @@ struct LuWorkspace {
     const int panels = (n + NB - 1) / NB;
     return BatchStrides{0, panels, n, (size_t)panels * NB * NB};
   }
-};
-
-static void lu_workspace(LuWorkspace& ws, int n, int batch) {
-  if (ws.n >= n && ws.batch >= batch && ws.ipiv) return;
-  n = std::max(n, ws.n);
-  batch = std::max(batch, std::max(ws.batch, 1));
-  ws.release();
-  int leaves = (n + LEAF - 1) / LEAF;
-  int levels = 1, nodes = leaves;
-  while (nodes > 1) {
-    nodes = (nodes + FAN - 1) / FAN;
-    ++levels;
+  void ensure(int n_, int batch_) {
+    if (n >= n_ && batch >= batch_ && ipiv) return;
+    n_ = std::max(n_, n);
+    batch_ = std::max(batch_, std::max(batch, 1));
+    release();
+    const int panels = (n_ + NB - 1) / NB;
+    if (std::is_same<T, double>::value) {
+      int leaves = (n_ + LEAF - 1) / LEAF;
+      int levels = 1, nodes = leaves;
+      while (nodes > 1) {
+        nodes = (nodes + FAN - 1) / FAN;
+        ++levels;
+      }
+      w.levels = levels;
+      w.max_nodes = leaves;
+      const size_t slots = (size_t)levels * leaves;
+      HBG_HIP(hipMalloc(&w.cand, slots * NB * sizeof(int)));
+      HBG_HIP(hipMalloc(&w.ccount, slots * sizeof(int)));
+      HBG_HIP(hipMalloc(&w.ctr, slots * sizeof(unsigned)));
+      HBG_HIP(hipMalloc(&w.top, (size_t)NB * NB * sizeof(double2)));
+      HBG_HIP(hipMalloc(&w.winners, NB * sizeof(int)));
+      HBG_HIP(hipMalloc(&w.zero_pivot, sizeof(int)));
+      HBG_HIP(hipMalloc(&save, (size_t)n_ * NB * sizeof(C2)));
+    }
+    HBG_HIP(hipMalloc(&ipiv, (size_t)batch_ * n_ * sizeof(int)));
+    HBG_HIP(hipMalloc(&flags, (size_t)batch_ * panels * sizeof(int)));
+    HBG_HIP(hipMalloc(&zp_fast, (size_t)batch_ * panels * sizeof(int)));
+    HBG_HIP(hipMalloc(&uinv, (size_t)batch_ * panels * NB * NB * sizeof(C2)));
+    HBG_HIP(hipMalloc(&linv, (size_t)batch_ * panels * NB * NB * sizeof(C2)));
+    HBG_HIP(hipMalloc(&udiag, (size_t)batch_ * NB * sizeof(C2)));
+    n = n_;
+    batch = batch_;
   }
-  const int panels = (n + NB - 1) / NB;
-  ws.w.levels = levels;
-  ws.w.max_nodes = leaves;
-  const size_t slots = (size_t)levels * leaves;
-  HBG_HIP(hipMalloc(&ws.w.cand, slots * NB * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.w.ccount, slots * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.w.ctr, slots * sizeof(unsigned)));
-  HBG_HIP(hipMalloc(&ws.w.top, (size_t)NB * NB * sizeof(double2)));
-  HBG_HIP(hipMalloc(&ws.w.winners, NB * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.w.zero_pivot, sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.ipiv, (size_t)batch * n * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.flags, (size_t)batch * panels * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.zp_fast, (size_t)batch * panels * sizeof(int)));
-  HBG_HIP(hipMalloc(&ws.uinv, (size_t)batch * panels * NB * NB * sizeof(double2)));
-  HBG_HIP(hipMalloc(&ws.linv, (size_t)batch * NB * NB * sizeof(double2)));
-  HBG_HIP(hipMalloc(&ws.udiag, (size_t)batch * NB * sizeof(double2)));
-  HBG_HIP(hipMalloc(&ws.save, (size_t)n * NB * sizeof(double2)));
-  ws.n = n;
-  ws.batch = batch;
-}
+};
 
 static int g_gemm_variant = 1;
 static int g_lu_depth = 4;
@@ -696,8 +720,9 @@ int lu_gemm_variant() { return g_gemm_variant; }
 void lu_set_depth(int q) { g_lu_depth = q; }
 int lu_depth() { return g_lu_depth; }
 
-static LuWorkspace& workspace() {
-  static thread_local LuWorkspace ws;
+template <typename T>
+static LuWorkspace<T>& workspace() {
+  static thread_local LuWorkspace<T> ws;
   return ws;
 }
 
@@ -728,8 +753,9 @@ struct GemmTimer {
 // M[r0 + i][c0 + j] -= sum_k M[r0 + i][kf + k] M[kf + k][c0 + j] (i < nrows, j < ncolsC, k < kk)
 // on `batch` matrices, on the matrix cores; timed when stats are kept. Variant 1: 64 x 64 tiles,
 // 2: 128 x 64.
-void gemm_sub(double2* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk, int batch, size_t mstride,
-              hipStream_t stream, GemmTimer& gt) {
+template <typename T>
+void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk, int batch,
+              size_t mstride, hipStream_t stream, GemmTimer& gt) {
   if (nrows <= 0 || ncolsC <= 0 || kk <= 0) return;
   hipEvent_t g0 = nullptr;
   if (gt.stats) {
@@ -737,10 +763,10 @@ void gemm_sub(double2* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf
     HBG_HIP(hipEventRecord(g0, stream));
   }
   if (g_gemm_variant == 2)
-    gemm_mfma_kernel<double, 128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+    gemm_mfma_kernel<T, 128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
         M, lda, r0, nrows, c0, ncolsC, kf, kk, GemmBatch{mstride});
   else
-    gemm_mfma_kernel<double, 64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+    gemm_mfma_kernel<T, 64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
         M, lda, r0, nrows, c0, ncolsC, kf, kk, GemmBatch{mstride});
   if (gt.stats) {
     hipEvent_t g1 = gt.ep->next();
@@ -751,20 +777,15 @@ void gemm_sub(double2* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf
   }
 }
 
-// A22 -= L21 U12 for the panel (k0, kb): rows and columns from k0 + kb
-void trailing_update(double2* M, int n, int lda, int ncols, int k0, int kb, int batch, BatchStrides bs,
-                     hipStream_t stream, GemmTimer& gt) {
-  gemm_sub(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, batch, bs.m, stream, gt);
-}
-
 // Every panel of every matrix of the batch without interchanges: the diagonal block,
 // L21 = A21 U11^-1 (both checking that partial pivoting would make no interchange; the panel's
 // flag is set where it would) and U12 = L11^-1 A12. The panels go in blocks of g_lu_depth: before
 // a panel, the earlier panels of its block update its columns (rows from the panel down) and its
 // rows (columns right of it); after the block one trailing update with K = the block's width, so
 // the trailing matrix is read and written once per block rather than once per panel.
-void factor_optimistic(double2* M, int n, int lda, int ncols, int batch, BatchStrides bs, hipStream_t stream,
-                       LuWorkspace& ws, GemmTimer& gt) {
+template <typename T>
+void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int batch, BatchStrides bs,
+                       hipStream_t stream, LuWorkspace<T>& ws, GemmTimer& gt) {
   const int Q = std::max(1, g_lu_depth);
   for (int K0 = 0; K0 < n; K0 += Q * NB) {
     const int qend = std::min(n, K0 + Q * NB);
@@ -773,65 +794,81 @@ void factor_optimistic(double2* M, int n, int lda, int ncols, int batch, BatchSt
       const int kb = min(NB, n - k0);
       const int m = n - k0;
       if (k0 > K0) {
-        gemm_sub(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, stream, gt);
-        gemm_sub(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, stream, gt);
+        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, stream, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, stream, gt);
       }
-      double2* uinv = ws.uinv + (size_t)p * NB * NB;
-      diag_kernel<<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
-                                                         uinv, ws.linv, ws.udiag, bs);
+      const size_t po = (size_t)p * NB * NB;
+      diag_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
+                                                            ws.uinv + po, ws.linv + po, ws.udiag, bs);
       const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
       if (rb + cb > 0)
-        apply_kernel<<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv,
-                                                                  ws.udiag, ws.flags + p, nullptr, bs);
+        apply_kernel<T><<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, ws.uinv + po,
+                                                                     ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs);
     }
-    gemm_sub(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, stream, gt);
+    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, stream, gt);
   }
 }
 
-// One matrix, panel by panel: the attempt without interchanges, then, gated by the panel's flag,
-// the saved panel restored and pivoted by the tournament. The result of a panel without a flag is
-// the one partial pivoting gives.
-void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, LuWorkspace& ws, GemmTimer& gt) {
+// One FP64 matrix, panel by panel: the attempt without interchanges, then, gated by the panel's
+// flag, the saved panel restored and pivoted by the tournament. The result of a panel without a
+// flag is the one partial pivoting gives.
+void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, LuWorkspace<double>& ws,
+                    GemmTimer& gt) {
   const size_t slots = (size_t)ws.w.levels * ws.w.max_nodes;
   const BatchStrides bs = ws.strides();
   for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
     const int kb = min(NB, n - k0);
     const int m = n - k0;
     int* flag = ws.flags + p;
-    double2* uinv = ws.uinv + (size_t)p * NB * NB;
+    const size_t po = (size_t)p * NB * NB;
+    double2* uinv = ws.uinv + po;
+    double2* linv = ws.linv + po;
     const unsigned pe = (unsigned)(((size_t)m * kb + 255) / 256);
     panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 0, flag);
-    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag, bs);
+    diag_kernel<double><<<1, 256, 0, stream>>>(M, lda, k0, kb, 0, flag, ws.zp_fast + p, ws.ipiv, uinv, linv, ws.udiag,
+                                               bs);
     const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
     if (rb > 0)
-      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, flag, nullptr, bs);
+      apply_kernel<double><<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, linv, ws.udiag, flag, nullptr,
+                                                   bs);
     panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 1, flag);
     fill_u32<<<(unsigned)((slots + 255) / 256), 256, 0, stream>>>(ws.w.ctr, slots, 0u);
     tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w, flag);
     ipiv_kernel<<<1, 64, 0, stream>>>(k0, kb, ws.w.winners, ws.ipiv, flag);
     swap_kernel<<<(ncols + 255) / 256, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.ipiv, ws.w.top, flag);
-    diag_kernel<<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, ws.linv, ws.udiag, bs);
+    diag_kernel<double><<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, linv, ws.udiag,
+                                               bs);
     if (rb > 0)
-      apply_kernel<<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, ws.linv, ws.udiag, nullptr, flag, bs);
+      apply_kernel<double><<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, linv, ws.udiag, nullptr, flag,
+                                                   bs);
     panel_finish_kernel<<<1, 64, 0, stream>>>(flag, ws.zp_fast + p, ws.w.zero_pivot);
     if (cb > 0)
-      apply_kernel<<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, ws.linv, ws.udiag, nullptr, nullptr, bs);
-    trailing_update(M, n, lda, ncols, k0, kb, 1, bs, stream, gt);
+      apply_kernel<double><<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, linv, ws.udiag, nullptr,
+                                                   nullptr, bs);
+    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, stream, gt);
   }
 }
 
-// U X = Y on the augmented columns of every matrix of the batch, at most MAXRHS_LU of them at a
-// time (the kernels take the first column of the chunk as their `n`).
-void back_substitute(double2* M, int n, int lda, int nrhs, int batch, BatchStrides bs, hipStream_t stream,
-                     LuWorkspace& ws) {
+// U X = Y (upper) or L X = Y (lower, unit) on right-hand sides Y (ldy, ystride) of every system of
+// the batch, at most MAXRHS_LU columns at a time.
+template <typename T>
+void substitute(const typename Cplx<T>::type* M, int n, int lda, size_t mstride, typename Cplx<T>::type* Y, int ldy,
+                size_t ystride, int nrhs, int batch, bool upper, hipStream_t stream, LuWorkspace<T>& ws) {
+  const BatchStrides bs = ws.strides();
+  const typename Cplx<T>::type* Tinv = upper ? ws.uinv : ws.linv;
+  const int panels = (n + NB - 1) / NB;
   for (int r0 = 0; r0 < nrhs; r0 += MAXRHS_LU) {
     const int nr = min(MAXRHS_LU, nrhs - r0);
-    for (int k0 = ((n - 1) / NB) * NB, p = (n - 1) / NB; k0 >= 0; k0 -= NB, --p) {
-      const int kb = min(NB, n - k0);
-      backsolve_block_kernel<<<dim3(1, 1, batch), 256, 0, stream>>>(M, n + r0, lda, k0, kb,
-                                                                    ws.uinv + (size_t)p * NB * NB, nr, bs);
-      if (k0 > 0)
-        backsolve_update_kernel<<<dim3((k0 + 63) / 64, 1, batch), 64, 0, stream>>>(M, n + r0, lda, k0, kb, nr, bs);
+    typename Cplx<T>::type* Yc = Y + (size_t)r0 * ldy;
+    for (int i = 0; i < panels; ++i) {
+      const int p = upper ? panels - 1 - i : i;
+      const int k0 = p * NB, kb = min(NB, n - k0);
+      subst_block_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(Tinv + (size_t)p * NB * NB, bs.uinv, Yc, ldy,
+                                                                   ystride, k0, kb, nr, upper ? 1 : 0);
+      const int ur0 = upper ? 0 : k0 + kb, unr = upper ? k0 : n - k0 - kb;
+      if (unr > 0)
+        subst_update_kernel<T><<<dim3((unr + 63) / 64, 1, batch), 64, 0, stream>>>(M, lda, mstride, Yc, ldy, ystride,
+                                                                                  k0, kb, ur0, unr, nr);
     }
   }
 }
@@ -851,15 +888,9 @@ void finish_stats(LuStats* stats, EventPool& ep, hipEvent_t t0, GemmTimer& gt, h
   gt.ev.clear();
 }
 
-}  // namespace
-
-// Factor and solve `batch` systems [A | B] (n x (n + nrhs), leading dimension lda, matrix b at
-// M + b * mstride) without interchanges, checking on every panel that partial pivoting would make
-// none. status[b]: 0 solved (the partially pivoted result), k > 0 zero pivot k (1-based; the
-// column is zero on and below the diagonal, as partial pivoting would find it), -1 a panel needs
-// interchanges: the matrix is left overwritten and must be rebuilt and given to lu_solve_pivoted.
-void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
-                      LuStats* stats, int* status) {
+template <typename T>
+void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
+                   LuStats* stats, int* status) {
   EventPool& ep = events();
   ep.used = 0;
   GemmTimer gt{&ep, stats, {}};
@@ -868,8 +899,8 @@ void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int
     t0 = ep.next();
     HBG_HIP(hipEventRecord(t0, stream));
   }
-  LuWorkspace& ws = workspace();
-  lu_workspace(ws, n, batch);
+  LuWorkspace<T>& ws = workspace<T>();
+  ws.ensure(n, batch);
   BatchStrides bs = ws.strides();
   bs.m = mstride;
   const int ncols = n + nrhs;
@@ -877,8 +908,8 @@ void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int
   const size_t pf = (size_t)bs.flag * batch;
   fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.flags, pf, 0);
   fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.zp_fast, pf, 0x7fffffff);
-  factor_optimistic(M, n, lda, ncols, batch, bs, stream, ws, gt);
-  if (nrhs > 0) back_substitute(M, n, lda, nrhs, batch, bs, stream, ws);
+  factor_optimistic<T>(M, n, lda, ncols, batch, bs, stream, ws, gt);
+  if (nrhs > 0) substitute<T>(M, n, lda, mstride, M + (size_t)n * lda, lda, mstride, nrhs, batch, true, stream, ws);
   HBG_HIP(hipGetLastError());
   std::vector<int> flags(pf), zps(pf);
   HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, pf * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -896,6 +927,35 @@ void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int
   }
 }
 
+}  // namespace
+
+// Factor and solve `batch` systems [A | B] (n x (n + nrhs), leading dimension lda, matrix b at
+// M + b * mstride) without interchanges, checking on every panel that partial pivoting would make
+// none. status[b]: 0 solved (the partially pivoted result), k > 0 zero pivot k (1-based; the
+// column is zero on and below the diagonal, as partial pivoting would find it), -1 a panel needs
+// interchanges: the matrix is left overwritten and must be rebuilt and given to lu_solve_pivoted.
+void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
+                      LuStats* stats, int* status) {
+  solve_batched<double>(M, mstride, batch, n, lda, nrhs, stream, stats, status);
+}
+
+// The same in complex64 (FP32 arithmetic, FP32 MFMA); the factors and the inverses of their
+// diagonal blocks stay for lu_resolve32 until the next call.
+void lu_solve_batched32(float2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
+                        LuStats* stats, int* status) {
+  solve_batched<float>(M, mstride, batch, n, lda, nrhs, stream, stats, status);
+}
+
+// X = U^-1 L^-1 X with the factors of the last lu_solve_batched32 (systems without interchanges):
+// column c of system b at X + b * xstride + c * ldx.
+void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, float2* X, int ldx, size_t xstride,
+                  int nrhs, hipStream_t stream) {
+  LuWorkspace<float>& ws = workspace<float>();
+  substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, false, stream, ws);
+  substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, true, stream, ws);
+  HBG_HIP(hipGetLastError());
+}
+
 // One system with partial pivoting where a panel needs it (tournament pivoting over several
 // leaves); returns the first zero pivot (1-based) or 0.
 int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats) {
@@ -907,15 +967,15 @@ int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, L
     t0 = ep.next();
     HBG_HIP(hipEventRecord(t0, stream));
   }
-  LuWorkspace& ws = workspace();
-  lu_workspace(ws, n, 1);
+  LuWorkspace<double>& ws = workspace<double>();
+  ws.ensure(n, 1);
   const int ncols = n + nrhs;
   const int panels = (n + NB - 1) / NB;
   fill_i32<<<1, 64, 0, stream>>>(ws.w.zero_pivot, 1, 0x7fffffff);
   fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
   fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
   factor_pivoted(M, n, lda, ncols, stream, ws, gt);
-  if (nrhs > 0) back_substitute(M, n, lda, nrhs, 1, ws.strides(), stream, ws);
+  if (nrhs > 0) substitute<double>(M, n, lda, 0, M + (size_t)n * lda, lda, 0, nrhs, 1, true, stream, ws);
   HBG_HIP(hipGetLastError());
   int zp = 0;
   std::vector<int> flags(panels);
@@ -932,8 +992,8 @@ int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, L
 // restore it for the pivoted path when a panel needs interchanges. Returns the first zero pivot
 // (1-based) or 0.
 int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats) {
-  LuWorkspace& ws = workspace();
-  lu_workspace(ws, n, 1);
+  LuWorkspace<double>& ws = workspace<double>();
+  ws.ensure(n, 1);
   const size_t whole = (size_t)lda * (n + nrhs);
   if (ws.backup_elems < whole) {
     hipFree(ws.backup);
@@ -950,6 +1010,9 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
   return lu_solve_pivoted(M, n, lda, nrhs, stream, stats);
 }
 
-void lu_release() { workspace().release(); }
+void lu_release() {
+  workspace<double>().release();
+  workspace<float>().release();
+}
 
 }  // namespace hbg

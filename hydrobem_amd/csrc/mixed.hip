@@ -1,0 +1,202 @@
+// The mixed-precision solve: the FP64-assembled system [A | b] is factored in complex64 on the
+// matrix cores (lu.hip) and the solution recovered by iterative refinement in FP64,
+//   x0 = (LU)^-1 b (FP32),   r = b - A x (FP64, against the FP64 matrix),   x += (LU)^-1 r,
+// until the normwise backward error |r| / (|A| |x| + |b|) (infinity norms) is at most the
+// tolerance. No counterpart in the reference (whose single-precision mode factors in complex64
+// and stops there, linalg.rs:321-328); the kernels here are the casts, the residual with its
+// norms, and the update.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/hydrobem_amd.h"
+#include "hbg_internal.h"
+
+namespace hbg {
+
+namespace {
+
+constexpr int RES_THREADS = 256;
+constexpr int RES_MAXR = 32;
+constexpr int RES_JC = 32;  // columns of A per LDS stage
+
+__global__ void cast_kernel(const double2* __restrict__ src, size_t sstride, float2* __restrict__ dst, size_t dstride,
+                            size_t count) {
+  src += blockIdx.z * sstride;
+  dst += blockIdx.z * dstride;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 v = src[i];
+    dst[i] = float2((float)v.x, (float)v.y);
+  }
+}
+
+// X (n x R, ld n) = the complex64 columns [n, n + R) of M32
+__global__ void widen_kernel(const float2* __restrict__ M32, size_t mstride, int n, double2* __restrict__ X,
+                             size_t xstride, int R) {
+  M32 += blockIdx.z * mstride + (size_t)n * n;
+  X += blockIdx.z * xstride;
+  const size_t count = (size_t)n * R;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const float2 v = M32[i];
+    X[i] = double2((double)v.x, (double)v.y);
+  }
+}
+
+// D = (complex64) (b - A X) for the system [A | b] (n x (n + R), ld n) in FP64, one row per
+// thread, X staged through LDS; per workgroup the maxima over its rows of max_c |r_ic|,
+// sum_j |a_ij|, max_c |x_ic|, max_c |b_ic| (|re| + |im|) go to part[z][block][4].
+__global__ void __launch_bounds__(RES_THREADS) residual_kernel(const double2* __restrict__ M, size_t mstride, int n,
+                                                               const double2* __restrict__ X, size_t xstride, int R,
+                                                               float2* __restrict__ D, size_t dstride,
+                                                               double* __restrict__ part) {
+  __shared__ double2 xs[RES_JC][RES_MAXR];
+  __shared__ double red[4][RES_THREADS / 64];
+  M += blockIdx.z * mstride;
+  X += blockIdx.z * xstride;
+  D += blockIdx.z * dstride;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * RES_THREADS + tid;
+  const bool live = i < n;
+  double2 acc[RES_MAXR];
+#pragma unroll
+  for (int c = 0; c < RES_MAXR; ++c) acc[c] = double2(0.0, 0.0);
+  double rowsum = 0.0;
+  for (int j0 = 0; j0 < n; j0 += RES_JC) {
+    const int jn = min(RES_JC, n - j0);
+    __syncthreads();
+    for (int e = tid; e < RES_JC * RES_MAXR; e += RES_THREADS) {
+      const int jj = e % RES_JC, c = e / RES_JC;
+      xs[jj][c] = (jj < jn && c < R) ? X[(size_t)c * n + j0 + jj] : double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (live) {
+      for (int jj = 0; jj < jn; ++jj) {
+        const double2 a = M[(size_t)(j0 + jj) * n + i];
+        rowsum += fabs(a.x) + fabs(a.y);
+#pragma unroll
+        for (int c = 0; c < RES_MAXR; ++c) {
+          if (c < R) {
+            const double2 x = xs[jj][c];
+            acc[c].x = fma(a.x, x.x, fma(-a.y, x.y, acc[c].x));
+            acc[c].y = fma(a.x, x.y, fma(a.y, x.x, acc[c].y));
+          }
+        }
+      }
+    }
+  }
+  double rmax = 0.0, xmax = 0.0, bmax = 0.0;
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < RES_MAXR; ++c) {
+      if (c < R) {
+        const double2 b = M[(size_t)(n + c) * n + i];
+        const double2 x = X[(size_t)c * n + i];
+        const double rx = b.x - acc[c].x, ry = b.y - acc[c].y;
+        D[(size_t)c * n + i] = float2((float)rx, (float)ry);
+        rmax = fmax(rmax, fabs(rx) + fabs(ry));
+        xmax = fmax(xmax, fabs(x.x) + fabs(x.y));
+        bmax = fmax(bmax, fabs(b.x) + fabs(b.y));
+      }
+    }
+  }
+  double v[4] = {rmax, rowsum, xmax, bmax};
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    for (int off = 32; off >= 1; off >>= 1) v[q] = fmax(v[q], __shfl_xor(v[q], off));
+  if ((tid & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[q][tid >> 6] = v[q];
+  __syncthreads();
+  if (tid < 4) {
+    double m = 0.0;
+    for (int w = 0; w < RES_THREADS / 64; ++w) m = fmax(m, red[tid][w]);
+    part[((size_t)blockIdx.z * gridDim.x + blockIdx.x) * 4 + tid] = m;
+  }
+}
+
+// X += D (the correction, complex64)
+__global__ void update_kernel(double2* __restrict__ X, size_t xstride, const float2* __restrict__ D, size_t dstride,
+                              size_t count) {
+  X += blockIdx.z * xstride;
+  D += blockIdx.z * dstride;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const float2 d = D[i];
+    double2 x = X[i];
+    x.x += (double)d.x;
+    x.y += (double)d.y;
+    X[i] = x;
+  }
+}
+
+// the columns [n, n + R) of M (FP64) = X
+__global__ void store_kernel(double2* __restrict__ M, size_t mstride, int n, const double2* __restrict__ X,
+                             size_t xstride, int R) {
+  M += blockIdx.z * mstride + (size_t)n * n;
+  X += blockIdx.z * xstride;
+  const size_t count = (size_t)n * R;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    M[i] = X[i];
+}
+
+unsigned grid_for(size_t count) { return (unsigned)std::min<size_t>((count + 255) / 256, 4096); }
+
+}  // namespace
+
+void lu_solve_batched32(float2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
+                        LuStats* stats, int* status);
+void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, float2* X, int ldx, size_t xstride,
+                  int nrhs, hipStream_t stream);
+
+int residual_blocks(int n) { return (n + RES_THREADS - 1) / RES_THREADS; }
+
+// Solve the `batch` FP64 systems M64 (n x (n + R), ld n, stride n (n + R)) in mixed precision:
+// M32 (same shape, complex64) and D (n x R complex64), X (n x R complex128), part
+// (batch x residual_blocks(n) x 4) are workspaces. On return the solution is in the columns
+// [n, n + R) of M64 for the systems with status 0; status -1: the complex64 factorization needed
+// interchanges or the refinement did not reach the tolerance in max_iter steps (solve it in FP64);
+// k > 0: zero pivot k. iters[b], eta[b]: refinement steps and the final backward error.
+void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, int batch, int n, int R,
+                         double tol, int max_iter, hipStream_t stream, LuStats* stats, int* status, int* iters,
+                         double* eta) {
+  if (R > RES_MAXR) throw Error{HBG_ERR_UNSUPPORTED, "mixed precision: more than 32 right-hand sides"};
+  const size_t ms = (size_t)n * (n + R), xs = (size_t)n * R;
+  const int nblk = residual_blocks(n);
+  cast_kernel<<<dim3(grid_for(ms), 1, batch), 256, 0, stream>>>(M64, ms, M32, ms, ms);
+  lu_solve_batched32(M32, ms, batch, n, n, R, stream, stats, status);
+  widen_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M32, ms, n, X, xs, R);
+  std::vector<double> hp((size_t)batch * nblk * 4);
+  for (int b = 0; b < batch; ++b) {
+    iters[b] = 0;
+    eta[b] = INFINITY;
+  }
+  bool open = false;
+  for (int b = 0; b < batch; ++b) open = open || status[b] == 0;
+  for (int it = 0; open; ++it) {
+    residual_kernel<<<dim3(nblk, 1, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, D, xs, part);
+    HBG_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HBG_HIP(hipStreamSynchronize(stream));
+    open = false;
+    for (int b = 0; b < batch; ++b) {
+      if (status[b] != 0) continue;
+      double v[4] = {0, 0, 0, 0};
+      for (int k = 0; k < nblk; ++k)
+        for (int q = 0; q < 4; ++q) v[q] = std::max(v[q], hp[((size_t)b * nblk + k) * 4 + q]);
+      const double den = v[1] * v[2] + v[3];
+      eta[b] = den > 0.0 ? v[0] / den : 0.0;
+      if (!(eta[b] <= tol) && iters[b] < max_iter) open = true;
+    }
+    if (!open) break;
+    // correct every system (those already converged take one more harmless step)
+    lu_resolve32(M32, ms, batch, n, n, D, n, xs, R, stream);
+    update_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(X, xs, D, xs, xs);
+    for (int b = 0; b < batch; ++b)
+      if (status[b] == 0 && !(eta[b] <= tol)) ++iters[b];
+  }
+  for (int b = 0; b < batch; ++b)
+    if (status[b] == 0 && !(eta[b] <= tol)) status[b] = -1;
+  store_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M64, ms, n, X, xs, R);
+  HBG_HIP(hipGetLastError());
+}
+
+}  // namespace hbg

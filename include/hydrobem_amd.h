@@ -52,11 +52,15 @@ enum { HBG_STAGE_STATICS = 0, HBG_STAGE_ASSEMBLY = 1, HBG_STAGE_LU = 2, HBG_STAG
 typedef int32_t (*HbgProgressFn)(void* user, int32_t stage, int32_t done, int32_t total);
 
 /* Precision of a solve. HBG_PRECISION_FP64: complex128 assembly and LU on the GPU (the
- * reference's double mode, env_hooks.rs:51-56). */
-enum { HBG_PRECISION_FP64 = 0 };
+ * reference's double mode, env_hooks.rs:51-56). HBG_PRECISION_MIXED (no counterpart in the
+ * reference): the FP64-assembled system factored in complex64 on the matrix cores (FP32
+ * accumulation), the solution refined in FP64 against the FP64 matrix until the normwise backward
+ * error |b - A x| / (|A| |x| + |b|) (infinity norms) is at most 1e-12; a frequency whose complex64
+ * factorization needs row interchanges or does not converge in 10 steps is solved in FP64. */
+enum { HBG_PRECISION_FP64 = 0, HBG_PRECISION_MIXED = 1 };
 
 typedef struct HbgOptions {
-  int32_t precision;     /* HBG_PRECISION_FP64 */
+  int32_t precision;     /* HBG_PRECISION_FP64 or HBG_PRECISION_MIXED */
   int32_t logarithmic;   /* analytic log-singularity integral (document_schema.rs:218, default 1) */
   double log_tolerance;  /* document_schema.rs:219, default 1e-4 */
 } HbgOptions;
@@ -151,7 +155,9 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);
  *   (factorization + substitution), 4 LU trailing-update GEMM ms, 5 their algorithmic flops
  *   (8 m n k each), 6 GEMM launches, 7 forces ms, 8 frequencies, 9 LU algorithmic flops
  *   (8 N^3 / 3 each), 10 assembly algorithmic bytes (N^2 (32 + 2 x 16) each), 11 statics runs,
- *   12 LU panels that needed row interchanges (the pivoting fallback). */
+ *   12 LU panels that needed row interchanges (the pivoting fallback), 13 refinement steps of the
+ *   mixed-precision solves, 14 their largest final backward error, 15 frequencies of a
+ *   mixed-precision solve solved in FP64 instead. */
 #define HBG_PROFILE_COUNTERS 16
 HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable);
 HbgStatus hbg_profile_read(HbgSession* s, double* out, int32_t n);
