@@ -40,7 +40,16 @@ PROGRESS = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, cty
 
 
 class Options(ctypes.Structure):
-    _fields_ = [("precision", ctypes.c_int32), ("logarithmic", ctypes.c_int32), ("log_tolerance", ctypes.c_double)]
+    _fields_ = [("precision", ctypes.c_int32), ("logarithmic", ctypes.c_int32), ("log_tolerance", ctypes.c_double),
+                ("surrogate", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class SurrogateTraining(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("n_train", ctypes.c_int32), ("n_holdout", ctypes.c_int32),
+                ("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("learning_rate", ctypes.c_double)]
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 _lib = None
@@ -65,6 +74,14 @@ SIGNATURES = {
     "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "hbg_set_profiling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "hbg_surrogate_size": (ctypes.c_int32, []),
+    "hbg_surrogate_train": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(SurrogateTraining), _dp]),
+    "hbg_surrogate_set": (ctypes.c_int32, [ctypes.c_void_p, _fp, ctypes.c_int32]),
+    "hbg_surrogate_get": (ctypes.c_int32, [ctypes.c_void_p, _fp, ctypes.c_int32]),
+    "hbg_surrogate_eval": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, _dp]),
+    "hbg_finite_remainder": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                              _dp, _dp, _dp, _dp]),
+    "hbg_surrogate_gradient": (ctypes.c_int32, [ctypes.c_void_p, _fp, ctypes.c_int32, _fp, _fp, _fp, _dp]),
     "hbg_set_batch": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "hbg_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_get_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
@@ -103,8 +120,12 @@ def _p(a):
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
 
-def _options(logarithmic=True, log_tolerance=1e-4, precision=0) -> Options:
-    return Options(int(precision), int(bool(logarithmic)), float(log_tolerance))
+def _options(logarithmic=True, log_tolerance=1e-4, precision=0, surrogate=False) -> Options:
+    return Options(int(precision), int(bool(logarithmic)), float(log_tolerance), int(bool(surrogate)), 0)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 class Session:
@@ -147,9 +168,11 @@ class Session:
                                         _p(rc)))
         self.n_panels, self.n_bodies = n, nb
 
-    def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None, precision="fp64"):
+    def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None, precision="fp64",
+              surrogate=False):
         """The quantities of one solve (the reference's result names and axes). precision "fp64"
-        (the plain path) or "mixed" (complex64 LU on the matrix cores + FP64 refinement)."""
+        (the plain path) or "mixed" (complex64 LU on the matrix cores + FP64 refinement);
+        surrogate: the finite-depth evanescent remainder from the trained network."""
         w = np.ascontiguousarray(omegas, dtype=np.float64).reshape(-1)
         h = np.ascontiguousarray(headings_deg, dtype=np.float64).reshape(-1)
         F, H, D = len(w), len(h), 6 * self.n_bodies
@@ -160,7 +183,7 @@ class Session:
         SC = np.zeros((F, H, D), np.complex128)
         HK = np.zeros((F, D, H), np.complex128)
         cb = PROGRESS(lambda user, stage, done, total: int(progress(stage, done, total)) if progress else 1)
-        opt = _options(logarithmic, log_tolerance, PRECISIONS[precision])
+        opt = _options(logarithmic, log_tolerance, PRECISIONS[precision], surrogate)
         _check(library().hbg_solve(self._h, _p(w), F, _p(h), H, ctypes.byref(opt), _p(A), _p(B), _p(X), _p(FK),
                                    _p(SC), _p(HK), cb, None))
         return {"omegas": w, "headings": h, "added_mass": A, "radiation_damping": B,
@@ -175,6 +198,51 @@ class Session:
     def set_batch(self, max_batch: int):
         """Frequencies in flight in solve (0: as many as fit, at most 16)."""
         _check(library().hbg_set_batch(self._h, int(max_batch)))
+
+    # ---- the learned surrogate
+    @staticmethod
+    def surrogate_size() -> int:
+        return int(library().hbg_surrogate_size())
+
+    REPORT_KEYS = ("train_loss", "holdout_rms", "holdout_max_T", "holdout_max_TR", "holdout_max_Tz",
+                   "holdout_max_Tzeta", "data_seconds", "train_seconds", "steps")
+
+    def surrogate_train(self, n_train=2_000_000, n_holdout=100_000, epochs=4, batch=4096, learning_rate=2e-3,
+                        seed=1) -> dict:
+        t = SurrogateTraining(int(seed), int(n_train), int(n_holdout), int(epochs), int(batch), float(learning_rate))
+        rep = np.zeros(9)
+        _check(library().hbg_surrogate_train(self._h, ctypes.byref(t), _p(rep)))
+        return {k: float(rep[i]) for i, k in enumerate(self.REPORT_KEYS)}
+
+    def surrogate_set(self, params):
+        p = _f32(params)
+        _check(library().hbg_surrogate_set(self._h, p.ctypes.data_as(_fp), int(p.size)))
+
+    def surrogate_get(self) -> np.ndarray:
+        p = np.zeros(self.surrogate_size(), np.float32)
+        _check(library().hbg_surrogate_get(self._h, p.ctypes.data_as(_fp), int(p.size)))
+        return p
+
+    def surrogate_eval(self, points) -> np.ndarray:
+        """points [n][4] = (R/h, z/h, zeta/h, nu h) -> [n][4] = (h T, h^2 T_R, h^2 T_z, h^2 T_zeta)."""
+        x = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 4)
+        out = np.zeros_like(x)
+        _check(library().hbg_surrogate_eval(self._h, x.shape[0], _p(x), _p(out)))
+        return out
+
+    def finite_remainder(self, h, K, R, z, zeta) -> np.ndarray:
+        R, z, zeta = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (R, z, zeta))
+        out = np.zeros((R.size, 4))
+        _check(library().hbg_finite_remainder(self._h, float(h), float(K), R.size, _p(R), _p(z), _p(zeta), _p(out)))
+        return out
+
+    def surrogate_gradient(self, params, inputs, targets):
+        p, x, y = _f32(params), _f32(inputs), _f32(targets)
+        g = np.zeros_like(p)
+        loss = np.zeros(1)
+        _check(library().hbg_surrogate_gradient(self._h, p.ctypes.data_as(_fp), x.shape[0], x.ctypes.data_as(_fp),
+                                                y.ctypes.data_as(_fp), g.ctypes.data_as(_fp), _p(loss)))
+        return float(loss[0]), g
 
     TUNE_LU_GEMM = 0
     TUNE_LU_DEPTH = 1

@@ -11,6 +11,7 @@
 #include "../../include/hydrobem_amd.h"
 #include "green_fin_dev.h"
 #include "hbg_internal.h"
+#include "surrogate.h"
 
 namespace hbg {
 
@@ -49,6 +50,17 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
                          double tol, int max_iter, hipStream_t stream, LuStats* stats, int* status, int* iters,
                          double* eta);
 int residual_blocks(int n);
+void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
+                            const StaticD* st, double2* S, double2* A, int lda, const NnDevice& nn, int2* queue,
+                            unsigned* qcount, hipStream_t stream);
+void launch_remainder(const FinFreq* f, FinTables F, WaveTables T, int n, const double* R, const double* z,
+                      const double* zeta, double* out, hipStream_t stream);
+void nn_eval(const NnDevice& nn, const float4* X, int n, float4* Y, hipStream_t stream);
+double nn_gradient(const float* P_host, const float* X_host, const float* Y_host, int B, float* G_host,
+                   hipStream_t st);
+void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int batch, float lr0, const WaveTables& T,
+              const double* ktab, const double* fitm, const double* g16, std::vector<float>& P, double* report,
+              NnDevice& nn, hipStream_t st);
 void lu_set_gemm_variant(int v);
 int lu_gemm_variant();
 void lu_set_depth(int q);
@@ -273,6 +285,9 @@ struct HbgSession {
   DevBuf<float2> d_M32, d_D32;
   DevBuf<double2> d_X64;
   DevBuf<double> d_rpart;
+  NnDevice nn;          // the learned surrogate of the finite-depth evanescent remainder
+  DevBuf<int2> d_nnq;   // pairs left to the FP64 evaluator by the surrogate's assembly
+  DevBuf<unsigned> d_nnc;
   int max_batch = 0;  // frequencies in flight (0: as many as fit, at most 16)
   std::vector<hipEvent_t> bev;
   bool statics_valid = false;
@@ -369,7 +384,7 @@ FinTables fin_tables(HbgSession* s) {
 
 // S and A of the frequency K into S (ld N) and M (ld N): infinite depth, or finite depth with the
 // per-frequency tables built first; returns the wave number of the incident wave
-double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout) {
+double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, bool surrogate = false) {
   const int N = s->N;
   if (!std::isfinite(s->depth)) {
     launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, Mout, N, s->stream);
@@ -384,8 +399,15 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout) {
                     s->d_pcs.p, s->stream);
   s->d_ff.ensure(1);
   HBG_HIP(hipMemcpyAsync(s->d_ff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, s->stream));
-  launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, N,
-                      s->stream);
+  const double nuh = K * s->depth;
+  if (surrogate && s->nn.ready && nuh >= std::exp(NN_LNNU_LO) && nuh <= std::exp(NN_LNNU_HI)) {
+    s->d_nnq.ensure((size_t)N * N);
+    s->d_nnc.ensure(1);
+    launch_assemble_fin_nn(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, N,
+                           s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
+  } else
+    launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, N,
+                        s->stream);
   return f.k0;
 }
 
@@ -410,6 +432,7 @@ void check_options(const HbgOptions* o) {
   if (o && o->precision != HBG_PRECISION_FP64 && o->precision != HBG_PRECISION_MIXED)
     throw Error{HBG_ERR_UNSUPPORTED, "precision not supported"};
   if (o && !(o->log_tolerance >= 0.0)) throw Error{HBG_ERR_VALUE, "log_tolerance must be >= 0"};
+  if (o && o->surrogate != 0 && o->surrogate != 1) throw Error{HBG_ERR_VALUE, "surrogate must be 0 or 1"};
 }
 
 HbgOptions defaults(const HbgOptions* o) {
@@ -417,6 +440,8 @@ HbgOptions defaults(const HbgOptions* o) {
   d.precision = HBG_PRECISION_FP64;
   d.logarithmic = 1;
   d.log_tolerance = 1e-4;
+  d.surrogate = 0;
+  d.reserved = 0;
   return o ? *o : d;
 }
 
@@ -678,7 +703,7 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   s->d_M.ensure((size_t)N * N);
   const double K = omega * omega / s->grav;
   check_depth(s);
-  assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0, s->d_M.p);
+  assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0, s->d_M.p, o.surrogate != 0);
   std::vector<double2> hs((size_t)N * N), ha((size_t)N * N);
   HBG_HIP(hipMemcpyAsync(hs.data(), s->d_S.p, hs.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
   HBG_HIP(hipMemcpyAsync(ha.data(), s->d_M.p, ha.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
@@ -785,7 +810,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     const double w = omegas[fi], K = w * w / s->grav;
     double2* Mb = s->d_M.p + mstride * b;
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], s->stream));
-    const double kw = assemble_frequency(s, K, logc, Mb);
+    const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0);
     launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
                     s->d_phiI.p + hstride * b, s->d_dphiI.p + hstride * b, s->d_u.p, s->stream);
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], s->stream));
@@ -879,6 +904,142 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_CANCELLED, "cancelled"};
   return HBG_OK;
   GUARD_END("hbg_solve")
+}
+
+int32_t hbg_surrogate_size(void) { return NN_PARAMS; }
+
+HbgStatus hbg_surrogate_train(HbgSession* s, const HbgSurrogateTraining* t, double* report) {
+  if (!s || !t) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_surrogate_train");
+  if (t->n_train < t->batch || t->batch < 64 || t->n_holdout < 0 || t->epochs < 1 ||
+      !(t->learning_rate > 0.0) || t->n_train > (1 << 28) || t->n_holdout > (1 << 26))
+    return fail(HBG_ERR_VALUE, "training sizes out of range", "hbg_surrogate_train");
+  GUARD_BEGIN
+  use_device(s);
+  std::vector<float> P;
+  double rep[HBG_SURROGATE_REPORT] = {};
+  nn_train(t->seed, t->n_train, t->n_holdout, t->epochs, t->batch, (float)t->learning_rate, s->T, s->ktab, s->fitm,
+           s->d_gl.p + 16 * 32, P, rep, s->nn, s->stream);
+  if (report) std::memcpy(report, rep, sizeof(rep));
+  return HBG_OK;
+  GUARD_END("hbg_surrogate_train")
+}
+
+HbgStatus hbg_surrogate_set(HbgSession* s, const float* params, int32_t n) {
+  if (!s || !params) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_surrogate_set");
+  if (n != NN_PARAMS) return fail(HBG_ERR_VALUE, "expected hbg_surrogate_size() parameters", "hbg_surrogate_set");
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(params[i])) return fail(HBG_ERR_VALUE, "non-finite parameter", "hbg_surrogate_set");
+  GUARD_BEGIN
+  use_device(s);
+  s->nn.upload(params);
+  return HBG_OK;
+  GUARD_END("hbg_surrogate_set")
+}
+
+HbgStatus hbg_surrogate_get(HbgSession* s, float* params, int32_t n) {
+  if (!s || !params) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_surrogate_get");
+  if (!s->nn.ready) return fail(HBG_ERR_STATE, "no surrogate trained or set", "hbg_surrogate_get");
+  if (n != NN_PARAMS) return fail(HBG_ERR_VALUE, "expected hbg_surrogate_size() parameters", "hbg_surrogate_get");
+  std::memcpy(params, s->nn.params.data(), NN_PARAMS * sizeof(float));
+  return HBG_OK;
+}
+
+HbgStatus hbg_surrogate_eval(HbgSession* s, int32_t n, const double* in, double* out) {
+  if (!s || (n > 0 && (!in || !out))) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_surrogate_eval");
+  if (!s->nn.ready) return fail(HBG_ERR_STATE, "no surrogate trained or set", "hbg_surrogate_eval");
+  if (n < 0) return fail(HBG_ERR_VALUE, "n must be >= 0", "hbg_surrogate_eval");
+  for (int i = 0; i < n; ++i)
+    if (!(in[4 * i] > 0.0) || !(in[4 * i + 3] > 0.0) || !std::isfinite(in[4 * i]) || !std::isfinite(in[4 * i + 3]))
+      return fail(HBG_ERR_VALUE, "point " + std::to_string(i) + ": R/h and nu h must be finite and > 0",
+                  "hbg_surrogate_eval");
+  if (n == 0) return HBG_OK;
+  GUARD_BEGIN
+  use_device(s);
+  std::vector<float4> X(n), Y(n);
+  for (int i = 0; i < n; ++i) X[i] = nn_input(in[4 * i], in[4 * i + 1], in[4 * i + 2], in[4 * i + 3]);
+  DevBuf<float4> dX, dY;
+  dX.ensure(n);
+  dY.ensure(n);
+  HBG_HIP(hipMemcpy(dX.p, X.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  nn_eval(s->nn, dX.p, n, dY.p, s->stream);
+  HBG_HIP(hipMemcpyAsync(Y.data(), dY.p, n * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < n; ++i) {
+    double s0, s1;
+    nn_scales(in[4 * i], in[4 * i + 3], s0, s1);
+    out[4 * i] = Y[i].x * s0;
+    out[4 * i + 1] = Y[i].y * s1;
+    out[4 * i + 2] = Y[i].z * s1;
+    out[4 * i + 3] = Y[i].w * s1;
+  }
+  return HBG_OK;
+  GUARD_END("hbg_surrogate_eval")
+}
+
+HbgStatus hbg_finite_remainder(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
+                               const double* zeta, double* out) {
+  if (!s || (n > 0 && (!R || !z || !zeta || !out))) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_finite_remainder");
+  if (!(h > 0.0) || !std::isfinite(h) || !(K > 0.0) || !std::isfinite(K) || n < 0)
+    return fail(HBG_ERR_VALUE, "h and K must be finite and > 0", "hbg_finite_remainder");
+  for (int i = 0; i < n; ++i)
+    if (!(R[i] > 0.0) || !(z[i] <= 0.0) || !(zeta[i] <= 0.0) || z[i] < -h || zeta[i] < -h)
+      return fail(HBG_ERR_VALUE, "point " + std::to_string(i) + " outside R > 0, -h <= z, zeta <= 0",
+                  "hbg_finite_remainder");
+  if (n == 0) return HBG_OK;
+  GUARD_BEGIN
+  use_device(s);
+  const FinFreq f = make_fin_freq(h, K);
+  std::vector<PanelD> pts(2 * (size_t)n);
+  std::memset(pts.data(), 0, pts.size() * sizeof(PanelD));
+  for (int i = 0; i < n; ++i) {
+    pts[i].c[2] = z[i];
+    pts[n + i].c[2] = zeta[i];
+  }
+  DevBuf<PanelD> dp;
+  DevBuf<double> dphi1, dphi2, dR, dz, dzeta, dout;
+  DevBuf<double4> dpf;
+  DevBuf<double2> dpcs;
+  DevBuf<FinFreq> dff;
+  dp.ensure(pts.size());
+  dphi1.ensure((size_t)FTS * FTW * FTILE);
+  dphi2.ensure((size_t)FTS * FTV * FTILE);
+  dpf.ensure(pts.size());
+  dpcs.ensure(pts.size() * FIN_ROOTS);
+  dR.ensure(n);
+  dz.ensure(n);
+  dzeta.ensure(n);
+  dout.ensure(4 * (size_t)n);
+  dff.ensure(1);
+  HBG_HIP(hipMemcpy(dp.p, pts.data(), pts.size() * sizeof(PanelD), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dR.p, R, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dz.p, z, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dzeta.p, zeta, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice));
+  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, dphi1.p, dphi2.p, s->fitm, dp.p, 2 * n, dpf.p, dpcs.p, s->stream);
+  FinTables F;
+  F.phi1 = dphi1.p;
+  F.phi2 = dphi2.p;
+  F.ktab = s->ktab;
+  F.pf = dpf.p;
+  F.pcs = dpcs.p;
+  launch_remainder(dff.p, F, s->T, n, dR.p, dz.p, dzeta.p, dout.p, s->stream);
+  HBG_HIP(hipMemcpyAsync(out, dout.p, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  return HBG_OK;
+  GUARD_END("hbg_finite_remainder")
+}
+
+HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t batch, const float* inputs,
+                                 const float* targets, float* gradient, double* loss) {
+  if (!s || !params || !inputs || !targets || !gradient)
+    return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_surrogate_gradient");
+  if (batch < 1 || batch > (1 << 22)) return fail(HBG_ERR_VALUE, "batch out of range", "hbg_surrogate_gradient");
+  GUARD_BEGIN
+  use_device(s);
+  const double l = nn_gradient(params, inputs, targets, batch, gradient, s->stream);
+  if (loss) *loss = l;
+  return HBG_OK;
+  GUARD_END("hbg_surrogate_gradient")
 }
 
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {

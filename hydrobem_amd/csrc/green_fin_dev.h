@@ -192,4 +192,25 @@ __device__ __forceinline__ FinK fin_kernel(const FinFreq* __restrict__ fp, const
   return o;
 }
 
+// The propagating mode of the pair: a = A0 E, its derivatives in z (field) and zeta (source),
+// the Bessel functions at k0 R. With it the real part splits as Wr = T - a Y0(k0 R) (T the
+// evanescent remainder, smooth over R / h), and Wi = -a J0(k0 R) exactly.
+struct FinProp {
+  double a, az, af, j0, y0, j1, y1;
+};
+
+__device__ __forceinline__ FinProp fin_prop(const FinFreq* __restrict__ fp, const FinTables& F, const WaveTables& T,
+                                            int k, int n, double R) {
+  const FinFreq& f = *fp;
+  const double k0 = f.k0;
+  const double4 pz = F.pf[k], pq = F.pf[n];
+  const double e3 = pz.x * pq.x, e4 = pz.y * pq.z, e5 = pz.z * pq.y, e6 = pz.w * pq.w;
+  FinProp p;
+  p.a = f.A0 * (e3 + e4 + e5 + e6);
+  p.az = f.A0 * k0 * (e3 + e4 - e5 - e6);
+  p.af = f.A0 * k0 * (e3 - e4 + e5 - e6);
+  bessel_jy01(T, k0 * R, p.j0, p.y0, p.j1, p.y1);
+  return p;
+}
+
 }  // namespace hbg

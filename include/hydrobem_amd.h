@@ -63,6 +63,9 @@ typedef struct HbgOptions {
   int32_t precision;     /* HBG_PRECISION_FP64 or HBG_PRECISION_MIXED */
   int32_t logarithmic;   /* analytic log-singularity integral (document_schema.rs:218, default 1) */
   double log_tolerance;  /* document_schema.rs:219, default 1e-4 */
+  int32_t surrogate;     /* 1: finite-depth evanescent remainder from the learned surrogate (see
+                            hbg_surrogate_train) where 0.1 <= R/h <= 20 and 0.05 <= nu h <= 20; 0 (default) */
+  int32_t reserved;      /* 0 */
 } HbgOptions;
 
 typedef struct HbgSession HbgSession;
@@ -140,6 +143,40 @@ HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double
  * and their LUs advance panel by panel in the same launches. 0 (the default): as many as a quarter
  * of the free device memory holds, at most 16. */
 HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch);
+
+/* ---- the learned surrogate (no counterpart in the reference; it stands beside the finite-depth
+ * evaluator of green_fin.rs:1007-1095, which remains the fallback and judges it). The network
+ * (4 -> 128 -> 128 -> 128 -> 4, SiLU; BF16 weights with FP32 accumulation on the matrix cores)
+ * maps (R/h, z/h, zeta/h, nu h) to the evanescent remainder T of the real kernel, Wr = T - A0 E
+ * Y0(k0 R), as (h T, h^2 T_R, h^2 T_z, h^2 T_zeta); the propagating mode and Im G stay analytic. */
+typedef struct HbgSurrogateTraining {
+  uint64_t seed;
+  int32_t n_train;         /* samples drawn, in groups of 8192 with one nu h each */
+  int32_t n_holdout;       /* held-out samples for the error report */
+  int32_t epochs;
+  int32_t batch;           /* minibatch (>= 64, <= n_train) */
+  double learning_rate;    /* Adam, decayed by cosine to 1 % */
+} HbgSurrogateTraining;
+#define HBG_SURROGATE_REPORT 9
+/* Parameters of the network (FP32, output-major weights W1 [128][4], b1, W2 [128][128], b2, W3, b3,
+ * W4 [4][128], b4). */
+int32_t hbg_surrogate_size(void);
+/* Train on the GPU (sampling, the FP64 evaluator's targets, forward and backward passes, Adam) and
+ * keep the network in the session. report [9]: final training loss, held-out RMS error (network
+ * output units), held-out max |error| of the four outputs, data seconds, training seconds, steps. */
+HbgStatus hbg_surrogate_train(HbgSession* s, const HbgSurrogateTraining* training, double* report);
+HbgStatus hbg_surrogate_set(HbgSession* s, const float* params, int32_t n);
+HbgStatus hbg_surrogate_get(HbgSession* s, float* params, int32_t n);
+/* The network at points in [n][4] = (R/h, z/h, zeta/h, nu h): out [n][4] = (h T, h^2 T_R, h^2 T_z,
+ * h^2 T_zeta) through the BF16 inference kernel. */
+HbgStatus hbg_surrogate_eval(HbgSession* s, int32_t n, const double* in, double* out);
+/* The same four values from the FP64 evaluator (the network's targets and judge). */
+HbgStatus hbg_finite_remainder(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
+                               const double* zeta, double* out);
+/* Loss (mean squared error over batch x 4) and its gradient for the parameters, by the training's
+ * forward and backward kernels: inputs [batch][4] (the network's scaled inputs), targets [batch][4]. */
+HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t batch, const float* inputs,
+                                 const float* targets, float* gradient, double* loss);
 
 /* Kernel variants (no counterpart in the reference), per process: HBG_TUNE_LU_GEMM selects the
  * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64;

