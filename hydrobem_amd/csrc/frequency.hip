@@ -12,62 +12,91 @@ namespace hbg {
 //   S(k, n) = rs + A_n Wr - 2 K logc dl + i A_n Wi
 //   D(k, n) = rd + A_n (-WrR a + WrZ n_z) - 2 K logc dd + i A_n (-WiR a + WiZ n_z)
 //   A(k, n) = 2 pi delta_kn - D(k, n)
-// with the wave kernel at the centroids (one-point rule, low_order.rs:738-769) and
-// a = ((x_k - c_n)_h . n_h) / R. Column-major outputs with leading dimension lda; one thread per
-// entry, the 64 lanes of a wavefront along k (coalesced loads of the statics and stores of the
-// matrices, the panel record of n a wavefront-uniform load).
+// with the wave kernel at the centroids (one-point rule, low_order.rs:738-769),
+// a = ((x_k - c_n)_h . n_h) / R and WrZ the derivative in the source depth. Column-major outputs
+// with leading dimension lda.
+//
+// The kernel of a pair is symmetric: (n, k) has the same value and R-derivative, and its
+// source-depth derivative is the field-depth derivative of (k, n) (the same one in infinite
+// depth, where the kernel depends on z + zeta). So each unordered pair is evaluated once: a
+// workgroup takes a 64 x 64 tile (k-block i <= n-block j; the diagonal tiles the pairs k <= n),
+// the 64 lanes of a wavefront along k and 16 n per wavefront, and writes (k, n) (coalesced) and
+// (n, k) (one column per lane; the wavefront's consecutive n fill the same cache lines).
+__device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __restrict__ A, int lda, int row, int col,
+                                          const StaticD& e, double Ar, double nz, double a, double c2, double Wr,
+                                          double WrR, double WrZ, double Wi, double WiR, double WiZ) {
+  S[(size_t)col * lda + row] = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
+  const double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
+  A[(size_t)col * lda + row] = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
+}
+
+constexpr int ASM_TILE = 64, ASM_NPW = 16;
+
 __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict__ P, int N, double K, double logc,
                                                        WaveTables T, const StaticD* __restrict__ st,
                                                        double2* __restrict__ S, double2* __restrict__ A, int lda) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int n = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
-  if (k >= N || n >= N) return;
-  const PanelD* pn = P + n;
-  const double xk = P[k].c[0], yk = P[k].c[1], zk = P[k].c[2];
-  const double dx = xk - pn->c[0], dy = yk - pn->c[1];
-  const double R = sqrt(dx * dx + dy * dy);
-  const double X = K * R, Y = -K * (zk + pn->c[2]);
-  const WaveK w = wave_kernel(T, X, Y);
-  const double R0 = sqrt(X * X + Y * Y);
-  const double FY = -w.F - 2.0 / R0, IY = -w.I;
-  const double K2 = K * K;
-  const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
-  const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
-  const double nx = pn->n[0], ny = pn->n[1], nz = pn->n[2];
-  const double a = R > 0.0 ? (dx * nx + dy * ny) / R : 0.0;
-  const StaticD e = st[(size_t)n * N + k];
-  const double Ar = pn->area;
-  const double c2 = 2.0 * K * logc;
-  const double2 s = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
-  const double2 d = make_double2(e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, Ar * (-WiR * a + WiZ * nz));
-  S[(size_t)n * lda + k] = s;
-  A[(size_t)n * lda + k] = make_double2((k == n ? 2.0 * PI : 0.0) - d.x, -d.y);
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi > bj) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = bi * ASM_TILE + lane;
+  if (k >= N) return;
+  const PanelD* pk = P + k;
+  const double xk = pk->c[0], yk = pk->c[1], zk = pk->c[2];
+  const double c2 = 2.0 * K * logc, K2 = K * K;
+  for (int t = 0; t < ASM_NPW; ++t) {
+    const int n = bj * ASM_TILE + wave * ASM_NPW + t;
+    if (n >= N) break;
+    if (bi == bj && k > n) continue;
+    const PanelD* pn = P + n;
+    const double dx = xk - pn->c[0], dy = yk - pn->c[1];
+    const double R = sqrt(dx * dx + dy * dy);
+    const double X = K * R, Y = -K * (zk + pn->c[2]);
+    const WaveK w = wave_kernel(T, X, Y);
+    const double R0 = sqrt(X * X + Y * Y);
+    const double FY = -w.F - 2.0 / R0, IY = -w.I;
+    const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
+    const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
+    const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+    put_entry(S, A, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
+    if (k != n) {
+      const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
+      put_entry(S, A, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
+    }
+  }
 }
 
-// The matrices in finite depth: the same entries with the kernel of green_fin_dev.h; the (k, n)
-// entry takes the source-depth derivative WrZ of the pair kernel evaluated at (x_k, c_n).
+// The matrices in finite depth: the same entries with the kernel of green_fin_dev.h, evaluated
+// at (field x_k, source c_n); (n, k) takes its field-depth derivative Wrz as its source derivative.
 __global__ void __launch_bounds__(256) assemble_fin_kernel(const PanelD* __restrict__ P, int N,
                                                            const FinFreq* __restrict__ f,
                                                            FinTables F, WaveTables T, double logc,
                                                            const StaticD* __restrict__ st, double2* __restrict__ S,
                                                            double2* __restrict__ A, int lda) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int n = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
-  if (k >= N || n >= N) return;
-  const PanelD* pn = P + n;
-  const double xk = P[k].c[0], yk = P[k].c[1], zk = P[k].c[2];
-  const double dx = xk - pn->c[0], dy = yk - pn->c[1];
-  const double R = sqrt(dx * dx + dy * dy);
-  const FinK w = fin_kernel(f, F, T, k, n, R, zk, pn->c[2]);
-  const double nx = pn->n[0], ny = pn->n[1], nz = pn->n[2];
-  const double a = R > 0.0 ? (dx * nx + dy * ny) / R : 0.0;
-  const StaticD e = st[(size_t)n * N + k];
-  const double Ar = pn->area, K = f->K;
-  const double c2 = 2.0 * K * logc;
-  const double2 s = make_double2(e.rs + Ar * w.Wr - c2 * e.dl, Ar * w.Wi);
-  const double2 d = make_double2(e.rd + Ar * (-w.WrR * a + w.WrZ * nz) - c2 * e.dd, Ar * (-w.WiR * a + w.WiZ * nz));
-  S[(size_t)n * lda + k] = s;
-  A[(size_t)n * lda + k] = make_double2((k == n ? 2.0 * PI : 0.0) - d.x, -d.y);
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi > bj) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = bi * ASM_TILE + lane;
+  if (k >= N) return;
+  const PanelD* pk = P + k;
+  const double xk = pk->c[0], yk = pk->c[1], zk = pk->c[2];
+  const double c2 = 2.0 * f->K * logc;
+  for (int t = 0; t < ASM_NPW; ++t) {
+    const int n = bj * ASM_TILE + wave * ASM_NPW + t;
+    if (n >= N) break;
+    if (bi == bj && k > n) continue;
+    const PanelD* pn = P + n;
+    const double dx = xk - pn->c[0], dy = yk - pn->c[1];
+    const double R = sqrt(dx * dx + dy * dy);
+    const FinK w = fin_kernel(f, F, T, k, n, R, zk, pn->c[2]);
+    const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+    put_entry(S, A, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
+              w.WiZ);
+    if (k != n) {
+      const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
+      put_entry(S, A, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
+                w.Wiz);
+    }
+  }
 }
 
 // The finite-depth kernel at n point pairs for the tests: the point depths z[i] at "panel" i and
@@ -297,15 +326,15 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
 
 void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
                      double2* A, int lda, hipStream_t stream) {
-  dim3 block(64, 4), grid((N + 63) / 64, (N + 3) / 4);
-  assemble_kernel<<<grid, block, 0, stream>>>(P, N, K, logc, T, st, S, A, lda);
+  const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
+  assemble_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, K, logc, T, st, S, A, lda);
   HBG_HIP(hipGetLastError());
 }
 
 void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                          const StaticD* st, double2* S, double2* A, int lda, hipStream_t stream) {
-  dim3 block(64, 4), grid((N + 63) / 64, (N + 3) / 4);
-  assemble_fin_kernel<<<grid, block, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda);
+  const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
+  assemble_fin_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda);
   HBG_HIP(hipGetLastError());
 }
 

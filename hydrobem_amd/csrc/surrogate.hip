@@ -369,73 +369,88 @@ __global__ void __launch_bounds__(256) nn_eval_kernel(const v8bf* __restrict__ g
 }
 
 // The finite-depth S and A of one frequency (as assemble_fin_kernel in frequency.hip) with the
-// evanescent remainder from the network where NN_RMIN <= R/h <= 20; the other pairs (k, n) are
-// queued (wave-aggregated appends) for fallback_kernel, which evaluates them in FP64. Workgroups of
-// 4 waves; a wave takes 64 field points (k) against NN_PER_WAVE sources (n); the packed weights are
-// staged once per workgroup.
+// evanescent remainder from the network where NN_RMIN <= R/h <= 20; the other pairs are queued
+// (wave-aggregated appends) for fallback_kernel, which evaluates them in FP64. As in frequency.hip
+// each unordered pair is taken once (64 x 64 tiles with k-block <= n-block, the diagonal tiles
+// the pairs k <= n) and both entries are written: the network's third output is the remainder's
+// derivative in the source depth, its fourth the one in the field depth. A wave takes 64 field
+// points (k) against NN_PER_WAVE sources (n); the packed weights are staged once per workgroup.
 constexpr int NN_PER_WAVE = 16;
-__device__ __forceinline__ void write_pair(const PanelD* __restrict__ pn, const StaticD& e, const FinK& w, double R,
-                                           double dx, double dy, double c2, int k, int n, int lda,
-                                           double2* __restrict__ S, double2* __restrict__ A) {
-  const double nx = pn->n[0], ny = pn->n[1], nz = pn->n[2];
-  const double a = R > 0.0 ? (dx * nx + dy * ny) / R : 0.0;
-  const double Ar = pn->area;
-  S[(size_t)n * lda + k] = make_double2(e.rs + Ar * w.Wr - c2 * e.dl, Ar * w.Wi);
-  const double dre = e.rd + Ar * (-w.WrR * a + w.WrZ * nz) - c2 * e.dd, dim = Ar * (-w.WiR * a + w.WiZ * nz);
-  A[(size_t)n * lda + k] = make_double2((k == n ? 2.0 * PI : 0.0) - dre, -dim);
+__device__ __forceinline__ void write_entry(double2* __restrict__ S, double2* __restrict__ A, int lda, int row, int col,
+                                            const StaticD& e, double Ar, double nz, double a, double c2, double Wr,
+                                            double WrR, double WrZ, double Wi, double WiR, double WiZ) {
+  S[(size_t)col * lda + row] = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
+  const double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
+  A[(size_t)col * lda + row] = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N,
-                                                              const FinFreq* __restrict__ f, FinTables F, WaveTables T,
-                                                              double logc, const StaticD* __restrict__ st,
-                                                              double2* __restrict__ S, double2* __restrict__ A, int lda,
-                                                              const v8bf* __restrict__ gpk,
-                                                              const float* __restrict__ gbias,
-                                                              int2* __restrict__ queue, unsigned* __restrict__ qcount) {
+// both entries of the pair (k, n), k <= n, from the pair's kernel values
+__device__ __forceinline__ void write_pair(const PanelD* __restrict__ P, int N, const StaticD* __restrict__ st,
+                                           const FinK& w, double c2, int k, int n, int lda, double2* __restrict__ S,
+                                           double2* __restrict__ A) {
+  const PanelD* pk = P + k;
+  const PanelD* pn = P + n;
+  const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
+  const double R = sqrt(dx * dx + dy * dy);
+  const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+  write_entry(S, A, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
+              w.WiZ);
+  if (k != n) {
+    const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
+    write_entry(S, A, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
+                w.Wiz);
+  }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __restrict__ f, FinTables F, WaveTables T,
+                       double logc, const StaticD* __restrict__ st, double2* __restrict__ S, double2* __restrict__ A,
+                       int lda, const v8bf* __restrict__ gpk, const float* __restrict__ gbias, int2* __restrict__ queue,
+                       unsigned* __restrict__ qcount) {
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi > bj) return;  // workgroup-uniform
   __shared__ v8bf pk[NN_PK_V8];
   __shared__ float bias[NN_PK_BIAS];
   nn_stage(gpk, gbias, pk, bias);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = blockIdx.x * 64 + lane;
-  const bool valid = k < N;
-  const int kk = valid ? k : N - 1;
+  const int k = bi * 64 + lane;
+  const int kk = k < N ? k : N - 1;
   const double h = f->h, K = f->K, k0 = f->k0, nuh = K * h, ih = 1.0 / h;
   const double xk = P[kk].c[0], yk = P[kk].c[1], zk = P[kk].c[2];
   const double c2 = 2.0 * K * logc;
   for (int t = 0; t < NN_PER_WAVE; ++t) {
-    const int n = (blockIdx.y * 4 + wave) * NN_PER_WAVE + t;
-    if (n >= N) break;
+    const int n = bj * 64 + wave * NN_PER_WAVE + t;
+    if (n >= N) break;  // wave-uniform
+    const bool live = k < N && !(bi == bj && k > n);
     const PanelD* pn = P + n;
     const double dx = xk - pn->c[0], dy = yk - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy), r = R * ih;
     const bool nn = r >= NN_RMIN && r <= 20.0;
     const float4 u = nn_input(nn ? r : 1.0, zk * ih, pn->c[2] * ih, nuh);
-    const float4 y = nn_wave(pk, bias, u);
-    // queue the pairs outside the network's range
-    const unsigned long long fb = __ballot(valid && !nn);
+    const float4 y = nn_wave(pk, bias, u);  // every lane takes part
+    const unsigned long long fb = __ballot(live && !nn);
     if (fb) {
       unsigned base = 0;
       if (lane == 0) base = atomicAdd(qcount, (unsigned)__popcll(fb));
       base = __shfl(base, 0);
-      if (valid && !nn) {
-        const unsigned slot = base + (unsigned)__popcll(fb & ((1ull << lane) - 1ull));
-        queue[slot] = make_int2(k, n);
-      }
+      if (live && !nn) queue[base + (unsigned)__popcll(fb & ((1ull << lane) - 1ull))] = make_int2(k, n);
     }
-    if (!valid || !nn) continue;
+    if (!live || !nn) continue;
     const FinProp pp = fin_prop(f, F, T, kk, n, R);
     double s0, s1;
     nn_scales(r, nuh, s0, s1);
-    const double Tv = y.x * s0 * ih, TR = y.y * s1 * ih * ih, Tz = y.z * s1 * ih * ih;
+    const double Tv = y.x * s0 * ih, sd = s1 * ih * ih;
     FinK w;
     w.Wr = Tv - pp.a * pp.y0;
-    w.WrR = TR + pp.a * k0 * pp.y1;
-    w.WrZ = Tz - pp.az * pp.y0;
+    w.WrR = y.y * sd + pp.a * k0 * pp.y1;
+    w.WrZ = y.z * sd - pp.az * pp.y0;
+    w.Wrz = y.w * sd - pp.af * pp.y0;
     w.Wi = -pp.a * pp.j0;
     w.WiR = pp.a * k0 * pp.j1;
     w.WiZ = -pp.az * pp.j0;
-    write_pair(pn, st[(size_t)n * N + k], w, R, dx, dy, c2, k, n, lda, S, A);
+    w.Wiz = -pp.af * pp.j0;
+    write_pair(P, N, st, w, c2, k, n, lda, S, A);
   }
 }
 
@@ -451,11 +466,9 @@ __global__ void __launch_bounds__(256) fallback_kernel(const PanelD* __restrict_
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const int2 q = queue[i];
     const int k = q.x, n = q.y;
-    const PanelD* pn = P + n;
-    const double dx = P[k].c[0] - pn->c[0], dy = P[k].c[1] - pn->c[1];
-    const double R = sqrt(dx * dx + dy * dy);
-    const FinK w = fin_kernel(f, F, T, k, n, R, P[k].c[2], pn->c[2]);
-    write_pair(pn, st[(size_t)n * N + k], w, R, dx, dy, c2, k, n, lda, S, A);
+    const double dx = P[k].c[0] - P[n].c[0], dy = P[k].c[1] - P[n].c[1];
+    const FinK w = fin_kernel(f, F, T, k, n, sqrt(dx * dx + dy * dy), P[k].c[2], P[n].c[2]);
+    write_pair(P, N, st, w, c2, k, n, lda, S, A);
   }
 }
 
@@ -843,12 +856,13 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
   report[8] = (double)step;
 }
 
-// queue: N * N pairs of room (every pair may fall back), qcount: one counter
+// queue: N (N + 1) / 2 pairs of room (every pair may fall back), qcount: one counter
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                             const StaticD* st, double2* S, double2* A, int lda, const NnDevice& nn, int2* queue,
                             unsigned* qcount, hipStream_t stream) {
   zero_u32_kernel<<<1, 64, 0, stream>>>(qcount);
-  dim3 grid((N + 63) / 64, (N + 4 * NN_PER_WAVE - 1) / (4 * NN_PER_WAVE));
+  const unsigned nt = (unsigned)((N + 63) / 64);
+  dim3 grid(nt, nt);
   assemble_fin_nn_kernel<<<grid, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda, (const v8bf*)nn.pk, nn.bias,
                                                    queue, qcount);
   fallback_kernel<<<2048, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda, queue, qcount);
