@@ -74,6 +74,8 @@ SIGNATURES = {
     "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "hbg_set_profiling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "hbg_motions": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
+                                     _dp, _dp, _dp, _dp, _dp]),
     "hbg_surrogate_size": (ctypes.c_int32, []),
     "hbg_surrogate_train": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(SurrogateTraining), _dp]),
     "hbg_surrogate_set": (ctypes.c_int32, [ctypes.c_void_p, _fp, ctypes.c_int32]),
@@ -198,6 +200,21 @@ class Session:
     def set_batch(self, max_batch: int):
         """Frequencies in flight in solve (0: as many as fit, at most 16)."""
         _check(library().hbg_set_batch(self._h, int(max_batch)))
+
+    def motions(self, result, mass, stiffness, external_damping=None) -> np.ndarray:
+        """RAOs [F][H][D] of a solve's result (hbg_motions): mass, stiffness, external damping [D][D]."""
+        w = np.ascontiguousarray(result["omegas"], dtype=np.float64)
+        A = np.ascontiguousarray(result["added_mass"], dtype=np.float64)
+        B = np.ascontiguousarray(result["radiation_damping"], dtype=np.float64)
+        X = np.ascontiguousarray(result["excitation_diffraction"], dtype=np.complex128)
+        F, H, D = X.shape
+        M = np.ascontiguousarray(mass, dtype=np.float64)
+        C = np.ascontiguousarray(stiffness, dtype=np.float64)
+        Bx = None if external_damping is None else np.ascontiguousarray(external_damping, dtype=np.float64)
+        out = np.zeros((F, H, D), np.complex128)
+        _check(library().hbg_motions(self._h, F, _p(w), H, D, _p(M), _p(C), _p(Bx), _p(A), _p(B),
+                                     X.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
+        return out
 
     # ---- the learned surrogate
     @staticmethod

@@ -61,6 +61,11 @@ double nn_gradient(const float* P_host, const float* X_host, const float* Y_host
 void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int batch, float lr0, const WaveTables& T,
               const double* ktab, const double* fitm, const double* g16, std::vector<float>& P, double* report,
               NnDevice& nn, hipStream_t st);
+void launch_motions(int F, int D, int H, const double* omegas, const double* M, const double* C, const double* Bx,
+                    const double* A, const double* B, const double2* X, double2* out, int* status,
+                    hipStream_t stream);
+int motions_max_dof();
+int motions_max_headings();
 void lu_set_gemm_variant(int v);
 int lu_gemm_variant();
 void lu_set_depth(int q);
@@ -904,6 +909,57 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_CANCELLED, "cancelled"};
   return HBG_OK;
   GUARD_END("hbg_solve")
+}
+
+HbgStatus hbg_motions(HbgSession* s, int32_t n_omegas, const double* omegas, int32_t n_headings, int32_t n_dof,
+                      const double* mass, const double* stiffness, const double* external_damping,
+                      const double* added_mass, const double* damping, const double* excitation, double* rao) {
+  if (!s || (n_omegas > 0 && (!omegas || !added_mass || !damping || !excitation || !rao)) || !mass || !stiffness)
+    return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_motions");
+  if (n_omegas < 0 || n_headings < 0 || n_dof < 1)
+    return fail(HBG_ERR_VALUE, "negative count or no degree of freedom", "hbg_motions");
+  if (n_dof > motions_max_dof() || n_headings > motions_max_headings())
+    return fail(HBG_ERR_UNSUPPORTED, "more than 64 degrees of freedom or 32 headings", "hbg_motions");
+  if (n_omegas == 0 || n_headings == 0) return HBG_OK;
+  GUARD_BEGIN
+  use_device(s);
+  const size_t F = n_omegas, D = n_dof, H = n_headings;
+  DevBuf<double> dw, dM, dC, dBx, dA, dB;
+  DevBuf<double2> dX, dout;
+  DevBuf<int> dst;
+  dw.ensure(F);
+  dM.ensure(D * D);
+  dC.ensure(D * D);
+  dA.ensure(F * D * D);
+  dB.ensure(F * D * D);
+  dX.ensure(F * H * D);
+  dout.ensure(F * H * D);
+  dst.ensure(F);
+  HBG_HIP(hipMemcpy(dw.p, omegas, F * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dM.p, mass, D * D * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dC.p, stiffness, D * D * sizeof(double), hipMemcpyHostToDevice));
+  if (external_damping) {
+    dBx.ensure(D * D);
+    HBG_HIP(hipMemcpy(dBx.p, external_damping, D * D * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HBG_HIP(hipMemcpy(dA.p, added_mass, F * D * D * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dB.p, damping, F * D * D * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dX.p, excitation, F * H * D * sizeof(double2), hipMemcpyHostToDevice));
+  launch_motions((int)F, (int)D, (int)H, dw.p, dM.p, dC.p, external_damping ? dBx.p : nullptr, dA.p, dB.p, dX.p,
+                 dout.p, dst.p, s->stream);
+  std::vector<int> st(F);
+  HBG_HIP(hipMemcpyAsync(st.data(), dst.p, F * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipMemcpyAsync(rao, dout.p, F * H * D * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  for (size_t f = 0; f < F; ++f)
+    if (st[f] > 0) {
+      char buf[64];
+      std::snprintf(buf, sizeof(buf), "%.6g", omegas[f]);
+      throw Error{HBG_ERR_NUMERICAL, std::string("motion equation at omega = ") + buf +
+                                         ": singular system matrix (zero pivot " + std::to_string(st[f]) + ")"};
+    }
+  return HBG_OK;
+  GUARD_END("hbg_motions")
 }
 
 int32_t hbg_surrogate_size(void) { return NN_PARAMS; }

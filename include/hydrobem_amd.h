@@ -108,6 +108,18 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                     double* excitation, double* froude_krylov, double* scattering, double* haskind,
                     HbgProgressFn progress, void* user);
 
+/* The motions of the bodies (RAOs) from a solve's outputs (motion.rs:121-158 solve_motions):
+ * per frequency (C - w^2 (M + A) + i w (B + B_ext)) xi = X on the n_dof = 6 n_bodies modes, complex,
+ * partial pivoting. mass, stiffness, external_damping (NULL: none): [D][D] row-major, about the
+ * rotation centres (the reference's motion_matrices, motion.rs:85-115, are the caller's);
+ * added_mass, damping [F][D][D] and excitation [F][H][D] complex as hbg_solve gives them (a body
+ * whose excitation is the Haskind force passes those columns instead). rao [F][H][D] complex.
+ * D <= 64, H <= 32. HBG_ERR_NUMERICAL "motion equation at omega = w: singular system matrix
+ * (zero pivot k)". */
+HbgStatus hbg_motions(HbgSession* s, int32_t n_omegas, const double* omegas, int32_t n_headings, int32_t n_dof,
+                      const double* mass, const double* stiffness, const double* external_damping,
+                      const double* added_mass, const double* damping, const double* excitation, double* rao);
+
 /* ---- kernel-level entry points, at the granularity of the reference's component tests
  * (crates/hydrobem/tests/core.rs, lo_a.rs; linalg.rs:422-489): */
 
