@@ -108,6 +108,22 @@ def cpu_baseline(w):
                       f"production path (docs/theory/13-linear-algebra-and-threads.md:141-144)"}
 
 
+def motion_matrices(w):
+    """Mass and hydrostatic stiffness of the floating truncated cylinder of the workloads (radius 1 m,
+    draft 1 m, centre of gravity at the centre of buoyancy), about the origin on the free surface."""
+    rho, g, R, T = w["rho"], w["g"], 1.0, 1.0
+    m = rho * np.pi * R ** 2 * T
+    zg = -T / 2
+    Ixx = m * (R ** 2 / 4 + T ** 2 / 12) + m * zg ** 2
+    M = np.diag([m, m, m, Ixx, Ixx, m * R ** 2 / 2])
+    M[0, 4] = M[4, 0] = m * zg
+    M[1, 3] = M[3, 1] = -m * zg
+    C = np.zeros((6, 6))
+    C[2, 2] = rho * g * np.pi * R ** 2
+    C[3, 3] = C[4, 4] = rho * g * np.pi * R ** 4 / 4
+    return M, C
+
+
 def rel_err(a, b):
     """max |a - b| / scale, the scale of an entry its largest magnitude over the frequencies, at least
     1e-3 of the peak (the reference's regression-test convention)."""
@@ -118,6 +134,8 @@ def rel_err(a, b):
 
 
 def run_path(s, w, path, steps, warmup, label):
+    """Warm-up steps, `steps` timed steps (stage profiling off, so the LU runs from its captured
+    hipGraphs), then one profiled step for the stage times and the GEMM's live roofline."""
     v, omegas, heads = w["vertices"], w["omegas"], w["headings"]
     precision = "fp64" if path == "fp64" else "mixed"
     surrogate = path == "mixed_nn"
@@ -130,18 +148,21 @@ def run_path(s, w, path, steps, warmup, label):
     for i in range(warmup):
         out = step()
         print(f"[bench] {label}: warmup step {i + 1} done", file=sys.stderr, flush=True)
-    s.set_profiling(True)
     t0 = time.perf_counter()
     for i in range(steps):
         out = step()
         print(f"[bench] {label}: timed step {i + 1} done", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
+    s.set_profiling(True)
+    step()
     prof = s.profile()
     s.set_profiling(False)
+    print(f"[bench] {label}: profiled step done", file=sys.stderr, flush=True)
     return out, dt, prof
 
 
 def summary(prof, dt, steps, F, peak):
+    """Throughput of the timed steps; stage times and the GEMM figures of the profiled step."""
     nf = max(prof["frequencies"], 1.0)
     gemm = prof["lu_gemm_flops"] / (prof["lu_gemm_ms"] * 1e-3) if prof["lu_gemm_ms"] > 0 else 0.0
     lu = prof["lu_flops"] / (prof["lu_ms"] * 1e-3) if prof["lu_ms"] > 0 else 0.0
@@ -175,6 +196,7 @@ def main():
     ap.add_argument("--gemm", type=int, default=-1, help="LU GEMM variant (-1: the library's default)")
     ap.add_argument("--depth", type=int, default=-1, help="LU panels per trailing update (-1: the library's default)")
     ap.add_argument("--paths", default="fp64,mixed,mixed_nn", help="paths timed, the last one gives `value`")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the LU autotuner before each path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -206,7 +228,18 @@ def main():
             print("[bench] no trained surrogate: training one (2e6 samples)", file=sys.stderr, flush=True)
             config_nn = {"weights": "trained in this run",
                          "training": s.surrogate_train(n_train=2_000_000, n_holdout=100_000, epochs=6, batch=4096)}
+    tuning = {}
     for p in paths:
+        if not a.no_autotune and a.gemm < 0 and a.depth < 0:
+            prec = "fp64" if p == "fp64" else "mixed"
+            if prec not in tuning:
+                rows = s.autotune(N, 6 + len(heads), prec)
+                best = min(rows, key=lambda r: r[3])
+                tuning[prec] = {"gemm_tiles": "64x64" if best[0] == 1 else "128x64", "panels_per_update": int(best[1]),
+                                "batch": int(best[2]), "ms_per_system": round(best[3], 3), "candidates": len(rows)}
+                print(f"[bench] autotuned {prec}: {tuning[prec]}", file=sys.stderr, flush=True)
+            if a.batch == 0:
+                s.set_batch(tuning[prec]["batch"])
         out, dt, prof = run_path(s, w, p, a.steps, a.warmup, p)
         res[p] = (out, summary(prof, dt, a.steps, F, peaks[p]))
     s.close()
@@ -225,12 +258,19 @@ def main():
         config[p] = res[p][1]
     if "mixed_nn" in paths:
         config["surrogate"] = config_nn
-    if "fp64" in res and main_path != "fp64":
+    if tuning:
+        config["autotuned_lu"] = tuning
+    if "fp64" in res and len(paths) > 1:
         ref = res["fp64"][0]
+        M, C = motion_matrices(w)
+        s2 = hb.Session(0)
+        rao_ref = s2.motions(ref, M, C)
         for p in [q for q in paths if q != "fp64"]:
             config[p]["rel_err_vs_fp64_gpu"] = {k: rel_err(res[p][0][k], ref[k]) for k in
                                                 ("added_mass", "radiation_damping", "excitation_diffraction")}
+            config[p]["rel_err_vs_fp64_gpu"]["rao"] = rel_err(s2.motions(res[p][0], M, C), rao_ref)
             config[p]["speedup_vs_fp64_gpu"] = round(res[p][1]["freq_per_s"] / res["fp64"][1]["freq_per_s"], 3)
+        s2.close()
     gemm_tf = summ["gemm_achieved_tflops"]
     line = {
         "metric": METRIC,

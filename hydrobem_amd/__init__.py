@@ -85,6 +85,8 @@ SIGNATURES = {
                                               _dp, _dp, _dp, _dp]),
     "hbg_surrogate_gradient": (ctypes.c_int32, [ctypes.c_void_p, _fp, ctypes.c_int32, _fp, _fp, _fp, _dp]),
     "hbg_set_batch": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "hbg_autotune": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp,
+                                      ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "hbg_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_get_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "hbg_profile_read": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
@@ -263,6 +265,16 @@ class Session:
 
     TUNE_LU_GEMM = 0
     TUNE_LU_DEPTH = 1
+    TUNE_LU_GRAPHS = 2
+
+    def autotune(self, n, nrhs, precision="fp64") -> list:
+        """Time the batched LU variants on synthetic systems and keep the fastest (hbg_autotune);
+        returns the rows (variant, depth, batch, ms per system)."""
+        rep = np.zeros((64, 4))
+        rows = ctypes.c_int32(0)
+        _check(library().hbg_autotune(self._h, int(n), int(nrhs), PRECISIONS[precision], _p(rep), 64,
+                                      ctypes.byref(rows)))
+        return [tuple(r) for r in rep[:rows.value]]
 
     def set_tuning(self, key: int, value: int):
         _check(library().hbg_set_tuning(self._h, int(key), int(value)))

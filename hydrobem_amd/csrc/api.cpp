@@ -66,6 +66,10 @@ void launch_motions(int F, int D, int H, const double* omegas, const double* M, 
                     hipStream_t stream);
 int motions_max_dof();
 int motions_max_headings();
+int lu_autotune(int precision64, int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t stream, double* report,
+                int max_rows, int* rows);
+void lu_set_graphs(int on);
+int lu_graphs();
 void lu_set_gemm_variant(int v);
 int lu_gemm_variant();
 void lu_set_depth(int q);
@@ -1109,6 +1113,10 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       if (value < 1 || value > 16) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..16", "hbg_set_tuning");
       lu_set_depth(value);
       return HBG_OK;
+    case HBG_TUNE_LU_GRAPHS:
+      if (value != 0 && value != 1) return fail(HBG_ERR_VALUE, "graphs: 0 or 1", "hbg_set_tuning");
+      lu_set_graphs(value);
+      return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_set_tuning");
   }
@@ -1123,9 +1131,30 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value) {
     case HBG_TUNE_LU_DEPTH:
       *value = lu_depth();
       return HBG_OK;
+    case HBG_TUNE_LU_GRAPHS:
+      *value = lu_graphs();
+      return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_get_tuning");
   }
+}
+
+HbgStatus hbg_autotune(HbgSession* s, int32_t n, int32_t nrhs, int32_t precision, double* report, int32_t max_rows,
+                       int32_t* rows) {
+  if (!s || (report && !rows)) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_autotune");
+  if (n < 64 || nrhs < 0 || nrhs > 32 || (precision != HBG_PRECISION_FP64 && precision != HBG_PRECISION_MIXED))
+    return fail(HBG_ERR_VALUE, "n >= 64, 0 <= nrhs <= 32 and a known precision", "hbg_autotune");
+  GUARD_BEGIN
+  use_device(s);
+  size_t free_b = 0, total_b = 0;
+  HBG_HIP(hipMemGetInfo(&free_b, &total_b));
+  int r = 0;
+  const int b = lu_autotune(precision == HBG_PRECISION_FP64 ? 1 : 0, n, nrhs, 16, free_b / 4, s->stream, report,
+                            report ? max_rows : 0, &r);
+  if (rows) *rows = r;
+  s->max_batch = b;
+  return HBG_OK;
+  GUARD_END("hbg_autotune")
 }
 
 HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch) {

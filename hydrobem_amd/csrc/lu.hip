@@ -657,6 +657,7 @@ struct LuWorkspace {
   C2* save = nullptr;        // the panel before the attempt without interchanges (pivoted path)
   C2* backup = nullptr;      // [A | B] of a single solve, restored for the pivoted path
   size_t backup_elems = 0;
+  unsigned long generation = 0;  // bumped at every reallocation (captured graphs hold the pointers)
   void release() {
     hipFree(w.cand);
     hipFree(w.ccount);
@@ -672,7 +673,9 @@ struct LuWorkspace {
     hipFree(udiag);
     hipFree(save);
     hipFree(backup);
+    const unsigned long g = generation;
     *this = LuWorkspace{};
+    generation = g + 1;
   }
   BatchStrides strides() const {
     const int panels = (n + NB - 1) / NB;
@@ -713,18 +716,66 @@ struct LuWorkspace {
   }
 };
 
-static int g_gemm_variant = 1;
-static int g_lu_depth = 4;
-void lu_set_gemm_variant(int v) { g_gemm_variant = v; }
-int lu_gemm_variant() { return g_gemm_variant; }
-void lu_set_depth(int q) { g_lu_depth = q; }
-int lu_depth() { return g_lu_depth; }
+// tuned settings per arithmetic: [0] complex128, [1] complex64
+static int g_gemm_variant_[2] = {1, 1};
+static int g_lu_depth_[2] = {4, 4};
+template <typename T>
+constexpr int prec_index() {
+  return std::is_same<T, double>::value ? 0 : 1;
+}
+void lu_set_gemm_variant(int v) { g_gemm_variant_[0] = g_gemm_variant_[1] = v; }
+int lu_gemm_variant() { return g_gemm_variant_[0]; }
+void lu_set_depth(int q) { g_lu_depth_[0] = g_lu_depth_[1] = q; }
+int lu_depth() { return g_lu_depth_[0]; }
 
 template <typename T>
 static LuWorkspace<T>& workspace() {
   static thread_local LuWorkspace<T> ws;
   return ws;
 }
+
+// hipGraphs of the batched LU's launch sequence (no host work inside: fills, not memsets)
+static int g_lu_graphs = 1;
+void lu_set_graphs(int on) { g_lu_graphs = on; }
+int lu_graphs() { return g_lu_graphs; }
+
+namespace {
+struct GraphKey {
+  const void* M;
+  size_t mstride;
+  int batch, n, lda, nrhs, variant, depth, real_bytes;
+  unsigned long generation;
+  bool operator==(const GraphKey& o) const {
+    return M == o.M && mstride == o.mstride && batch == o.batch && n == o.n && lda == o.lda && nrhs == o.nrhs &&
+           variant == o.variant && depth == o.depth && real_bytes == o.real_bytes && generation == o.generation;
+  }
+};
+struct GraphCache {
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> items;
+  hipGraphExec_t find(const GraphKey& k) {
+    for (auto& it : items)
+      if (it.first == k) return it.second;
+    return nullptr;
+  }
+  void put(const GraphKey& k, hipGraphExec_t e) {
+    if (items.size() >= 16) {  // the oldest goes
+      hipGraphExecDestroy(items.front().second);
+      items.erase(items.begin());
+    }
+    items.push_back({k, e});
+  }
+  void clear() {
+    for (auto& it : items) hipGraphExecDestroy(it.second);
+    items.clear();
+  }
+};
+GraphCache& graphs() {
+  static thread_local GraphCache g;
+  return g;
+}
+}  // namespace
+
+void lu_graphs_clear() { graphs().clear(); }
 
 namespace {
 struct EventPool {
@@ -762,7 +813,7 @@ void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int
     g0 = gt.ep->next();
     HBG_HIP(hipEventRecord(g0, stream));
   }
-  if (g_gemm_variant == 2)
+  if (g_gemm_variant_[prec_index<T>()] == 2)
     gemm_mfma_kernel<T, 128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
         M, lda, r0, nrows, c0, ncolsC, kf, kk, GemmBatch{mstride});
   else
@@ -779,14 +830,14 @@ void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int
 
 // Every panel of every matrix of the batch without interchanges: the diagonal block,
 // L21 = A21 U11^-1 (both checking that partial pivoting would make no interchange; the panel's
-// flag is set where it would) and U12 = L11^-1 A12. The panels go in blocks of g_lu_depth: before
+// flag is set where it would) and U12 = L11^-1 A12. The panels go in blocks of the tuned depth: before
 // a panel, the earlier panels of its block update its columns (rows from the panel down) and its
 // rows (columns right of it); after the block one trailing update with K = the block's width, so
 // the trailing matrix is read and written once per block rather than once per panel.
 template <typename T>
 void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int batch, BatchStrides bs,
                        hipStream_t stream, LuWorkspace<T>& ws, GemmTimer& gt) {
-  const int Q = std::max(1, g_lu_depth);
+  const int Q = std::max(1, g_lu_depth_[prec_index<T>()]);
   for (int K0 = 0; K0 < n; K0 += Q * NB) {
     const int qend = std::min(n, K0 + Q * NB);
     for (int k0 = K0; k0 < qend; k0 += NB) {
@@ -906,10 +957,32 @@ void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, 
   const int ncols = n + nrhs;
   const int panels = (n + NB - 1) / NB;
   const size_t pf = (size_t)bs.flag * batch;
-  fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.flags, pf, 0);
-  fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.zp_fast, pf, 0x7fffffff);
-  factor_optimistic<T>(M, n, lda, ncols, batch, bs, stream, ws, gt);
-  if (nrhs > 0) substitute<T>(M, n, lda, mstride, M + (size_t)n * lda, lda, mstride, nrhs, batch, true, stream, ws);
+  auto launches = [&]() {
+    fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.flags, pf, 0);
+    fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.zp_fast, pf, 0x7fffffff);
+    factor_optimistic<T>(M, n, lda, ncols, batch, bs, stream, ws, gt);
+    if (nrhs > 0) substitute<T>(M, n, lda, mstride, M + (size_t)n * lda, lda, mstride, nrhs, batch, true, stream, ws);
+  };
+  if (g_lu_graphs && !stats) {
+    // the launch sequence depends only on these; replayed from a captured hipGraph
+    const GraphKey key{(const void*)M, mstride, batch, n, lda, nrhs, g_gemm_variant_[prec_index<T>()],
+                       g_lu_depth_[prec_index<T>()],
+                       std::is_same<T, double>::value ? 8 : 4, ws.generation};
+    GraphCache& gc = graphs();
+    hipGraphExec_t exec = gc.find(key);
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      HBG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      launches();
+      HBG_HIP(hipStreamEndCapture(stream, &graph));
+      HBG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      HBG_HIP(hipGraphDestroy(graph));
+      gc.put(key, exec);
+    }
+    HBG_HIP(hipGraphLaunch(exec, stream));
+  } else {
+    launches();
+  }
   HBG_HIP(hipGetLastError());
   std::vector<int> flags(pf), zps(pf);
   HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, pf * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -954,6 +1027,110 @@ void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, fl
   substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, false, stream, ws);
   substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, true, stream, ws);
   HBG_HIP(hipGetLastError());
+}
+
+namespace {
+
+// synthetic systems for the autotuner: |off-diagonal| <= 1 / n, diagonal 2 pi (as the BEM's free
+// term), so the pass without interchanges succeeds
+template <typename T>
+__global__ void synth_kernel(typename Cplx<T>::type* M, size_t count, int n, int lda, unsigned long long seed) {
+  typedef typename Cplx<T>::type C2;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t within = e % ((size_t)lda * (n + 1));
+    const int i = (int)(within % lda), j = (int)(within / lda);
+    unsigned long long x = seed ^ (e * 0x9E3779B97F4A7C15ull);
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    const double u = (double)(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    const double v = (double)((x >> 3) & 0xFFFF) / 65536.0 - 0.5;
+    const double d = (i == j && j < n) ? 6.283185307179586 : 0.0;
+    M[e] = C2((T)(d + 2.0 * u / n), (T)(2.0 * v / n));
+  }
+}
+
+}  // namespace
+
+// Time the batched LU on synthetic systems (n, nrhs) for every GEMM tile variant (1, 2), block depth
+// (2, 4, 8) and batch (candidates up to max_batch that fit in mem_bytes) and keep the fastest
+// variant and depth; returns the fastest batch. rows of report: variant, depth, batch, ms per
+// system (up to max_rows rows).
+template <typename T>
+int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t stream, double* report, int max_rows,
+                int* rows) {
+  typedef typename Cplx<T>::type C2;
+  const size_t ms = (size_t)n * (n + nrhs);
+  int bmax = (int)std::min<size_t>((size_t)std::max(max_batch, 1), std::max<size_t>(1, mem_bytes / (ms * sizeof(C2))));
+  std::vector<int> batches;
+  for (int b : {2, 4, 8, 16})
+    if (b <= bmax) batches.push_back(b);
+  if (batches.empty()) batches.push_back(1);
+  C2* M = nullptr;
+  HBG_HIP(hipMalloc(&M, ms * batches.back() * sizeof(C2)));
+  hipEvent_t e0, e1;
+  HBG_HIP(hipEventCreate(&e0));
+  HBG_HIP(hipEventCreate(&e1));
+  int& gv = g_gemm_variant_[prec_index<T>()];
+  int& gd = g_lu_depth_[prec_index<T>()];
+  const int old_v = gv, old_d = gd;
+  double best = 1e300;
+  int bv = old_v, bd = old_d, bb = batches.back();
+  *rows = 0;
+  std::vector<int> status(batches.back());
+  try {
+    for (int v : {1, 2})
+      for (int d : {2, 4, 8})
+        for (int b : batches) {
+          gv = v;
+          gd = d;
+          double ms_sys = 0.0;
+          for (int rep = 0; rep < 2; ++rep) {  // the first captures the graph
+            synth_kernel<T><<<4096, 256, 0, stream>>>(M, ms * b, n, n, 1234567ull + rep);
+            HBG_HIP(hipEventRecord(e0, stream));
+            solve_batched<T>(M, ms, b, n, n, nrhs, stream, nullptr, status.data());
+            HBG_HIP(hipEventRecord(e1, stream));
+            HBG_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            HBG_HIP(hipEventElapsedTime(&t, e0, e1));
+            ms_sys = t / b;
+          }
+          if (*rows < max_rows && report) {
+            double* r = report + 4 * (*rows);
+            r[0] = v;
+            r[1] = d;
+            r[2] = b;
+            r[3] = ms_sys;
+            ++*rows;
+          }
+          if (ms_sys < best) {
+            best = ms_sys;
+            bv = v;
+            bd = d;
+            bb = b;
+          }
+        }
+  } catch (...) {
+    gv = old_v;
+    gd = old_d;
+    hipFree(M);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    throw;
+  }
+  gv = bv;
+  gd = bd;
+  graphs().clear();  // the captures hold the tuning buffer
+  hipFree(M);
+  HBG_HIP(hipEventDestroy(e0));
+  HBG_HIP(hipEventDestroy(e1));
+  return bb;
+}
+
+int lu_autotune(int precision64, int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t stream, double* report,
+                int max_rows, int* rows) {
+  return precision64 ? autotune_lu<double>(n, nrhs, max_batch, mem_bytes, stream, report, max_rows, rows)
+                     : autotune_lu<float>(n, nrhs, max_batch, mem_bytes, stream, report, max_rows, rows);
 }
 
 // One system with partial pivoting where a panel needs it (tournament pivoting over several
@@ -1011,6 +1188,7 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
 }
 
 void lu_release() {
+  graphs().clear();
   workspace<double>().release();
   workspace<float>().release();
 }

@@ -192,9 +192,19 @@ HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t bat
 
 /* Kernel variants (no counterpart in the reference), per process: HBG_TUNE_LU_GEMM selects the
  * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64;
- * HBG_TUNE_LU_DEPTH the number of 64-column panels per trailing update (1..16, default 4). */
-enum { HBG_TUNE_LU_GEMM = 0, HBG_TUNE_LU_DEPTH = 1 };
+ * HBG_TUNE_LU_DEPTH the number of 64-column panels per trailing update (1..16, default 4);
+ * HBG_TUNE_LU_GRAPHS 1 (default): the batched LU's launches are captured once per system shape
+ * and buffer into a hipGraph and replayed (not while stage profiling is on), 0: launched one by one. */
+enum { HBG_TUNE_LU_GEMM = 0, HBG_TUNE_LU_DEPTH = 1, HBG_TUNE_LU_GRAPHS = 2 };
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
+/* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
+ * systems of order n with nrhs right-hand sides in the given precision for every GEMM tile variant
+ * (1, 2), block depth (2, 4, 8) and batch (2, 4, 8, 16 that fit in a quarter of the free memory),
+ * then keeps the fastest variant and depth (HBG_TUNE_LU_GEMM, HBG_TUNE_LU_DEPTH) and sets the
+ * session's batch. report: up to max_rows rows of (variant, depth, batch, ms per system), rows the
+ * number written. */
+HbgStatus hbg_autotune(HbgSession* s, int32_t n, int32_t nrhs, int32_t precision, double* report, int32_t max_rows,
+                       int32_t* rows);
 HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);
 
 /* Stage timing (the reference's HB_PROFILE phase marks, low_order.rs:184-208), by HIP events on the
