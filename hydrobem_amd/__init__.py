@@ -243,7 +243,7 @@ class Session:
         return p
 
     def surrogate_eval(self, points) -> np.ndarray:
-        """points [n][4] = (R/h, z/h, zeta/h, nu h) -> [n][4] = (h T, h^2 T_R, h^2 T_z, h^2 T_zeta)."""
+        """points [n][4] = (R/h, z/h, zeta/h, nu h) -> [n][4] = (h T, h^2 T_R, h^2 T_zeta, h^2 T_z)."""
         x = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 4)
         out = np.zeros_like(x)
         _check(library().hbg_surrogate_eval(self._h, x.shape[0], _p(x), _p(out)))

@@ -37,7 +37,7 @@ __host__ __device__ inline float4 nn_input(double r, double z, double zeta, doub
                      (float)(2.0 * (log(nuh) - NN_LNNU_LO) / (NN_LNNU_HI - NN_LNNU_LO) - 1.0));
 }
 
-// The network's outputs are (h T / s0, h^2 T_R / s1, h^2 T_z / s1, h^2 T_zeta / s1) with
+// The network's outputs are (h T / s0, h^2 T_R / s1, h^2 T_zeta / s1, h^2 T_z / s1) with
 // s0 = 1 + nu h and s1 = (1 + nu h)^2 / min(1, R/h): the remainder grows like nu h and its
 // derivatives like (nu h)^2 and like 1 / R near the axis.
 __host__ __device__ inline void nn_scales(double r, double nuh, double& s0, double& s1) {

@@ -9,7 +9,8 @@
 // roots or the Phi integrals, the expensive part), is smooth in log(R / h). The network maps
 //   u = (ln(R/h) scaled to [-1, 1] over [1e-2, 20], 2 z/h + 1, 2 zeta/h + 1,
 //        ln(nu h) scaled to [-1, 1] over [0.05, 20])
-// to y = (h T, h^2 T_R, h^2 T_z, h^2 T_zeta) / (1 + nu h): 4 -> 128 -> 128 -> 128 -> 4, SiLU.
+// to y = (h T, h^2 T_R, h^2 T_zeta, h^2 T_z) (zeta the source depth, z the field depth), each divided
+// by the scale of nn_scales in surrogate.h and standardized: 4 -> 128 -> 128 -> 128 -> 4, SiLU.
 //
 // Training (surrogate_train): samples drawn on the device (R/h log-uniform, z/h and zeta/h
 // uniform in [-1, 0], nu h log-uniform, one nu h per group of samples), targets from the FP64
@@ -476,7 +477,7 @@ __global__ void zero_u32_kernel(unsigned* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }
 
-// FP64 evanescent remainder (h T, h^2 T_R, h^2 T_z, h^2 T_zeta) of points, for the tests and
+// FP64 evanescent remainder (h T, h^2 T_R, h^2 T_zeta, h^2 T_z) of points, for the tests and
 // the held-out judge: the evaluator of green_fin_dev.h minus the propagating mode
 __global__ void remainder_kernel(const FinFreq* __restrict__ f, FinTables F, WaveTables T, int n,
                                  const double* __restrict__ R, const double* __restrict__ z,

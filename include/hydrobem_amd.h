@@ -160,7 +160,7 @@ HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch);
  * evaluator of green_fin.rs:1007-1095, which remains the fallback and judges it). The network
  * (4 -> 128 -> 128 -> 128 -> 4, SiLU; BF16 weights with FP32 accumulation on the matrix cores)
  * maps (R/h, z/h, zeta/h, nu h) to the evanescent remainder T of the real kernel, Wr = T - A0 E
- * Y0(k0 R), as (h T, h^2 T_R, h^2 T_z, h^2 T_zeta); the propagating mode and Im G stay analytic. */
+ * Y0(k0 R), as (h T, h^2 T_R, h^2 T_zeta, h^2 T_z); the propagating mode and Im G stay analytic. */
 typedef struct HbgSurrogateTraining {
   uint64_t seed;
   int32_t n_train;         /* samples drawn, in groups of 8192 with one nu h each */
@@ -179,8 +179,9 @@ int32_t hbg_surrogate_size(void);
 HbgStatus hbg_surrogate_train(HbgSession* s, const HbgSurrogateTraining* training, double* report);
 HbgStatus hbg_surrogate_set(HbgSession* s, const float* params, int32_t n);
 HbgStatus hbg_surrogate_get(HbgSession* s, float* params, int32_t n);
-/* The network at points in [n][4] = (R/h, z/h, zeta/h, nu h): out [n][4] = (h T, h^2 T_R, h^2 T_z,
- * h^2 T_zeta) through the BF16 inference kernel. */
+/* The network at points in [n][4] = (R/h, z/h, zeta/h, nu h): out [n][4] = (h T, h^2 T_R, h^2 T_zeta,
+ * h^2 T_z) through the BF16 inference kernel; T_zeta is the derivative in the source depth, T_z in
+ * the field depth (z the field point's depth, zeta the source's). */
 HbgStatus hbg_surrogate_eval(HbgSession* s, int32_t n, const double* in, double* out);
 /* The same four values from the FP64 evaluator (the network's targets and judge). */
 HbgStatus hbg_finite_remainder(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,

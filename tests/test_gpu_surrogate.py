@@ -92,10 +92,10 @@ def test_remainder_is_the_evaluator_minus_the_propagating_mode(session):
         y0, y1 = special.y0(k0 * R), special.y1(k0 * R)
         exp_T = (ker[:, 0] + A0 * E * y0) * h
         exp_TR = (ker[:, 1] - A0 * E * k0 * y1) * h * h
-        exp_Tz = (ker[:, 2] + A0 * Ez * y0) * h * h
+        exp_Tzeta = (ker[:, 2] + A0 * Ez * y0) * h * h
         scale = np.abs(exp_T).max() + 1.0
         err = max(np.abs(rem[:, 0] - exp_T).max(), np.abs(rem[:, 1] - exp_TR).max() / h,
-                  np.abs(rem[:, 2] - exp_Tz).max() / h) / scale
+                  np.abs(rem[:, 2] - exp_Tzeta).max() / h) / scale
         say(f"  nu h {nuh}: {err:.2e}")
         assert err < 1e-9
         # Im G is the propagating mode alone: Wi = -A0 E J0(k0 R)
