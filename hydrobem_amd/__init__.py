@@ -224,12 +224,13 @@ class Session:
         return int(library().hbg_surrogate_size())
 
     REPORT_KEYS = ("train_loss", "holdout_rms", "holdout_max_T", "holdout_max_TR", "holdout_max_Tz",
-                   "holdout_max_Tzeta", "data_seconds", "train_seconds", "steps")
+                   "holdout_max_Tzeta", "data_seconds", "train_seconds", "steps", "nonfinite_targets",
+                   "nonfinite_params", "reserved")
 
     def surrogate_train(self, n_train=2_000_000, n_holdout=100_000, epochs=4, batch=4096, learning_rate=2e-3,
                         seed=1) -> dict:
         t = SurrogateTraining(int(seed), int(n_train), int(n_holdout), int(epochs), int(batch), float(learning_rate))
-        rep = np.zeros(9)
+        rep = np.zeros(12)
         _check(library().hbg_surrogate_train(self._h, ctypes.byref(t), _p(rep)))
         return {k: float(rep[i]) for i, k in enumerate(self.REPORT_KEYS)}
 

@@ -19,7 +19,7 @@ namespace hbg {
 void launch_statics(const PanelD* P, int N, int logint, double tol, double depth, const double* gl, StaticD* st,
                     int2* queue, unsigned* qcount, unsigned qcap, hipStream_t stream);
 void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
-                         const StaticD* st, double2* S, double2* A, int lda, hipStream_t stream);
+                         const StaticD* st, double2* S, double2* A, float2* A32, int lda, hipStream_t stream);
 void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
                             const double* zeta, int n, double* out, hipStream_t stream);
 void launch_fin_tables(const FinFreq& f, WaveTables T, const double* g16, double* phi1, double* phi2,
@@ -31,7 +31,7 @@ std::vector<double> build_k_table();
 void launch_adaptive(const PanelD* P, int N, const double* gl, StaticD* st, const int2* queue, unsigned count,
                      hipStream_t stream);
 void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
-                     double2* A, int lda, hipStream_t stream);
+                     double2* A, float2* A32, int lda, hipStream_t stream);
 void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream);
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
@@ -46,13 +46,14 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
 void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
                       LuStats* stats, int* status);
 int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
-void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, int batch, int n, int R,
-                         double tol, int max_iter, hipStream_t stream, LuStats* stats, int* status, int* iters,
-                         double* eta);
+void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2, int batch,
+                         int n, int R, bool a32_written, double tol, int max_iter, hipStream_t stream, LuStats* stats,
+                         int* status, int* iters, double* eta);
 int residual_blocks(int n);
+size_t residual_part_elems(int n, int R);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
-                            const StaticD* st, double2* S, double2* A, int lda, const NnDevice& nn, int2* queue,
-                            unsigned* qcount, hipStream_t stream);
+                            const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
+                            int2* queue, unsigned* qcount, hipStream_t stream);
 void launch_remainder(const FinFreq* f, FinTables F, WaveTables T, int n, const double* R, const double* z,
                       const double* zeta, double* out, hipStream_t stream);
 void nn_eval(const NnDevice& nn, const float4* X, int n, float4* Y, hipStream_t stream);
@@ -222,6 +223,10 @@ template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   void ensure(size_t count) {
     if (count <= n && p) return;
     if (p) HBG_HIP(hipFree(p));
@@ -294,6 +299,7 @@ struct HbgSession {
   DevBuf<float2> d_M32, d_D32;
   DevBuf<double2> d_X64;
   DevBuf<double> d_rpart;
+  DevBuf<double2> d_rpart2;
   NnDevice nn;          // the learned surrogate of the finite-depth evanescent remainder
   DevBuf<int2> d_nnq;   // pairs left to the FP64 evaluator by the surrogate's assembly
   DevBuf<unsigned> d_nnc;
@@ -393,10 +399,11 @@ FinTables fin_tables(HbgSession* s) {
 
 // S and A of the frequency K into S (ld N) and M (ld N): infinite depth, or finite depth with the
 // per-frequency tables built first; returns the wave number of the incident wave
-double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, bool surrogate = false) {
+double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, bool surrogate = false,
+                          float2* A32 = nullptr) {
   const int N = s->N;
   if (!std::isfinite(s->depth)) {
-    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, Mout, N, s->stream);
+    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, Mout, A32, N, s->stream);
     return K;
   }
   const FinFreq f = make_fin_freq(s->depth, K);
@@ -412,10 +419,10 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   if (surrogate && s->nn.ready && nuh >= std::exp(NN_LNNU_LO) && nuh <= std::exp(NN_LNNU_HI)) {
     s->d_nnq.ensure((size_t)N * N);
     s->d_nnc.ensure(1);
-    launch_assemble_fin_nn(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, N,
-                           s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
+    launch_assemble_fin_nn(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32,
+                           N, s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
   } else
-    launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, N,
+    launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, N,
                         s->stream);
   return f.k0;
 }
@@ -472,7 +479,9 @@ HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
   std::unique_ptr<HbgSession> s(new HbgSession());
   s->device = device;
   HBG_HIP(hipSetDevice(device));
-  HBG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  // a blocking stream: ordered with the null stream, so the synchronous copies and fills of the
+  // host code (hipMemcpy, hipMemset) complete before the session's kernels read their buffers
+  HBG_HIP(hipStreamCreate(&s->stream));
   for (auto& e : s->ev) HBG_HIP(hipEventCreate(&e));
   TablesOnDevice& t = tables_for(device);
   s->T.t2d = t.t2d;
@@ -800,6 +809,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     s->d_D32.ensure((size_t)N * R * B);
     s->d_X64.ensure((size_t)N * R * B);
     s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * B);
+    s->d_rpart2.ensure(residual_part_elems(N, R) * B);
   }
   std::vector<double> hout(nout * B);
   std::vector<int> iters(B);
@@ -819,7 +829,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     const double w = omegas[fi], K = w * w / s->grav;
     double2* Mb = s->d_M.p + mstride * b;
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], s->stream));
-    const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0);
+    const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0,
+                                         mixed ? s->d_M32.p + mstride * b : nullptr);
     launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
                     s->d_phiI.p + hstride * b, s->d_dphiI.p + hstride * b, s->d_u.p, s->stream);
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], s->stream));
@@ -849,8 +860,9 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       if (progress && progress(user, HBG_STAGE_LU, f0 + b, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     LuStats lus;
     if (mixed) {
-      mixed_solve_batched(s->d_M.p, s->d_M32.p, s->d_X64.p, s->d_D32.p, s->d_rpart.p, nb, N, R, MIXED_TOL,
-                          MIXED_MAX_ITER, s->stream, prof ? &lus : nullptr, status.data(), iters.data(), eta.data());
+      mixed_solve_batched(s->d_M.p, s->d_M32.p, s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p, nb, N, R, true,
+                          MIXED_TOL, MIXED_MAX_ITER, s->stream, prof ? &lus : nullptr, status.data(), iters.data(),
+                          eta.data());
       for (int b = 0; b < nb; ++b) {
         if (status[b] < 0) {  // the plain FP64 path for this frequency
           build(f0 + b, b);

@@ -22,19 +22,23 @@ namespace hbg {
 // workgroup takes a 64 x 64 tile (k-block i <= n-block j; the diagonal tiles the pairs k <= n),
 // the 64 lanes of a wavefront along k and 16 n per wavefront, and writes (k, n) (coalesced) and
 // (n, k) (one column per lane; the wavefront's consecutive n fill the same cache lines).
-__device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __restrict__ A, int lda, int row, int col,
-                                          const StaticD& e, double Ar, double nz, double a, double c2, double Wr,
-                                          double WrR, double WrZ, double Wi, double WiR, double WiZ) {
+__device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __restrict__ A, float2* __restrict__ A32,
+                                          int lda, int row, int col, const StaticD& e, double Ar, double nz, double a,
+                                          double c2, double Wr, double WrR, double WrZ, double Wi, double WiR,
+                                          double WiZ) {
   S[(size_t)col * lda + row] = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
   const double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
-  A[(size_t)col * lda + row] = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
+  const double2 v = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
+  A[(size_t)col * lda + row] = v;
+  if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
 }
 
 constexpr int ASM_TILE = 64, ASM_NPW = 16;
 
 __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict__ P, int N, double K, double logc,
                                                        WaveTables T, const StaticD* __restrict__ st,
-                                                       double2* __restrict__ S, double2* __restrict__ A, int lda) {
+                                                       double2* __restrict__ S, double2* __restrict__ A,
+                                                       float2* __restrict__ A32, int lda) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -57,10 +61,10 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
     const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
     const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
     const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
-    put_entry(S, A, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
+    put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
     if (k != n) {
       const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
-      put_entry(S, A, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
+      put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
     }
   }
 }
@@ -71,7 +75,8 @@ __global__ void __launch_bounds__(256) assemble_fin_kernel(const PanelD* __restr
                                                            const FinFreq* __restrict__ f,
                                                            FinTables F, WaveTables T, double logc,
                                                            const StaticD* __restrict__ st, double2* __restrict__ S,
-                                                           double2* __restrict__ A, int lda) {
+                                                           double2* __restrict__ A, float2* __restrict__ A32,
+                                                           int lda) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -89,11 +94,11 @@ __global__ void __launch_bounds__(256) assemble_fin_kernel(const PanelD* __restr
     const double R = sqrt(dx * dx + dy * dy);
     const FinK w = fin_kernel(f, F, T, k, n, R, zk, pn->c[2]);
     const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
-    put_entry(S, A, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
+    put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
               w.WiZ);
     if (k != n) {
       const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
-      put_entry(S, A, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
+      put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
                 w.Wiz);
     }
   }
@@ -324,17 +329,18 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
   }
 }
 
+// A32 (nullable): A in complex64 too
 void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
-                     double2* A, int lda, hipStream_t stream) {
+                     double2* A, float2* A32, int lda, hipStream_t stream) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
-  assemble_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, K, logc, T, st, S, A, lda);
+  assemble_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, K, logc, T, st, S, A, A32, lda);
   HBG_HIP(hipGetLastError());
 }
 
 void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
-                         const StaticD* st, double2* S, double2* A, int lda, hipStream_t stream) {
+                         const StaticD* st, double2* S, double2* A, float2* A32, int lda, hipStream_t stream) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
-  assemble_fin_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda);
+  assemble_fin_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda);
   HBG_HIP(hipGetLastError());
 }
 

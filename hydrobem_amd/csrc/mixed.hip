@@ -43,27 +43,32 @@ __global__ void widen_kernel(const float2* __restrict__ M32, size_t mstride, int
   }
 }
 
-// D = (complex64) (b - A X) for the system [A | b] (n x (n + R), ld n) in FP64, one row per
-// thread, X staged through LDS; per workgroup the maxima over its rows of max_c |r_ic|,
-// sum_j |a_ij|, max_c |x_ic|, max_c |b_ic| (|re| + |im|) go to part[z][block][4].
-__global__ void __launch_bounds__(RES_THREADS) residual_kernel(const double2* __restrict__ M, size_t mstride, int n,
-                                                               const double2* __restrict__ X, size_t xstride, int R,
-                                                               float2* __restrict__ D, size_t dstride,
-                                                               double* __restrict__ part) {
+// The residual b - A X of the systems [A | b] (n x (n + R), ld n) in FP64, in two passes:
+// residual_part_kernel: the products of column chunk cy (n / RES_SPLIT columns) for one row per
+//   thread, X staged through LDS, into part2[z][cy][R + 1][n] (the last row: sum_j |a_ij|);
+// residual_sum_kernel: the chunks summed in a fixed order, D = (complex64) r, and per workgroup
+//   the maxima over its rows of max_c |r_ic|, sum_j |a_ij|, max_c |x_ic|, max_c |b_ic| (|re| + |im|)
+//   in part[z][block][4].
+constexpr int RES_SPLIT = 8;
+
+__global__ void __launch_bounds__(RES_THREADS) residual_part_kernel(const double2* __restrict__ M, size_t mstride,
+                                                                    int n, const double2* __restrict__ X,
+                                                                    size_t xstride, int R,
+                                                                    double2* __restrict__ part2) {
   __shared__ double2 xs[RES_JC][RES_MAXR];
-  __shared__ double red[4][RES_THREADS / 64];
   M += blockIdx.z * mstride;
   X += blockIdx.z * xstride;
-  D += blockIdx.z * dstride;
   const int tid = threadIdx.x;
   const int i = blockIdx.x * RES_THREADS + tid;
   const bool live = i < n;
+  const int cw = (n + RES_SPLIT - 1) / RES_SPLIT;
+  const int jlo = blockIdx.y * cw, jhi = min(n, jlo + cw);
   double2 acc[RES_MAXR];
 #pragma unroll
   for (int c = 0; c < RES_MAXR; ++c) acc[c] = double2(0.0, 0.0);
   double rowsum = 0.0;
-  for (int j0 = 0; j0 < n; j0 += RES_JC) {
-    const int jn = min(RES_JC, n - j0);
+  for (int j0 = jlo; j0 < jhi; j0 += RES_JC) {
+    const int jn = min(RES_JC, jhi - j0);
     __syncthreads();
     for (int e = tid; e < RES_JC * RES_MAXR; e += RES_THREADS) {
       const int jj = e % RES_JC, c = e / RES_JC;
@@ -85,19 +90,44 @@ __global__ void __launch_bounds__(RES_THREADS) residual_kernel(const double2* __
       }
     }
   }
-  double rmax = 0.0, xmax = 0.0, bmax = 0.0;
-  if (live) {
+  if (!live) return;
+  double2* o = part2 + (((size_t)blockIdx.z * RES_SPLIT + blockIdx.y) * (R + 1)) * n + i;
 #pragma unroll
-    for (int c = 0; c < RES_MAXR; ++c) {
-      if (c < R) {
-        const double2 b = M[(size_t)(n + c) * n + i];
-        const double2 x = X[(size_t)c * n + i];
-        const double rx = b.x - acc[c].x, ry = b.y - acc[c].y;
-        D[(size_t)c * n + i] = float2((float)rx, (float)ry);
-        rmax = fmax(rmax, fabs(rx) + fabs(ry));
-        xmax = fmax(xmax, fabs(x.x) + fabs(x.y));
-        bmax = fmax(bmax, fabs(b.x) + fabs(b.y));
+  for (int c = 0; c < RES_MAXR; ++c)
+    if (c < R) o[(size_t)c * n] = acc[c];
+  o[(size_t)R * n] = double2(rowsum, 0.0);
+}
+
+__global__ void __launch_bounds__(RES_THREADS) residual_sum_kernel(const double2* __restrict__ M, size_t mstride,
+                                                                   int n, const double2* __restrict__ X,
+                                                                   size_t xstride, int R,
+                                                                   const double2* __restrict__ part2,
+                                                                   float2* __restrict__ D, size_t dstride,
+                                                                   double* __restrict__ part) {
+  __shared__ double red[4][RES_THREADS / 64];
+  M += blockIdx.z * mstride;
+  X += blockIdx.z * xstride;
+  D += blockIdx.z * dstride;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * RES_THREADS + tid;
+  double rmax = 0.0, rowsum = 0.0, xmax = 0.0, bmax = 0.0;
+  if (i < n) {
+    const double2* p = part2 + ((size_t)blockIdx.z * RES_SPLIT * (R + 1)) * n + i;
+    for (int cy = 0; cy < RES_SPLIT; ++cy) rowsum += p[((size_t)cy * (R + 1) + R) * n].x;
+    for (int c = 0; c < R; ++c) {
+      double2 sacc = double2(0.0, 0.0);
+      for (int cy = 0; cy < RES_SPLIT; ++cy) {
+        const double2 v = p[((size_t)cy * (R + 1) + c) * n];
+        sacc.x += v.x;
+        sacc.y += v.y;
       }
+      const double2 b = M[(size_t)(n + c) * n + i];
+      const double2 x = X[(size_t)c * n + i];
+      const double rx = b.x - sacc.x, ry = b.y - sacc.y;
+      D[(size_t)c * n + i] = float2((float)rx, (float)ry);
+      rmax = fmax(rmax, fabs(rx) + fabs(ry));
+      xmax = fmax(xmax, fabs(x.x) + fabs(x.y));
+      bmax = fmax(bmax, fabs(b.x) + fabs(b.y));
     }
   }
   double v[4] = {rmax, rowsum, xmax, bmax};
@@ -149,20 +179,26 @@ void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, fl
                   int nrhs, hipStream_t stream);
 
 int residual_blocks(int n) { return (n + RES_THREADS - 1) / RES_THREADS; }
+// the chunk partials of the residual, per system
+size_t residual_part_elems(int n, int R) { return (size_t)RES_SPLIT * (R + 1) * n; }
 
 // Solve the `batch` FP64 systems M64 (n x (n + R), ld n, stride n (n + R)) in mixed precision:
-// M32 (same shape, complex64) and D (n x R complex64), X (n x R complex128), part
-// (batch x residual_blocks(n) x 4) are workspaces. On return the solution is in the columns
+// M32 (same shape, complex64; its first n columns already written by the assembly when
+// a32_written) and D (n x R complex64), X (n x R complex128), part (batch x residual_blocks(n) x 4),
+// part2 (batch x residual_part_elems(n, R)) are workspaces. On return the solution is in the columns
 // [n, n + R) of M64 for the systems with status 0; status -1: the complex64 factorization needed
 // interchanges or the refinement did not reach the tolerance in max_iter steps (solve it in FP64);
 // k > 0: zero pivot k. iters[b], eta[b]: refinement steps and the final backward error.
-void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, int batch, int n, int R,
-                         double tol, int max_iter, hipStream_t stream, LuStats* stats, int* status, int* iters,
-                         double* eta) {
+void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2, int batch,
+                         int n, int R, bool a32_written, double tol, int max_iter, hipStream_t stream, LuStats* stats,
+                         int* status, int* iters, double* eta) {
   if (R > RES_MAXR) throw Error{HBG_ERR_UNSUPPORTED, "mixed precision: more than 32 right-hand sides"};
   const size_t ms = (size_t)n * (n + R), xs = (size_t)n * R;
   const int nblk = residual_blocks(n);
-  cast_kernel<<<dim3(grid_for(ms), 1, batch), 256, 0, stream>>>(M64, ms, M32, ms, ms);
+  if (a32_written)  // the assembly wrote A in complex64 too: the right-hand sides only
+    cast_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M64 + (size_t)n * n, ms, M32 + (size_t)n * n, ms, xs);
+  else
+    cast_kernel<<<dim3(grid_for(ms), 1, batch), 256, 0, stream>>>(M64, ms, M32, ms, ms);
   lu_solve_batched32(M32, ms, batch, n, n, R, stream, stats, status);
   widen_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M32, ms, n, X, xs, R);
   std::vector<double> hp((size_t)batch * nblk * 4);
@@ -173,7 +209,8 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
   bool open = false;
   for (int b = 0; b < batch; ++b) open = open || status[b] == 0;
   for (int it = 0; open; ++it) {
-    residual_kernel<<<dim3(nblk, 1, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, D, xs, part);
+    residual_part_kernel<<<dim3(nblk, RES_SPLIT, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
+    residual_sum_kernel<<<dim3(nblk, 1, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, D, xs, part);
     HBG_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     HBG_HIP(hipStreamSynchronize(stream));
     open = false;

@@ -377,28 +377,31 @@ __global__ void __launch_bounds__(256) nn_eval_kernel(const v8bf* __restrict__ g
 // derivative in the source depth, its fourth the one in the field depth. A wave takes 64 field
 // points (k) against NN_PER_WAVE sources (n); the packed weights are staged once per workgroup.
 constexpr int NN_PER_WAVE = 16;
-__device__ __forceinline__ void write_entry(double2* __restrict__ S, double2* __restrict__ A, int lda, int row, int col,
-                                            const StaticD& e, double Ar, double nz, double a, double c2, double Wr,
-                                            double WrR, double WrZ, double Wi, double WiR, double WiZ) {
+__device__ __forceinline__ void write_entry(double2* __restrict__ S, double2* __restrict__ A,
+                                            float2* __restrict__ A32, int lda, int row, int col, const StaticD& e,
+                                            double Ar, double nz, double a, double c2, double Wr, double WrR,
+                                            double WrZ, double Wi, double WiR, double WiZ) {
   S[(size_t)col * lda + row] = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
   const double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
-  A[(size_t)col * lda + row] = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
+  const double2 v = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
+  A[(size_t)col * lda + row] = v;
+  if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
 }
 
 // both entries of the pair (k, n), k <= n, from the pair's kernel values
 __device__ __forceinline__ void write_pair(const PanelD* __restrict__ P, int N, const StaticD* __restrict__ st,
                                            const FinK& w, double c2, int k, int n, int lda, double2* __restrict__ S,
-                                           double2* __restrict__ A) {
+                                           double2* __restrict__ A, float2* __restrict__ A32) {
   const PanelD* pk = P + k;
   const PanelD* pn = P + n;
   const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
   const double R = sqrt(dx * dx + dy * dy);
   const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
-  write_entry(S, A, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
+  write_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
               w.WiZ);
   if (k != n) {
     const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
-    write_entry(S, A, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
+    write_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
                 w.Wiz);
   }
 }
@@ -406,8 +409,8 @@ __device__ __forceinline__ void write_pair(const PanelD* __restrict__ P, int N, 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __restrict__ f, FinTables F, WaveTables T,
                        double logc, const StaticD* __restrict__ st, double2* __restrict__ S, double2* __restrict__ A,
-                       int lda, const v8bf* __restrict__ gpk, const float* __restrict__ gbias, int2* __restrict__ queue,
-                       unsigned* __restrict__ qcount) {
+                       float2* __restrict__ A32, int lda, const v8bf* __restrict__ gpk, const float* __restrict__ gbias,
+                       int2* __restrict__ queue, unsigned* __restrict__ qcount) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;  // workgroup-uniform
   __shared__ v8bf pk[NN_PK_V8];
@@ -451,7 +454,7 @@ assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __res
     w.WiR = pp.a * k0 * pp.j1;
     w.WiZ = -pp.az * pp.j0;
     w.Wiz = -pp.af * pp.j0;
-    write_pair(P, N, st, w, c2, k, n, lda, S, A);
+    write_pair(P, N, st, w, c2, k, n, lda, S, A, A32);
   }
 }
 
@@ -459,7 +462,8 @@ assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __res
 __global__ void __launch_bounds__(256) fallback_kernel(const PanelD* __restrict__ P, int N,
                                                        const FinFreq* __restrict__ f, FinTables F, WaveTables T,
                                                        double logc, const StaticD* __restrict__ st,
-                                                       double2* __restrict__ S, double2* __restrict__ A, int lda,
+                                                       double2* __restrict__ S, double2* __restrict__ A,
+                                                       float2* __restrict__ A32, int lda,
                                                        const int2* __restrict__ queue,
                                                        const unsigned* __restrict__ qcount) {
   const unsigned count = *qcount;
@@ -469,7 +473,7 @@ __global__ void __launch_bounds__(256) fallback_kernel(const PanelD* __restrict_
     const int k = q.x, n = q.y;
     const double dx = P[k].c[0] - P[n].c[0], dy = P[k].c[1] - P[n].c[1];
     const FinK w = fin_kernel(f, F, T, k, n, sqrt(dx * dx + dy * dy), P[k].c[2], P[n].c[2]);
-    write_pair(P, N, st, w, c2, k, n, lda, S, A);
+    write_pair(P, N, st, w, c2, k, n, lda, S, A, A32);
   }
 }
 
@@ -560,6 +564,7 @@ void NnDevice::upload(const float* prm) {
   if (!bias) HBG_HIP(hipMalloc(&bias, hb.size() * sizeof(float)));
   HBG_HIP(hipMemcpy(pk, hpk.data(), hpk.size() * sizeof(__bf16), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(bias, hb.data(), hb.size() * sizeof(float), hipMemcpyHostToDevice));
+  HBG_HIP(hipDeviceSynchronize());  // the DMA of a pageable copy may outlive hipMemcpy's return
   params.assign(prm, prm + NN_PARAMS);
   ready = true;
 }
@@ -672,10 +677,10 @@ double nn_gradient(const float* P_host, const float* X_host, const float* Y_host
   X.ensure((size_t)B * NN_IN);
   Y.ensure(B);
   loss.ensure(1);
-  HBG_HIP(hipMemcpy(P.p, P_host, NN_PARAMS * sizeof(float), hipMemcpyHostToDevice));
-  HBG_HIP(hipMemcpy(X.p, X_host, (size_t)B * NN_IN * sizeof(float), hipMemcpyHostToDevice));
-  HBG_HIP(hipMemcpy(Y.p, Y_host, (size_t)B * sizeof(float4), hipMemcpyHostToDevice));
-  HBG_HIP(hipMemset(loss.p, 0, sizeof(double)));
+  HBG_HIP(hipMemcpyAsync(P.p, P_host, NN_PARAMS * sizeof(float), hipMemcpyHostToDevice, st));
+  HBG_HIP(hipMemcpyAsync(X.p, X_host, (size_t)B * NN_IN * sizeof(float), hipMemcpyHostToDevice, st));
+  HBG_HIP(hipMemcpyAsync(Y.p, Y_host, (size_t)B * sizeof(float4), hipMemcpyHostToDevice, st));
+  HBG_HIP(hipMemsetAsync(loss.p, 0, sizeof(double), st));
   Acts a;
   forward(P.p, X.p, B, a, st);
   a.dB.ensure((size_t)B * NN_H);
@@ -725,7 +730,7 @@ void nn_make_data(unsigned long long seed, unsigned long long first, int n, int 
     F.ktab = ktab;
     F.pf = pf.p;
     F.pcs = pcs.p;
-    target_kernel<<<blocks(m), 128, 0, st>>>(ff.p, F, T, m, R.p, z.p, zeta.p, X + g0, Y + g0);
+    target_kernel<<<(unsigned)((m + 127) / 128), 128, 0, st>>>(ff.p, F, T, m, R.p, z.p, zeta.p, X + g0, Y + g0);
     HBG_HIP(hipGetLastError());
     HBG_HIP(hipStreamSynchronize(st));  // the group's host-side FinFreq is reused next
   }
@@ -735,7 +740,8 @@ void nn_make_data(unsigned long long seed, unsigned long long first, int n, int 
 // `batch`, learning rate lr0 decayed by cosine to lr0 / 100. The parameters go to P (NnParams
 // layout); report: [0] final training loss (normalized MSE), [1] held-out RMS error of the BF16
 // network (normalized units), [2..5] held-out max |error| per output (normalized), [6] data
-// seconds, [7] training seconds, [8] steps.
+// seconds, [7] training seconds, [8] steps, [9] non-finite targets among the first 2^20 samples,
+// [10] non-finite parameters after the training.
 void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int batch, float lr0, const WaveTables& T,
               const double* ktab, const double* fitm, const double* g16, std::vector<float>& P, double* report,
               NnDevice& nn, hipStream_t st) {
@@ -749,20 +755,27 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
   // per-output mean and standard deviation (from up to 2^20 training samples); the network learns
   // the standardized targets and the last layer is scaled back after the training
   double mu[4] = {0, 0, 0, 0}, sd[4] = {0, 0, 0, 0};
+  long nonfinite = 0;
   {
     const int m = std::min(n_train, 1 << 20);
     std::vector<float4> hy(m);
-    HBG_HIP(hipMemcpy(hy.data(), Y.p, m * sizeof(float4), hipMemcpyDeviceToHost));
+    HBG_HIP(hipMemcpyAsync(hy.data(), Y.p, m * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HBG_HIP(hipStreamSynchronize(st));
     for (const float4& v : hy) {
       const double a[4] = {v.x, v.y, v.z, v.w};
+      if (!(std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]) && std::isfinite(a[3]))) {
+        ++nonfinite;
+        continue;
+      }
       for (int q = 0; q < 4; ++q) {
         mu[q] += a[q];
         sd[q] += a[q] * a[q];
       }
     }
+    const double me = (double)std::max<long>(m - nonfinite, 1);
     for (int q = 0; q < 4; ++q) {
-      mu[q] /= m;
-      sd[q] = std::sqrt(std::max(sd[q] / m - mu[q] * mu[q], 1e-30));
+      mu[q] /= me;
+      sd[q] = std::sqrt(std::max(sd[q] / me - mu[q] * mu[q], 1e-30));
     }
     standardize_kernel<<<blocks(n), 256, 0, st>>>(Y.p, n, make_float4((float)mu[0], (float)mu[1], (float)mu[2], (float)mu[3]),
                                                   make_float4((float)(1.0 / sd[0]), (float)(1.0 / sd[1]),
@@ -782,9 +795,9 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
   Yb.ensure(batch);
   loss.ensure(1);
   perm.ensure(n_train);
-  HBG_HIP(hipMemcpy(dP.p, P.data(), NN_PARAMS * sizeof(float), hipMemcpyHostToDevice));
-  HBG_HIP(hipMemset(dM.p, 0, NN_PARAMS * sizeof(float)));
-  HBG_HIP(hipMemset(dV.p, 0, NN_PARAMS * sizeof(float)));
+  HBG_HIP(hipMemcpyAsync(dP.p, P.data(), NN_PARAMS * sizeof(float), hipMemcpyHostToDevice, st));
+  HBG_HIP(hipMemsetAsync(dM.p, 0, NN_PARAMS * sizeof(float), st));
+  HBG_HIP(hipMemsetAsync(dV.p, 0, NN_PARAMS * sizeof(float), st));
   std::vector<int> hperm(n_train);
   for (int i = 0; i < n_train; ++i) hperm[i] = i;
   unsigned long long rs = seed ^ 0xA5A5A5A5ull;
@@ -799,7 +812,8 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
       rs = rs * 6364136223846793005ull + 1442695040888963407ull;
       std::swap(hperm[i], hperm[(size_t)(rs >> 33) % (size_t)(i + 1)]);
     }
-    HBG_HIP(hipMemcpy(perm.p, hperm.data(), n_train * sizeof(int), hipMemcpyHostToDevice));
+    HBG_HIP(hipStreamSynchronize(st));  // the previous epoch has read perm
+    HBG_HIP(hipMemcpyAsync(perm.p, hperm.data(), n_train * sizeof(int), hipMemcpyHostToDevice, st));
     for (int b = 0; b < per_epoch; ++b, ++step) {
       gather_kernel<<<blocks(batch), 256, 0, st>>>(X.p, Y.p, perm.p, b * batch, batch, Xb.p, Yb.p);
       forward(dP.p, Xb.p, batch, a, st);
@@ -820,7 +834,8 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
     }
   }
   HBG_HIP(hipGetLastError());
-  HBG_HIP(hipMemcpy(P.data(), dP.p, NN_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+  HBG_HIP(hipMemcpyAsync(P.data(), dP.p, NN_PARAMS * sizeof(float), hipMemcpyDeviceToHost, st));
+  HBG_HIP(hipStreamSynchronize(st));
   for (int q = 0; q < NN_OUT; ++q) {  // undo the standardization in the last layer
     for (int i = 0; i < NN_H; ++i) P[NN_OFF_W4 + q * NN_H + i] *= (float)sd[q];
     P[NN_OFF_B4 + q] = (float)(P[NN_OFF_B4 + q] * sd[q] + mu[q]);
@@ -837,8 +852,9 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
   Yh.ensure(std::max(n_hold, 1));
   nn_eval(nn, X.p + n_train, n_hold, Yh.p, st);
   std::vector<float4> ht(n_hold), hp(n_hold);
-  HBG_HIP(hipMemcpy(ht.data(), Y.p + n_train, n_hold * sizeof(float4), hipMemcpyDeviceToHost));
-  HBG_HIP(hipMemcpy(hp.data(), Yh.p, n_hold * sizeof(float4), hipMemcpyDeviceToHost));
+  HBG_HIP(hipMemcpyAsync(ht.data(), Y.p + n_train, n_hold * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HBG_HIP(hipMemcpyAsync(hp.data(), Yh.p, n_hold * sizeof(float4), hipMemcpyDeviceToHost, st));
+  HBG_HIP(hipStreamSynchronize(st));
   double se = 0.0, mx[4] = {0, 0, 0, 0};
   for (int i = 0; i < n_hold; ++i) {
     const float d[4] = {hp[i].x - ht[i].x, hp[i].y - ht[i].y, hp[i].z - ht[i].z, hp[i].w - ht[i].w};
@@ -849,6 +865,10 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
   }
   for (int q = 0; q < 4; ++q) mx[q] /= sd[q];  // in units of each output's standard deviation
   se /= (sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2] + sd[3] * sd[3]) / 4.0;
+  long bad_params = 0;
+  for (float v : P) bad_params += std::isfinite(v) ? 0 : 1;
+  report[9] = (double)nonfinite;
+  report[10] = (double)bad_params;
   report[0] = last;
   report[1] = n_hold > 0 ? std::sqrt(se / (4.0 * n_hold)) : 0.0;
   for (int q = 0; q < 4; ++q) report[2 + q] = mx[q];
@@ -859,14 +879,14 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
 
 // queue: N (N + 1) / 2 pairs of room (every pair may fall back), qcount: one counter
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
-                            const StaticD* st, double2* S, double2* A, int lda, const NnDevice& nn, int2* queue,
-                            unsigned* qcount, hipStream_t stream) {
+                            const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
+                            int2* queue, unsigned* qcount, hipStream_t stream) {
   zero_u32_kernel<<<1, 64, 0, stream>>>(qcount);
   const unsigned nt = (unsigned)((N + 63) / 64);
   dim3 grid(nt, nt);
-  assemble_fin_nn_kernel<<<grid, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda, (const v8bf*)nn.pk, nn.bias,
-                                                   queue, qcount);
-  fallback_kernel<<<2048, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, lda, queue, qcount);
+  assemble_fin_nn_kernel<<<grid, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda, (const v8bf*)nn.pk,
+                                                   nn.bias, queue, qcount);
+  fallback_kernel<<<2048, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda, queue, qcount);
   HBG_HIP(hipGetLastError());
 }
 

@@ -169,13 +169,14 @@ typedef struct HbgSurrogateTraining {
   int32_t batch;           /* minibatch (>= 64, <= n_train) */
   double learning_rate;    /* Adam, decayed by cosine to 1 % */
 } HbgSurrogateTraining;
-#define HBG_SURROGATE_REPORT 9
+#define HBG_SURROGATE_REPORT 12
 /* Parameters of the network (FP32, output-major weights W1 [128][4], b1, W2 [128][128], b2, W3, b3,
  * W4 [4][128], b4). */
 int32_t hbg_surrogate_size(void);
 /* Train on the GPU (sampling, the FP64 evaluator's targets, forward and backward passes, Adam) and
  * keep the network in the session. report [9]: final training loss, held-out RMS error (network
- * output units), held-out max |error| of the four outputs, data seconds, training seconds, steps. */
+ * output units), held-out max |error| of the four outputs, data seconds, training seconds, steps,
+ * non-finite targets, non-finite parameters after the training, 0. */
 HbgStatus hbg_surrogate_train(HbgSession* s, const HbgSurrogateTraining* training, double* report);
 HbgStatus hbg_surrogate_set(HbgSession* s, const float* params, int32_t n);
 HbgStatus hbg_surrogate_get(HbgSession* s, float* params, int32_t n);

@@ -107,6 +107,7 @@ def test_short_training_learns(session):
     rep = session.surrogate_train(n_train=200_000, n_holdout=20_000, epochs=3, batch=2048, learning_rate=2e-3, seed=7)
     say("  " + ", ".join(f"{k} {v:.3g}" for k, v in rep.items()))
     assert rep["steps"] == 3 * (200_000 // 2048)
+    assert rep["nonfinite_targets"] == 0 and rep["nonfinite_params"] == 0
     # standardized targets: an untrained network sits near 1; 291 steps bring it well below
     assert np.isfinite(rep["train_loss"]) and rep["train_loss"] < 0.3
     assert rep["holdout_rms"] < 0.6
