@@ -31,6 +31,10 @@ import numpy as np
 METRIC = "freq solves/s @8192 panels: mixed+NN vs plain FP64 GPU, A/B/X rel err<=1e-5"
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
+# HBM bytes per launch of the path's GEMM kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# calibrated by a device copy of known size (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md):
+# (workload, path, GEMM tiles, panels per trailing update, batch) -> bytes
+TRAFFIC = {("cylinder8192fd", "mixed", "64x64", 8, 16): 1.076e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on FP32 MFMA, FP64 iterative refinement to backward error 1e-12",
              "mixed_nn": "mixed+NN: finite-depth evanescent remainder from the BF16-MFMA surrogate (FP64 evaluator for "
@@ -198,7 +202,20 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed,mixed_nn", help="paths timed, the last one gives `value`")
     ap.add_argument("--no-autotune", action="store_true", help="skip the LU autotuner before each path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only copy this many GiB device to device and exit (the calibration of HBM counters)")
     a = ap.parse_args()
+    if a.copy_probe > 0:
+        import torch
+
+        n = int(a.copy_probe * (1 << 30)) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}), flush=True)
+        return 0
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the frequency sweep "
               f"would shard across GPUs with no exchange, see the design notes)", file=sys.stderr)
@@ -272,6 +289,13 @@ def main():
             config[p]["speedup_vs_fp64_gpu"] = round(res[p][1]["freq_per_s"] / res["fp64"][1]["freq_per_s"], 3)
         s2.close()
     gemm_tf = summ["gemm_achieved_tflops"]
+    # the LU settings the value path ran with: autotuned, or given on the command line, or the defaults
+    tun = tuning.get("fp64" if main_path == "fp64" else "mixed",
+                     {"gemm_tiles": "128x64" if a.gemm == 2 else "64x64",
+                      "panels_per_update": a.depth if a.depth >= 0 else 4,
+                      "batch": a.batch or None})
+    traffic = TRAFFIC.get((a.workload, main_path, tun["gemm_tiles"], tun["panels_per_update"], tun["batch"]))
+    tile = tun["gemm_tiles"].replace("x", ", ")
     line = {
         "metric": METRIC,
         "value": summ["freq_per_s"],
@@ -286,10 +310,10 @@ def main():
         "dtype": DTYPE[main_path],
         "data": "synthetic: procedural mesh of the workload (tests/python/meshes.py layout)",
         "config": config,
-        "roofline": {"bound": "mfma", "kernel": f"gemm_mfma_kernel<{'double' if main_path == 'fp64' else 'float'}, 64, 64> "
+        "roofline": {"bound": "mfma", "kernel": f"gemm_mfma_kernel<{'double' if main_path == 'fp64' else 'float'}, {tile}> "
                                                 "(LU trailing updates, complex GEMM on MFMA)",
                      "achieved": gemm_tf, "peak": peaks[main_path] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(gemm_tf * 1e12 / peaks[main_path], 4), "traffic": None,
+                     "frac": round(gemm_tf * 1e12 / peaks[main_path], 4), "traffic": traffic,
                      "per_launch": summ["gemm_per_launch"]},
     }
     if not a.no_cpu_baseline:
