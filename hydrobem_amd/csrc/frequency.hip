@@ -33,45 +33,130 @@ __device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __re
   if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
 }
 
-constexpr int ASM_TILE = 64, ASM_NPW = 16;
+constexpr int ASM_TILE = 64;
 
+// The tables make the kernels' cost: a pair reads hundreds of coefficients (the V tile and the 1D
+// tile in infinite depth; the two Phi tiles and the V tile near the axis, one K0/K1 tile per root
+// in the series of finite depth), and read lane by lane they bound the kernel by the vector
+// memory pipeline. So a workgroup first sorts its tile's pairs by table tile (a counting sort in
+// LDS), and each wavefront then evaluates 64 consecutive sorted pairs with tile reads that are
+// wavefront-uniform: scalar loads, made once per distinct tile of the wavefront.
+constexpr int PS_KEYS = 4096;                              // keys of the sort
+constexpr int PS_PER_THREAD = ASM_TILE * ASM_TILE / 256;   // pairs per thread of the tile
+struct PairSort {
+  unsigned cnt[PS_KEYS];
+  unsigned short order[ASM_TILE * ASM_TILE];  // pair p = k_local + 64 n_local, in key order
+  unsigned wsum[4];
+};
+
+// Sorts the pairs (k, n) of the tile (bi, bj) (k <= n on the diagonal tile, both < N) by
+// key_of(k, n) in [0, PS_KEYS); returns their number. Order within a key is arbitrary: every
+// pair is evaluated on its own.
+template <class KeyOf>
+__device__ __forceinline__ unsigned sort_tile_pairs(PairSort& L, int bi, int bj, int N, KeyOf key_of) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < PS_KEYS; i += 256) L.cnt[i] = 0;
+  __syncthreads();
+  unsigned kr[PS_PER_THREAD];  // key << 13 | rank within the key, or ~0 for no pair
+#pragma unroll
+  for (int j = 0; j < PS_PER_THREAD; ++j) {
+    const int p = tid + 256 * j, k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
+    kr[j] = 0xffffffffu;
+    if (k < N && n < N && !(bi == bj && k > n)) {
+      const int kf = key_of(k, n);
+      const unsigned key = kf < 0 ? 0u : (kf >= PS_KEYS ? (unsigned)PS_KEYS - 1 : (unsigned)kf);
+      kr[j] = (key << 13) | atomicAdd(&L.cnt[key], 1u);
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the counts: thread t owns cnt[16 t .. 16 t + 15]
+  constexpr int OWN = PS_KEYS / 256;
+  unsigned loc[OWN], sum = 0;
+#pragma unroll
+  for (int i = 0; i < OWN; ++i) {
+    loc[i] = sum;
+    sum += L.cnt[OWN * tid + i];
+  }
+  unsigned incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned o = __shfl_up(incl, off);
+    if (lane >= off) incl += o;
+  }
+  if (lane == 63) L.wsum[wave] = incl;
+  __syncthreads();
+  unsigned base = incl - sum;
+  for (int w = 0; w < wave; ++w) base += L.wsum[w];
+  const unsigned total = L.wsum[0] + L.wsum[1] + L.wsum[2] + L.wsum[3];
+#pragma unroll
+  for (int i = 0; i < OWN; ++i) L.cnt[OWN * tid + i] = base + loc[i];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PS_PER_THREAD; ++j)
+    if (kr[j] != 0xffffffffu) L.order[L.cnt[kr[j] >> 13] + (kr[j] & 8191u)] = (unsigned short)(tid + 256 * j);
+  __syncthreads();
+  return total;
+}
+
+// Infinite depth: the pairs sorted by the V tile of (X, Y) = (K R, -K (z + zeta)) (then also
+// their 1D tile of X is shared); pairs beyond the tables by their 1D tile, or in one bin.
 __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict__ P, int N, double K, double logc,
                                                        WaveTables T, const StaticD* __restrict__ st,
                                                        double2* __restrict__ S, double2* __restrict__ A,
                                                        float2* __restrict__ A32, int lda) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = bi * ASM_TILE + lane;
-  if (k >= N) return;
-  const PanelD* pk = P + k;
-  const double xk = pk->c[0], yk = pk->c[1], zk = pk->c[2];
-  const double c2 = 2.0 * K * logc, K2 = K * K;
-  for (int t = 0; t < ASM_NPW; ++t) {
-    const int n = bj * ASM_TILE + wave * ASM_NPW + t;
-    if (n >= N) break;
-    if (bi == bj && k > n) continue;
+  __shared__ PairSort L;
+  static_assert(XT * YT + XT + 1 <= PS_KEYS, "infinite-depth keys");
+  const unsigned total = sort_tile_pairs(L, bi, bj, N, [&](int k, int n) {
+    const PanelD* pk = P + k;
     const PanelD* pn = P + n;
-    const double dx = xk - pn->c[0], dy = yk - pn->c[1];
+    const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
+    const double X = K * sqrt(dx * dx + dy * dy), Y = -K * (pk->c[2] + pn->c[2]);
+    if (X >= (double)XT) return XT * YT + XT;
+    double tx, ty;
+    if (Y >= (double)YT) return XT * YT + tile1d(X, tx);
+    return tile2d(X, Y, tx, ty);
+  });
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double c2 = 2.0 * K * logc, K2 = K * K;
+  const int chunks = (int)((total + 63) / 64);
+  for (int ch = wave; ch < chunks; ch += 4) {
+    const unsigned pos = (unsigned)ch * 64 + lane;
+    const bool valid = pos < total;
+    const int p = valid ? L.order[pos] : 0;
+    const int k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
+    const PanelD* pk = P + k;
+    const PanelD* pn = P + n;
+    const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy);
-    const double X = K * R, Y = -K * (zk + pn->c[2]);
-    const WaveK w = wave_kernel(T, X, Y);
-    const double R0 = sqrt(X * X + Y * Y);
-    const double FY = -w.F - 2.0 / R0, IY = -w.I;
-    const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
-    const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
-    const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
-    put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
-    if (k != n) {
-      const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
-      put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
+    const double X = K * R, Y = -K * (pk->c[2] + pn->c[2]);
+    const WaveK w = wave_kernel_u(T, valid, X, Y);
+    if (valid) {
+      const double R0 = sqrt(X * X + Y * Y);
+      const double FY = -w.F - 2.0 / R0, IY = -w.I;
+      const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
+      const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
+      const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+      put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
+      if (k != n) {
+        const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
+        put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR,
+                  WiZ);
+      }
     }
   }
 }
 
 // The matrices in finite depth: the same entries with the kernel of green_fin_dev.h, evaluated
 // at (field x_k, source c_n); (n, k) takes its field-depth derivative Wrz as its source derivative.
-__global__ void __launch_bounds__(256) assemble_fin_kernel(const PanelD* __restrict__ P, int N,
+//
+// The pairs are sorted by table tile: near the axis (R < h/2) by their Phi tiles (R/h, v1/h,
+// v2/h), in the series by R/h in bins of 1/32 (then the K0/K1 tiles of a root span one or two
+// unit tiles among a wavefront's lanes).
+constexpr int FS_NEAR = FTS * FTW * FTV;  // keys of the near pairs: (ix, iy1, iy2)
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) assemble_fin_kernel(const PanelD* __restrict__ P, int N,
                                                            const FinFreq* __restrict__ f,
                                                            FinTables F, WaveTables T, double logc,
                                                            const StaticD* __restrict__ st, double2* __restrict__ S,
@@ -79,27 +164,45 @@ __global__ void __launch_bounds__(256) assemble_fin_kernel(const PanelD* __restr
                                                            int lda) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = bi * ASM_TILE + lane;
-  if (k >= N) return;
-  const PanelD* pk = P + k;
-  const double xk = pk->c[0], yk = pk->c[1], zk = pk->c[2];
-  const double c2 = 2.0 * f->K * logc;
-  for (int t = 0; t < ASM_NPW; ++t) {
-    const int n = bj * ASM_TILE + wave * ASM_NPW + t;
-    if (n >= N) break;
-    if (bi == bj && k > n) continue;
+  __shared__ PairSort L;
+  static_assert(FS_NEAR + 1024 <= PS_KEYS, "finite-depth keys");
+  const double h = f->h, ih = 1.0 / h;
+  const unsigned total = sort_tile_pairs(L, bi, bj, N, [&](int k, int n) {
+    const PanelD* pk = P + k;
     const PanelD* pn = P + n;
-    const double dx = xk - pn->c[0], dy = yk - pn->c[1];
+    const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
+    const double R = sqrt(dx * dx + dy * dy), z = pk->c[2], zeta = pn->c[2];
+    if (R >= 0.5 * h) {
+      const double b = (R * ih - 0.5) * 32.0;
+      return FS_NEAR + (b < 1023.0 ? (int)b : 1023);
+    }
+    double tx, ty;
+    const int i1 = phi_index(FTW, 1.0, R * ih, fabs(z - zeta) * ih, tx, ty);  // ix FTW + iy1
+    const int i2 = phi_index(FTV, 2.0, R * ih, (z + zeta + 2.0 * h) * ih, tx, ty);  // ix FTV + iy2
+    return i1 * FTV + i2 % FTV;
+  });
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double c2 = 2.0 * f->K * logc;
+  const int chunks = (int)((total + 63) / 64);
+  for (int ch = wave; ch < chunks; ch += 4) {
+    const unsigned pos = (unsigned)ch * 64 + lane;
+    const bool valid = pos < total;
+    const int p = valid ? L.order[pos] : 0;
+    const int k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
+    const PanelD* pk = P + k;
+    const PanelD* pn = P + n;
+    const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy);
-    const FinK w = fin_kernel(f, F, T, k, n, R, zk, pn->c[2]);
-    const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
-    put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
-              w.WiZ);
-    if (k != n) {
-      const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
-      put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
-                w.Wiz);
+    const FinK w = fin_kernel_u(f, F, T, valid, k, n, R, pk->c[2], pn->c[2]);
+    if (valid) {
+      const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+      put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi,
+                w.WiR, w.WiZ);
+      if (k != n) {
+        const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
+        put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi,
+                  w.WiR, w.Wiz);
+      }
     }
   }
 }

@@ -19,12 +19,64 @@ struct WaveK {
   double F, FX, I, IX;  // Ft, dFt/dX, Gi, dGi/dX; Ft_Y = -Ft - 2/R0 and Gi_Y = -Gi
 };
 
-__device__ __forceinline__ void table1d(const double* __restrict__ t1d, double X, double& j0, double& j1,
-                                        double& yt, double& dyt) {
+// A table pointer that loads through the constant address space: where the tile index is the
+// same on every lane of the wavefront (fin_kernel_u below), such loads become scalar loads (one
+// fetch serves the 64 lanes); elsewhere they are ordinary vector loads.
+typedef __attribute__((address_space(4))) const double cdouble;
+
+// Tile u of a table whose tiles are `stride` doubles apart, as a pointer that the compiler cannot
+// tie to any lane's own index (the empty asm pins it to scalar registers; without it, inside
+// `if (key == u)` the compiler substitutes the lane's key and the loads become vector loads).
+__device__ __forceinline__ cdouble* uniform_tile(const double* base, int u, int stride) {
+  cdouble* c = (cdouble*)(base + (size_t)u * stride);
+  asm volatile("" : "+s"(c));
+  return c;
+}
+
+// Powers t^0 .. t^(M - 1) of a tile variable, for the power-basis evaluations below
+template <int M>
+__device__ __forceinline__ void powers(double t, double* pw) {
+  pw[0] = 1.0;
+  pw[1] = t;
+#pragma unroll
+  for (int a = 2; a < M; ++a) pw[a] = pw[a - 1] * t;
+}
+
+// sum_a c[a stride] t^a with the powers given: with c in scalar registers each term is one
+// fused multiply-add whose scalar operand is the coefficient (Horner's fma(p, t, c) would need
+// the coefficient copied into vector registers first)
+template <int M, int STRIDE>
+__device__ __forceinline__ double poly_u(cdouble* c, const double* pw) {
+  double r = c[0];
+#pragma unroll
+  for (int a = 1; a < M; ++a) r = fma(c[a * STRIDE], pw[a], r);
+  return r;
+}
+
+// Runs BODY once per distinct value of KEY among the lanes where ACTIVE holds: U is the
+// round's key, the same on every lane, and MINE tells whether this lane's key is U. With the
+// lanes' keys mostly equal the loop runs once or twice, and the tile loads inside are scalar.
+#define HBG_UNIFORM_ROUNDS(KEY, ACTIVE, U, MINE, BODY)                                          \
+  {                                                                                             \
+    unsigned long long todo_ = __ballot(ACTIVE);                                                \
+    while (todo_) {                                                                             \
+      const int U = __builtin_amdgcn_readlane((KEY), (int)__builtin_ctzll(todo_));             \
+      const bool MINE = (ACTIVE) && (KEY) == U;                                                 \
+      BODY                                                                                      \
+      todo_ &= ~__ballot(MINE);                                                                 \
+    }                                                                                           \
+  }
+
+// The 1D tile of x in [0, XT): index and tile variable t = 2 (x - i) - 1
+__device__ __forceinline__ int tile1d(double X, double& t) {
   int i = (int)X;
   if (i >= XT) i = XT - 1;
-  const double t = 2.0 * (X - (double)i) - 1.0;
-  const double* c = t1d + (size_t)i * TILE1;
+  t = 2.0 * (X - (double)i) - 1.0;
+  return i;
+}
+
+template <class P>
+__device__ __forceinline__ void table1d_eval(P c, double t, double& j0, double& j1, double& yt, double& dyt) {
   double r0 = c[4 * D1], r1 = c[4 * D1 + 1], r2 = c[4 * D1 + 2], r3 = c[4 * D1 + 3];
 #pragma unroll
   for (int j = D1 - 1; j >= 0; --j) {
@@ -39,29 +91,69 @@ __device__ __forceinline__ void table1d(const double* __restrict__ t1d, double X
   dyt = r3;
 }
 
-__device__ __forceinline__ void table2d(const double* __restrict__ t2d, double X, double Y, double& V, double& VX) {
+// table1d_eval and table2d_eval for a wavefront-uniform tile, in the power basis
+__device__ __forceinline__ void table1d_eval_u(cdouble* c, double t, double& j0, double& j1, double& yt, double& dyt) {
+  double pw[M1];
+  powers<M1>(t, pw);
+  j0 = poly_u<M1, 4>(c, pw);
+  j1 = poly_u<M1, 4>(c + 1, pw);
+  yt = poly_u<M1, 4>(c + 2, pw);
+  dyt = poly_u<M1, 4>(c + 3, pw);
+}
+
+__device__ __forceinline__ void table1d(const double* __restrict__ t1d, double X, double& j0, double& j1,
+                                        double& yt, double& dyt) {
+  double t;
+  const int i = tile1d(X, t);
+  table1d_eval(t1d + (size_t)i * TILE1, t, j0, j1, yt, dyt);
+}
+
+// The 2D tile of (X, Y) in [0, XT) x [0, YT): index iy XT + ix and the tile variables
+__device__ __forceinline__ int tile2d(double X, double Y, double& tx, double& ty) {
   int ix = (int)X, iy = (int)Y;
   if (ix >= XT) ix = XT - 1;
   if (iy >= YT) iy = YT - 1;
-  const double tx = 2.0 * (X - (double)ix) - 1.0;
-  const double ty = 2.0 * (Y - (double)iy) - 1.0;
-  const double* c = t2d + ((size_t)iy * XT + ix) * TILE2;
+  tx = 2.0 * (X - (double)ix) - 1.0;
+  ty = 2.0 * (Y - (double)iy) - 1.0;
+  return iy * XT + ix;
+}
+
+template <class P>
+__device__ __forceinline__ void table2d_eval(P c, double tx, double ty, double& V, double& VX) {
   double s0 = 0.0, s1 = 0.0;
 #pragma unroll
   for (int b = D2; b >= 0; --b) {
-    const double* c0 = c + b * M2;
-    const double* c1 = c + M2 * M2 + b * M2;
-    double r0 = c0[D2], r1 = c1[D2];
+    const int o0 = b * M2, o1 = M2 * M2 + b * M2;
+    double r0 = c[o0 + D2], r1 = c[o1 + D2];
 #pragma unroll
     for (int a = D2 - 1; a >= 0; --a) {
-      r0 = fma(r0, tx, c0[a]);
-      r1 = fma(r1, tx, c1[a]);
+      r0 = fma(r0, tx, c[o0 + a]);
+      r1 = fma(r1, tx, c[o1 + a]);
     }
     s0 = fma(s0, ty, r0);
     s1 = fma(s1, ty, r1);
   }
   V = s0;
   VX = s1;
+}
+
+__device__ __forceinline__ void table2d_eval_u(cdouble* c, double tx, double ty, double& V, double& VX) {
+  double pw[M2];
+  powers<M2>(tx, pw);
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int b = D2; b >= 0; --b) {
+    s0 = fma(s0, ty, poly_u<M2, 1>(c + b * M2, pw));
+    s1 = fma(s1, ty, poly_u<M2, 1>(c + M2 * M2 + b * M2, pw));
+  }
+  V = s0;
+  VX = s1;
+}
+
+__device__ __forceinline__ void table2d(const double* __restrict__ t2d, double X, double Y, double& V, double& VX) {
+  double tx, ty;
+  const int i = tile2d(X, Y, tx, ty);
+  table2d_eval(t2d + (size_t)i * TILE2, tx, ty, V, VX);
 }
 
 // Hankel asymptotic expansions of J0, Y0, J1, Y1 for x >= XT (A&S 9.2.5-10):
@@ -148,25 +240,21 @@ __device__ __forceinline__ void taylor_sum(double X, double t0, double& P, doubl
   }
 }
 
-__device__ __forceinline__ WaveK wave_kernel(const WaveTables& T, double X, double Y) {
+// The wave kernel from the table values it needs: (j0, j1, yt, dyt) of the 1D tile of X when
+// X < XT, and (V, VX) of the 2D tile of (X, Y) when also Y < YT (the others are not read).
+__device__ __forceinline__ WaveK wave_kernel_tab(double X, double Y, double j0, double j1, double yt, double dyt,
+                                                 double V, double VX) {
   WaveK w;
   const double R0 = sqrt(X * X + Y * Y);
   const double eY = exp(-Y);
-  double j0, j1;
   if (X < (double)XT && Y < (double)YT) {
-    double yt, dyt, V, VX;
-    table1d(T.t1d, X, j0, j1, yt, dyt);
-    table2d(T.t2d, X, Y, V, VX);
     const double lg = log(Y + R0) + (GAMMA - LOG2);
     w.F = -2.0 * eY * j0 * lg - PI * eY * yt - 2.0 * R0 * V;
     w.FX = 2.0 * eY * j1 * lg - 2.0 * eY * j0 * X / (R0 * (Y + R0)) - PI * eY * dyt - 2.0 * (X / R0) * V -
            2.0 * R0 * VX;
   } else if (Y >= (double)YT) {
     // the exp(-Y) terms are below 1e-20 and dropped
-    if (X < (double)XT) {
-      double yt, dyt;
-      table1d(T.t1d, X, j0, j1, yt, dyt);
-    } else {
+    if (X >= (double)XT) {
       double y0, y1;
       hankel01(X, j0, y0, j1, y1);
     }
@@ -191,6 +279,41 @@ __device__ __forceinline__ WaveK wave_kernel(const WaveTables& T, double X, doub
   w.I = -2.0 * PI * eY * j0;
   w.IX = 2.0 * PI * eY * j1;
   return w;
+}
+
+// wave_kernel for a wavefront whose lanes take different points (`valid` false: no point, the
+// lane only joins the rounds), the table reads wavefront-uniform (HBG_UNIFORM_ROUNDS, scalar
+// loads, power basis).
+__device__ __forceinline__ WaveK wave_kernel_u(const WaveTables& T, bool valid, double X, double Y) {
+  double j0 = 0.0, j1 = 0.0, yt = 0.0, dyt = 0.0, V = 0.0, VX = 0.0;
+  {
+    double t;
+    const int i1 = tile1d(X, t);
+    const bool act = valid && X < (double)XT;
+    HBG_UNIFORM_ROUNDS(i1, act, u, mine, {
+      cdouble* c = uniform_tile(T.t1d, u, TILE1);
+      if (mine) table1d_eval_u(c, t, j0, j1, yt, dyt);
+    })
+  }
+  {
+    double tx, ty;
+    const int i2 = tile2d(X, Y, tx, ty);
+    const bool act = valid && X < (double)XT && Y < (double)YT;
+    HBG_UNIFORM_ROUNDS(i2, act, u, mine, {
+      cdouble* c = uniform_tile(T.t2d, u, TILE2);
+      if (mine) table2d_eval_u(c, tx, ty, V, VX);
+    })
+  }
+  return wave_kernel_tab(X, Y, j0, j1, yt, dyt, V, VX);
+}
+
+__device__ __forceinline__ WaveK wave_kernel(const WaveTables& T, double X, double Y) {
+  double j0 = 0.0, j1 = 0.0, yt = 0.0, dyt = 0.0, V = 0.0, VX = 0.0;
+  if (X < (double)XT) {
+    table1d(T.t1d, X, j0, j1, yt, dyt);
+    if (Y < (double)YT) table2d(T.t2d, X, Y, V, VX);
+  }
+  return wave_kernel_tab(X, Y, j0, j1, yt, dyt, V, VX);
 }
 
 }  // namespace hbg
