@@ -1118,7 +1118,7 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_tuning");
   switch (key) {
     case HBG_TUNE_LU_GEMM:
-      if (value < 1 || value > 2) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1 or 2", "hbg_set_tuning");
+      if (value < 1 || value > 4) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..4", "hbg_set_tuning");
       lu_set_gemm_variant(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH:

@@ -145,4 +145,151 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(typename Cplx<T>::type* 
       }
 }
 
+// The complex64 update on the FP16 matrix cores (v_mfma_f32_16x16x32_f16, FP32 accumulation),
+// with each FP32 operand split into two FP16 numbers, x = hi + lo, hi = fp16(x),
+// lo = fp16(x - hi), and each real product taken as hi hi' + hi lo' + lo hi' (the lo lo' term,
+// below 2^-22 of the product, dropped): three FP16 products per FP32 one, at sixteen times the
+// FP32 matrix rate. With |x| <= 1 the split keeps x to within 2^-23 |x| + 2^-25 (lo may be
+// subnormal: its absolute error stays 2^-25), so the update's error is FP32's relative to the
+// matrix's scale (normwise), not per entry. Values beyond the FP16 range (65504) are not
+// representable: the caller keeps such columns (the right-hand sides) on the FP32 kernel, and
+// the mixed-precision solve that uses this kernel checks every solution by its FP64 residual.
+//
+// Layout as gemm_mfma_kernel (C^T = B^T A^T, output lanes along rows of C); the operands pass
+// through LDS as [part][row or column][k] with part = re hi, re lo, im hi, im lo, rows padded to
+// 40 halves (80 B: the 16-byte reads of 8 consecutive lanes hit distinct banks); lane l of an
+// MFMA reads 8 consecutive k (8 (l / 16) ..) of row or column l % 16. The next chunk of k is
+// loaded into registers while the current one is multiplied.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int H16_KC = 32, H16_PAD = 40;
+
+__device__ __forceinline__ f4v mma16(h8 a, h8 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+
+// 8 consecutive complex values split into the four parts, stored at row r, k offset o
+__device__ __forceinline__ void split_store(_Float16 (*S)[H16_PAD], int rows, int r, int o, const float2* v) {
+  h8 rh, rl, ih, il;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const _Float16 a = (_Float16)v[q].x, b = (_Float16)v[q].y;
+    rh[q] = a;
+    rl[q] = (_Float16)(v[q].x - (float)a);
+    ih[q] = b;
+    il[q] = (_Float16)(v[q].y - (float)b);
+  }
+  *(h8*)&S[0 * rows + r][o] = rh;
+  *(h8*)&S[1 * rows + r][o] = rl;
+  *(h8*)&S[2 * rows + r][o] = ih;
+  *(h8*)&S[3 * rows + r][o] = il;
+}
+
+// C -= A B as gemm_mfma_kernel<float, TM, TN>; grid (ceil(nrows / TM), ceil(ncolsC / TN), batch),
+// 256 threads.
+template <int TM, int TN>
+__global__ void __launch_bounds__(256) gemm_f16x3_kernel(float2* __restrict__ M, int lda, int r0, int nrows, int c0,
+                                                         int ncolsC, int k0, int kb, GemmBatch gb) {
+  constexpr int BR = TM / 32, BC = TN / 32;   // 16 x 16 blocks per wave: rows of C, columns of C
+  constexpr int UA = TM / 64, UB = TN / 64;   // 8-element loads per thread and chunk
+  __shared__ __attribute__((aligned(16))) _Float16 As[4 * TM][H16_PAD];  // [part][row of C][k]
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * TN][H16_PAD];  // [part][column of C][k]
+  M += blockIdx.z * gb.m;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave & 1) * (TM / 2), wc = (wave >> 1) * (TN / 2);
+  const int tr = blockIdx.x * TM, tc = blockIdx.y * TN;
+  float2 ra[UA][8], rb[UB][8];
+  // chunk kc into registers: A unit u = tid + 256 s: row u % TM, k 8 (u / TM) ..; B unit: column
+  // u / 4, k 8 (u % 4) .. (four threads read 256 contiguous bytes of a column)
+  auto load = [&](int kc) {
+#pragma unroll
+    for (int s = 0; s < UA; ++s) {
+      const int u = tid + 256 * s, row = tr + u % TM, kk = kc + 8 * (u / TM);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float2 v = make_float2(0.f, 0.f);
+        if (row < nrows && kk + q < kb) v = M[(size_t)(k0 + kk + q) * lda + r0 + row];
+        ra[s][q] = v;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < UB; ++s) {
+      const int u = tid + 256 * s, col = tc + u / 4, kk = kc + 8 * (u % 4);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float2 v = make_float2(0.f, 0.f);
+        if (col < ncolsC && kk + q < kb) v = M[(size_t)(c0 + col) * lda + k0 + kk + q];
+        rb[s][q] = v;
+      }
+    }
+  };
+  f4v are[BC][BR], aim[BC][BR];
+#pragma unroll
+  for (int c = 0; c < BC; ++c)
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+      are[c][r] = f4v{0.f, 0.f, 0.f, 0.f};
+      aim[c][r] = f4v{0.f, 0.f, 0.f, 0.f};
+    }
+  load(0);
+  for (int kc = 0; kc < kb; kc += H16_KC) {
+#pragma unroll
+    for (int s = 0; s < UA; ++s) {
+      const int u = tid + 256 * s;
+      split_store(As, TM, u % TM, 8 * (u / TM), ra[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < UB; ++s) {
+      const int u = tid + 256 * s;
+      split_store(Bs, TN, u / 4, 8 * (u % 4), rb[s]);
+    }
+    __syncthreads();
+    if (kc + H16_KC < kb) load(kc + H16_KC);
+    const int ko = 8 * (lane >> 4);
+    h8 b[BC][4], a[BR][4];
+#pragma unroll
+    for (int c = 0; c < BC; ++c)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) b[c][p] = *(const h8*)&Bs[p * TN + wc + c * 16 + (lane & 15)][ko];
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) a[r][p] = *(const h8*)&As[p * TM + wr + r * 16 + (lane & 15)][ko];
+    // re += Br Ar - Bi Ai, im += Br Ai + Bi Ar; each real product hi hi' + hi lo' + lo hi'
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+      for (int c = 0; c < BC; ++c) {
+        const h8 br = b[c][t == 2 ? 1 : 0], bi = b[c][t == 5 ? 3 : 2];
+#pragma unroll
+        for (int r = 0; r < BR; ++r) {
+          if (t < 3) {
+            const h8 ar = a[r][t == 1 ? 1 : 0], ai = a[r][t == 1 ? 3 : 2];
+            are[c][r] = mma16(br, ar, are[c][r]);
+            aim[c][r] = mma16(br, ai, aim[c][r]);
+          } else {
+            const h8 ar = a[r][t == 4 ? 1 : 0], ai = a[r][t == 4 ? 3 : 2];
+            are[c][r] = mma16(-bi, ai, are[c][r]);
+            aim[c][r] = mma16(bi, ar, aim[c][r]);
+          }
+        }
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < BC; ++c)
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = tc + wc + c * 16 + 4 * (lane >> 4) + q;
+        const int row = tr + wr + r * 16 + (lane & 15);
+        if (col < ncolsC && row < nrows) {
+          float2* p = M + (size_t)(c0 + col) * lda + r0 + row;
+          float2 v = *p;
+          v.x -= are[c][r][q];
+          v.y -= aim[c][r][q];
+          *p = v;
+        }
+      }
+}
+
 }  // namespace hbg

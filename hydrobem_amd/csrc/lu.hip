@@ -723,8 +723,11 @@ template <typename T>
 constexpr int prec_index() {
   return std::is_same<T, double>::value ? 0 : 1;
 }
-void lu_set_gemm_variant(int v) { g_gemm_variant_[0] = g_gemm_variant_[1] = v; }
-int lu_gemm_variant() { return g_gemm_variant_[0]; }
+void lu_set_gemm_variant(int v) {
+  g_gemm_variant_[0] = v > 2 ? v - 2 : v;  // complex128 has the FP64 kernels only
+  g_gemm_variant_[1] = v;
+}
+int lu_gemm_variant() { return g_gemm_variant_[1]; }
 void lu_set_depth(int q) { g_lu_depth_[0] = g_lu_depth_[1] = q; }
 int lu_depth() { return g_lu_depth_[0]; }
 
@@ -801,24 +804,55 @@ struct GemmTimer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
+// One launch of the trailing-update GEMM in the given tile variant: 1 64 x 64, 2 128 x 64 (FP32 or
+// FP64 matrix cores); complex64 only: 3 64 x 64, 4 128 x 64 on the FP16 matrix cores with split
+// operands (gemm_f16x3_kernel).
+template <typename T>
+void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf,
+                 int kk, int batch, size_t mstride, hipStream_t stream) {
+  if (nrows <= 0 || ncolsC <= 0) return;
+  const GemmBatch gb{mstride};
+  if constexpr (std::is_same<T, float>::value) {
+    if (variant == 3) {
+      gemm_f16x3_kernel<64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+          M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+      return;
+    }
+    if (variant == 4) {
+      gemm_f16x3_kernel<128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+          M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+      return;
+    }
+  }
+  if (variant == 2 || variant == 4)
+    gemm_mfma_kernel<T, 128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+        M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+  else
+    gemm_mfma_kernel<T, 64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
+        M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+}
+
 // M[r0 + i][c0 + j] -= sum_k M[r0 + i][kf + k] M[kf + k][c0 + j] (i < nrows, j < ncolsC, k < kk)
-// on `batch` matrices, on the matrix cores; timed when stats are kept. Variant 1: 64 x 64 tiles,
-// 2: 128 x 64.
+// on `batch` matrices, on the matrix cores; timed when stats are kept. Columns from nmat on are
+// right-hand sides: under the split-FP16 variants they take the FP32 kernel (their values are
+// not bounded by the matrix's scale).
 template <typename T>
 void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk, int batch,
-              size_t mstride, hipStream_t stream, GemmTimer& gt) {
+              size_t mstride, int nmat, hipStream_t stream, GemmTimer& gt) {
   if (nrows <= 0 || ncolsC <= 0 || kk <= 0) return;
   hipEvent_t g0 = nullptr;
   if (gt.stats) {
     g0 = gt.ep->next();
     HBG_HIP(hipEventRecord(g0, stream));
   }
-  if (g_gemm_variant_[prec_index<T>()] == 2)
-    gemm_mfma_kernel<T, 128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
-        M, lda, r0, nrows, c0, ncolsC, kf, kk, GemmBatch{mstride});
-  else
-    gemm_mfma_kernel<T, 64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
-        M, lda, r0, nrows, c0, ncolsC, kf, kk, GemmBatch{mstride});
+  const int v = g_gemm_variant_[prec_index<T>()];
+  if (v >= 3 && c0 + ncolsC > nmat) {
+    const int nm = nmat > c0 ? nmat - c0 : 0;
+    gemm_launch<T>(v, M, lda, r0, nrows, c0, nm, kf, kk, batch, mstride, stream);
+    gemm_launch<T>(v - 2, M, lda, r0, nrows, c0 + nm, ncolsC - nm, kf, kk, batch, mstride, stream);
+  } else {
+    gemm_launch<T>(v, M, lda, r0, nrows, c0, ncolsC, kf, kk, batch, mstride, stream);
+  }
   if (gt.stats) {
     hipEvent_t g1 = gt.ep->next();
     HBG_HIP(hipEventRecord(g1, stream));
@@ -845,8 +879,8 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int kb = min(NB, n - k0);
       const int m = n - k0;
       if (k0 > K0) {
-        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, stream, gt);
-        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, stream, gt);
+        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, n, stream, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, n, stream, gt);
       }
       const size_t po = (size_t)p * NB * NB;
       diag_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
@@ -856,7 +890,7 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
         apply_kernel<T><<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, ws.uinv + po,
                                                                      ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs);
     }
-    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, stream, gt);
+    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, n, stream, gt);
   }
 }
 
@@ -896,7 +930,7 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
     if (cb > 0)
       apply_kernel<double><<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, linv, ws.udiag, nullptr,
                                                    nullptr, bs);
-    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, stream, gt);
+    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, n, stream, gt);
   }
 }
 
@@ -1052,7 +1086,8 @@ __global__ void synth_kernel(typename Cplx<T>::type* M, size_t count, int n, int
 
 }  // namespace
 
-// Time the batched LU on synthetic systems (n, nrhs) for every GEMM tile variant (1, 2), block depth
+// Time the batched LU on synthetic systems (n, nrhs) for every GEMM tile variant (1, 2; complex64
+// also 3, 4, the split-FP16 kernels), block depth
 // (2, 4, 8) and batch (candidates up to max_batch that fit in mem_bytes) and keep the fastest
 // variant and depth; returns the fastest batch. rows of report: variant, depth, batch, ms per
 // system (up to max_rows rows).
@@ -1079,7 +1114,9 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   *rows = 0;
   std::vector<int> status(batches.back());
   try {
-    for (int v : {1, 2})
+    const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4}
+                                                                      : std::vector<int>{1, 2};
+    for (int v : variants)
       for (int d : {2, 4, 8})
         for (int b : batches) {
           gv = v;

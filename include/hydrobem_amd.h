@@ -193,7 +193,10 @@ HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t bat
                                  const float* targets, float* gradient, double* loss);
 
 /* Kernel variants (no counterpart in the reference), per process: HBG_TUNE_LU_GEMM selects the
- * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64;
+ * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64, and for
+ * the complex64 LU of the mixed precision also 3: 64 x 64, 4: 128 x 64 on the FP16 matrix cores
+ * with each FP32 operand split into two FP16 numbers (three FP16 products per FP32 product; the
+ * complex128 LU takes 1 for 3 and 2 for 4); hbg_get_tuning reports the complex64 variant;
  * HBG_TUNE_LU_DEPTH the number of 64-column panels per trailing update (1..16, default 4);
  * HBG_TUNE_LU_GRAPHS 1 (default): the batched LU's launches are captured once per system shape
  * and buffer into a hipGraph and replayed (not while stage profiling is on), 0: launched one by one. */
@@ -201,7 +204,7 @@ enum { HBG_TUNE_LU_GEMM = 0, HBG_TUNE_LU_DEPTH = 1, HBG_TUNE_LU_GRAPHS = 2 };
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
  * systems of order n with nrhs right-hand sides in the given precision for every GEMM tile variant
- * (1, 2), block depth (2, 4, 8) and batch (2, 4, 8, 16 that fit in a quarter of the free memory),
+ * (1, 2; mixed precision also 3, 4), block depth (2, 4, 8) and batch (2, 4, 8, 16 that fit in a quarter of the free memory),
  * then keeps the fastest variant and depth (HBG_TUNE_LU_GEMM, HBG_TUNE_LU_DEPTH) and sets the
  * session's batch. report: up to max_rows rows of (variant, depth, batch, ms per system), rows the
  * number written. */

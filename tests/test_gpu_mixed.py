@@ -65,3 +65,40 @@ def test_mixed_matches_plain_cylinder(session):
         e = rel_per_entry(mixed[key], plain[key])
         say(f"  {key}: {e:.2e}")
         assert e < 1e-9
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+def test_mixed_split_fp16_gemm(session, variant):
+    """The complex64 LU's GEMM on the FP16 matrix cores with split operands: the refined results
+    stay those of the plain path, in at most two refinement steps per frequency and without an
+    FP64 fallback."""
+    say(f"case: mixed with GEMM variant {variant} (split FP16), cylinder 2304 panels, finite depth, "
+        "4 frequencies, 4 headings")
+    v = meshes.cylinder_panels(1.0, 1.0, 48, 24, 24, quarter=False, cosine=True)
+    w = np.sqrt(9.81 * np.array([0.05, 0.7, 4.0, 15.0]) / 3.0)
+    heads = [0.0, 45.0, 90.0, 135.0]
+    session.set_environment(1025.0, 9.81, 3.0)
+    session.set_panels(v)
+    plain = session.solve(w, heads)
+    old = session.get_tuning(session.TUNE_LU_GEMM)
+    session.set_tuning(session.TUNE_LU_GEMM, variant)
+    try:
+        for graphs in (1, 0):  # replayed graph, then launched one by one with stage timing
+            session.set_tuning(session.TUNE_LU_GRAPHS, graphs)
+            session.set_profiling(graphs == 0)
+            mixed = session.solve(w, heads, precision="mixed")
+            if graphs == 0:
+                prof = session.profile()
+                say(f"  refinement steps {prof['mixed_refinement_steps']:.0f}, backward error "
+                    f"{prof['mixed_max_backward_error']:.2e}, FP64 fallbacks {prof['mixed_fp64_fallbacks']:.0f}")
+                assert prof["mixed_fp64_fallbacks"] == 0
+                assert prof["mixed_refinement_steps"] <= 2 * len(w)
+                assert prof["mixed_max_backward_error"] <= 1e-12
+            for key in ("added_mass", "radiation_damping", "excitation_diffraction"):
+                e = rel_per_entry(mixed[key], plain[key])
+                say(f"  graphs {graphs} {key}: {e:.2e}")
+                assert e < 1e-9
+    finally:
+        session.set_profiling(False)
+        session.set_tuning(session.TUNE_LU_GRAPHS, 1)
+        session.set_tuning(session.TUNE_LU_GEMM, old)
