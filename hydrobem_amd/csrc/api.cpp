@@ -22,8 +22,10 @@ void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, 
                          const StaticD* st, double2* S, double2* A, float2* A32, int lda, hipStream_t stream);
 void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
                             const double* zeta, int n, double* out, hipStream_t stream);
+size_t fin_tables_work();
 void launch_fin_tables(const FinFreq& f, WaveTables T, const double* g16, double* phi1, double* phi2,
-                       const double* M1dev, const PanelD* P, int N, double4* pf, double2* pcs, hipStream_t stream);
+                       const double* M1dev, const PanelD* P, int N, double4* pf, double2* pcs, double* work,
+                       hipStream_t stream);
 FinFreq make_fin_freq(double h, double K);
 double dispersion_root(double H, int n);
 std::vector<double> fit_matrix(int M);
@@ -73,6 +75,10 @@ void lu_set_graphs(int on);
 int lu_graphs();
 void lu_set_gemm_variant(int v);
 int lu_gemm_variant();
+void lu_set_gemm_variant32(int v);
+int lu_gemm_variant32();
+void lu_set_depth32(int q);
+int lu_depth32();
 void lu_set_depth(int q);
 int lu_depth();
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
@@ -310,7 +316,7 @@ struct HbgSession {
   double st_tol = -1.0;
   WaveTables T;
   // finite depth: per-frequency tiles and per-panel factors
-  DevBuf<double> d_phi1, d_phi2;
+  DevBuf<double> d_phi1, d_phi2, d_fwork;
   DevBuf<double4> d_pf;
   DevBuf<double2> d_pcs;
   DevBuf<FinFreq> d_ff;
@@ -411,8 +417,9 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   s->d_phi2.ensure((size_t)FTS * FTV * FTILE);
   s->d_pf.ensure(N);
   s->d_pcs.ensure((size_t)N * FIN_ROOTS);
+  s->d_fwork.ensure(fin_tables_work());
   launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1.p, s->d_phi2.p, s->fitm, s->d_panels.p, N, s->d_pf.p,
-                    s->d_pcs.p, s->stream);
+                    s->d_pcs.p, s->d_fwork.p, s->stream);
   s->d_ff.ensure(1);
   HBG_HIP(hipMemcpyAsync(s->d_ff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, s->stream));
   const double nuh = K * s->depth;
@@ -518,6 +525,7 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_part.release();
   s->d_phi1.release();
   s->d_phi2.release();
+  s->d_fwork.release();
   s->d_pf.release();
   s->d_pcs.release();
   s->d_ff.release();
@@ -651,7 +659,9 @@ HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const 
   HBG_HIP(hipMemcpy(dR.p, R, n * sizeof(double), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(dz.p, z, n * sizeof(double), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(dzeta.p, zeta, n * sizeof(double), hipMemcpyHostToDevice));
-  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, dphi1.p, dphi2.p, s->fitm, dp.p, 2 * n, dpf.p, dpcs.p, s->stream);
+  s->d_fwork.ensure(fin_tables_work());
+  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, dphi1.p, dphi2.p, s->fitm, dp.p, 2 * n, dpf.p, dpcs.p,
+                    s->d_fwork.p, s->stream);
   FinTables F;
   F.phi1 = dphi1.p;
   F.phi2 = dphi2.p;
@@ -1087,7 +1097,9 @@ HbgStatus hbg_finite_remainder(HbgSession* s, double h, double K, int32_t n, con
   HBG_HIP(hipMemcpy(dz.p, z, n * sizeof(double), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(dzeta.p, zeta, n * sizeof(double), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(dff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice));
-  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, dphi1.p, dphi2.p, s->fitm, dp.p, 2 * n, dpf.p, dpcs.p, s->stream);
+  s->d_fwork.ensure(fin_tables_work());
+  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, dphi1.p, dphi2.p, s->fitm, dp.p, 2 * n, dpf.p, dpcs.p,
+                    s->d_fwork.p, s->stream);
   FinTables F;
   F.phi1 = dphi1.p;
   F.phi2 = dphi2.p;
@@ -1129,6 +1141,14 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       if (value != 0 && value != 1) return fail(HBG_ERR_VALUE, "graphs: 0 or 1", "hbg_set_tuning");
       lu_set_graphs(value);
       return HBG_OK;
+    case HBG_TUNE_LU_GEMM_MIXED:
+      if (value < 1 || value > 4) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..4", "hbg_set_tuning");
+      lu_set_gemm_variant32(value);
+      return HBG_OK;
+    case HBG_TUNE_LU_DEPTH_MIXED:
+      if (value < 1 || value > 16) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..16", "hbg_set_tuning");
+      lu_set_depth32(value);
+      return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_set_tuning");
   }
@@ -1145,6 +1165,12 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value) {
       return HBG_OK;
     case HBG_TUNE_LU_GRAPHS:
       *value = lu_graphs();
+      return HBG_OK;
+    case HBG_TUNE_LU_GEMM_MIXED:
+      *value = lu_gemm_variant32();
+      return HBG_OK;
+    case HBG_TUNE_LU_DEPTH_MIXED:
+      *value = lu_depth32();
       return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_get_tuning");

@@ -48,15 +48,52 @@ struct Mfma<float> {
   static __device__ __forceinline__ int mrow(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
-// Strides between the systems of a batch (blockIdx.z), in elements.
+// The systems of a batch: their number and the stride between them, in elements.
 struct GemmBatch {
   size_t m;
+  int batch;
 };
 
 constexpr int GEMM_KC = 16;
 
+// The tile of C of a workgroup. The grid is 1D: 8 C workgroups for T = tm tn batch tiles,
+// C = ceil(T / 8). The hardware hands workgroup w to XCD w % 8, so XCD x takes the contiguous
+// range [x C, x C + C) of the tile order, and the tile order goes by bands of 8 tile rows, column
+// by column within a band: the workgroups in flight on one XCD then share a few row panels of A
+// and column panels of B in that XCD's L2, instead of streaming all of A once per column of tiles.
+struct GemmTile {
+  int b, row, col;
+  bool ok;
+};
+constexpr int GEMM_BAND = 8;
+
+__device__ __forceinline__ GemmTile gemm_tile(int tm, int tn, int batch) {
+  const long T = (long)tm * tn * batch, C = (T + 7) / 8;
+  const long w = blockIdx.x;
+  const long g = (w % 8) * C + w / 8;
+  GemmTile t;
+  t.ok = g < T;
+  t.b = t.row = t.col = 0;
+  if (!t.ok) return t;
+  const long per = (long)tm * tn;
+  t.b = (int)(g / per);
+  const long r = g % per;
+  const int band = (int)(r / ((long)GEMM_BAND * tn));
+  const int rows = min(GEMM_BAND, tm - band * GEMM_BAND);
+  const long q = r - (long)band * GEMM_BAND * tn;
+  t.col = (int)(q / rows);
+  t.row = band * GEMM_BAND + (int)(q % rows);
+  return t;
+}
+
+// grid size of gemm_tile's order
+inline unsigned gemm_grid(int tm, int tn, int batch) {
+  const long T = (long)tm * tn * batch;
+  return (unsigned)(8 * ((T + 7) / 8));
+}
+
 // C[r0 + i][c0 + j] -= sum_k M[r0 + i][k0 + k] M[k0 + k][c0 + j], i < nrows, j < ncolsC, k < kb;
-// grid (ceil(nrows / TM), ceil(ncolsC / TN), batch), 256 threads.
+// grid gemm_grid(ceil(nrows / TM), ceil(ncolsC / TN), batch), 256 threads.
 template <typename T, int TM, int TN>
 __global__ void __launch_bounds__(256) gemm_mfma_kernel(typename Cplx<T>::type* __restrict__ M, int lda, int r0,
                                                         int nrows, int c0, int ncolsC, int k0, int kb, GemmBatch gb) {
@@ -66,10 +103,12 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(typename Cplx<T>::type* 
   constexpr int PA = TM + 2, PB = TN + 2;     // padded LDS rows
   __shared__ C2 As[GEMM_KC][PA];             // [k][row of C]
   __shared__ C2 Bs[GEMM_KC][PB];             // [k][column of C]
-  M += blockIdx.z * gb.m;
+  const GemmTile tile = gemm_tile((nrows + TM - 1) / TM, (ncolsC + TN - 1) / TN, gb.batch);
+  if (!tile.ok) return;
+  M += tile.b * gb.m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave & 1) * (TM / 2), wc = (wave >> 1) * (TN / 2);
-  const int tr = blockIdx.x * TM, tc = blockIdx.y * TN;
+  const int tr = tile.row * TM, tc = tile.col * TN;
   v4 are[BC][BR], aim[BC][BR];
 #pragma unroll
   for (int c = 0; c < BC; ++c)
@@ -183,8 +222,8 @@ __device__ __forceinline__ void split_store(_Float16 (*S)[H16_PAD], int rows, in
   *(h8*)&S[3 * rows + r][o] = il;
 }
 
-// C -= A B as gemm_mfma_kernel<float, TM, TN>; grid (ceil(nrows / TM), ceil(ncolsC / TN), batch),
-// 256 threads.
+// C -= A B as gemm_mfma_kernel<float, TM, TN>; grid gemm_grid(ceil(nrows / TM), ceil(ncolsC / TN),
+// batch), 256 threads.
 template <int TM, int TN>
 __global__ void __launch_bounds__(256) gemm_f16x3_kernel(float2* __restrict__ M, int lda, int r0, int nrows, int c0,
                                                          int ncolsC, int k0, int kb, GemmBatch gb) {
@@ -192,10 +231,12 @@ __global__ void __launch_bounds__(256) gemm_f16x3_kernel(float2* __restrict__ M,
   constexpr int UA = TM / 64, UB = TN / 64;   // 8-element loads per thread and chunk
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * TM][H16_PAD];  // [part][row of C][k]
   __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * TN][H16_PAD];  // [part][column of C][k]
-  M += blockIdx.z * gb.m;
+  const GemmTile tile = gemm_tile((nrows + TM - 1) / TM, (ncolsC + TN - 1) / TN, gb.batch);
+  if (!tile.ok) return;
+  M += tile.b * gb.m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave & 1) * (TM / 2), wc = (wave >> 1) * (TN / 2);
-  const int tr = blockIdx.x * TM, tc = blockIdx.y * TN;
+  const int tr = tile.row * TM, tc = tile.col * TN;
   float2 ra[UA][8], rb[UB][8];
   // chunk kc into registers: A unit u = tid + 256 s: row u % TM, k 8 (u / TM) ..; B unit: column
   // u / 4, k 8 (u % 4) .. (four threads read 256 contiguous bytes of a column)

@@ -1,13 +1,13 @@
 // Per-frequency data of the finite-depth kernel, built on the GPU (green_fin.rs:874-952
 // make_finite_depth / build_tiles / fin_point, restated without the reference's Q tables):
 //   - the dispersion roots and series coefficients (host, dispersion.rs:19-95);
-//   - the Phi1 and Phi2 tiles: at the Chebyshev nodes of every tile one thread evaluates Q by
-//     Gauss-Legendre quadrature of its principal-value integral along the path
+//   - the Phi1 and Phi2 tiles: at the Chebyshev nodes of every tile Q by Gauss-Legendre
+//     quadrature of its principal-value integral along the path
 //     0 -> i d -> T + i d -> T -> T + 24 (above the real poles k = x0 and k = H, below the first
 //     imaginary pole i x1, x1 > pi/2; the real part of the path integral is the principal value
-//     since the integrand is real on the real axis), adds the Ft terms from the infinite-depth
-//     tables, and one workgroup per tile fits the value and both derivatives (monomial
-//     coefficients from the node values by one matrix, M F M^T);
+//     since the integrand is real on the real axis), factored over the nodes (fq_* kernels), plus
+//     the Ft terms from the infinite-depth tables; one workgroup per tile fits the value and both
+//     derivatives (monomial coefficients from the node values by one matrix, M F M^T);
 //   - per panel the propagating-mode factors and cos, sin k_n (z + h).
 // The scaled K0, K1 tiles are frequency-independent (host, once per process).
 #include <cmath>
@@ -61,89 +61,197 @@ __device__ __forceinline__ void cj01x(cx z, cx& j0, cx& j1) {
   j1 = (0.5 * z) * s1;
 }
 
-// Q, dQ/dxb, dQ/dvb by the path integral (see the file header)
-__device__ void q_path(double H, double x0, double xb, double vb, const double* __restrict__ g16, double* q) {
-  const double* gx = g16;
-  const double* gw = g16 + 16;
-  cx s0{0, 0}, s1{0, 0}, s2{0, 0};
-  auto seg = [&](cx a, cx b, int panels) {
-    const cx d = (1.0 / panels) * (b - a);
-    for (int p = 0; p < panels; ++p) {
-      const cx pa = a + (double)p * d;
-      const cx hh = 0.5 * d;
-      for (int i = 0; i < 16; ++i) {
-        const cx k = pa + (gx[i] + 1.0) * hh;
-        const cx dk = gw[i] * hh;
-        cx sh, ch, shv, chv, j0, j1;
-        csinhcosh(k, sh, ch);
-        csinhcosh(vb * k, shv, chv);
-        cj01x(xb * k, j0, j1);
-        const cx kp = k + cx{H, 0.0}, km = k - cx{H, 0.0};
-        const cx num = (kp * kp) * cexpx(-3.0 * k);
-        const cx den = (k * sh - H * ch) * km;
-        const cx wgt = cdivx(num, den) * dk;
-        s0 = s0 + wgt * chv * j0;
-        s1 = s1 - wgt * chv * k * j1;
-        s2 = s2 + wgt * k * shv * j0;
-      }
-    }
-  };
-  if (fmin(x0, H) > 28.0) {
-    seg(cx{0, 0}, cx{28.0, 0}, 56);
-  } else {
-    const double d = 0.7, T = fmax(x0, H) + 3.0;
-    double y = fmin(0.05, 1e-3 * sqrt(H));
-    seg(cx{0, 0}, cx{0, y}, 1);
-    while (2.0 * y < d) {
-      seg(cx{0, y}, cx{0, 2.0 * y}, 1);
-      y *= 2.0;
-    }
-    seg(cx{0, y}, cx{0, d}, 1);
-    seg(cx{0, d}, cx{T, d}, (int)ceil(T) + 1);
-    seg(cx{T, d}, cx{T, 0}, 1);
-    seg(cx{T, 0}, cx{T + 24.0, 0}, 12);
-  }
-  q[0] = s0.r;
-  q[1] = s1.r;
-  q[2] = s2.r;
+// Q at the tile nodes, factored. The path and its points do not depend on the node, and along
+// the path the integrand of Q (and of its two derivatives) is w(k) times a factor of xb (J0(xb k),
+// J1(xb k)) times a factor of vb (cosh(vb k), sinh(vb k)), w(k) = (k + H)^2 e^{-3k} dk /
+// [(k sinh k - H cosh k)(k - H)]. The nodes of a family of tiles are all pairs of its xb values
+// (FTS tiles x FM nodes) and vb values (FTW or FTV tiles x FM nodes), so per frequency the factors
+// are evaluated once per (point, xb) and (point, vb) and each node is three dot products along
+// the path:
+//   Q = Re sum_i [w chv]_i J0_i,  Q_xb = -Re sum_i [w chv k]_i J1_i,  Q_vb = Re sum_i [w k shv]_i J0_i,
+// summed in the order of the points (segment by segment, Gauss point by point).
+constexpr int FQ_NPAN = 64, FQ_NP = FQ_NPAN * 16;  // panels of 16 Gauss points along the path, at most
+constexpr int FQ_NX = FTS * FM;                     // xb values
+constexpr int FQ_NV1 = FTW * FM, FQ_NV = FQ_NV1 + FTV * FM;  // vb values of Phi1, then of Phi2
+constexpr int FQ_VB = 4, FQ_CHUNK = 256;            // vb rows per workgroup of the sums, points per pass
+
+// Work layout (doubles): point count, points k_i and weights w_i, factors J [FQ_NP][FQ_NX][J0, J1],
+// factors V [FQ_NV][FQ_NP][w chv, w chv k, w k shv]
+constexpr size_t FQ_OFF_PTS = 2, FQ_OFF_J = FQ_OFF_PTS + 4 * (size_t)FQ_NP,
+                 FQ_OFF_V = FQ_OFF_J + 4 * (size_t)FQ_NP * FQ_NX, FQ_WORK = FQ_OFF_V + 6 * (size_t)FQ_NV * FQ_NP;
+
+__device__ __forceinline__ double xb_value(int x) {
+  const int ix = x / FM, jx = x % FM;
+  return (0.5 / FTS) * (ix + 0.5 * (cos(PI * (jx + 0.5) / FM) + 1.0));
+}
+__device__ __forceinline__ double vb_value(int v) {
+  const bool one = v < FQ_NV1;
+  const int l = one ? v : v - FQ_NV1, iy = l / FM, jy = l % FM;
+  const double ymax = one ? 1.0 : 2.0;
+  const int ny = one ? FTW : FTV;
+  return (ymax / ny) * (iy + 0.5 * (cos(PI * (jy + 0.5) / FM) + 1.0));
 }
 
-// node values of the Phi tiles: node index -> (tile, jy, jx); out [tile][q][jy][jx]
-__global__ void __launch_bounds__(64) fin_nodes_kernel(FinFreq f, WaveTables T, const double* __restrict__ g16,
-                                                       double* __restrict__ out1, double* __restrict__ out2) {
-  const int n1 = FTS * FTW * FM * FM, n2 = FTS * FTV * FM * FM;
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n1 + n2) return;
-  const int which = id < n1 ? 1 : 2;
-  const int local = which == 1 ? id : id - n1;
-  const int ny = which == 1 ? FTW : FTV;
-  const double ymax = which == 1 ? 1.0 : 2.0;
-  const int tile = local / (FM * FM), node = local % (FM * FM);
-  const int jy = node / FM, jx = node % FM;
-  const int ix = tile / ny, iy = tile % ny;
-  const double tx = cos(PI * (jx + 0.5) / FM), ty = cos(PI * (jy + 0.5) / FM);
-  const double xb = (0.5 / FTS) * (ix + 0.5 * (tx + 1.0));
-  const double vb = (ymax / ny) * (iy + 0.5 * (ty + 1.0));
-  double q[3];
-  q_path(f.H, f.x0, xb, vb, g16, q);
-  const double H = f.H, X = H * xb, H2 = H * H;
-  if (which == 1) {
-    const double Ya = H * (2.0 - vb), Yb = H * (2.0 + vb);
-    const WaveK a = wave_kernel(T, X, Ya), b = wave_kernel(T, X, Yb);
-    const double FYa = -a.F - 2.0 / sqrt(X * X + Ya * Ya), FYb = -b.F - 2.0 / sqrt(X * X + Yb * Yb);
-    q[0] += H * (a.F + b.F);
-    q[1] += H2 * (a.FX + b.FX);
-    q[2] += H2 * (-FYa + FYb);
-  } else {
-    const double Yb = H * (2.0 + vb);
-    const WaveK b = wave_kernel(T, X, Yb);
-    const double FYb = -b.F - 2.0 / sqrt(X * X + Yb * Yb);
-    q[0] += H * b.F;
-    q[1] += H2 * b.FX;
-    q[2] += H2 * FYb;
+// The path 0 -> i d -> T + i d -> T -> T + 24 (or the real axis to 28 when both poles lie beyond
+// 28) in panels, each 16 Gauss points: one workgroup of FQ_NP threads, thread 16 p + g the point g
+// of panel p; points past the last panel get weight zero.
+__global__ void __launch_bounds__(FQ_NP) fq_points_kernel(double H, double x0, const double* __restrict__ g16,
+                                                          double* __restrict__ work) {
+  __shared__ cx pa[FQ_NPAN], pd[FQ_NPAN];
+  __shared__ int npan;
+  if (threadIdx.x == 0) {
+    int np = 0;
+    auto seg = [&](cx a, cx b, int panels) {
+      const cx d = (1.0 / panels) * (b - a);
+      for (int p = 0; p < panels && np < FQ_NPAN; ++p) {
+        pa[np] = a + (double)p * d;
+        pd[np] = d;
+        ++np;
+      }
+    };
+    if (fmin(x0, H) > 28.0) {
+      seg(cx{0, 0}, cx{28.0, 0}, 56);
+    } else {
+      const double d = 0.7, T = fmax(x0, H) + 3.0;
+      double y = fmin(0.05, 1e-3 * sqrt(H));
+      seg(cx{0, 0}, cx{0, y}, 1);
+      while (2.0 * y < d && np < FQ_NPAN) {
+        seg(cx{0, y}, cx{0, 2.0 * y}, 1);
+        y *= 2.0;
+      }
+      seg(cx{0, y}, cx{0, d}, 1);
+      seg(cx{0, d}, cx{T, d}, (int)ceil(T) + 1);
+      seg(cx{T, d}, cx{T, 0}, 1);
+      seg(cx{T, 0}, cx{T + 24.0, 0}, 12);
+    }
+    npan = np;
+    work[0] = 16.0 * np;
   }
-  double* o = (which == 1 ? out1 : out2) + (size_t)tile * FTILE;
-  for (int k = 0; k < 3; ++k) o[k * FM * FM + node] = q[k];
+  __syncthreads();
+  const int p = threadIdx.x / 16, i = threadIdx.x % 16;
+  cx k{1.0, 0.0}, w{0.0, 0.0};
+  if (p < npan) {
+    const cx hh = 0.5 * pd[p];
+    k = pa[p] + (g16[i] + 1.0) * hh;
+    const cx dk = g16[16 + i] * hh;
+    cx sh, ch;
+    csinhcosh(k, sh, ch);
+    const cx kp = k + cx{H, 0.0}, km = k - cx{H, 0.0};
+    const cx num = (kp * kp) * cexpx(-3.0 * k);
+    const cx den = (k * sh - H * ch) * km;
+    w = cdivx(num, den) * dk;
+  }
+  double* o = work + FQ_OFF_PTS + 4 * (size_t)threadIdx.x;
+  o[0] = k.r;
+  o[1] = k.i;
+  o[2] = w.r;
+  o[3] = w.i;
+}
+
+// J0(xb k_i), J1(xb k_i) for every point and xb value: thread = (point, xb)
+__global__ void fq_jfactor_kernel(double* __restrict__ work) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = (int)work[0];
+  if (e >= np * FQ_NX) return;
+  const int i = e / FQ_NX, x = e % FQ_NX;
+  const double* pt = work + FQ_OFF_PTS + 4 * (size_t)i;
+  const cx k{pt[0], pt[1]};
+  cx j0, j1;
+  cj01x(xb_value(x) * k, j0, j1);
+  double* o = work + FQ_OFF_J + 4 * (size_t)e;
+  o[0] = j0.r;
+  o[1] = j0.i;
+  o[2] = j1.r;
+  o[3] = j1.i;
+}
+
+// w chv, w chv k, w k shv for every vb value and point: thread = (vb, point)
+__global__ void fq_vfactor_kernel(double* __restrict__ work) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = (int)work[0];
+  if (e >= FQ_NV * FQ_NP) return;
+  const int v = e / FQ_NP, i = e % FQ_NP;
+  if (i >= np) return;
+  const double* pt = work + FQ_OFF_PTS + 4 * (size_t)i;
+  const cx k{pt[0], pt[1]}, w{pt[2], pt[3]};
+  cx shv, chv;
+  csinhcosh(vb_value(v) * k, shv, chv);
+  const cx v0 = w * chv, v0k = v0 * k, v1 = (w * k) * shv;
+  double* o = work + FQ_OFF_V + 6 * (size_t)e;
+  o[0] = v0.r;
+  o[1] = v0.i;
+  o[2] = v0k.r;
+  o[3] = v0k.i;
+  o[4] = v1.r;
+  o[5] = v1.i;
+}
+
+// The node values: Q by the dot products, plus the Ft terms of the infinite-depth tables;
+// out [tile][q][jy][jx]. Workgroup: FQ_VB vb rows (their factors staged in LDS a chunk of points
+// at a time), thread x the xb value.
+__global__ void __launch_bounds__(128) fq_nodes_kernel(FinFreq f, WaveTables T, const double* __restrict__ work,
+                                                       double* __restrict__ out1, double* __restrict__ out2) {
+  __shared__ cx Vs[FQ_VB][FQ_CHUNK][3];
+  const int v0 = blockIdx.x * FQ_VB, x = threadIdx.x;
+  const int np = (int)work[0];
+  cx s[FQ_VB][3];
+#pragma unroll
+  for (int r = 0; r < FQ_VB; ++r) s[r][0] = s[r][1] = s[r][2] = cx{0.0, 0.0};
+  for (int c0 = 0; c0 < np; c0 += FQ_CHUNK) {
+    const int nc = min(FQ_CHUNK, np - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < FQ_VB * nc; e += blockDim.x) {
+      const int r = e / nc, i = e % nc;
+      if (v0 + r < FQ_NV) {
+        const double* src = work + FQ_OFF_V + 6 * ((size_t)(v0 + r) * FQ_NP + c0 + i);
+        Vs[r][i][0] = cx{src[0], src[1]};
+        Vs[r][i][1] = cx{src[2], src[3]};
+        Vs[r][i][2] = cx{src[4], src[5]};
+      }
+    }
+    __syncthreads();
+    if (x < FQ_NX)
+      for (int i = 0; i < nc; ++i) {
+        const double* jp = work + FQ_OFF_J + 4 * ((size_t)(c0 + i) * FQ_NX + x);
+        const cx j0{jp[0], jp[1]}, j1{jp[2], jp[3]};
+#pragma unroll
+        for (int r = 0; r < FQ_VB; ++r) {
+          s[r][0] = s[r][0] + Vs[r][i][0] * j0;
+          s[r][1] = s[r][1] - Vs[r][i][1] * j1;
+          s[r][2] = s[r][2] + Vs[r][i][2] * j0;
+        }
+      }
+  }
+  if (x >= FQ_NX) return;
+  const double H = f.H, H2 = H * H, xb = xb_value(x), X = H * xb;
+  const int ix = x / FM, jx = x % FM;
+#pragma unroll
+  for (int r = 0; r < FQ_VB; ++r) {
+    const int v = v0 + r;
+    if (v >= FQ_NV) break;
+    const bool one = v < FQ_NV1;
+    const int l = one ? v : v - FQ_NV1, iy = l / FM, jy = l % FM, ny = one ? FTW : FTV;
+    const double vb = vb_value(v);
+    double q[3] = {s[r][0].r, s[r][1].r, s[r][2].r};
+    if (one) {
+      const double Ya = H * (2.0 - vb), Yb = H * (2.0 + vb);
+      const WaveK a = wave_kernel(T, X, Ya), b = wave_kernel(T, X, Yb);
+      const double FYa = -a.F - 2.0 / sqrt(X * X + Ya * Ya), FYb = -b.F - 2.0 / sqrt(X * X + Yb * Yb);
+      q[0] += H * (a.F + b.F);
+      q[1] += H2 * (a.FX + b.FX);
+      q[2] += H2 * (-FYa + FYb);
+    } else {
+      const double Yb = H * (2.0 + vb);
+      const WaveK b = wave_kernel(T, X, Yb);
+      const double FYb = -b.F - 2.0 / sqrt(X * X + Yb * Yb);
+      q[0] += H * b.F;
+      q[1] += H2 * b.FX;
+      q[2] += H2 * FYb;
+    }
+    double* o = (one ? out1 : out2) + (size_t)(ix * ny + iy) * FTILE;
+    const int node = jy * FM + jx;
+    for (int k = 0; k < 3; ++k) o[k * FM * FM + node] = q[k];
+  }
 }
 
 // monomial coefficients of every tile and value: C = M1 F M1^T, in place over the node values
@@ -303,10 +411,16 @@ std::vector<double> build_k_table() {
   return tab;
 }
 
+size_t fin_tables_work() { return FQ_WORK; }
+
+// work: fin_tables_work() doubles of device scratch
 void launch_fin_tables(const FinFreq& f, WaveTables T, const double* g16, double* phi1, double* phi2,
-                       const double* M1dev, const PanelD* P, int N, double4* pf, double2* pcs, hipStream_t stream) {
-  const int n = (FTS * FTW + FTS * FTV) * FM * FM;
-  fin_nodes_kernel<<<(n + 63) / 64, 64, 0, stream>>>(f, T, g16, phi1, phi2);
+                       const double* M1dev, const PanelD* P, int N, double4* pf, double2* pcs, double* work,
+                       hipStream_t stream) {
+  fq_points_kernel<<<1, FQ_NP, 0, stream>>>(f.H, f.x0, g16, work);
+  fq_jfactor_kernel<<<(FQ_NP * FQ_NX + 255) / 256, 256, 0, stream>>>(work);
+  fq_vfactor_kernel<<<(FQ_NV * FQ_NP + 255) / 256, 256, 0, stream>>>(work);
+  fq_nodes_kernel<<<(FQ_NV + FQ_VB - 1) / FQ_VB, 128, 0, stream>>>(f, T, work, phi1, phi2);
   HBG_HIP(hipGetLastError());
   fin_fit_kernel<<<FTS * FTW * 3, 128, 0, stream>>>(phi1, FTS * FTW, M1dev);
   fin_fit_kernel<<<FTS * FTV * 3, 128, 0, stream>>>(phi2, FTS * FTV, M1dev);

@@ -727,7 +727,11 @@ void lu_set_gemm_variant(int v) {
   g_gemm_variant_[0] = v > 2 ? v - 2 : v;  // complex128 has the FP64 kernels only
   g_gemm_variant_[1] = v;
 }
-int lu_gemm_variant() { return g_gemm_variant_[1]; }
+int lu_gemm_variant() { return g_gemm_variant_[0]; }
+void lu_set_gemm_variant32(int v) { g_gemm_variant_[1] = v; }
+int lu_gemm_variant32() { return g_gemm_variant_[1]; }
+void lu_set_depth32(int q) { g_lu_depth_[1] = q; }
+int lu_depth32() { return g_lu_depth_[1]; }
 void lu_set_depth(int q) { g_lu_depth_[0] = g_lu_depth_[1] = q; }
 int lu_depth() { return g_lu_depth_[0]; }
 
@@ -811,25 +815,23 @@ template <typename T>
 void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf,
                  int kk, int batch, size_t mstride, hipStream_t stream) {
   if (nrows <= 0 || ncolsC <= 0) return;
-  const GemmBatch gb{mstride};
+  const GemmBatch gb{mstride, batch};
+  const int tn = (ncolsC + 63) / 64;
+  const unsigned g64 = gemm_grid((nrows + 63) / 64, tn, batch), g128 = gemm_grid((nrows + 127) / 128, tn, batch);
   if constexpr (std::is_same<T, float>::value) {
     if (variant == 3) {
-      gemm_f16x3_kernel<64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
-          M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+      gemm_f16x3_kernel<64, 64><<<g64, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
       return;
     }
     if (variant == 4) {
-      gemm_f16x3_kernel<128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
-          M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+      gemm_f16x3_kernel<128, 64><<<g128, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
       return;
     }
   }
   if (variant == 2 || variant == 4)
-    gemm_mfma_kernel<T, 128, 64><<<dim3((nrows + 127) / 128, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
-        M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+    gemm_mfma_kernel<T, 128, 64><<<g128, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
   else
-    gemm_mfma_kernel<T, 64, 64><<<dim3((nrows + 63) / 64, (ncolsC + 63) / 64, batch), 256, 0, stream>>>(
-        M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+    gemm_mfma_kernel<T, 64, 64><<<g64, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
 }
 
 // M[r0 + i][c0 + j] -= sum_k M[r0 + i][kf + k] M[kf + k][c0 + j] (i < nrows, j < ncolsC, k < kk)

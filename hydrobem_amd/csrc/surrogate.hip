@@ -33,8 +33,10 @@
 
 namespace hbg {
 
+size_t fin_tables_work();
 void launch_fin_tables(const FinFreq& f, WaveTables T, const double* g16, double* phi1, double* phi2,
-                       const double* M1dev, const PanelD* P, int N, double4* pf, double2* pcs, hipStream_t stream);
+                       const double* M1dev, const PanelD* P, int N, double4* pf, double2* pcs, double* work,
+                       hipStream_t stream);
 FinFreq make_fin_freq(double h, double K);
 
 namespace {
@@ -699,7 +701,7 @@ double nn_gradient(const float* P_host, const float* X_host, const float* Y_host
 void nn_make_data(unsigned long long seed, unsigned long long first, int n, int group, const WaveTables& T,
                   const double* ktab, const double* fitm, const double* g16, float4* X, float4* Y,
                   hipStream_t st) {
-  Buf<double> R, z, zeta, phi1, phi2;
+  Buf<double> R, z, zeta, phi1, phi2, fwork;
   Buf<PanelD> pts;
   Buf<double4> pf;
   Buf<double2> pcs;
@@ -712,6 +714,7 @@ void nn_make_data(unsigned long long seed, unsigned long long first, int n, int 
   pcs.ensure(2 * (size_t)group * FIN_ROOTS);
   phi1.ensure((size_t)FTS * FTW * FTILE);
   phi2.ensure((size_t)FTS * FTV * FTILE);
+  fwork.ensure(fin_tables_work());
   ff.ensure(1);
   HBG_HIP(hipMemsetAsync(pts.p, 0, 2 * (size_t)group * sizeof(PanelD), st));
   unsigned long long gs = seed * 0x2545F4914F6CDD1Dull + 0x1234567ull;
@@ -722,7 +725,7 @@ void nn_make_data(unsigned long long seed, unsigned long long first, int n, int 
     const double nuh = std::exp(NN_LNNU_LO + (NN_LNNU_HI - NN_LNNU_LO) * u);
     const FinFreq f = make_fin_freq(1.0, nuh);
     sample_kernel<<<blocks(m), 256, 0, st>>>(seed, first + g0, m, R.p, z.p, zeta.p, pts.p);
-    launch_fin_tables(f, T, g16, phi1.p, phi2.p, fitm, pts.p, 2 * m, pf.p, pcs.p, st);
+    launch_fin_tables(f, T, g16, phi1.p, phi2.p, fitm, pts.p, 2 * m, pf.p, pcs.p, fwork.p, st);
     HBG_HIP(hipMemcpyAsync(ff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, st));
     FinTables F;
     F.phi1 = phi1.p;
