@@ -194,72 +194,100 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(typename Cplx<T>::type* 
 // representable: the caller keeps such columns (the right-hand sides) on the FP32 kernel, and
 // the mixed-precision solve that uses this kernel checks every solution by its FP64 residual.
 //
-// Layout as gemm_mfma_kernel (C^T = B^T A^T, output lanes along rows of C); the operands pass
-// through LDS as [part][row or column][k] with part = re hi, re lo, im hi, im lo, rows padded to
-// 40 halves (80 B: the 16-byte reads of 8 consecutive lanes hit distinct banks); lane l of an
-// MFMA reads 8 consecutive k (8 (l / 16) ..) of row or column l % 16. The next chunk of k is
-// loaded into registers while the current one is multiplied.
+// The operands are split once, when their panel is final (split_panel_kernel), into buffers laid
+// out for the GEMM: L rows as [k group of 8][part][row][8 halves], U columns as
+// [k group][part][column][8 halves], part = re hi, re lo, im hi, im lo, with `pad` rows or
+// columns per (k group, part) and the k groups counted from the first panel of the current block.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr int H16_KC = 32, H16_PAD = 40;
 
+struct SplitOps {
+  h8* ls;           // split L of the block's panels, per system lstride h8
+  h8* us;           // split U
+  int pad;          // rows (columns) per k group and part; those past the matrix stay zero
+  size_t lstride, ustride;
+  int K0;           // first column of the block
+};
+
 __device__ __forceinline__ f4v mma16(h8 a, h8 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 
-// 8 consecutive complex values split into the four parts, stored at row r, k offset o
-__device__ __forceinline__ void split_store(_Float16 (*S)[H16_PAD], int rows, int r, int o, const float2* v) {
-  h8 rh, rl, ih, il;
+// 8 consecutive complex values into the four parts
+__device__ __forceinline__ void split8(const float2* v, h8* part) {
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const _Float16 a = (_Float16)v[q].x, b = (_Float16)v[q].y;
-    rh[q] = a;
-    rl[q] = (_Float16)(v[q].x - (float)a);
-    ih[q] = b;
-    il[q] = (_Float16)(v[q].y - (float)b);
+    part[0][q] = a;
+    part[1][q] = (_Float16)(v[q].x - (float)a);
+    part[2][q] = b;
+    part[3][q] = (_Float16)(v[q].y - (float)b);
   }
-  *(h8*)&S[0 * rows + r][o] = rh;
-  *(h8*)&S[1 * rows + r][o] = rl;
-  *(h8*)&S[2 * rows + r][o] = ih;
-  *(h8*)&S[3 * rows + r][o] = il;
 }
 
-// C -= A B as gemm_mfma_kernel<float, TM, TN>; grid gemm_grid(ceil(nrows / TM), ceil(ncolsC / TN),
-// batch), 256 threads.
+// The final L21 (rows k0 + 64 .. n of the panel's 64 columns) and U12 (the panel's 64 rows,
+// columns k0 + 64 .. n) of the panel at k0 into the split buffers; thread = (row or column, k
+// group), consecutive threads along rows (columns); grid (ceil(16 (n - k0 - 64) / 256), 1, batch).
+__global__ void __launch_bounds__(256) split_panel_kernel(const float2* __restrict__ M, int lda, int n, int k0,
+                                                          SplitOps so, size_t mstride) {
+  const int m = n - k0 - 64;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (m <= 0 || e >= 16 * (size_t)m) return;
+  M += blockIdx.z * mstride;
+  const bool isl = e < 8 * (size_t)m;
+  const int idx = (int)(isl ? e : e - 8 * (size_t)m);
+  const int r = k0 + 64 + idx % m, kg = idx / m;
+  float2 v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    v[q] = isl ? M[(size_t)(k0 + 8 * kg + q) * lda + r] : M[(size_t)r * lda + k0 + 8 * kg + q];
+  h8 part[4];
+  split8(v, part);
+  h8* dst = (isl ? so.ls + blockIdx.z * so.lstride : so.us + blockIdx.z * so.ustride) +
+            (size_t)((k0 - so.K0) / 8 + kg) * 4 * so.pad + r;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) dst[(size_t)p * so.pad] = part[p];
+}
+
+// C[r0 + i][c0 + j] -= sum_k L[r0 + i][k] U[k][c0 + j] over the k groups kg0 .. kg0 + kb / 8 of
+// the split buffers (kb a multiple of 32; rows and columns past the matrix read the zeros of the
+// padding). Layout as gemm_mfma_kernel (C^T = B^T A^T, output lanes along rows of C); the
+// operands pass through LDS as [part][row or column][k], rows padded to 40 halves (80 B: the
+// 16-byte reads of 8 consecutive lanes hit distinct banks); lane l of an MFMA reads 8
+// consecutive k (8 (l / 16) ..) of row or column l % 16; the next chunk of k is loaded into
+// registers while the current one is multiplied. Grid gemm_grid(ceil(nrows / TM),
+// ceil(ncolsC / TN), batch), 256 threads.
 template <int TM, int TN>
-__global__ void __launch_bounds__(256) gemm_f16x3_kernel(float2* __restrict__ M, int lda, int r0, int nrows, int c0,
-                                                         int ncolsC, int k0, int kb, GemmBatch gb) {
+__global__ void __launch_bounds__(256) gemm_f16s_kernel(float2* __restrict__ M, int lda, int r0, int nrows, int c0,
+                                                        int ncolsC, SplitOps so, int kg0, int kb, GemmBatch gb) {
   constexpr int BR = TM / 32, BC = TN / 32;   // 16 x 16 blocks per wave: rows of C, columns of C
-  constexpr int UA = TM / 64, UB = TN / 64;   // 8-element loads per thread and chunk
+  constexpr int UA = TM / 64, UB = TN / 64;   // (row or column, k group) units per thread and chunk
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * TM][H16_PAD];  // [part][row of C][k]
   __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * TN][H16_PAD];  // [part][column of C][k]
   const GemmTile tile = gemm_tile((nrows + TM - 1) / TM, (ncolsC + TN - 1) / TN, gb.batch);
   if (!tile.ok) return;
   M += tile.b * gb.m;
+  const h8* __restrict__ Ls = so.ls + tile.b * so.lstride;
+  const h8* __restrict__ Us = so.us + tile.b * so.ustride;
+  const size_t pad = so.pad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave & 1) * (TM / 2), wc = (wave >> 1) * (TN / 2);
   const int tr = tile.row * TM, tc = tile.col * TN;
-  float2 ra[UA][8], rb[UB][8];
-  // chunk kc into registers: A unit u = tid + 256 s: row u % TM, k 8 (u / TM) ..; B unit: column
-  // u / 4, k 8 (u % 4) .. (four threads read 256 contiguous bytes of a column)
+  h8 ra[UA][4], rb[UB][4];
+  // chunk kc (4 k groups) into registers: unit u = tid + 256 s: row (column) u % T, k group u / T
   auto load = [&](int kc) {
 #pragma unroll
     for (int s = 0; s < UA; ++s) {
-      const int u = tid + 256 * s, row = tr + u % TM, kk = kc + 8 * (u / TM);
+      const int u = tid + 256 * s;
+      const h8* src = Ls + (size_t)(kg0 + kc / 8 + u / TM) * 4 * pad + r0 + tr + u % TM;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        float2 v = make_float2(0.f, 0.f);
-        if (row < nrows && kk + q < kb) v = M[(size_t)(k0 + kk + q) * lda + r0 + row];
-        ra[s][q] = v;
-      }
+      for (int p = 0; p < 4; ++p) ra[s][p] = src[p * pad];
     }
 #pragma unroll
     for (int s = 0; s < UB; ++s) {
-      const int u = tid + 256 * s, col = tc + u / 4, kk = kc + 8 * (u % 4);
+      const int u = tid + 256 * s;
+      const h8* src = Us + (size_t)(kg0 + kc / 8 + u / TN) * 4 * pad + c0 + tc + u % TN;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        float2 v = make_float2(0.f, 0.f);
-        if (col < ncolsC && kk + q < kb) v = M[(size_t)(c0 + col) * lda + k0 + kk + q];
-        rb[s][q] = v;
-      }
+      for (int p = 0; p < 4; ++p) rb[s][p] = src[p * pad];
     }
   };
   f4v are[BC][BR], aim[BC][BR];
@@ -275,12 +303,14 @@ __global__ void __launch_bounds__(256) gemm_f16x3_kernel(float2* __restrict__ M,
 #pragma unroll
     for (int s = 0; s < UA; ++s) {
       const int u = tid + 256 * s;
-      split_store(As, TM, u % TM, 8 * (u / TM), ra[s]);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) *(h8*)&As[p * TM + u % TM][8 * (u / TM)] = ra[s][p];
     }
 #pragma unroll
     for (int s = 0; s < UB; ++s) {
       const int u = tid + 256 * s;
-      split_store(Bs, TN, u / 4, 8 * (u % 4), rb[s]);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) *(h8*)&Bs[p * TN + u % TN][8 * (u / TN)] = rb[s][p];
     }
     __syncthreads();
     if (kc + H16_KC < kb) load(kc + H16_KC);

@@ -657,8 +657,32 @@ struct LuWorkspace {
   C2* save = nullptr;        // the panel before the attempt without interchanges (pivoted path)
   C2* backup = nullptr;      // [A | B] of a single solve, restored for the pivoted path
   size_t backup_elems = 0;
+  // the split FP16 operands of a block's panels (complex64 LU, GEMM variants 3 and 4)
+  h8* ls = nullptr;
+  h8* us = nullptr;
+  int split_pad = 0, split_kmax = 0, split_batch = 0;
   unsigned long generation = 0;  // bumped at every reallocation (captured graphs hold the pointers)
+  size_t split_stride() const { return (size_t)(split_kmax / 8) * 4 * split_pad; }
+  // buffers for systems of order up to n, blocks of up to kmax columns, `batch` systems; zeroed
+  // once (rows and columns past the matrix are read as zeros)
+  void ensure_split(int n_, int batch_, int kmax) {
+    if (ls && split_pad >= n_ + 128 && split_kmax >= kmax && split_batch >= batch_) return;
+    hipFree(ls);
+    hipFree(us);
+    ls = us = nullptr;
+    split_pad = std::max(split_pad, n_ + 128);
+    split_kmax = std::max(split_kmax, kmax);
+    split_batch = std::max(split_batch, batch_);
+    const size_t bytes = split_stride() * split_batch * sizeof(h8);
+    HBG_HIP(hipMalloc(&ls, bytes));
+    HBG_HIP(hipMalloc(&us, bytes));
+    HBG_HIP(hipMemset(ls, 0, bytes));
+    HBG_HIP(hipMemset(us, 0, bytes));
+    ++generation;
+  }
   void release() {
+    hipFree(ls);
+    hipFree(us);
     hipFree(w.cand);
     hipFree(w.ccount);
     hipFree(w.ctr);
@@ -813,18 +837,19 @@ struct GemmTimer {
 // operands (gemm_f16x3_kernel).
 template <typename T>
 void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf,
-                 int kk, int batch, size_t mstride, hipStream_t stream) {
+                 int kk, int batch, size_t mstride, const SplitOps* so, hipStream_t stream) {
   if (nrows <= 0 || ncolsC <= 0) return;
   const GemmBatch gb{mstride, batch};
   const int tn = (ncolsC + 63) / 64;
   const unsigned g64 = gemm_grid((nrows + 63) / 64, tn, batch), g128 = gemm_grid((nrows + 127) / 128, tn, batch);
   if constexpr (std::is_same<T, float>::value) {
-    if (variant == 3) {
-      gemm_f16x3_kernel<64, 64><<<g64, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
-      return;
-    }
-    if (variant == 4) {
-      gemm_f16x3_kernel<128, 64><<<g128, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, kf, kk, gb);
+    if (variant >= 3) {
+      if (!so || (kf - so->K0) % 8 || kk % H16_KC) throw Error{6, "split-FP16 GEMM without its split operands"};
+      const int kg0 = (kf - so->K0) / 8;
+      if (variant == 3)
+        gemm_f16s_kernel<64, 64><<<g64, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
+      else
+        gemm_f16s_kernel<128, 64><<<g128, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
       return;
     }
   }
@@ -840,7 +865,7 @@ void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nr
 // not bounded by the matrix's scale).
 template <typename T>
 void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk, int batch,
-              size_t mstride, int nmat, hipStream_t stream, GemmTimer& gt) {
+              size_t mstride, int nmat, const SplitOps* so, hipStream_t stream, GemmTimer& gt) {
   if (nrows <= 0 || ncolsC <= 0 || kk <= 0) return;
   hipEvent_t g0 = nullptr;
   if (gt.stats) {
@@ -850,10 +875,10 @@ void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int
   const int v = g_gemm_variant_[prec_index<T>()];
   if (v >= 3 && c0 + ncolsC > nmat) {
     const int nm = nmat > c0 ? nmat - c0 : 0;
-    gemm_launch<T>(v, M, lda, r0, nrows, c0, nm, kf, kk, batch, mstride, stream);
-    gemm_launch<T>(v - 2, M, lda, r0, nrows, c0 + nm, ncolsC - nm, kf, kk, batch, mstride, stream);
+    gemm_launch<T>(v, M, lda, r0, nrows, c0, nm, kf, kk, batch, mstride, so, stream);
+    gemm_launch<T>(v - 2, M, lda, r0, nrows, c0 + nm, ncolsC - nm, kf, kk, batch, mstride, so, stream);
   } else {
-    gemm_launch<T>(v, M, lda, r0, nrows, c0, ncolsC, kf, kk, batch, mstride, stream);
+    gemm_launch<T>(v, M, lda, r0, nrows, c0, ncolsC, kf, kk, batch, mstride, so, stream);
   }
   if (gt.stats) {
     hipEvent_t g1 = gt.ep->next();
@@ -874,15 +899,20 @@ template <typename T>
 void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int batch, BatchStrides bs,
                        hipStream_t stream, LuWorkspace<T>& ws, GemmTimer& gt) {
   const int Q = std::max(1, g_lu_depth_[prec_index<T>()]);
+  // the split-FP16 GEMM reads its operands from the split buffers, filled panel by panel
+  const bool split = std::is_same<T, float>::value && g_gemm_variant_[prec_index<T>()] >= 3;
+  SplitOps so{ws.ls, ws.us, ws.split_pad, ws.split_stride(), ws.split_stride(), 0};
+  const SplitOps* sop = split ? &so : nullptr;
   for (int K0 = 0; K0 < n; K0 += Q * NB) {
     const int qend = std::min(n, K0 + Q * NB);
+    so.K0 = K0;
     for (int k0 = K0; k0 < qend; k0 += NB) {
       const int p = k0 / NB;
       const int kb = min(NB, n - k0);
       const int m = n - k0;
       if (k0 > K0) {
-        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, n, stream, gt);
-        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, n, stream, gt);
+        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, n, sop, stream, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, n, sop, stream, gt);
       }
       const size_t po = (size_t)p * NB * NB;
       diag_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
@@ -891,8 +921,12 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       if (rb + cb > 0)
         apply_kernel<T><<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, ws.uinv + po,
                                                                      ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs);
+      if constexpr (std::is_same<T, float>::value)
+        if (split && k0 + NB < n)
+          split_panel_kernel<<<dim3((unsigned)((16 * (size_t)(n - k0 - NB) + 255) / 256), 1, batch), 256, 0,
+                               stream>>>(M, lda, n, k0, so, bs.m);
     }
-    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, n, stream, gt);
+    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, n, sop, stream, gt);
   }
 }
 
@@ -932,7 +966,7 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
     if (cb > 0)
       apply_kernel<double><<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, linv, ws.udiag, nullptr,
                                                    nullptr, bs);
-    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, n, stream, gt);
+    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, n, nullptr, stream, gt);
   }
 }
 
@@ -988,6 +1022,8 @@ void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, 
   }
   LuWorkspace<T>& ws = workspace<T>();
   ws.ensure(n, batch);
+  if (std::is_same<T, float>::value && g_gemm_variant_[prec_index<T>()] >= 3)
+    ws.ensure_split(n, batch, NB * std::max(1, g_lu_depth_[prec_index<T>()]));
   BatchStrides bs = ws.strides();
   bs.m = mstride;
   const int ncols = n + nrhs;
