@@ -31,14 +31,20 @@ import numpy as np
 METRIC = "freq solves/s @8192 panels: mixed+NN vs plain FP64 GPU, A/B/X rel err<=1e-5"
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
+PEAK_FP16 = 2.5e15    # MI355X dense FP16 matrix, FLOP/s
+# LU GEMM variants: tiles, kernel, arithmetic of the products
+VARIANTS = {1: ("64x64", "gemm_mfma_kernel<T, 64, 64>", None), 2: ("128x64", "gemm_mfma_kernel<T, 128, 64>", None),
+            3: ("64x64 split-fp16", "gemm_f16s_kernel<64, 64>", "f16x3"),
+            4: ("128x64 split-fp16", "gemm_f16s_kernel<128, 64>", "f16x3")}
 # HBM bytes per launch of the path's GEMM kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
 # calibrated by a device copy of known size (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md):
 # (workload, path, GEMM tiles, panels per trailing update, batch) -> bytes
 TRAFFIC = {("cylinder8192fd", "mixed", "64x64", 8, 16): 1.076e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
-             "mixed": "mixed: FP64 assembly, complex64 LU on FP32 MFMA, FP64 iterative refinement to backward error 1e-12",
+             "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
+                      "operands split into hi + lo, autotuned), FP64 iterative refinement to backward error 1e-12",
              "mixed_nn": "mixed+NN: finite-depth evanescent remainder from the BF16-MFMA surrogate (FP64 evaluator for "
-                         "R/h < 0.1), complex64 LU on FP32 MFMA, FP64 iterative refinement to backward error 1e-12"}
+                         "R/h < 0.1), complex64 LU as the mixed path, FP64 iterative refinement to backward error 1e-12"}
 DTYPE = {"fp64": "f64", "mixed": "f32+f64", "mixed_nn": "bf16+f32+f64"}
 PEAK_HBM = 8.0e12    # MI355X HBM3E, B/s
 
@@ -252,13 +258,16 @@ def main():
             if prec not in tuning:
                 rows = s.autotune(N, 6 + len(heads), prec)
                 best = min(rows, key=lambda r: r[3])
-                tuning[prec] = {"gemm_tiles": "64x64" if best[0] == 1 else "128x64", "panels_per_update": int(best[1]),
+                tuning[prec] = {"gemm_variant": int(best[0]), "gemm_tiles": VARIANTS[int(best[0])][0],
+                                "panels_per_update": int(best[1]),
                                 "batch": int(best[2]), "ms_per_system": round(best[3], 3), "candidates": len(rows)}
                 print(f"[bench] autotuned {prec}: {tuning[prec]}", file=sys.stderr, flush=True)
             if a.batch == 0:
                 s.set_batch(tuning[prec]["batch"])
         out, dt, prof = run_path(s, w, p, a.steps, a.warmup, p)
-        res[p] = (out, summary(prof, dt, a.steps, F, peaks[p]))
+        gvar = s.get_tuning(s.TUNE_LU_GEMM if p == "fp64" else s.TUNE_LU_GEMM_MIXED)
+        res[p] = (out, summary(prof, dt, a.steps, F, PEAK_FP16 if VARIANTS[gvar][2] else peaks[p]))
+        res[p][1]["gemm_variant"] = gvar
     s.close()
     # `value`: the fastest path whose A, B and X stay within 1e-5 of the plain FP64 results (which
     # match the FP64 oracle to 1e-10); the others are reported beside it with their errors
@@ -290,12 +299,17 @@ def main():
         s2.close()
     gemm_tf = summ["gemm_achieved_tflops"]
     # the LU settings the value path ran with: autotuned, or given on the command line, or the defaults
+    gv = summ["gemm_variant"]
     tun = tuning.get("fp64" if main_path == "fp64" else "mixed",
-                     {"gemm_tiles": "128x64" if a.gemm == 2 else "64x64",
+                     {"gemm_variant": gv, "gemm_tiles": VARIANTS[gv][0],
                       "panels_per_update": a.depth if a.depth >= 0 else 4,
                       "batch": a.batch or None})
     traffic = TRAFFIC.get((a.workload, main_path, tun["gemm_tiles"], tun["panels_per_update"], tun["batch"]))
-    tile = tun["gemm_tiles"].replace("x", ", ")
+    tiles, kernel, split = VARIANTS[tun["gemm_variant"]]
+    kernel = kernel.replace("<T,", "<double," if main_path == "fp64" else "<float,")
+    # the split-FP16 GEMM is priced against the FP16 matrix peak (each FP32 product is three FP16
+    # products, so its FP32-equivalent ceiling is a third of that)
+    gpeak = PEAK_FP16 if split else peaks[main_path]
     line = {
         "metric": METRIC,
         "value": summ["freq_per_s"],
@@ -307,13 +321,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPE[main_path],
+        "dtype": ("f16x3+" if split else "") + DTYPE[main_path],
         "data": "synthetic: procedural mesh of the workload (tests/python/meshes.py layout)",
         "config": config,
-        "roofline": {"bound": "mfma", "kernel": f"gemm_mfma_kernel<{'double' if main_path == 'fp64' else 'float'}, {tile}> "
-                                                "(LU trailing updates, complex GEMM on MFMA)",
-                     "achieved": gemm_tf, "peak": peaks[main_path] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(gemm_tf * 1e12 / peaks[main_path], 4), "traffic": traffic,
+        "roofline": {"bound": "mfma", "kernel": kernel + " (LU trailing updates, complex GEMM on MFMA"
+                                                + (", FP32 operands split into FP16 hi + lo)" if split else ")"),
+                     "achieved": gemm_tf, "peak": gpeak / 1e12, "unit": "TFLOP/s",
+                     "frac": round(gemm_tf * 1e12 / gpeak, 4), "traffic": traffic,
                      "per_launch": summ["gemm_per_launch"]},
     }
     if not a.no_cpu_baseline:

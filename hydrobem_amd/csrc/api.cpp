@@ -48,9 +48,9 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
 void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
                       LuStats* stats, int* status);
 int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
-void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2, int batch,
-                         int n, int R, bool a32_written, double tol, int max_iter, hipStream_t stream, LuStats* stats,
-                         int* status, int* iters, double* eta);
+void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
+                         double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
+                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta);
 int residual_blocks(int n);
 size_t residual_part_elems(int n, int R);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
@@ -304,7 +304,7 @@ struct HbgSession {
   // mixed precision: the complex64 systems, the FP64 solutions, the complex64 corrections, norms
   DevBuf<float2> d_M32, d_D32;
   DevBuf<double2> d_X64;
-  DevBuf<double> d_rpart;
+  DevBuf<double> d_rpart, d_rscale;
   DevBuf<double2> d_rpart2;
   NnDevice nn;          // the learned surrogate of the finite-depth evanescent remainder
   DevBuf<int2> d_nnq;   // pairs left to the FP64 evaluator by the surrogate's assembly
@@ -820,6 +820,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     s->d_X64.ensure((size_t)N * R * B);
     s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * B);
     s->d_rpart2.ensure(residual_part_elems(N, R) * B);
+    s->d_rscale.ensure((size_t)R * B);
   }
   std::vector<double> hout(nout * B);
   std::vector<int> iters(B);
@@ -870,7 +871,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       if (progress && progress(user, HBG_STAGE_LU, f0 + b, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     LuStats lus;
     if (mixed) {
-      mixed_solve_batched(s->d_M.p, s->d_M32.p, s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p, nb, N, R, true,
+      mixed_solve_batched(s->d_M.p, s->d_M32.p, s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p, s->d_rscale.p, nb, N, R, true,
                           MIXED_TOL, MIXED_MAX_ITER, s->stream, prof ? &lus : nullptr, status.data(), iters.data(),
                           eta.data());
       for (int b = 0; b < nb; ++b) {

@@ -294,29 +294,40 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
 // of M.
 constexpr int MAXRHS = 32;
 constexpr int RHS_CHUNK = 256;
+// RM: right-hand sides held in registers (R <= RM); the columns of S go 8 at a time so that eight
+// loads are in flight per lane (the kernel streams S once; its rate is that of the loads in flight)
+template <int RM>
 __global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restrict__ S, int N, int lds,
                                                           const double2* __restrict__ u, int R,
                                                           double2* __restrict__ part) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int chunk = blockIdx.y;
   if (k >= N) return;
-  double2 acc[MAXRHS];
+  double2 acc[RM];
 #pragma unroll
-  for (int c = 0; c < MAXRHS; ++c) acc[c] = make_double2(0.0, 0.0);
-  const int l0 = chunk * RHS_CHUNK, l1 = min(N, l0 + RHS_CHUNK);
-  for (int l = l0; l < l1; ++l) {
-    const double2 s = S[(size_t)l * lds + k];
+  for (int c = 0; c < RM; ++c) acc[c] = make_double2(0.0, 0.0);
+  auto column = [&](double2 s, int l) {
 #pragma unroll
-    for (int c = 0; c < MAXRHS; ++c) {
+    for (int c = 0; c < RM; ++c) {
       if (c < R) {
         const double2 v = u[(size_t)c * N + l];
         acc[c].x -= s.x * v.x - s.y * v.y;
         acc[c].y -= s.x * v.y + s.y * v.x;
       }
     }
-  }
+  };
+  const int l0 = chunk * RHS_CHUNK, l1 = min(N, l0 + RHS_CHUNK);
+  int l = l0;
+  for (; l + 8 <= l1; l += 8) {
+    double2 sv[8];
 #pragma unroll
-  for (int c = 0; c < MAXRHS; ++c)
+    for (int q = 0; q < 8; ++q) sv[q] = S[(size_t)(l + q) * lds + k];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) column(sv[q], l + q);
+  }
+  for (; l < l1; ++l) column(S[(size_t)l * lds + k], l);
+#pragma unroll
+  for (int c = 0; c < RM; ++c)
     if (c < R) part[((size_t)chunk * R + c) * N + k] = acc[c];
 }
 
@@ -472,7 +483,13 @@ int rhs_chunks(int N) { return (N + RHS_CHUNK - 1) / RHS_CHUNK; }
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,
                 hipStream_t stream) {
   const int chunks = rhs_chunks(N);
-  rhs_partial_kernel<<<dim3((N + 255) / 256, chunks), 256, 0, stream>>>(S, N, lds, u, R, part);
+  const dim3 g((N + 255) / 256, chunks);
+  if (R <= 8)
+    rhs_partial_kernel<8><<<g, 256, 0, stream>>>(S, N, lds, u, R, part);
+  else if (R <= 16)
+    rhs_partial_kernel<16><<<g, 256, 0, stream>>>(S, N, lds, u, R, part);
+  else
+    rhs_partial_kernel<MAXRHS><<<g, 256, 0, stream>>>(S, N, lds, u, R, part);
   rhs_reduce_kernel<<<(unsigned)(((size_t)R * N + 255) / 256), 256, 0, stream>>>(part, N, R, chunks, M, ldm);
   HBG_HIP(hipGetLastError());
 }

@@ -191,8 +191,9 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(typename Cplx<T>::type* 
 // FP32 matrix rate. With |x| <= 1 the split keeps x to within 2^-23 |x| + 2^-25 (lo may be
 // subnormal: its absolute error stays 2^-25), so the update's error is FP32's relative to the
 // matrix's scale (normwise), not per entry. Values beyond the FP16 range (65504) are not
-// representable: the caller keeps such columns (the right-hand sides) on the FP32 kernel, and
-// the mixed-precision solve that uses this kernel checks every solution by its FP64 residual.
+// representable: the mixed-precision solve that uses this kernel hands over its right-hand
+// sides scaled by powers of two to |b| <= 1 (the matrix of the low-order method is O(1)), and
+// checks every solution by its FP64 residual.
 //
 // The operands are split once, when their panel is final (split_panel_kernel), into buffers laid
 // out for the GEMM: L rows as [k group of 8][part][row][8 halves], U columns as
@@ -225,16 +226,18 @@ __device__ __forceinline__ void split8(const float2* v, h8* part) {
 }
 
 // The final L21 (rows k0 + 64 .. n of the panel's 64 columns) and U12 (the panel's 64 rows,
-// columns k0 + 64 .. n) of the panel at k0 into the split buffers; thread = (row or column, k
-// group), consecutive threads along rows (columns); grid (ceil(16 (n - k0 - 64) / 256), 1, batch).
-__global__ void __launch_bounds__(256) split_panel_kernel(const float2* __restrict__ M, int lda, int n, int k0,
-                                                          SplitOps so, size_t mstride) {
-  const int m = n - k0 - 64;
+// columns k0 + 64 .. ncols, the right-hand sides included) of the panel at k0 into the split
+// buffers; thread = (row or column, k group), consecutive threads along rows (columns);
+// grid (ceil(8 (n + ncols - 2 k0 - 128) / 256), 1, batch).
+__global__ void __launch_bounds__(256) split_panel_kernel(const float2* __restrict__ M, int lda, int n, int ncols,
+                                                          int k0, SplitOps so, size_t mstride) {
+  const int ml = n - k0 - 64, mu = ncols - k0 - 64;
   const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (m <= 0 || e >= 16 * (size_t)m) return;
+  if (ml <= 0 || e >= 8 * (size_t)(ml + mu)) return;
   M += blockIdx.z * mstride;
-  const bool isl = e < 8 * (size_t)m;
-  const int idx = (int)(isl ? e : e - 8 * (size_t)m);
+  const bool isl = e < 8 * (size_t)ml;
+  const int m = isl ? ml : mu;
+  const int idx = (int)(isl ? e : e - 8 * (size_t)ml);
   const int r = k0 + 64 + idx % m, kg = idx / m;
   float2 v[8];
 #pragma unroll

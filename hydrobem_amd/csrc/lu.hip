@@ -860,26 +860,18 @@ void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nr
 }
 
 // M[r0 + i][c0 + j] -= sum_k M[r0 + i][kf + k] M[kf + k][c0 + j] (i < nrows, j < ncolsC, k < kk)
-// on `batch` matrices, on the matrix cores; timed when stats are kept. Columns from nmat on are
-// right-hand sides: under the split-FP16 variants they take the FP32 kernel (their values are
-// not bounded by the matrix's scale).
+// on `batch` matrices, on the matrix cores; timed when stats are kept. Under the split-FP16
+// variants the right-hand-side columns ride along in FP16: the caller scales them to |b| <= 1.
 template <typename T>
 void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk, int batch,
-              size_t mstride, int nmat, const SplitOps* so, hipStream_t stream, GemmTimer& gt) {
+              size_t mstride, const SplitOps* so, hipStream_t stream, GemmTimer& gt) {
   if (nrows <= 0 || ncolsC <= 0 || kk <= 0) return;
   hipEvent_t g0 = nullptr;
   if (gt.stats) {
     g0 = gt.ep->next();
     HBG_HIP(hipEventRecord(g0, stream));
   }
-  const int v = g_gemm_variant_[prec_index<T>()];
-  if (v >= 3 && c0 + ncolsC > nmat) {
-    const int nm = nmat > c0 ? nmat - c0 : 0;
-    gemm_launch<T>(v, M, lda, r0, nrows, c0, nm, kf, kk, batch, mstride, so, stream);
-    gemm_launch<T>(v - 2, M, lda, r0, nrows, c0 + nm, ncolsC - nm, kf, kk, batch, mstride, so, stream);
-  } else {
-    gemm_launch<T>(v, M, lda, r0, nrows, c0, ncolsC, kf, kk, batch, mstride, so, stream);
-  }
+  gemm_launch<T>(g_gemm_variant_[prec_index<T>()], M, lda, r0, nrows, c0, ncolsC, kf, kk, batch, mstride, so, stream);
   if (gt.stats) {
     hipEvent_t g1 = gt.ep->next();
     HBG_HIP(hipEventRecord(g1, stream));
@@ -911,8 +903,8 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int kb = min(NB, n - k0);
       const int m = n - k0;
       if (k0 > K0) {
-        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, n, sop, stream, gt);
-        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, n, sop, stream, gt);
+        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, sop, stream, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, sop, stream, gt);
       }
       const size_t po = (size_t)p * NB * NB;
       diag_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
@@ -923,10 +915,10 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
                                                                      ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs);
       if constexpr (std::is_same<T, float>::value)
         if (split && k0 + NB < n)
-          split_panel_kernel<<<dim3((unsigned)((16 * (size_t)(n - k0 - NB) + 255) / 256), 1, batch), 256, 0,
-                               stream>>>(M, lda, n, k0, so, bs.m);
+          split_panel_kernel<<<dim3((unsigned)((8 * (size_t)(n + ncols - 2 * k0 - 2 * NB) + 255) / 256), 1, batch),
+                               256, 0, stream>>>(M, lda, n, ncols, k0, so, bs.m);
     }
-    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, n, sop, stream, gt);
+    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, sop, stream, gt);
   }
 }
 
@@ -966,7 +958,7 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
     if (cb > 0)
       apply_kernel<double><<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, linv, ws.udiag, nullptr,
                                                    nullptr, bs);
-    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, n, nullptr, stream, gt);
+    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, nullptr, stream, gt);
   }
 }
 

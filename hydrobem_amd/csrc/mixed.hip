@@ -31,15 +31,49 @@ __global__ void cast_kernel(const double2* __restrict__ src, size_t sstride, flo
   }
 }
 
-// X (n x R, ld n) = the complex64 columns [n, n + R) of M32
+// The right-hand sides go to the complex64 LU scaled column by column by a power of two, so that
+// their largest |re| + |im| lies in (1/2, 1] (exact both ways): the LU's split-FP16 GEMM holds
+// them in FP16 along with the matrix. scale[z][c] = 2^-e, one workgroup per (c, z).
+__global__ void __launch_bounds__(256) rhs_scale_kernel(const double2* __restrict__ M64, size_t mstride, int n,
+                                                        int R, double* __restrict__ scale) {
+  __shared__ double red[4];
+  const double2* b = M64 + blockIdx.z * mstride + (size_t)(n + blockIdx.x) * n;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(b[i].x) + fabs(b[i].y));
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    int e = 0;
+    if (m > 0.0 && isfinite(m)) frexp(m, &e);  // m = f 2^e, f in [1/2, 1)
+    scale[(size_t)blockIdx.z * R + blockIdx.x] = ldexp(1.0, -e);
+  }
+}
+
+// the right-hand sides [n, n + R) of M64 into M32, times their column scales
+__global__ void cast_rhs_kernel(const double2* __restrict__ M64, float2* __restrict__ M32, size_t mstride, int n,
+                                int R, const double* __restrict__ scale) {
+  M64 += blockIdx.z * mstride + (size_t)n * n;
+  M32 += blockIdx.z * mstride + (size_t)n * n;
+  const size_t count = (size_t)n * R;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const double s = scale[(size_t)blockIdx.z * R + i / n];
+    const double2 v = M64[i];
+    M32[i] = float2((float)(v.x * s), (float)(v.y * s));
+  }
+}
+
+// X (n x R, ld n) = the complex64 columns [n, n + R) of M32, divided by their column scales
 __global__ void widen_kernel(const float2* __restrict__ M32, size_t mstride, int n, double2* __restrict__ X,
-                             size_t xstride, int R) {
+                             size_t xstride, int R, const double* __restrict__ scale) {
   M32 += blockIdx.z * mstride + (size_t)n * n;
   X += blockIdx.z * xstride;
   const size_t count = (size_t)n * R;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const double s = 1.0 / scale[(size_t)blockIdx.z * R + i / n];
     const float2 v = M32[i];
-    X[i] = double2((double)v.x, (double)v.y);
+    X[i] = double2((double)v.x * s, (double)v.y * s);
   }
 }
 
@@ -51,11 +85,15 @@ __global__ void widen_kernel(const float2* __restrict__ M32, size_t mstride, int
 //   in part[z][block][4].
 constexpr int RES_SPLIT = 8;
 
+// RM: right-hand sides held in registers (R <= RM); the columns go 8 at a time so that eight
+// loads of A are in flight per lane (the kernel streams A once: its rate is that of the loads in
+// flight).
+template <int RM>
 __global__ void __launch_bounds__(RES_THREADS) residual_part_kernel(const double2* __restrict__ M, size_t mstride,
                                                                     int n, const double2* __restrict__ X,
                                                                     size_t xstride, int R,
                                                                     double2* __restrict__ part2) {
-  __shared__ double2 xs[RES_JC][RES_MAXR];
+  __shared__ double2 xs[RES_JC][RM];
   M += blockIdx.z * mstride;
   X += blockIdx.z * xstride;
   const int tid = threadIdx.x;
@@ -63,37 +101,46 @@ __global__ void __launch_bounds__(RES_THREADS) residual_part_kernel(const double
   const bool live = i < n;
   const int cw = (n + RES_SPLIT - 1) / RES_SPLIT;
   const int jlo = blockIdx.y * cw, jhi = min(n, jlo + cw);
-  double2 acc[RES_MAXR];
+  double2 acc[RM];
 #pragma unroll
-  for (int c = 0; c < RES_MAXR; ++c) acc[c] = double2(0.0, 0.0);
+  for (int c = 0; c < RM; ++c) acc[c] = double2(0.0, 0.0);
   double rowsum = 0.0;
+  auto column = [&](double2 a, int jj) {
+    rowsum += fabs(a.x) + fabs(a.y);
+#pragma unroll
+    for (int c = 0; c < RM; ++c) {
+      if (c < R) {
+        const double2 x = xs[jj][c];
+        acc[c].x = fma(a.x, x.x, fma(-a.y, x.y, acc[c].x));
+        acc[c].y = fma(a.x, x.y, fma(a.y, x.x, acc[c].y));
+      }
+    }
+  };
   for (int j0 = jlo; j0 < jhi; j0 += RES_JC) {
     const int jn = min(RES_JC, jhi - j0);
     __syncthreads();
-    for (int e = tid; e < RES_JC * RES_MAXR; e += RES_THREADS) {
+    for (int e = tid; e < RES_JC * RM; e += RES_THREADS) {
       const int jj = e % RES_JC, c = e / RES_JC;
       xs[jj][c] = (jj < jn && c < R) ? X[(size_t)c * n + j0 + jj] : double2(0.0, 0.0);
     }
     __syncthreads();
     if (live) {
-      for (int jj = 0; jj < jn; ++jj) {
-        const double2 a = M[(size_t)(j0 + jj) * n + i];
-        rowsum += fabs(a.x) + fabs(a.y);
+      const double2* col = M + (size_t)j0 * n + i;
+      int jj = 0;
+      for (; jj + 8 <= jn; jj += 8) {
+        double2 a[8];
 #pragma unroll
-        for (int c = 0; c < RES_MAXR; ++c) {
-          if (c < R) {
-            const double2 x = xs[jj][c];
-            acc[c].x = fma(a.x, x.x, fma(-a.y, x.y, acc[c].x));
-            acc[c].y = fma(a.x, x.y, fma(a.y, x.x, acc[c].y));
-          }
-        }
+        for (int u = 0; u < 8; ++u) a[u] = col[(size_t)(jj + u) * n];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) column(a[u], jj + u);
       }
+      for (; jj < jn; ++jj) column(col[(size_t)jj * n], jj);
     }
   }
   if (!live) return;
   double2* o = part2 + (((size_t)blockIdx.z * RES_SPLIT + blockIdx.y) * (R + 1)) * n + i;
 #pragma unroll
-  for (int c = 0; c < RES_MAXR; ++c)
+  for (int c = 0; c < RM; ++c)
     if (c < R) o[(size_t)c * n] = acc[c];
   o[(size_t)R * n] = double2(rowsum, 0.0);
 }
@@ -189,18 +236,20 @@ size_t residual_part_elems(int n, int R) { return (size_t)RES_SPLIT * (R + 1) * 
 // [n, n + R) of M64 for the systems with status 0; status -1: the complex64 factorization needed
 // interchanges or the refinement did not reach the tolerance in max_iter steps (solve it in FP64);
 // k > 0: zero pivot k. iters[b], eta[b]: refinement steps and the final backward error.
-void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2, int batch,
-                         int n, int R, bool a32_written, double tol, int max_iter, hipStream_t stream, LuStats* stats,
-                         int* status, int* iters, double* eta) {
+void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
+                         double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
+                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta) {
   if (R > RES_MAXR) throw Error{HBG_ERR_UNSUPPORTED, "mixed precision: more than 32 right-hand sides"};
   const size_t ms = (size_t)n * (n + R), xs = (size_t)n * R;
   const int nblk = residual_blocks(n);
-  if (a32_written)  // the assembly wrote A in complex64 too: the right-hand sides only
-    cast_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M64 + (size_t)n * n, ms, M32 + (size_t)n * n, ms, xs);
-  else
-    cast_kernel<<<dim3(grid_for(ms), 1, batch), 256, 0, stream>>>(M64, ms, M32, ms, ms);
+  if (!a32_written)  // else the assembly wrote A in complex64 too: the right-hand sides only
+    cast_kernel<<<dim3(grid_for((size_t)n * n), 1, batch), 256, 0, stream>>>(M64, ms, M32, ms, (size_t)n * n);
+  if (R > 0) {
+    rhs_scale_kernel<<<dim3(R, 1, batch), 256, 0, stream>>>(M64, ms, n, R, rscale);
+    cast_rhs_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M64, M32, ms, n, R, rscale);
+  }
   lu_solve_batched32(M32, ms, batch, n, n, R, stream, stats, status);
-  widen_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M32, ms, n, X, xs, R);
+  widen_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M32, ms, n, X, xs, R, rscale);
   std::vector<double> hp((size_t)batch * nblk * 4);
   for (int b = 0; b < batch; ++b) {
     iters[b] = 0;
@@ -209,7 +258,13 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
   bool open = false;
   for (int b = 0; b < batch; ++b) open = open || status[b] == 0;
   for (int it = 0; open; ++it) {
-    residual_part_kernel<<<dim3(nblk, RES_SPLIT, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
+    const dim3 rg(nblk, RES_SPLIT, batch);
+    if (R <= 8)
+      residual_part_kernel<8><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
+    else if (R <= 16)
+      residual_part_kernel<16><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
+    else
+      residual_part_kernel<RES_MAXR><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
     residual_sum_kernel<<<dim3(nblk, 1, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, D, xs, part);
     HBG_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     HBG_HIP(hipStreamSynchronize(stream));
