@@ -17,7 +17,7 @@ import numpy as np
 
 from . import meshes  # noqa: F401
 
-__all__ = ["Session", "HbgError", "library", "LIB_PATH", "wamit"]
+__all__ = ["Session", "HbgError", "library", "LIB_PATH", "wamit", "solve_group"]
 
 LIB_PATH = Path(__file__).resolve().parent / "_build" / "libhydrobem_amd.so"
 
@@ -66,6 +66,8 @@ SIGNATURES = {
     "hbg_solve": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                    ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
                                    ctypes.c_void_p]),
+    "hbg_solve_group": (ctypes.c_int32, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, _dp, ctypes.c_int32, _dp,
+                                         ctypes.c_int32, ctypes.POINTER(Options)] + [ctypes.POINTER(_dp)] * 6),
     "hbg_wave_kernel": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, ctypes.c_int64, _dp]),
     "hbg_finite_kernel": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _dp,
                                            _dp, _dp, _dp]),
@@ -267,6 +269,8 @@ class Session:
     TUNE_LU_GEMM = 0
     TUNE_LU_DEPTH = 1
     TUNE_LU_GRAPHS = 2
+    TUNE_LU_GEMM_MIXED = 3
+    TUNE_LU_DEPTH_MIXED = 4
 
     def autotune(self, n, nrhs, precision="fp64") -> list:
         """Time the batched LU variants on synthetic systems and keep the fastest (hbg_autotune);
@@ -329,6 +333,32 @@ class Session:
         bb = np.array(b, dtype=np.complex128, order="F", copy=True).reshape(n, -1, order="F")
         _check(library().hbg_lu_solve(self._h, n, af.ctypes.data_as(_dp), bb.shape[1], bb.ctypes.data_as(_dp)))
         return bb.reshape(b.shape, order="F")
+
+
+def solve_group(sessions, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, precision="fp64",
+                surrogate=False):
+    """Session.solve for several geometries at once (hbg_solve_group): their (geometry, frequency)
+    systems of uneven order share the launches of the batched LU. Returns one result per session."""
+    w = np.ascontiguousarray(omegas, dtype=np.float64).reshape(-1)
+    h = np.ascontiguousarray(headings_deg, dtype=np.float64).reshape(-1)
+    F, H = len(w), len(h)
+    outs, ptrs = [], {k: [] for k in ("A", "B", "X", "FK", "SC", "HK")}
+    for s in sessions:
+        D = 6 * s.n_bodies
+        o = {"A": np.zeros((F, D, D)), "B": np.zeros((F, D, D)), "X": np.zeros((F, H, D), np.complex128),
+             "FK": np.zeros((F, H, D), np.complex128), "SC": np.zeros((F, H, D), np.complex128),
+             "HK": np.zeros((F, D, H), np.complex128)}
+        outs.append(o)
+        for k, v in o.items():
+            ptrs[k].append(v.ctypes.data_as(_dp))
+    arr = {k: (_dp * len(v))(*v) for k, v in ptrs.items()}
+    hs = (ctypes.c_void_p * len(sessions))(*[s._h.value for s in sessions])
+    opt = _options(logarithmic, log_tolerance, PRECISIONS[precision], surrogate)
+    _check(library().hbg_solve_group(hs, len(sessions), _p(w), F, _p(h), H, ctypes.byref(opt), arr["A"], arr["B"],
+                                     arr["X"], arr["FK"], arr["SC"], arr["HK"]))
+    return [{"omegas": w, "headings": h, "added_mass": o["A"], "radiation_damping": o["B"],
+             "excitation_diffraction": o["X"], "excitation_froude_krylov": o["FK"], "excitation_scattering": o["SC"],
+             "excitation_haskind": np.transpose(o["HK"], (0, 2, 1)).copy()} for o in outs]
 
 
 def dispersion_roots(H, n):

@@ -41,16 +41,18 @@ void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, 
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,
                 hipStream_t stream);
 int rhs_chunks(int N);
+void launch_pad_system(double2* M, float2* M32, int n, int nz, int R, int rz, hipStream_t stream);
 void launch_forces(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double rho,
                    const double2* M, int ldm, const double2* phiI, const double2* dphiI, double* out,
                    hipStream_t stream);
 int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
 void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
-                      LuStats* stats, int* status);
+                      LuStats* stats, int* status, const int* nsys = nullptr);
 int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
 void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
                          double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
-                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta);
+                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta,
+                         const int* nsys = nullptr);
 int residual_blocks(int n);
 size_t residual_part_elems(int n, int R);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
@@ -405,11 +407,13 @@ FinTables fin_tables(HbgSession* s) {
 
 // S and A of the frequency K into S (ld N) and M (ld N): infinite depth, or finite depth with the
 // per-frequency tables built first; returns the wave number of the incident wave
+// S and A (and A32) with leading dimension lda (at least N; S in s->d_S, lda x N)
 double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, bool surrogate = false,
-                          float2* A32 = nullptr) {
+                          float2* A32 = nullptr, int lda = 0) {
   const int N = s->N;
+  if (lda <= 0) lda = N;
   if (!std::isfinite(s->depth)) {
-    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, Mout, A32, N, s->stream);
+    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, Mout, A32, lda, s->stream);
     return K;
   }
   const FinFreq f = make_fin_freq(s->depth, K);
@@ -427,9 +431,9 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
     s->d_nnq.ensure((size_t)N * N);
     s->d_nnc.ensure(1);
     launch_assemble_fin_nn(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32,
-                           N, s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
+                           lda, s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
   } else
-    launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, N,
+    launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, lda,
                         s->stream);
   return f.k0;
 }
@@ -936,6 +940,163 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_CANCELLED, "cancelled"};
   return HBG_OK;
   GUARD_END("hbg_solve")
+}
+
+HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const double* omegas, int32_t n_omegas,
+                          const double* headings_deg, int32_t n_headings, const HbgOptions* options,
+                          double* const* added_mass, double* const* damping, double* const* excitation,
+                          double* const* froude_krylov, double* const* scattering, double* const* haskind) {
+  if (!sessions || count < 1 || (n_omegas > 0 && !omegas) || (n_headings > 0 && !headings_deg))
+    return fail(HBG_ERR_ARGUMENT, "null argument or no session", "hbg_solve_group");
+  if (n_omegas < 0 || n_headings < 0) return fail(HBG_ERR_ARGUMENT, "negative count", "hbg_solve_group");
+  for (int i = 0; i < count; ++i) {
+    if (!sessions[i]) return fail(HBG_ERR_ARGUMENT, "null session", "hbg_solve_group");
+    if (sessions[i]->N == 0) return fail(HBG_ERR_STATE, "a session has no panels", "hbg_solve_group");
+    if (sessions[i]->device != sessions[0]->device)
+      return fail(HBG_ERR_VALUE, "the sessions of a group must share their device", "hbg_solve_group");
+    for (int j = 0; j < i; ++j)
+      if (sessions[j] == sessions[i]) return fail(HBG_ERR_VALUE, "a session twice in the group", "hbg_solve_group");
+  }
+  for (int i = 0; i < n_omegas; ++i)
+    if (!(omegas[i] > 0.0) || !std::isfinite(omegas[i]))
+      return fail(HBG_ERR_VALUE, "omega " + std::to_string(i) + " must be finite and > 0", "hbg_solve_group");
+  HbgSession* lead = sessions[0];
+  std::vector<hipStream_t> own(count);
+  for (int i = 0; i < count; ++i) own[i] = sessions[i]->stream;
+  // every session's work goes on the lead session's stream for the call (restored on every exit)
+  struct StreamRestore {
+    HbgSession* const* s;
+    std::vector<hipStream_t>& own;
+    ~StreamRestore() {
+      for (size_t i = 0; i < own.size(); ++i) s[i]->stream = own[i];
+    }
+  } restore{sessions, own};
+  GUARD_BEGIN
+  check_options(options);
+  HbgOptions o = defaults(options);
+  use_device(lead);
+  const int H = n_headings;
+  int nmax = 0, Rmax = 0;
+  std::vector<int> Ns(count), nrad(count), Rs(count);
+  std::vector<double> heads(std::max(H, 1), 0.0);
+  for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
+  for (int i = 0; i < count; ++i) {
+    HbgSession* si = sessions[i];
+    Ns[i] = si->N;
+    nrad[i] = 6 * si->nb;
+    Rs[i] = nrad[i] + H;
+    if (Rs[i] > 32) throw Error{HBG_ERR_UNSUPPORTED, "more than 32 right-hand sides"};
+    nmax = std::max(nmax, Ns[i]);
+    Rmax = std::max(Rmax, Rs[i]);
+    check_depth(si);
+    ensure_statics(si, o.logarithmic ? 1 : 0, o.log_tolerance);
+    si->d_heads.ensure(heads.size());
+    HBG_HIP(hipMemcpy(si->d_heads.p, heads.data(), heads.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  for (int i = 0; i < count; ++i) sessions[i]->stream = lead->stream;
+  hipStream_t st = lead->stream;
+  const bool mixed = o.precision == HBG_PRECISION_MIXED;
+  const int systems = count * n_omegas;  // system q: frequency q / count of session q % count
+  const size_t mstride = (size_t)nmax * (nmax + Rmax);
+  const int B = batch_size(lead, nmax, Rmax, systems, mixed);
+  lead->d_M.ensure(mstride * B);
+  if (mixed) {
+    lead->d_M32.ensure(mstride * B);
+    lead->d_D32.ensure((size_t)nmax * Rmax * B);
+    lead->d_X64.ensure((size_t)nmax * Rmax * B);
+    lead->d_rpart.ensure((size_t)residual_blocks(nmax) * 4 * B);
+    lead->d_rpart2.ensure(residual_part_elems(nmax, Rmax) * B);
+    lead->d_rscale.ensure((size_t)Rmax * B);
+  }
+  std::vector<size_t> hstride(count), nout(count);
+  for (int i = 0; i < count; ++i) {
+    HbgSession* si = sessions[i];
+    const size_t D = nrad[i], nAB = D * nrad[i], nX = (size_t)H * D, nHK = (size_t)nrad[i] * H;
+    hstride[i] = (size_t)std::max(H, 1) * Ns[i];
+    nout[i] = 2 * nAB + 6 * nX + 2 * nHK;
+    si->d_S.ensure((size_t)nmax * Ns[i]);  // leading dimension nmax, as A
+    si->d_phiI.ensure(hstride[i] * B);
+    si->d_dphiI.ensure(hstride[i] * B);
+    si->d_u.ensure((size_t)Rs[i] * Ns[i]);
+    si->d_part.ensure((size_t)rhs_chunks(Ns[i]) * Rs[i] * Ns[i]);
+    si->d_out.ensure(nout[i] * B);
+  }
+  const double logc = o.logarithmic ? 1.0 : 0.0;
+  // system q into slot b: its matrices at order nmax, its right-hand sides at column nmax + c
+  auto build = [&](int q, int b) {
+    const int i = q % count, fi = q / count;
+    HbgSession* si = sessions[i];
+    const int N = Ns[i];
+    const double w = omegas[fi], K = w * w / si->grav;
+    const double depth = std::isfinite(si->depth) ? si->depth : 0.0;
+    double2* Mb = lead->d_M.p + mstride * b;
+    float2* M32b = mixed ? lead->d_M32.p + mstride * b : nullptr;
+    const double kw = assemble_frequency(si, K, logc, Mb, o.surrogate != 0, M32b, nmax);
+    launch_boundary(si->d_panels.p, si->d_body.p, si->d_nj.p, N, nrad[i], H, w, kw, si->grav, depth, si->d_heads.p,
+                    si->d_phiI.p + hstride[i] * b, si->d_dphiI.p + hstride[i] * b, si->d_u.p, st);
+    launch_rhs(si->d_S.p, N, nmax, si->d_u.p, Rs[i], Mb + (size_t)(nmax - N) * nmax, nmax, si->d_part.p, st);
+    launch_pad_system(Mb, M32b, nmax, N, Rmax, Rs[i], st);
+  };
+  std::vector<int> status(B), iters(B), nsys(B);
+  std::vector<double> eta(B), hout;
+  for (int q0 = 0; q0 < systems; q0 += B) {
+    const int nb = std::min(B, systems - q0);
+    for (int b = 0; b < nb; ++b) {
+      build(q0 + b, b);
+      nsys[b] = Ns[(q0 + b) % count];
+    }
+    if (mixed) {
+      mixed_solve_batched(lead->d_M.p, lead->d_M32.p, lead->d_X64.p, lead->d_D32.p, lead->d_rpart.p, lead->d_rpart2.p,
+                          lead->d_rscale.p, nb, nmax, Rmax, true, MIXED_TOL, MIXED_MAX_ITER, st, nullptr,
+                          status.data(), iters.data(), eta.data(), nsys.data());
+      for (int b = 0; b < nb; ++b)
+        if (status[b] < 0) {  // the plain FP64 path for this system
+          build(q0 + b, b);
+          lu_solve_batched(lead->d_M.p + mstride * b, mstride, 1, nmax, nmax, Rmax, st, nullptr, &status[b], &nsys[b]);
+        }
+    } else {
+      lu_solve_batched(lead->d_M.p, mstride, nb, nmax, nmax, Rmax, st, nullptr, status.data(), nsys.data());
+    }
+    for (int b = 0; b < nb; ++b) {
+      if (status[b] < 0) {  // a panel needs interchanges: rebuild the system, pivoted path
+        build(q0 + b, b);
+        status[b] = lu_solve_pivoted(lead->d_M.p + mstride * b, nmax, nmax, Rmax, st, nullptr);
+      }
+      if (status[b] > 0)
+        throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(status[b]) + ")"};
+    }
+    for (int b = 0; b < nb; ++b) {
+      const int i = (q0 + b) % count, fi = (q0 + b) / count;
+      HbgSession* si = sessions[i];
+      launch_forces(si->d_panels.p, si->d_body.p, si->d_nj.p, Ns[i], nrad[i], H, omegas[fi], si->rho,
+                    lead->d_M.p + mstride * b + (size_t)(nmax - Ns[i]) * nmax, nmax, si->d_phiI.p + hstride[i] * b,
+                    si->d_dphiI.p + hstride[i] * b, si->d_out.p + nout[i] * b, st);
+    }
+    HBG_HIP(hipStreamSynchronize(st));
+    for (int b = 0; b < nb; ++b) {
+      const int i = (q0 + b) % count, fi = (q0 + b) / count;
+      HbgSession* si = sessions[i];
+      hout.resize(nout[i]);
+      HBG_HIP(hipMemcpy(hout.data(), si->d_out.p + nout[i] * b, nout[i] * sizeof(double), hipMemcpyDeviceToHost));
+      for (double v : hout)
+        if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
+      const size_t D = nrad[i], nAB = D * nrad[i], nX = (size_t)H * D, nHK = (size_t)nrad[i] * H;
+      const double* hA = hout.data();
+      const double* hB = hA + nAB;
+      const double* hX = hB + nAB;
+      const double* hFK = hX + 2 * nX;
+      const double* hSC = hFK + 2 * nX;
+      const double* hHK = hSC + 2 * nX;
+      if (added_mass && added_mass[i]) std::memcpy(added_mass[i] + fi * nAB, hA, nAB * sizeof(double));
+      if (damping && damping[i]) std::memcpy(damping[i] + fi * nAB, hB, nAB * sizeof(double));
+      if (excitation && excitation[i]) std::memcpy(excitation[i] + fi * 2 * nX, hX, 2 * nX * sizeof(double));
+      if (froude_krylov && froude_krylov[i]) std::memcpy(froude_krylov[i] + fi * 2 * nX, hFK, 2 * nX * sizeof(double));
+      if (scattering && scattering[i]) std::memcpy(scattering[i] + fi * 2 * nX, hSC, 2 * nX * sizeof(double));
+      if (haskind && haskind[i]) std::memcpy(haskind[i] + fi * 2 * nHK, hHK, 2 * nHK * sizeof(double));
+    }
+  }
+  return HBG_OK;
+  GUARD_END("hbg_solve_group")
 }
 
 HbgStatus hbg_motions(HbgSession* s, int32_t n_omegas, const double* omegas, int32_t n_headings, int32_t n_dof,

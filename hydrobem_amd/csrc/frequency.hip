@@ -345,6 +345,23 @@ __global__ void rhs_reduce_kernel(const double2* __restrict__ part, int N, int R
   M[(size_t)(N + c) * ldm + k] = s;
 }
 
+// A system of order nz stored at order n (leading dimension n) among systems of uneven order:
+// the identity on rows and columns [nz, n), zeros elsewhere in them, zero rows [nz, n) of the
+// right-hand sides [0, rz) and zero right-hand sides [rz, R) (columns n + c). Its LU is then
+// block-diagonal and its solution zero on [nz, n). M32 (optional): the matrix part in complex64.
+// One workgroup per column.
+__global__ void __launch_bounds__(256) pad_system_kernel(double2* __restrict__ M, float2* __restrict__ M32, int n,
+                                                         int nz, int R, int rz) {
+  const int c = blockIdx.x;
+  if (c >= n + R) return;
+  const int lo = (c < nz || (c >= n && c - n < rz)) ? nz : 0;
+  for (int r = lo + threadIdx.x; r < n; r += 256) {
+    const double v = (c == r) ? 1.0 : 0.0;
+    M[(size_t)c * n + r] = make_double2(v, 0.0);
+    if (M32 && c < n) M32[(size_t)c * n + r] = float2((float)v, 0.0f);
+  }
+}
+
 // The forces of one frequency (lo_frequency.rs:1039-1171), each output a sum over the panels by
 // one workgroup (fixed-order tree reduction: results independent of timing).
 // phi: the solved potentials, column c at M[(N + c) * ldm]. Outputs (double, complex interleaved):
@@ -478,6 +495,12 @@ void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, 
 }
 
 int rhs_chunks(int N) { return (N + RHS_CHUNK - 1) / RHS_CHUNK; }
+
+void launch_pad_system(double2* M, float2* M32, int n, int nz, int R, int rz, hipStream_t stream) {
+  if (nz >= n && rz >= R) return;
+  pad_system_kernel<<<n + R, 256, 0, stream>>>(M, M32, n, nz, R, rz);
+  HBG_HIP(hipGetLastError());
+}
 
 // part: rhs_chunks(N) * R * N values of scratch
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,

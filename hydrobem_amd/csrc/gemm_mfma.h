@@ -48,11 +48,23 @@ struct Mfma<float> {
   static __device__ __forceinline__ int mrow(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
-// The systems of a batch: their number and the stride between them, in elements.
+// The systems of a batch: their number and the stride between them, in elements; for a batch of
+// systems of uneven order, each stored at the order nmat of the largest with an identity block
+// after its own order nsys[z] (the LU of such a system is block-diagonal), the orders: the tiles
+// that lie in the identity block, or that take their k from it, are zero updates and skipped.
 struct GemmBatch {
   size_t m;
   int batch;
+  const int* nsys;  // nullptr: every system of order nmat
+  int nmat;
 };
+
+// A tile of C rows [rlo, ..), columns [clo, clo + tn), k from kf on, of system z is a zero update
+__device__ __forceinline__ bool gemm_tile_padding(const GemmBatch& gb, int z, int rlo, int clo, int tn, int kf) {
+  if (!gb.nsys) return false;
+  const int nz = gb.nsys[z];
+  return rlo >= nz || kf >= nz || (clo >= nz && clo + tn <= gb.nmat);
+}
 
 constexpr int GEMM_KC = 16;
 
@@ -104,7 +116,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(typename Cplx<T>::type* 
   __shared__ C2 As[GEMM_KC][PA];             // [k][row of C]
   __shared__ C2 Bs[GEMM_KC][PB];             // [k][column of C]
   const GemmTile tile = gemm_tile((nrows + TM - 1) / TM, (ncolsC + TN - 1) / TN, gb.batch);
-  if (!tile.ok) return;
+  if (!tile.ok || gemm_tile_padding(gb, tile.b, r0 + tile.row * TM, c0 + tile.col * TN, TN, k0)) return;
   M += tile.b * gb.m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave & 1) * (TM / 2), wc = (wave >> 1) * (TN / 2);
@@ -267,7 +279,7 @@ __global__ void __launch_bounds__(256) gemm_f16s_kernel(float2* __restrict__ M, 
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * TM][H16_PAD];  // [part][row of C][k]
   __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * TN][H16_PAD];  // [part][column of C][k]
   const GemmTile tile = gemm_tile((nrows + TM - 1) / TM, (ncolsC + TN - 1) / TN, gb.batch);
-  if (!tile.ok) return;
+  if (!tile.ok || gemm_tile_padding(gb, tile.b, r0 + tile.row * TM, c0 + tile.col * TN, TN, so.K0 + 8 * kg0)) return;
   M += tile.b * gb.m;
   const h8* __restrict__ Ls = so.ls + tile.b * so.lstride;
   const h8* __restrict__ Us = so.us + tile.b * so.ustride;

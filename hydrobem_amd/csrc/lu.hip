@@ -649,6 +649,7 @@ struct LuWorkspace {
   int n = 0, batch = 0;
   TsluWork w{};              // the tournament (FP64 only)
   int* ipiv = nullptr;       // [batch][n]
+  int* nsys = nullptr;       // [batch]: orders of a batch of uneven systems (GemmBatch)
   int* flags = nullptr;      // [batch][panels]: 1 if the panel needed interchanges
   int* zp_fast = nullptr;    // [batch][panels]: zero pivot found without interchanges
   C2* uinv = nullptr;        // [batch][panels]: U11^-1 (NB x NB), for the back substitution
@@ -690,6 +691,7 @@ struct LuWorkspace {
     hipFree(w.winners);
     hipFree(w.zero_pivot);
     hipFree(ipiv);
+    hipFree(nsys);
     hipFree(flags);
     hipFree(zp_fast);
     hipFree(uinv);
@@ -730,6 +732,7 @@ struct LuWorkspace {
       HBG_HIP(hipMalloc(&save, (size_t)n_ * NB * sizeof(C2)));
     }
     HBG_HIP(hipMalloc(&ipiv, (size_t)batch_ * n_ * sizeof(int)));
+    HBG_HIP(hipMalloc(&nsys, (size_t)batch_ * sizeof(int)));
     HBG_HIP(hipMalloc(&flags, (size_t)batch_ * panels * sizeof(int)));
     HBG_HIP(hipMalloc(&zp_fast, (size_t)batch_ * panels * sizeof(int)));
     HBG_HIP(hipMalloc(&uinv, (size_t)batch_ * panels * NB * NB * sizeof(C2)));
@@ -774,11 +777,12 @@ namespace {
 struct GraphKey {
   const void* M;
   size_t mstride;
-  int batch, n, lda, nrhs, variant, depth, real_bytes;
+  int batch, n, lda, nrhs, variant, depth, real_bytes, uneven;
   unsigned long generation;
   bool operator==(const GraphKey& o) const {
     return M == o.M && mstride == o.mstride && batch == o.batch && n == o.n && lda == o.lda && nrhs == o.nrhs &&
-           variant == o.variant && depth == o.depth && real_bytes == o.real_bytes && generation == o.generation;
+           variant == o.variant && depth == o.depth && real_bytes == o.real_bytes && uneven == o.uneven &&
+           generation == o.generation;
   }
 };
 struct GraphCache {
@@ -837,9 +841,9 @@ struct GemmTimer {
 // operands (gemm_f16x3_kernel).
 template <typename T>
 void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf,
-                 int kk, int batch, size_t mstride, const SplitOps* so, hipStream_t stream) {
+                 int kk, const GemmBatch& gb, const SplitOps* so, hipStream_t stream) {
   if (nrows <= 0 || ncolsC <= 0) return;
-  const GemmBatch gb{mstride, batch};
+  const int batch = gb.batch;
   const int tn = (ncolsC + 63) / 64;
   const unsigned g64 = gemm_grid((nrows + 63) / 64, tn, batch), g128 = gemm_grid((nrows + 127) / 128, tn, batch);
   if constexpr (std::is_same<T, float>::value) {
@@ -863,15 +867,16 @@ void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nr
 // on `batch` matrices, on the matrix cores; timed when stats are kept. Under the split-FP16
 // variants the right-hand-side columns ride along in FP16: the caller scales them to |b| <= 1.
 template <typename T>
-void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk, int batch,
-              size_t mstride, const SplitOps* so, hipStream_t stream, GemmTimer& gt) {
+void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf, int kk,
+              const GemmBatch& gb, const SplitOps* so, hipStream_t stream, GemmTimer& gt) {
+  const int batch = gb.batch;
   if (nrows <= 0 || ncolsC <= 0 || kk <= 0) return;
   hipEvent_t g0 = nullptr;
   if (gt.stats) {
     g0 = gt.ep->next();
     HBG_HIP(hipEventRecord(g0, stream));
   }
-  gemm_launch<T>(g_gemm_variant_[prec_index<T>()], M, lda, r0, nrows, c0, ncolsC, kf, kk, batch, mstride, so, stream);
+  gemm_launch<T>(g_gemm_variant_[prec_index<T>()], M, lda, r0, nrows, c0, ncolsC, kf, kk, gb, so, stream);
   if (gt.stats) {
     hipEvent_t g1 = gt.ep->next();
     HBG_HIP(hipEventRecord(g1, stream));
@@ -889,8 +894,9 @@ void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int
 // the trailing matrix is read and written once per block rather than once per panel.
 template <typename T>
 void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int batch, BatchStrides bs,
-                       hipStream_t stream, LuWorkspace<T>& ws, GemmTimer& gt) {
+                       const int* nsys, hipStream_t stream, LuWorkspace<T>& ws, GemmTimer& gt) {
   const int Q = std::max(1, g_lu_depth_[prec_index<T>()]);
+  const GemmBatch gb{bs.m, batch, nsys, n};
   // the split-FP16 GEMM reads its operands from the split buffers, filled panel by panel
   const bool split = std::is_same<T, float>::value && g_gemm_variant_[prec_index<T>()] >= 3;
   SplitOps so{ws.ls, ws.us, ws.split_pad, ws.split_stride(), ws.split_stride(), 0};
@@ -903,8 +909,8 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int kb = min(NB, n - k0);
       const int m = n - k0;
       if (k0 > K0) {
-        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, batch, bs.m, sop, stream, gt);
-        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, batch, bs.m, sop, stream, gt);
+        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, gb, sop, stream, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, gb, sop, stream, gt);
       }
       const size_t po = (size_t)p * NB * NB;
       diag_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
@@ -918,7 +924,7 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
           split_panel_kernel<<<dim3((unsigned)((8 * (size_t)(n + ncols - 2 * k0 - 2 * NB) + 255) / 256), 1, batch),
                                256, 0, stream>>>(M, lda, n, ncols, k0, so, bs.m);
     }
-    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, batch, bs.m, sop, stream, gt);
+    gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, gb, sop, stream, gt);
   }
 }
 
@@ -958,7 +964,8 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
     if (cb > 0)
       apply_kernel<double><<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, linv, ws.udiag, nullptr,
                                                    nullptr, bs);
-    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, 1, 0, nullptr, stream, gt);
+    gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, GemmBatch{0, 1, nullptr, n},
+                     nullptr, stream, gt);
   }
 }
 
@@ -1003,7 +1010,7 @@ void finish_stats(LuStats* stats, EventPool& ep, hipEvent_t t0, GemmTimer& gt, h
 
 template <typename T>
 void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
-                   LuStats* stats, int* status) {
+                   LuStats* stats, int* status, const int* nsys_host = nullptr) {
   EventPool& ep = events();
   ep.used = 0;
   GemmTimer gt{&ep, stats, {}};
@@ -1021,17 +1028,20 @@ void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, 
   const int ncols = n + nrhs;
   const int panels = (n + NB - 1) / NB;
   const size_t pf = (size_t)bs.flag * batch;
+  // uneven orders: copied before the launches (the captured graph reads them from ws.nsys)
+  if (nsys_host) HBG_HIP(hipMemcpyAsync(ws.nsys, nsys_host, batch * sizeof(int), hipMemcpyHostToDevice, stream));
+  const int* nsys = nsys_host ? ws.nsys : nullptr;
   auto launches = [&]() {
     fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.flags, pf, 0);
     fill_i32<<<(unsigned)((pf + 255) / 256), 256, 0, stream>>>(ws.zp_fast, pf, 0x7fffffff);
-    factor_optimistic<T>(M, n, lda, ncols, batch, bs, stream, ws, gt);
+    factor_optimistic<T>(M, n, lda, ncols, batch, bs, nsys, stream, ws, gt);
     if (nrhs > 0) substitute<T>(M, n, lda, mstride, M + (size_t)n * lda, lda, mstride, nrhs, batch, true, stream, ws);
   };
   if (g_lu_graphs && !stats) {
     // the launch sequence depends only on these; replayed from a captured hipGraph
     const GraphKey key{(const void*)M, mstride, batch, n, lda, nrhs, g_gemm_variant_[prec_index<T>()],
                        g_lu_depth_[prec_index<T>()],
-                       std::is_same<T, double>::value ? 8 : 4, ws.generation};
+                       std::is_same<T, double>::value ? 8 : 4, nsys ? 1 : 0, ws.generation};
     GraphCache& gc = graphs();
     hipGraphExec_t exec = gc.find(key);
     if (!exec) {
@@ -1071,16 +1081,21 @@ void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, 
 // none. status[b]: 0 solved (the partially pivoted result), k > 0 zero pivot k (1-based; the
 // column is zero on and below the diagonal, as partial pivoting would find it), -1 a panel needs
 // interchanges: the matrix is left overwritten and must be rebuilt and given to lu_solve_pivoted.
+//
+// nsys: for a batch of systems of uneven order, their orders (host array): system b of order
+// nsys[b] <= n stored at order n with the identity on rows and columns [nsys[b], n) and zero
+// right-hand-side rows there, so that its LU is block-diagonal; the GEMMs skip the zero updates
+// of the identity blocks.
 void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
-                      LuStats* stats, int* status) {
-  solve_batched<double>(M, mstride, batch, n, lda, nrhs, stream, stats, status);
+                      LuStats* stats, int* status, const int* nsys) {
+  solve_batched<double>(M, mstride, batch, n, lda, nrhs, stream, stats, status, nsys);
 }
 
 // The same in complex64 (FP32 arithmetic, FP32 MFMA); the factors and the inverses of their
 // diagonal blocks stay for lu_resolve32 until the next call.
 void lu_solve_batched32(float2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
-                        LuStats* stats, int* status) {
-  solve_batched<float>(M, mstride, batch, n, lda, nrhs, stream, stats, status);
+                        LuStats* stats, int* status, const int* nsys) {
+  solve_batched<float>(M, mstride, batch, n, lda, nrhs, stream, stats, status, nsys);
 }
 
 // X = U^-1 L^-1 X with the factors of the last lu_solve_batched32 (systems without interchanges):
@@ -1248,7 +1263,7 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
   }
   HBG_HIP(hipMemcpyAsync(ws.backup, M, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
   int st = 0;
-  lu_solve_batched(M, whole, 1, n, lda, nrhs, stream, stats, &st);
+  lu_solve_batched(M, whole, 1, n, lda, nrhs, stream, stats, &st, nullptr);
   if (st >= 0) return st;
   HBG_HIP(hipMemcpyAsync(M, ws.backup, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
   return lu_solve_pivoted(M, n, lda, nrhs, stream, stats);

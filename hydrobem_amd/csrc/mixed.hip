@@ -221,7 +221,7 @@ unsigned grid_for(size_t count) { return (unsigned)std::min<size_t>((count + 255
 }  // namespace
 
 void lu_solve_batched32(float2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
-                        LuStats* stats, int* status);
+                        LuStats* stats, int* status, const int* nsys = nullptr);
 void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, float2* X, int ldx, size_t xstride,
                   int nrhs, hipStream_t stream);
 
@@ -238,7 +238,7 @@ size_t residual_part_elems(int n, int R) { return (size_t)RES_SPLIT * (R + 1) * 
 // k > 0: zero pivot k. iters[b], eta[b]: refinement steps and the final backward error.
 void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
                          double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
-                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta) {
+                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta, const int* nsys) {
   if (R > RES_MAXR) throw Error{HBG_ERR_UNSUPPORTED, "mixed precision: more than 32 right-hand sides"};
   const size_t ms = (size_t)n * (n + R), xs = (size_t)n * R;
   const int nblk = residual_blocks(n);
@@ -248,7 +248,7 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
     rhs_scale_kernel<<<dim3(R, 1, batch), 256, 0, stream>>>(M64, ms, n, R, rscale);
     cast_rhs_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M64, M32, ms, n, R, rscale);
   }
-  lu_solve_batched32(M32, ms, batch, n, n, R, stream, stats, status);
+  lu_solve_batched32(M32, ms, batch, n, n, R, stream, stats, status, nsys);
   widen_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M32, ms, n, X, xs, R, rscale);
   std::vector<double> hp((size_t)batch * nblk * 4);
   for (int b = 0; b < batch; ++b) {

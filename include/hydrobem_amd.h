@@ -108,6 +108,20 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                     double* excitation, double* froude_krylov, double* scattering, double* haskind,
                     HbgProgressFn progress, void* user);
 
+/* The solve of several geometries, one session each (all on one device; uneven panel counts and
+ * body counts allowed), at the same frequencies and headings, with hbg_solve's options: the
+ * (geometry, frequency) systems share every launch of the batched LU. No counterpart in the
+ * reference, which solves each model on its own (the multi-body case of a single geometry is
+ * hbg_solve's). A system of N panels goes to the LU at the order of the largest geometry, with
+ * the identity on the rows and columns past N (its LU is block-diagonal, its solution unchanged)
+ * and the GEMM tiles of that identity block skipped. Outputs: arrays of count pointers (arrays
+ * or pointers may be NULL), entry i laid out as hbg_solve's for session i. The sessions' streams
+ * are used in turn; a session must not be in use elsewhere during the call. */
+HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const double* omegas, int32_t n_omegas,
+                          const double* headings_deg, int32_t n_headings, const HbgOptions* options,
+                          double* const* added_mass, double* const* damping, double* const* excitation,
+                          double* const* froude_krylov, double* const* scattering, double* const* haskind);
+
 /* The motions of the bodies (RAOs) from a solve's outputs (motion.rs:121-158 solve_motions):
  * per frequency (C - w^2 (M + A) + i w (B + B_ext)) xi = X on the n_dof = 6 n_bodies modes, complex,
  * partial pivoting. mass, stiffness, external_damping (NULL: none): [D][D] row-major, about the
@@ -196,11 +210,19 @@ HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t bat
  * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64, and for
  * the complex64 LU of the mixed precision also 3: 64 x 64, 4: 128 x 64 on the FP16 matrix cores
  * with each FP32 operand split into two FP16 numbers (three FP16 products per FP32 product; the
- * complex128 LU takes 1 for 3 and 2 for 4); hbg_get_tuning reports the complex64 variant;
- * HBG_TUNE_LU_DEPTH the number of 64-column panels per trailing update (1..16, default 4);
- * HBG_TUNE_LU_GRAPHS 1 (default): the batched LU's launches are captured once per system shape
- * and buffer into a hipGraph and replayed (not while stage profiling is on), 0: launched one by one. */
-enum { HBG_TUNE_LU_GEMM = 0, HBG_TUNE_LU_DEPTH = 1, HBG_TUNE_LU_GRAPHS = 2 };
+ * complex128 LU takes 1 for 3 and 2 for 4); HBG_TUNE_LU_DEPTH the number of 64-column panels per
+ * trailing update (1..16, default 4). Both set the complex128 and the complex64 LU and read back
+ * the complex128 one; HBG_TUNE_LU_GEMM_MIXED and HBG_TUNE_LU_DEPTH_MIXED set and read the
+ * complex64 LU's (the mixed precision) alone. HBG_TUNE_LU_GRAPHS 1 (default): the batched LU's
+ * launches are captured once per system shape and buffer into a hipGraph and replayed (not while
+ * stage profiling is on), 0: launched one by one. */
+enum {
+  HBG_TUNE_LU_GEMM = 0,
+  HBG_TUNE_LU_DEPTH = 1,
+  HBG_TUNE_LU_GRAPHS = 2,
+  HBG_TUNE_LU_GEMM_MIXED = 3,
+  HBG_TUNE_LU_DEPTH_MIXED = 4
+};
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
  * systems of order n with nrhs right-hand sides in the given precision for every GEMM tile variant

@@ -55,7 +55,13 @@ def test_autotune_sweep_and_solve(oracle):
     best = min(rows, key=lambda r: r[3])
     assert s.get_tuning(s.TUNE_LU_GEMM) == int(best[0]) and s.get_tuning(s.TUNE_LU_DEPTH) == int(best[1])
     rows32 = s.autotune(1024, 7, "mixed")
-    assert len(rows32) == 24
+    say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} b{int(r[2])} {r[3]:.3f} ms" for r in rows32))
+    assert len(rows32) == 4 * 3 * 4 and all(r[3] > 0 for r in rows32)
+    best32 = min(rows32, key=lambda r: r[3])
+    assert s.get_tuning(s.TUNE_LU_GEMM_MIXED) == int(best32[0])
+    assert s.get_tuning(s.TUNE_LU_DEPTH_MIXED) == int(best32[1])
+    # the complex128 settings stay those of its own sweep
+    assert s.get_tuning(s.TUNE_LU_GEMM) == int(best[0]) and s.get_tuning(s.TUNE_LU_DEPTH) == int(best[1])
     v = meshes.cylinder_panels(1.0, 1.0, 32, 16, 16, quarter=False, cosine=True)
     s.set_environment(1025.0, 9.81)
     s.set_panels(v)

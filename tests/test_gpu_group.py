@@ -1,0 +1,60 @@
+"""The group solve (hbg_solve_group): geometries of uneven panel and body counts, and of their own
+environments, solved through one batched LU (GPU), give the results of their own solves."""
+
+import sys
+
+import numpy as np
+import pytest
+
+import hydrobem_amd as hb
+from hydrobem_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    return float(np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def geometries():
+    hemi = meshes.hemisphere_panels(1.0, 10, 20, quarter=False)
+    cyl = meshes.cylinder_panels(1.0, 1.0, 22, 7, 7, quarter=False, cosine=True)
+    small = meshes.cylinder_panels(0.75, 1.0, 14, 5, 5, quarter=False, cosine=True)
+    shift = np.array([4.0, 0.0, 0.0])
+    pair = np.concatenate([small, small + shift])
+    body = np.repeat([0, 1], len(small)).astype(np.int32)
+    centres = np.array([[0.0, 0.0, 0.0], shift])
+    # (vertices, body, rotation centres, depth)
+    return [(hemi, None, None, np.inf), (cyl, None, None, 3.0), (pair, body, centres, np.inf)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_solve_matches_separate_solves(precision):
+    sessions = []
+    for v, body, centres, depth in geometries():
+        s = hb.Session(0)
+        s.set_environment(1025.0, 9.81, depth)
+        s.set_panels(v, body, centres)
+        sessions.append(s)
+    orders = [s.n_panels for s in sessions]
+    say(f"case: group solve {precision}, panels {orders}, bodies {[s.n_bodies for s in sessions]}")
+    assert len(set(orders)) == len(orders) and any(n % 64 for n in orders)
+    w = np.array([0.6, 1.3, 2.2])
+    heads = [0.0, 30.0]
+    sessions[0].set_batch(4)  # 9 systems in batches of 4: every batch mixes the orders
+    group = hb.solve_group(sessions, w, heads, precision=precision)
+    tol = 1e-11 if precision == "fp64" else 1e-9
+    for s, g in zip(sessions, group):
+        ref = s.solve(w, heads)
+        for key in ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_haskind"):
+            assert g[key].shape == ref[key].shape
+            e = rel(g[key], ref[key])
+            say(f"  N {s.n_panels} {key}: {e:.2e}")
+            assert e < tol
+    for s in sessions:
+        s.close()

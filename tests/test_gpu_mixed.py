@@ -80,8 +80,8 @@ def test_mixed_split_fp16_gemm(session, variant):
     session.set_environment(1025.0, 9.81, 3.0)
     session.set_panels(v)
     plain = session.solve(w, heads)
-    old = session.get_tuning(session.TUNE_LU_GEMM)
-    session.set_tuning(session.TUNE_LU_GEMM, variant)
+    old = session.get_tuning(session.TUNE_LU_GEMM_MIXED)
+    session.set_tuning(session.TUNE_LU_GEMM_MIXED, variant)
     try:
         for graphs in (1, 0):  # replayed graph, then launched one by one with stage timing
             session.set_tuning(session.TUNE_LU_GRAPHS, graphs)
@@ -101,4 +101,4 @@ def test_mixed_split_fp16_gemm(session, variant):
     finally:
         session.set_profiling(False)
         session.set_tuning(session.TUNE_LU_GRAPHS, 1)
-        session.set_tuning(session.TUNE_LU_GEMM, old)
+        session.set_tuning(session.TUNE_LU_GEMM_MIXED, old)
