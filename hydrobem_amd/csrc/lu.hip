@@ -570,7 +570,17 @@ __global__ void __launch_bounds__(256) subst_block_kernel(const typename Cplx<T>
   Tinv += blockIdx.z * tstride;
   Y += blockIdx.z * ystride;
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) U[e >> 6][e & 63] = Tinv[e];
+  {
+    // all 16 loads of a thread in flight before the LDS stores
+    C2 t[NB * NB / 256];
+#pragma unroll
+    for (int j = 0; j < NB * NB / 256; ++j) t[j] = Tinv[tid + 256 * j];
+#pragma unroll
+    for (int j = 0; j < NB * NB / 256; ++j) {
+      const int e = tid + 256 * j;
+      U[e >> 6][e & 63] = t[j];
+    }
+  }
   for (int e = tid; e < NB * nrhs; e += 256) {
     const int r = e & (NB - 1), c = e >> 6;
     Ys[r][c] = r < kb ? Y[(size_t)c * ldy + k0 + r] : C2(0, 0);
@@ -608,19 +618,29 @@ __global__ void __launch_bounds__(64) subst_update_kernel(const typename Cplx<T>
   const int i = blockIdx.x * 64 + tid;
   if (i >= nr) return;
   const int row = r0 + i;
+  const C2* col = M + (size_t)k0 * lda + row;
   for (int q0 = 0; q0 < nrhs; q0 += 8) {
     C2 acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = C2(0, 0);
-    for (int c = 0; c < kb; ++c) {
-      const C2 u = M[(size_t)(k0 + c) * lda + row];
+    auto term = [&](C2 u, int c) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const C2 x = Xb[c][min(q0 + q, MAXRHS_LU - 1)];
         acc[q].x = fma(u.x, x.x, fma(-u.y, x.y, acc[q].x));
         acc[q].y = fma(u.x, x.y, fma(u.y, x.x, acc[q].y));
       }
+    };
+    // eight loads in flight per lane (the block's column of M is streamed once per 8 right-hand sides)
+    int c = 0;
+    for (; c + 8 <= kb; c += 8) {
+      C2 u[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) u[j] = col[(size_t)(c + j) * lda];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) term(u[j], c + j);
     }
+    for (; c < kb; ++c) term(col[(size_t)c * lda], c);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       if (q0 + q >= nrhs) break;
