@@ -453,12 +453,13 @@ __global__ void __launch_bounds__(256) diag_kernel(typename Cplx<T>::type* __res
 }
 
 // The panel's block products with the triangle inverses, in place:
-//   blocks [0, rblocks): L21 = A21 Uinv on 16 rows each (rows k0 + kb + 16 i .. < lim), checking
+//   blocks [0, rblocks): L21 = A21 Uinv on AB rows each (rows k0 + kb + AB i .. < lim), checking
 //     that |l_ij u_jj| (|re| + |im|) does not exceed |u_jj|: the entry at step j of the
 //     elimination below the pivot (else the panel's flag);
-//   blocks [rblocks, gridDim.x): U12 = Linv A12 on 16 columns each (< ncols).
-// gate: run only if the flag is set.
-constexpr int AB = 16;
+//   blocks [rblocks, gridDim.x): U12 = Linv A12 on AB columns each (< ncols).
+// gate: run only if the flag is set. The inverse is staged in LDS with all of a thread's loads in
+// flight at once.
+constexpr int AB = 32;
 template <typename T>
 __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __restrict__ M, int lda, int lim, int ncols, int k0,
                                                     int kb, int rblocks, const typename Cplx<T>::type* __restrict__ Uinv,
@@ -481,24 +482,44 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
   const int tid = threadIdx.x;
   const bool right = (int)blockIdx.x < rblocks;
   const C2* __restrict__ Tm = right ? Uinv : Linv;
-  for (int e = tid; e < NB * NB; e += 256) Ts[e >> 6][e & 63] = Tm[e];
+  {
+    C2 t[NB * NB / 256];
+#pragma unroll
+    for (int j = 0; j < NB * NB / 256; ++j) t[j] = Tm[tid + 256 * j];
+#pragma unroll
+    for (int j = 0; j < NB * NB / 256; ++j) {
+      const int e = tid + 256 * j;
+      Ts[e >> 6][e & 63] = t[j];
+    }
+  }
+  constexpr int VR = NB * AB / 256;  // outputs per thread
   if (right) {
     const int base = k0 + kb + blockIdx.x * AB;
     if (tid == 0) s_fail = 0;
-    for (int e = tid; e < NB * AB; e += 256) {
-      const int i = e & (AB - 1), k = e >> 4;
-      Xs[k][i] = (base + i < lim && k < kb) ? M[(size_t)(k0 + k) * lda + base + i] : C2(0.0, 0.0);
+    {
+      C2 t[VR];
+#pragma unroll
+      for (int j = 0; j < VR; ++j) {
+        const int e = tid + 256 * j, i = e % AB, k = e / AB;
+        t[j] = (base + i < lim && k < kb) ? M[(size_t)(k0 + k) * lda + base + i] : C2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int j = 0; j < VR; ++j) {
+        const int e = tid + 256 * j;
+        Xs[e / AB][e % AB] = t[j];
+      }
     }
     __syncthreads();
-    const int i = tid & 15, cq = tid >> 4;
-    C2 acc[4];
+    constexpr int CG = 256 / AB;  // column groups
+    const int i = tid % AB, cq = tid / AB;
+    C2 acc[VR];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] = C2(0.0, 0.0);
+    for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
     for (int k = 0; k < NB; ++k) {
       const C2 x = Xs[k][i];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const C2 u = Ts[k][cq + 16 * v];
+      for (int v = 0; v < VR; ++v) {
+        const C2 u = Ts[k][cq + CG * v];
         acc[v].x = fma(x.x, u.x, fma(-x.y, u.y, acc[v].x));
         acc[v].y = fma(x.x, u.y, fma(x.y, u.x, acc[v].y));
       }
@@ -506,8 +527,8 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     bool fail = false;
     if (base + i < lim) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int c = cq + 16 * v;
+      for (int v = 0; v < VR; ++v) {
+        const int c = cq + CG * v;
         if (c >= kb) continue;
         M[(size_t)(k0 + c) * lda + base + i] = acc[v];
         if (flag) {
@@ -521,19 +542,28 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     if (tid == 0 && s_fail && flag) *flag = 1;
   } else {
     const int base = k0 + kb + (blockIdx.x - rblocks) * AB;
-    for (int e = tid; e < NB * AB; e += 256) {
-      const int k = e & 63, c = e >> 6;
-      Xs[k][c] = (base + c < ncols && k < kb) ? M[(size_t)(base + c) * lda + k0 + k] : C2(0.0, 0.0);
+    {
+      C2 t[VR];
+#pragma unroll
+      for (int j = 0; j < VR; ++j) {
+        const int e = tid + 256 * j, k = e & 63, c = e >> 6;
+        t[j] = (base + c < ncols && k < kb) ? M[(size_t)(base + c) * lda + k0 + k] : C2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int j = 0; j < VR; ++j) {
+        const int e = tid + 256 * j;
+        Xs[e & 63][e >> 6] = t[j];
+      }
     }
     __syncthreads();
     const int r = tid & 63, cq = tid >> 6;
-    C2 acc[4];
+    C2 acc[VR];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] = C2(0.0, 0.0);
+    for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
     for (int k = 0; k < NB; ++k) {
       const C2 l = Ts[r][k];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
+      for (int v = 0; v < VR; ++v) {
         const C2 x = Xs[k][cq + 4 * v];
         acc[v].x = fma(l.x, x.x, fma(-l.y, x.y, acc[v].x));
         acc[v].y = fma(l.x, x.y, fma(l.y, x.x, acc[v].y));
@@ -541,7 +571,7 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     }
     if (r < kb) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
+      for (int v = 0; v < VR; ++v) {
         const int c = base + cq + 4 * v;
         if (c < ncols) M[(size_t)c * lda + k0 + r] = acc[v];
       }
