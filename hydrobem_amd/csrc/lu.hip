@@ -801,7 +801,7 @@ constexpr int prec_index() {
   return std::is_same<T, double>::value ? 0 : 1;
 }
 void lu_set_gemm_variant(int v) {
-  g_gemm_variant_[0] = v > 2 ? v - 2 : v;  // complex128 has the FP64 kernels only
+  g_gemm_variant_[0] = (v == 2 || v == 4) ? 2 : 1;  // complex128 has the FP64 kernels only
   g_gemm_variant_[1] = v;
 }
 int lu_gemm_variant() { return g_gemm_variant_[0]; }
@@ -888,7 +888,7 @@ struct GemmTimer {
 
 // One launch of the trailing-update GEMM in the given tile variant: 1 64 x 64, 2 128 x 64 (FP32 or
 // FP64 matrix cores); complex64 only: 3 64 x 64, 4 128 x 64 on the FP16 matrix cores with split
-// operands (gemm_f16x3_kernel).
+// operands (gemm_f16s_kernel).
 template <typename T>
 void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf,
                  int kk, const GemmBatch& gb, const SplitOps* so, hipStream_t stream) {
