@@ -10,6 +10,7 @@ C++/OpenMP restatement of the reference's algorithm) timed on this host on a bou
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cylinder8192fd] [--paths fp64,mixed]
                     [--batch B] [--no-cpu-baseline]
+    python bench.py --workload batching4    (uneven geometries through one group solve)
 
 By default three paths run on the 8192-panel finite-depth cylinder (the workload the metric is
 quoted on): plain FP64, mixed (complex64 LU + FP64 refinement) and mixed+NN (the learned surrogate
@@ -77,6 +78,64 @@ def workload(name):
         return dict(name=name, vertices=v, omegas=np.sqrt(9.81 * np.linspace(0.1, 3.0, 16)), headings=[0.0],
                     rho=1025.0, g=9.81, desc="floating hemisphere R=1, 512 panels, 16 kR 0.1-3.0, heading 0")
     raise SystemExit(f"unknown workload {name}")
+
+
+def bench_group(a):
+    """The batching configuration: truncated cylinders R = 1.0 / 0.75 / 1.5 m (draft 1 m) of 1536 / 2816 /
+    6144 panels as separate bodies plus their array at 4 m spacing (10496 panels, three bodies),
+    infinite depth, 64 frequencies kR 0.05-4.0 (R = 1 m), heading 0: 256 (N, omega) systems of uneven
+    order, solved by one group solve (mixed precision; their systems share the batched LU) and, for
+    comparison, by a solve of each geometry on its own. `value`: systems per second of the group
+    solve."""
+    import hydrobem_amd as hb
+    from hydrobem_amd import meshes
+
+    bodies = [meshes.cylinder_panels(R, 1.0, *d, quarter=False, cosine=True)
+              for R, d in ((1.0, (48, 16, 16)), (0.75, (64, 22, 22)), (1.5, (96, 32, 32)))]
+    shifts = [np.array([4.0 * i, 0.0, 0.0]) for i in range(3)]
+    array = np.concatenate([b + sft for b, sft in zip(bodies, shifts)])
+    body = np.concatenate([np.full(len(b), i, np.int32) for i, b in enumerate(bodies)])
+    geoms = [(b, None, None) for b in bodies] + [(array, body, np.array(shifts))]
+    omegas = np.sqrt(9.81 * np.linspace(0.05, 4.0, 64))
+    heads = [0.0]
+    sessions = []
+    for v, bd, rc in geoms:
+        s = hb.Session(0)
+        s.set_environment(1025.0, 9.81)
+        s.set_panels(v, bd, rc)
+        s.set_batch(a.batch)
+        sessions.append(s)
+    s0 = sessions[0]
+    s0.set_tuning(s0.TUNE_LU_GEMM_MIXED, a.gemm if a.gemm > 0 else 3)
+    s0.set_tuning(s0.TUNE_LU_DEPTH_MIXED, a.depth if a.depth > 0 else 8)
+    print(f"[bench] batching: panels {[s.n_panels for s in sessions]}, {len(omegas)} frequencies each",
+          file=sys.stderr, flush=True)
+    hb.solve_group(sessions, omegas[:2], heads, precision="mixed")  # warm-up: statics, buffers, graphs
+    t0 = time.perf_counter()
+    group = hb.solve_group(sessions, omegas, heads, precision="mixed")
+    t_group = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sep = [s.solve(omegas, heads, precision="mixed") for s in sessions]
+    t_sep = time.perf_counter() - t0
+    err = max(rel_err(g[k], r[k]) for g, r in zip(group, sep)
+              for k in ("added_mass", "radiation_damping", "excitation_diffraction"))
+    systems = len(omegas) * len(sessions)
+    line = {
+        "metric": "(N, omega) systems solved/s, 4 geometries of uneven panel counts in one group solve",
+        "value": round(systems / t_group, 3), "unit": "systems/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(1e3 * t_group, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16x3+f32+f64", "data": "synthetic: procedural meshes (tests/python/meshes.py layout)",
+        "config": {"workload": "cylinders R=1.0/0.75/1.5 draft 1 (1536/2816/6144 panels) + their array (10496 "
+                               "panels, 3 bodies), infinite depth, 64 kR 0.05-4.0, heading 0, mixed precision",
+                   "panels": [s.n_panels for s in sessions], "systems": systems,
+                   "group_s": round(t_group, 3), "separate_s": round(t_sep, 3),
+                   "group_speedup_vs_separate": round(t_sep / t_group, 3),
+                   "rel_err_group_vs_separate": err},
+    }
+    for s in sessions:
+        s.close()
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def cpu_baseline(w):
@@ -226,6 +285,8 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the frequency sweep "
               f"would shard across GPUs with no exchange, see the design notes)", file=sys.stderr)
         return 2
+    if a.workload == "batching4":
+        return bench_group(a)
     import hydrobem_amd as hb
 
     w = workload(a.workload)

@@ -1,6 +1,7 @@
 // The C ABI of include/hydrobem_amd.h: sessions, geometry, the frequency loop on the GPU.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -996,12 +997,19 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   for (int i = 0; i < count; ++i) sessions[i]->stream = lead->stream;
   hipStream_t st = lead->stream;
   const bool mixed = o.precision == HBG_PRECISION_MIXED;
-  const int systems = count * n_omegas;  // system q: frequency q / count of session q % count
-  const size_t mstride = (size_t)nmax * (nmax + Rmax);
+  // the systems (session, frequency) in decreasing order of N (stable): a batch is padded to its
+  // own largest order only, so systems of very different sizes rarely share a batch
+  std::vector<std::pair<int, int>> order;
+  for (int fi = 0; fi < n_omegas; ++fi)
+    for (int i = 0; i < count; ++i) order.push_back({i, fi});
+  std::stable_sort(order.begin(), order.end(),
+                   [&](const std::pair<int, int>& x, const std::pair<int, int>& y) { return Ns[x.first] > Ns[y.first]; });
+  const int systems = (int)order.size();
+  const size_t mstride_max = (size_t)nmax * (nmax + Rmax);
   const int B = batch_size(lead, nmax, Rmax, systems, mixed);
-  lead->d_M.ensure(mstride * B);
+  lead->d_M.ensure(mstride_max * B);
   if (mixed) {
-    lead->d_M32.ensure(mstride * B);
+    lead->d_M32.ensure(mstride_max * B);
     lead->d_D32.ensure((size_t)nmax * Rmax * B);
     lead->d_X64.ensure((size_t)nmax * Rmax * B);
     lead->d_rpart.ensure((size_t)residual_blocks(nmax) * 4 * B);
@@ -1014,7 +1022,7 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     const size_t D = nrad[i], nAB = D * nrad[i], nX = (size_t)H * D, nHK = (size_t)nrad[i] * H;
     hstride[i] = (size_t)std::max(H, 1) * Ns[i];
     nout[i] = 2 * nAB + 6 * nX + 2 * nHK;
-    si->d_S.ensure((size_t)nmax * Ns[i]);  // leading dimension nmax, as A
+    si->d_S.ensure((size_t)nmax * Ns[i]);  // leading dimension up to nmax, as A
     si->d_phiI.ensure(hstride[i] * B);
     si->d_dphiI.ensure(hstride[i] * B);
     si->d_u.ensure((size_t)Rs[i] * Ns[i]);
@@ -1022,59 +1030,70 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     si->d_out.ensure(nout[i] * B);
   }
   const double logc = o.logarithmic ? 1.0 : 0.0;
-  // system q into slot b: its matrices at order nmax, its right-hand sides at column nmax + c
-  auto build = [&](int q, int b) {
-    const int i = q % count, fi = q / count;
+  // system q into slot b of a batch of order n (leading dimension n) with R right-hand sides: its
+  // matrices in the leading N x N block, its right-hand sides at columns n + c, the rest padding
+  auto build = [&](int q, int b, int n, int R, size_t mstride) {
+    const int i = order[q].first, fi = order[q].second;
     HbgSession* si = sessions[i];
     const int N = Ns[i];
     const double w = omegas[fi], K = w * w / si->grav;
     const double depth = std::isfinite(si->depth) ? si->depth : 0.0;
     double2* Mb = lead->d_M.p + mstride * b;
     float2* M32b = mixed ? lead->d_M32.p + mstride * b : nullptr;
-    const double kw = assemble_frequency(si, K, logc, Mb, o.surrogate != 0, M32b, nmax);
+    const double kw = assemble_frequency(si, K, logc, Mb, o.surrogate != 0, M32b, n);
     launch_boundary(si->d_panels.p, si->d_body.p, si->d_nj.p, N, nrad[i], H, w, kw, si->grav, depth, si->d_heads.p,
                     si->d_phiI.p + hstride[i] * b, si->d_dphiI.p + hstride[i] * b, si->d_u.p, st);
-    launch_rhs(si->d_S.p, N, nmax, si->d_u.p, Rs[i], Mb + (size_t)(nmax - N) * nmax, nmax, si->d_part.p, st);
-    launch_pad_system(Mb, M32b, nmax, N, Rmax, Rs[i], st);
+    launch_rhs(si->d_S.p, N, n, si->d_u.p, Rs[i], Mb + (size_t)(n - N) * n, n, si->d_part.p, st);
+    launch_pad_system(Mb, M32b, n, N, R, Rs[i], st);
   };
   std::vector<int> status(B), iters(B), nsys(B);
   std::vector<double> eta(B), hout;
   for (int q0 = 0; q0 < systems; q0 += B) {
     const int nb = std::min(B, systems - q0);
+    int n = 0, R = 0;
     for (int b = 0; b < nb; ++b) {
-      build(q0 + b, b);
-      nsys[b] = Ns[(q0 + b) % count];
+      n = std::max(n, Ns[order[q0 + b].first]);
+      R = std::max(R, Rs[order[q0 + b].first]);
     }
+    const size_t mstride = (size_t)n * (n + R);
+    for (int b = 0; b < nb; ++b) {
+      build(q0 + b, b, n, R, mstride);
+      nsys[b] = Ns[order[q0 + b].first];
+    }
+    bool uneven = false;
+    for (int b = 0; b < nb; ++b) uneven = uneven || nsys[b] != n;
+    const int* ns = uneven ? nsys.data() : nullptr;
     if (mixed) {
       mixed_solve_batched(lead->d_M.p, lead->d_M32.p, lead->d_X64.p, lead->d_D32.p, lead->d_rpart.p, lead->d_rpart2.p,
-                          lead->d_rscale.p, nb, nmax, Rmax, true, MIXED_TOL, MIXED_MAX_ITER, st, nullptr,
-                          status.data(), iters.data(), eta.data(), nsys.data());
+                          lead->d_rscale.p, nb, n, R, true, MIXED_TOL, MIXED_MAX_ITER, st, nullptr, status.data(),
+                          iters.data(), eta.data(), ns);
       for (int b = 0; b < nb; ++b)
         if (status[b] < 0) {  // the plain FP64 path for this system
-          build(q0 + b, b);
-          lu_solve_batched(lead->d_M.p + mstride * b, mstride, 1, nmax, nmax, Rmax, st, nullptr, &status[b], &nsys[b]);
+          build(q0 + b, b, n, R, mstride);
+          lu_solve_batched(lead->d_M.p + mstride * b, mstride, 1, n, n, R, st, nullptr, &status[b],
+                           uneven ? &nsys[b] : nullptr);
         }
     } else {
-      lu_solve_batched(lead->d_M.p, mstride, nb, nmax, nmax, Rmax, st, nullptr, status.data(), nsys.data());
+      lu_solve_batched(lead->d_M.p, mstride, nb, n, n, R, st, nullptr, status.data(), ns);
     }
     for (int b = 0; b < nb; ++b) {
       if (status[b] < 0) {  // a panel needs interchanges: rebuild the system, pivoted path
-        build(q0 + b, b);
-        status[b] = lu_solve_pivoted(lead->d_M.p + mstride * b, nmax, nmax, Rmax, st, nullptr);
+        build(q0 + b, b, n, R, mstride);
+        status[b] = lu_solve_pivoted(lead->d_M.p + mstride * b, n, n, R, st, nullptr);
       }
       if (status[b] > 0)
         throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(status[b]) + ")"};
     }
     for (int b = 0; b < nb; ++b) {
-      const int i = (q0 + b) % count, fi = (q0 + b) / count;
+      const int i = order[q0 + b].first, fi = order[q0 + b].second;
       HbgSession* si = sessions[i];
       launch_forces(si->d_panels.p, si->d_body.p, si->d_nj.p, Ns[i], nrad[i], H, omegas[fi], si->rho,
-                    lead->d_M.p + mstride * b + (size_t)(nmax - Ns[i]) * nmax, nmax, si->d_phiI.p + hstride[i] * b,
+                    lead->d_M.p + mstride * b + (size_t)(n - Ns[i]) * n, n, si->d_phiI.p + hstride[i] * b,
                     si->d_dphiI.p + hstride[i] * b, si->d_out.p + nout[i] * b, st);
     }
     HBG_HIP(hipStreamSynchronize(st));
     for (int b = 0; b < nb; ++b) {
-      const int i = (q0 + b) % count, fi = (q0 + b) / count;
+      const int i = order[q0 + b].first, fi = order[q0 + b].second;
       HbgSession* si = sessions[i];
       hout.resize(nout[i]);
       HBG_HIP(hipMemcpy(hout.data(), si->d_out.p + nout[i] * b, nout[i] * sizeof(double), hipMemcpyDeviceToHost));

@@ -112,11 +112,12 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
  * body counts allowed), at the same frequencies and headings, with hbg_solve's options: the
  * (geometry, frequency) systems share every launch of the batched LU. No counterpart in the
  * reference, which solves each model on its own (the multi-body case of a single geometry is
- * hbg_solve's). A system of N panels goes to the LU at the order of the largest geometry, with
- * the identity on the rows and columns past N (its LU is block-diagonal, its solution unchanged)
- * and the GEMM tiles of that identity block skipped. Outputs: arrays of count pointers (arrays
- * or pointers may be NULL), entry i laid out as hbg_solve's for session i. The sessions' streams
- * are used in turn; a session must not be in use elsewhere during the call. */
+ * hbg_solve's). The systems are taken in decreasing order of N, in batches; a system of N panels
+ * goes to the LU at the largest order of its batch, with the identity on the rows and columns
+ * past N (its LU is block-diagonal, its solution unchanged) and the GEMM tiles of that identity
+ * block skipped. Outputs: arrays of count pointers (arrays or pointers may be NULL), entry i laid
+ * out as hbg_solve's for session i. All work runs on the first session's stream; no session may
+ * be in use elsewhere during the call. */
 HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const double* omegas, int32_t n_omegas,
                           const double* headings_deg, int32_t n_headings, const HbgOptions* options,
                           double* const* added_mass, double* const* damping, double* const* excitation,
