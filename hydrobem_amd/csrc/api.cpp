@@ -291,7 +291,9 @@ using namespace hbg;
 
 struct HbgSession {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;     // assembly, boundary values, S U; everything outside a solve
+  hipStream_t lu_stream = nullptr;  // a solve's LU, refinement and forces, beside the next batch's assembly
+  hipEvent_t assembled[2] = {};     // the assembly of the batch in each half of the buffers is done
   double rho = 1000.0, grav = 9.80665, depth = INFINITY;
   int N = 0, nb = 0;
   std::vector<PanelD> panels;
@@ -444,9 +446,10 @@ constexpr int MIXED_MAX_ITER = 10;
 
 // Frequencies in flight: the session's setting, else as many as a quarter of the free device
 // memory holds (each needs its N x (N + R) system), at most 16, at most the number of frequencies.
-int batch_size(HbgSession* s, int N, int R, int F, bool mixed) {
+int batch_size(HbgSession* s, int N, int R, int F, bool mixed, int copies = 1) {
   size_t per = (size_t)N * (N + R) * sizeof(double2) + (size_t)N * 64 * sizeof(double2) * 2;
   if (mixed) per += (size_t)N * (N + R) * sizeof(float2) + (size_t)N * R * (sizeof(double2) + sizeof(float2));
+  per *= copies;
   int B = s->max_batch;
   if (B <= 0) {
     size_t free_b = 0, total_b = 0;
@@ -494,6 +497,8 @@ HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
   // a blocking stream: ordered with the null stream, so the synchronous copies and fills of the
   // host code (hipMemcpy, hipMemset) complete before the session's kernels read their buffers
   HBG_HIP(hipStreamCreate(&s->stream));
+  HBG_HIP(hipStreamCreate(&s->lu_stream));
+  for (auto& e : s->assembled) HBG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto& e : s->ev) HBG_HIP(hipEventCreate(&e));
   TablesOnDevice& t = tables_for(device);
   s->T.t2d = t.t2d;
@@ -534,8 +539,12 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_pf.release();
   s->d_pcs.release();
   s->d_ff.release();
+  hipStreamSynchronize(s->lu_stream);
   for (auto& e : s->ev)
     if (e) hipEventDestroy(e);
+  for (auto& e : s->assembled)
+    if (e) hipEventDestroy(e);
+  hipStreamDestroy(s->lu_stream);
   hipStreamDestroy(s->stream);
   delete s;
   return HBG_OK;
@@ -811,16 +820,21 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   const size_t mstride = (size_t)N * (N + R);
   const size_t hstride = (size_t)std::max(H, 1) * N;
   const bool mixed = o.precision == HBG_PRECISION_MIXED;
-  int B = batch_size(s, N, R, n_omegas, mixed);
+  const bool prof = s->profiling;
+  // two halves of the batch buffers: the assembly of batch t + 1 (session stream) runs beside the
+  // LU, refinement and forces of batch t (LU stream); with stage timing one stream, one half
+  const int halves = prof ? 1 : 2;
+  hipStream_t sa = s->stream, sb = prof ? s->stream : s->lu_stream;
+  const int B = batch_size(s, N, R, n_omegas, mixed, halves);
   s->d_S.ensure((size_t)N * N);
-  s->d_M.ensure(mstride * B);
-  s->d_phiI.ensure(hstride * B);
-  s->d_dphiI.ensure(hstride * B);
+  s->d_M.ensure(mstride * B * halves);
+  s->d_phiI.ensure(hstride * B * halves);
+  s->d_dphiI.ensure(hstride * B * halves);
   s->d_u.ensure((size_t)R * N);
   s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
-  s->d_out.ensure(nout * B);
+  s->d_out.ensure(nout * B * halves);
   if (mixed) {
-    s->d_M32.ensure(mstride * B);
+    s->d_M32.ensure(mstride * B * halves);
     s->d_D32.ensure((size_t)N * R * B);
     s->d_X64.ensure((size_t)N * R * B);
     s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * B);
@@ -830,7 +844,6 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   std::vector<double> hout(nout * B);
   std::vector<int> iters(B);
   std::vector<double> eta(B);
-  const bool prof = s->profiling;
   if (prof && s->bev.size() < (size_t)(3 * B + 3)) {
     for (size_t i = s->bev.size(); i < (size_t)(3 * B + 3); ++i) {
       hipEvent_t e;
@@ -840,18 +853,38 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   }
   const double logc = o.logarithmic ? 1.0 : 0.0;
   const double depth = std::isfinite(s->depth) ? s->depth : 0.0;
-  // assembly, incident wave and S U of frequency fi into slot b
-  auto build = [&](int fi, int b) {
+  // assembly, incident wave and S U of frequency fi into slot b of the buffers' half h (session
+  // stream); the slot's place in the buffers is slot(h, b)
+  auto slot = [&](int h, int b) { return (size_t)h * B + b; };
+  auto build = [&](int fi, int b, int h) {
     const double w = omegas[fi], K = w * w / s->grav;
-    double2* Mb = s->d_M.p + mstride * b;
-    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], s->stream));
+    double2* Mb = s->d_M.p + mstride * slot(h, b);
+    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], sa));
     const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0,
-                                         mixed ? s->d_M32.p + mstride * b : nullptr);
+                                         mixed ? s->d_M32.p + mstride * slot(h, b) : nullptr);
     launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
-                    s->d_phiI.p + hstride * b, s->d_dphiI.p + hstride * b, s->d_u.p, s->stream);
-    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], s->stream));
-    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, s->stream);
-    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 2], s->stream));
+                    s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b), s->d_u.p, sa);
+    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], sa));
+    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, sa);
+    if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 2], sa));
+  };
+  // a rebuilt system of the LU stream's batch: the LU stream waits for it
+  auto rebuild = [&](int fi, int b, int h) {
+    build(fi, b, h);
+    if (sb != sa) {
+      HBG_HIP(hipEventRecord(s->assembled[h], sa));
+      HBG_HIP(hipStreamWaitEvent(sb, s->assembled[h], 0));
+    }
+  };
+  const int nbatches = (n_omegas + B - 1) / B;
+  auto enqueue_batch = [&](int t) {
+    const int f0 = t * B, nb = std::min(B, n_omegas - f0), h = t % halves;
+    for (int b = 0; b < nb; ++b) {
+      if (progress && progress(user, HBG_STAGE_ASSEMBLY, f0 + b, n_omegas) == 0)
+        throw Error{HBG_ERR_CANCELLED, "cancelled"};
+      build(f0 + b, b, h);
+    }
+    if (sb != sa) HBG_HIP(hipEventRecord(s->assembled[h], sa));
   };
   auto stage_ms = [&](hipEvent_t a0, hipEvent_t a1) {
     float ms = 0.0f;
@@ -860,13 +893,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     return (double)ms;
   };
   std::vector<int> status(B);
-  for (int f0 = 0; f0 < n_omegas; f0 += B) {
-    const int nb = std::min(B, n_omegas - f0);
-    for (int b = 0; b < nb; ++b) {
-      if (progress && progress(user, HBG_STAGE_ASSEMBLY, f0 + b, n_omegas) == 0)
-        throw Error{HBG_ERR_CANCELLED, "cancelled"};
-      build(f0 + b, b);
-    }
+  if (nbatches > 0) enqueue_batch(0);
+  for (int t = 0; t < nbatches; ++t) {
+    const int f0 = t * B, nb = std::min(B, n_omegas - f0), h = t % halves;
+    double2* Mh = s->d_M.p + mstride * slot(h, 0);
+    // the next batch's assembly goes ahead on the session stream (its half was last used by
+    // batch t - 1, whose results the host has already copied)
+    if (halves > 1 && t + 1 < nbatches) enqueue_batch(t + 1);
+    if (sb != sa) HBG_HIP(hipStreamWaitEvent(sb, s->assembled[h], 0));
     if (prof)
       for (int b = 0; b < nb; ++b) {
         s->prof[1] += stage_ms(s->bev[3 * b], s->bev[3 * b + 1]);
@@ -876,13 +910,13 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       if (progress && progress(user, HBG_STAGE_LU, f0 + b, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     LuStats lus;
     if (mixed) {
-      mixed_solve_batched(s->d_M.p, s->d_M32.p, s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p, s->d_rscale.p, nb, N, R, true,
-                          MIXED_TOL, MIXED_MAX_ITER, s->stream, prof ? &lus : nullptr, status.data(), iters.data(),
-                          eta.data());
+      mixed_solve_batched(Mh, s->d_M32.p + mstride * slot(h, 0), s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p,
+                          s->d_rscale.p, nb, N, R, true, MIXED_TOL, MIXED_MAX_ITER, sb, prof ? &lus : nullptr,
+                          status.data(), iters.data(), eta.data());
       for (int b = 0; b < nb; ++b) {
         if (status[b] < 0) {  // the plain FP64 path for this frequency
-          build(f0 + b, b);
-          lu_solve_batched(s->d_M.p + mstride * b, mstride, 1, N, N, R, s->stream, prof ? &lus : nullptr, &status[b]);
+          rebuild(f0 + b, b, h);
+          lu_solve_batched(Mh + mstride * b, mstride, 1, N, N, R, sb, prof ? &lus : nullptr, &status[b]);
           if (prof) s->prof[15] += 1.0;
         } else if (prof) {
           s->prof[13] += iters[b];
@@ -890,24 +924,27 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
         }
       }
     } else {
-      lu_solve_batched(s->d_M.p, mstride, nb, N, N, R, s->stream, prof ? &lus : nullptr, status.data());
+      lu_solve_batched(Mh, mstride, nb, N, N, R, sb, prof ? &lus : nullptr, status.data());
     }
     for (int b = 0; b < nb; ++b) {
       if (status[b] < 0) {  // a panel needs interchanges: rebuild the system, pivoted path
-        build(f0 + b, b);
-        status[b] = lu_solve_pivoted(s->d_M.p + mstride * b, N, N, R, s->stream, prof ? &lus : nullptr);
+        rebuild(f0 + b, b, h);
+        status[b] = lu_solve_pivoted(Mh + mstride * b, N, N, R, sb, prof ? &lus : nullptr);
       }
       if (status[b] > 0)
         throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(status[b]) + ")"};
     }
     const int e0 = 3 * B, e1 = 3 * B + 1;
-    if (prof) HBG_HIP(hipEventRecord(s->bev[e0], s->stream));
+    if (prof) HBG_HIP(hipEventRecord(s->bev[e0], sb));
     for (int b = 0; b < nb; ++b)
-      launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, omegas[f0 + b], s->rho, s->d_M.p + mstride * b,
-                    N, s->d_phiI.p + hstride * b, s->d_dphiI.p + hstride * b, s->d_out.p + nout * b, s->stream);
-    if (prof) HBG_HIP(hipEventRecord(s->bev[e1], s->stream));
-    HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p, nb * nout * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    HBG_HIP(hipStreamSynchronize(s->stream));
+      launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, omegas[f0 + b], s->rho, Mh + mstride * b, N,
+                    s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
+                    s->d_out.p + nout * slot(h, b), sb);
+    if (prof) HBG_HIP(hipEventRecord(s->bev[e1], sb));
+    HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p + nout * slot(h, 0), nb * nout * sizeof(double),
+                           hipMemcpyDeviceToHost, sb));
+    HBG_HIP(hipStreamSynchronize(sb));
+    if (halves == 1 && t + 1 < nbatches) enqueue_batch(t + 1);
     if (prof) {
       s->prof[3] += lus.total_ms;
       s->prof[4] += lus.gemm_ms;
