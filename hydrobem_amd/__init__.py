@@ -202,7 +202,7 @@ class Session:
                     "mixed_fp64_fallbacks")
 
     def set_batch(self, max_batch: int):
-        """Frequencies in flight in solve (0: as many as fit, at most 16)."""
+        """Frequencies in flight in solve (0: as many as fit, at most 48)."""
         _check(library().hbg_set_batch(self._h, int(max_batch)))
 
     def motions(self, result, mass, stiffness, external_damping=None) -> np.ndarray:
@@ -275,9 +275,9 @@ class Session:
     def autotune(self, n, nrhs, precision="fp64") -> list:
         """Time the batched LU variants on synthetic systems and keep the fastest (hbg_autotune);
         returns the rows (variant, depth, batch, ms per system)."""
-        rep = np.zeros((64, 4))
+        rep = np.zeros((128, 4))
         rows = ctypes.c_int32(0)
-        _check(library().hbg_autotune(self._h, int(n), int(nrhs), PRECISIONS[precision], _p(rep), 64,
+        _check(library().hbg_autotune(self._h, int(n), int(nrhs), PRECISIONS[precision], _p(rep), len(rep),
                                       ctypes.byref(rows)))
         return [tuple(r) for r in rep[:rows.value]]
 

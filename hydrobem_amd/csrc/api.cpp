@@ -314,7 +314,7 @@ struct HbgSession {
   NnDevice nn;          // the learned surrogate of the finite-depth evanescent remainder
   DevBuf<int2> d_nnq;   // pairs left to the FP64 evaluator by the surrogate's assembly
   DevBuf<unsigned> d_nnc;
-  int max_batch = 0;  // frequencies in flight (0: as many as fit, at most 16)
+  int max_batch = 0;  // frequencies in flight (0: as many as fit, at most 48)
   std::vector<hipEvent_t> bev;
   bool statics_valid = false;
   int st_logint = -1;
@@ -444,17 +444,23 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
 constexpr double MIXED_TOL = 1e-12;  // normwise backward error that ends the refinement
 constexpr int MIXED_MAX_ITER = 10;
 
-// Frequencies in flight: the session's setting, else as many as a quarter of the free device
-// memory holds (each needs its N x (N + R) system), at most 16, at most the number of frequencies.
-int batch_size(HbgSession* s, int N, int R, int F, bool mixed, int copies = 1) {
+// Device bytes per frequency in flight: its N x (N + R) system (and its complex64 copy), the
+// LU's per-panel inverses, the refinement's vectors.
+size_t bytes_per_frequency(int N, int R, bool mixed) {
   size_t per = (size_t)N * (N + R) * sizeof(double2) + (size_t)N * 64 * sizeof(double2) * 2;
   if (mixed) per += (size_t)N * (N + R) * sizeof(float2) + (size_t)N * R * (sizeof(double2) + sizeof(float2));
-  per *= copies;
+  return per;
+}
+
+// Frequencies in flight: the session's setting, else as many as half the free device memory
+// holds (copies of the batch buffers each), at most 48, at most the number of frequencies.
+int batch_size(HbgSession* s, int N, int R, int F, bool mixed, int copies = 1) {
+  const size_t per = bytes_per_frequency(N, R, mixed) * copies;
   int B = s->max_batch;
   if (B <= 0) {
     size_t free_b = 0, total_b = 0;
     HBG_HIP(hipMemGetInfo(&free_b, &total_b));
-    B = (int)std::min<size_t>(16, std::max<size_t>(1, free_b / 4 / per));
+    B = (int)std::min<size_t>(48, std::max<size_t>(1, free_b / 2 / per));
   }
   return std::max(1, std::min(B, std::max(F, 1)));
 }
@@ -822,10 +828,16 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   const bool mixed = o.precision == HBG_PRECISION_MIXED;
   const bool prof = s->profiling;
   // two halves of the batch buffers: the assembly of batch t + 1 (session stream) runs beside the
-  // LU, refinement and forces of batch t (LU stream); with stage timing one stream, one half
-  const int halves = prof ? 1 : 2;
-  hipStream_t sa = s->stream, sb = prof ? s->stream : s->lu_stream;
-  const int B = batch_size(s, N, R, n_omegas, mixed, halves);
+  // LU, refinement and forces of batch t (LU stream). One half (and one stream) when one batch
+  // takes every frequency, or with stage timing. The batch size is balanced over the batches.
+  int B = batch_size(s, N, R, n_omegas, mixed, 1);
+  const int halves = (prof || B >= n_omegas) ? 1 : 2;
+  if (halves == 2) B = batch_size(s, N, R, n_omegas, mixed, 2);
+  if (B > 0 && n_omegas > 0) {
+    const int nbt = (n_omegas + B - 1) / B;
+    B = (n_omegas + nbt - 1) / nbt;
+  }
+  hipStream_t sa = s->stream, sb = halves == 1 ? s->stream : s->lu_stream;
   s->d_S.ensure((size_t)N * N);
   s->d_M.ensure(mstride * B * halves);
   s->d_phiI.ensure(hstride * B * halves);
@@ -1405,7 +1417,7 @@ HbgStatus hbg_autotune(HbgSession* s, int32_t n, int32_t nrhs, int32_t precision
   size_t free_b = 0, total_b = 0;
   HBG_HIP(hipMemGetInfo(&free_b, &total_b));
   int r = 0;
-  const int b = lu_autotune(precision == HBG_PRECISION_FP64 ? 1 : 0, n, nrhs, 16, free_b / 4, s->stream, report,
+  const int b = lu_autotune(precision == HBG_PRECISION_FP64 ? 1 : 0, n, nrhs, 48, free_b / 3, s->stream, report,
                             report ? max_rows : 0, &r);
   if (rows) *rows = r;
   s->max_batch = b;

@@ -1183,7 +1183,7 @@ __global__ void synth_kernel(typename Cplx<T>::type* M, size_t count, int n, int
 
 // Time the batched LU on synthetic systems (n, nrhs) for every GEMM tile variant (1, 2; complex64
 // also 3, 4, the split-FP16 kernels), block depth
-// (2, 4, 8) and batch (candidates up to max_batch that fit in mem_bytes) and keep the fastest
+// (2, 4, 8) and batch (4, 8, 16, 32, 48: those up to max_batch that fit in mem_bytes) and keep the fastest
 // variant and depth; returns the fastest batch. rows of report: variant, depth, batch, ms per
 // system (up to max_rows rows).
 template <typename T>
@@ -1193,7 +1193,7 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   const size_t ms = (size_t)n * (n + nrhs);
   int bmax = (int)std::min<size_t>((size_t)std::max(max_batch, 1), std::max<size_t>(1, mem_bytes / (ms * sizeof(C2))));
   std::vector<int> batches;
-  for (int b : {2, 4, 8, 16})
+  for (int b : {4, 8, 16, 32, 48})
     if (b <= bmax) batches.push_back(b);
   if (batches.empty()) batches.push_back(1);
   C2* M = nullptr;

@@ -167,8 +167,8 @@ HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double
 
 /* Frequencies in flight in hbg_solve (no counterpart in the reference, whose frequency pools run
  * one CPU solve per worker, parallel.rs:63-89): each holds its N x (N + R) system on the device
- * and their LUs advance panel by panel in the same launches. 0 (the default): as many as a quarter
- * of the free device memory holds, at most 16. */
+ * and their LUs advance panel by panel in the same launches. 0 (the default): as many as half the
+ * free device memory holds, at most 48, balanced over the batches of a solve. */
 HbgStatus hbg_set_batch(HbgSession* s, int32_t max_batch);
 
 /* ---- the learned surrogate (no counterpart in the reference; it stands beside the finite-depth
@@ -227,9 +227,10 @@ enum {
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
  * systems of order n with nrhs right-hand sides in the given precision for every GEMM tile variant
- * (1, 2; mixed precision also 3, 4), block depth (2, 4, 8) and batch (2, 4, 8, 16 that fit in a quarter of the free memory),
- * then keeps the fastest variant and depth (HBG_TUNE_LU_GEMM, HBG_TUNE_LU_DEPTH) and sets the
- * session's batch. report: up to max_rows rows of (variant, depth, batch, ms per system), rows the
+ * (1, 2; mixed precision also 3, 4), block depth (2, 4, 8) and batch (4, 8, 16, 32, 48 that fit in
+ * a third of the free memory), then keeps the fastest variant and depth (HBG_TUNE_LU_GEMM,
+ * HBG_TUNE_LU_DEPTH; for the mixed precision HBG_TUNE_LU_GEMM_MIXED, HBG_TUNE_LU_DEPTH_MIXED) and
+ * sets the session's batch. report: up to max_rows rows of (variant, depth, batch, ms per system), rows the
  * number written. */
 HbgStatus hbg_autotune(HbgSession* s, int32_t n, int32_t nrhs, int32_t precision, double* report, int32_t max_rows,
                        int32_t* rows);
