@@ -207,8 +207,8 @@ __global__ void __launch_bounds__(256) gemm_mfma_kernel(typename Cplx<T>::type* 
 // sides scaled by powers of two to |b| <= 1 (the matrix of the low-order method is O(1)), and
 // checks every solution by its FP64 residual.
 //
-// The operands are split once, when their panel is final (split_panel_kernel), into buffers laid
-// out for the GEMM: L rows as [k group of 8][part][row][8 halves], U columns as
+// The operands are split once, when their panel is final (by the kernel that computes the panel's
+// L21 and U12, beside its FP32 result), into buffers laid out for the GEMM: L rows as [k group of 8][part][row][8 halves], U columns as
 // [k group][part][column][8 halves], part = re hi, re lo, im hi, im lo, with `pad` rows or
 // columns per (k group, part) and the k groups counted from the first panel of the current block.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
@@ -235,32 +235,6 @@ __device__ __forceinline__ void split8(const float2* v, h8* part) {
     part[2][q] = b;
     part[3][q] = (_Float16)(v[q].y - (float)b);
   }
-}
-
-// The final L21 (rows k0 + 64 .. n of the panel's 64 columns) and U12 (the panel's 64 rows,
-// columns k0 + 64 .. ncols, the right-hand sides included) of the panel at k0 into the split
-// buffers; thread = (row or column, k group), consecutive threads along rows (columns);
-// grid (ceil(8 (n + ncols - 2 k0 - 128) / 256), 1, batch).
-__global__ void __launch_bounds__(256) split_panel_kernel(const float2* __restrict__ M, int lda, int n, int ncols,
-                                                          int k0, SplitOps so, size_t mstride) {
-  const int ml = n - k0 - 64, mu = ncols - k0 - 64;
-  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (ml <= 0 || e >= 8 * (size_t)(ml + mu)) return;
-  M += blockIdx.z * mstride;
-  const bool isl = e < 8 * (size_t)ml;
-  const int m = isl ? ml : mu;
-  const int idx = (int)(isl ? e : e - 8 * (size_t)ml);
-  const int r = k0 + 64 + idx % m, kg = idx / m;
-  float2 v[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q)
-    v[q] = isl ? M[(size_t)(k0 + 8 * kg + q) * lda + r] : M[(size_t)r * lda + k0 + 8 * kg + q];
-  h8 part[4];
-  split8(v, part);
-  h8* dst = (isl ? so.ls + blockIdx.z * so.lstride : so.us + blockIdx.z * so.ustride) +
-            (size_t)((k0 - so.K0) / 8 + kg) * 4 * so.pad + r;
-#pragma unroll
-  for (int p = 0; p < 4; ++p) dst[(size_t)p * so.pad] = part[p];
 }
 
 // C[r0 + i][c0 + j] -= sum_k L[r0 + i][k] U[k][c0 + j] over the k groups kg0 .. kg0 + kb / 8 of

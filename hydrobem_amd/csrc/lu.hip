@@ -458,15 +458,22 @@ __global__ void __launch_bounds__(256) diag_kernel(typename Cplx<T>::type* __res
 //     elimination below the pivot (else the panel's flag);
 //   blocks [rblocks, gridDim.x): U12 = Linv A12 on AB columns each (< ncols).
 // gate: run only if the flag is set. The inverse is staged in LDS with all of a thread's loads in
-// flight at once.
+// flight at once. A thread's outputs are 8 consecutive columns of L21 or 8 consecutive rows of
+// U12: with split set (complex64, a full panel), they also go into the split-FP16 operands of the
+// trailing updates (split8: L21 row r, columns 8 g .. 8 g + 7 at k group g of the row; U12 column
+// c, rows 8 g .. 8 g + 7 at k group g of the column), so the panel is not read again.
 constexpr int AB = 32;
 template <typename T>
 __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __restrict__ M, int lda, int lim, int ncols, int k0,
                                                     int kb, int rblocks, const typename Cplx<T>::type* __restrict__ Uinv,
                                                     const typename Cplx<T>::type* __restrict__ Linv,
                                                     const typename Cplx<T>::type* __restrict__ udiag, int* __restrict__ flag,
-                                                    const int* __restrict__ gate, BatchStrides bs) {
+                                                    const int* __restrict__ gate, BatchStrides bs, SplitOps so, int split) {
   typedef typename Cplx<T>::type C2;
+  if constexpr (std::is_same<T, float>::value) {
+    so.ls += blockIdx.z * so.lstride;
+    so.us += blockIdx.z * so.ustride;
+  }
   {
     const int z = blockIdx.z;
     M += z * bs.m;
@@ -510,8 +517,8 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
       }
     }
     __syncthreads();
-    constexpr int CG = 256 / AB;  // column groups
-    const int i = tid % AB, cq = tid / AB;
+    static_assert(VR == 8 && VR * (256 / AB) == NB, "8 consecutive columns per thread");
+    const int i = tid % AB, cq = tid / AB;  // row of the block, columns VR cq .. VR cq + 7
     C2 acc[VR];
 #pragma unroll
     for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
@@ -519,7 +526,7 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
       const C2 x = Xs[k][i];
 #pragma unroll
       for (int v = 0; v < VR; ++v) {
-        const C2 u = Ts[k][cq + CG * v];
+        const C2 u = Ts[k][VR * cq + v];
         acc[v].x = fma(x.x, u.x, fma(-x.y, u.y, acc[v].x));
         acc[v].y = fma(x.x, u.y, fma(x.y, u.x, acc[v].y));
       }
@@ -528,7 +535,7 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     if (base + i < lim) {
 #pragma unroll
       for (int v = 0; v < VR; ++v) {
-        const int c = cq + CG * v;
+        const int c = VR * cq + v;
         if (c >= kb) continue;
         M[(size_t)(k0 + c) * lda + base + i] = acc[v];
         if (flag) {
@@ -536,6 +543,14 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
           if (mag1(cmul(acc[v], d)) > mag1(d)) fail = true;
         }
       }
+      if constexpr (std::is_same<T, float>::value)
+        if (split) {
+          h8 part[4];
+          split8(acc, part);
+          h8* dst = so.ls + (size_t)((k0 - so.K0) / 8 + cq) * 4 * so.pad + base + i;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) dst[(size_t)q * so.pad] = part[q];
+        }
     }
     if (fail) s_fail = 1;
     __syncthreads();
@@ -556,25 +571,32 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
       }
     }
     __syncthreads();
-    const int r = tid & 63, cq = tid >> 6;
+    const int cl = tid % AB, rg = tid / AB;  // column of the block, rows VR rg .. VR rg + 7
     C2 acc[VR];
 #pragma unroll
     for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
     for (int k = 0; k < NB; ++k) {
-      const C2 l = Ts[r][k];
+      const C2 x = Xs[k][cl];
 #pragma unroll
       for (int v = 0; v < VR; ++v) {
-        const C2 x = Xs[k][cq + 4 * v];
+        const C2 l = Ts[VR * rg + v][k];
         acc[v].x = fma(l.x, x.x, fma(-l.y, x.y, acc[v].x));
         acc[v].y = fma(l.x, x.y, fma(l.y, x.x, acc[v].y));
       }
     }
-    if (r < kb) {
+    const int c = base + cl;
+    if (c < ncols) {
 #pragma unroll
-      for (int v = 0; v < VR; ++v) {
-        const int c = base + cq + 4 * v;
-        if (c < ncols) M[(size_t)c * lda + k0 + r] = acc[v];
-      }
+      for (int v = 0; v < VR; ++v)
+        if (VR * rg + v < kb) M[(size_t)c * lda + k0 + VR * rg + v] = acc[v];
+      if constexpr (std::is_same<T, float>::value)
+        if (split) {
+          h8 part[4];
+          split8(acc, part);
+          h8* dst = so.us + (size_t)((k0 - so.K0) / 8 + rg) * 4 * so.pad + c;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) dst[(size_t)q * so.pad] = part[q];
+        }
     }
   }
 }
@@ -968,11 +990,8 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
       if (rb + cb > 0)
         apply_kernel<T><<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, ws.uinv + po,
-                                                                     ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs);
-      if constexpr (std::is_same<T, float>::value)
-        if (split && k0 + NB < n)
-          split_panel_kernel<<<dim3((unsigned)((8 * (size_t)(n + ncols - 2 * k0 - 2 * NB) + 255) / 256), 1, batch),
-                               256, 0, stream>>>(M, lda, n, ncols, k0, so, bs.m);
+                                                                     ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs,
+                                                                     so, split && k0 + NB < n ? 1 : 0);
     }
     gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, gb, sop, stream, gt);
   }
@@ -999,7 +1018,7 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
     const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
     if (rb > 0)
       apply_kernel<double><<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, linv, ws.udiag, flag, nullptr,
-                                                   bs);
+                                                   bs, SplitOps{}, 0);
     panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 1, flag);
     fill_u32<<<(unsigned)((slots + 255) / 256), 256, 0, stream>>>(ws.w.ctr, slots, 0u);
     tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w, flag);
@@ -1009,11 +1028,11 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
                                                bs);
     if (rb > 0)
       apply_kernel<double><<<rb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, uinv, linv, ws.udiag, nullptr, flag,
-                                                   bs);
+                                                   bs, SplitOps{}, 0);
     panel_finish_kernel<<<1, 64, 0, stream>>>(flag, ws.zp_fast + p, ws.w.zero_pivot);
     if (cb > 0)
       apply_kernel<double><<<cb, 256, 0, stream>>>(M, lda, n, ncols, k0, kb, 0, uinv, linv, ws.udiag, nullptr,
-                                                   nullptr, bs);
+                                                   nullptr, bs, SplitOps{}, 0);
     gemm_sub<double>(M, lda, k0 + kb, n - k0 - kb, k0 + kb, ncols - k0 - kb, k0, kb, GemmBatch{0, 1, nullptr, n},
                      nullptr, stream, gt);
   }
