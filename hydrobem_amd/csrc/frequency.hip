@@ -295,40 +295,54 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
 constexpr int MAXRHS = 32;
 constexpr int RHS_CHUNK = 256;
 // RM: right-hand sides held in registers (R <= RM); the columns of S go 8 at a time so that eight
-// loads are in flight per lane (the kernel streams S once; its rate is that of the loads in flight)
+// loads are in flight per lane (the kernel streams S once; its rate is that of the loads in flight),
+// and the wavefront-uniform u of 64 columns at a time is staged in LDS and read as broadcasts.
+constexpr int RHS_SUB = 64;
 template <int RM>
 __global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restrict__ S, int N, int lds,
                                                           const double2* __restrict__ u, int R,
                                                           double2* __restrict__ part) {
+  __shared__ double2 us[RHS_SUB][RM];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int chunk = blockIdx.y;
-  if (k >= N) return;
+  const bool row = k < N;
+  const int kk = row ? k : N - 1;
   double2 acc[RM];
 #pragma unroll
   for (int c = 0; c < RM; ++c) acc[c] = make_double2(0.0, 0.0);
-  auto column = [&](double2 s, int l) {
+  auto column = [&](double2 s, int j) {
 #pragma unroll
     for (int c = 0; c < RM; ++c) {
       if (c < R) {
-        const double2 v = u[(size_t)c * N + l];
+        const double2 v = us[j][c];
         acc[c].x -= s.x * v.x - s.y * v.y;
         acc[c].y -= s.x * v.y + s.y * v.x;
       }
     }
   };
   const int l0 = chunk * RHS_CHUNK, l1 = min(N, l0 + RHS_CHUNK);
-  int l = l0;
-  for (; l + 8 <= l1; l += 8) {
-    double2 sv[8];
+  for (int s0 = l0; s0 < l1; s0 += RHS_SUB) {
+    const int ns = min(RHS_SUB, l1 - s0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < RHS_SUB * R; e += blockDim.x) {
+      const int c = e / RHS_SUB, j = e % RHS_SUB;
+      if (j < ns) us[j][c] = u[(size_t)c * N + s0 + j];
+    }
+    __syncthreads();
+    int j = 0;
+    for (; j + 8 <= ns; j += 8) {
+      double2 sv[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) sv[q] = S[(size_t)(l + q) * lds + k];
+      for (int q = 0; q < 8; ++q) sv[q] = S[(size_t)(s0 + j + q) * lds + kk];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) column(sv[q], l + q);
+      for (int q = 0; q < 8; ++q) column(sv[q], j + q);
+    }
+    for (; j < ns; ++j) column(S[(size_t)(s0 + j) * lds + kk], j);
   }
-  for (; l < l1; ++l) column(S[(size_t)l * lds + k], l);
+  if (row)
 #pragma unroll
-  for (int c = 0; c < RM; ++c)
-    if (c < R) part[((size_t)chunk * R + c) * N + k] = acc[c];
+    for (int c = 0; c < RM; ++c)
+      if (c < R) part[((size_t)chunk * R + c) * N + k] = acc[c];
 }
 
 __global__ void rhs_reduce_kernel(const double2* __restrict__ part, int N, int R, int chunks, double2* __restrict__ M,
