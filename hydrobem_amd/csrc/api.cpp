@@ -244,7 +244,7 @@ struct DevBuf {
     n = count;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
   }
@@ -522,8 +522,8 @@ HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
 HbgStatus hbg_session_free(HbgSession* s) {
   if (!s) return HBG_OK;
   GUARD_BEGIN
-  hipSetDevice(s->device);
-  hipStreamSynchronize(s->stream);
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->stream);
   s->d_panels.release();
   s->d_body.release();
   s->d_nj.release();
@@ -545,13 +545,13 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_pf.release();
   s->d_pcs.release();
   s->d_ff.release();
-  hipStreamSynchronize(s->lu_stream);
+  (void)hipStreamSynchronize(s->lu_stream);
   for (auto& e : s->ev)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : s->assembled)
-    if (e) hipEventDestroy(e);
-  hipStreamDestroy(s->lu_stream);
-  hipStreamDestroy(s->stream);
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(s->lu_stream);
+  (void)hipStreamDestroy(s->stream);
   delete s;
   return HBG_OK;
   GUARD_END("hbg_session_free")

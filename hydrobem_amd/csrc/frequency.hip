@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
   __shared__ double2 red[256];
   __shared__ double2 red2[256];
   const int D = nrad;
-  const int nAB = D * nrad, nX = H * D, nHK = nrad * H;
+  const int nAB = D * nrad, nX = H * D;
   const int item = blockIdx.x;
   const int tid = threadIdx.x;
   double2 s = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);

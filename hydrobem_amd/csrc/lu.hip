@@ -740,8 +740,8 @@ struct LuWorkspace {
   // once (rows and columns past the matrix are read as zeros)
   void ensure_split(int n_, int batch_, int kmax) {
     if (ls && split_pad >= n_ + 128 && split_kmax >= kmax && split_batch >= batch_) return;
-    hipFree(ls);
-    hipFree(us);
+    (void)hipFree(ls);
+    (void)hipFree(us);
     ls = us = nullptr;
     split_pad = std::max(split_pad, n_ + 128);
     split_kmax = std::max(split_kmax, kmax);
@@ -754,23 +754,23 @@ struct LuWorkspace {
     ++generation;
   }
   void release() {
-    hipFree(ls);
-    hipFree(us);
-    hipFree(w.cand);
-    hipFree(w.ccount);
-    hipFree(w.ctr);
-    hipFree(w.top);
-    hipFree(w.winners);
-    hipFree(w.zero_pivot);
-    hipFree(ipiv);
-    hipFree(nsys);
-    hipFree(flags);
-    hipFree(zp_fast);
-    hipFree(uinv);
-    hipFree(linv);
-    hipFree(udiag);
-    hipFree(save);
-    hipFree(backup);
+    (void)hipFree(ls);
+    (void)hipFree(us);
+    (void)hipFree(w.cand);
+    (void)hipFree(w.ccount);
+    (void)hipFree(w.ctr);
+    (void)hipFree(w.top);
+    (void)hipFree(w.winners);
+    (void)hipFree(w.zero_pivot);
+    (void)hipFree(ipiv);
+    (void)hipFree(nsys);
+    (void)hipFree(flags);
+    (void)hipFree(zp_fast);
+    (void)hipFree(uinv);
+    (void)hipFree(linv);
+    (void)hipFree(udiag);
+    (void)hipFree(save);
+    (void)hipFree(backup);
     const unsigned long g = generation;
     *this = LuWorkspace{};
     generation = g + 1;
@@ -866,13 +866,13 @@ struct GraphCache {
   }
   void put(const GraphKey& k, hipGraphExec_t e) {
     if (items.size() >= 16) {  // the oldest goes
-      hipGraphExecDestroy(items.front().second);
+      (void)hipGraphExecDestroy(items.front().second);
       items.erase(items.begin());
     }
     items.push_back({k, e});
   }
   void clear() {
-    for (auto& it : items) hipGraphExecDestroy(it.second);
+    for (auto& it : items) (void)hipGraphExecDestroy(it.second);
     items.clear();
   }
 };
@@ -1264,15 +1264,15 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   } catch (...) {
     gv = old_v;
     gd = old_d;
-    hipFree(M);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipFree(M);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     throw;
   }
   gv = bv;
   gd = bd;
   graphs().clear();  // the captures hold the tuning buffer
-  hipFree(M);
+  (void)hipFree(M);
   HBG_HIP(hipEventDestroy(e0));
   HBG_HIP(hipEventDestroy(e1));
   return bb;
@@ -1324,7 +1324,7 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
   ws.ensure(n, 1);
   const size_t whole = (size_t)lda * (n + nrhs);
   if (ws.backup_elems < whole) {
-    hipFree(ws.backup);
+    (void)hipFree(ws.backup);
     ws.backup = nullptr;
     ws.backup_elems = 0;
     HBG_HIP(hipMalloc(&ws.backup, whole * sizeof(double2)));

@@ -257,7 +257,7 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
   }
   bool open = false;
   for (int b = 0; b < batch; ++b) open = open || status[b] == 0;
-  for (int it = 0; open; ++it) {
+  while (open) {
     const dim3 rg(nblk, RES_SPLIT, batch);
     if (R <= 8)
       residual_part_kernel<8><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);

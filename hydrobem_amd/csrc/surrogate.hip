@@ -551,8 +551,8 @@ void nn_pack(const float* prm, std::vector<__bf16>& pk, std::vector<float>& bias
 NnDevice::~NnDevice() { release(); }
 
 void NnDevice::release() {
-  hipFree(pk);
-  hipFree(bias);
+  (void)hipFree(pk);
+  (void)hipFree(bias);
   pk = nullptr;
   bias = nullptr;
   ready = false;
@@ -585,13 +585,13 @@ struct Buf {
   size_t n = 0;
   void ensure(size_t m) {
     if (m <= n) return;
-    hipFree(p);
+    (void)hipFree(p);
     p = nullptr;
     n = 0;
     HBG_HIP(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(T)));
     n = m;
   }
-  ~Buf() { hipFree(p); }
+  ~Buf() { (void)hipFree(p); }
 };
 
 void gemm(int M, int N, int K, const float* A, int ars, int acs, const float* B, int brs, int bcs, float* C, int crs,
