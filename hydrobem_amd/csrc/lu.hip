@@ -81,7 +81,9 @@ __device__ __forceinline__ void better(double& v, int& r, double v2, int r2) {
 // Partial-pivot elimination of the rows held by the workgroup (row `grow` per thread, -1: none)
 // over kb columns: returns the rows chosen at each step in `win` (LDS), `nwin` of them. With
 // `store_top` the chosen rows (multipliers left of the diagonal, U on and right of it) are
-// written to top[step][*]. All threads of the workgroup must call it.
+// written to top[step][*]. All threads of the workgroup must call it. A NaN magnitude counts as
+// the largest (every comparison with NaN is false, so otherwise a step whose candidates are all
+// NaN would choose no row); the NaN then reaches the solution, which the solve reports.
 __device__ void eliminate(double2 (&a)[NB], int grow, int nrows, int kb, int* win, double2* urow, double* redv,
                           int* redr, bool store_top, double2* top, int* zero_pivot, int col0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -93,6 +95,7 @@ __device__ void eliminate(double2 (&a)[NB], int grow, int nrows, int kb, int* wi
     for (int c = 0; c < NB; ++c)
       if (c == j) aj = a[c];
     double v = used ? -1.0 : mag1(aj);
+    if (v != v) v = INFINITY;
     int r = used ? 0x7fffffff : grow;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -217,14 +220,20 @@ __global__ void __launch_bounds__(LEAF) tslu_kernel(const double2* __restrict__ 
 // The pivot interchanges of the panel in LAPACK's ipiv form from its pivot rows, by one
 // wavefront: lane l tracks the position of the panel's original row k0 + l (pos) and the original
 // row now at position k0 + l (at); a pivot row outside the block sits where it started until it
-// is chosen. Gated by the panel's flag.
-__global__ void __launch_bounds__(64) ipiv_kernel(int k0, int kb, const int* __restrict__ winners,
-                                                  int* __restrict__ ipiv, const int* __restrict__ gate) {
+// is chosen. Gated by the panel's flag. A pivot row outside [k0, n) (none is chosen so) is taken
+// as no interchange and reported as a numerical failure at its column, never used as an index.
+__global__ void __launch_bounds__(64) ipiv_kernel(int k0, int kb, int n, const int* __restrict__ winners,
+                                                  int* __restrict__ ipiv, int* __restrict__ zero_pivot,
+                                                  const int* __restrict__ gate) {
   if (*gate == 0) return;
   const int l = threadIdx.x;
   int pos = k0 + l, at = k0 + l;
   for (int i = 0; i < kb; ++i) {
-    const int w = winners[i];
+    int w = winners[i];
+    if (w < k0 || w >= n) {
+      if (l == 0) atomicMin(zero_pivot, k0 + i + 1);
+      w = __shfl(at, i);
+    }
     const bool inside = w < k0 + kb;
     const int p = inside ? __shfl(pos, w - k0) : w;  // current position of row w
     const int o = __shfl(at, i);                        // the original block row at position k0 + i
@@ -1022,7 +1031,7 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
     panel_copy_kernel<<<pe, 256, 0, stream>>>(M, n, lda, k0, kb, ws.save, 1, flag);
     fill_u32<<<(unsigned)((slots + 255) / 256), 256, 0, stream>>>(ws.w.ctr, slots, 0u);
     tslu_kernel<<<(m + LEAF - 1) / LEAF, LEAF, 0, stream>>>(M, n, lda, k0, kb, ws.w, flag);
-    ipiv_kernel<<<1, 64, 0, stream>>>(k0, kb, ws.w.winners, ws.ipiv, flag);
+    ipiv_kernel<<<1, 64, 0, stream>>>(k0, kb, n, ws.w.winners, ws.ipiv, ws.w.zero_pivot, flag);
     swap_kernel<<<(ncols + 255) / 256, 256, 0, stream>>>(M, lda, ncols, k0, kb, ws.ipiv, ws.w.top, flag);
     diag_kernel<double><<<1, 256, 0, stream>>>(M, lda, k0, kb, 1, flag, ws.zp_fast + p, ws.ipiv, uinv, linv, ws.udiag,
                                                bs);

@@ -136,6 +136,28 @@ def test_lu_singular_reports_zero_pivot(session):
     assert e.value.status == 8 and "zero pivot" in e.value.message
 
 
+def test_lu_overflow_to_nan_reports_numerical(session):
+    """Finite entries whose elimination overflows: the panel needs interchanges (|a10| > |a00|),
+    the pivoted elimination turns every remaining row into NaN by its fourth step, and the pivot
+    search must still choose a row (NaN counts as the largest magnitude) so that the solve ends
+    with a numerical failure, not with an interchange taken from no row."""
+    say("case: lu n=64, elimination overflowing to NaN in the tournament")
+    n = 64
+    a = np.eye(n, dtype=complex)
+    a[1:, 0] = 2.0
+    a[1, 1:] = 1e308
+    a[2:, 1:] = -1e308
+    a[np.arange(2, n), np.arange(2, n)] = -1e308
+    with pytest.raises(hb.HbgError) as e:
+        session.lu_solve(a, np.ones((n, 1), complex))
+    assert e.value.status == 8
+    # the device and the workspace serve the next solve
+    rng = np.random.default_rng(3)
+    b = np.eye(n) * 4.0 + 0.1 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    x = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    assert np.abs(session.lu_solve(b, b @ x) - x).max() < 1e-12
+
+
 def test_solve_regression_baseline_and_oracle(session, oracle, cylinder):
     say("case: solve lo_cylinder_drift (256 panels) vs baseline and oracle")
     ref = np.load(GOLDEN / "lo_cylinder_drift.npz", allow_pickle=False)
