@@ -488,7 +488,10 @@ extern "C" {
 
 const char* hbg_last_error_message(void) { return g_msg.c_str(); }
 const char* hbg_last_error_location(void) { return g_loc.c_str(); }
-const char* hbg_version(void) { return "hydrobem_amd 0.1.0 (gfx950, FP64 low-order infinite depth)"; }
+const char* hbg_version(void) {
+  return "hydrobem_amd 0.1.0 (gfx950; low-order frequency-domain solve, infinite and finite depth, FP64 and mixed "
+         "precision)";
+}
 
 HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
   if (!out) return fail(HBG_ERR_ARGUMENT, "out is null", "hbg_session_create");

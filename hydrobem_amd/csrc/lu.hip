@@ -617,22 +617,30 @@ __global__ void panel_finish_kernel(const int* __restrict__ flag, const int* __r
 }
 
 // Block substitutions with the stored inverses of the diagonal blocks, on right-hand sides Y
-// (column c of system z at Y + z * ystride + c * ldy, rows [0, n)), nrhs <= MAXRHS_LU:
-//   subst_block_kernel: Y_b = Tinv_b Y_b on the rows (k0, kb) (Uinv for U X = Y, Linv for L X = Y);
-//   subst_update_kernel: Y[r0 : r0 + nr] -= M[r0 : r0 + nr, k0 : k0 + kb] Y_b, one row per thread
-//     (rows above the block for U, below it for L).
+// (column c of system z at Y + z * ystride + c * ldy, rows [0, n)), nrhs <= MAXRHS_LU per pass.
+
+// One panel of a block substitution in one launch. The panel's block of the solution,
+// Y_b = Tinv_b W_b (Uinv for U X = Y, Linv for L X = Y), is formed by every workgroup from the
+// work copy W of the right-hand sides (its rows [k0, k0 + kb) are not written by this launch) and
+// stored into Y by workgroup 0; each workgroup then updates its rows of W,
+// W[r0 : r0 + nr] -= M[r0 : r0 + nr, k0 : k0 + kb] Y_b, one row per thread (rows above the block
+// for U, below it for L). Grid (max(1, ceil(nr / 256)), 1, batch), 256 threads; nrhs <= MAXRHS_LU.
 template <typename T>
-__global__ void __launch_bounds__(256) subst_block_kernel(const typename Cplx<T>::type* __restrict__ Tinv, size_t tstride,
-                                                          typename Cplx<T>::type* __restrict__ Y, int ldy,
-                                                          size_t ystride, int k0, int kb, int nrhs, int upper) {
+__global__ void __launch_bounds__(256) subst_panel_kernel(const typename Cplx<T>::type* __restrict__ M, int lda,
+                                                          size_t mstride, const typename Cplx<T>::type* __restrict__ Tinv,
+                                                          size_t tstride, typename Cplx<T>::type* __restrict__ W,
+                                                          size_t wstride, int ldw, typename Cplx<T>::type* __restrict__ Y,
+                                                          int ldy, size_t ystride, int k0, int kb, int r0, int nr,
+                                                          int nrhs, int upper) {
   typedef typename Cplx<T>::type C2;
   __shared__ C2 U[NB][NB + 1];
-  __shared__ C2 Ys[NB][MAXRHS_LU + 1];
+  __shared__ C2 Xb[NB][MAXRHS_LU + 1];
+  M += blockIdx.z * mstride;
   Tinv += blockIdx.z * tstride;
+  W += blockIdx.z * wstride;
   Y += blockIdx.z * ystride;
   const int tid = threadIdx.x;
   {
-    // all 16 loads of a thread in flight before the LDS stores
     C2 t[NB * NB / 256];
 #pragma unroll
     for (int j = 0; j < NB * NB / 256; ++j) t[j] = Tinv[tid + 256 * j];
@@ -644,39 +652,37 @@ __global__ void __launch_bounds__(256) subst_block_kernel(const typename Cplx<T>
   }
   for (int e = tid; e < NB * nrhs; e += 256) {
     const int r = e & (NB - 1), c = e >> 6;
-    Ys[r][c] = r < kb ? Y[(size_t)c * ldy + k0 + r] : C2(0, 0);
+    Xb[r][c] = r < kb ? W[(size_t)c * ldw + k0 + r] : C2(0, 0);
   }
   __syncthreads();
-  for (int e = tid; e < NB * nrhs; e += 256) {
-    const int r = e & (NB - 1), c = e >> 6;
-    if (r >= kb) continue;
+  // Y_b in registers (at most MAXRHS_LU / 4 entries per thread), then over W_b in LDS
+  constexpr int PER = NB * MAXRHS_LU / 256;
+  C2 yb[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + 256 * q, r = e & (NB - 1), c = e >> 6;
     C2 s = C2(0, 0);
-    const int lo = upper ? r : 0, hi = upper ? kb : r + 1;
-    for (int k = lo; k < hi; ++k) {
-      const C2 u = U[r][k], y = Ys[k][c];
-      s.x += u.x * y.x - u.y * y.y;
-      s.y += u.x * y.y + u.y * y.x;
+    if (c < nrhs && r < kb) {
+      const int lo = upper ? r : 0, hi = upper ? kb : r + 1;
+      for (int k = lo; k < hi; ++k) {
+        const C2 u = U[r][k], y = Xb[k][c];
+        s.x += u.x * y.x - u.y * y.y;
+        s.y += u.x * y.y + u.y * y.x;
+      }
     }
-    Y[(size_t)c * ldy + k0 + r] = s;
-  }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(64) subst_update_kernel(const typename Cplx<T>::type* __restrict__ M, int lda,
-                                                          size_t mstride, typename Cplx<T>::type* __restrict__ Y,
-                                                          int ldy, size_t ystride, int k0, int kb, int r0, int nr,
-                                                          int nrhs) {
-  typedef typename Cplx<T>::type C2;
-  __shared__ C2 Xb[NB][MAXRHS_LU];
-  M += blockIdx.z * mstride;
-  Y += blockIdx.z * ystride;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < NB * nrhs; e += 64) {
-    const int c = e & (NB - 1), r = e >> 6;
-    Xb[c][r] = c < kb ? Y[(size_t)r * ldy + k0 + c] : C2(0, 0);
+    yb[q] = s;
   }
   __syncthreads();
-  const int i = blockIdx.x * 64 + tid;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + 256 * q, r = e & (NB - 1), c = e >> 6;
+    if (c < nrhs) {
+      Xb[r][c] = yb[q];
+      if (blockIdx.x == 0 && r < kb) Y[(size_t)c * ldy + k0 + r] = yb[q];
+    }
+  }
+  __syncthreads();
+  const int i = blockIdx.x * 256 + tid;
   if (i >= nr) return;
   const int row = r0 + i;
   const C2* col = M + (size_t)k0 * lda + row;
@@ -692,7 +698,6 @@ __global__ void __launch_bounds__(64) subst_update_kernel(const typename Cplx<T>
         acc[q].y = fma(u.x, x.y, fma(u.y, x.x, acc[q].y));
       }
     };
-    // eight loads in flight per lane (the block's column of M is streamed once per 8 right-hand sides)
     int c = 0;
     for (; c + 8 <= kb; c += 8) {
       C2 u[8];
@@ -705,13 +710,23 @@ __global__ void __launch_bounds__(64) subst_update_kernel(const typename Cplx<T>
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       if (q0 + q >= nrhs) break;
-      C2* b = Y + (size_t)(q0 + q) * ldy + row;
+      C2* b = W + (size_t)(q0 + q) * ldw + row;
       C2 v = *b;
       v.x -= acc[q].x;
       v.y -= acc[q].y;
       *b = v;
     }
   }
+}
+
+// W[z][c][r] = Y[z][c][r] for the nrhs columns (the work copy of the block substitutions)
+template <typename C2>
+__global__ void copy_rhs_kernel(const C2* __restrict__ Y, int ldy, size_t ystride, C2* __restrict__ W, int ldw,
+                                size_t wstride, int n, int nrhs) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * nrhs) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  W[blockIdx.z * wstride + (size_t)c * ldw + r] = Y[blockIdx.z * ystride + (size_t)c * ldy + r];
 }
 
 __global__ void fill_u32(unsigned* p, size_t n, unsigned v) {
@@ -738,6 +753,7 @@ struct LuWorkspace {
   C2* udiag = nullptr;       // [batch]
   C2* save = nullptr;        // the panel before the attempt without interchanges (pivoted path)
   C2* backup = nullptr;      // [A | B] of a single solve, restored for the pivoted path
+  C2* subw = nullptr;        // [batch][MAXRHS_LU][n]: work copy of the right-hand sides (substitutions)
   size_t backup_elems = 0;
   // the split FP16 operands of a block's panels (complex64 LU, GEMM variants 3 and 4)
   h8* ls = nullptr;
@@ -780,6 +796,7 @@ struct LuWorkspace {
     (void)hipFree(udiag);
     (void)hipFree(save);
     (void)hipFree(backup);
+    (void)hipFree(subw);
     const unsigned long g = generation;
     *this = LuWorkspace{};
     generation = g + 1;
@@ -819,6 +836,7 @@ struct LuWorkspace {
     HBG_HIP(hipMalloc(&uinv, (size_t)batch_ * panels * NB * NB * sizeof(C2)));
     HBG_HIP(hipMalloc(&linv, (size_t)batch_ * panels * NB * NB * sizeof(C2)));
     HBG_HIP(hipMalloc(&udiag, (size_t)batch_ * NB * sizeof(C2)));
+    HBG_HIP(hipMalloc(&subw, (size_t)batch_ * MAXRHS_LU * n_ * sizeof(C2)));
     n = n_;
     batch = batch_;
   }
@@ -1055,18 +1073,20 @@ void substitute(const typename Cplx<T>::type* M, int n, int lda, size_t mstride,
   const BatchStrides bs = ws.strides();
   const typename Cplx<T>::type* Tinv = upper ? ws.uinv : ws.linv;
   const int panels = (n + NB - 1) / NB;
+  if (n > ws.n || batch > ws.batch) throw Error{6, "substitution beyond the LU workspace"};
+  const size_t wstride = (size_t)MAXRHS_LU * ws.n;
   for (int r0 = 0; r0 < nrhs; r0 += MAXRHS_LU) {
     const int nr = min(MAXRHS_LU, nrhs - r0);
     typename Cplx<T>::type* Yc = Y + (size_t)r0 * ldy;
+    copy_rhs_kernel<<<dim3((unsigned)(((size_t)n * nr + 255) / 256), 1, batch), 256, 0, stream>>>(
+        Yc, ldy, ystride, ws.subw, ws.n, wstride, n, nr);
     for (int i = 0; i < panels; ++i) {
       const int p = upper ? panels - 1 - i : i;
       const int k0 = p * NB, kb = min(NB, n - k0);
-      subst_block_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(Tinv + (size_t)p * NB * NB, bs.uinv, Yc, ldy,
-                                                                   ystride, k0, kb, nr, upper ? 1 : 0);
       const int ur0 = upper ? 0 : k0 + kb, unr = upper ? k0 : n - k0 - kb;
-      if (unr > 0)
-        subst_update_kernel<T><<<dim3((unr + 63) / 64, 1, batch), 64, 0, stream>>>(M, lda, mstride, Yc, ldy, ystride,
-                                                                                  k0, kb, ur0, unr, nr);
+      subst_panel_kernel<T><<<dim3((unsigned)std::max(1, (unr + 255) / 256), 1, batch), 256, 0, stream>>>(
+          M, lda, mstride, Tinv + (size_t)p * NB * NB, bs.uinv, ws.subw, wstride, ws.n, Yc, ldy, ystride, k0, kb, ur0,
+          unr, nr, upper ? 1 : 0);
     }
   }
 }
