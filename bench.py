@@ -241,7 +241,8 @@ def summary(prof, dt, steps, F, peak):
         "freq_per_s": round(steps * F / dt, 4),
         "ms_per_step": round(1e3 * dt / steps, 3),
         "stage_ms_per_frequency": {k: round(prof[k] / nf, 4)
-                                   for k in ("assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "forces_ms")},
+                                   for k in ("assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "mixed_refinement_ms",
+                                             "forces_ms")},
         "statics_ms_per_solve": round(prof["statics_ms"] / max(prof["statics_runs"], 1.0), 3),
         "lu_achieved_tflops": round(lu / 1e12, 3),
         "gemm_achieved_tflops": round(gemm / 1e12, 3),

@@ -199,7 +199,7 @@ class Session:
     PROFILE_KEYS = ("statics_ms", "assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "lu_gemm_flops",
                     "lu_gemm_launches", "forces_ms", "frequencies", "lu_flops", "assembly_bytes", "statics_runs",
                     "lu_fallback_panels", "mixed_refinement_steps", "mixed_max_backward_error",
-                    "mixed_fp64_fallbacks")
+                    "mixed_fp64_fallbacks", "mixed_refinement_ms")
 
     def set_batch(self, max_batch: int):
         """Frequencies in flight in solve (0: as many as fit, at most 48)."""
@@ -293,8 +293,8 @@ class Session:
         _check(library().hbg_set_profiling(self._h, int(bool(enable))))
 
     def profile(self) -> dict:
-        out = np.zeros(16)
-        _check(library().hbg_profile_read(self._h, _p(out), 16))
+        out = np.zeros(len(self.PROFILE_KEYS))
+        _check(library().hbg_profile_read(self._h, _p(out), len(self.PROFILE_KEYS)))
         return {k: float(out[i]) for i, k in enumerate(self.PROFILE_KEYS)}
 
     def wave_kernel(self, X, Y):

@@ -53,7 +53,7 @@ int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, L
 void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
                          double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
                          hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta,
-                         const int* nsys = nullptr);
+                         const int* nsys, int* active);
 int residual_blocks(int n);
 size_t residual_part_elems(int n, int R);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
@@ -310,6 +310,7 @@ struct HbgSession {
   DevBuf<float2> d_M32, d_D32;
   DevBuf<double2> d_X64;
   DevBuf<double> d_rpart, d_rscale;
+  DevBuf<int> d_active;  // systems still refined (mixed precision)
   DevBuf<double2> d_rpart2;
   NnDevice nn;          // the learned surrogate of the finite-depth evanescent remainder
   DevBuf<int2> d_nnq;   // pairs left to the FP64 evaluator by the surrogate's assembly
@@ -548,6 +549,7 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_pf.release();
   s->d_pcs.release();
   s->d_ff.release();
+  s->d_active.release();
   (void)hipStreamSynchronize(s->lu_stream);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
@@ -855,6 +857,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * B);
     s->d_rpart2.ensure(residual_part_elems(N, R) * B);
     s->d_rscale.ensure((size_t)R * B);
+    s->d_active.ensure((size_t)B);
   }
   std::vector<double> hout(nout * B);
   std::vector<int> iters(B);
@@ -927,7 +930,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     if (mixed) {
       mixed_solve_batched(Mh, s->d_M32.p + mstride * slot(h, 0), s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p,
                           s->d_rscale.p, nb, N, R, true, MIXED_TOL, MIXED_MAX_ITER, sb, prof ? &lus : nullptr,
-                          status.data(), iters.data(), eta.data());
+                          status.data(), iters.data(), eta.data(), nullptr, s->d_active.p);
       for (int b = 0; b < nb; ++b) {
         if (status[b] < 0) {  // the plain FP64 path for this frequency
           rebuild(f0 + b, b, h);
@@ -970,6 +973,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       s->prof[9] += nb * 8.0 / 3.0 * (double)N * N * N;
       s->prof[10] += nb * (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
       s->prof[12] += (double)lus.fallback_panels;
+      s->prof[16] += lus.refine_ms;
     }
     for (size_t i = 0; i < nb * nout; ++i)
       if (!std::isfinite(hout[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
@@ -1067,6 +1071,7 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     lead->d_rpart.ensure((size_t)residual_blocks(nmax) * 4 * B);
     lead->d_rpart2.ensure(residual_part_elems(nmax, Rmax) * B);
     lead->d_rscale.ensure((size_t)Rmax * B);
+    lead->d_active.ensure((size_t)B);
   }
   std::vector<size_t> hstride(count), nout(count);
   for (int i = 0; i < count; ++i) {
@@ -1118,7 +1123,7 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     if (mixed) {
       mixed_solve_batched(lead->d_M.p, lead->d_M32.p, lead->d_X64.p, lead->d_D32.p, lead->d_rpart.p, lead->d_rpart2.p,
                           lead->d_rscale.p, nb, n, R, true, MIXED_TOL, MIXED_MAX_ITER, st, nullptr, status.data(),
-                          iters.data(), eta.data(), ns);
+                          iters.data(), eta.data(), ns, lead->d_active.p);
       for (int b = 0; b < nb; ++b)
         if (status[b] < 0) {  // the plain FP64 path for this system
           build(q0 + b, b, n, R, mstride);

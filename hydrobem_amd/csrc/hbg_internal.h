@@ -60,6 +60,7 @@ struct WaveTables {
 // interchanges (the pivoting fallback).
 struct LuStats {
   double total_ms = 0.0, gemm_ms = 0.0, gemm_flops = 0.0;
+  double refine_ms = 0.0;  // mixed precision: residuals, corrections and the FP64 store
   long gemm_launches = 0;
   long fallback_panels = 0;
 };

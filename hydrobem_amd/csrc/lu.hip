@@ -625,14 +625,16 @@ __global__ void panel_finish_kernel(const int* __restrict__ flag, const int* __r
 // stored into Y by workgroup 0; each workgroup then updates its rows of W,
 // W[r0 : r0 + nr] -= M[r0 : r0 + nr, k0 : k0 + kb] Y_b, one row per thread (rows above the block
 // for U, below it for L). Grid (max(1, ceil(nr / 256)), 1, batch), 256 threads; nrhs <= MAXRHS_LU.
+// active (optional): systems z with active[z] == 0 are left as they are.
 template <typename T>
 __global__ void __launch_bounds__(256) subst_panel_kernel(const typename Cplx<T>::type* __restrict__ M, int lda,
                                                           size_t mstride, const typename Cplx<T>::type* __restrict__ Tinv,
                                                           size_t tstride, typename Cplx<T>::type* __restrict__ W,
                                                           size_t wstride, int ldw, typename Cplx<T>::type* __restrict__ Y,
                                                           int ldy, size_t ystride, int k0, int kb, int r0, int nr,
-                                                          int nrhs, int upper) {
+                                                          int nrhs, int upper, const int* __restrict__ active) {
   typedef typename Cplx<T>::type C2;
+  if (active && active[blockIdx.z] == 0) return;
   __shared__ C2 U[NB][NB + 1];
   __shared__ C2 Xb[NB][MAXRHS_LU + 1];
   M += blockIdx.z * mstride;
@@ -722,7 +724,8 @@ __global__ void __launch_bounds__(256) subst_panel_kernel(const typename Cplx<T>
 // W[z][c][r] = Y[z][c][r] for the nrhs columns (the work copy of the block substitutions)
 template <typename C2>
 __global__ void copy_rhs_kernel(const C2* __restrict__ Y, int ldy, size_t ystride, C2* __restrict__ W, int ldw,
-                                size_t wstride, int n, int nrhs) {
+                                size_t wstride, int n, int nrhs, const int* __restrict__ active) {
+  if (active && active[blockIdx.z] == 0) return;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)n * nrhs) return;
   const int r = (int)(e % n), c = (int)(e / n);
@@ -1069,7 +1072,8 @@ void factor_pivoted(double2* M, int n, int lda, int ncols, hipStream_t stream, L
 // the batch, at most MAXRHS_LU columns at a time.
 template <typename T>
 void substitute(const typename Cplx<T>::type* M, int n, int lda, size_t mstride, typename Cplx<T>::type* Y, int ldy,
-                size_t ystride, int nrhs, int batch, bool upper, hipStream_t stream, LuWorkspace<T>& ws) {
+                size_t ystride, int nrhs, int batch, bool upper, hipStream_t stream, LuWorkspace<T>& ws,
+                const int* active = nullptr) {
   const BatchStrides bs = ws.strides();
   const typename Cplx<T>::type* Tinv = upper ? ws.uinv : ws.linv;
   const int panels = (n + NB - 1) / NB;
@@ -1079,14 +1083,14 @@ void substitute(const typename Cplx<T>::type* M, int n, int lda, size_t mstride,
     const int nr = min(MAXRHS_LU, nrhs - r0);
     typename Cplx<T>::type* Yc = Y + (size_t)r0 * ldy;
     copy_rhs_kernel<<<dim3((unsigned)(((size_t)n * nr + 255) / 256), 1, batch), 256, 0, stream>>>(
-        Yc, ldy, ystride, ws.subw, ws.n, wstride, n, nr);
+        Yc, ldy, ystride, ws.subw, ws.n, wstride, n, nr, active);
     for (int i = 0; i < panels; ++i) {
       const int p = upper ? panels - 1 - i : i;
       const int k0 = p * NB, kb = min(NB, n - k0);
       const int ur0 = upper ? 0 : k0 + kb, unr = upper ? k0 : n - k0 - kb;
       subst_panel_kernel<T><<<dim3((unsigned)std::max(1, (unr + 255) / 256), 1, batch), 256, 0, stream>>>(
           M, lda, mstride, Tinv + (size_t)p * NB * NB, bs.uinv, ws.subw, wstride, ws.n, Yc, ldy, ystride, k0, kb, ur0,
-          unr, nr, upper ? 1 : 0);
+          unr, nr, upper ? 1 : 0, active);
     }
   }
 }
@@ -1197,12 +1201,13 @@ void lu_solve_batched32(float2* M, size_t mstride, int batch, int n, int lda, in
 }
 
 // X = U^-1 L^-1 X with the factors of the last lu_solve_batched32 (systems without interchanges):
-// column c of system b at X + b * xstride + c * ldx.
+// column c of system b at X + b * xstride + c * ldx; systems b with active[b] == 0 (device array,
+// optional) are left as they are.
 void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, float2* X, int ldx, size_t xstride,
-                  int nrhs, hipStream_t stream) {
+                  int nrhs, hipStream_t stream, const int* active) {
   LuWorkspace<float>& ws = workspace<float>();
-  substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, false, stream, ws);
-  substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, true, stream, ws);
+  substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, false, stream, ws, active);
+  substitute<float>(M, n, lda, mstride, X, ldx, xstride, nrhs, batch, true, stream, ws, active);
   HBG_HIP(hipGetLastError());
 }
 

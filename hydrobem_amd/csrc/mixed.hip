@@ -92,7 +92,9 @@ template <int RM>
 __global__ void __launch_bounds__(RES_THREADS) residual_part_kernel(const double2* __restrict__ M, size_t mstride,
                                                                     int n, const double2* __restrict__ X,
                                                                     size_t xstride, int R,
-                                                                    double2* __restrict__ part2) {
+                                                                    double2* __restrict__ part2,
+                                                                    const int* __restrict__ active) {
+  if (active[blockIdx.z] == 0) return;
   __shared__ double2 xs[RES_JC][RM];
   M += blockIdx.z * mstride;
   X += blockIdx.z * xstride;
@@ -150,7 +152,9 @@ __global__ void __launch_bounds__(RES_THREADS) residual_sum_kernel(const double2
                                                                    size_t xstride, int R,
                                                                    const double2* __restrict__ part2,
                                                                    float2* __restrict__ D, size_t dstride,
-                                                                   double* __restrict__ part) {
+                                                                   double* __restrict__ part,
+                                                                   const int* __restrict__ active) {
+  if (active[blockIdx.z] == 0) return;
   __shared__ double red[4][RES_THREADS / 64];
   M += blockIdx.z * mstride;
   X += blockIdx.z * xstride;
@@ -192,9 +196,10 @@ __global__ void __launch_bounds__(RES_THREADS) residual_sum_kernel(const double2
   }
 }
 
-// X += D (the correction, complex64)
+// X += D (the correction, complex64) for the systems still refined
 __global__ void update_kernel(double2* __restrict__ X, size_t xstride, const float2* __restrict__ D, size_t dstride,
-                              size_t count) {
+                              size_t count, const int* __restrict__ active) {
+  if (active[blockIdx.z] == 0) return;
   X += blockIdx.z * xstride;
   D += blockIdx.z * dstride;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
@@ -223,7 +228,7 @@ unsigned grid_for(size_t count) { return (unsigned)std::min<size_t>((count + 255
 void lu_solve_batched32(float2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
                         LuStats* stats, int* status, const int* nsys = nullptr);
 void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, float2* X, int ldx, size_t xstride,
-                  int nrhs, hipStream_t stream);
+                  int nrhs, hipStream_t stream, const int* active);
 
 int residual_blocks(int n) { return (n + RES_THREADS - 1) / RES_THREADS; }
 // the chunk partials of the residual, per system
@@ -232,13 +237,16 @@ size_t residual_part_elems(int n, int R) { return (size_t)RES_SPLIT * (R + 1) * 
 // Solve the `batch` FP64 systems M64 (n x (n + R), ld n, stride n (n + R)) in mixed precision:
 // M32 (same shape, complex64; its first n columns already written by the assembly when
 // a32_written) and D (n x R complex64), X (n x R complex128), part (batch x residual_blocks(n) x 4),
-// part2 (batch x residual_part_elems(n, R)) are workspaces. On return the solution is in the columns
+// part2 (batch x residual_part_elems(n, R)) and active (batch ints) are workspaces; the systems
+// that meet the tolerance take no further step (their solution is the one checked). On return the
+// solution is in the columns
 // [n, n + R) of M64 for the systems with status 0; status -1: the complex64 factorization needed
 // interchanges or the refinement did not reach the tolerance in max_iter steps (solve it in FP64);
 // k > 0: zero pivot k. iters[b], eta[b]: refinement steps and the final backward error.
 void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
                          double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
-                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta, const int* nsys) {
+                         hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta, const int* nsys,
+                         int* active) {
   if (R > RES_MAXR) throw Error{HBG_ERR_UNSUPPORTED, "mixed precision: more than 32 right-hand sides"};
   const size_t ms = (size_t)n * (n + R), xs = (size_t)n * R;
   const int nblk = residual_blocks(n);
@@ -255,40 +263,66 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
     iters[b] = 0;
     eta[b] = INFINITY;
   }
+  // act[b]: system b takes the next residual (and, unless it then meets the tolerance, the next
+  // correction); the kernels skip the others
+  hipEvent_t r0 = nullptr, r1 = nullptr;
+  if (stats) {
+    HBG_HIP(hipEventCreate(&r0));
+    HBG_HIP(hipEventCreate(&r1));
+    HBG_HIP(hipEventRecord(r0, stream));
+  }
+  std::vector<int> act(batch);
   bool open = false;
-  for (int b = 0; b < batch; ++b) open = open || status[b] == 0;
+  for (int b = 0; b < batch; ++b) {
+    act[b] = status[b] == 0 ? 1 : 0;
+    open = open || act[b];
+  }
+  HBG_HIP(hipMemcpyAsync(active, act.data(), batch * sizeof(int), hipMemcpyHostToDevice, stream));
   while (open) {
     const dim3 rg(nblk, RES_SPLIT, batch);
     if (R <= 8)
-      residual_part_kernel<8><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
+      residual_part_kernel<8><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, active);
     else if (R <= 16)
-      residual_part_kernel<16><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
+      residual_part_kernel<16><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, active);
     else
-      residual_part_kernel<RES_MAXR><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2);
-    residual_sum_kernel<<<dim3(nblk, 1, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, D, xs, part);
+      residual_part_kernel<RES_MAXR><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, active);
+    residual_sum_kernel<<<dim3(nblk, 1, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, D, xs, part,
+                                                                          active);
     HBG_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     HBG_HIP(hipStreamSynchronize(stream));
     open = false;
     for (int b = 0; b < batch; ++b) {
-      if (status[b] != 0) continue;
+      if (!act[b]) continue;
       double v[4] = {0, 0, 0, 0};
       for (int k = 0; k < nblk; ++k)
         for (int q = 0; q < 4; ++q) v[q] = std::max(v[q], hp[((size_t)b * nblk + k) * 4 + q]);
       const double den = v[1] * v[2] + v[3];
       eta[b] = den > 0.0 ? v[0] / den : 0.0;
-      if (!(eta[b] <= tol) && iters[b] < max_iter) open = true;
+      if (!(eta[b] <= tol) && iters[b] < max_iter)
+        open = true;
+      else
+        act[b] = 0;
     }
     if (!open) break;
-    // correct every system (those already converged take one more harmless step)
-    lu_resolve32(M32, ms, batch, n, n, D, n, xs, R, stream);
-    update_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(X, xs, D, xs, xs);
+    HBG_HIP(hipMemcpyAsync(active, act.data(), batch * sizeof(int), hipMemcpyHostToDevice, stream));
+    lu_resolve32(M32, ms, batch, n, n, D, n, xs, R, stream, active);
+    update_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(X, xs, D, xs, xs, active);
     for (int b = 0; b < batch; ++b)
-      if (status[b] == 0 && !(eta[b] <= tol)) ++iters[b];
+      if (act[b]) ++iters[b];
   }
   for (int b = 0; b < batch; ++b)
     if (status[b] == 0 && !(eta[b] <= tol)) status[b] = -1;
   store_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M64, ms, n, X, xs, R);
   HBG_HIP(hipGetLastError());
+  if (stats) {
+    HBG_HIP(hipEventRecord(r1, stream));
+    HBG_HIP(hipEventSynchronize(r1));
+    float t = 0.0f;
+    HBG_HIP(hipEventElapsedTime(&t, r0, r1));
+    stats->refine_ms += t;
+    (void)hipEventDestroy(r0);
+    (void)hipEventDestroy(r1);
+  }
 }
 
 }  // namespace hbg

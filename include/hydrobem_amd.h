@@ -245,8 +245,9 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);
  *   (8 N^3 / 3 each), 10 assembly algorithmic bytes (N^2 (32 + 2 x 16) each), 11 statics runs,
  *   12 LU panels that needed row interchanges (the pivoting fallback), 13 refinement steps of the
  *   mixed-precision solves, 14 their largest final backward error, 15 frequencies of a
- *   mixed-precision solve solved in FP64 instead. */
-#define HBG_PROFILE_COUNTERS 16
+ *   mixed-precision solve solved in FP64 instead, 16 the mixed-precision refinement ms (FP64
+ *   residuals, corrections with the complex64 factors, the FP64 solution stored). */
+#define HBG_PROFILE_COUNTERS 17
 HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable);
 HbgStatus hbg_profile_read(HbgSession* s, double* out, int32_t n);
 
