@@ -22,6 +22,7 @@
 // Then U12 = L11^-1 A12 and the trailing update A22 -= L21 U12 (gemm_mfma_kernel, complex FP64
 // on the matrix cores), once per block of panels in the pass without interchanges.
 #include <algorithm>
+#include <array>
 #include <type_traits>
 #include <vector>
 
@@ -1234,11 +1235,12 @@ __global__ void synth_kernel(typename Cplx<T>::type* M, size_t count, int n, int
 
 }  // namespace
 
-// Time the batched LU on synthetic systems (n, nrhs) for every GEMM tile variant (1, 2; complex64
-// also 3, 4, the split-FP16 kernels), block depth
-// (2, 4, 8) and batch (4, 8, 16, 32, 48: those up to max_batch that fit in mem_bytes) and keep the fastest
-// variant and depth; returns the fastest batch. rows of report: variant, depth, batch, ms per
-// system (up to max_rows rows).
+// Time the batched LU on synthetic systems (n, nrhs) in two stages: every GEMM tile variant (1, 2;
+// complex64 also 3, 4, the split-FP16 kernels) and block depth (2, 4, 8, 12, 16) at the largest
+// batch of 4, 8, 16, 32, 48 that is at most max_batch and fits in mem_bytes, then the smaller
+// batches at the fastest variant and depth; keeps the fastest variant and depth and returns the
+// fastest batch (by time per system). rows of report: variant, depth, batch, ms per system (up to
+// max_rows rows).
 template <typename T>
 int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t stream, double* report, int max_rows,
                 int* rows) {
@@ -1264,9 +1266,12 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   try {
     const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4}
                                                                       : std::vector<int>{1, 2};
+    std::vector<std::array<int, 3>> configs;  // variant, depth, batch
     for (int v : variants)
-      for (int d : {2, 4, 8})
-        for (int b : batches) {
+      for (int d : {2, 4, 8, 12, 16}) configs.push_back({v, d, batches.back()});
+    for (size_t c = 0; c < configs.size(); ++c) {
+      const int v = configs[c][0], d = configs[c][1], b = configs[c][2];
+      {
           gv = v;
           gd = d;
           double ms_sys = 0.0;
@@ -1294,7 +1299,10 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
             bd = d;
             bb = b;
           }
-        }
+      }
+      if (c + 1 == variants.size() * 5)  // second stage: the smaller batches at the fastest variant and depth
+        for (size_t i = 0; i + 1 < batches.size(); ++i) configs.push_back({bv, bd, batches[i]});
+    }
   } catch (...) {
     gv = old_v;
     gd = old_d;

@@ -226,12 +226,13 @@ enum {
 };
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
- * systems of order n with nrhs right-hand sides in the given precision for every GEMM tile variant
- * (1, 2; mixed precision also 3, 4), block depth (2, 4, 8) and batch (4, 8, 16, 32, 48 that fit in
- * a third of the free memory), then keeps the fastest variant and depth (HBG_TUNE_LU_GEMM,
+ * systems of order n with nrhs right-hand sides in the given precision, first for every GEMM tile
+ * variant (1, 2; mixed precision also 3, 4) and block depth (2, 4, 8, 12, 16) at the largest batch
+ * of 4, 8, 16, 32, 48 that fits in a third of the free memory, then for the smaller batches at the
+ * fastest variant and depth; keeps the fastest variant and depth (HBG_TUNE_LU_GEMM,
  * HBG_TUNE_LU_DEPTH; for the mixed precision HBG_TUNE_LU_GEMM_MIXED, HBG_TUNE_LU_DEPTH_MIXED) and
- * sets the session's batch. report: up to max_rows rows of (variant, depth, batch, ms per system), rows the
- * number written. */
+ * sets the session's batch (the fastest per system). report: up to max_rows rows of (variant,
+ * depth, batch, ms per system), rows the number written. */
 HbgStatus hbg_autotune(HbgSession* s, int32_t n, int32_t nrhs, int32_t precision, double* report, int32_t max_rows,
                        int32_t* rows);
 HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);

@@ -51,12 +51,14 @@ def test_autotune_sweep_and_solve(oracle):
     s = hb.Session(0)
     rows = s.autotune(1024, 7, "fp64")
     say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} b{int(r[2])} {r[3]:.3f} ms" for r in rows))
-    assert len(rows) == 2 * 3 * 5 and all(r[3] > 0 for r in rows)
+    # two variants x five depths at batch 48, then batches 4..32 at the fastest of them
+    assert len(rows) == 2 * 5 + 4 and all(r[3] > 0 for r in rows)
+    assert {int(r[1]) for r in rows} == {2, 4, 8, 12, 16}
     best = min(rows, key=lambda r: r[3])
     assert s.get_tuning(s.TUNE_LU_GEMM) == int(best[0]) and s.get_tuning(s.TUNE_LU_DEPTH) == int(best[1])
     rows32 = s.autotune(1024, 7, "mixed")
     say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} b{int(r[2])} {r[3]:.3f} ms" for r in rows32))
-    assert len(rows32) == 4 * 3 * 5 and all(r[3] > 0 for r in rows32)
+    assert len(rows32) == 4 * 5 + 4 and all(r[3] > 0 for r in rows32)
     best32 = min(rows32, key=lambda r: r[3])
     assert s.get_tuning(s.TUNE_LU_GEMM_MIXED) == int(best32[0])
     assert s.get_tuning(s.TUNE_LU_DEPTH_MIXED) == int(best32[1])
