@@ -41,7 +41,8 @@ VARIANTS = {1: ("64x64", "gemm_mfma_kernel<T, 64, 64>", None), 2: ("128x64", "ge
 # calibrated by a device copy of known size (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md):
 # (workload, path, GEMM tiles, panels per trailing update, batch) -> bytes
 TRAFFIC = {("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 16): 8.496e8,
-           ("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 48): 2.548e9}
+           ("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 48): 2.548e9,
+           ("cylinder8192fd", "mixed", "64x64 split-fp16", 16, 48): 2.852e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
                       "operands split into hi + lo, autotuned), FP64 iterative refinement to backward error 1e-12",
