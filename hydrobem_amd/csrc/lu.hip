@@ -473,6 +473,60 @@ __global__ void __launch_bounds__(256) diag_kernel(typename Cplx<T>::type* __res
 // trailing updates (split8: L21 row r, columns 8 g .. 8 g + 7 at k group g of the row; U12 column
 // c, rows 8 g .. 8 g + 7 at k group g of the column), so the panel is not read again.
 constexpr int AB = 32;
+
+// The complex64 block products of apply_kernel on the FP32 matrix cores (v_mfma_f32_16x16x4f32,
+// the lane layout of gemm_mfma_kernel: operand lane l holds column (row) l % 16 at k = l / 16 of
+// the k-step, output item q of lane l is row l % 16, column 4 (l / 16) + q), in place in Xs:
+//   right: Xs[c][i] = sum_k Xs[k][i] Ts[k][c]  (L21 = A21 Uinv; i < AB, c < NB; wave w: columns
+//          16 w .., both 16-row blocks),
+//   left:  Xs[r][c] = sum_k Ts[r][k] Xs[k][c]  (U12 = Linv A12; r < NB, c < AB; wave w: rows
+//          16 w .., both 16-column blocks).
+// Every thread of the workgroup calls it (it synchronizes).
+__device__ __forceinline__ void apply_mfma(const float2 (&Ts)[NB][NB + 1], float2 (&Xs)[NB][AB + 1], bool right) {
+  typedef Mfma<float> F;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+  F::v4 re[2], im[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) re[b] = im[b] = F::v4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int s = 0; s < NB / 4; ++s) {
+    const int k = 4 * s + kq;
+    if (right) {
+      const float2 u = Ts[k][16 * wave + l16];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const float2 x = Xs[k][16 * b + l16];
+        re[b] = F::mma(u.x, x.x, re[b]);
+        re[b] = F::mma(-u.y, x.y, re[b]);
+        im[b] = F::mma(u.y, x.x, im[b]);
+        im[b] = F::mma(u.x, x.y, im[b]);
+      }
+    } else {
+      const float2 l = Ts[16 * wave + l16][k];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const float2 x = Xs[k][16 * b + l16];
+        re[b] = F::mma(x.x, l.x, re[b]);
+        re[b] = F::mma(-x.y, l.y, re[b]);
+        im[b] = F::mma(x.y, l.x, im[b]);
+        im[b] = F::mma(x.x, l.y, im[b]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 v = float2(re[b][q], im[b][q]);
+      if (right)
+        Xs[16 * wave + 4 * kq + q][16 * b + l16] = v;
+      else
+        Xs[16 * wave + l16][16 * b + 4 * kq + q] = v;
+    }
+  __syncthreads();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __restrict__ M, int lda, int lim, int ncols, int k0,
                                                     int kb, int rblocks, const typename Cplx<T>::type* __restrict__ Uinv,
@@ -530,15 +584,21 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     static_assert(VR == 8 && VR * (256 / AB) == NB, "8 consecutive columns per thread");
     const int i = tid % AB, cq = tid / AB;  // row of the block, columns VR cq .. VR cq + 7
     C2 acc[VR];
+    if constexpr (std::is_same<T, float>::value) {
+      apply_mfma(Ts, Xs, true);
 #pragma unroll
-    for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
-    for (int k = 0; k < NB; ++k) {
-      const C2 x = Xs[k][i];
+      for (int v = 0; v < VR; ++v) acc[v] = Xs[VR * cq + v][i];
+    } else {
 #pragma unroll
-      for (int v = 0; v < VR; ++v) {
-        const C2 u = Ts[k][VR * cq + v];
-        acc[v].x = fma(x.x, u.x, fma(-x.y, u.y, acc[v].x));
-        acc[v].y = fma(x.x, u.y, fma(x.y, u.x, acc[v].y));
+      for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
+      for (int k = 0; k < NB; ++k) {
+        const C2 x = Xs[k][i];
+#pragma unroll
+        for (int v = 0; v < VR; ++v) {
+          const C2 u = Ts[k][VR * cq + v];
+          acc[v].x = fma(x.x, u.x, fma(-x.y, u.y, acc[v].x));
+          acc[v].y = fma(x.x, u.y, fma(x.y, u.x, acc[v].y));
+        }
       }
     }
     bool fail = false;
@@ -583,15 +643,21 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     __syncthreads();
     const int cl = tid % AB, rg = tid / AB;  // column of the block, rows VR rg .. VR rg + 7
     C2 acc[VR];
+    if constexpr (std::is_same<T, float>::value) {
+      apply_mfma(Ts, Xs, false);
 #pragma unroll
-    for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
-    for (int k = 0; k < NB; ++k) {
-      const C2 x = Xs[k][cl];
+      for (int v = 0; v < VR; ++v) acc[v] = Xs[VR * rg + v][cl];
+    } else {
 #pragma unroll
-      for (int v = 0; v < VR; ++v) {
-        const C2 l = Ts[VR * rg + v][k];
-        acc[v].x = fma(l.x, x.x, fma(-l.y, x.y, acc[v].x));
-        acc[v].y = fma(l.x, x.y, fma(l.y, x.x, acc[v].y));
+      for (int v = 0; v < VR; ++v) acc[v] = C2(0.0, 0.0);
+      for (int k = 0; k < NB; ++k) {
+        const C2 x = Xs[k][cl];
+#pragma unroll
+        for (int v = 0; v < VR; ++v) {
+          const C2 l = Ts[VR * rg + v][k];
+          acc[v].x = fma(l.x, x.x, fma(-l.y, x.y, acc[v].x));
+          acc[v].y = fma(l.x, x.y, fma(l.y, x.x, acc[v].y));
+        }
       }
     }
     const int c = base + cl;
