@@ -999,6 +999,23 @@ EventPool& events() {
   return p;
 }
 
+// A second stream (default priority) for the work of a panel that can run beside its diagonal
+// block, with the events that fork it from and join it to the calling stream (also inside a graph
+// capture, where they become the graph's edges). One per thread, as the workspaces.
+struct AuxStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+AuxStream& aux_stream() {
+  static thread_local AuxStream a;
+  if (!a.s) {
+    HBG_HIP(hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking));
+    HBG_HIP(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming));
+    HBG_HIP(hipEventCreateWithFlags(&a.join, hipEventDisableTiming));
+  }
+  return a;
+}
+
 struct GemmTimer {
   EventPool* ep;
   LuStats* stats;
@@ -1077,13 +1094,21 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int p = k0 / NB;
       const int kb = min(NB, n - k0);
       const int m = n - k0;
+      AuxStream* ax = nullptr;
       if (k0 > K0) {
         gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, gb, sop, stream, gt);
-        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, gb, sop, stream, gt);
+        // the panel's rows (the columns right of it) are needed only by apply_kernel: they are
+        // updated on the second stream beside the diagonal block (disjoint parts of M)
+        ax = &aux_stream();
+        HBG_HIP(hipEventRecord(ax->fork, stream));
+        HBG_HIP(hipStreamWaitEvent(ax->s, ax->fork, 0));
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, gb, sop, ax->s, gt);
+        HBG_HIP(hipEventRecord(ax->join, ax->s));
       }
       const size_t po = (size_t)p * NB * NB;
       diag_kernel<T><<<dim3(1, 1, batch), 256, 0, stream>>>(M, lda, k0, kb, 0, ws.flags + p, ws.zp_fast + p, ws.ipiv,
                                                             ws.uinv + po, ws.linv + po, ws.udiag, bs);
+      if (ax) HBG_HIP(hipStreamWaitEvent(stream, ax->join, 0));
       const int rb = (m - kb + AB - 1) / AB, cb = (ncols - k0 - kb + AB - 1) / AB;
       if (rb + cb > 0)
         apply_kernel<T><<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, ws.uinv + po,
