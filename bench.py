@@ -43,6 +43,9 @@ VARIANTS = {1: ("64x64", "gemm_mfma_kernel<T, 64, 64>", None), 2: ("128x64", "ge
 TRAFFIC = {("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 16): 8.496e8,
            ("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 48): 2.548e9,
            ("cylinder8192fd", "mixed", "64x64 split-fp16", 16, 48): 2.852e9}
+# HBM bytes per launch (one frequency) of the FP64 assembly kernel, from the same calibrated runs:
+# workload -> bytes (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md)
+ASM_TRAFFIC = {"cylinder8192fd": 8.97e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
                       "operands split into hi + lo, autotuned), FP64 iterative refinement to backward error 1e-12",
@@ -393,6 +396,15 @@ def main():
                      "achieved": gemm_tf, "peak": gpeak / 1e12, "unit": "TFLOP/s",
                      "frac": round(gemm_tf * 1e12 / gpeak, 4), "traffic": traffic,
                      "per_launch": summ["gemm_per_launch"]},
+        # the other large kernel of the step: the FP64 assembly (one launch per frequency), bound by
+        # FP64 vector issue, not HBM; its algorithmic bytes are N^2 (32 + 2 x 16) (statics read,
+        # S and A written), against the stage time (boundary values, tables and matrices)
+        "roofline_assembly": {"bound": "hbm", "kernel": "assemble_fin_kernel" if np.isfinite(w.get("depth", np.inf))
+                              else "assemble_kernel", "achieved": summ["assembly_achieved_gbs"],
+                              "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                              "frac": round(summ["assembly_achieved_gbs"] * 1e9 / PEAK_HBM, 4),
+                              "traffic": ASM_TRAFFIC.get(a.workload),
+                              "ms_per_launch": summ["stage_ms_per_frequency"]["assembly_ms"]},
     }
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w)
