@@ -44,7 +44,7 @@ TRAFFIC = {("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 16): 8.496e8,
            ("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 48): 2.548e9,
            ("cylinder8192fd", "mixed", "64x64 split-fp16", 16, 48): 2.852e9}
 # HBM bytes per launch (one frequency) of the FP64 assembly kernel, from the same calibrated runs:
-# workload -> bytes (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md)
+# workload -> bytes (profiles/assembly_hbm_traffic_cylinder8192_finite_depth.md)
 ASM_TRAFFIC = {"cylinder8192fd": 8.97e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
