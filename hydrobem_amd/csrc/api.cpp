@@ -255,7 +255,7 @@ struct TablesOnDevice {
   int device = -1;
   double* t2d = nullptr;
   double* t1d = nullptr;
-  double* ktab = nullptr;  // scaled K0, K1 tiles of the finite-depth series
+  double* ktab = nullptr;  // K0, K1 tiles of the finite-depth series
   double* fitm = nullptr;  // node values -> monomial coefficients of the Phi tiles
 };
 TablesOnDevice& tables_for(int device) {

@@ -9,7 +9,7 @@
 //     the Ft terms from the infinite-depth tables; one workgroup per tile fits the value and both
 //     derivatives (monomial coefficients from the node values by one matrix, M F M^T);
 //   - per panel the propagating-mode factors and cos, sin k_n (z + h).
-// The scaled K0, K1 tiles are frequency-independent (host, once per process).
+// The K0, K1 tiles are frequency-independent (host, once per process).
 #include <cmath>
 #include <vector>
 
@@ -374,8 +374,10 @@ std::vector<double> fit_matrix(int M) {
   return M1;
 }
 
-// e^x sqrt(x) K0(x) and K1(x) on the unit tiles of [1/2, 18): the trapezoidal rule of
+// K0(x) and K1(x) on the unit tiles of [1/2, 18): the trapezoidal rule of
 // K_nu(x) = int_0^inf e^{-x cosh t} cosh(nu t) dt at the Chebyshev nodes, fitted to monomials
+// (the tiles are narrow enough for the exponential decay: no scaling, no exp per root in the
+// kernels)
 std::vector<double> build_k_table() {
   const int M = KD + 1;
   std::vector<double> tab((size_t)KTILES * M * 2);
@@ -394,9 +396,10 @@ std::vector<double> build_k_table() {
         s1 += e * c;
         if (e * c < 1e-22L) break;
       }
-      // e^x sqrt(x) K(x) = sqrt(x) int e^{-x (cosh t - 1)} ... dt
-      v0[j] = (double)(s0 * step * sqrtl((long double)x));
-      v1[j] = (double)(s1 * step * sqrtl((long double)x));
+      // K(x) = e^{-x} int e^{-x (cosh t - 1)} ... dt
+      const long double ex = expl(-(long double)x);
+      v0[j] = (double)(s0 * step * ex);
+      v1[j] = (double)(s1 * step * ex);
     }
     for (int p = 0; p < M; ++p) {
       long double c0 = 0.0L, c1 = 0.0L;

@@ -14,8 +14,8 @@
 //         A0 = 2 pi k0 / (1 - e^{-4 k0 h} + 4 k0 h e^{-2 k0 h}), c_n = 4 (k_n^2 + K^2)/(k_n^2 h + K^2 h - K).
 // Phi1 and Phi2 come from per-frequency tiles (8 x 8 over [0, 1/2] x [0, 1] and 8 x 16 over
 // [0, 1/2] x [0, 2], degree FD, value and both derivatives fitted separately) that
-// green_fin.hip builds on the device from the contour integral of Q; e^x sqrt(x) K0 and K1 from
-// unit tiles (degree 17) of [1/2, 18).
+// green_fin.hip builds on the device from the contour integral of Q; K0 and K1 from unit tiles
+// (degree 17) of [1/2, 18).
 #pragma once
 
 #include "green_inf_dev.h"
@@ -29,7 +29,7 @@ constexpr int FTS = 8, FTW = 8, FTV = 16;  // tiles of Phi1 (FTS x FTW) and Phi2
 constexpr int FD = 10;                  // degree of the Phi tiles
 constexpr int FM = FD + 1;
 constexpr int FTILE = 3 * FM * FM;      // doubles per tile: [value, d/dxb, d/dvb][vb power][xb power]
-constexpr int KD = 17;                  // degree of the scaled K0, K1 tiles
+constexpr int KD = 17;                  // degree of the K0, K1 tiles
 constexpr int KTILES = 18;              // [1/2, 1), [1, 2), ..., [17, 18)
 
 // Per-frequency constants of the finite-depth kernel.
@@ -43,7 +43,7 @@ struct FinFreq {
 struct FinTables {
   const double* phi1;   // [FTS][FTW][FTILE]
   const double* phi2;   // [FTS][FTV][FTILE]
-  const double* ktab;   // [KTILES][KD + 1][2] monomial coefficients of e^x sqrt(x) (K0, K1)
+  const double* ktab;   // [KTILES][KD + 1][2] monomial coefficients of K0, K1
   const double4* pf;    // [N] propagating factors
   const double2* pcs;   // [N][FIN_ROOTS] (cos, sin) k_n (z + h)
 };
@@ -52,7 +52,7 @@ struct FinK {
   double Wr, WrR, WrZ, Wrz, Wi, WiR, WiZ, Wiz;
 };
 
-// The tile of x in [1/2, 18) of the scaled K0, K1 table and its tile variable
+// The tile of x in [1/2, 18) of the K0, K1 table and its tile variable
 __device__ __forceinline__ int ktile(double x, double& t) {
   int i;
   if (x < 1.0) {
@@ -87,7 +87,7 @@ __device__ __forceinline__ void ktile_eval_u(cdouble* c, double t, double& g0, d
   g1 = poly_u<KD + 1, 2>(c + 1, pw);
 }
 
-__device__ __forceinline__ void scaled_k01(const double* __restrict__ kt, double x, double& g0, double& g1) {
+__device__ __forceinline__ void k01(const double* __restrict__ kt, double x, double& g0, double& g1) {
   double t;
   const int i = ktile(x, t);
   ktile_eval(kt + (size_t)i * KTILE, t, g0, g1);
@@ -190,10 +190,8 @@ __device__ __forceinline__ FinK fin_kernel(const FinFreq* __restrict__ fp, const
     for (int m = 0; m < FIN_ROOTS; ++m) {
       const double kk = f.kn[m], xm = kk * R;
       if (xm >= SERIES_STOP) break;
-      double g0, g1;
-      scaled_k01(F.ktab, xm, g0, g1);
-      const double ex = exp(-xm) * rsqrt(xm);
-      const double K0 = g0 * ex, K1 = g1 * ex;
+      double K0, K1;
+      k01(F.ktab, xm, K0, K1);
       const double2 a = cz[m], b = cq[m];
       const double c = f.cn[m];
       re += c * a.x * b.x * K0;
@@ -286,15 +284,13 @@ __device__ __forceinline__ FinK fin_kernel_u(const FinFreq* __restrict__ fp, con
       const double kk = f.kn[m], xm = kk * R;
       on = on && xm < SERIES_STOP;
       if (!__ballot(on)) break;
-      double t, g0 = 0.0, g1 = 0.0;
+      double t, K0 = 0.0, K1 = 0.0;
       const int it = ktile(xm, t);
       HBG_UNIFORM_ROUNDS(it, on, u, mine, {
         cdouble* c = uniform_tile(F.ktab, u, KTILE);
-        if (mine) ktile_eval_u(c, t, g0, g1);
+        if (mine) ktile_eval_u(c, t, K0, K1);
       })
       if (on) {
-        const double ex = exp(-xm) * rsqrt(xm);
-        const double K0 = g0 * ex, K1 = g1 * ex;
         const double2 a = cz[m], b = cq[m];
         const double c = f.cn[m];
         re += c * a.x * b.x * K0;
