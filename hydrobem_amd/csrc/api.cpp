@@ -18,7 +18,8 @@ namespace hbg {
 
 // kernels (statics.hip, frequency.hip, lu.hip)
 void launch_statics(const PanelD* P, int N, int logint, double tol, double depth, const double* gl, StaticD* st,
-                    int2* queue, unsigned* qcount, unsigned qcap, hipStream_t stream);
+                    int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream);
+void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2* S, double2* A, hipStream_t stream);
 void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                          const StaticD* st, double2* S, double2* A, float2* A32, int lda, hipStream_t stream);
 void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
@@ -318,6 +319,8 @@ struct HbgSession {
   int max_batch = 0;  // frequencies in flight (0: as many as fit, at most 48)
   std::vector<hipEvent_t> bev;
   bool statics_valid = false;
+  bool stinf_valid = false;  // d_stinf holds the omega = infinity row of the statics
+  DevBuf<double2> d_stinf;
   int st_logint = -1;
   double st_tol = -1.0;
   WaveTables T;
@@ -356,10 +359,11 @@ HbgStatus fail(int status, const std::string& msg, const char* where) {
 
 void use_device(HbgSession* s) { HBG_HIP(hipSetDevice(s->device)); }
 
-void ensure_statics(HbgSession* s, int logint, double tol) {
-  if (s->statics_valid && s->st_logint == logint && s->st_tol == tol) return;
+void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false) {
+  if (s->statics_valid && s->st_logint == logint && s->st_tol == tol && (!want_inf || s->stinf_valid)) return;
   const int N = s->N;
   s->d_st.ensure((size_t)N * N);
+  if (want_inf) s->d_stinf.ensure((size_t)N * N);
   unsigned qcap = (unsigned)std::min<size_t>((size_t)N * 64, (size_t)1 << 28);
   if (s->profiling) HBG_HIP(hipEventRecord(s->ev[0], s->stream));
   for (int attempt = 0; attempt < 2; ++attempt) {
@@ -367,7 +371,7 @@ void ensure_statics(HbgSession* s, int logint, double tol) {
     s->d_qcount.ensure(1);
     HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
     launch_statics(s->d_panels.p, N, logint, tol, std::isfinite(s->depth) ? s->depth : 0.0, s->d_gl.p, s->d_st.p,
-                   s->d_queue.p, s->d_qcount.p, qcap, s->stream);
+                   s->d_queue.p, s->d_qcount.p, qcap, want_inf ? s->d_stinf.p : nullptr, s->stream);
     unsigned count = 0;
     HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
@@ -382,6 +386,7 @@ void ensure_statics(HbgSession* s, int logint, double tol) {
         s->prof[11] += 1.0;
       }
       s->statics_valid = true;
+      s->stinf_valid = want_inf;
       s->st_logint = logint;
       s->st_tol = tol;
       return;
@@ -550,6 +555,7 @@ HbgStatus hbg_session_free(HbgSession* s) {
   s->d_pcs.release();
   s->d_ff.release();
   s->d_active.release();
+  s->d_stinf.release();
   (void)hipStreamSynchronize(s->lu_stream);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
@@ -800,6 +806,42 @@ HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double
   GUARD_END("hbg_lu_solve")
 }
 
+namespace {
+
+// A frequency limit of the radiation problem in infinite depth (lo_frequency.rs with
+// FreqMode::Zero / Infinite; no diffraction there): the matrices of the statics alone
+// (launch_limit_matrices), S U for the radiation columns, the FP64 LU and the forces with omega = 0
+// in the force kernel (B = 0). Writes the added mass [D][K] into A (host). The statics (and at
+// infinity their omega = infinity row) must be current.
+void solve_limit(HbgSession* s, bool at_infinity, double* A) {
+  const int N = s->N, nrad = 6 * s->nb, R = nrad;
+  const size_t nAB = (size_t)nrad * nrad;
+  s->d_S.ensure((size_t)N * N);
+  s->d_M.ensure((size_t)N * (N + R));
+  s->d_u.ensure((size_t)R * N);
+  s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
+  s->d_out.ensure(2 * nAB);
+  s->d_phiI.ensure((size_t)N);
+  s->d_dphiI.ensure((size_t)N);
+  launch_limit_matrices(s->d_st.p, at_infinity ? s->d_stinf.p : nullptr, N, s->d_S.p, s->d_M.p, s->stream);
+  launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, 0, 1.0, 1.0, s->grav, 0.0, s->d_heads.p,
+                  s->d_phiI.p, s->d_dphiI.p, s->d_u.p, s->stream);
+  launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->d_part.p, s->stream);
+  const int zp = lu_solve_device(s->d_M.p, N, N, R, s->stream, nullptr);
+  if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix at a frequency limit (zero pivot " +
+                                                 std::to_string(zp) + ")"};
+  launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, 0, 0.0, s->rho, s->d_M.p, N, s->d_phiI.p,
+                s->d_dphiI.p, s->d_out.p, s->stream);
+  std::vector<double> h(2 * nAB);
+  HBG_HIP(hipMemcpyAsync(h.data(), s->d_out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  for (double v : h)
+    if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution at a frequency limit"};
+  std::memcpy(A, h.data(), nAB * sizeof(double));
+}
+
+}  // namespace
+
 HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const double* headings_deg,
                     int32_t n_headings, const HbgOptions* options, double* added_mass, double* damping,
                     double* excitation, double* froude_krylov, double* scattering, double* haskind,
@@ -809,8 +851,9 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   if (n_omegas < 0 || n_headings < 0) return fail(HBG_ERR_ARGUMENT, "negative count", "hbg_solve");
   if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_solve");
   for (int i = 0; i < n_omegas; ++i)
-    if (!(omegas[i] > 0.0) || !std::isfinite(omegas[i]))
-      return fail(HBG_ERR_VALUE, "omega " + std::to_string(i) + " must be finite and > 0", "hbg_solve");
+    if (!(omegas[i] >= 0.0))
+      return fail(HBG_ERR_VALUE,
+                  "omega " + std::to_string(i) + " must be > 0, or 0 or +infinity (the radiation limits)", "hbg_solve");
   GUARD_BEGIN
   check_options(options);
   HbgOptions o = defaults(options);
@@ -818,8 +861,27 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   const int N = s->N, H = n_headings, nrad = 6 * s->nb, D = nrad, R = nrad + H;
   if (R > 32) throw Error{HBG_ERR_UNSUPPORTED, "more than 32 right-hand sides"};
   check_depth(s);
+  // the ordinary frequencies go through the batched path (om[i] is the caller's omegas[io[i]]);
+  // the limits omega = 0 and infinity, infinite depth only, are solved after them (solve_limit)
+  std::vector<double> om;
+  std::vector<int> io, il;
+  bool want_inf = false;
+  for (int i = 0; i < n_omegas; ++i) {
+    if (omegas[i] > 0.0 && std::isfinite(omegas[i])) {
+      om.push_back(omegas[i]);
+      io.push_back(i);
+    } else {
+      il.push_back(i);
+      want_inf = want_inf || std::isinf(omegas[i]);
+    }
+  }
+  if (!il.empty() && std::isfinite(s->depth))
+    throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
+  const double* const omegas_all = omegas;
+  omegas = om.data();
+  n_omegas = (int32_t)om.size();
   if (progress && progress(user, HBG_STAGE_STATICS, 0, 1) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
-  ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
+  ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance, want_inf);
   std::vector<double> heads(std::max(H, 1), 0.0);
   for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
   s->d_heads.ensure(heads.size());
@@ -978,7 +1040,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     for (size_t i = 0; i < nb * nout; ++i)
       if (!std::isfinite(hout[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
     for (int b = 0; b < nb; ++b) {
-      const int fi = f0 + b;
+      const int fi = io[f0 + b];
       const double* hA = hout.data() + nout * b;
       const double* hB = hA + nAB;
       const double* hX = hB + nAB;
@@ -992,6 +1054,17 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       if (scattering) std::memcpy(scattering + fi * 2 * nX, hSC, 2 * nX * sizeof(double));
       if (haskind) std::memcpy(haskind + fi * 2 * nHK, hHK, 2 * nHK * sizeof(double));
     }
+  }
+  // the limits: added mass only (damping and the diffraction quantities are zero there)
+  for (int i : il) {
+    std::vector<double> a(nAB);
+    solve_limit(s, std::isinf(omegas_all[i]), a.data());
+    if (added_mass) std::memcpy(added_mass + i * nAB, a.data(), nAB * sizeof(double));
+    if (damping) std::fill(damping + i * nAB, damping + (i + 1) * nAB, 0.0);
+    if (excitation) std::fill(excitation + i * 2 * nX, excitation + (i + 1) * 2 * nX, 0.0);
+    if (froude_krylov) std::fill(froude_krylov + i * 2 * nX, froude_krylov + (i + 1) * 2 * nX, 0.0);
+    if (scattering) std::fill(scattering + i * 2 * nX, scattering + (i + 1) * 2 * nX, 0.0);
+    if (haskind) std::fill(haskind + i * 2 * nHK, haskind + (i + 1) * 2 * nHK, 0.0);
   }
   if (progress && progress(user, HBG_STAGE_OUTPUTS, n_omegas, n_omegas) == 0)
     throw Error{HBG_ERR_CANCELLED, "cancelled"};

@@ -510,6 +510,25 @@ void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, 
 
 int rhs_chunks(int N) { return (N + RHS_CHUNK - 1) / RHS_CHUNK; }
 
+// The matrices of a frequency limit in infinite depth (low_order.rs:635-637, 757-758: the wave
+// terms and the log terms vanish, K = 0): S = rs, D = rd (1/r + 1/r') at omega = 0, and with
+// phi = 0 on z = 0 the integrals of 1/r - 1/r' (inf) at omega = infinity; A = 2 pi I - D.
+// Column-major, leading dimension N (the layout of the statics).
+__global__ void limit_matrices_kernel(const StaticD* __restrict__ st, const double2* __restrict__ inf, int N,
+                                      double2* __restrict__ S, double2* __restrict__ A) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N) return;
+  const int k = (int)(e % N), n = (int)(e / N);
+  const double s = inf ? inf[e].x : st[e].rs, d = inf ? inf[e].y : st[e].rd;
+  S[e] = make_double2(s, 0.0);
+  A[e] = make_double2((k == n ? 2.0 * PI : 0.0) - d, 0.0);
+}
+
+void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2* S, double2* A, hipStream_t stream) {
+  limit_matrices_kernel<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, stream>>>(st, inf, N, S, A);
+  HBG_HIP(hipGetLastError());
+}
+
 void launch_pad_system(double2* M, float2* M32, int n, int nz, int R, int rz, hipStream_t stream) {
   if (nz >= n && rz >= R) return;
   pad_system_kernel<<<n + R, 256, 0, stream>>>(M, M32, n, nz, R, rz);

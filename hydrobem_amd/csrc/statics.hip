@@ -16,6 +16,7 @@
 // and finished by one wavefront per pair (adaptive_kernel): its 64 lanes evaluate the four
 // child cells of a quadrisection step (16 Gauss points each) at once, and the depth-first stack
 // of cells lives in LDS.
+#include "../../include/hydrobem_amd.h"
 #include "hbg_internal.h"
 
 namespace hbg {
@@ -136,11 +137,14 @@ __device__ __forceinline__ bool log_pair_d(const PanelD* __restrict__ p, P3 x, d
 
 }  // namespace
 
-// gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]
+// gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]. inf (optional, infinite depth):
+// the omega = infinity row of the pair, the integrals of 1/r - 1/r' (lo_static.rs:89-96), at
+// inf[j * N + k] as (source, dipole).
 __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, int N, int logint, double tol,
                                                       double depth, const double* __restrict__ gl,
                                                       StaticD* __restrict__ st, int2* __restrict__ queue,
-                                                      unsigned* __restrict__ qcount, unsigned qcap) {
+                                                      unsigned* __restrict__ qcount, unsigned qcap,
+                                                      double2* __restrict__ inf) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
   if (j >= N || k >= N) return;
@@ -153,6 +157,7 @@ __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__
   StaticD e;
   e.rs = s0 + sb;
   e.rd = d0 + db;
+  if (inf) inf[(size_t)j * N + k] = make_double2(s0 - sb, d0 - db);
   if (depth > 0.0) {
     // finite depth: the bottom images of lo_static.rs:29-38 (x, y, -z - 2h), (x, y, z + 2h),
     // (x, y, z - 2h), (x, y, -z - 4h), summed in the reference's order
@@ -305,10 +310,11 @@ __global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__
 }
 
 void launch_statics(const PanelD* P, int N, int logint, double tol, double depth, const double* gl, StaticD* st,
-                    int2* queue, unsigned* qcount, unsigned qcap, hipStream_t stream) {
+                    int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream) {
+  if (inf && depth > 0.0) throw Error{HBG_ERR_UNSUPPORTED, "the omega = infinity statics in finite depth"};
   dim3 block(64, 4);
   dim3 grid((N + 63) / 64, (N + 3) / 4);
-  statics_kernel<<<grid, block, 0, stream>>>(P, N, logint, tol, depth, gl, st, queue, qcount, qcap);
+  statics_kernel<<<grid, block, 0, stream>>>(P, N, logint, tol, depth, gl, st, queue, qcount, qcap, inf);
   HBG_HIP(hipGetLastError());
 }
 

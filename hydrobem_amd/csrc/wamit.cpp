@@ -1,7 +1,9 @@
 // WAMIT numeric files .1, .2, .3 of a solve (wamit_files.rs:794-867 build, :534-550 force_file,
 // :92-97 complex_cols, :1718-1757 fortran_e / fortran_i, :1822-1867 write_wamit; normalization
 // docs/reference/file-formats.md section 1):
-//   .1  period, I, J, A/(rho L^(3+ri+rj)), B/(w rho L^(3+ri+rj))            per frequency, I, J
+//   .1  period, I, J, A/(rho L^(3+ri+rj)), B/(w rho L^(3+ri+rj))            per frequency, I, J;
+//       the radiation limits as rows of period -1 (omega = 0) and 0 (omega = infinity) with the
+//       added mass only (wamit_files.rs:794-833; file-formats.md), in the order of the frequencies
 //   .2  period, beta, I, |X|, phase, Re, Im of the Haskind force / (rho g L^(2+ri))
 //   .3  the same columns for the pressure-integrated force
 // Period 2 pi / w in E14.6 0P form, values E14.6 1P, integers I6, CRLF line ends, a header line
@@ -100,12 +102,19 @@ int write_wamit(const char* stem, int overwrite, double rho, double g, double L,
     return 1;
   }
   auto r = [](int i) { return (i % 6) >= 3 ? 1 : 0; };
+  auto limit = [&](int f) { return omegas[f] == 0.0 || std::isinf(omegas[f]); };
   std::vector<std::string> per(F), bcol(H);
-  for (int f = 0; f < F; ++f)
-    if (!fortran_e(2.0 * PI_W / omegas[f], false, 14, 6, per[f])) {
+  for (int f = 0; f < F; ++f) {
+    if (!(omegas[f] >= 0.0)) {
+      err = "frequencies must be > 0, or 0 or +infinity (the limits)";
+      return 1;
+    }
+    const double p = limit(f) ? (omegas[f] == 0.0 ? -1.0 : 0.0) : 2.0 * PI_W / omegas[f];
+    if (!fortran_e(p, false, 14, 6, per[f])) {
       err = "frequency does not fit E14.6";
       return 1;
     }
+  }
   for (int h = 0; h < H; ++h)
     if (!fortran_e(heads[h], false, 14, 6, bcol[h])) {
       err = "heading does not fit E14.6";
@@ -127,7 +136,7 @@ int write_wamit(const char* stem, int overwrite, double rho, double g, double L,
           const size_t o = ((size_t)f * D + i) * K + k;
           std::string l = per[f];
           if (!fortran_i(i + 1, 6, l) || !fortran_i(k + 1, 6, l) || !fortran_e(A[o] / s, true, 14, 6, l) ||
-              !fortran_e(B[o] / (omegas[f] * s), true, 14, 6, l)) {
+              (!limit(f) && !fortran_e(B[o] / (omegas[f] * s), true, 14, 6, l))) {
             err = "value does not fit E14.6";
             return 1;
           }
@@ -138,7 +147,7 @@ int write_wamit(const char* stem, int overwrite, double rho, double g, double L,
   if (H > 0) {
     std::vector<std::string> l2, l3;
     for (int f = 0; f < F; ++f)
-      for (int h = 0; h < H; ++h) {
+      for (int h = 0; h < H && !limit(f); ++h) {
         for (int n = 0; n < K; ++n) {
           const size_t o = ((size_t)f * K + n) * H + h;
           const double s = rho * g * std::pow(L, 2 + r(n));

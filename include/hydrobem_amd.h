@@ -101,6 +101,11 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
  *   excitation, froude_krylov,
  *   scattering                  [F][H][D] complex              (X_i = FK + SC)
  *   haskind                     [F][D][H] complex              (Haskind exciting force, Output.HK)
+ * An omega of 0 or +infinity (infinite depth only; HBG_ERR_UNSUPPORTED in finite depth) requests
+ * that radiation limit (lo_static.rs:89-96, low_order.rs:635-637, 757-758: the statics alone, with
+ * 1/r - 1/r' at infinity), solved in FP64 whatever the precision option: its added mass row is
+ * written and its damping and diffraction rows are zero (the reference's
+ * added_mass_zero_frequency / added_mass_infinite_frequency).
  * HBG_ERR_NUMERICAL on a zero pivot ("singular system matrix (zero pivot k)") or a non-finite
  * matrix or solution (linalg.rs:221-252); HBG_ERR_CANCELLED when the callback returns 0. */
 HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const double* headings_deg,

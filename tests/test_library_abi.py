@@ -76,3 +76,32 @@ def test_wamit_files_on_host(tmp_path):
     with pytest.raises(hb.HbgError) as e:
         hb.wamit(stem, res, density=1025.0, gravity=9.81, overwrite=False)
     assert e.value.status == 10
+
+
+def test_wamit_limit_rows_on_host(tmp_path):
+    # the radiation limits in the order of the frequencies: period -1 (omega = 0) and 0 (omega =
+    # infinity), the added mass alone; the force files list the ordinary frequencies only
+    import math
+
+    import numpy as np
+
+    F, H, D = 3, 1, 6
+    rng = np.random.default_rng(1)
+    res = {"omegas": np.array([0.0, 0.8, math.inf]), "headings": np.array([0.0]),
+           "added_mass": rng.normal(size=(F, D, D)) * 1e3, "radiation_damping": np.zeros((F, D, D)),
+           "excitation_diffraction": rng.normal(size=(F, H, D)) + 1j * rng.normal(size=(F, H, D)),
+           "excitation_haskind": rng.normal(size=(F, H, D)) + 1j * rng.normal(size=(F, H, D))}
+    res["radiation_damping"][1] = rng.normal(size=(D, D)) * 1e2
+    stem = tmp_path / "lim"
+    hb.wamit(stem, res, density=1025.0, gravity=9.81, stamp="2026-01-01 00:00:00")
+    lines = (tmp_path / "lim.1").read_bytes().decode().split("\r\n")[1:-1]
+    assert len(lines) == F * D * D
+    zero, ordinary, inf = lines[:36], lines[36:72], lines[72:]
+    assert all(len(r) == 14 + 6 + 6 + 14 for r in zero + inf)
+    assert all(len(r) == 14 + 6 + 6 + 14 + 14 for r in ordinary)
+    assert float(zero[0][:14]) == -1.0 and float(inf[0][:14]) == 0.0
+    assert float(ordinary[0][:14]) == pytest.approx(2 * np.pi / 0.8, rel=5e-6)
+    assert float(zero[7][26:40]) == pytest.approx(res["added_mass"][0, 1, 1] / 1025.0, rel=1e-6)
+    assert float(inf[7][26:40]) == pytest.approx(res["added_mass"][2, 1, 1] / 1025.0, rel=1e-6)
+    t3 = (tmp_path / "lim.3").read_bytes().decode().split("\r\n")[1:-1]
+    assert len(t3) == 1 * H * D and all(float(r[:14]) == pytest.approx(2 * np.pi / 0.8, rel=5e-6) for r in t3)
