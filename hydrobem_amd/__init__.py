@@ -95,6 +95,9 @@ SIGNATURES = {
     "hbg_write_wamit": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
                                          _dp, _dp, _dp, _dp, ctypes.c_char_p]),
+    "hbg_write_wamit_files": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
+                                               ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_char_p]),
 }
 
 
@@ -368,14 +371,23 @@ def dispersion_roots(H, n):
     return out
 
 
-def wamit(stem, result, density, gravity, reference_length=1.0, overwrite=True, stamp="", n_bodies=1):
-    """The WAMIT .1/.2/.3 files of a solve (the C ABI's hbg_write_wamit)."""
+def wamit(stem, result, density, gravity, reference_length=1.0, overwrite=True, stamp="", n_bodies=1, rao=None):
+    """The WAMIT files of a solve (the C ABI's hbg_write_wamit_files): .1, .2, .3, and .2fk, .2sc,
+    .3fk, .3sc when the result holds the Froude-Krylov and scattering forces, .4 with rao (the
+    motions [F][H][D] of Session.motions)."""
     w = np.ascontiguousarray(result["omegas"], dtype=np.float64)
     h = np.ascontiguousarray(result["headings"], dtype=np.float64)
     A = np.ascontiguousarray(result["added_mass"], dtype=np.float64)
     B = np.ascontiguousarray(result["radiation_damping"], dtype=np.float64)
     HK = np.ascontiguousarray(np.transpose(result["excitation_haskind"], (0, 2, 1)), dtype=np.complex128)
     X = np.ascontiguousarray(result["excitation_diffraction"], dtype=np.complex128)
-    _check(library().hbg_write_wamit(str(stem).encode(), int(bool(overwrite)), float(density), float(gravity),
-                                     float(reference_length), _p(w), len(w), _p(h), len(h), int(n_bodies), _p(A),
-                                     _p(B), HK.ctypes.data_as(_dp), X.ctypes.data_as(_dp), stamp.encode()))
+
+    def opt(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.complex128)
+
+    FK, SC, RAO = opt(result.get("excitation_froude_krylov")), opt(result.get("excitation_scattering")), opt(rao)
+    ptr = (lambda a: None if a is None else a.ctypes.data_as(_dp))
+    _check(library().hbg_write_wamit_files(str(stem).encode(), int(bool(overwrite)), float(density), float(gravity),
+                                           float(reference_length), _p(w), len(w), _p(h), len(h), int(n_bodies),
+                                           _p(A), _p(B), HK.ctypes.data_as(_dp), X.ctypes.data_as(_dp), ptr(FK),
+                                           ptr(SC), ptr(RAO), stamp.encode()))

@@ -87,7 +87,8 @@ void lu_set_depth(int q);
 int lu_depth();
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
-                const double* X, const char* stamp, std::string& err);
+                const double* X, const double* FK, const double* SC, const double* RAO, const char* stamp,
+                std::string& err);
 
 namespace {
 
@@ -1537,10 +1538,29 @@ HbgStatus hbg_write_wamit(const char* stem, int32_t overwrite, double density, d
   GUARD_BEGIN
   std::string err;
   int st = write_wamit(stem, overwrite, density, gravity, reference_length, omegas, n_omegas, headings_deg,
-                       n_headings, n_bodies, added_mass, damping, haskind, excitation, stamp, err);
+                       n_headings, n_bodies, added_mass, damping, haskind, excitation, nullptr, nullptr, nullptr,
+                       stamp, err);
   if (st) throw Error{st, err};
   return HBG_OK;
   GUARD_END("hbg_write_wamit")
+}
+
+HbgStatus hbg_write_wamit_files(const char* stem, int32_t overwrite, double density, double gravity,
+                                double reference_length, const double* omegas, int32_t n_omegas,
+                                const double* headings_deg, int32_t n_headings, int32_t n_bodies,
+                                const double* added_mass, const double* damping, const double* haskind,
+                                const double* excitation, const double* froude_krylov, const double* scattering,
+                                const double* rao, const char* stamp) {
+  if (!stem || !omegas || !added_mass || !damping || (n_headings > 0 && (!headings_deg || !haskind || !excitation)))
+    return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_write_wamit_files");
+  GUARD_BEGIN
+  std::string err;
+  int st = write_wamit(stem, overwrite, density, gravity, reference_length, omegas, n_omegas, headings_deg,
+                       n_headings, n_bodies, added_mass, damping, haskind, excitation, froude_krylov, scattering, rao,
+                       stamp, err);
+  if (st) throw Error{st, err};
+  return HBG_OK;
+  GUARD_END("hbg_write_wamit_files")
 }
 
 }  // extern "C"

@@ -5,7 +5,10 @@
 //       the radiation limits as rows of period -1 (omega = 0) and 0 (omega = infinity) with the
 //       added mass only (wamit_files.rs:794-833; file-formats.md), in the order of the frequencies
 //   .2  period, beta, I, |X|, phase, Re, Im of the Haskind force / (rho g L^(2+ri))
-//   .3  the same columns for the pressure-integrated force
+//   .2fk, .2sc  the same for its Froude-Krylov part and Haskind - Froude-Krylov (:836-853)
+//   .3  the same columns for the pressure-integrated force; .3fk, .3sc its Froude-Krylov and
+//       scattering parts; .4 the motions (RAO) xi L^ri (:855-866)
+// (each written when its arrays are given; the limits have no rows in them)
 // Period 2 pi / w in E14.6 0P form, values E14.6 1P, integers I6, CRLF line ends, a header line
 // " HydroBEM <version>   <file>   <stamp>"; the files are written to temporary names and renamed
 // once all exist (all or none).
@@ -84,7 +87,8 @@ bool complex_cols(std::complex<double> z, std::string& out) {
 // A, B [F][D][D]; HK [F][D][H] complex (Output.HK order); X [F][H][D] complex.
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
-                const double* X, const char* stamp, std::string& err) {
+                const double* X, const double* FK, const double* SC, const double* RAO, const char* stamp,
+                std::string& err) {
   const int D = 6 * nb, K = D;
   if (!(rho > 0.0) || !(g > 0.0) || !(L > 0.0)) {
     err = "density, gravity and reference_length must be > 0";
@@ -144,33 +148,40 @@ int write_wamit(const char* stem, int overwrite, double rho, double g, double L,
         }
     files.push_back({".1", lines});
   }
-  if (H > 0) {
-    std::vector<std::string> l2, l3;
+  // a force-style file: for each ordinary frequency and heading, row i of value(f, h, i) / scale(i)
+  auto force_file = [&](const char* ext, auto value, auto scale) {
+    std::vector<std::string> lines;
     for (int f = 0; f < F; ++f)
-      for (int h = 0; h < H && !limit(f); ++h) {
-        for (int n = 0; n < K; ++n) {
-          const size_t o = ((size_t)f * K + n) * H + h;
-          const double s = rho * g * std::pow(L, 2 + r(n));
-          std::string l = per[f] + bcol[h];
-          if (!fortran_i(n + 1, 6, l) || !complex_cols(std::complex<double>(HK[2 * o], HK[2 * o + 1]) / s, l)) {
-            err = "value does not fit E14.6";
-            return 1;
-          }
-          l2.push_back(l);
-        }
+      for (int h = 0; h < H && !limit(f); ++h)
         for (int i = 0; i < D; ++i) {
-          const size_t o = ((size_t)f * H + h) * D + i;
-          const double s = rho * g * std::pow(L, 2 + r(i));
           std::string l = per[f] + bcol[h];
-          if (!fortran_i(i + 1, 6, l) || !complex_cols(std::complex<double>(X[2 * o], X[2 * o + 1]) / s, l)) {
+          if (!fortran_i(i + 1, 6, l) || !complex_cols(value(f, h, i) / scale(i), l)) {
             err = "value does not fit E14.6";
-            return 1;
+            return false;
           }
-          l3.push_back(l);
+          lines.push_back(l);
         }
-      }
-    files.push_back({".2", l2});
-    files.push_back({".3", l3});
+    files.push_back({ext, lines});
+    return true;
+  };
+  typedef std::complex<double> C;
+  auto at = [](const double* p, size_t o) { return C(p[2 * o], p[2 * o + 1]); };
+  auto hk = [&](int f, int h, int n) { return at(HK, ((size_t)f * K + n) * H + h); };  // [F][K][H]
+  auto fhd = [&](const double* p) { return [=](int f, int h, int i) { return at(p, ((size_t)f * H + h) * D + i); }; };
+  auto force_scale = [&](int i) { return rho * g * std::pow(L, 2 + r(i)); };
+  auto rao_scale = [&](int i) { return std::pow(L, -r(i)); };
+  if (H > 0) {
+    // the Haskind force by solved column (every mode of every body: column n is mode n)
+    if (!force_file(".2", hk, force_scale) || !force_file(".3", fhd(X), force_scale)) return 1;
+    if (FK) {
+      auto fk = fhd(FK);
+      if (!force_file(".2fk", fk, force_scale) ||
+          !force_file(".2sc", [&](int f, int h, int n) { return hk(f, h, n) - fk(f, h, n); }, force_scale) ||
+          !force_file(".3fk", fk, force_scale))
+        return 1;
+    }
+    if (SC && !force_file(".3sc", fhd(SC), force_scale)) return 1;
+    if (RAO && !force_file(".4", fhd(RAO), rao_scale)) return 1;
   }
   const std::string sstem(stem);
   if (!overwrite)

@@ -266,6 +266,17 @@ HbgStatus hbg_write_wamit(const char* stem, int32_t overwrite, double density, d
                           const double* headings_deg, int32_t n_headings, int32_t n_bodies,
                           const double* added_mass, const double* damping, const double* haskind,
                           const double* excitation, const char* stamp);
+/* The same and, for the arrays given (NULL: the file is not written), .2fk (the Froude-Krylov force
+ * by solved column), .2sc (Haskind - Froude-Krylov), .3fk, .3sc (the Froude-Krylov and scattering
+ * parts of the pressure force) and .4 (the motions, hbg_motions's rao, xi L^ri)
+ * (wamit_files.rs:836-866): froude_krylov, scattering, rao [F][H][D] complex as hbg_solve and
+ * hbg_motions give them. The radiation limits (omega 0, +infinity) have rows in .1 only. */
+HbgStatus hbg_write_wamit_files(const char* stem, int32_t overwrite, double density, double gravity,
+                                double reference_length, const double* omegas, int32_t n_omegas,
+                                const double* headings_deg, int32_t n_headings, int32_t n_bodies,
+                                const double* added_mass, const double* damping, const double* haskind,
+                                const double* excitation, const double* froude_krylov, const double* scattering,
+                                const double* rao, const char* stamp);
 
 #ifdef __cplusplus
 }

@@ -105,3 +105,41 @@ def test_wamit_limit_rows_on_host(tmp_path):
     assert float(inf[7][26:40]) == pytest.approx(res["added_mass"][2, 1, 1] / 1025.0, rel=1e-6)
     t3 = (tmp_path / "lim.3").read_bytes().decode().split("\r\n")[1:-1]
     assert len(t3) == 1 * H * D and all(float(r[:14]) == pytest.approx(2 * np.pi / 0.8, rel=5e-6) for r in t3)
+
+
+def test_wamit_force_parts_and_motions_on_host(tmp_path):
+    # .2fk / .2sc: the Froude-Krylov force by solved column and Haskind - Froude-Krylov; .3fk / .3sc:
+    # the Froude-Krylov and scattering parts; .4: the motions times L^ri (rotations), not divided by rho g
+    import numpy as np
+
+    F, H, D, L = 2, 2, 6, 2.0
+    rng = np.random.default_rng(2)
+
+    def c(*shape):
+        return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+    res = {"omegas": np.array([0.5, 1.0]), "headings": np.array([0.0, 90.0]),
+           "added_mass": rng.normal(size=(F, D, D)), "radiation_damping": rng.normal(size=(F, D, D)),
+           "excitation_diffraction": c(F, H, D), "excitation_haskind": c(F, H, D),
+           "excitation_froude_krylov": c(F, H, D), "excitation_scattering": c(F, H, D)}
+    rao = c(F, H, D)
+    stem = tmp_path / "parts"
+    hb.wamit(stem, res, density=1000.0, gravity=10.0, reference_length=L, rao=rao, stamp="s")
+
+    def rows(ext):
+        lines = (tmp_path / ("parts" + ext)).read_bytes().decode().split("\r\n")[1:-1]
+        assert len(lines) == F * H * D
+        return [complex(float(r[62:76]), float(r[76:90])) for r in lines]
+
+    def scale(i):
+        return 1000.0 * 10.0 * L ** (2 + (1 if i >= 3 else 0))
+
+    q = 1 * D + 4  # frequency 0, heading 1, row 5 (a rotation)
+    hk, fk, sc = res["excitation_haskind"][0, 1, 4], res["excitation_froude_krylov"][0, 1, 4], \
+        res["excitation_scattering"][0, 1, 4]
+    assert rows(".2fk")[q] == pytest.approx(fk / scale(4), rel=1e-5)
+    assert rows(".2sc")[q] == pytest.approx((hk - fk) / scale(4), rel=1e-5)
+    assert rows(".3fk")[q] == pytest.approx(fk / scale(4), rel=1e-5)
+    assert rows(".3sc")[q] == pytest.approx(sc / scale(4), rel=1e-5)
+    assert rows(".4")[q] == pytest.approx(rao[0, 1, 4] * L, rel=1e-5)
+    assert rows(".4")[2] == pytest.approx(rao[0, 0, 2], rel=1e-5)
