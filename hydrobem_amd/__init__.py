@@ -63,6 +63,7 @@ SIGNATURES = {
     "hbg_session_free": (ctypes.c_int32, [ctypes.c_void_p]),
     "hbg_set_environment": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "hbg_set_panels": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip, ctypes.c_int32, _dp]),
+    "hbg_set_symmetry": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_solve": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                    ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
                                    ctypes.c_void_p]),
@@ -176,6 +177,11 @@ class Session:
         _check(library().hbg_set_panels(self._h, _p(v), n, b.ctypes.data_as(_ip) if b is not None else None, nb,
                                         _p(rc)))
         self.n_panels, self.n_bodies = n, nb
+
+    def set_symmetry(self, sym_x=False, sym_y=False):
+        """Symmetry planes x = 0 (sym_x) and y = 0 (sym_y): the panels set are one sector of the
+        body and the solves return the whole body's quantities."""
+        _check(library().hbg_set_symmetry(self._h, int(bool(sym_x)), int(bool(sym_y))))
 
     def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None, precision="fp64",
               surrogate=False):

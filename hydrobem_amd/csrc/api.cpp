@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,11 +18,21 @@
 namespace hbg {
 
 // kernels (statics.hip, frequency.hip, lu.hip)
-void launch_statics(const PanelD* P, int N, int logint, double tol, double depth, const double* gl, StaticD* st,
-                    int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream);
+void launch_statics(const PanelD* P, int N, int Nsrc, int logint, double tol, double depth, const double* gl,
+                    StaticD* st, int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream);
+void launch_forces_sol(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w,
+                       double rho, const double2* sol, int lds, const double2* phiI, const double2* dphiI, double* out,
+                       hipStream_t stream);
+void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, int g, const double* chi, double2* S,
+                             double2* M, size_t mstride, float2* A32, hipStream_t stream);
+void launch_symmetry_project(const double2* u, int N, int G, int R, int c, const double* chi, double2* uc,
+                             hipStream_t stream);
+void launch_symmetry_recombine(const double2* M, size_t mstride, int N, int G, int R, const double* chi, double2* phi,
+                               hipStream_t stream);
 void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2* S, double2* A, hipStream_t stream);
-void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
-                         const StaticD* st, double2* S, double2* A, float2* A32, int lda, hipStream_t stream);
+void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
+                         double logc, const StaticD* st, double2* S, double2* A, float2* A32, int lda,
+                         double free_term, hipStream_t stream);
 void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
                             const double* zeta, int n, double* out, hipStream_t stream);
 size_t fin_tables_work();
@@ -34,8 +45,9 @@ std::vector<double> fit_matrix(int M);
 std::vector<double> build_k_table();
 void launch_adaptive(const PanelD* P, int N, const double* gl, StaticD* st, const int2* queue, unsigned count,
                      hipStream_t stream);
-void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
-                     double2* A, float2* A32, int lda, hipStream_t stream);
+void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
+                     const StaticD* st, double2* S, double2* A, float2* A32, int lda, double free_term,
+                     hipStream_t stream);
 void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream);
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
@@ -297,7 +309,14 @@ struct HbgSession {
   hipStream_t lu_stream = nullptr;  // a solve's LU, refinement and forces, beside the next batch's assembly
   hipEvent_t assembled[2] = {};     // the assembly of the batch in each half of the buffers is done
   double rho = 1000.0, grav = 9.80665, depth = INFINITY;
-  int N = 0, nb = 0;
+  int N = 0, nb = 0;  // N: the panels set (a symmetric body: its sector); the device arrays hold G N
+  // symmetry planes (lo_geometry.rs:25-35): G = 1, 2 or 4 reflections in the order e, sigma_x
+  // (x -> -x), sigma_y, sigma_x sigma_y (those present), chi[c][g] the characters
+  int sym_x = 0, sym_y = 0, G = 1;
+  std::vector<std::array<double, 2>> refl{{1.0, 1.0}};
+  std::vector<double> chi{1.0};
+  std::vector<double> raw_vertices, raw_centres;  // the panels as set, to rebuild the images
+  std::vector<int> raw_body;
   std::vector<PanelD> panels;
   std::vector<int> body;
   std::vector<double> nj;
@@ -312,6 +331,8 @@ struct HbgSession {
   DevBuf<float2> d_M32, d_D32;
   DevBuf<double2> d_X64;
   DevBuf<double> d_rpart, d_rscale;
+  DevBuf<double> d_chi;                    // the characters [G][G]
+  DevBuf<double2> d_Sg, d_Ag, d_uc, d_phif;  // symmetric solves: an image block, projected u, phi
   DevBuf<int> d_active;  // systems still refined (mixed precision)
   DevBuf<double2> d_rpart2;
   NnDevice nn;          // the learned surrogate of the finite-depth evanescent remainder
@@ -362,8 +383,9 @@ void use_device(HbgSession* s) { HBG_HIP(hipSetDevice(s->device)); }
 
 void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false) {
   if (s->statics_valid && s->st_logint == logint && s->st_tol == tol && (!want_inf || s->stinf_valid)) return;
-  const int N = s->N;
-  s->d_st.ensure((size_t)N * N);
+  const int N = s->N, NF = s->G * N;
+  if (want_inf && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
+  s->d_st.ensure((size_t)N * NF);
   if (want_inf) s->d_stinf.ensure((size_t)N * N);
   unsigned qcap = (unsigned)std::min<size_t>((size_t)N * 64, (size_t)1 << 28);
   if (s->profiling) HBG_HIP(hipEventRecord(s->ev[0], s->stream));
@@ -371,8 +393,8 @@ void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false
     s->d_queue.ensure(qcap);
     s->d_qcount.ensure(1);
     HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
-    launch_statics(s->d_panels.p, N, logint, tol, std::isfinite(s->depth) ? s->depth : 0.0, s->d_gl.p, s->d_st.p,
-                   s->d_queue.p, s->d_qcount.p, qcap, want_inf ? s->d_stinf.p : nullptr, s->stream);
+    launch_statics(s->d_panels.p, N, NF, logint, tol, std::isfinite(s->depth) ? s->depth : 0.0, s->d_gl.p,
+                   s->d_st.p, s->d_queue.p, s->d_qcount.p, qcap, want_inf ? s->d_stinf.p : nullptr, s->stream);
     unsigned count = 0;
     HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
@@ -418,12 +440,35 @@ FinTables fin_tables(HbgSession* s) {
 // S and A of the frequency K into S (ld N) and M (ld N): infinite depth, or finite depth with the
 // per-frequency tables built first; returns the wave number of the incident wave
 // S and A (and A32) with leading dimension lda (at least N; S in s->d_S, lda x N)
+// The matrices of one frequency: S (d_S) and A into Mout (leading dimension lda), A32 the
+// complex64 copy (nullable). A symmetric body (G > 1): the G character systems at Mout + c mstride
+// (leading dimension N) and S_c at d_S + c N N, from its image blocks (scratch d_Sg, d_Ag).
 double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, bool surrogate = false,
-                          float2* A32 = nullptr, int lda = 0) {
-  const int N = s->N;
+                          float2* A32 = nullptr, int lda = 0, size_t mstride = 0) {
+  const int N = s->N, G = s->G;
   if (lda <= 0) lda = N;
+  if (G > 1 && (surrogate || lda != N || !mstride))
+    throw Error{HBG_ERR_UNSUPPORTED, "symmetry planes with the surrogate or a padded system"};
+  if (G > 1) {
+    s->d_Sg.ensure((size_t)N * N);
+    s->d_Ag.ensure((size_t)N * N);
+  }
+  auto blocks = [&](auto one) {
+    if (G == 1) {
+      one(s->d_panels.p, s->d_st.p, s->d_S.p, Mout, A32, 2.0 * PI);
+      return;
+    }
+    for (int g = 0; g < G; ++g) {
+      one(s->d_panels.p + (size_t)g * N, s->d_st.p + (size_t)g * N * N, s->d_Sg.p, s->d_Ag.p, (float2*)nullptr,
+          g == 0 ? 2.0 * PI : 0.0);
+      launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, g, s->d_chi.p, s->d_S.p, Mout, mstride,
+                              g == G - 1 ? A32 : nullptr, s->stream);
+    }
+  };
   if (!std::isfinite(s->depth)) {
-    launch_assemble(s->d_panels.p, N, K, logc, s->T, s->d_st.p, s->d_S.p, Mout, A32, lda, s->stream);
+    blocks([&](const PanelD* Psrc, const StaticD* st, double2* S, double2* Ab, float2* a32, double free_term) {
+      launch_assemble(s->d_panels.p, Psrc, N, K, logc, s->T, st, S, Ab, a32, lda, free_term, s->stream);
+    });
     return K;
   }
   const FinFreq f = make_fin_freq(s->depth, K);
@@ -443,8 +488,10 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
     launch_assemble_fin_nn(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32,
                            lda, s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
   } else
-    launch_assemble_fin(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, lda,
-                        s->stream);
+    blocks([&](const PanelD* Psrc, const StaticD* st, double2* S, double2* Ab, float2* a32, double free_term) {
+      launch_assemble_fin(s->d_panels.p, Psrc, N, s->d_ff.p, fin_tables(s), s->T, logc, st, S, Ab, a32, lda, free_term,
+                          s->stream);
+    });
   return f.k0;
 }
 
@@ -581,41 +628,67 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
   return HBG_OK;
 }
 
+void build_geometry(HbgSession* s);
+
 HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
                          int32_t n_bodies, const double* rotation_centres) {
   if (!s || !vertices) return fail(HBG_ERR_ARGUMENT, "session or vertices is null", "hbg_set_panels");
   if (n_panels <= 0 || n_bodies <= 0) return fail(HBG_ERR_ARGUMENT, "n_panels and n_bodies must be > 0", "hbg_set_panels");
   GUARD_BEGIN
-  std::vector<PanelD> panels(n_panels);
-  std::vector<int> b(n_panels, 0);
-  std::vector<double> nj(6 * (size_t)n_panels);
-  for (int j = 0; j < n_panels; ++j) {
-    V3 proj[4], poly[4], normal, centroid;
-    int m;
-    double area;
-    std::string why = make_panel(vertices + 12 * (size_t)j, proj, poly, m, normal, centroid, area);
-    if (!why.empty()) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": " + why};
-    if (!(centroid[2] < 0.0))
-      throw Error{HBG_ERR_UNSUPPORTED,
-                  "panel " + std::to_string(j + 1) + ": centroid not below z = 0 (surfaces in z = 0 are not supported)"};
-    panels[j] = panel_record(proj, normal, centroid, area);
-    const int bi = body ? body[j] : 0;
-    if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
-    b[j] = bi;
-    V3 rc = rotation_centres ? v3(rotation_centres[3 * bi], rotation_centres[3 * bi + 1], rotation_centres[3 * bi + 2])
-                             : v3(0, 0, 0);
-    V3 mom = cross(centroid - rc, normal);
-    for (int d = 0; d < 3; ++d) {
-      nj[6 * (size_t)j + d] = normal[d];
-      nj[6 * (size_t)j + 3 + d] = mom[d];
+  s->raw_vertices.assign(vertices, vertices + 12 * (size_t)n_panels);
+  s->raw_body.assign(n_panels, 0);
+  for (int j = 0; j < n_panels; ++j) s->raw_body[j] = body ? body[j] : 0;
+  s->raw_centres.assign(3 * (size_t)n_bodies, 0.0);
+  if (rotation_centres) s->raw_centres.assign(rotation_centres, rotation_centres + 3 * (size_t)n_bodies);
+  s->N = n_panels;
+  s->nb = n_bodies;
+  build_geometry(s);
+  return HBG_OK;
+  GUARD_END("hbg_set_panels")
+}
+
+// The device panels of the session: the panels as set and, with symmetry planes, their images
+// in the order of the reflections (geometry.rs:209-234: the vertices reflected, their order
+// reversed where one reflection reverses the orientation), each with its body's modes.
+void build_geometry(HbgSession* s) {
+  const int n_panels = s->N, n_bodies = s->nb, G = s->G, NF = G * n_panels;
+  std::vector<PanelD> panels(NF);
+  std::vector<int> b(NF, 0);
+  std::vector<double> nj(6 * (size_t)NF);
+  for (int g = 0; g < G; ++g)
+    for (int j = 0; j < n_panels; ++j) {
+      const double sx = s->refl[g][0], sy = s->refl[g][1];
+      double v[12];
+      for (int q = 0; q < 4; ++q) {
+        const int src = (sx * sy < 0.0 && q > 0) ? 4 - q : q;  // 0, 3, 2, 1 where the orientation flips
+        v[3 * q] = sx * s->raw_vertices[12 * (size_t)j + 3 * src];
+        v[3 * q + 1] = sy * s->raw_vertices[12 * (size_t)j + 3 * src + 1];
+        v[3 * q + 2] = s->raw_vertices[12 * (size_t)j + 3 * src + 2];
+      }
+      V3 proj[4], poly[4], normal, centroid;
+      int m;
+      double area;
+      std::string why = make_panel(v, proj, poly, m, normal, centroid, area);
+      if (!why.empty()) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": " + why};
+      if (!(centroid[2] < 0.0))
+        throw Error{HBG_ERR_UNSUPPORTED,
+                    "panel " + std::to_string(j + 1) + ": centroid not below z = 0 (surfaces in z = 0 are not supported)"};
+      const size_t o = (size_t)g * n_panels + j;
+      panels[o] = panel_record(proj, normal, centroid, area);
+      const int bi = s->raw_body[j];
+      if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
+      b[o] = bi;
+      V3 rc = v3(s->raw_centres[3 * bi], s->raw_centres[3 * bi + 1], s->raw_centres[3 * bi + 2]);
+      V3 mom = cross(centroid - rc, normal);
+      for (int d = 0; d < 3; ++d) {
+        nj[6 * o + d] = normal[d];
+        nj[6 * o + 3 + d] = mom[d];
+      }
     }
-  }
   use_device(s);
   s->panels = panels;
   s->body = b;
   s->nj = nj;
-  s->N = n_panels;
-  s->nb = n_bodies;
   s->d_panels.ensure(panels.size());
   s->d_body.ensure(b.size());
   s->d_nj.ensure(nj.size());
@@ -623,8 +696,33 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
   HBG_HIP(hipMemcpy(s->d_body.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(s->d_nj.p, nj.data(), nj.size() * sizeof(double), hipMemcpyHostToDevice));
   s->statics_valid = false;
+}
+
+HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_symmetry");
+  if ((sym_x != 0 && sym_x != 1) || (sym_y != 0 && sym_y != 1))
+    return fail(HBG_ERR_VALUE, "sym_x and sym_y must be 0 or 1", "hbg_set_symmetry");
+  GUARD_BEGIN
+  s->sym_x = sym_x;
+  s->sym_y = sym_y;
+  s->refl.assign(1, {1.0, 1.0});
+  if (sym_x) s->refl.push_back({-1.0, 1.0});
+  if (sym_y) s->refl.push_back({1.0, -1.0});
+  if (sym_x && sym_y) s->refl.push_back({-1.0, -1.0});
+  s->G = (int)s->refl.size();
+  // the characters of the group (lo_geometry.rs:25-35): row c, column g
+  if (s->G == 1)
+    s->chi = {1.0};
+  else if (s->G == 2)
+    s->chi = {1.0, 1.0, 1.0, -1.0};
+  else
+    s->chi = {1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0};
+  use_device(s);
+  s->d_chi.ensure(s->chi.size());
+  HBG_HIP(hipMemcpy(s->d_chi.p, s->chi.data(), s->chi.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (s->N > 0) build_geometry(s);
   return HBG_OK;
-  GUARD_END("hbg_set_panels")
+  GUARD_END("hbg_set_symmetry")
 }
 
 HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64_t n, double* out) {
@@ -730,6 +828,7 @@ HbgStatus hbg_dispersion_roots(double H, int32_t n, double* out) {
 HbgStatus hbg_statics(HbgSession* s, int32_t logarithmic, double log_tolerance, double* out) {
   if (!s || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_statics");
   if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_statics");
+  if (s->G > 1) return fail(HBG_ERR_UNSUPPORTED, "the matrices of a body with symmetry planes", "hbg_statics");
   GUARD_BEGIN
   use_device(s);
   ensure_statics(s, logarithmic ? 1 : 0, log_tolerance);
@@ -752,6 +851,7 @@ HbgStatus hbg_statics(HbgSession* s, int32_t logarithmic, double log_tolerance, 
 HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, double* S, double* A) {
   if (!s || !S || !A) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_assemble");
   if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_assemble");
+  if (s->G > 1) return fail(HBG_ERR_UNSUPPORTED, "the matrices of a body with symmetry planes", "hbg_assemble");
   if (!(omega > 0.0) || !std::isfinite(omega)) return fail(HBG_ERR_VALUE, "omega must be finite and > 0", "hbg_assemble");
   GUARD_BEGIN
   check_options(options);
@@ -878,6 +978,11 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   }
   if (!il.empty() && std::isfinite(s->depth))
     throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
+  if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
+  if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
+  // a symmetric body (G > 1): every frequency is G character systems of order N (the sector),
+  // consecutive in the batch buffers; the boundary values and the forces take the whole body
+  const int G = s->G, NF = G * N;
   const double* const omegas_all = omegas;
   omegas = om.data();
   n_omegas = (int32_t)om.size();
@@ -892,39 +997,44 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // frequencies in flight: each holds its [A | B] (N x (N + R)) and incident-wave values until
   // its forces are taken; S, the right-hand-side partials and the finite-depth tables are shared
   const size_t mstride = (size_t)N * (N + R);
-  const size_t hstride = (size_t)std::max(H, 1) * N;
+  const size_t hstride = (size_t)std::max(H, 1) * NF;
   const bool mixed = o.precision == HBG_PRECISION_MIXED;
   const bool prof = s->profiling;
   // two halves of the batch buffers: the assembly of batch t + 1 (session stream) runs beside the
   // LU, refinement and forces of batch t (LU stream). One half (and one stream) when one batch
   // takes every frequency, or with stage timing. The batch size is balanced over the batches.
-  int B = batch_size(s, N, R, n_omegas, mixed, 1);
+  int B = batch_size(s, N, R, n_omegas, mixed, G);
   const int halves = (prof || B >= n_omegas) ? 1 : 2;
-  if (halves == 2) B = batch_size(s, N, R, n_omegas, mixed, 2);
+  if (halves == 2) B = batch_size(s, N, R, n_omegas, mixed, 2 * G);
   if (B > 0 && n_omegas > 0) {
     const int nbt = (n_omegas + B - 1) / B;
     B = (n_omegas + nbt - 1) / nbt;
   }
   hipStream_t sa = s->stream, sb = halves == 1 ? s->stream : s->lu_stream;
-  s->d_S.ensure((size_t)N * N);
-  s->d_M.ensure(mstride * B * halves);
+  const int BG = B * G;  // systems in flight per half
+  s->d_S.ensure((size_t)N * N * G);
+  s->d_M.ensure(mstride * BG * halves);
   s->d_phiI.ensure(hstride * B * halves);
   s->d_dphiI.ensure(hstride * B * halves);
-  s->d_u.ensure((size_t)R * N);
+  s->d_u.ensure((size_t)R * NF);
   s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
   s->d_out.ensure(nout * B * halves);
+  if (G > 1) {
+    s->d_uc.ensure((size_t)R * N);
+    s->d_phif.ensure((size_t)R * NF);
+  }
   if (mixed) {
-    s->d_M32.ensure(mstride * B * halves);
-    s->d_D32.ensure((size_t)N * R * B);
-    s->d_X64.ensure((size_t)N * R * B);
-    s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * B);
-    s->d_rpart2.ensure(residual_part_elems(N, R) * B);
-    s->d_rscale.ensure((size_t)R * B);
-    s->d_active.ensure((size_t)B);
+    s->d_M32.ensure(mstride * BG * halves);
+    s->d_D32.ensure((size_t)N * R * BG);
+    s->d_X64.ensure((size_t)N * R * BG);
+    s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * BG);
+    s->d_rpart2.ensure(residual_part_elems(N, R) * BG);
+    s->d_rscale.ensure((size_t)R * BG);
+    s->d_active.ensure((size_t)BG);
   }
   std::vector<double> hout(nout * B);
-  std::vector<int> iters(B);
-  std::vector<double> eta(B);
+  std::vector<int> iters(BG);
+  std::vector<double> eta(BG);
   if (prof && s->bev.size() < (size_t)(3 * B + 3)) {
     for (size_t i = s->bev.size(); i < (size_t)(3 * B + 3); ++i) {
       hipEvent_t e;
@@ -939,14 +1049,22 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   auto slot = [&](int h, int b) { return (size_t)h * B + b; };
   auto build = [&](int fi, int b, int h) {
     const double w = omegas[fi], K = w * w / s->grav;
-    double2* Mb = s->d_M.p + mstride * slot(h, b);
+    double2* Mb = s->d_M.p + mstride * slot(h, b) * G;
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], sa));
     const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0,
-                                         mixed ? s->d_M32.p + mstride * slot(h, b) : nullptr);
-    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
+                                         mixed ? s->d_M32.p + mstride * slot(h, b) * G : nullptr, N, mstride);
+    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
                     s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b), s->d_u.p, sa);
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], sa));
-    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, sa);
+    if (G == 1) {
+      launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, sa);
+    } else {
+      // character c: S_c times the projection of u (lo_frequency.rs:873-893)
+      for (int c = 0; c < G; ++c) {
+        launch_symmetry_project(s->d_u.p, N, G, R, c, s->d_chi.p, s->d_uc.p, sa);
+        launch_rhs(s->d_S.p + (size_t)c * N * N, N, N, s->d_uc.p, R, Mb + mstride * c, N, s->d_part.p, sa);
+      }
+    }
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 2], sa));
   };
   // a rebuilt system of the LU stream's batch: the LU stream waits for it
@@ -973,11 +1091,11 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     HBG_HIP(hipEventElapsedTime(&ms, a0, a1));
     return (double)ms;
   };
-  std::vector<int> status(B);
+  std::vector<int> status(BG);
   if (nbatches > 0) enqueue_batch(0);
   for (int t = 0; t < nbatches; ++t) {
-    const int f0 = t * B, nb = std::min(B, n_omegas - f0), h = t % halves;
-    double2* Mh = s->d_M.p + mstride * slot(h, 0);
+    const int f0 = t * B, nb = std::min(B, n_omegas - f0), h = t % halves, nbg = nb * G;
+    double2* Mh = s->d_M.p + mstride * slot(h, 0) * G;
     // the next batch's assembly goes ahead on the session stream (its half was last used by
     // batch t - 1, whose results the host has already copied)
     if (halves > 1 && t + 1 < nbatches) enqueue_batch(t + 1);
@@ -990,7 +1108,37 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     for (int b = 0; b < nb; ++b)
       if (progress && progress(user, HBG_STAGE_LU, f0 + b, n_omegas) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
     LuStats lus;
-    if (mixed) {
+    if (G > 1) {
+      // the character systems of the batch; a frequency with a system that needs the FP64 or the
+      // pivoted path is rebuilt (all its characters) and each solved by the FP64 LU with fallback
+      if (mixed)
+        mixed_solve_batched(Mh, s->d_M32.p + mstride * slot(h, 0) * G, s->d_X64.p, s->d_D32.p, s->d_rpart.p,
+                            s->d_rpart2.p, s->d_rscale.p, nbg, N, R, true, MIXED_TOL, MIXED_MAX_ITER, sb,
+                            prof ? &lus : nullptr, status.data(), iters.data(), eta.data(), nullptr, s->d_active.p);
+      else
+        lu_solve_batched(Mh, mstride, nbg, N, N, R, sb, prof ? &lus : nullptr, status.data());
+      for (int b = 0; b < nb; ++b) {
+        bool again = false;
+        for (int c = 0; c < G; ++c) {
+          const int q = b * G + c;
+          if (status[q] > 0)
+            throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(status[q]) + ")"};
+          again = again || status[q] < 0;
+          if (mixed && prof && status[q] == 0) {
+            s->prof[13] += iters[q];
+            s->prof[14] = std::max(s->prof[14], eta[q]);
+          }
+        }
+        if (!again) continue;
+        rebuild(f0 + b, b, h);
+        for (int c = 0; c < G; ++c) {
+          const int zp = lu_solve_device(Mh + mstride * (b * G + c), N, N, R, sb, prof ? &lus : nullptr);
+          if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(zp) + ")"};
+        }
+        if (mixed && prof) s->prof[15] += 1.0;
+      }
+      std::fill(status.begin(), status.begin() + nbg, 0);
+    } else if (mixed) {
       mixed_solve_batched(Mh, s->d_M32.p + mstride * slot(h, 0), s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p,
                           s->d_rscale.p, nb, N, R, true, MIXED_TOL, MIXED_MAX_ITER, sb, prof ? &lus : nullptr,
                           status.data(), iters.data(), eta.data(), nullptr, s->d_active.p);
@@ -1007,7 +1155,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     } else {
       lu_solve_batched(Mh, mstride, nb, N, N, R, sb, prof ? &lus : nullptr, status.data());
     }
-    for (int b = 0; b < nb; ++b) {
+    for (int b = 0; b < nb && G == 1; ++b) {
       if (status[b] < 0) {  // a panel needs interchanges: rebuild the system, pivoted path
         rebuild(f0 + b, b, h);
         status[b] = lu_solve_pivoted(Mh + mstride * b, N, N, R, sb, prof ? &lus : nullptr);
@@ -1017,10 +1165,19 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     }
     const int e0 = 3 * B, e1 = 3 * B + 1;
     if (prof) HBG_HIP(hipEventRecord(s->bev[e0], sb));
-    for (int b = 0; b < nb; ++b)
-      launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, omegas[f0 + b], s->rho, Mh + mstride * b, N,
-                    s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
-                    s->d_out.p + nout * slot(h, b), sb);
+    for (int b = 0; b < nb; ++b) {
+      if (G == 1) {
+        launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, omegas[f0 + b], s->rho, Mh + mstride * b, N,
+                      s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
+                      s->d_out.p + nout * slot(h, b), sb);
+      } else {
+        // the potential on the whole body from the characters (lo_frequency.rs:931-943)
+        launch_symmetry_recombine(Mh + mstride * b * G, mstride, N, G, R, s->d_chi.p, s->d_phif.p, sb);
+        launch_forces_sol(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, omegas[f0 + b], s->rho, s->d_phif.p,
+                          NF, s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
+                          s->d_out.p + nout * slot(h, b), sb);
+      }
+    }
     if (prof) HBG_HIP(hipEventRecord(s->bev[e1], sb));
     HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p + nout * slot(h, 0), nb * nout * sizeof(double),
                            hipMemcpyDeviceToHost, sb));
@@ -1033,8 +1190,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       s->prof[6] += (double)lus.gemm_launches;
       s->prof[7] += stage_ms(s->bev[e0], s->bev[e1]);
       s->prof[8] += nb;
-      s->prof[9] += nb * 8.0 / 3.0 * (double)N * N * N;
-      s->prof[10] += nb * (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
+      s->prof[9] += nbg * 8.0 / 3.0 * (double)N * N * N;
+      s->prof[10] += nbg * (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
       s->prof[12] += (double)lus.fallback_panels;
       s->prof[16] += lus.refine_ms;
     }
@@ -1083,6 +1240,8 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   for (int i = 0; i < count; ++i) {
     if (!sessions[i]) return fail(HBG_ERR_ARGUMENT, "null session", "hbg_solve_group");
     if (sessions[i]->N == 0) return fail(HBG_ERR_STATE, "a session has no panels", "hbg_solve_group");
+    if (sessions[i]->G > 1)
+      return fail(HBG_ERR_UNSUPPORTED, "a session with symmetry planes in a group", "hbg_solve_group");
     if (sessions[i]->device != sessions[0]->device)
       return fail(HBG_ERR_VALUE, "the sessions of a group must share their device", "hbg_solve_group");
     for (int j = 0; j < i; ++j)

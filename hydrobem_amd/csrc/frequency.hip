@@ -25,10 +25,10 @@ namespace hbg {
 __device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __restrict__ A, float2* __restrict__ A32,
                                           int lda, int row, int col, const StaticD& e, double Ar, double nz, double a,
                                           double c2, double Wr, double WrR, double WrZ, double Wi, double WiR,
-                                          double WiZ) {
+                                          double WiZ, double free_term) {
   S[(size_t)col * lda + row] = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
   const double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
-  const double2 v = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
+  const double2 v = make_double2((row == col ? free_term : 0.0) - dre, -dim);
   A[(size_t)col * lda + row] = v;
   if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
 }
@@ -100,17 +100,18 @@ __device__ __forceinline__ unsigned sort_tile_pairs(PairSort& L, int bi, int bj,
 
 // Infinite depth: the pairs sorted by the V tile of (X, Y) = (K R, -K (z + zeta)) (then also
 // their 1D tile of X is shared); pairs beyond the tables by their 1D tile, or in one bin.
-__global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict__ P, int N, double K, double logc,
-                                                       WaveTables T, const StaticD* __restrict__ st,
-                                                       double2* __restrict__ S, double2* __restrict__ A,
-                                                       float2* __restrict__ A32, int lda) {
+__global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict__ P, const PanelD* __restrict__ Psrc,
+                                                       int N, double K, double logc, WaveTables T,
+                                                       const StaticD* __restrict__ st, double2* __restrict__ S,
+                                                       double2* __restrict__ A, float2* __restrict__ A32, int lda,
+                                                       double free_term) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
   __shared__ PairSort L;
   static_assert(XT * YT + XT + 1 <= PS_KEYS, "infinite-depth keys");
   const unsigned total = sort_tile_pairs(L, bi, bj, N, [&](int k, int n) {
     const PanelD* pk = P + k;
-    const PanelD* pn = P + n;
+    const PanelD* pn = Psrc + n;
     const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
     const double X = K * sqrt(dx * dx + dy * dy), Y = -K * (pk->c[2] + pn->c[2]);
     if (X >= (double)XT) return XT * YT + XT;
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
     const int p = valid ? L.order[pos] : 0;
     const int k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
     const PanelD* pk = P + k;
-    const PanelD* pn = P + n;
+    const PanelD* pn = Psrc + n;
     const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy);
     const double X = K * R, Y = -K * (pk->c[2] + pn->c[2]);
@@ -138,11 +139,12 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
       const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
       const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
       const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
-      put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ);
+      put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ,
+                free_term);
       if (k != n) {
         const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
         put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR,
-                  WiZ);
+                  WiZ, free_term);
       }
     }
   }
@@ -156,12 +158,13 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
 // unit tiles among a wavefront's lanes).
 constexpr int FS_NEAR = FTS * FTW * FTV;  // keys of the near pairs: (ix, iy1, iy2)
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) assemble_fin_kernel(const PanelD* __restrict__ P, int N,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) assemble_fin_kernel(const PanelD* __restrict__ P,
+                                                           const PanelD* __restrict__ Psrc, int N,
                                                            const FinFreq* __restrict__ f,
                                                            FinTables F, WaveTables T, double logc,
                                                            const StaticD* __restrict__ st, double2* __restrict__ S,
                                                            double2* __restrict__ A, float2* __restrict__ A32,
-                                                           int lda) {
+                                                           int lda, double free_term) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
   __shared__ PairSort L;
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
   const double h = f->h, ih = 1.0 / h;
   const unsigned total = sort_tile_pairs(L, bi, bj, N, [&](int k, int n) {
     const PanelD* pk = P + k;
-    const PanelD* pn = P + n;
+    const PanelD* pn = Psrc + n;
     const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy), z = pk->c[2], zeta = pn->c[2];
     if (R >= 0.5 * h) {
@@ -190,18 +193,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
     const int p = valid ? L.order[pos] : 0;
     const int k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
     const PanelD* pk = P + k;
-    const PanelD* pn = P + n;
+    const PanelD* pn = Psrc + n;
     const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy);
     const FinK w = fin_kernel_u(f, F, T, valid, k, n, R, pk->c[2], pn->c[2]);
     if (valid) {
       const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
       put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi,
-                w.WiR, w.WiZ);
+                w.WiR, w.WiZ, free_term);
       if (k != n) {
         const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
         put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi,
-                  w.WiR, w.Wiz);
+                  w.WiR, w.Wiz, free_term);
       }
     }
   }
@@ -382,7 +385,7 @@ __global__ void __launch_bounds__(256) pad_system_kernel(double2* __restrict__ M
 // A, B [D][nrad]; X, FK, SC [H][D]; HK [nrad][H]; D = nrad = 6 nbodies.
 __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ P, const int* __restrict__ body,
                                                      const double* __restrict__ nj, int N, int nrad, int H,
-                                                     double w, double rho, const double2* __restrict__ M, int ldm,
+                                                     double w, double rho, const double2* __restrict__ sol, int lds,
                                                      const double2* __restrict__ phiI,
                                                      const double2* __restrict__ dphiI, double* __restrict__ out) {
   __shared__ double2 red[256];
@@ -411,14 +414,14 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
     if (kind == 0) {
       if (body[j] == i / 6) {
         const double ni = nj[6 * (size_t)j + i % 6] * area;
-        const double2 ph = M[(size_t)(N + c) * ldm + j];
+        const double2 ph = sol[(size_t)c * lds + j];
         s.x += ni * ph.x;
         s.y += ni * ph.y;
       }
     } else if (kind == 1) {
       if (body[j] == i / 6) {
         const double ni = nj[6 * (size_t)j + i % 6] * area;
-        const double2 ps = M[(size_t)(N + nrad + h) * ldm + j];
+        const double2 ps = sol[(size_t)(nrad + h) * lds + j];
         const double2 pi = phiI[(size_t)h * N + j];
         s.x += ni * (ps.x + pi.x);
         s.y += ni * (ps.y + pi.y);
@@ -428,7 +431,7 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
     } else {
       const double ncv = body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0;
       const double2 pi = phiI[(size_t)h * N + j];
-      const double2 ph = M[(size_t)(N + c) * ldm + j];
+      const double2 ph = sol[(size_t)c * lds + j];
       const double2 dn = dphiI[(size_t)h * N + j];
       // (n_c phi_I - phi_c dphi_I/dn) A
       s.x += (ncv * pi.x - (ph.x * dn.x - ph.y * dn.y)) * area;
@@ -474,18 +477,105 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
   }
 }
 
-// A32 (nullable): A in complex64 too
-void launch_assemble(const PanelD* P, int N, double K, double logc, WaveTables T, const StaticD* st, double2* S,
-                     double2* A, float2* A32, int lda, hipStream_t stream) {
+// A32 (nullable): A in complex64 too. Psrc: the source panels (P: the block of the panels with
+// themselves; the images of P under a reflection: that image block, whose pairs keep the symmetry
+// used here since |c_k - g c_n| = |c_n - g c_k|, with st its statics and free_term 0).
+void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
+                     const StaticD* st, double2* S, double2* A, float2* A32, int lda, double free_term,
+                     hipStream_t stream) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
-  assemble_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, K, logc, T, st, S, A, A32, lda);
+  assemble_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, Psrc, N, K, logc, T, st, S, A, A32, lda, free_term);
   HBG_HIP(hipGetLastError());
 }
 
-void launch_assemble_fin(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
-                         const StaticD* st, double2* S, double2* A, float2* A32, int lda, hipStream_t stream) {
+void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
+                         double logc, const StaticD* st, double2* S, double2* A, float2* A32, int lda,
+                         double free_term, hipStream_t stream) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
-  assemble_fin_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda);
+  assemble_fin_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, Psrc, N, f, F, T, logc, st, S, A, A32, lda, free_term);
+  HBG_HIP(hipGetLastError());
+}
+
+// The character systems of a symmetric body from its image blocks (lo_frequency.rs:855-943): image g
+// (S_g, A_g of the block (k, g n)) adds chi_c(g) times itself into S_c and the matrix part of M_c for
+// every character c (g = 0 sets them); with A32 (the last image) the complex64 copy of M_c too.
+// chi: [G][G] row c, column g; M_c at M + c mstride (leading dimension N), S_c at S + c N N.
+__global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const double2* __restrict__ Ag, int N, int G,
+                                        int g, const double* __restrict__ chi, double2* __restrict__ S,
+                                        double2* __restrict__ M, size_t mstride, float2* __restrict__ A32) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N) return;
+  const double2 sg = Sg[e], ag = Ag[e];
+  for (int c = 0; c < G; ++c) {
+    const double x = chi[c * G + g];
+    double2* s = S + (size_t)c * N * N + e;
+    double2* m = M + (size_t)c * mstride + e;
+    double2 sv = make_double2(x * sg.x, x * sg.y), mv = make_double2(x * ag.x, x * ag.y);
+    if (g > 0) {
+      const double2 s0 = *s, m0 = *m;
+      sv = make_double2(s0.x + sv.x, s0.y + sv.y);
+      mv = make_double2(m0.x + mv.x, m0.y + mv.y);
+    }
+    *s = sv;
+    *m = mv;
+    if (A32) A32[(size_t)c * mstride + e] = float2((float)mv.x, (float)mv.y);
+  }
+}
+
+void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, int g, const double* chi, double2* S,
+                             double2* M, size_t mstride, float2* A32, hipStream_t stream) {
+  symmetry_combine_kernel<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, stream>>>(Sg, Ag, N, G, g, chi, S, M,
+                                                                                      mstride, A32);
+  HBG_HIP(hipGetLastError());
+}
+
+// The right-hand-side values of character c (lo_frequency.rs:873-880): uc[col][n] =
+// (1 / G) sum_g chi_c(g) u[col][g N + n] for the R columns of u on the G N panels of the whole body
+// (the S U product negates them).
+__global__ void symmetry_project_kernel(const double2* __restrict__ u, int N, int G, int R, int c,
+                                        const double* __restrict__ chi, double2* __restrict__ uc) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * R) return;
+  const int n = (int)(e % N), col = (int)(e / N);
+  double2 v = make_double2(0.0, 0.0);
+  for (int g = 0; g < G; ++g) {
+    const double x = chi[c * G + g];
+    const double2 a = u[(size_t)col * G * N + (size_t)g * N + n];
+    v.x += x * a.x;
+    v.y += x * a.y;
+  }
+  uc[e] = make_double2(v.x / G, v.y / G);
+}
+
+void launch_symmetry_project(const double2* u, int N, int G, int R, int c, const double* chi, double2* uc,
+                             hipStream_t stream) {
+  symmetry_project_kernel<<<(unsigned)(((size_t)N * R + 255) / 256), 256, 0, stream>>>(u, N, G, R, c, chi, uc);
+  HBG_HIP(hipGetLastError());
+}
+
+// The potential on the whole body from the character solutions (lo_frequency.rs:931-943):
+// phi[col][g N + n] = sum_c chi_c(g) phi_c[col][n], phi_c the solution columns of M_c
+// (M + c mstride + N N, leading dimension N).
+__global__ void symmetry_recombine_kernel(const double2* __restrict__ M, size_t mstride, int N, int G, int R,
+                                          const double* __restrict__ chi, double2* __restrict__ phi) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)G * N * R) return;
+  const int m = (int)(e % ((size_t)G * N)), col = (int)(e / ((size_t)G * N));
+  const int g = m / N, n = m % N;
+  double2 v = make_double2(0.0, 0.0);
+  for (int c = 0; c < G; ++c) {
+    const double x = chi[c * G + g];
+    const double2 a = M[(size_t)c * mstride + (size_t)N * N + (size_t)col * N + n];
+    v.x += x * a.x;
+    v.y += x * a.y;
+  }
+  phi[e] = v;
+}
+
+void launch_symmetry_recombine(const double2* M, size_t mstride, int N, int G, int R, const double* chi, double2* phi,
+                               hipStream_t stream) {
+  symmetry_recombine_kernel<<<(unsigned)(((size_t)G * N * R + 255) / 256), 256, 0, stream>>>(M, mstride, N, G, R,
+                                                                                            chi, phi);
   HBG_HIP(hipGetLastError());
 }
 
@@ -550,11 +640,23 @@ void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, doubl
   HBG_HIP(hipGetLastError());
 }
 
+// the solution of [A | B] in the columns [N, N + R) of M (leading dimension ldm)
 void launch_forces(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double rho,
                    const double2* M, int ldm, const double2* phiI, const double2* dphiI, double* out,
                    hipStream_t stream) {
   const int items = nrad * nrad + H * nrad + nrad * H;
-  forces_kernel<<<items, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, rho, M, ldm, phiI, dphiI, out);
+  forces_kernel<<<items, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, rho, M + (size_t)N * ldm, ldm, phiI, dphiI,
+                                           out);
+  HBG_HIP(hipGetLastError());
+}
+
+// the same with the solution given on its own: column c at sol + c lds (the whole body of a
+// symmetric solve)
+void launch_forces_sol(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w,
+                       double rho, const double2* sol, int lds, const double2* phiI, const double2* dphiI, double* out,
+                       hipStream_t stream) {
+  const int items = nrad * nrad + H * nrad + nrad * H;
+  forces_kernel<<<items, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, rho, sol, lds, phiI, dphiI, out);
   HBG_HIP(hipGetLastError());
 }
 

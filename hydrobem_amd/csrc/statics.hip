@@ -140,14 +140,14 @@ __device__ __forceinline__ bool log_pair_d(const PanelD* __restrict__ p, P3 x, d
 // gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]. inf (optional, infinite depth):
 // the omega = infinity row of the pair, the integrals of 1/r - 1/r' (lo_static.rs:89-96), at
 // inf[j * N + k] as (source, dipole).
-__global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, int N, int logint, double tol,
-                                                      double depth, const double* __restrict__ gl,
+__global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, int N, int Nsrc, int logint,
+                                                      double tol, double depth, const double* __restrict__ gl,
                                                       StaticD* __restrict__ st, int2* __restrict__ queue,
                                                       unsigned* __restrict__ qcount, unsigned qcap,
                                                       double2* __restrict__ inf) {
   const int k = blockIdx.x * 64 + threadIdx.x;
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
-  if (j >= N || k >= N) return;
+  if (j >= Nsrc || k >= N) return;
   const PanelD* p = P + j;
   const P3 x = ld3(P[k].c);
   double s0, d0, sb, db;
@@ -309,12 +309,14 @@ __global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__
   }
 }
 
-void launch_statics(const PanelD* P, int N, int logint, double tol, double depth, const double* gl, StaticD* st,
-                    int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream) {
+// field points: the centroids of the first N panels; sources: the first Nsrc >= N panels (a
+// symmetric body: its N panels and their images); st[j * N + k] for the pair (field k, source j)
+void launch_statics(const PanelD* P, int N, int Nsrc, int logint, double tol, double depth, const double* gl,
+                    StaticD* st, int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream) {
   if (inf && depth > 0.0) throw Error{HBG_ERR_UNSUPPORTED, "the omega = infinity statics in finite depth"};
   dim3 block(64, 4);
-  dim3 grid((N + 63) / 64, (N + 3) / 4);
-  statics_kernel<<<grid, block, 0, stream>>>(P, N, logint, tol, depth, gl, st, queue, qcount, qcap, inf);
+  dim3 grid((N + 63) / 64, (Nsrc + 3) / 4);
+  statics_kernel<<<grid, block, 0, stream>>>(P, N, Nsrc, logint, tol, depth, gl, st, queue, qcount, qcap, inf);
   HBG_HIP(hipGetLastError());
 }
 

@@ -94,6 +94,16 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
 HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
                          int32_t n_bodies, const double* rotation_centres);
 
+/* Symmetry planes of the wetted surface (the mesh file's ISX, ISY; mesh.rs symmetry "yz", "xz",
+ * "xz_and_yz"): sym_x 1 reflects the panels about x = 0, sym_y 1 about y = 0 (each 0 or 1). The
+ * panels given to hbg_set_panels are then one sector; the solve adds their images (geometry.rs:
+ * 209-234; each image panel belongs to its original's body, so every body must itself be symmetric)
+ * and splits each frequency into G = 1, 2 or 4 systems of the sector's order by the characters of
+ * the reflection group (lo_geometry.rs:25-35, lo_frequency.rs:860-943); the results are those of
+ * the whole body. Kept across hbg_set_panels; set before or after it. With G > 1, hbg_statics,
+ * hbg_assemble, hbg_solve_group, the surrogate and the frequency limits are HBG_ERR_UNSUPPORTED. */
+HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
+
 /* The solve of n_omegas frequencies (rad/s) and n_headings headings (degrees), radiation of the
  * six rigid modes of every body plus diffraction (solver::run, lo_frequency.rs:1565-1635).
  * D = 6 n_bodies. Outputs (any may be NULL):
