@@ -76,6 +76,7 @@ def workload(name):
         v = meshes.cylinder_panels(1.0, 1.0, 128, 32, 32, quarter=False, cosine=True)
         nu = np.logspace(np.log10(0.05), np.log10(20.0), 48) / 3.0
         return dict(name=name, vertices=v, omegas=np.sqrt(9.81 * nu), headings=[0.0], rho=1025.0, g=9.81,
+                    quadrant=meshes.cylinder_panels(1.0, 1.0, 32, 32, 32, quarter=True, cosine=True),
                     depth=3.0, desc="truncated cylinder R=1 draft=1, finite depth h=3, 8192 panels, "
                                     "48 frequencies nu*h log-spaced 0.05-20, heading 0 (7 RHS), complex128")
     if name == "hemisphere512":
@@ -273,6 +274,9 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed,mixed_nn", help="paths timed, the last one gives `value`")
     ap.add_argument("--no-autotune", action="store_true", help="skip the LU autotuner before each path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--symmetry", action="store_true",
+                    help="solve the body as its quadrant with the symmetry planes x = 0 and y = 0 (not the "
+                         "headline workload; the surrogate path is not available there)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only copy this many GiB device to device and exit (the calibration of HBM counters)")
     a = ap.parse_args()
@@ -296,17 +300,27 @@ def main():
     import hydrobem_amd as hb
 
     w = workload(a.workload)
+    wkey = a.workload
+    if a.symmetry:
+        if "quadrant" not in w:
+            raise SystemExit(f"workload {a.workload} has no quadrant mesh")
+        nq = w["quadrant"].shape[0]
+        w = dict(w, vertices=w["quadrant"], desc=w["desc"] + f"; solved as its {nq}-panel quadrant with the "
+                 f"symmetry planes x = 0 and y = 0 (4 character systems of {nq} unknowns per frequency)")
+        wkey = a.workload + "_quadrant"
     v, omegas, heads = w["vertices"], w["omegas"], w["headings"]
     N, F = v.shape[0], len(omegas)
     s = hb.Session(0)
     s.set_environment(w["rho"], w["g"], w.get("depth", float("inf")))
+    if a.symmetry:
+        s.set_symmetry(True, True)
     s.set_batch(a.batch)
     if a.gemm >= 0:
         s.set_tuning(s.TUNE_LU_GEMM, a.gemm)
     if a.depth >= 0:
         s.set_tuning(s.TUNE_LU_DEPTH, a.depth)
     print(f"[bench] workload {w['name']}: {N} panels, {F} frequencies", file=sys.stderr, flush=True)
-    paths = [p for p in a.paths.split(",") if p]
+    paths = [p for p in a.paths.split(",") if p and not (a.symmetry and p == "mixed_nn")]
     peaks = {"fp64": PEAK_FP64, "mixed": PEAK_FP32, "mixed_nn": PEAK_FP32}
     res = {}
     if "mixed_nn" in paths:
@@ -347,6 +361,9 @@ def main():
               "path": PATH_DESC[main_path], "value_path": main_path,
               "value_rule": "fastest path with A, B, X within 1e-5 of the plain FP64 GPU results",
               "batch": a.batch or "auto"}
+    if a.symmetry:
+        config["symmetry_planes"] = "x = 0 and y = 0"
+        config["panels_whole_body"] = 4 * N
     for p in paths:
         config[p] = res[p][1]
     if "mixed_nn" in paths:
@@ -371,7 +388,7 @@ def main():
                      {"gemm_variant": gv, "gemm_tiles": VARIANTS[gv][0],
                       "panels_per_update": a.depth if a.depth >= 0 else 4,
                       "batch": a.batch or None})
-    traffic = TRAFFIC.get((a.workload, main_path, tun["gemm_tiles"], tun["panels_per_update"], tun["batch"]))
+    traffic = TRAFFIC.get((wkey, main_path, tun["gemm_tiles"], tun["panels_per_update"], tun["batch"]))
     tiles, kernel, split = VARIANTS[tun["gemm_variant"]]
     kernel = kernel.replace("<T,", "<double," if main_path == "fp64" else "<float,")
     # the split-FP16 GEMM is priced against the FP16 matrix peak (each FP32 product is three FP16
@@ -403,7 +420,7 @@ def main():
                               else "assemble_kernel", "achieved": summ["assembly_achieved_gbs"],
                               "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                               "frac": round(summ["assembly_achieved_gbs"] * 1e9 / PEAK_HBM, 4),
-                              "traffic": ASM_TRAFFIC.get(a.workload),
+                              "traffic": ASM_TRAFFIC.get(wkey),
                               "ms_per_launch": summ["stage_ms_per_frequency"]["assembly_ms"]},
     }
     if not a.no_cpu_baseline:

@@ -2,7 +2,8 @@
 mesh solved without it, for the planes x = 0, y = 0 and both, FP64 and mixed precision, infinite and
 finite depth, headings that break the symmetry. The character systems and the whole mesh are the
 same problem, so the two agree to the solver's rounding; the whole mesh itself is held to the FP64
-oracle in the parity tests, and once here."""
+oracle in the parity tests, and once here. The reference holds its sectors to the whole body within
+1e-6 (tests/python/test_low_order.py:143-155)."""
 
 import sys
 
@@ -60,6 +61,8 @@ CASES = [
     ("cylinder", 3.0, 1, 1),
     ("cylinder", 3.0, 0, 1),
     ("box", 2.0, 1, 0),
+    ("cylinder_large", 3.0, 1, 1),  # a 320-panel sector: five 64-column LU panels per character system
+    ("cylinder_large", float("inf"), 1, 0),
 ]
 
 
@@ -70,6 +73,7 @@ def test_sector_solve_equals_whole_mesh(sessions, shape, depth, sym_x, sym_y, pr
         "hemisphere": lambda: meshes.hemisphere_panels(1.0, 6, 7, quarter=True),
         "cylinder": lambda: meshes.cylinder_panels(1.0, 0.8, 9, 4, 3, quarter=True),
         "box": lambda: meshes.box_panels(1.0, 0.6, 0.5, (5, 3, 3), quarter=True),
+        "cylinder_large": lambda: meshes.cylinder_panels(1.0, 1.0, 16, 12, 8, quarter=True, cosine=True),
     }[shape]()
     sector, whole = sector_and_whole(quadrant, sym_x, sym_y)
     centre = np.array([[0.3, -0.2, -0.1]])  # off the planes: the modes of the whole body still hold
