@@ -23,9 +23,9 @@ void launch_statics(const PanelD* P, int N, int Nsrc, int logint, double tol, do
 void launch_forces_sol(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w,
                        double rho, const double2* sol, int lds, const double2* phiI, const double2* dphiI, double* out,
                        hipStream_t stream);
-void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, int g, const double* chi, double2* S,
+void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, double2* S,
                              double2* M, size_t mstride, float2* A32, hipStream_t stream);
-void launch_symmetry_project(const double2* u, int N, int G, int R, int c, const double* chi, double2* uc,
+void launch_symmetry_project(const double2* u, int N, int G, int R, const double* chi, double2* uc,
                              hipStream_t stream);
 void launch_symmetry_recombine(const double2* M, size_t mstride, int N, int G, int R, const double* chi, double2* phi,
                                hipStream_t stream);
@@ -52,6 +52,8 @@ void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, 
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
                      hipStream_t stream);
+void launch_rhs_batched(const double2* S, size_t sstride, int N, int lds, const double2* u, size_t ustride, int R,
+                        double2* M, size_t mstride, int ldm, double2* part, int nsys, hipStream_t stream);
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,
                 hipStream_t stream);
 int rhs_chunks(int N);
@@ -450,20 +452,21 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   if (G > 1 && (surrogate || lda != N || !mstride))
     throw Error{HBG_ERR_UNSUPPORTED, "symmetry planes with the surrogate or a padded system"};
   if (G > 1) {
-    s->d_Sg.ensure((size_t)N * N);
-    s->d_Ag.ensure((size_t)N * N);
+    s->d_Sg.ensure((size_t)N * N * G);
+    s->d_Ag.ensure((size_t)N * N * G);
   }
   auto blocks = [&](auto one) {
     if (G == 1) {
       one(s->d_panels.p, s->d_st.p, s->d_S.p, Mout, A32, 2.0 * PI);
       return;
     }
+    // each image block into its own buffer, then the G characters in one pass
     for (int g = 0; g < G; ++g) {
-      one(s->d_panels.p + (size_t)g * N, s->d_st.p + (size_t)g * N * N, s->d_Sg.p, s->d_Ag.p, (float2*)nullptr,
+      const size_t o = (size_t)g * N * N;
+      one(s->d_panels.p + (size_t)g * N, s->d_st.p + o, s->d_Sg.p + o, s->d_Ag.p + o, (float2*)nullptr,
           g == 0 ? 2.0 * PI : 0.0);
-      launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, g, s->d_chi.p, s->d_S.p, Mout, mstride,
-                              g == G - 1 ? A32 : nullptr, s->stream);
     }
+    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->d_S.p, Mout, mstride, A32, s->stream);
   };
   if (!std::isfinite(s->depth)) {
     blocks([&](const PanelD* Psrc, const StaticD* st, double2* S, double2* Ab, float2* a32, double free_term) {
@@ -1017,10 +1020,10 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   s->d_phiI.ensure(hstride * B * halves);
   s->d_dphiI.ensure(hstride * B * halves);
   s->d_u.ensure((size_t)R * NF);
-  s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
+  s->d_part.ensure((size_t)rhs_chunks(N) * R * N * G);
   s->d_out.ensure(nout * B * halves);
   if (G > 1) {
-    s->d_uc.ensure((size_t)R * N);
+    s->d_uc.ensure((size_t)R * N * G);
     s->d_phif.ensure((size_t)R * NF);
   }
   if (mixed) {
@@ -1059,11 +1062,10 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     if (G == 1) {
       launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, sa);
     } else {
-      // character c: S_c times the projection of u (lo_frequency.rs:873-893)
-      for (int c = 0; c < G; ++c) {
-        launch_symmetry_project(s->d_u.p, N, G, R, c, s->d_chi.p, s->d_uc.p, sa);
-        launch_rhs(s->d_S.p + (size_t)c * N * N, N, N, s->d_uc.p, R, Mb + mstride * c, N, s->d_part.p, sa);
-      }
+      // every character c: S_c times the projection of u (lo_frequency.rs:873-893), one launch
+      launch_symmetry_project(s->d_u.p, N, G, R, s->d_chi.p, s->d_uc.p, sa);
+      launch_rhs_batched(s->d_S.p, (size_t)N * N, N, N, s->d_uc.p, (size_t)R * N, R, Mb, mstride, N, s->d_part.p, G,
+                         sa);
     }
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 2], sa));
   };

@@ -290,22 +290,27 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
   }
 }
 
-// rhs(:, c) = S U(:, c), U = -u (lo_frequency.rs:855-893 with G = 1), in two passes: every
-// workgroup multiplies a block of 256 rows of S by one chunk of RHS_CHUNK columns (one row per
-// thread, S read column by column, coalesced along the rows, u wavefront-uniform), the partial
-// sums [chunk][c][row] are then added in chunk order (deterministic), into the augmented columns
-// of M.
+// rhs(:, c) = S U(:, c), U = -u (lo_frequency.rs:855-893), in two passes: every workgroup
+// multiplies a block of 256 rows of S by one chunk of columns (one row per thread, S read column
+// by column, coalesced along the rows, u wavefront-uniform), the partial sums [chunk][c][row] are
+// then added in chunk order (deterministic), into the augmented columns of M. Several systems
+// (the characters of a symmetric body) go in one launch, one per blockIdx.z. Chunks of 128
+// columns up to order 4096 and 256 above keep at least 256 workgroups in flight from order 2048.
 constexpr int MAXRHS = 32;
-constexpr int RHS_CHUNK = 256;
+int rhs_chunk_len(int N) { return N <= 4096 ? 128 : 256; }
 // RM: right-hand sides held in registers (R <= RM); the columns of S go 8 at a time so that eight
 // loads are in flight per lane (the kernel streams S once; its rate is that of the loads in flight),
 // and the wavefront-uniform u of 64 columns at a time is staged in LDS and read as broadcasts.
 constexpr int RHS_SUB = 64;
 template <int RM>
-__global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restrict__ S, int N, int lds,
-                                                          const double2* __restrict__ u, int R,
-                                                          double2* __restrict__ part) {
+__global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restrict__ S, size_t sstride, int N,
+                                                          int lds, const double2* __restrict__ u, size_t ustride,
+                                                          int R, int chunk_len, double2* __restrict__ part,
+                                                          size_t pstride) {
   __shared__ double2 us[RHS_SUB][RM];
+  S += blockIdx.z * sstride;
+  u += blockIdx.z * ustride;
+  part += blockIdx.z * pstride;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int chunk = blockIdx.y;
   const bool row = k < N;
@@ -323,7 +328,7 @@ __global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restr
       }
     }
   };
-  const int l0 = chunk * RHS_CHUNK, l1 = min(N, l0 + RHS_CHUNK);
+  const int l0 = chunk * chunk_len, l1 = min(N, l0 + chunk_len);
   for (int s0 = l0; s0 < l1; s0 += RHS_SUB) {
     const int ns = min(RHS_SUB, l1 - s0);
     __syncthreads();
@@ -348,10 +353,12 @@ __global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restr
       if (c < R) part[((size_t)chunk * R + c) * N + k] = acc[c];
 }
 
-__global__ void rhs_reduce_kernel(const double2* __restrict__ part, int N, int R, int chunks, double2* __restrict__ M,
-                                  int ldm) {
+__global__ void rhs_reduce_kernel(const double2* __restrict__ part, size_t pstride, int N, int R, int chunks,
+                                  double2* __restrict__ M, size_t mstride, int ldm) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)R * N) return;
+  part += blockIdx.y * pstride;
+  M += blockIdx.y * mstride;
   const int c = (int)(e / N), k = (int)(e % N);
   double2 s = make_double2(0.0, 0.0);
   for (int q = 0; q < chunks; ++q) {
@@ -496,47 +503,61 @@ void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFr
   HBG_HIP(hipGetLastError());
 }
 
-// The character systems of a symmetric body from its image blocks (lo_frequency.rs:855-943): image g
-// (S_g, A_g of the block (k, g n)) adds chi_c(g) times itself into S_c and the matrix part of M_c for
-// every character c (g = 0 sets them); with A32 (the last image) the complex64 copy of M_c too.
-// chi: [G][G] row c, column g; M_c at M + c mstride (leading dimension N), S_c at S + c N N.
-__global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const double2* __restrict__ Ag, int N, int G,
-                                        int g, const double* __restrict__ chi, double2* __restrict__ S,
+// The character systems of a symmetric body from its image blocks (lo_frequency.rs:855-943):
+// S_c = sum_g chi_c(g) S_g and the matrix part of M_c = sum_g chi_c(g) A_g for every character c,
+// with A32 (nullable) the complex64 copy of M_c. One pass: each entry of the G blocks is read once
+// and each character entry written once. chi: [G][G] row c, column g; S_g, A_g at Sg + g N N,
+// Ag + g N N; M_c at M + c mstride (leading dimension N), S_c at S + c N N.
+template <int G>
+__global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const double2* __restrict__ Ag, int N,
+                                        const double* __restrict__ chi, double2* __restrict__ S,
                                         double2* __restrict__ M, size_t mstride, float2* __restrict__ A32) {
+  const size_t NN = (size_t)N * N;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * N) return;
-  const double2 sg = Sg[e], ag = Ag[e];
+  if (e >= NN) return;
+  double2 sg[G], ag[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    sg[g] = Sg[g * NN + e];
+    ag[g] = Ag[g * NN + e];
+  }
+#pragma unroll
   for (int c = 0; c < G; ++c) {
-    const double x = chi[c * G + g];
-    double2* s = S + (size_t)c * N * N + e;
-    double2* m = M + (size_t)c * mstride + e;
-    double2 sv = make_double2(x * sg.x, x * sg.y), mv = make_double2(x * ag.x, x * ag.y);
-    if (g > 0) {
-      const double2 s0 = *s, m0 = *m;
-      sv = make_double2(s0.x + sv.x, s0.y + sv.y);
-      mv = make_double2(m0.x + mv.x, m0.y + mv.y);
+    double2 sv = make_double2(0.0, 0.0), mv = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const double x = chi[c * G + g];
+      sv.x += x * sg[g].x;
+      sv.y += x * sg[g].y;
+      mv.x += x * ag[g].x;
+      mv.y += x * ag[g].y;
     }
-    *s = sv;
-    *m = mv;
-    if (A32) A32[(size_t)c * mstride + e] = float2((float)mv.x, (float)mv.y);
+    S[c * NN + e] = sv;
+    M[c * mstride + e] = mv;
+    if (A32) A32[c * mstride + e] = float2((float)mv.x, (float)mv.y);
   }
 }
 
-void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, int g, const double* chi, double2* S,
+void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, double2* S,
                              double2* M, size_t mstride, float2* A32, hipStream_t stream) {
-  symmetry_combine_kernel<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, stream>>>(Sg, Ag, N, G, g, chi, S, M,
-                                                                                      mstride, A32);
+  const unsigned blocks = (unsigned)(((size_t)N * N + 255) / 256);
+  if (G == 2)
+    symmetry_combine_kernel<2><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, S, M, mstride, A32);
+  else if (G == 4)
+    symmetry_combine_kernel<4><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, S, M, mstride, A32);
+  else
+    throw Error{6, "symmetry groups have 2 or 4 elements"};
   HBG_HIP(hipGetLastError());
 }
 
-// The right-hand-side values of character c (lo_frequency.rs:873-880): uc[col][n] =
+// The right-hand-side values of every character c (lo_frequency.rs:873-880): uc[c][col][n] =
 // (1 / G) sum_g chi_c(g) u[col][g N + n] for the R columns of u on the G N panels of the whole body
 // (the S U product negates them).
-__global__ void symmetry_project_kernel(const double2* __restrict__ u, int N, int G, int R, int c,
+__global__ void symmetry_project_kernel(const double2* __restrict__ u, int N, int G, int R,
                                         const double* __restrict__ chi, double2* __restrict__ uc) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * R) return;
-  const int n = (int)(e % N), col = (int)(e / N);
+  if (e >= (size_t)N * R * G) return;
+  const int n = (int)(e % N), col = (int)(e / N % R), c = (int)(e / ((size_t)N * R));
   double2 v = make_double2(0.0, 0.0);
   for (int g = 0; g < G; ++g) {
     const double x = chi[c * G + g];
@@ -547,9 +568,9 @@ __global__ void symmetry_project_kernel(const double2* __restrict__ u, int N, in
   uc[e] = make_double2(v.x / G, v.y / G);
 }
 
-void launch_symmetry_project(const double2* u, int N, int G, int R, int c, const double* chi, double2* uc,
+void launch_symmetry_project(const double2* u, int N, int G, int R, const double* chi, double2* uc,
                              hipStream_t stream) {
-  symmetry_project_kernel<<<(unsigned)(((size_t)N * R + 255) / 256), 256, 0, stream>>>(u, N, G, R, c, chi, uc);
+  symmetry_project_kernel<<<(unsigned)(((size_t)N * R * G + 255) / 256), 256, 0, stream>>>(u, N, G, R, chi, uc);
   HBG_HIP(hipGetLastError());
 }
 
@@ -598,7 +619,7 @@ void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, 
   HBG_HIP(hipGetLastError());
 }
 
-int rhs_chunks(int N) { return (N + RHS_CHUNK - 1) / RHS_CHUNK; }
+int rhs_chunks(int N) { return (N + rhs_chunk_len(N) - 1) / rhs_chunk_len(N); }
 
 // The matrices of a frequency limit in infinite depth (low_order.rs:635-637, 757-758: the wave
 // terms and the log terms vanish, K = 0): S = rs, D = rd (1/r + 1/r') at omega = 0, and with
@@ -626,18 +647,25 @@ void launch_pad_system(double2* M, float2* M32, int n, int nz, int R, int rz, hi
 }
 
 // part: rhs_chunks(N) * R * N values of scratch
+void launch_rhs_batched(const double2* S, size_t sstride, int N, int lds, const double2* u, size_t ustride, int R,
+                        double2* M, size_t mstride, int ldm, double2* part, int nsys, hipStream_t stream) {
+  const int chunks = rhs_chunks(N), len = rhs_chunk_len(N);
+  const size_t pstride = (size_t)chunks * R * N;
+  const dim3 g((N + 255) / 256, chunks, nsys);
+  if (R <= 8)
+    rhs_partial_kernel<8><<<g, 256, 0, stream>>>(S, sstride, N, lds, u, ustride, R, len, part, pstride);
+  else if (R <= 16)
+    rhs_partial_kernel<16><<<g, 256, 0, stream>>>(S, sstride, N, lds, u, ustride, R, len, part, pstride);
+  else
+    rhs_partial_kernel<MAXRHS><<<g, 256, 0, stream>>>(S, sstride, N, lds, u, ustride, R, len, part, pstride);
+  rhs_reduce_kernel<<<dim3((unsigned)(((size_t)R * N + 255) / 256), nsys), 256, 0, stream>>>(part, pstride, N, R,
+                                                                                           chunks, M, mstride, ldm);
+  HBG_HIP(hipGetLastError());
+}
+
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,
                 hipStream_t stream) {
-  const int chunks = rhs_chunks(N);
-  const dim3 g((N + 255) / 256, chunks);
-  if (R <= 8)
-    rhs_partial_kernel<8><<<g, 256, 0, stream>>>(S, N, lds, u, R, part);
-  else if (R <= 16)
-    rhs_partial_kernel<16><<<g, 256, 0, stream>>>(S, N, lds, u, R, part);
-  else
-    rhs_partial_kernel<MAXRHS><<<g, 256, 0, stream>>>(S, N, lds, u, R, part);
-  rhs_reduce_kernel<<<(unsigned)(((size_t)R * N + 255) / 256), 256, 0, stream>>>(part, N, R, chunks, M, ldm);
-  HBG_HIP(hipGetLastError());
+  launch_rhs_batched(S, 0, N, lds, u, 0, R, M, 0, ldm, part, 1, stream);
 }
 
 // the solution of [A | B] in the columns [N, N + R) of M (leading dimension ldm)
