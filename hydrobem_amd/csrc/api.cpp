@@ -32,7 +32,7 @@ void launch_symmetry_recombine(const double2* M, size_t mstride, int N, int G, i
 void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2* S, double2* A, hipStream_t stream);
 void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
                          double logc, const StaticD* st, double2* S, double2* A, float2* A32, int lda,
-                         double free_term, hipStream_t stream);
+                         double free_term, hipStream_t stream, int images = 1);
 void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
                             const double* zeta, int n, double* out, hipStream_t stream);
 size_t fin_tables_work();
@@ -47,7 +47,7 @@ void launch_adaptive(const PanelD* P, int N, const double* gl, StaticD* st, cons
                      hipStream_t stream);
 void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
                      const StaticD* st, double2* S, double2* A, float2* A32, int lda, double free_term,
-                     hipStream_t stream);
+                     hipStream_t stream, int images = 1);
 void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream);
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
@@ -457,20 +457,17 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   }
   auto blocks = [&](auto one) {
     if (G == 1) {
-      one(s->d_panels.p, s->d_st.p, s->d_S.p, Mout, A32, 2.0 * PI);
+      one(s->d_S.p, Mout, A32, 1);
       return;
     }
-    // each image block into its own buffer, then the G characters in one pass
-    for (int g = 0; g < G; ++g) {
-      const size_t o = (size_t)g * N * N;
-      one(s->d_panels.p + (size_t)g * N, s->d_st.p + o, s->d_Sg.p + o, s->d_Ag.p + o, (float2*)nullptr,
-          g == 0 ? 2.0 * PI : 0.0);
-    }
+    // the G image blocks in one launch, each into its own buffer, then the G characters in one pass
+    one(s->d_Sg.p, s->d_Ag.p, (float2*)nullptr, G);
     launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->d_S.p, Mout, mstride, A32, s->stream);
   };
   if (!std::isfinite(s->depth)) {
-    blocks([&](const PanelD* Psrc, const StaticD* st, double2* S, double2* Ab, float2* a32, double free_term) {
-      launch_assemble(s->d_panels.p, Psrc, N, K, logc, s->T, st, S, Ab, a32, lda, free_term, s->stream);
+    blocks([&](double2* S, double2* Ab, float2* a32, int images) {
+      launch_assemble(s->d_panels.p, s->d_panels.p, N, K, logc, s->T, s->d_st.p, S, Ab, a32, lda, 2.0 * PI, s->stream,
+                      images);
     });
     return K;
   }
@@ -491,9 +488,9 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
     launch_assemble_fin_nn(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32,
                            lda, s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
   } else
-    blocks([&](const PanelD* Psrc, const StaticD* st, double2* S, double2* Ab, float2* a32, double free_term) {
-      launch_assemble_fin(s->d_panels.p, Psrc, N, s->d_ff.p, fin_tables(s), s->T, logc, st, S, Ab, a32, lda, free_term,
-                          s->stream);
+    blocks([&](double2* S, double2* Ab, float2* a32, int images) {
+      launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32,
+                          lda, 2.0 * PI, s->stream, images);
     });
   return f.k0;
 }

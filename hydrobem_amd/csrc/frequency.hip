@@ -107,6 +107,14 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
                                                        double free_term) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
+  if (blockIdx.z) {  // image block g of a symmetric body: sources g n, no free term
+    const size_t g = blockIdx.z;
+    Psrc += g * N;
+    st += g * N * N;
+    S += g * N * N;
+    A += g * N * lda;
+    free_term = 0.0;
+  }
   __shared__ PairSort L;
   static_assert(XT * YT + XT + 1 <= PS_KEYS, "infinite-depth keys");
   const unsigned total = sort_tile_pairs(L, bi, bj, N, [&](int k, int n) {
@@ -167,6 +175,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
                                                            int lda, double free_term) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
+  if (blockIdx.z) {  // image block g of a symmetric body: sources g n, no free term
+    const size_t g = blockIdx.z;
+    Psrc += g * N;
+    st += g * N * N;
+    S += g * N * N;
+    A += g * N * lda;
+    free_term = 0.0;
+  }
   __shared__ PairSort L;
   static_assert(FS_NEAR + 1024 <= PS_KEYS, "finite-depth keys");
   const double h = f->h, ih = 1.0 / h;
@@ -485,21 +501,27 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
 }
 
 // A32 (nullable): A in complex64 too. Psrc: the source panels (P: the block of the panels with
-// themselves; the images of P under a reflection: that image block, whose pairs keep the symmetry
-// used here since |c_k - g c_n| = |c_n - g c_k|, with st its statics and free_term 0).
+// themselves). images > 1 (a symmetric body): blockIdx.z = g assembles the image block g, the
+// sources Psrc + g N (P reflected by g), its statics st + g N N, into S + g N N and A + g N lda,
+// with no free term for g > 0; its pairs keep the symmetry used here since |c_k - g c_n| =
+// |c_n - g c_k| for a reflection. One launch for all blocks fills the chip at sector orders where
+// one block alone would not (528 workgroups at order 2048).
 void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
                      const StaticD* st, double2* S, double2* A, float2* A32, int lda, double free_term,
-                     hipStream_t stream) {
+                     hipStream_t stream, int images) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
-  assemble_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, Psrc, N, K, logc, T, st, S, A, A32, lda, free_term);
+  if (images > 1 && A32) throw Error{6, "image blocks without a complex64 copy"};
+  assemble_kernel<<<dim3(nt, nt, images), 256, 0, stream>>>(P, Psrc, N, K, logc, T, st, S, A, A32, lda, free_term);
   HBG_HIP(hipGetLastError());
 }
 
 void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
                          double logc, const StaticD* st, double2* S, double2* A, float2* A32, int lda,
-                         double free_term, hipStream_t stream) {
+                         double free_term, hipStream_t stream, int images) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
-  assemble_fin_kernel<<<dim3(nt, nt), 256, 0, stream>>>(P, Psrc, N, f, F, T, logc, st, S, A, A32, lda, free_term);
+  if (images > 1 && A32) throw Error{6, "image blocks without a complex64 copy"};
+  assemble_fin_kernel<<<dim3(nt, nt, images), 256, 0, stream>>>(P, Psrc, N, f, F, T, logc, st, S, A, A32, lda,
+                                                                free_term);
   HBG_HIP(hipGetLastError());
 }
 
