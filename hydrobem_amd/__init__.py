@@ -64,6 +64,7 @@ SIGNATURES = {
     "hbg_set_environment": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "hbg_set_panels": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip, ctypes.c_int32, _dp]),
     "hbg_set_symmetry": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "hbg_set_walls": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_solve": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                    ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
                                    ctypes.c_void_p]),
@@ -182,6 +183,11 @@ class Session:
         """Symmetry planes x = 0 (sym_x) and y = 0 (sym_y): the panels set are one sector of the
         body and the solves return the whole body's quantities."""
         _check(library().hbg_set_symmetry(self._h, int(bool(sym_x)), int(bool(sym_y))))
+
+    def set_walls(self, wall_x=False, wall_y=False):
+        """Reflecting walls x = 0 (wall_x) and y = 0 (wall_y), the bodies on their positive side:
+        the solves return the real bodies' quantities in the walled domain."""
+        _check(library().hbg_set_walls(self._h, int(bool(wall_x)), int(bool(wall_y))))
 
     def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None, precision="fp64",
               surrogate=False):

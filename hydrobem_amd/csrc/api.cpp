@@ -23,7 +23,7 @@ void launch_statics(const PanelD* P, int N, int Nsrc, int logint, double tol, do
 void launch_forces_sol(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w,
                        double rho, const double2* sol, int lds, const double2* phiI, const double2* dphiI, double* out,
                        hipStream_t stream);
-void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, double2* S,
+void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, int nchar, double2* S,
                              double2* M, size_t mstride, float2* A32, hipStream_t stream);
 void launch_symmetry_project(const double2* u, int N, int G, int R, const double* chi, double2* uc,
                              hipStream_t stream);
@@ -51,7 +51,7 @@ void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, doubl
 void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream);
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
-                     hipStream_t stream);
+                     hipStream_t stream, int walls = 0);
 void launch_rhs_batched(const double2* S, size_t sstride, int N, int lds, const double2* u, size_t ustride, int R,
                         double2* M, size_t mstride, int ldm, double2* part, int nsys, hipStream_t stream);
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,
@@ -315,6 +315,10 @@ struct HbgSession {
   // symmetry planes (lo_geometry.rs:25-35): G = 1, 2 or 4 reflections in the order e, sigma_x
   // (x -> -x), sigma_y, sigma_x sigma_y (those present), chi[c][g] the characters
   int sym_x = 0, sym_y = 0, G = 1;
+  // walls (lo_geometry.rs:947-1031; bit 0: x = 0, bit 1: y = 0): the reflections are then those
+  // of the walls, the image panels move as the mirror image of the body (their n_j copied), only
+  // the character even under every wall is solved and the forces are those on the real panels
+  int walls = 0;
   std::vector<std::array<double, 2>> refl{{1.0, 1.0}};
   std::vector<double> chi{1.0};
   std::vector<double> raw_vertices, raw_centres;  // the panels as set, to rebuild the images
@@ -462,7 +466,8 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
     }
     // the G image blocks in one launch, each into its own buffer, then the G characters in one pass
     one(s->d_Sg.p, s->d_Ag.p, (float2*)nullptr, G);
-    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->d_S.p, Mout, mstride, A32, s->stream);
+    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->walls ? 1 : G, s->d_S.p, Mout, mstride, A32,
+                            s->stream);
   };
   if (!std::isfinite(s->depth)) {
     blocks([&](double2* S, double2* Ab, float2* a32, int images) {
@@ -629,6 +634,7 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
 }
 
 void build_geometry(HbgSession* s);
+static void set_reflections(HbgSession* s, int rx, int ry);
 
 HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
                          int32_t n_bodies, const double* rotation_centres) {
@@ -658,6 +664,12 @@ void build_geometry(HbgSession* s) {
   for (int g = 0; g < G; ++g)
     for (int j = 0; j < n_panels; ++j) {
       const double sx = s->refl[g][0], sy = s->refl[g][1];
+      if (g == 0 && s->walls)  // the body in the fluid: x >= 0 (a wall at x = 0), y >= 0 (y = 0)
+        for (int q = 0; q < 4; ++q)
+          for (int a = 0; a < 2; ++a)
+            if ((s->walls >> a & 1) && s->raw_vertices[12 * (size_t)j + 3 * q + a] < 0.0)
+              throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": behind the wall " +
+                                             (a ? "y" : "x") + " = 0"};
       double v[12];
       for (int q = 0; q < 4; ++q) {
         const int src = (sx * sy < 0.0 && q > 0) ? 4 - q : q;  // 0, 3, 2, 1 where the orientation flips
@@ -678,6 +690,10 @@ void build_geometry(HbgSession* s) {
       const int bi = s->raw_body[j];
       if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
       b[o] = bi;
+      if (s->walls && g > 0) {  // a wall image moves as the mirror image of its panel (lo_geometry.rs:981)
+        for (int d = 0; d < 6; ++d) nj[6 * o + d] = nj[6 * (size_t)j + d];
+        continue;
+      }
       V3 rc = v3(s->raw_centres[3 * bi], s->raw_centres[3 * bi + 1], s->raw_centres[3 * bi + 2]);
       V3 mom = cross(centroid - rc, normal);
       for (int d = 0; d < 3; ++d) {
@@ -702,13 +718,43 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_symmetry");
   if ((sym_x != 0 && sym_x != 1) || (sym_y != 0 && sym_y != 1))
     return fail(HBG_ERR_VALUE, "sym_x and sym_y must be 0 or 1", "hbg_set_symmetry");
+  if ((sym_x || sym_y) && s->walls)
+    return fail(HBG_ERR_UNSUPPORTED, "symmetry planes together with walls", "hbg_set_symmetry");
   GUARD_BEGIN
   s->sym_x = sym_x;
   s->sym_y = sym_y;
+  set_reflections(s, sym_x, sym_y);
+  return HBG_OK;
+  GUARD_END("hbg_set_symmetry")
+}
+
+HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_walls");
+  if ((wall_x != 0 && wall_x != 1) || (wall_y != 0 && wall_y != 1))
+    return fail(HBG_ERR_VALUE, "wall_x and wall_y must be 0 or 1", "hbg_set_walls");
+  if ((wall_x || wall_y) && (s->sym_x || s->sym_y))
+    return fail(HBG_ERR_UNSUPPORTED, "walls together with symmetry planes", "hbg_set_walls");
+  GUARD_BEGIN
+  const int before = s->walls;
+  s->walls = wall_x | wall_y << 1;
+  try {
+    set_reflections(s, wall_x, wall_y);
+  } catch (...) {  // panels behind a new wall: the walls as they were
+    s->walls = before;
+    set_reflections(s, before & 1, before >> 1);
+    throw;
+  }
+  return HBG_OK;
+  GUARD_END("hbg_set_walls")
+}
+
+// The reflections e, sigma_x, sigma_y, sigma_x sigma_y (those present) and the characters of their
+// group, for symmetry planes or walls; rebuilds the device panels when panels are set.
+static void set_reflections(HbgSession* s, int rx, int ry) {
   s->refl.assign(1, {1.0, 1.0});
-  if (sym_x) s->refl.push_back({-1.0, 1.0});
-  if (sym_y) s->refl.push_back({1.0, -1.0});
-  if (sym_x && sym_y) s->refl.push_back({-1.0, -1.0});
+  if (rx) s->refl.push_back({-1.0, 1.0});
+  if (ry) s->refl.push_back({1.0, -1.0});
+  if (rx && ry) s->refl.push_back({-1.0, -1.0});
   s->G = (int)s->refl.size();
   // the characters of the group (lo_geometry.rs:25-35): row c, column g
   if (s->G == 1)
@@ -721,8 +767,6 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
   s->d_chi.ensure(s->chi.size());
   HBG_HIP(hipMemcpy(s->d_chi.p, s->chi.data(), s->chi.size() * sizeof(double), hipMemcpyHostToDevice));
   if (s->N > 0) build_geometry(s);
-  return HBG_OK;
-  GUARD_END("hbg_set_symmetry")
 }
 
 HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64_t n, double* out) {
@@ -981,8 +1025,10 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
   // a symmetric body (G > 1): every frequency is G character systems of order N (the sector),
-  // consecutive in the batch buffers; the boundary values and the forces take the whole body
-  const int G = s->G, NF = G * N;
+  // consecutive in the batch buffers; the boundary values and the forces take the whole body.
+  // With walls one system of order N (the even character), the boundary values and the forces on
+  // the real panels, the wall images only in the matrices.
+  const int G = s->walls ? 1 : s->G, NF = G * N;
   const double* const omegas_all = omegas;
   omegas = om.data();
   n_omegas = (int32_t)om.size();
@@ -1054,7 +1100,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0,
                                          mixed ? s->d_M32.p + mstride * slot(h, b) * G : nullptr, N, mstride);
     launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
-                    s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b), s->d_u.p, sa);
+                    s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b), s->d_u.p, sa, s->walls);
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], sa));
     if (G == 1) {
       launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, sa);

@@ -266,10 +266,12 @@ __global__ void wave_points_kernel(WaveTables T, const double* __restrict__ X, c
 // Incident wave and boundary values (lo_frequency.rs:214-273, low_order.rs:1458-1468, frame at
 // the origin): phi_I = (i g / w) e^{k z} e^{-i k (x cos b + y sin b)}; u = n_j on the body of the
 // radiation column, -dphi_I/dn for each heading (the scattered formulation).
-// phiI, dphiI: [H][N]; u: [R][N], R = nrad + H, nrad = 6 nbodies.
+// phiI, dphiI: [H][N]; u: [R][N], R = nrad + H, nrad = 6 nbodies. walls (bit 0: x = 0, bit 1:
+// y = 0): the incident wave includes its reflections in the walls, the sum of the wave at the
+// reflected points with the gradient reflected back (low_order.rs:1470-1486).
 __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restrict__ body,
                                 const double* __restrict__ nj, int N, int nrad, int H, double w, double k, double grav,
-                                double depth, const double* __restrict__ heads, double2* __restrict__ phiI,
+                                double depth, const double* __restrict__ heads, int walls, double2* __restrict__ phiI,
                                 double2* __restrict__ dphiI, double2* __restrict__ u) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
@@ -287,17 +289,27 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
   for (int h = 0; h < H; ++h) {
     double sb, cb;
     sincos(heads[h], &sb, &cb);
-    const double ph = -k * (x * cb + y * sb);
-    double sp, cp;
-    sincos(ph, &sp, &cp);
-    // e = (i g / w) (cos ph + i sin ph)
     const double gw = grav / w;
-    const double2 e = make_double2(-gw * sp, gw * cp);
-    const double2 phi = make_double2(e.x * Z, e.y * Z);
-    // gradient: phi (-i k cb), phi (-i k sb), e Zz
-    const double2 g0 = make_double2(phi.y * k * cb, -phi.x * k * cb);
-    const double2 g1 = make_double2(phi.y * k * sb, -phi.x * k * sb);
-    const double2 g2 = make_double2(e.x * Zz, e.y * Zz);
+    double2 phi = make_double2(0.0, 0.0), g0 = phi, g1 = phi, g2 = phi;
+    for (int r = 0; r < 4; ++r) {  // the identity, then the wall reflections x, y, xy present
+      if ((r & walls) != r) continue;
+      const double sx = (r & 1) ? -1.0 : 1.0, sy = (r & 2) ? -1.0 : 1.0;
+      const double ph = -k * (sx * x * cb + sy * y * sb);
+      double sp, cp;
+      sincos(ph, &sp, &cp);
+      // e = (i g / w) (cos ph + i sin ph)
+      const double2 e = make_double2(-gw * sp, gw * cp);
+      const double2 f = make_double2(e.x * Z, e.y * Z);
+      phi.x += f.x;
+      phi.y += f.y;
+      // gradient: phi (-i k cb), phi (-i k sb), e Zz at the reflected point, reflected back
+      g0.x += sx * f.y * k * cb;
+      g0.y -= sx * f.x * k * cb;
+      g1.x += sy * f.y * k * sb;
+      g1.y -= sy * f.x * k * sb;
+      g2.x += e.x * Zz;
+      g2.y += e.y * Zz;
+    }
     const double2 dn = make_double2(g0.x * p->n[0] + g1.x * p->n[1] + g2.x * p->n[2],
                                     g0.y * p->n[0] + g1.y * p->n[1] + g2.y * p->n[2]);
     phiI[(size_t)h * N + j] = phi;
@@ -527,12 +539,13 @@ void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFr
 
 // The character systems of a symmetric body from its image blocks (lo_frequency.rs:855-943):
 // S_c = sum_g chi_c(g) S_g and the matrix part of M_c = sum_g chi_c(g) A_g for every character c,
-// with A32 (nullable) the complex64 copy of M_c. One pass: each entry of the G blocks is read once
-// and each character entry written once. chi: [G][G] row c, column g; S_g, A_g at Sg + g N N,
+// with A32 (nullable) the complex64 copy of M_c, for the first nchar characters (all of them for a
+// symmetric body; with walls, the first only: the one even under every wall reflection). One pass:
+// each entry of the G blocks is read once and each character entry written once. chi: [G][G] row c, column g; S_g, A_g at Sg + g N N,
 // Ag + g N N; M_c at M + c mstride (leading dimension N), S_c at S + c N N.
 template <int G>
 __global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const double2* __restrict__ Ag, int N,
-                                        const double* __restrict__ chi, double2* __restrict__ S,
+                                        const double* __restrict__ chi, int nchar, double2* __restrict__ S,
                                         double2* __restrict__ M, size_t mstride, float2* __restrict__ A32) {
   const size_t NN = (size_t)N * N;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -545,6 +558,7 @@ __global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const do
   }
 #pragma unroll
   for (int c = 0; c < G; ++c) {
+    if (c >= nchar) break;
     double2 sv = make_double2(0.0, 0.0), mv = make_double2(0.0, 0.0);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -560,13 +574,14 @@ __global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const do
   }
 }
 
-void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, double2* S,
-                             double2* M, size_t mstride, float2* A32, hipStream_t stream) {
+void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, int nchar,
+                             double2* S, double2* M, size_t mstride, float2* A32, hipStream_t stream) {
   const unsigned blocks = (unsigned)(((size_t)N * N + 255) / 256);
+  if (nchar < 1 || nchar > G) throw Error{6, "characters out of range"};
   if (G == 2)
-    symmetry_combine_kernel<2><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, S, M, mstride, A32);
+    symmetry_combine_kernel<2><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, nchar, S, M, mstride, A32);
   else if (G == 4)
-    symmetry_combine_kernel<4><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, S, M, mstride, A32);
+    symmetry_combine_kernel<4><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, nchar, S, M, mstride, A32);
   else
     throw Error{6, "symmetry groups have 2 or 4 elements"};
   HBG_HIP(hipGetLastError());
@@ -635,9 +650,9 @@ void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, 
 
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
-                     hipStream_t stream) {
-  boundary_kernel<<<(N + 255) / 256, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, k, grav, depth, heads, phiI,
-                                                       dphiI, u);
+                     hipStream_t stream, int walls) {
+  boundary_kernel<<<(N + 255) / 256, 256, 0, stream>>>(P, body, nj, N, nrad, H, w, k, grav, depth, heads, walls,
+                                                       phiI, dphiI, u);
   HBG_HIP(hipGetLastError());
 }
 

@@ -104,6 +104,17 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
  * hbg_assemble, hbg_solve_group, the surrogate and the frequency limits are HBG_ERR_UNSUPPORTED. */
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
 
+/* Walls (the domain x_wall, y_wall, orthogonal_walls of model_codes.rs:15-31): wall_x 1 a
+ * reflecting vertical wall at x = 0, wall_y 1 at y = 0 (each 0 or 1). The panels of
+ * hbg_set_panels are the real bodies, in the fluid (every vertex x >= 0, resp. y >= 0, or
+ * HBG_ERR_VALUE); the solve adds their images in the walls, which move as the mirror image of the
+ * bodies (lo_geometry.rs:947-983), solves the character even under every wall (lo_geometry.rs:
+ * 1010-1030: one system of order N per frequency), takes the incident wave with its reflections
+ * (low_order.rs:1470-1486) and returns the quantities of the real bodies. Kept across
+ * hbg_set_panels. Walls together with symmetry planes, and with walls hbg_statics, hbg_assemble,
+ * hbg_solve_group, the surrogate and the frequency limits, are HBG_ERR_UNSUPPORTED. */
+HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
+
 /* The solve of n_omegas frequencies (rad/s) and n_headings headings (degrees), radiation of the
  * six rigid modes of every body plus diffraction (solver::run, lo_frequency.rs:1565-1635).
  * D = 6 n_bodies. Outputs (any may be NULL):

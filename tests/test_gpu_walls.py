@@ -1,0 +1,207 @@
+"""Walls x = 0, y = 0 and both (GPU): the solve in the walled domain against the open sea with the
+mirror images of the body as bodies of their own, which is the same problem written out.
+
+In the walled domain the image bodies move as the mirror image of the real one and the incident
+wave is the sum of the wave at the reflected points (lo_geometry.rs:947-983, low_order.rs:
+1470-1486). In the open sea, with the image body g (reflection diag(sx, sy, 1)) given its own six
+rigid modes about the reflected rotation centre, the mirrored motion of mode j is s_g(j) times the
+image's mode j: s = (sx, sy, 1) for the translations and det(sigma) (sx, sy, 1) for the rotations
+(a reflected moment arm crossed with a reflected normal is det(sigma) sigma of the moment). So
+
+    A_wall(i, j) = sum_g s_g(j) A_open(body 0 mode i, body g mode j), the same for B,
+    X_wall(beta) = sum_g X_open(body 0, beta_g), beta_g the heading reflected by g (and FK, SC),
+
+with every image body present as an obstacle in each open-sea problem. The open-sea path itself
+is held to the FP64 oracle in the parity tests, and directly here once. The Haskind excitation of
+the walled problem is checked against its diffraction excitation (Green's theorem holds with the
+walls, where both potentials satisfy the wall condition) at the discretization level."""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import hydrobem_amd as hb
+from hydrobem_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_per_entry(a, b):
+    b = np.asarray(b)
+    peak = np.abs(b).max()
+    scale = np.maximum(np.abs(b).max(axis=0, keepdims=True), 1e-3 * peak)
+    return float((np.abs(np.asarray(a) - b) / scale).max())
+
+
+def reflections(wx, wy):
+    r = [(1.0, 1.0)]
+    if wx:
+        r.append((-1.0, 1.0))
+    if wy:
+        r.append((1.0, -1.0))
+    if wx and wy:
+        r.append((-1.0, -1.0))
+    return r
+
+
+def reflect(v, sx, sy):
+    r = np.asarray(v, dtype=float) * np.array([sx, sy, 1.0])
+    return r[:, [0, 3, 2, 1]] if sx * sy < 0 else r
+
+
+def mirror_signs(sx, sy):
+    det = sx * sy
+    return np.array([sx, sy, 1.0, det * sx, det * sy, det])
+
+
+def open_sea_equivalent(v, centre, wx, wy):
+    """The body and its wall images as separate bodies (the images about the reflected centres)."""
+    refl = reflections(wx, wy)
+    verts = np.concatenate([reflect(v, sx, sy) for sx, sy in refl])
+    body = np.repeat(np.arange(len(refl), dtype=np.int32), len(v))
+    centres = np.array([[sx * centre[0], sy * centre[1], centre[2]] for sx, sy in refl])
+    return refl, verts, body, centres
+
+
+def headings_reflected(heads_deg, refl):
+    out = []
+    for b in np.radians(heads_deg):
+        for sx, sy in refl:
+            out.append(math.degrees(math.atan2(sy * math.sin(b), sx * math.cos(b))))
+    return out
+
+
+def wall_from_open(o, refl, H):
+    G = len(refl)
+    s = np.stack([mirror_signs(*r) for r in refl])  # [G][6]
+    out = {}
+    for key in ("added_mass", "radiation_damping"):
+        a = o[key]  # [F][6G][6G]
+        out[key] = sum(a[:, :6, 6 * g:6 * g + 6] * s[g][None, None, :] for g in range(G))
+    for key in ("excitation_diffraction", "excitation_froude_krylov", "excitation_scattering"):
+        x = o[key][:, :, :6].reshape(len(o[key]), H, G, 6)  # headings [beta][g]
+        out[key] = x.sum(axis=2)
+    return out
+
+
+@pytest.fixture(scope="module")
+def session():
+    s = hb.Session(0)
+    yield s
+    s.set_walls(False, False)
+    s.close()
+
+
+BODIES = {
+    "box": lambda: meshes.place(meshes.box_panels(1.0, 0.5, 0.5, (6, 3, 3), quarter=False), (2.0, 1.5, 0.0)),
+    "hemisphere": lambda: meshes.place(meshes.hemisphere_panels(0.8, 6, 16, quarter=False), (1.5, 2.5, 0.0)),
+}
+CASES = [
+    ("box", float("inf"), 1, 0, "fp64"),
+    ("box", float("inf"), 0, 1, "mixed"),
+    ("box", 3.0, 1, 1, "fp64"),
+    ("hemisphere", float("inf"), 1, 1, "mixed"),
+    ("hemisphere", 2.5, 1, 0, "mixed"),
+    ("hemisphere", 2.5, 0, 1, "fp64"),
+]
+
+
+@pytest.mark.parametrize("shape,depth,wx,wy,precision", CASES)
+def test_walls_against_the_open_sea_with_images(session, shape, depth, wx, wy, precision):
+    v = BODIES[shape]()
+    centre = np.array([v[:, :, 0].mean(), v[:, :, 1].mean(), -0.1])
+    heads = [0.0, 40.0]
+    w = np.array([0.6, 1.3, 2.2])
+    refl, verts, body, centres = open_sea_equivalent(v, centre, wx, wy)
+    say(f"case: {shape} {len(v)} panels, walls x {wx} y {wy}, depth {depth}, {precision}; open sea with "
+        f"{len(refl)} bodies, {len(refl) * len(heads)} headings")
+    s = session
+    s.set_environment(1025.0, 9.81, depth)
+    s.set_walls(False, False)
+    s.set_panels(verts, body=body, rotation_centres=centres)
+    o = s.solve(w, headings_reflected(heads, refl), precision=precision)
+    ref = wall_from_open(o, refl, len(heads))
+    s.set_panels(v, rotation_centres=centre[None])  # the real body first: the images lie behind the walls
+    s.set_walls(wx, wy)
+    got = s.solve(w, heads, precision=precision)
+    for key, r in ref.items():
+        e = rel_per_entry(got[key], r)
+        say(f"  {key}: walls vs open sea with images {e:.2e}")
+        assert e < 1e-9
+    say(f"  Haskind vs diffraction excitation: walls "
+        f"{rel_per_entry(got['excitation_haskind'], got['excitation_diffraction']):.2e}, open sea with images "
+        f"{rel_per_entry(o['excitation_haskind'], o['excitation_diffraction']):.2e}")
+    s.set_walls(False, False)
+
+
+@pytest.mark.parametrize("wx,wy", [(1, 0), (0, 1), (1, 1)])
+def test_wall_haskind_relation_converges(session, wx, wy):
+    # the Haskind and the pressure excitation differ by the discretization error only: the gap
+    # falls when the panels are halved
+    s = session
+    s.set_environment(1025.0, 9.81, float("inf"))
+    w, heads = np.array([0.8, 1.5]), [40.0]
+    gaps = []
+    for n in ((4, 2, 2), (8, 4, 4), (16, 8, 8)):
+        v = meshes.place(meshes.box_panels(1.0, 0.5, 0.5, n, quarter=False), (2.0, 1.5, 0.0))
+        s.set_walls(False, False)
+        s.set_panels(v, rotation_centres=np.array([[2.0, 1.5, -0.1]]))
+        s.set_walls(wx, wy)
+        got = s.solve(w, heads)
+        X, HK = got["excitation_diffraction"], got["excitation_haskind"]
+        gaps.append(float(np.abs(HK - X).max() / np.abs(X).max()))
+    s.set_walls(False, False)
+    say(f"case: box, walls x {wx} y {wy}, |HK - X| / max |X| at {len(v) // 16}, {len(v) // 4}, {len(v)} panels: "
+        f"{gaps}")
+    assert gaps[1] < 0.7 * gaps[0] and gaps[2] < 0.7 * gaps[1]
+
+
+def test_walls_against_the_oracle(session, oracle):
+    v = meshes.place(meshes.box_panels(0.6, 0.4, 0.4, (3, 2, 2), quarter=False), (1.0, 0.8, 0.0))
+    centre = np.array([1.0, 0.8, -0.1])
+    refl, verts, body, centres = open_sea_equivalent(v, centre, 1, 1)
+    heads = [30.0]
+    w = np.array([0.9, 1.7])
+    say(f"case: box {len(v)} panels, walls x and y, FP64 vs the oracle's open sea with the images")
+    orc = oracle.Problem(verts, body=body, rotation_centres=centres, rho=1025.0, grav=9.81).solve(
+        w, headings_reflected(heads, refl))
+    ref = wall_from_open(orc, refl, len(heads))
+    s = session
+    s.set_environment(1025.0, 9.81, float("inf"))
+    s.set_walls(False, False)
+    s.set_panels(v, rotation_centres=centre[None])
+    s.set_walls(True, True)
+    got = s.solve(w, heads)
+    s.set_walls(False, False)
+    for key in ("added_mass", "radiation_damping", "excitation_diffraction"):
+        e = rel_per_entry(got[key], ref[key])
+        say(f"  {key}: vs oracle {e:.2e}")
+        assert e < 1e-10
+
+
+def test_wall_refusals(session):
+    s = session
+    s.set_environment(1025.0, 9.81, float("inf"))
+    s.set_walls(False, False)
+    s.set_symmetry(False, False)
+    v = meshes.place(meshes.box_panels(0.6, 0.4, 0.4, (3, 2, 2), quarter=False), (0.3, 0.8, 0.0))  # x in [-0.3, 0.9]
+    say("case: walls with a body behind the wall, with symmetry planes, with the statics and the limits")
+    s.set_panels(v)
+    with pytest.raises(hb.HbgError) as e:
+        s.set_walls(True, False)
+    assert e.value.status == 1
+    s.set_walls(False, True)  # y >= 0 holds: accepted
+    with pytest.raises(hb.HbgError) as e:
+        s.set_symmetry(True, False)
+    assert e.value.status == 3
+    for call in (lambda: s.statics(), lambda: s.solve([0.0], [])):
+        with pytest.raises(hb.HbgError) as e:
+            call()
+        assert e.value.status == 3
+    s.set_walls(False, False)
