@@ -65,6 +65,8 @@ SIGNATURES = {
     "hbg_set_panels": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip, ctypes.c_int32, _dp]),
     "hbg_set_symmetry": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_set_walls": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "hbg_set_field_points": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
+    "hbg_field_values": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "hbg_solve": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                    ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
                                    ctypes.c_void_p]),
@@ -143,7 +145,7 @@ class Session:
     """One geometry on one GPU: panels, environment, statics cached on the device.
 
     The Python face of the reference's solve (hb.Model.solve, python/hydrobem/engine.py:286-301)
-    for wetted panels in infinite depth without symmetry planes.
+    for wetted panels in infinite or finite depth, with symmetry planes or walls, and field points.
     """
 
     def __init__(self, device: int = 0):
@@ -151,6 +153,7 @@ class Session:
         _check(library().hbg_session_create(int(device), ctypes.byref(self._h)))
         self.n_panels = 0
         self.n_bodies = 0
+        self.n_points = 0
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -183,6 +186,20 @@ class Session:
         """Symmetry planes x = 0 (sym_x) and y = 0 (sym_y): the panels set are one sector of the
         body and the solves return the whole body's quantities."""
         _check(library().hbg_set_symmetry(self._h, int(bool(sym_x)), int(bool(sym_y))))
+
+    def set_field_points(self, points=None):
+        """Points [n][3] (z <= 0) whose elevations every later solve evaluates (None: none)."""
+        p = np.zeros((0, 3)) if points is None else np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        _check(library().hbg_set_field_points(self._h, _p(p) if len(p) else None, len(p)))
+        self.n_points = len(p)
+
+    def field_values(self, n_omegas, n_headings):
+        """The field values of the last solve: frad [F][P][K], finc, fsca, ftot [F][P][H]."""
+        P, K = self.n_points, 6 * self.n_bodies
+        frad = np.zeros((n_omegas, P, K), np.complex128)
+        finc, fsca, ftot = (np.zeros((n_omegas, P, n_headings), np.complex128) for _ in range(3))
+        _check(library().hbg_field_values(self._h, _p(frad), _p(finc), _p(fsca), _p(ftot)))
+        return {"frad": frad, "finc": finc, "fsca": fsca, "ftot": ftot}
 
     def set_walls(self, wall_x=False, wall_y=False):
         """Reflecting walls x = 0 (wall_x) and y = 0 (wall_y), the bodies on their positive side:

@@ -18,8 +18,9 @@
 namespace hbg {
 
 // kernels (statics.hip, frequency.hip, lu.hip)
-void launch_statics(const PanelD* P, int N, int Nsrc, int logint, double tol, double depth, const double* gl,
-                    StaticD* st, int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream);
+void launch_statics(const PanelD* P, const PanelD* Pf, int N, int Nsrc, int logint, double tol, double depth,
+                    const double* gl, StaticD* st, int2* queue, unsigned* qcount, unsigned qcap, double2* inf,
+                    hipStream_t stream);
 void launch_forces_sol(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w,
                        double rho, const double2* sol, int lds, const double2* phiI, const double2* dphiI, double* out,
                        hipStream_t stream);
@@ -43,8 +44,12 @@ FinFreq make_fin_freq(double h, double K);
 double dispersion_root(double H, int n);
 std::vector<double> fit_matrix(int M);
 std::vector<double> build_k_table();
-void launch_adaptive(const PanelD* P, int N, const double* gl, StaticD* st, const int2* queue, unsigned count,
-                     hipStream_t stream);
+void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nrad,
+                         int H, double w, double K, double logc, WaveTables T, double grav, const double* heads,
+                         const StaticD* stf, double2* Sx, double2* Ax, const double2* sol, const double2* phiI,
+                         double2* out, hipStream_t stream);
+void launch_adaptive(const PanelD* P, const PanelD* Pf, int N, const double* gl, StaticD* st, const int2* queue,
+                     unsigned count, hipStream_t stream);
 void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
                      const StaticD* st, double2* S, double2* A, float2* A32, int lda, double free_term,
                      hipStream_t stream, int images = 1);
@@ -349,6 +354,18 @@ struct HbgSession {
   bool statics_valid = false;
   bool stinf_valid = false;  // d_stinf holds the omega = infinity row of the statics
   DevBuf<double2> d_stinf;
+  // field points (lo_frequency.rs:1392-1495): the points as records (their centroids), their
+  // statics rows [N][np], the per-frequency influence S_x, A_x [N][np], the values of the batch,
+  // and those of the last solve on the host ([F][np][nrad + 3 H] complex)
+  int np = 0;
+  DevBuf<PanelD> d_fpts;
+  DevBuf<StaticD> d_stf;
+  bool stf_valid = false;
+  int stf_logint = -1;
+  double stf_tol = -1.0;
+  DevBuf<double2> d_Sx, d_Ax, d_fout;
+  std::vector<double2> field_vals;
+  int field_F = -1, field_H = 0, field_nrad = 0;
   int st_logint = -1;
   double st_tol = -1.0;
   WaveTables T;
@@ -399,13 +416,13 @@ void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false
     s->d_queue.ensure(qcap);
     s->d_qcount.ensure(1);
     HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
-    launch_statics(s->d_panels.p, N, NF, logint, tol, std::isfinite(s->depth) ? s->depth : 0.0, s->d_gl.p,
+    launch_statics(s->d_panels.p, s->d_panels.p, N, NF, logint, tol, std::isfinite(s->depth) ? s->depth : 0.0, s->d_gl.p,
                    s->d_st.p, s->d_queue.p, s->d_qcount.p, qcap, want_inf ? s->d_stinf.p : nullptr, s->stream);
     unsigned count = 0;
     HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
     if (count <= qcap) {
-      launch_adaptive(s->d_panels.p, N, s->d_gl.p, s->d_st.p, s->d_queue.p, count, s->stream);
+      launch_adaptive(s->d_panels.p, s->d_panels.p, N, s->d_gl.p, s->d_st.p, s->d_queue.p, count, s->stream);
       if (s->profiling) HBG_HIP(hipEventRecord(s->ev[1], s->stream));
       HBG_HIP(hipStreamSynchronize(s->stream));
       if (s->profiling) {
@@ -423,6 +440,34 @@ void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false
     qcap = count;
   }
   throw Error{HBG_ERR_INTERNAL, "statics: adaptive queue overflow"};
+}
+
+// The statics rows of the field points (the same kernels, the points as field records).
+void ensure_field_statics(HbgSession* s, int logint, double tol) {
+  if (s->stf_valid && s->stf_logint == logint && s->stf_tol == tol) return;
+  const int N = s->N, np = s->np;
+  s->d_stf.ensure((size_t)N * np);
+  unsigned qcap = (unsigned)std::min<size_t>((size_t)np * 64 + 64, (size_t)1 << 28);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    s->d_queue.ensure(qcap);
+    s->d_qcount.ensure(1);
+    HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
+    launch_statics(s->d_panels.p, s->d_fpts.p, np, N, logint, tol, 0.0, s->d_gl.p, s->d_stf.p, s->d_queue.p,
+                   s->d_qcount.p, qcap, nullptr, s->stream);
+    unsigned count = 0;
+    HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipStreamSynchronize(s->stream));
+    if (count <= qcap) {
+      launch_adaptive(s->d_panels.p, s->d_fpts.p, np, s->d_gl.p, s->d_stf.p, s->d_queue.p, count, s->stream);
+      HBG_HIP(hipStreamSynchronize(s->stream));
+      s->stf_valid = true;
+      s->stf_logint = logint;
+      s->stf_tol = tol;
+      return;
+    }
+    qcap = count;
+  }
+  throw Error{HBG_ERR_INTERNAL, "the adaptive queue of the field points did not fit"};
 }
 
 // the depths of the panels must lie in the water: -h <= z < 0 at the centroids
@@ -626,7 +671,7 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
   if (!(density > 0.0) || !std::isfinite(density)) return fail(HBG_ERR_VALUE, "density must be > 0", "hbg_set_environment");
   if (!(gravity > 0.0) || !std::isfinite(gravity)) return fail(HBG_ERR_VALUE, "gravity must be > 0", "hbg_set_environment");
   if (!(depth > 0.0)) return fail(HBG_ERR_VALUE, "depth must be > 0", "hbg_set_environment");
-  if (depth != s->depth) s->statics_valid = false;
+  if (depth != s->depth) s->statics_valid = s->stf_valid = false;
   s->rho = density;
   s->grav = gravity;
   s->depth = depth;
@@ -712,6 +757,7 @@ void build_geometry(HbgSession* s) {
   HBG_HIP(hipMemcpy(s->d_body.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(s->d_nj.p, nj.data(), nj.size() * sizeof(double), hipMemcpyHostToDevice));
   s->statics_valid = false;
+  s->stf_valid = false;
 }
 
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
@@ -726,6 +772,53 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
   set_reflections(s, sym_x, sym_y);
   return HBG_OK;
   GUARD_END("hbg_set_symmetry")
+}
+
+HbgStatus hbg_set_field_points(HbgSession* s, const double* points, int32_t n) {
+  if (!s || (n > 0 && !points)) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_set_field_points");
+  if (n < 0) return fail(HBG_ERR_ARGUMENT, "negative count", "hbg_set_field_points");
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(points[3 * i]) || !std::isfinite(points[3 * i + 1]) || !(points[3 * i + 2] <= 0.0))
+      return fail(HBG_ERR_VALUE, "field point " + std::to_string(i) + ": not finite or above z = 0",
+                  "hbg_set_field_points");
+  GUARD_BEGIN
+  use_device(s);
+  s->np = n;
+  s->stf_valid = false;
+  s->field_F = -1;
+  if (n > 0) {
+    std::vector<PanelD> rec(n);
+    std::memset(rec.data(), 0, rec.size() * sizeof(PanelD));
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) rec[i].c[d] = points[3 * i + d];
+    s->d_fpts.ensure(n);
+    HBG_HIP(hipMemcpy(s->d_fpts.p, rec.data(), n * sizeof(PanelD), hipMemcpyHostToDevice));
+  }
+  return HBG_OK;
+  GUARD_END("hbg_set_field_points")
+}
+
+HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fsca, double* ftot) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_field_values");
+  if (s->field_F < 0) return fail(HBG_ERR_STATE, "no solve with field points yet", "hbg_field_values");
+  const int F = s->field_F, P = s->np, K = s->field_nrad, H = s->field_H, W = K + 3 * H;
+  for (int f = 0; f < F; ++f)
+    for (int p = 0; p < P; ++p) {
+      const double2* v = s->field_vals.data() + ((size_t)f * P + p) * W;
+      for (int c = 0; c < K; ++c)
+        if (frad) {
+          frad[(((size_t)f * P + p) * K + c) * 2] = v[c].x;
+          frad[(((size_t)f * P + p) * K + c) * 2 + 1] = v[c].y;
+        }
+      double* outs[3] = {finc, fsca, ftot};
+      for (int q = 0; q < 3; ++q)
+        if (outs[q])
+          for (int h = 0; h < H; ++h) {
+            outs[q][(((size_t)f * P + p) * H + h) * 2] = v[K + q * H + h].x;
+            outs[q][(((size_t)f * P + p) * H + h) * 2 + 1] = v[K + q * H + h].y;
+          }
+    }
+  return HBG_OK;
 }
 
 HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
@@ -1024,6 +1117,9 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
   if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
+  if (s->np > 0 && (std::isfinite(s->depth) || s->G > 1 || !il.empty()))
+    throw Error{HBG_ERR_UNSUPPORTED, "field points in finite depth, with symmetry planes or walls, or at the "
+                                     "frequency limits"};
   // a symmetric body (G > 1): every frequency is G character systems of order N (the sector),
   // consecutive in the batch buffers; the boundary values and the forces take the whole body.
   // With walls one system of order N (the even character), the boundary values and the forces on
@@ -1034,6 +1130,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   n_omegas = (int32_t)om.size();
   if (progress && progress(user, HBG_STAGE_STATICS, 0, 1) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
   ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance, want_inf);
+  if (s->np > 0) ensure_field_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
   std::vector<double> heads(std::max(H, 1), 0.0);
   for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
   s->d_heads.ensure(heads.size());
@@ -1079,6 +1176,17 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     s->d_active.ensure((size_t)BG);
   }
   std::vector<double> hout(nout * B);
+  // field values: per frequency [np][nrad + 3 H] on the device, all frequencies on the host
+  const size_t fw = (size_t)nrad + 3 * (size_t)H, fstride = (size_t)s->np * fw;
+  std::vector<double2> hfield;
+  if (s->np > 0) {
+    s->d_Sx.ensure((size_t)N * s->np);
+    s->d_Ax.ensure((size_t)N * s->np);
+    s->d_fout.ensure(fstride * B * halves);
+    hfield.resize(fstride * B);
+    s->field_vals.assign(fstride * (size_t)n_omegas, make_double2(0.0, 0.0));
+    s->field_F = -1;
+  }
   std::vector<int> iters(BG);
   std::vector<double> eta(BG);
   if (prof && s->bev.size() < (size_t)(3 * B + 3)) {
@@ -1224,6 +1332,17 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       }
     }
     if (prof) HBG_HIP(hipEventRecord(s->bev[e1], sb));
+    if (s->np > 0) {
+      for (int b = 0; b < nb; ++b) {
+        const double w = omegas[f0 + b];
+        launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, N, nrad, H, w,
+                            w * w / s->grav, logc, s->T, s->grav, s->d_heads.p, s->d_stf.p, s->d_Sx.p, s->d_Ax.p,
+                            Mh + mstride * b + (size_t)N * N, s->d_phiI.p + hstride * slot(h, b),
+                            s->d_fout.p + fstride * slot(h, b), sb);
+      }
+      HBG_HIP(hipMemcpyAsync(hfield.data(), s->d_fout.p + fstride * slot(h, 0), nb * fstride * sizeof(double2),
+                             hipMemcpyDeviceToHost, sb));
+    }
     HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p + nout * slot(h, 0), nb * nout * sizeof(double),
                            hipMemcpyDeviceToHost, sb));
     HBG_HIP(hipStreamSynchronize(sb));
@@ -1256,7 +1375,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       if (froude_krylov) std::memcpy(froude_krylov + fi * 2 * nX, hFK, 2 * nX * sizeof(double));
       if (scattering) std::memcpy(scattering + fi * 2 * nX, hSC, 2 * nX * sizeof(double));
       if (haskind) std::memcpy(haskind + fi * 2 * nHK, hHK, 2 * nHK * sizeof(double));
+      if (s->np > 0)
+        std::copy(hfield.begin() + fstride * b, hfield.begin() + fstride * (b + 1), s->field_vals.begin() + fstride * fi);
     }
+  }
+  if (s->np > 0) {
+    s->field_F = n_omegas;
+    s->field_H = H;
+    s->field_nrad = nrad;
   }
   // the limits: added mass only (damping and the diffraction quantities are zero there)
   for (int i : il) {

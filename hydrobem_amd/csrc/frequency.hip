@@ -537,6 +537,105 @@ void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFr
   HBG_HIP(hipGetLastError());
 }
 
+// Field values (lo_frequency.rs:1392-1495, infinite depth): the influence of panel j at point k,
+// S_x and the dipole part as the assembly writes it (A_x = -D_x, no free term: the points are off
+// the panels), from the statics rows of the points st[j * Np + k] and the same wave kernel. Not
+// the pair-symmetric tiles of the assembly (points and panels differ): 64 points per wavefront
+// along k, one panel per wavefront, so the panel record and its tile reads are wavefront-uniform.
+__global__ void __launch_bounds__(256) field_influence_kernel(const PanelD* __restrict__ Pf,
+                                                              const PanelD* __restrict__ P, int Np, int N, double K,
+                                                              double logc, WaveTables T,
+                                                              const StaticD* __restrict__ st, double2* __restrict__ Sx,
+                                                              double2* __restrict__ Ax) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  if (j >= N) return;
+  const bool valid = k < Np;
+  const PanelD* pk = Pf + (valid ? k : 0);
+  const PanelD* pn = P + j;
+  const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
+  const double R = sqrt(dx * dx + dy * dy);
+  const double X = K * R, Y = -K * (pk->c[2] + pn->c[2]);
+  const WaveK w = wave_kernel_u(T, valid, X, Y);
+  if (!valid) return;
+  const double K2 = K * K, c2 = 2.0 * K * logc;
+  const double R0 = sqrt(X * X + Y * Y);
+  const double FY = -w.F - 2.0 / R0, IY = -w.I;
+  const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
+  const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
+  const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+  put_entry(Sx, Ax, nullptr, Np, k, j, st[(size_t)j * Np + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ,
+            0.0);
+}
+
+// The field values of one frequency at every point k (lo_frequency.rs:1461-1491): radiation column
+// c: frad = (w^2 / g) (1 / 4 pi) sum_j (phi_j D_x - S_x u_j) with u the body's n_j; heading h
+// (the scattered formulation): phi_S = (1 / 4 pi) sum_j (phi_S + phi_I)_j D_x, and the elevations
+// eta = -(i w / g) phi of the incident wave at the point, the scattered wave and their sum.
+// sol: the N x R solution columns (leading dimension N); phiI: [H][N] on the panels; out:
+// [Np][nrad + 3 H], per point frad (nrad), finc, fsca, ftot (H each).
+__global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict__ Pf, const int* __restrict__ body,
+                                                        const double* __restrict__ nj, int Np, int N, int nrad, int H,
+                                                        double w, double k, double grav,
+                                                        const double* __restrict__ heads,
+                                                        const double2* __restrict__ Sx,
+                                                        const double2* __restrict__ Ax,
+                                                        const double2* __restrict__ sol,
+                                                        const double2* __restrict__ phiI, double2* __restrict__ out) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  const int c = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  const int R = nrad + H;
+  if (p >= Np || c >= R) return;
+  double vr = 0.0, vi = 0.0;
+  const double2* phi = sol + (size_t)c * N;
+  if (c < nrad) {
+    for (int j = 0; j < N; ++j) {
+      const double2 a = Ax[(size_t)j * Np + p], sv = Sx[(size_t)j * Np + p], f = phi[j];
+      const double u = body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0;
+      // phi D - S u with D = -A
+      vr += -(f.x * a.x - f.y * a.y) - sv.x * u;
+      vi += -(f.x * a.y + f.y * a.x) - sv.y * u;
+    }
+    const double sc = w * w / grav / (4.0 * PI);
+    out[(size_t)p * (nrad + 3 * H) + c] = make_double2(sc * vr, sc * vi);
+    return;
+  }
+  const int h = c - nrad;
+  const double2* pI = phiI + (size_t)h * N;
+  for (int j = 0; j < N; ++j) {
+    const double2 a = Ax[(size_t)j * Np + p], f = make_double2(phi[j].x + pI[j].x, phi[j].y + pI[j].y);
+    vr += -(f.x * a.x - f.y * a.y);
+    vi += -(f.x * a.y + f.y * a.x);
+  }
+  // phi_I at the point (infinite depth): (i g / w) e^{k z} e^{-i k (x cos b + y sin b)}
+  const PanelD* q = Pf + p;
+  double sb, cb, sp, cp;
+  sincos(heads[h], &sb, &cb);
+  sincos(-k * (q->c[0] * cb + q->c[1] * sb), &sp, &cp);
+  const double Z = exp(k * q->c[2]), gw = grav / w;
+  const double2 pi_ = make_double2(-gw * sp * Z, gw * cp * Z);
+  const double2 ps = make_double2(vr / (4.0 * PI), vi / (4.0 * PI));
+  // eta = -(i w / g) phi
+  const double e = w / grav;
+  const double2 eI = make_double2(e * pi_.y, -e * pi_.x), eS = make_double2(e * ps.y, -e * ps.x);
+  double2* o = out + (size_t)p * (nrad + 3 * H) + nrad;
+  o[h] = eI;
+  o[H + h] = eS;
+  o[2 * H + h] = make_double2(eI.x + eS.x, eI.y + eS.y);
+}
+
+void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nrad,
+                         int H, double w, double K, double logc, WaveTables T, double grav, const double* heads,
+                         const StaticD* stf, double2* Sx, double2* Ax, const double2* sol, const double2* phiI,
+                         double2* out, hipStream_t stream) {
+  const dim3 block(64, 4);
+  field_influence_kernel<<<dim3((Np + 63) / 64, (N + 3) / 4), block, 0, stream>>>(Pf, P, Np, N, K, logc, T, stf, Sx,
+                                                                                 Ax);
+  field_sum_kernel<<<dim3((Np + 63) / 64, (nrad + H + 3) / 4), block, 0, stream>>>(
+      Pf, body, nj, Np, N, nrad, H, w, K, grav, heads, Sx, Ax, sol, phiI, out);
+  HBG_HIP(hipGetLastError());
+}
+
 // The character systems of a symmetric body from its image blocks (lo_frequency.rs:855-943):
 // S_c = sum_g chi_c(g) S_g and the matrix part of M_c = sum_g chi_c(g) A_g for every character c,
 // with A32 (nullable) the complex64 copy of M_c, for the first nchar characters (all of them for a

@@ -140,7 +140,8 @@ __device__ __forceinline__ bool log_pair_d(const PanelD* __restrict__ p, P3 x, d
 // gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]. inf (optional, infinite depth):
 // the omega = infinity row of the pair, the integrals of 1/r - 1/r' (lo_static.rs:89-96), at
 // inf[j * N + k] as (source, dipole).
-__global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, int N, int Nsrc, int logint,
+__global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, const PanelD* __restrict__ Pf,
+                                                      int N, int Nsrc, int logint,
                                                       double tol, double depth, const double* __restrict__ gl,
                                                       StaticD* __restrict__ st, int2* __restrict__ queue,
                                                       unsigned* __restrict__ qcount, unsigned qcap,
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
   if (j >= Nsrc || k >= N) return;
   const PanelD* p = P + j;
-  const P3 x = ld3(P[k].c);
+  const P3 x = ld3(Pf[k].c);
   double s0, d0, sb, db;
   rankine_d(p, x, gl, s0, d0);
   const P3 xb = mk(x.x, x.y, -x.z);
@@ -206,7 +207,7 @@ __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__
 // :436-449, depth-first in the reference's order of the quarters, to at most 14 levels.
 constexpr int ASTACK = 48;  // 3 * 14 + 1 entries suffice for a depth-first quadrisection of depth 14
 
-__global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__ P, int N,
+__global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__ P, const PanelD* __restrict__ Pf, int N,
                                                       const double* __restrict__ gl, StaticD* __restrict__ st,
                                                       const int2* __restrict__ queue, unsigned count) {
   __shared__ double stk[ASTACK][7];  // u0, u1, v0, v1, prev L, prev Ld, depth
@@ -217,7 +218,7 @@ __global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__
   const int2 kj = queue[item];
   const int k = kj.x, j = kj.y;
   const PanelD* p = P + j;
-  const P3 x = ld3(P[k].c);
+  const P3 x = ld3(Pf[k].c);
   const P3 xb = mk(x.x, x.y, -x.z);
   const P3 n = ld3(p->n);
   const double dist = nrm(sub(xb, ld3(p->c)));
@@ -309,21 +310,23 @@ __global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__
   }
 }
 
-// field points: the centroids of the first N panels; sources: the first Nsrc >= N panels (a
-// symmetric body: its N panels and their images); st[j * N + k] for the pair (field k, source j)
-void launch_statics(const PanelD* P, int N, int Nsrc, int logint, double tol, double depth, const double* gl,
-                    StaticD* st, int2* queue, unsigned* qcount, unsigned qcap, double2* inf, hipStream_t stream) {
+// field points: the centroids of the N records Pf (the panels themselves, or the points of the
+// field values); sources: the first Nsrc panels of P (a symmetric body or walls: its panels and
+// their images); st[j * N + k] for the pair (field k, source j)
+void launch_statics(const PanelD* P, const PanelD* Pf, int N, int Nsrc, int logint, double tol, double depth,
+                    const double* gl, StaticD* st, int2* queue, unsigned* qcount, unsigned qcap, double2* inf,
+                    hipStream_t stream) {
   if (inf && depth > 0.0) throw Error{HBG_ERR_UNSUPPORTED, "the omega = infinity statics in finite depth"};
   dim3 block(64, 4);
   dim3 grid((N + 63) / 64, (Nsrc + 3) / 4);
-  statics_kernel<<<grid, block, 0, stream>>>(P, N, Nsrc, logint, tol, depth, gl, st, queue, qcount, qcap, inf);
+  statics_kernel<<<grid, block, 0, stream>>>(P, Pf, N, Nsrc, logint, tol, depth, gl, st, queue, qcount, qcap, inf);
   HBG_HIP(hipGetLastError());
 }
 
-void launch_adaptive(const PanelD* P, int N, const double* gl, StaticD* st, const int2* queue, unsigned count,
-                     hipStream_t stream) {
+void launch_adaptive(const PanelD* P, const PanelD* Pf, int N, const double* gl, StaticD* st, const int2* queue,
+                     unsigned count, hipStream_t stream) {
   if (count == 0) return;
-  adaptive_kernel<<<count, 64, 0, stream>>>(P, N, gl, st, queue, count);
+  adaptive_kernel<<<count, 64, 0, stream>>>(P, Pf, N, gl, st, queue, count);
   HBG_HIP(hipGetLastError());
 }
 

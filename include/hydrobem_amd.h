@@ -115,6 +115,19 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
  * hbg_solve_group, the surrogate and the frequency limits, are HBG_ERR_UNSUPPORTED. */
 HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
 
+/* Field points (lo_frequency.rs:1392-1495, `Output` frad/finc/fsca/ftot, low_order.rs:1490-1500):
+ * points [n][3] on or below z = 0 (in the fluid, on the free surface, or inside a body), n = 0
+ * clears them. Every later hbg_solve (infinite depth, no symmetry planes or walls, no frequency
+ * limits: else HBG_ERR_UNSUPPORTED) also evaluates, per frequency and point, the radiation
+ * elevation of each rigid mode frad = (w^2 / g) phi_j(x) (per unit amplitude of motion) and the
+ * incident, scattered and total elevations eta = -(i w / g) phi of each heading. */
+HbgStatus hbg_set_field_points(HbgSession* s, const double* points, int32_t n);
+
+/* The field values of the last hbg_solve with field points: frad [F][n][6 nbodies], finc, fsca,
+ * ftot [F][n][H], complex (re, im) doubles; any output may be NULL. HBG_ERR_STATE before such a
+ * solve. */
+HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fsca, double* ftot);
+
 /* The solve of n_omegas frequencies (rad/s) and n_headings headings (degrees), radiation of the
  * six rigid modes of every body plus diffraction (solver::run, lo_frequency.rs:1565-1635).
  * D = 6 n_bodies. Outputs (any may be NULL):
