@@ -67,6 +67,8 @@ SIGNATURES = {
     "hbg_set_walls": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_set_field_points": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
     "hbg_field_values": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "hbg_set_kochin": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "hbg_kochin_values": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "hbg_solve": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                    ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
                                    ctypes.c_void_p]),
@@ -154,6 +156,7 @@ class Session:
         self.n_panels = 0
         self.n_bodies = 0
         self.n_points = 0
+        self.n_kochin = 0
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -200,6 +203,19 @@ class Session:
         finc, fsca, ftot = (np.zeros((n_omegas, P, n_headings), np.complex128) for _ in range(3))
         _check(library().hbg_field_values(self._h, _p(frad), _p(finc), _p(fsca), _p(ftot)))
         return {"frad": frad, "finc": finc, "fsca": fsca, "ftot": ftot}
+
+    def set_kochin(self, n_angles=0):
+        """Kochin functions on n_angles angles 2 pi t / n_angles in every later solve (0: none)."""
+        _check(library().hbg_set_kochin(self._h, int(n_angles)))
+        self.n_kochin = int(n_angles)
+
+    def kochin_values(self, n_omegas, n_headings):
+        """The Kochin functions of the last solve: kr, krt [F][NK][K], kd, kdt [F][NK][H]."""
+        NK, K = self.n_kochin, 6 * self.n_bodies
+        kr, krt = (np.zeros((n_omegas, NK, K), np.complex128) for _ in range(2))
+        kd, kdt = (np.zeros((n_omegas, NK, n_headings), np.complex128) for _ in range(2))
+        _check(library().hbg_kochin_values(self._h, _p(kr), _p(krt), _p(kd), _p(kdt)))
+        return {"kr": kr, "krt": krt, "kd": kd, "kdt": kdt}
 
     def set_walls(self, wall_x=False, wall_y=False):
         """Reflecting walls x = 0 (wall_x) and y = 0 (wall_y), the bodies on their positive side:

@@ -44,6 +44,8 @@ FinFreq make_fin_freq(double h, double K);
 double dispersion_root(double H, int n);
 std::vector<double> fit_matrix(int M);
 std::vector<double> build_k_table();
+void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double k, double depth,
+                   int NK, const double2* sol, const double2* phiI, double2* out, hipStream_t stream);
 void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nrad,
                          int H, double w, double K, double logc, WaveTables T, double grav, const double* heads,
                          const StaticD* stf, double2* Sx, double2* Ax, const double2* sol, const double2* phiI,
@@ -366,6 +368,12 @@ struct HbgSession {
   DevBuf<double2> d_Sx, d_Ax, d_fout;
   std::vector<double2> field_vals;
   int field_F = -1, field_H = 0, field_nrad = 0;
+  // Kochin functions (lo_frequency.rs:1279-1372): NK angles, the batch's values [2][NK][R] per
+  // frequency, those of the last solve on the host ([F][2][NK][R] complex)
+  int nk = 0;
+  DevBuf<double2> d_kout;
+  std::vector<double2> kochin_vals;
+  int kochin_F = -1, kochin_H = 0, kochin_nrad = 0;
   int st_logint = -1;
   double st_tol = -1.0;
   WaveTables T;
@@ -821,6 +829,38 @@ HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fs
   return HBG_OK;
 }
 
+HbgStatus hbg_set_kochin(HbgSession* s, int32_t n_angles) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_set_kochin");
+  if (n_angles < 0 || n_angles > 1 << 20) return fail(HBG_ERR_VALUE, "n_angles out of range", "hbg_set_kochin");
+  s->nk = n_angles;
+  s->kochin_F = -1;
+  return HBG_OK;
+}
+
+HbgStatus hbg_kochin_values(HbgSession* s, double* kr, double* krt, double* kd, double* kdt) {
+  if (!s) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_kochin_values");
+  if (s->kochin_F < 0) return fail(HBG_ERR_STATE, "no solve with Kochin angles yet", "hbg_kochin_values");
+  const int F = s->kochin_F, NK = s->nk, K = s->kochin_nrad, H = s->kochin_H, R = K + H;
+  for (int f = 0; f < F; ++f)
+    for (int d = 0; d < 2; ++d)
+      for (int t = 0; t < NK; ++t) {
+        const double2* v = s->kochin_vals.data() + (((size_t)f * 2 + d) * NK + t) * R;
+        double* rad = d ? krt : kr;
+        double* dif = d ? kdt : kd;
+        if (rad)
+          for (int c = 0; c < K; ++c) {
+            rad[(((size_t)f * NK + t) * K + c) * 2] = v[c].x;
+            rad[(((size_t)f * NK + t) * K + c) * 2 + 1] = v[c].y;
+          }
+        if (dif)
+          for (int h = 0; h < H; ++h) {
+            dif[(((size_t)f * NK + t) * H + h) * 2] = v[K + h].x;
+            dif[(((size_t)f * NK + t) * H + h) * 2 + 1] = v[K + h].y;
+          }
+      }
+  return HBG_OK;
+}
+
 HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_walls");
   if ((wall_x != 0 && wall_x != 1) || (wall_y != 0 && wall_y != 1))
@@ -1117,6 +1157,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
   if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
+  if (s->nk > 0 && (s->G > 1 || !il.empty()))
+    throw Error{HBG_ERR_UNSUPPORTED, "Kochin functions with symmetry planes or walls, or at the frequency limits"};
   if (s->np > 0 && (std::isfinite(s->depth) || s->G > 1 || !il.empty()))
     throw Error{HBG_ERR_UNSUPPORTED, "field points in finite depth, with symmetry planes or walls, or at the "
                                      "frequency limits"};
@@ -1178,6 +1220,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   std::vector<double> hout(nout * B);
   // field values: per frequency [np][nrad + 3 H] on the device, all frequencies on the host
   const size_t fw = (size_t)nrad + 3 * (size_t)H, fstride = (size_t)s->np * fw;
+  const size_t kstride = 2 * (size_t)s->nk * R;
+  std::vector<double2> hkochin;
+  if (s->nk > 0) {
+    s->d_kout.ensure(kstride * B * halves);
+    hkochin.resize(kstride * B);
+    s->kochin_vals.assign(kstride * (size_t)n_omegas, make_double2(0.0, 0.0));
+    s->kochin_F = -1;
+  }
   std::vector<double2> hfield;
   if (s->np > 0) {
     s->d_Sx.ensure((size_t)N * s->np);
@@ -1332,6 +1382,17 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       }
     }
     if (prof) HBG_HIP(hipEventRecord(s->bev[e1], sb));
+    if (s->nk > 0) {
+      for (int b = 0; b < nb; ++b) {
+        const double w = omegas[f0 + b], K = w * w / s->grav;
+        const double kw = std::isfinite(s->depth) ? make_fin_freq(s->depth, K).k0 : K;
+        launch_kochin(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, kw, std::isfinite(s->depth) ? s->depth : 0.0,
+                      s->nk, Mh + mstride * b + (size_t)N * N, s->d_phiI.p + hstride * slot(h, b),
+                      s->d_kout.p + kstride * slot(h, b), sb);
+      }
+      HBG_HIP(hipMemcpyAsync(hkochin.data(), s->d_kout.p + kstride * slot(h, 0), nb * kstride * sizeof(double2),
+                             hipMemcpyDeviceToHost, sb));
+    }
     if (s->np > 0) {
       for (int b = 0; b < nb; ++b) {
         const double w = omegas[f0 + b];
@@ -1377,7 +1438,15 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       if (haskind) std::memcpy(haskind + fi * 2 * nHK, hHK, 2 * nHK * sizeof(double));
       if (s->np > 0)
         std::copy(hfield.begin() + fstride * b, hfield.begin() + fstride * (b + 1), s->field_vals.begin() + fstride * fi);
+      if (s->nk > 0)
+        std::copy(hkochin.begin() + kstride * b, hkochin.begin() + kstride * (b + 1),
+                  s->kochin_vals.begin() + kstride * fi);
     }
+  }
+  if (s->nk > 0) {
+    s->kochin_F = n_omegas;
+    s->kochin_H = H;
+    s->kochin_nrad = nrad;
   }
   if (s->np > 0) {
     s->field_F = n_omegas;

@@ -636,6 +636,66 @@ void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, con
   HBG_HIP(hipGetLastError());
 }
 
+// Kochin functions of one frequency (lo_frequency.rs:1279-1372, far_kernel low_order.rs:1417-1433)
+// on the angles theta_t = 2 pi t / NK: Green's identity with the far-field kernel Phi =
+// Z(z) e^{i k (x cos + y sin)} over the panels (one point each, like the matrices): radiation
+// column c: H = sum_j area_j (phi_j dPhi/dn - Phi u_j), u the body's n_j; heading h (the scattered
+// formulation): H = sum_j area_j (phi_S + phi_I)_j dPhi/dn; and the theta derivatives Ht of both.
+// One thread per (angle, column); the panel loop reads wavefront-uniform panel records.
+// out: [2][NK][R] (H, then Ht).
+__global__ void __launch_bounds__(256) kochin_kernel(const PanelD* __restrict__ P, const int* __restrict__ body,
+                                                     const double* __restrict__ nj, int N, int nrad, int H, double k,
+                                                     double depth, int NK, const double2* __restrict__ sol,
+                                                     const double2* __restrict__ phiI, double2* __restrict__ out) {
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  const int c = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  const int R = nrad + H;
+  if (t >= NK || c >= R) return;
+  double sn, cs;
+  sincos(2.0 * PI * t / NK, &sn, &cs);
+  const double2* phi = sol + (size_t)c * N;
+  const double2* pI = c >= nrad ? phiI + (size_t)(c - nrad) * N : nullptr;
+  double hr = 0.0, hi = 0.0, tr = 0.0, ti = 0.0;
+  for (int j = 0; j < N; ++j) {
+    const PanelD* p = P + j;
+    const double x = p->c[0], y = p->c[1], z = p->c[2];
+    double Z = exp(k * z), Zz = k * Z;
+    if (depth > 0.0) {
+      const double b = exp(-k * (z + 2.0 * depth)), d = 1.0 + exp(-2.0 * k * depth);
+      Zz = k * (Z - b) / d;
+      Z = (Z + b) / d;
+    }
+    double es, ec;
+    sincos(k * (x * cs + y * sn), &es, &ec);
+    const double dt = -x * sn + y * cs, dnt = -p->n[0] * sn + p->n[1] * cs;
+    const double a = p->area, kn = k * (p->n[0] * cs + p->n[1] * sn);
+    // g = Z e, gn = e (i k Z (n . dir) + Zz n_z), gt = g i k dt, gnt = gn i k dt + e Z i k dnt
+    const double2 g = make_double2(a * Z * ec, a * Z * es);
+    const double2 gn = make_double2(a * (Zz * p->n[2] * ec - Z * kn * es), a * (Zz * p->n[2] * es + Z * kn * ec));
+    const double2 gt = make_double2(-k * dt * g.y, k * dt * g.x);
+    const double2 gnt = make_double2(-k * dt * gn.y - a * Z * k * dnt * es, k * dt * gn.x + a * Z * k * dnt * ec);
+    double2 f = phi[j];
+    double u = 0.0;
+    if (c < nrad)
+      u = body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0;
+    else
+      f = make_double2(f.x + pI[j].x, f.y + pI[j].y);
+    hr += f.x * gn.x - f.y * gn.y - u * g.x;
+    hi += f.x * gn.y + f.y * gn.x - u * g.y;
+    tr += f.x * gnt.x - f.y * gnt.y - u * gt.x;
+    ti += f.x * gnt.y + f.y * gnt.x - u * gt.y;
+  }
+  out[(size_t)t * R + c] = make_double2(hr, hi);
+  out[((size_t)NK + t) * R + c] = make_double2(tr, ti);
+}
+
+void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double k, double depth,
+                   int NK, const double2* sol, const double2* phiI, double2* out, hipStream_t stream) {
+  kochin_kernel<<<dim3((NK + 63) / 64, (nrad + H + 3) / 4), dim3(64, 4), 0, stream>>>(P, body, nj, N, nrad, H, k,
+                                                                                     depth, NK, sol, phiI, out);
+  HBG_HIP(hipGetLastError());
+}
+
 // The character systems of a symmetric body from its image blocks (lo_frequency.rs:855-943):
 // S_c = sum_g chi_c(g) S_g and the matrix part of M_c = sum_g chi_c(g) A_g for every character c,
 // with A32 (nullable) the complex64 copy of M_c, for the first nchar characters (all of them for a

@@ -128,6 +128,19 @@ HbgStatus hbg_set_field_points(HbgSession* s, const double* points, int32_t n);
  * solve. */
 HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fsca, double* ftot);
 
+/* Kochin functions (lo_frequency.rs:1279-1372, `Output` kr/krt/kd/kdt): n_angles angles
+ * theta_t = 2 pi t / n_angles (0: none). Every later hbg_solve (infinite or finite depth, no
+ * symmetry planes or walls, no frequency limits: else HBG_ERR_UNSUPPORTED) evaluates per
+ * frequency H(theta) = sum_j area_j (phi_j dPhi/dn - Phi u_j), Phi = Z(z) e^{i k (x cos + y sin)}
+ * the far-field kernel (low_order.rs:1417-1433), for every radiation column and, with the total
+ * potential and no source term, every heading, and the theta derivatives. */
+HbgStatus hbg_set_kochin(HbgSession* s, int32_t n_angles);
+
+/* The Kochin functions of the last hbg_solve with angles: kr, krt [F][n_angles][6 nbodies], kd,
+ * kdt [F][n_angles][H], complex (re, im) doubles; any output may be NULL. HBG_ERR_STATE before
+ * such a solve. */
+HbgStatus hbg_kochin_values(HbgSession* s, double* kr, double* krt, double* kd, double* kdt);
+
 /* The solve of n_omegas frequencies (rad/s) and n_headings headings (degrees), radiation of the
  * six rigid modes of every body plus diffraction (solver::run, lo_frequency.rs:1565-1635).
  * D = 6 n_bodies. Outputs (any may be NULL):
