@@ -47,9 +47,9 @@ std::vector<double> build_k_table();
 void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double k, double depth,
                    int NK, const double2* sol, const double2* phiI, double2* out, hipStream_t stream);
 void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nrad,
-                         int H, double w, double K, double logc, WaveTables T, double grav, const double* heads,
-                         const StaticD* stf, double2* Sx, double2* Ax, const double2* sol, const double2* phiI,
-                         double2* out, hipStream_t stream);
+                         int H, double w, double K, double kw, double depth, const FinFreq* f, FinTables F,
+                         double logc, WaveTables T, double grav, const double* heads, const StaticD* stf, double2* Sx,
+                         double2* Ax, const double2* sol, const double2* phiI, double2* out, hipStream_t stream);
 void launch_adaptive(const PanelD* P, const PanelD* Pf, int N, const double* gl, StaticD* st, const int2* queue,
                      unsigned count, hipStream_t stream);
 void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
@@ -360,7 +360,15 @@ struct HbgSession {
   // statics rows [N][np], the per-frequency influence S_x, A_x [N][np], the values of the batch,
   // and those of the last solve on the host ([F][np][nrad + 3 H] complex)
   int np = 0;
+  std::vector<double> fpts;  // the points [np][3]
   DevBuf<PanelD> d_fpts;
+  // finite depth: the panels and then the points (their factors of the tables at N + k), and
+  // tables of their own (the field values of a batch are taken beside the next batch's assembly)
+  DevBuf<PanelD> d_fcomb;
+  DevBuf<double> d_phi1f, d_phi2f, d_fworkf;
+  DevBuf<double4> d_pff;
+  DevBuf<double2> d_pcsf;
+  DevBuf<FinFreq> d_fff;
   DevBuf<StaticD> d_stf;
   bool stf_valid = false;
   int stf_logint = -1;
@@ -460,13 +468,20 @@ void ensure_field_statics(HbgSession* s, int logint, double tol) {
     s->d_queue.ensure(qcap);
     s->d_qcount.ensure(1);
     HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
-    launch_statics(s->d_panels.p, s->d_fpts.p, np, N, logint, tol, 0.0, s->d_gl.p, s->d_stf.p, s->d_queue.p,
-                   s->d_qcount.p, qcap, nullptr, s->stream);
+    launch_statics(s->d_panels.p, s->d_fpts.p, np, N, logint, tol, std::isfinite(s->depth) ? s->depth : 0.0, s->d_gl.p,
+                   s->d_stf.p, s->d_queue.p, s->d_qcount.p, qcap, nullptr, s->stream);
     unsigned count = 0;
     HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
     if (count <= qcap) {
       launch_adaptive(s->d_panels.p, s->d_fpts.p, np, s->d_gl.p, s->d_stf.p, s->d_queue.p, count, s->stream);
+      if (std::isfinite(s->depth)) {
+        s->d_fcomb.ensure((size_t)N + np);
+        HBG_HIP(hipMemcpyAsync(s->d_fcomb.p, s->d_panels.p, (size_t)N * sizeof(PanelD), hipMemcpyDeviceToDevice,
+                               s->stream));
+        HBG_HIP(hipMemcpyAsync(s->d_fcomb.p + N, s->d_fpts.p, (size_t)np * sizeof(PanelD), hipMemcpyDeviceToDevice,
+                               s->stream));
+      }
       HBG_HIP(hipStreamSynchronize(s->stream));
       s->stf_valid = true;
       s->stf_logint = logint;
@@ -792,6 +807,7 @@ HbgStatus hbg_set_field_points(HbgSession* s, const double* points, int32_t n) {
   GUARD_BEGIN
   use_device(s);
   s->np = n;
+  s->fpts.assign(points, points + 3 * (size_t)n);
   s->stf_valid = false;
   s->field_F = -1;
   if (n > 0) {
@@ -1159,9 +1175,12 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
   if (s->nk > 0 && (s->G > 1 || !il.empty()))
     throw Error{HBG_ERR_UNSUPPORTED, "Kochin functions with symmetry planes or walls, or at the frequency limits"};
-  if (s->np > 0 && (std::isfinite(s->depth) || s->G > 1 || !il.empty()))
-    throw Error{HBG_ERR_UNSUPPORTED, "field points in finite depth, with symmetry planes or walls, or at the "
-                                     "frequency limits"};
+  if (s->np > 0 && (s->G > 1 || !il.empty()))
+    throw Error{HBG_ERR_UNSUPPORTED, "field points with symmetry planes or walls, or at the frequency limits"};
+  if (s->np > 0 && std::isfinite(s->depth))
+    for (int i = 0; i < s->np; ++i)
+      if (s->fpts[3 * i + 2] < -s->depth)
+        throw Error{HBG_ERR_VALUE, "field point " + std::to_string(i) + " below the sea floor"};
   // a symmetric body (G > 1): every frequency is G character systems of order N (the sector),
   // consecutive in the batch buffers; the boundary values and the forces take the whole body.
   // With walls one system of order N (the even character), the boundary values and the forces on
@@ -1233,6 +1252,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     s->d_Sx.ensure((size_t)N * s->np);
     s->d_Ax.ensure((size_t)N * s->np);
     s->d_fout.ensure(fstride * B * halves);
+    if (std::isfinite(s->depth)) {
+      s->d_phi1f.ensure((size_t)FTS * FTW * FTILE);
+      s->d_phi2f.ensure((size_t)FTS * FTV * FTILE);
+      s->d_pff.ensure((size_t)N + s->np);
+      s->d_pcsf.ensure(((size_t)N + s->np) * FIN_ROOTS);
+      s->d_fworkf.ensure(fin_tables_work());
+      s->d_fff.ensure(1);
+    }
     hfield.resize(fstride * B);
     s->field_vals.assign(fstride * (size_t)n_omegas, make_double2(0.0, 0.0));
     s->field_F = -1;
@@ -1395,11 +1422,23 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     }
     if (s->np > 0) {
       for (int b = 0; b < nb; ++b) {
-        const double w = omegas[f0 + b];
-        launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, N, nrad, H, w,
-                            w * w / s->grav, logc, s->T, s->grav, s->d_heads.p, s->d_stf.p, s->d_Sx.p, s->d_Ax.p,
-                            Mh + mstride * b + (size_t)N * N, s->d_phiI.p + hstride * slot(h, b),
-                            s->d_fout.p + fstride * slot(h, b), sb);
+        const double w = omegas[f0 + b], K = w * w / s->grav;
+        double kw = K;
+        FinTables Ff{};
+        if (std::isfinite(s->depth)) {
+          // the frequency's tables for the panels and the points, on the LU stream
+          const FinFreq f = make_fin_freq(s->depth, K);
+          kw = f.k0;
+          HBG_HIP(hipMemcpyAsync(s->d_fff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, sb));
+          launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1f.p, s->d_phi2f.p, s->fitm, s->d_fcomb.p,
+                            N + s->np, s->d_pff.p, s->d_pcsf.p, s->d_fworkf.p, sb);
+          Ff = FinTables{s->d_phi1f.p, s->d_phi2f.p, s->ktab, s->d_pff.p, s->d_pcsf.p};
+        }
+        launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, N, nrad, H, w, K, kw,
+                            std::isfinite(s->depth) ? s->depth : 0.0, s->d_fff.p, Ff, logc, s->T, s->grav,
+                            s->d_heads.p, s->d_stf.p, s->d_Sx.p, s->d_Ax.p, Mh + mstride * b + (size_t)N * N,
+                            s->d_phiI.p + hstride * slot(h, b), s->d_fout.p + fstride * slot(h, b), sb);
+        if (std::isfinite(s->depth)) HBG_HIP(hipStreamSynchronize(sb));  // f (host) and the tables are reused
       }
       HBG_HIP(hipMemcpyAsync(hfield.data(), s->d_fout.p + fstride * slot(h, 0), nb * fstride * sizeof(double2),
                              hipMemcpyDeviceToHost, sb));

@@ -568,6 +568,30 @@ __global__ void __launch_bounds__(256) field_influence_kernel(const PanelD* __re
             0.0);
 }
 
+// The same in finite depth: the evaluator of the assembly (fin_kernel_u) with the per-point factors
+// of the tables at N + k (the tables hold the panels, then the points).
+__global__ void __launch_bounds__(256) field_influence_fin_kernel(const PanelD* __restrict__ Pf,
+                                                                  const PanelD* __restrict__ P, int Np, int N,
+                                                                  const FinFreq* __restrict__ f, FinTables F,
+                                                                  WaveTables T, double logc,
+                                                                  const StaticD* __restrict__ st,
+                                                                  double2* __restrict__ Sx, double2* __restrict__ Ax) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  if (j >= N) return;
+  const bool valid = k < Np;
+  const int kk = valid ? k : 0;
+  const PanelD* pk = Pf + kk;
+  const PanelD* pn = P + j;
+  const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
+  const double R = sqrt(dx * dx + dy * dy);
+  const FinK w = fin_kernel_u(f, F, T, valid, N + kk, j, R, pk->c[2], pn->c[2]);
+  if (!valid) return;
+  const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+  put_entry(Sx, Ax, nullptr, Np, k, j, st[(size_t)j * Np + k], pn->area, pn->n[2], an, 2.0 * f->K * logc, w.Wr, w.WrR,
+            w.WrZ, w.Wi, w.WiR, w.WiZ, 0.0);
+}
+
 // The field values of one frequency at every point k (lo_frequency.rs:1461-1491): radiation column
 // c: frad = (w^2 / g) (1 / 4 pi) sum_j (phi_j D_x - S_x u_j) with u the body's n_j; heading h
 // (the scattered formulation): phi_S = (1 / 4 pi) sum_j (phi_S + phi_I)_j D_x, and the elevations
@@ -576,7 +600,7 @@ __global__ void __launch_bounds__(256) field_influence_kernel(const PanelD* __re
 // [Np][nrad + 3 H], per point frad (nrad), finc, fsca, ftot (H each).
 __global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict__ Pf, const int* __restrict__ body,
                                                         const double* __restrict__ nj, int Np, int N, int nrad, int H,
-                                                        double w, double k, double grav,
+                                                        double w, double k, double depth, double grav,
                                                         const double* __restrict__ heads,
                                                         const double2* __restrict__ Sx,
                                                         const double2* __restrict__ Ax,
@@ -607,12 +631,15 @@ __global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict
     vr += -(f.x * a.x - f.y * a.y);
     vi += -(f.x * a.y + f.y * a.x);
   }
-  // phi_I at the point (infinite depth): (i g / w) e^{k z} e^{-i k (x cos b + y sin b)}
+  // phi_I at the point: (i g / w) Z(z) e^{-i k (x cos b + y sin b)}, Z = e^{k z} or, in finite
+  // depth, cosh k (z + h) / cosh k h with the real root k
   const PanelD* q = Pf + p;
   double sb, cb, sp, cp;
   sincos(heads[h], &sb, &cb);
   sincos(-k * (q->c[0] * cb + q->c[1] * sb), &sp, &cp);
-  const double Z = exp(k * q->c[2]), gw = grav / w;
+  double Z = exp(k * q->c[2]);
+  if (depth > 0.0) Z = (Z + exp(-k * (q->c[2] + 2.0 * depth))) / (1.0 + exp(-2.0 * k * depth));
+  const double gw = grav / w;
   const double2 pi_ = make_double2(-gw * sp * Z, gw * cp * Z);
   const double2 ps = make_double2(vr / (4.0 * PI), vi / (4.0 * PI));
   // eta = -(i w / g) phi
@@ -624,15 +651,19 @@ __global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict
   o[2 * H + h] = make_double2(eI.x + eS.x, eI.y + eS.y);
 }
 
+// f, F (finite depth, depth > 0): the frequency and its tables with the points' factors at N + k;
+// kw: the wave number of the incident wave (K in infinite depth, the real root in finite depth)
 void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nrad,
-                         int H, double w, double K, double logc, WaveTables T, double grav, const double* heads,
-                         const StaticD* stf, double2* Sx, double2* Ax, const double2* sol, const double2* phiI,
-                         double2* out, hipStream_t stream) {
-  const dim3 block(64, 4);
-  field_influence_kernel<<<dim3((Np + 63) / 64, (N + 3) / 4), block, 0, stream>>>(Pf, P, Np, N, K, logc, T, stf, Sx,
-                                                                                 Ax);
+                         int H, double w, double K, double kw, double depth, const FinFreq* f, FinTables F,
+                         double logc, WaveTables T, double grav, const double* heads, const StaticD* stf, double2* Sx,
+                         double2* Ax, const double2* sol, const double2* phiI, double2* out, hipStream_t stream) {
+  const dim3 block(64, 4), grid((Np + 63) / 64, (N + 3) / 4);
+  if (depth > 0.0)
+    field_influence_fin_kernel<<<grid, block, 0, stream>>>(Pf, P, Np, N, f, F, T, logc, stf, Sx, Ax);
+  else
+    field_influence_kernel<<<grid, block, 0, stream>>>(Pf, P, Np, N, K, logc, T, stf, Sx, Ax);
   field_sum_kernel<<<dim3((Np + 63) / 64, (nrad + H + 3) / 4), block, 0, stream>>>(
-      Pf, body, nj, Np, N, nrad, H, w, K, grav, heads, Sx, Ax, sol, phiI, out);
+      Pf, body, nj, Np, N, nrad, H, w, kw, depth, grav, heads, Sx, Ax, sol, phiI, out);
   HBG_HIP(hipGetLastError());
 }
 

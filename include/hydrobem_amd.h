@@ -117,8 +117,9 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
 
 /* Field points (lo_frequency.rs:1392-1495, `Output` frad/finc/fsca/ftot, low_order.rs:1490-1500):
  * points [n][3] on or below z = 0 (in the fluid, on the free surface, or inside a body), n = 0
- * clears them. Every later hbg_solve (infinite depth, no symmetry planes or walls, no frequency
- * limits: else HBG_ERR_UNSUPPORTED) also evaluates, per frequency and point, the radiation
+ * clears them. Every later hbg_solve (infinite or finite depth, every point above the sea floor
+ * or HBG_ERR_VALUE; no symmetry planes or walls, no frequency limits: else HBG_ERR_UNSUPPORTED)
+ * also evaluates, per frequency and point, the radiation
  * elevation of each rigid mode frad = (w^2 / g) phi_j(x) (per unit amplitude of motion) and the
  * incident, scattered and total elevations eta = -(i w / g) phi of each heading. */
 HbgStatus hbg_set_field_points(HbgSession* s, const double* points, int32_t n);
