@@ -90,7 +90,8 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
  * vertex), normals into the fluid by the vertex order; body [n_panels] (NULL: all body 0);
  * rotation_centres [n_bodies][3] (NULL: the origin). Each panel passes the reference's panel rule
  * (geometry.rs:120-199) or the call fails with HBG_ERR_VALUE naming the panel. Replaces the
- * geometry build of lo_geometry.rs:1100-1113 (body frame, no symmetry). Invalidates the statics. */
+ * geometry build of lo_geometry.rs:1100-1113 (body frame; the symmetry planes and walls of
+ * hbg_set_symmetry and hbg_set_walls add the images). Invalidates the statics. */
 HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
                          int32_t n_bodies, const double* rotation_centres);
 
