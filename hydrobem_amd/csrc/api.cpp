@@ -461,7 +461,7 @@ void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false
 // The statics rows of the field points (the same kernels, the points as field records).
 void ensure_field_statics(HbgSession* s, int logint, double tol) {
   if (s->stf_valid && s->stf_logint == logint && s->stf_tol == tol) return;
-  const int N = s->N, np = s->np;
+  const int N = s->G * s->N, np = s->np;  // sources: the panels and their images
   s->d_stf.ensure((size_t)N * np);
   unsigned qcap = (unsigned)std::min<size_t>((size_t)np * 64 + 64, (size_t)1 << 28);
   for (int attempt = 0; attempt < 2; ++attempt) {
@@ -1173,10 +1173,10 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
   if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
-  if (s->nk > 0 && (s->G > 1 || !il.empty()))
-    throw Error{HBG_ERR_UNSUPPORTED, "Kochin functions with symmetry planes or walls, or at the frequency limits"};
-  if (s->np > 0 && (s->G > 1 || !il.empty()))
-    throw Error{HBG_ERR_UNSUPPORTED, "field points with symmetry planes or walls, or at the frequency limits"};
+  if (s->nk > 0 && (s->walls || !il.empty()))
+    throw Error{HBG_ERR_UNSUPPORTED, "Kochin functions with walls, or at the frequency limits"};
+  if (s->np > 0 && (s->walls || !il.empty()))
+    throw Error{HBG_ERR_UNSUPPORTED, "field points with walls, or at the frequency limits"};
   if (s->np > 0 && std::isfinite(s->depth))
     for (int i = 0; i < s->np; ++i)
       if (s->fpts[3 * i + 2] < -s->depth)
@@ -1249,14 +1249,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   }
   std::vector<double2> hfield;
   if (s->np > 0) {
-    s->d_Sx.ensure((size_t)N * s->np);
-    s->d_Ax.ensure((size_t)N * s->np);
+    s->d_Sx.ensure((size_t)NF * s->np);
+    s->d_Ax.ensure((size_t)NF * s->np);
     s->d_fout.ensure(fstride * B * halves);
     if (std::isfinite(s->depth)) {
       s->d_phi1f.ensure((size_t)FTS * FTW * FTILE);
       s->d_phi2f.ensure((size_t)FTS * FTV * FTILE);
-      s->d_pff.ensure((size_t)N + s->np);
-      s->d_pcsf.ensure(((size_t)N + s->np) * FIN_ROOTS);
+      s->d_pff.ensure((size_t)NF + s->np);
+      s->d_pcsf.ensure(((size_t)NF + s->np) * FIN_ROOTS);
       s->d_fworkf.ensure(fin_tables_work());
       s->d_fff.ensure(1);
     }
@@ -1407,22 +1407,16 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                           NF, s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
                           s->d_out.p + nout * slot(h, b), sb);
       }
-    }
-    if (prof) HBG_HIP(hipEventRecord(s->bev[e1], sb));
-    if (s->nk > 0) {
-      for (int b = 0; b < nb; ++b) {
-        const double w = omegas[f0 + b], K = w * w / s->grav;
+      // Kochin functions and field values from the potential on all NF panels (a symmetric body:
+      // recombined just above, before the next frequency reuses the buffer)
+      const double2* solb = G == 1 ? Mh + mstride * b + (size_t)N * N : s->d_phif.p;
+      const double w = omegas[f0 + b], K = w * w / s->grav;
+      if (s->nk > 0) {
         const double kw = std::isfinite(s->depth) ? make_fin_freq(s->depth, K).k0 : K;
-        launch_kochin(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, kw, std::isfinite(s->depth) ? s->depth : 0.0,
-                      s->nk, Mh + mstride * b + (size_t)N * N, s->d_phiI.p + hstride * slot(h, b),
-                      s->d_kout.p + kstride * slot(h, b), sb);
+        launch_kochin(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, kw, std::isfinite(s->depth) ? s->depth : 0.0,
+                      s->nk, solb, s->d_phiI.p + hstride * slot(h, b), s->d_kout.p + kstride * slot(h, b), sb);
       }
-      HBG_HIP(hipMemcpyAsync(hkochin.data(), s->d_kout.p + kstride * slot(h, 0), nb * kstride * sizeof(double2),
-                             hipMemcpyDeviceToHost, sb));
-    }
-    if (s->np > 0) {
-      for (int b = 0; b < nb; ++b) {
-        const double w = omegas[f0 + b], K = w * w / s->grav;
+      if (s->np > 0) {
         double kw = K;
         FinTables Ff{};
         if (std::isfinite(s->depth)) {
@@ -1431,18 +1425,23 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
           kw = f.k0;
           HBG_HIP(hipMemcpyAsync(s->d_fff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, sb));
           launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1f.p, s->d_phi2f.p, s->fitm, s->d_fcomb.p,
-                            N + s->np, s->d_pff.p, s->d_pcsf.p, s->d_fworkf.p, sb);
+                            NF + s->np, s->d_pff.p, s->d_pcsf.p, s->d_fworkf.p, sb);
           Ff = FinTables{s->d_phi1f.p, s->d_phi2f.p, s->ktab, s->d_pff.p, s->d_pcsf.p};
         }
-        launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, N, nrad, H, w, K, kw,
+        launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, NF, nrad, H, w, K, kw,
                             std::isfinite(s->depth) ? s->depth : 0.0, s->d_fff.p, Ff, logc, s->T, s->grav,
-                            s->d_heads.p, s->d_stf.p, s->d_Sx.p, s->d_Ax.p, Mh + mstride * b + (size_t)N * N,
-                            s->d_phiI.p + hstride * slot(h, b), s->d_fout.p + fstride * slot(h, b), sb);
+                            s->d_heads.p, s->d_stf.p, s->d_Sx.p, s->d_Ax.p, solb, s->d_phiI.p + hstride * slot(h, b),
+                            s->d_fout.p + fstride * slot(h, b), sb);
         if (std::isfinite(s->depth)) HBG_HIP(hipStreamSynchronize(sb));  // f (host) and the tables are reused
       }
+    }
+    if (prof) HBG_HIP(hipEventRecord(s->bev[e1], sb));
+    if (s->nk > 0)
+      HBG_HIP(hipMemcpyAsync(hkochin.data(), s->d_kout.p + kstride * slot(h, 0), nb * kstride * sizeof(double2),
+                             hipMemcpyDeviceToHost, sb));
+    if (s->np > 0)
       HBG_HIP(hipMemcpyAsync(hfield.data(), s->d_fout.p + fstride * slot(h, 0), nb * fstride * sizeof(double2),
                              hipMemcpyDeviceToHost, sb));
-    }
     HBG_HIP(hipMemcpyAsync(hout.data(), s->d_out.p + nout * slot(h, 0), nb * nout * sizeof(double),
                            hipMemcpyDeviceToHost, sb));
     HBG_HIP(hipStreamSynchronize(sb));

@@ -154,3 +154,34 @@ def test_unsupported_uses_with_symmetry_are_refused(sessions):
             hb._check(hb.library().hbg_set_symmetry(s._h, 2, 0))
     finally:
         s.set_symmetry(False, False)
+
+
+@pytest.mark.parametrize("depth", [float("inf"), 3.0])
+def test_field_points_and_kochin_of_a_sector_equal_the_whole_mesh(sessions, depth):
+    quadrant = meshes.cylinder_panels(1.0, 0.8, 9, 4, 3, quarter=True)
+    whole = meshes.expand_symmetry(quadrant)
+    pts = np.array([[2.0, 0.5, 0.0], [-1.5, 2.5, -0.3], [0.3, -0.2, -0.4], [-6.0, -4.0, 0.0]])
+    w, heads = np.array([0.7, 1.6]), [0.0, 30.0]
+    say(f"case: cylinder sector {len(quadrant)} of {len(whole)} panels, both planes, depth {depth}, 4 field points, "
+        f"36 Kochin angles")
+    s, f = sessions
+    res = []
+    for x, v, sym in ((s, quadrant, True), (f, whole, False)):
+        x.set_environment(1025.0, 9.81, depth)
+        x.set_symmetry(sym, sym)
+        x.set_panels(v)
+        x.set_field_points(pts)
+        x.set_kochin(36)
+        x.solve(w, heads)
+        res.append((x.field_values(2, 2), x.kochin_values(2, 2)))
+        x.set_field_points(None)
+        x.set_kochin(0)
+    s.set_symmetry(False, False)
+    for key in ("frad", "finc", "fsca", "ftot"):
+        e = rel_per_entry(res[0][0][key], res[1][0][key])
+        say(f"  {key}: sector vs whole {e:.2e}")
+        assert e < 1e-9
+    for key in ("kr", "krt", "kd", "kdt"):
+        e = rel_per_entry(res[0][1][key], res[1][1][key])
+        say(f"  {key}: sector vs whole {e:.2e}")
+        assert e < 1e-9
