@@ -44,12 +44,13 @@ FinFreq make_fin_freq(double h, double K);
 double dispersion_root(double H, int n);
 std::vector<double> fit_matrix(int M);
 std::vector<double> build_k_table();
-void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double k, double depth,
-                   int NK, const double2* sol, const double2* phiI, double2* out, hipStream_t stream);
-void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nrad,
-                         int H, double w, double K, double kw, double depth, const FinFreq* f, FinTables F,
-                         double logc, WaveTables T, double grav, const double* heads, const StaticD* stf, double2* Sx,
-                         double2* Ax, const double2* sol, const double2* phiI, double2* out, hipStream_t stream);
+void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, int nsol, int nrad, int H, double k,
+                   double depth, int NK, const double2* sol, const double2* phiI, double2* out, hipStream_t stream);
+void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nsol,
+                         int nrad, int H, double w, double K, double kw, double depth, const FinFreq* f, FinTables F,
+                         double logc, WaveTables T, double grav, const double* heads, int walls, const StaticD* stf,
+                         double2* Sx, double2* Ax, const double2* sol, const double2* phiI, double2* out,
+                         hipStream_t stream);
 void launch_adaptive(const PanelD* P, const PanelD* Pf, int N, const double* gl, StaticD* st, const int2* queue,
                      unsigned count, hipStream_t stream);
 void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
@@ -1173,10 +1174,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
   if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
-  if (s->nk > 0 && (s->walls || !il.empty()))
-    throw Error{HBG_ERR_UNSUPPORTED, "Kochin functions with walls, or at the frequency limits"};
-  if (s->np > 0 && (s->walls || !il.empty()))
-    throw Error{HBG_ERR_UNSUPPORTED, "field points with walls, or at the frequency limits"};
+  if ((s->nk > 0 || s->np > 0) && !il.empty())
+    throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions at the frequency limits"};
   if (s->np > 0 && std::isfinite(s->depth))
     for (int i = 0; i < s->np; ++i)
       if (s->fpts[3 * i + 2] < -s->depth)
@@ -1186,6 +1185,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // With walls one system of order N (the even character), the boundary values and the forces on
   // the real panels, the wall images only in the matrices.
   const int G = s->walls ? 1 : s->G, NF = G * N;
+  const int NS = s->G * N;  // the source panels of the field values and Kochin functions, images included
   const double* const omegas_all = omegas;
   omegas = om.data();
   n_omegas = (int32_t)om.size();
@@ -1249,14 +1249,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   }
   std::vector<double2> hfield;
   if (s->np > 0) {
-    s->d_Sx.ensure((size_t)NF * s->np);
-    s->d_Ax.ensure((size_t)NF * s->np);
+    s->d_Sx.ensure((size_t)NS * s->np);
+    s->d_Ax.ensure((size_t)NS * s->np);
     s->d_fout.ensure(fstride * B * halves);
     if (std::isfinite(s->depth)) {
       s->d_phi1f.ensure((size_t)FTS * FTW * FTILE);
       s->d_phi2f.ensure((size_t)FTS * FTV * FTILE);
-      s->d_pff.ensure((size_t)NF + s->np);
-      s->d_pcsf.ensure(((size_t)NF + s->np) * FIN_ROOTS);
+      s->d_pff.ensure((size_t)NS + s->np);
+      s->d_pcsf.ensure(((size_t)NS + s->np) * FIN_ROOTS);
       s->d_fworkf.ensure(fin_tables_work());
       s->d_fff.ensure(1);
     }
@@ -1413,8 +1413,9 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       const double w = omegas[f0 + b], K = w * w / s->grav;
       if (s->nk > 0) {
         const double kw = std::isfinite(s->depth) ? make_fin_freq(s->depth, K).k0 : K;
-        launch_kochin(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, kw, std::isfinite(s->depth) ? s->depth : 0.0,
-                      s->nk, solb, s->d_phiI.p + hstride * slot(h, b), s->d_kout.p + kstride * slot(h, b), sb);
+        launch_kochin(s->d_panels.p, s->d_body.p, s->d_nj.p, NS, NF, nrad, H, kw,
+                      std::isfinite(s->depth) ? s->depth : 0.0, s->nk, solb, s->d_phiI.p + hstride * slot(h, b),
+                      s->d_kout.p + kstride * slot(h, b), sb);
       }
       if (s->np > 0) {
         double kw = K;
@@ -1425,13 +1426,13 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
           kw = f.k0;
           HBG_HIP(hipMemcpyAsync(s->d_fff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, sb));
           launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1f.p, s->d_phi2f.p, s->fitm, s->d_fcomb.p,
-                            NF + s->np, s->d_pff.p, s->d_pcsf.p, s->d_fworkf.p, sb);
+                            NS + s->np, s->d_pff.p, s->d_pcsf.p, s->d_fworkf.p, sb);
           Ff = FinTables{s->d_phi1f.p, s->d_phi2f.p, s->ktab, s->d_pff.p, s->d_pcsf.p};
         }
-        launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, NF, nrad, H, w, K, kw,
+        launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, NS, NF, nrad, H, w, K, kw,
                             std::isfinite(s->depth) ? s->depth : 0.0, s->d_fff.p, Ff, logc, s->T, s->grav,
-                            s->d_heads.p, s->d_stf.p, s->d_Sx.p, s->d_Ax.p, solb, s->d_phiI.p + hstride * slot(h, b),
-                            s->d_fout.p + fstride * slot(h, b), sb);
+                            s->d_heads.p, s->walls, s->d_stf.p, s->d_Sx.p, s->d_Ax.p, solb,
+                            s->d_phiI.p + hstride * slot(h, b), s->d_fout.p + fstride * slot(h, b), sb);
         if (std::isfinite(s->depth)) HBG_HIP(hipStreamSynchronize(sb));  // f (host) and the tables are reused
       }
     }

@@ -596,12 +596,15 @@ __global__ void __launch_bounds__(256) field_influence_fin_kernel(const PanelD* 
 // c: frad = (w^2 / g) (1 / 4 pi) sum_j (phi_j D_x - S_x u_j) with u the body's n_j; heading h
 // (the scattered formulation): phi_S = (1 / 4 pi) sum_j (phi_S + phi_I)_j D_x, and the elevations
 // eta = -(i w / g) phi of the incident wave at the point, the scattered wave and their sum.
-// sol: the N x R solution columns (leading dimension N); phiI: [H][N] on the panels; out:
-// [Np][nrad + 3 H], per point frad (nrad), finc, fsca, ftot (H each).
+// sol: the nsol x R solution columns (leading dimension nsol); phiI: [H][nsol] on the panels; N
+// source panels, panel j taking the values of j mod nsol (walls: an image carries its panel's
+// potential and incident wave, nsol = the real panels); walls (bit 0: x = 0, bit 1: y = 0): the
+// incident wave at the point with its reflections; out: [Np][nrad + 3 H], per point frad (nrad),
+// finc, fsca, ftot (H each).
 __global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict__ Pf, const int* __restrict__ body,
-                                                        const double* __restrict__ nj, int Np, int N, int nrad, int H,
-                                                        double w, double k, double depth, double grav,
-                                                        const double* __restrict__ heads,
+                                                        const double* __restrict__ nj, int Np, int N, int nsol,
+                                                        int nrad, int H, double w, double k, double depth,
+                                                        double grav, const double* __restrict__ heads, int walls,
                                                         const double2* __restrict__ Sx,
                                                         const double2* __restrict__ Ax,
                                                         const double2* __restrict__ sol,
@@ -611,10 +614,10 @@ __global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict
   const int R = nrad + H;
   if (p >= Np || c >= R) return;
   double vr = 0.0, vi = 0.0;
-  const double2* phi = sol + (size_t)c * N;
+  const double2* phi = sol + (size_t)c * nsol;
   if (c < nrad) {
     for (int j = 0; j < N; ++j) {
-      const double2 a = Ax[(size_t)j * Np + p], sv = Sx[(size_t)j * Np + p], f = phi[j];
+      const double2 a = Ax[(size_t)j * Np + p], sv = Sx[(size_t)j * Np + p], f = phi[j % nsol];
       const double u = body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0;
       // phi D - S u with D = -A
       vr += -(f.x * a.x - f.y * a.y) - sv.x * u;
@@ -625,22 +628,30 @@ __global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict
     return;
   }
   const int h = c - nrad;
-  const double2* pI = phiI + (size_t)h * N;
+  const double2* pI = phiI + (size_t)h * nsol;
   for (int j = 0; j < N; ++j) {
-    const double2 a = Ax[(size_t)j * Np + p], f = make_double2(phi[j].x + pI[j].x, phi[j].y + pI[j].y);
+    const int js = j % nsol;
+    const double2 a = Ax[(size_t)j * Np + p], f = make_double2(phi[js].x + pI[js].x, phi[js].y + pI[js].y);
     vr += -(f.x * a.x - f.y * a.y);
     vi += -(f.x * a.y + f.y * a.x);
   }
   // phi_I at the point: (i g / w) Z(z) e^{-i k (x cos b + y sin b)}, Z = e^{k z} or, in finite
   // depth, cosh k (z + h) / cosh k h with the real root k
   const PanelD* q = Pf + p;
-  double sb, cb, sp, cp;
+  double sb, cb;
   sincos(heads[h], &sb, &cb);
-  sincos(-k * (q->c[0] * cb + q->c[1] * sb), &sp, &cp);
   double Z = exp(k * q->c[2]);
   if (depth > 0.0) Z = (Z + exp(-k * (q->c[2] + 2.0 * depth))) / (1.0 + exp(-2.0 * k * depth));
   const double gw = grav / w;
-  const double2 pi_ = make_double2(-gw * sp * Z, gw * cp * Z);
+  double2 pi_ = make_double2(0.0, 0.0);
+  for (int r = 0; r < 4; ++r) {  // the wave and its reflections in the walls present
+    if ((r & walls) != r) continue;
+    const double sx = (r & 1) ? -1.0 : 1.0, sy = (r & 2) ? -1.0 : 1.0;
+    double sp, cp;
+    sincos(-k * (sx * q->c[0] * cb + sy * q->c[1] * sb), &sp, &cp);
+    pi_.x -= gw * sp * Z;
+    pi_.y += gw * cp * Z;
+  }
   const double2 ps = make_double2(vr / (4.0 * PI), vi / (4.0 * PI));
   // eta = -(i w / g) phi
   const double e = w / grav;
@@ -653,17 +664,18 @@ __global__ void __launch_bounds__(256) field_sum_kernel(const PanelD* __restrict
 
 // f, F (finite depth, depth > 0): the frequency and its tables with the points' factors at N + k;
 // kw: the wave number of the incident wave (K in infinite depth, the real root in finite depth)
-void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nrad,
-                         int H, double w, double K, double kw, double depth, const FinFreq* f, FinTables F,
-                         double logc, WaveTables T, double grav, const double* heads, const StaticD* stf, double2* Sx,
-                         double2* Ax, const double2* sol, const double2* phiI, double2* out, hipStream_t stream) {
+void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, const double* nj, int Np, int N, int nsol,
+                         int nrad, int H, double w, double K, double kw, double depth, const FinFreq* f, FinTables F,
+                         double logc, WaveTables T, double grav, const double* heads, int walls, const StaticD* stf,
+                         double2* Sx, double2* Ax, const double2* sol, const double2* phiI, double2* out,
+                         hipStream_t stream) {
   const dim3 block(64, 4), grid((Np + 63) / 64, (N + 3) / 4);
   if (depth > 0.0)
     field_influence_fin_kernel<<<grid, block, 0, stream>>>(Pf, P, Np, N, f, F, T, logc, stf, Sx, Ax);
   else
     field_influence_kernel<<<grid, block, 0, stream>>>(Pf, P, Np, N, K, logc, T, stf, Sx, Ax);
   field_sum_kernel<<<dim3((Np + 63) / 64, (nrad + H + 3) / 4), block, 0, stream>>>(
-      Pf, body, nj, Np, N, nrad, H, w, kw, depth, grav, heads, Sx, Ax, sol, phiI, out);
+      Pf, body, nj, Np, N, nsol, nrad, H, w, kw, depth, grav, heads, walls, Sx, Ax, sol, phiI, out);
   HBG_HIP(hipGetLastError());
 }
 
@@ -672,10 +684,12 @@ void launch_field_values(const PanelD* Pf, const PanelD* P, const int* body, con
 // Z(z) e^{i k (x cos + y sin)} over the panels (one point each, like the matrices): radiation
 // column c: H = sum_j area_j (phi_j dPhi/dn - Phi u_j), u the body's n_j; heading h (the scattered
 // formulation): H = sum_j area_j (phi_S + phi_I)_j dPhi/dn; and the theta derivatives Ht of both.
+// N panels, panel j taking the potential of j mod nsol (walls: the images carry their panels').
 // One thread per (angle, column); the panel loop reads wavefront-uniform panel records.
 // out: [2][NK][R] (H, then Ht).
 __global__ void __launch_bounds__(256) kochin_kernel(const PanelD* __restrict__ P, const int* __restrict__ body,
-                                                     const double* __restrict__ nj, int N, int nrad, int H, double k,
+                                                     const double* __restrict__ nj, int N, int nsol, int nrad, int H,
+                                                     double k,
                                                      double depth, int NK, const double2* __restrict__ sol,
                                                      const double2* __restrict__ phiI, double2* __restrict__ out) {
   const int t = blockIdx.x * 64 + threadIdx.x;
@@ -684,8 +698,8 @@ __global__ void __launch_bounds__(256) kochin_kernel(const PanelD* __restrict__ 
   if (t >= NK || c >= R) return;
   double sn, cs;
   sincos(2.0 * PI * t / NK, &sn, &cs);
-  const double2* phi = sol + (size_t)c * N;
-  const double2* pI = c >= nrad ? phiI + (size_t)(c - nrad) * N : nullptr;
+  const double2* phi = sol + (size_t)c * nsol;
+  const double2* pI = c >= nrad ? phiI + (size_t)(c - nrad) * nsol : nullptr;
   double hr = 0.0, hi = 0.0, tr = 0.0, ti = 0.0;
   for (int j = 0; j < N; ++j) {
     const PanelD* p = P + j;
@@ -705,12 +719,13 @@ __global__ void __launch_bounds__(256) kochin_kernel(const PanelD* __restrict__ 
     const double2 gn = make_double2(a * (Zz * p->n[2] * ec - Z * kn * es), a * (Zz * p->n[2] * es + Z * kn * ec));
     const double2 gt = make_double2(-k * dt * g.y, k * dt * g.x);
     const double2 gnt = make_double2(-k * dt * gn.y - a * Z * k * dnt * es, k * dt * gn.x + a * Z * k * dnt * ec);
-    double2 f = phi[j];
+    const int js = j % nsol;
+    double2 f = phi[js];
     double u = 0.0;
     if (c < nrad)
       u = body[j] == c / 6 ? nj[6 * (size_t)j + c % 6] : 0.0;
     else
-      f = make_double2(f.x + pI[j].x, f.y + pI[j].y);
+      f = make_double2(f.x + pI[js].x, f.y + pI[js].y);
     hr += f.x * gn.x - f.y * gn.y - u * g.x;
     hi += f.x * gn.y + f.y * gn.x - u * g.y;
     tr += f.x * gnt.x - f.y * gnt.y - u * gt.x;
@@ -720,10 +735,10 @@ __global__ void __launch_bounds__(256) kochin_kernel(const PanelD* __restrict__ 
   out[((size_t)NK + t) * R + c] = make_double2(tr, ti);
 }
 
-void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double k, double depth,
-                   int NK, const double2* sol, const double2* phiI, double2* out, hipStream_t stream) {
-  kochin_kernel<<<dim3((NK + 63) / 64, (nrad + H + 3) / 4), dim3(64, 4), 0, stream>>>(P, body, nj, N, nrad, H, k,
-                                                                                     depth, NK, sol, phiI, out);
+void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, int nsol, int nrad, int H, double k,
+                   double depth, int NK, const double2* sol, const double2* phiI, double2* out, hipStream_t stream) {
+  kochin_kernel<<<dim3((NK + 63) / 64, (nrad + H + 3) / 4), dim3(64, 4), 0, stream>>>(P, body, nj, N, nsol, nrad, H,
+                                                                                     k, depth, NK, sol, phiI, out);
   HBG_HIP(hipGetLastError());
 }
 

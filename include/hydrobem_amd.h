@@ -119,7 +119,7 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
 /* Field points (lo_frequency.rs:1392-1495, `Output` frad/finc/fsca/ftot, low_order.rs:1490-1500):
  * points [n][3] on or below z = 0 (in the fluid, on the free surface, or inside a body), n = 0
  * clears them. Every later hbg_solve (infinite or finite depth, every point above the sea floor
- * or HBG_ERR_VALUE; with symmetry planes too; no walls, no frequency limits: else
+ * or HBG_ERR_VALUE; with symmetry planes or walls too; no frequency limits: else
  * HBG_ERR_UNSUPPORTED)
  * also evaluates, per frequency and point, the radiation
  * elevation of each rigid mode frad = (w^2 / g) phi_j(x) (per unit amplitude of motion) and the
@@ -133,7 +133,7 @@ HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fs
 
 /* Kochin functions (lo_frequency.rs:1279-1372, `Output` kr/krt/kd/kdt): n_angles angles
  * theta_t = 2 pi t / n_angles (0: none). Every later hbg_solve (infinite or finite depth, with
- * symmetry planes too; no walls, no frequency limits: else HBG_ERR_UNSUPPORTED) evaluates per
+ * symmetry planes or walls too; no frequency limits: else HBG_ERR_UNSUPPORTED) evaluates per
  * frequency H(theta) = sum_j area_j (phi_j dPhi/dn - Phi u_j), Phi = Z(z) e^{i k (x cos + y sin)}
  * the far-field kernel (low_order.rs:1417-1433), for every radiation column and, with the total
  * potential and no source term, every heading, and the theta derivatives. */

@@ -205,3 +205,46 @@ def test_wall_refusals(session):
             call()
         assert e.value.status == 3
     s.set_walls(False, False)
+
+
+@pytest.mark.parametrize("depth,wx,wy", [(float("inf"), 1, 1), (2.5, 0, 1)])
+def test_wall_field_points_and_kochin_against_the_open_sea(session, depth, wx, wy):
+    v = BODIES["box"]()
+    centre = np.array([v[:, :, 0].mean(), v[:, :, 1].mean(), -0.1])
+    heads = [0.0, 40.0]
+    w = np.array([0.8, 1.7])
+    pts = np.array([[4.0, 3.0, 0.0], [0.5, 0.8, -0.3], [6.0, 0.2, -0.1]])
+    NK = 24
+    refl, verts, body, centres = open_sea_equivalent(v, centre, wx, wy)
+    G = len(refl)
+    sg = np.stack([mirror_signs(*r) for r in refl])
+    say(f"case: box {len(v)} panels, walls x {wx} y {wy}, depth {depth}: field points and Kochin functions "
+        f"against the open sea with {G} bodies")
+    s = session
+    s.set_environment(1025.0, 9.81, depth)
+    s.set_walls(False, False)
+    s.set_panels(verts, body=body, rotation_centres=centres)
+    s.set_field_points(pts)
+    s.set_kochin(NK)
+    hr = headings_reflected(heads, refl)
+    s.solve(w, hr)
+    fo, ko = s.field_values(len(w), len(hr)), s.kochin_values(len(w), len(hr))
+    s.set_panels(v, rotation_centres=centre[None])
+    s.set_walls(wx, wy)
+    s.solve(w, heads)
+    fw, kw = s.field_values(len(w), len(heads)), s.kochin_values(len(w), len(heads))
+    s.set_walls(False, False)
+    s.set_field_points(None)
+    s.set_kochin(0)
+    F = len(w)
+    ref = {"frad": sum(fo["frad"][:, :, 6 * g:6 * g + 6] * sg[g] for g in range(G)),
+           "kr": sum(ko["kr"][:, :, 6 * g:6 * g + 6] * sg[g] for g in range(G)),
+           "krt": sum(ko["krt"][:, :, 6 * g:6 * g + 6] * sg[g] for g in range(G))}
+    for key, src in (("finc", fo), ("fsca", fo), ("ftot", fo), ("kd", ko), ("kdt", ko)):
+        a = src[key]
+        ref[key] = a.reshape(F, a.shape[1], len(heads), G).sum(axis=3)
+    got = {**fw, **kw}
+    for key, r in ref.items():
+        e = rel_per_entry(got[key], r)
+        say(f"  {key}: walls vs open sea with images {e:.2e}")
+        assert e < 1e-9
