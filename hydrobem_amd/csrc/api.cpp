@@ -322,13 +322,13 @@ struct HbgSession {
   int N = 0, nb = 0;  // N: the panels set (a symmetric body: its sector); the device arrays hold G N
   // symmetry planes (lo_geometry.rs:25-35): G = 1, 2 or 4 reflections in the order e, sigma_x
   // (x -> -x), sigma_y, sigma_x sigma_y (those present), chi[c][g] the characters
-  int sym_x = 0, sym_y = 0, G = 1;
+  int sym_x = 0, sym_y = 0, G = 1, Gs = 1;  // G reflections, Gs solved characters per frequency
   // walls (lo_geometry.rs:947-1031; bit 0: x = 0, bit 1: y = 0): the reflections are then those
   // of the walls, the image panels move as the mirror image of the body (their n_j copied), only
   // the character even under every wall is solved and the forces are those on the real panels
   int walls = 0;
   std::vector<std::array<double, 2>> refl{{1.0, 1.0}};
-  std::vector<double> chi{1.0};
+  std::vector<double> chi{1.0}, chisym{1.0};  // [Gs][G], [Gs][Gs]
   std::vector<double> raw_vertices, raw_centres;  // the panels as set, to rebuild the images
   std::vector<int> raw_body;
   std::vector<PanelD> panels;
@@ -345,7 +345,7 @@ struct HbgSession {
   DevBuf<float2> d_M32, d_D32;
   DevBuf<double2> d_X64;
   DevBuf<double> d_rpart, d_rscale;
-  DevBuf<double> d_chi;                    // the characters [G][G]
+  DevBuf<double> d_chi, d_chisym;          // the solved characters [Gs][G], [Gs][Gs]
   DevBuf<double2> d_Sg, d_Ag, d_uc, d_phif;  // symmetric solves: an image block, projected u, phi
   DevBuf<int> d_active;  // systems still refined (mixed precision)
   DevBuf<double2> d_rpart2;
@@ -535,7 +535,7 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
     }
     // the G image blocks in one launch, each into its own buffer, then the G characters in one pass
     one(s->d_Sg.p, s->d_Ag.p, (float2*)nullptr, G);
-    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->walls ? 1 : G, s->d_S.p, Mout, mstride, A32,
+    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->Gs, s->d_S.p, Mout, mstride, A32,
                             s->stream);
   };
   if (!std::isfinite(s->depth)) {
@@ -703,7 +703,7 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
 }
 
 void build_geometry(HbgSession* s);
-static void set_reflections(HbgSession* s, int rx, int ry);
+static void set_reflections(HbgSession* s);
 
 HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
                          int32_t n_bodies, const double* rotation_centres) {
@@ -759,8 +759,9 @@ void build_geometry(HbgSession* s) {
       const int bi = s->raw_body[j];
       if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
       b[o] = bi;
-      if (s->walls && g > 0) {  // a wall image moves as the mirror image of its panel (lo_geometry.rs:981)
-        for (int d = 0; d < 6; ++d) nj[6 * o + d] = nj[6 * (size_t)j + d];
+      if (g >= s->Gs) {  // a wall image moves as the mirror image of its real panel (lo_geometry.rs:981)
+        const size_t r = (size_t)(g % s->Gs) * n_panels + j;
+        for (int d = 0; d < 6; ++d) nj[6 * o + d] = nj[6 * r + d];
         continue;
       }
       V3 rc = v3(s->raw_centres[3 * bi], s->raw_centres[3 * bi + 1], s->raw_centres[3 * bi + 2]);
@@ -788,12 +789,12 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_symmetry");
   if ((sym_x != 0 && sym_x != 1) || (sym_y != 0 && sym_y != 1))
     return fail(HBG_ERR_VALUE, "sym_x and sym_y must be 0 or 1", "hbg_set_symmetry");
-  if ((sym_x || sym_y) && s->walls)
-    return fail(HBG_ERR_UNSUPPORTED, "symmetry planes together with walls", "hbg_set_symmetry");
+  if ((sym_x && (s->walls & 1)) || (sym_y && (s->walls & 2)))
+    return fail(HBG_ERR_VALUE, "a symmetry plane where a wall stands", "hbg_set_symmetry");
   GUARD_BEGIN
   s->sym_x = sym_x;
   s->sym_y = sym_y;
-  set_reflections(s, sym_x, sym_y);
+  set_reflections(s);
   return HBG_OK;
   GUARD_END("hbg_set_symmetry")
 }
@@ -882,16 +883,16 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_walls");
   if ((wall_x != 0 && wall_x != 1) || (wall_y != 0 && wall_y != 1))
     return fail(HBG_ERR_VALUE, "wall_x and wall_y must be 0 or 1", "hbg_set_walls");
-  if ((wall_x || wall_y) && (s->sym_x || s->sym_y))
-    return fail(HBG_ERR_UNSUPPORTED, "walls together with symmetry planes", "hbg_set_walls");
+  if ((wall_x && s->sym_x) || (wall_y && s->sym_y))
+    return fail(HBG_ERR_VALUE, "a wall where a symmetry plane stands", "hbg_set_walls");
   GUARD_BEGIN
   const int before = s->walls;
   s->walls = wall_x | wall_y << 1;
   try {
-    set_reflections(s, wall_x, wall_y);
+    set_reflections(s);
   } catch (...) {  // panels behind a new wall: the walls as they were
     s->walls = before;
-    set_reflections(s, before & 1, before >> 1);
+    set_reflections(s);
     throw;
   }
   return HBG_OK;
@@ -900,22 +901,40 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
 
 // The reflections e, sigma_x, sigma_y, sigma_x sigma_y (those present) and the characters of their
 // group, for symmetry planes or walls; rebuilds the device panels when panels are set.
-static void set_reflections(HbgSession* s, int rx, int ry) {
-  s->refl.assign(1, {1.0, 1.0});
-  if (rx) s->refl.push_back({-1.0, 1.0});
-  if (ry) s->refl.push_back({1.0, -1.0});
-  if (rx && ry) s->refl.push_back({-1.0, -1.0});
-  s->G = (int)s->refl.size();
-  // the characters of the group (lo_geometry.rs:25-35): row c, column g
-  if (s->G == 1)
-    s->chi = {1.0};
-  else if (s->G == 2)
-    s->chi = {1.0, 1.0, 1.0, -1.0};
-  else
-    s->chi = {1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0};
+// The group of the symmetry planes and the walls (lo_geometry.rs:863-945), its elements as bit
+// masks (bit 0: x -> -x, bit 1: y -> -y): the symmetry elements first, then each of them times
+// the wall reflections, so that the real panels (the sector and its images under the symmetry)
+// come first and wall image block g mirrors real block g mod Gs. The characters of this
+// product of reflections are chi_c(g) = (-1)^|c & g| (lo_geometry.rs:25-35 in the order e,
+// sigma_x, sigma_y, sigma_x sigma_y); the solved ones, even under every wall reflection
+// (lo_geometry.rs:1010-1030), are the Gs masks c of the symmetry: chis [Gs][G] for the assembly,
+// chisym [Gs][Gs] over the symmetry elements for the right-hand sides and the recombination.
+static void set_reflections(HbgSession* s) {
+  const int smask = s->sym_x | s->sym_y << 1, wmask = s->walls;
+  std::vector<int> sym{0}, wal{0};
+  for (int m = 1; m < 4; ++m) {
+    if ((m & smask) == m) sym.push_back(m);
+    if ((m & wmask) == m) wal.push_back(m);
+  }
+  std::vector<int> masks;
+  for (int w : wal)
+    for (int g : sym) masks.push_back(w | g);
+  s->G = (int)masks.size();
+  s->Gs = (int)sym.size();
+  s->refl.clear();
+  for (int m : masks) s->refl.push_back({(m & 1) ? -1.0 : 1.0, (m & 2) ? -1.0 : 1.0});
+  auto chi = [](int c, int g) { return __builtin_popcount(c & g) & 1 ? -1.0 : 1.0; };
+  s->chi.assign((size_t)s->Gs * s->G, 0.0);
+  s->chisym.assign((size_t)s->Gs * s->Gs, 0.0);
+  for (int r = 0; r < s->Gs; ++r) {
+    for (int g = 0; g < s->G; ++g) s->chi[(size_t)r * s->G + g] = chi(sym[r], masks[g]);
+    for (int g = 0; g < s->Gs; ++g) s->chisym[(size_t)r * s->Gs + g] = chi(sym[r], masks[g]);
+  }
   use_device(s);
   s->d_chi.ensure(s->chi.size());
   HBG_HIP(hipMemcpy(s->d_chi.p, s->chi.data(), s->chi.size() * sizeof(double), hipMemcpyHostToDevice));
+  s->d_chisym.ensure(s->chisym.size());
+  HBG_HIP(hipMemcpy(s->d_chisym.p, s->chisym.data(), s->chisym.size() * sizeof(double), hipMemcpyHostToDevice));
   if (s->N > 0) build_geometry(s);
 }
 
@@ -1184,7 +1203,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // consecutive in the batch buffers; the boundary values and the forces take the whole body.
   // With walls one system of order N (the even character), the boundary values and the forces on
   // the real panels, the wall images only in the matrices.
-  const int G = s->walls ? 1 : s->G, NF = G * N;
+  const int G = s->Gs, NF = G * N;
   const int NS = s->G * N;  // the source panels of the field values and Kochin functions, images included
   const double* const omegas_all = omegas;
   omegas = om.data();
@@ -1291,7 +1310,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       launch_rhs(s->d_S.p, N, N, s->d_u.p, R, Mb, N, s->d_part.p, sa);
     } else {
       // every character c: S_c times the projection of u (lo_frequency.rs:873-893), one launch
-      launch_symmetry_project(s->d_u.p, N, G, R, s->d_chi.p, s->d_uc.p, sa);
+      launch_symmetry_project(s->d_u.p, N, G, R, s->d_chisym.p, s->d_uc.p, sa);
       launch_rhs_batched(s->d_S.p, (size_t)N * N, N, N, s->d_uc.p, (size_t)R * N, R, Mb, mstride, N, s->d_part.p, G,
                          sa);
     }
@@ -1402,7 +1421,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                       s->d_out.p + nout * slot(h, b), sb);
       } else {
         // the potential on the whole body from the characters (lo_frequency.rs:931-943)
-        launch_symmetry_recombine(Mh + mstride * b * G, mstride, N, G, R, s->d_chi.p, s->d_phif.p, sb);
+        launch_symmetry_recombine(Mh + mstride * b * G, mstride, N, G, R, s->d_chisym.p, s->d_phif.p, sb);
         launch_forces_sol(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, omegas[f0 + b], s->rho, s->d_phif.p,
                           NF, s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
                           s->d_out.p + nout * slot(h, b), sb);

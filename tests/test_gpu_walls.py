@@ -191,15 +191,15 @@ def test_wall_refusals(session):
     s.set_walls(False, False)
     s.set_symmetry(False, False)
     v = meshes.place(meshes.box_panels(0.6, 0.4, 0.4, (3, 2, 2), quarter=False), (0.3, 0.8, 0.0))  # x in [-0.3, 0.9]
-    say("case: walls with a body behind the wall, with symmetry planes, with the statics and the limits")
+    say("case: walls with a body behind the wall, a symmetry plane on a wall, with the statics and the limits")
     s.set_panels(v)
     with pytest.raises(hb.HbgError) as e:
         s.set_walls(True, False)
     assert e.value.status == 1
     s.set_walls(False, True)  # y >= 0 holds: accepted
     with pytest.raises(hb.HbgError) as e:
-        s.set_symmetry(True, False)
-    assert e.value.status == 3
+        s.set_symmetry(False, True)  # a symmetry plane where the wall stands
+    assert e.value.status == 1
     for call in (lambda: s.statics(), lambda: s.solve([0.0], [])):
         with pytest.raises(hb.HbgError) as e:
             call()
@@ -248,3 +248,47 @@ def test_wall_field_points_and_kochin_against_the_open_sea(session, depth, wx, w
         e = rel_per_entry(got[key], r)
         say(f"  {key}: walls vs open sea with images {e:.2e}")
         assert e < 1e-9
+
+
+@pytest.mark.parametrize("wx,depth,precision", [(1, float("inf"), "fp64"), (1, 2.5, "mixed"), (0, float("inf"), "mixed"),
+                                                (0, 2.5, "fp64")])
+def test_walls_with_a_symmetry_plane_against_the_whole_body(session, wx, depth, precision):
+    # a body symmetric about the plane normal to the wall (y = 0 beside the wall x = 0, or x = 0
+    # beside y = 0), stored as its half: the group of both, the two characters even under the wall
+    box = meshes.box_panels(1.0, 0.5, 0.5, (6, 4, 3), quarter=False)
+    if wx:
+        v = meshes.place(box, (2.0, 0.0, 0.0))
+        half = v[v.mean(axis=1)[:, 1] > 0.0]
+        whole = np.concatenate([half, reflect(half, 1.0, -1.0)])
+    else:
+        v = meshes.place(box, (0.0, 2.0, 0.0))
+        half = v[v.mean(axis=1)[:, 0] > 0.0]
+        whole = np.concatenate([half, reflect(half, -1.0, 1.0)])
+    centre = np.array([[2.0, 0.0, -0.1]]) if wx else np.array([[0.0, 2.0, -0.1]])
+    heads, w = [0.0, 40.0], np.array([0.7, 1.6])
+    pts = np.array([[4.0, 1.0, 0.0], [1.0, 4.0, -0.2]])
+    say(f"case: box half {len(half)} of {len(whole)} panels, wall {'x' if wx else 'y'} with the symmetry plane "
+        f"{'y' if wx else 'x'} = 0, depth {depth}, {precision}; field points and 24 Kochin angles")
+    s = session
+    s.set_environment(1025.0, 9.81, depth)
+    res = []
+    for v_, sym in ((half, True), (whole, False)):
+        s.set_walls(False, False)
+        s.set_symmetry(False, False)
+        s.set_panels(v_, rotation_centres=centre)
+        s.set_walls(wx, not wx)
+        s.set_symmetry(sym and not wx, sym and bool(wx))
+        s.set_field_points(pts)
+        s.set_kochin(24)
+        out = s.solve(w, heads, precision=precision)
+        res.append({**out, **s.field_values(len(w), len(heads)), **s.kochin_values(len(w), len(heads))})
+    s.set_symmetry(False, False)
+    s.set_walls(False, False)
+    s.set_field_points(None)
+    s.set_kochin(0)
+    for key in ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_haskind", "frad", "ftot",
+                "kr", "kd", "kdt"):
+        e = rel_per_entry(res[0][key], res[1][key])
+        say(f"  {key}: half with the symmetry plane vs whole body {e:.2e}")
+        # two mixed-precision solves each stop at a backward error of 1e-12
+        assert e < (1e-9 if precision == "fp64" else 1e-8)
