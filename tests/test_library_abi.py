@@ -35,6 +35,18 @@ def test_version_and_errors():
     assert L.hbg_session_free(None) == 0
 
 
+def test_argument_errors_of_the_geometry_and_output_entry_points():
+    # checked before any device call, so they run without a GPU
+    L = hb.library()
+    assert L.hbg_set_symmetry(None, 1, 0) == 7
+    assert L.hbg_set_walls(None, 0, 1) == 7
+    assert L.hbg_set_field_points(None, None, 0) == 7
+    assert L.hbg_field_values(None, None, None, None, None) == 7
+    assert L.hbg_set_kochin(None, 8) == 7
+    assert L.hbg_kochin_values(None, None, None, None, None) == 7
+    assert b"null" in L.hbg_last_error_message()
+
+
 @pytest.mark.skipif(__import__("torch").cuda.is_available(), reason="GPU present: covered by the GPU tests")
 def test_session_without_gpu_is_a_state_error():
     with pytest.raises(hb.HbgError) as e:
