@@ -73,7 +73,9 @@ __device__ __forceinline__ void cj01x(cx z, cx& j0, cx& j1) {
 constexpr int FQ_NPAN = 64, FQ_NP = FQ_NPAN * 16;  // panels of 16 Gauss points along the path, at most
 constexpr int FQ_NX = FTS * FM;                     // xb values
 constexpr int FQ_NV1 = FTW * FM, FQ_NV = FQ_NV1 + FTV * FM;  // vb values of Phi1, then of Phi2
-constexpr int FQ_VB = 4, FQ_CHUNK = 256;            // vb rows per workgroup of the sums, points per pass
+// vb rows per workgroup of the sums (one: 264 workgroups for 256 CUs; each sum runs in the same
+// point order whatever the rows per workgroup), points per pass
+constexpr int FQ_VB = 1, FQ_CHUNK = 256;
 
 // Work layout (doubles): point count, points k_i and weights w_i, factors J [FQ_NP][FQ_NX][J0, J1],
 // factors V [FQ_NV][FQ_NP][w chv, w chv k, w k shv]
