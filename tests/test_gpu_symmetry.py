@@ -185,3 +185,29 @@ def test_field_points_and_kochin_of_a_sector_equal_the_whole_mesh(sessions, dept
         e = rel_per_entry(res[0][1][key], res[1][1][key])
         say(f"  {key}: sector vs whole {e:.2e}")
         assert e < 1e-9
+
+
+def test_bench_body_as_its_quadrant_equals_the_whole_mesh(sessions):
+    # the 8192-panel finite-depth cylinder of the benchmark and its 2048-panel quadrant with both planes
+    quadrant = meshes.cylinder_panels(1.0, 1.0, 32, 32, 32, quarter=True, cosine=True)
+    whole = meshes.expand_symmetry(quadrant)
+    nu = np.array([0.05, 0.6, 4.0, 20.0]) / 3.0
+    w = np.sqrt(9.81 * nu)
+    heads = [0.0, 30.0]
+    say(f"case: cylinder quadrant {len(quadrant)} of {len(whole)} panels, depth 3, 4 frequencies, FP64 and mixed")
+    s, f = sessions
+    for x in (s, f):
+        x.set_environment(1025.0, 9.81, 3.0)
+    f.set_symmetry(False, False)
+    f.set_panels(whole)
+    ref = f.solve(w, heads)
+    s.set_symmetry(True, True)
+    s.set_panels(quadrant)
+    for precision in ("fp64", "mixed"):
+        got = s.solve(w, heads, precision=precision)
+        for key in KEYS:
+            e = rel_per_entry(got[key], ref[key])
+            say(f"  {precision} {key}: quadrant vs whole {e:.2e}")
+            assert e < 1e-9
+    s.set_symmetry(False, False)
+    f.set_panels(meshes.hemisphere_panels(1.0, 2, 4, quarter=False))  # release the large buffers' use
