@@ -80,12 +80,12 @@ def workload(name):
                     depth=3.0, desc="truncated cylinder R=1 draft=1, finite depth h=3, 8192 panels, "
                                     "48 frequencies nu*h log-spaced 0.05-20, heading 0 (7 RHS), complex128")
     if name == "barge12288":
-        # rectangular barge 80 x 20 m, draft 5 m, infinite depth, 12320 uniform quads of about
-        # 0.46 m (bottom 172 x 44, sides 172 x 11 and 44 x 11), 24 frequencies 0.2..2.0 rad/s,
-        # 8 headings 0..157.5 deg (14 right-hand sides)
-        v = meshes.box_panels(40.0, 10.0, 5.0, (172, 44, 11), quarter=False)
+        # rectangular barge 80 x 20 m, draft 5 m, infinite depth, 12288 quads of about 0.46 m
+        # (bottom 168 x 43 = 7224, sides 2 x 168 x 12 = 4032 and 2 x 43 x 12 = 1032), 24
+        # frequencies 0.2..2.0 rad/s, 8 headings 0..157.5 deg (14 right-hand sides)
+        v = meshes.box_panels(40.0, 10.0, 5.0, (168, 43, 12), quarter=False)
         return dict(name=name, vertices=v, omegas=np.linspace(0.2, 2.0, 24), headings=list(np.arange(8) * 22.5),
-                    rho=1025.0, g=9.81, desc="rectangular barge 80 x 20 m, draft 5 m, infinite depth, 12320 panels, "
+                    rho=1025.0, g=9.81, desc="rectangular barge 80 x 20 m, draft 5 m, infinite depth, 12288 panels, "
                                               "24 omegas 0.2-2.0 rad/s, 8 headings (14 RHS), complex128")
     if name == "hemisphere512":
         v = meshes.hemisphere_panels(1.0, 16, 32, quarter=False)

@@ -69,6 +69,8 @@ SIGNATURES = {
     "hbg_field_values": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "hbg_set_kochin": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "hbg_kochin_values": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "hbg_field_shape": (ctypes.c_int32, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4),
+    "hbg_kochin_shape": (ctypes.c_int32, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4),
     "hbg_solve": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                    ctypes.POINTER(Options), _dp, _dp, _dp, _dp, _dp, _dp, PROGRESS,
                                    ctypes.c_void_p]),
@@ -196,11 +198,20 @@ class Session:
         _check(library().hbg_set_field_points(self._h, _p(p) if len(p) else None, len(p)))
         self.n_points = len(p)
 
-    def field_values(self, n_omegas, n_headings):
-        """The field values of the last solve: frad [F][P][K], finc, fsca, ftot [F][P][H]."""
-        P, K = self.n_points, 6 * self.n_bodies
-        frad = np.zeros((n_omegas, P, K), np.complex128)
-        finc, fsca, ftot = (np.zeros((n_omegas, P, n_headings), np.complex128) for _ in range(3))
+    def _shape(self, name, n_omegas, n_headings):
+        dims = [ctypes.c_int32() for _ in range(4)]
+        _check(getattr(library(), name)(self._h, *[ctypes.byref(d) for d in dims]))
+        F, P, K, H = (d.value for d in dims)
+        if (n_omegas is not None and n_omegas != F) or (n_headings is not None and n_headings != H):
+            raise ValueError(f"the last solve had {F} frequencies and {H} headings, not {n_omegas} and {n_headings}")
+        return F, P, K, H
+
+    def field_values(self, n_omegas=None, n_headings=None):
+        """The field values of the last solve: frad [F][P][K], finc, fsca, ftot [F][P][H]; the sizes
+        are the library's record of that solve (n_omegas, n_headings, when given, must match)."""
+        F, P, K, H = self._shape("hbg_field_shape", n_omegas, n_headings)
+        frad = np.zeros((F, P, K), np.complex128)
+        finc, fsca, ftot = (np.zeros((F, P, H), np.complex128) for _ in range(3))
         _check(library().hbg_field_values(self._h, _p(frad), _p(finc), _p(fsca), _p(ftot)))
         return {"frad": frad, "finc": finc, "fsca": fsca, "ftot": ftot}
 
@@ -209,11 +220,12 @@ class Session:
         _check(library().hbg_set_kochin(self._h, int(n_angles)))
         self.n_kochin = int(n_angles)
 
-    def kochin_values(self, n_omegas, n_headings):
-        """The Kochin functions of the last solve: kr, krt [F][NK][K], kd, kdt [F][NK][H]."""
-        NK, K = self.n_kochin, 6 * self.n_bodies
-        kr, krt = (np.zeros((n_omegas, NK, K), np.complex128) for _ in range(2))
-        kd, kdt = (np.zeros((n_omegas, NK, n_headings), np.complex128) for _ in range(2))
+    def kochin_values(self, n_omegas=None, n_headings=None):
+        """The Kochin functions of the last solve: kr, krt [F][NK][K], kd, kdt [F][NK][H] (sizes as
+        field_values)."""
+        F, NK, K, H = self._shape("hbg_kochin_shape", n_omegas, n_headings)
+        kr, krt = (np.zeros((F, NK, K), np.complex128) for _ in range(2))
+        kd, kdt = (np.zeros((F, NK, H), np.complex128) for _ in range(2))
         _check(library().hbg_kochin_values(self._h, _p(kr), _p(krt), _p(kd), _p(kdt)))
         return {"kr": kr, "krt": krt, "kd": kd, "kdt": kdt}
 

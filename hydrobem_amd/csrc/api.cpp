@@ -78,6 +78,7 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
                          hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta,
                          const int* nsys, int* active);
 int residual_blocks(int n);
+size_t residual_max_elems(int n);
 size_t residual_part_elems(int n, int R);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                             const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
@@ -369,7 +370,6 @@ struct HbgSession {
   DevBuf<double> d_phi1f, d_phi2f, d_fworkf;
   DevBuf<double4> d_pff;
   DevBuf<double2> d_pcsf;
-  DevBuf<FinFreq> d_fff;
   DevBuf<StaticD> d_stf;
   bool stf_valid = false;
   int stf_logint = -1;
@@ -391,6 +391,7 @@ struct HbgSession {
   DevBuf<double4> d_pf;
   DevBuf<double2> d_pcs;
   DevBuf<FinFreq> d_ff;
+  DevBuf<FinFreq> d_ffs;  // a solve's FinFreq of every ordinary frequency, uploaded once
   const double* ktab = nullptr;
   const double* fitm = nullptr;
   // profiling (hbg_set_profiling): GPU milliseconds of the stages, by HIP events on the stream
@@ -518,8 +519,10 @@ FinTables fin_tables(HbgSession* s) {
 // The matrices of one frequency: S (d_S) and A into Mout (leading dimension lda), A32 the
 // complex64 copy (nullable). A symmetric body (G > 1): the G character systems at Mout + c mstride
 // (leading dimension N) and S_c at d_S + c N N, from its image blocks (scratch d_Sg, d_Ag).
+// dff: the frequency's FinFreq already on the device (finite depth; nullptr: uploaded here, which
+// waits for the stream, the host copy being pageable).
 double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, bool surrogate = false,
-                          float2* A32 = nullptr, int lda = 0, size_t mstride = 0) {
+                          float2* A32 = nullptr, int lda = 0, size_t mstride = 0, const FinFreq* dff = nullptr) {
   const int N = s->N, G = s->G;
   if (lda <= 0) lda = N;
   if (G > 1 && (surrogate || lda != N || !mstride))
@@ -553,18 +556,21 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   s->d_fwork.ensure(fin_tables_work());
   launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1.p, s->d_phi2.p, s->fitm, s->d_panels.p, N, s->d_pf.p,
                     s->d_pcs.p, s->d_fwork.p, s->stream);
-  s->d_ff.ensure(1);
-  HBG_HIP(hipMemcpyAsync(s->d_ff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, s->stream));
+  if (!dff) {
+    s->d_ff.ensure(1);
+    HBG_HIP(hipMemcpyAsync(s->d_ff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, s->stream));
+    dff = s->d_ff.p;
+  }
   const double nuh = K * s->depth;
   if (surrogate && s->nn.ready && nuh >= std::exp(NN_LNNU_LO) && nuh <= std::exp(NN_LNNU_HI)) {
     s->d_nnq.ensure((size_t)N * N);
     s->d_nnc.ensure(1);
-    launch_assemble_fin_nn(s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32,
-                           lda, s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
+    launch_assemble_fin_nn(s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, lda,
+                           s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
   } else
     blocks([&](double2* S, double2* Ab, float2* a32, int images) {
-      launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, s->d_ff.p, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32,
-                          lda, 2.0 * PI, s->stream, images);
+      launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32, lda,
+                          2.0 * PI, s->stream, images);
     });
   return f.k0;
 }
@@ -702,49 +708,77 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
   return HBG_OK;
 }
 
-void build_geometry(HbgSession* s);
-static void set_reflections(HbgSession* s);
+// The panels as the caller set them, kept to rebuild the images when the planes or walls change.
+struct RawPanels {
+  std::vector<double> vertices, centres;
+  std::vector<int> body;
+  int N = 0, nb = 0;
+};
+
+// The reflection group of the symmetry planes and walls and its solved characters.
+struct Group {
+  int G = 1, Gs = 1;
+  std::vector<std::array<double, 2>> refl{{1.0, 1.0}};
+  std::vector<double> chi{1.0}, chisym{1.0};  // [Gs][G], [Gs][Gs]
+};
+static Group make_group(int sym_x, int sym_y, int walls);
+static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, int walls, const RawPanels& raw);
+
+static RawPanels raw_of(const HbgSession* s) {
+  RawPanels r;
+  r.vertices = s->raw_vertices;
+  r.centres = s->raw_centres;
+  r.body = s->raw_body;
+  r.N = s->N;
+  r.nb = s->nb;
+  return r;
+}
 
 HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
                          int32_t n_bodies, const double* rotation_centres) {
   if (!s || !vertices) return fail(HBG_ERR_ARGUMENT, "session or vertices is null", "hbg_set_panels");
   if (n_panels <= 0 || n_bodies <= 0) return fail(HBG_ERR_ARGUMENT, "n_panels and n_bodies must be > 0", "hbg_set_panels");
   GUARD_BEGIN
-  s->raw_vertices.assign(vertices, vertices + 12 * (size_t)n_panels);
-  s->raw_body.assign(n_panels, 0);
-  for (int j = 0; j < n_panels; ++j) s->raw_body[j] = body ? body[j] : 0;
-  s->raw_centres.assign(3 * (size_t)n_bodies, 0.0);
-  if (rotation_centres) s->raw_centres.assign(rotation_centres, rotation_centres + 3 * (size_t)n_bodies);
-  s->N = n_panels;
-  s->nb = n_bodies;
-  build_geometry(s);
+  RawPanels raw;
+  raw.vertices.assign(vertices, vertices + 12 * (size_t)n_panels);
+  raw.body.assign(n_panels, 0);
+  for (int j = 0; j < n_panels; ++j) raw.body[j] = body ? body[j] : 0;
+  raw.centres.assign(3 * (size_t)n_bodies, 0.0);
+  if (rotation_centres) raw.centres.assign(rotation_centres, rotation_centres + 3 * (size_t)n_bodies);
+  raw.N = n_panels;
+  raw.nb = n_bodies;
+  // a refused panel leaves the session's geometry (and its statics) as they were
+  set_geometry(s, make_group(s->sym_x, s->sym_y, s->walls), s->sym_x, s->sym_y, s->walls, raw);
   return HBG_OK;
   GUARD_END("hbg_set_panels")
 }
 
-// The device panels of the session: the panels as set and, with symmetry planes, their images
-// in the order of the reflections (geometry.rs:209-234: the vertices reflected, their order
-// reversed where one reflection reverses the orientation), each with its body's modes.
-void build_geometry(HbgSession* s) {
-  const int n_panels = s->N, n_bodies = s->nb, G = s->G, NF = G * n_panels;
+// The device panels of a geometry: the panels as set and, with symmetry planes or walls, their
+// images in the order of the group's reflections (geometry.rs:209-234: the vertices reflected,
+// their order reversed where one reflection reverses the orientation), each with its body's
+// modes. Everything is built and checked on the host first; the session (group, panels, raw
+// arrays, device copies) changes only when every panel passed, and every result derived from the
+// previous geometry (statics, field statics, field and Kochin values) is invalidated.
+static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, int walls, const RawPanels& raw) {
+  const int n_panels = raw.N, n_bodies = raw.nb, G = grp.G, NF = G * n_panels;
   std::vector<PanelD> panels(NF);
   std::vector<int> b(NF, 0);
   std::vector<double> nj(6 * (size_t)NF);
   for (int g = 0; g < G; ++g)
     for (int j = 0; j < n_panels; ++j) {
-      const double sx = s->refl[g][0], sy = s->refl[g][1];
-      if (g == 0 && s->walls)  // the body in the fluid: x >= 0 (a wall at x = 0), y >= 0 (y = 0)
+      const double sx = grp.refl[g][0], sy = grp.refl[g][1];
+      if (g == 0 && walls)  // the body in the fluid: x >= 0 (a wall at x = 0), y >= 0 (y = 0)
         for (int q = 0; q < 4; ++q)
           for (int a = 0; a < 2; ++a)
-            if ((s->walls >> a & 1) && s->raw_vertices[12 * (size_t)j + 3 * q + a] < 0.0)
+            if ((walls >> a & 1) && raw.vertices[12 * (size_t)j + 3 * q + a] < 0.0)
               throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": behind the wall " +
                                              (a ? "y" : "x") + " = 0"};
       double v[12];
       for (int q = 0; q < 4; ++q) {
         const int src = (sx * sy < 0.0 && q > 0) ? 4 - q : q;  // 0, 3, 2, 1 where the orientation flips
-        v[3 * q] = sx * s->raw_vertices[12 * (size_t)j + 3 * src];
-        v[3 * q + 1] = sy * s->raw_vertices[12 * (size_t)j + 3 * src + 1];
-        v[3 * q + 2] = s->raw_vertices[12 * (size_t)j + 3 * src + 2];
+        v[3 * q] = sx * raw.vertices[12 * (size_t)j + 3 * src];
+        v[3 * q + 1] = sy * raw.vertices[12 * (size_t)j + 3 * src + 1];
+        v[3 * q + 2] = raw.vertices[12 * (size_t)j + 3 * src + 2];
       }
       V3 proj[4], poly[4], normal, centroid;
       int m;
@@ -756,15 +790,15 @@ void build_geometry(HbgSession* s) {
                     "panel " + std::to_string(j + 1) + ": centroid not below z = 0 (surfaces in z = 0 are not supported)"};
       const size_t o = (size_t)g * n_panels + j;
       panels[o] = panel_record(proj, normal, centroid, area);
-      const int bi = s->raw_body[j];
+      const int bi = raw.body[j];
       if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
       b[o] = bi;
-      if (g >= s->Gs) {  // a wall image moves as the mirror image of its real panel (lo_geometry.rs:981)
-        const size_t r = (size_t)(g % s->Gs) * n_panels + j;
+      if (g >= grp.Gs) {  // a wall image moves as the mirror image of its real panel (lo_geometry.rs:981)
+        const size_t r = (size_t)(g % grp.Gs) * n_panels + j;
         for (int d = 0; d < 6; ++d) nj[6 * o + d] = nj[6 * r + d];
         continue;
       }
-      V3 rc = v3(s->raw_centres[3 * bi], s->raw_centres[3 * bi + 1], s->raw_centres[3 * bi + 2]);
+      V3 rc = v3(raw.centres[3 * bi], raw.centres[3 * bi + 1], raw.centres[3 * bi + 2]);
       V3 mom = cross(centroid - rc, normal);
       for (int d = 0; d < 3; ++d) {
         nj[6 * o + d] = normal[d];
@@ -772,17 +806,42 @@ void build_geometry(HbgSession* s) {
       }
     }
   use_device(s);
-  s->panels = panels;
-  s->body = b;
-  s->nj = nj;
-  s->d_panels.ensure(panels.size());
-  s->d_body.ensure(b.size());
-  s->d_nj.ensure(nj.size());
-  HBG_HIP(hipMemcpy(s->d_panels.p, panels.data(), panels.size() * sizeof(PanelD), hipMemcpyHostToDevice));
-  HBG_HIP(hipMemcpy(s->d_body.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
-  HBG_HIP(hipMemcpy(s->d_nj.p, nj.data(), nj.size() * sizeof(double), hipMemcpyHostToDevice));
+  // from here the old geometry is gone: a failing upload leaves a session without panels
   s->statics_valid = false;
+  s->stinf_valid = false;
   s->stf_valid = false;
+  s->field_F = -1;
+  s->kochin_F = -1;
+  s->N = 0;
+  s->nb = 0;
+  s->sym_x = sym_x;
+  s->sym_y = sym_y;
+  s->walls = walls;
+  s->G = grp.G;
+  s->Gs = grp.Gs;
+  s->refl = grp.refl;
+  s->chi = grp.chi;
+  s->chisym = grp.chisym;
+  s->d_chi.ensure(s->chi.size());
+  HBG_HIP(hipMemcpy(s->d_chi.p, s->chi.data(), s->chi.size() * sizeof(double), hipMemcpyHostToDevice));
+  s->d_chisym.ensure(s->chisym.size());
+  HBG_HIP(hipMemcpy(s->d_chisym.p, s->chisym.data(), s->chisym.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (n_panels > 0) {
+    s->d_panels.ensure(panels.size());
+    s->d_body.ensure(b.size());
+    s->d_nj.ensure(nj.size());
+    HBG_HIP(hipMemcpy(s->d_panels.p, panels.data(), panels.size() * sizeof(PanelD), hipMemcpyHostToDevice));
+    HBG_HIP(hipMemcpy(s->d_body.p, b.data(), b.size() * sizeof(int), hipMemcpyHostToDevice));
+    HBG_HIP(hipMemcpy(s->d_nj.p, nj.data(), nj.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  s->panels = std::move(panels);
+  s->body = std::move(b);
+  s->nj = std::move(nj);
+  s->raw_vertices = raw.vertices;
+  s->raw_centres = raw.centres;
+  s->raw_body = raw.body;
+  s->N = n_panels;
+  s->nb = n_bodies;
 }
 
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
@@ -792,9 +851,8 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
   if ((sym_x && (s->walls & 1)) || (sym_y && (s->walls & 2)))
     return fail(HBG_ERR_VALUE, "a symmetry plane where a wall stands", "hbg_set_symmetry");
   GUARD_BEGIN
-  s->sym_x = sym_x;
-  s->sym_y = sym_y;
-  set_reflections(s);
+  // panels that fail as images leave the planes, the walls and the geometry as they were
+  set_geometry(s, make_group(sym_x, sym_y, s->walls), sym_x, sym_y, s->walls, raw_of(s));
   return HBG_OK;
   GUARD_END("hbg_set_symmetry")
 }
@@ -847,6 +905,28 @@ HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fs
   return HBG_OK;
 }
 
+HbgStatus hbg_field_shape(HbgSession* s, int32_t* n_omegas, int32_t* n_points, int32_t* n_rad, int32_t* n_headings) {
+  if (!s || !n_omegas || !n_points || !n_rad || !n_headings)
+    return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_field_shape");
+  if (s->field_F < 0) return fail(HBG_ERR_STATE, "no solve with field points yet", "hbg_field_shape");
+  *n_omegas = s->field_F;
+  *n_points = s->np;
+  *n_rad = s->field_nrad;
+  *n_headings = s->field_H;
+  return HBG_OK;
+}
+
+HbgStatus hbg_kochin_shape(HbgSession* s, int32_t* n_omegas, int32_t* n_angles, int32_t* n_rad, int32_t* n_headings) {
+  if (!s || !n_omegas || !n_angles || !n_rad || !n_headings)
+    return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_kochin_shape");
+  if (s->kochin_F < 0) return fail(HBG_ERR_STATE, "no solve with Kochin angles yet", "hbg_kochin_shape");
+  *n_omegas = s->kochin_F;
+  *n_angles = s->nk;
+  *n_rad = s->kochin_nrad;
+  *n_headings = s->kochin_H;
+  return HBG_OK;
+}
+
 HbgStatus hbg_set_kochin(HbgSession* s, int32_t n_angles) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_set_kochin");
   if (n_angles < 0 || n_angles > 1 << 20) return fail(HBG_ERR_VALUE, "n_angles out of range", "hbg_set_kochin");
@@ -886,21 +966,13 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
   if ((wall_x && s->sym_x) || (wall_y && s->sym_y))
     return fail(HBG_ERR_VALUE, "a wall where a symmetry plane stands", "hbg_set_walls");
   GUARD_BEGIN
-  const int before = s->walls;
-  s->walls = wall_x | wall_y << 1;
-  try {
-    set_reflections(s);
-  } catch (...) {  // panels behind a new wall: the walls as they were
-    s->walls = before;
-    set_reflections(s);
-    throw;
-  }
+  // panels behind a new wall leave the walls and the geometry as they were
+  const int walls = wall_x | wall_y << 1;
+  set_geometry(s, make_group(s->sym_x, s->sym_y, walls), s->sym_x, s->sym_y, walls, raw_of(s));
   return HBG_OK;
   GUARD_END("hbg_set_walls")
 }
 
-// The reflections e, sigma_x, sigma_y, sigma_x sigma_y (those present) and the characters of their
-// group, for symmetry planes or walls; rebuilds the device panels when panels are set.
 // The group of the symmetry planes and the walls (lo_geometry.rs:863-945), its elements as bit
 // masks (bit 0: x -> -x, bit 1: y -> -y): the symmetry elements first, then each of them times
 // the wall reflections, so that the real panels (the sector and its images under the symmetry)
@@ -909,8 +981,8 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
 // sigma_x, sigma_y, sigma_x sigma_y); the solved ones, even under every wall reflection
 // (lo_geometry.rs:1010-1030), are the Gs masks c of the symmetry: chis [Gs][G] for the assembly,
 // chisym [Gs][Gs] over the symmetry elements for the right-hand sides and the recombination.
-static void set_reflections(HbgSession* s) {
-  const int smask = s->sym_x | s->sym_y << 1, wmask = s->walls;
+static Group make_group(int sym_x, int sym_y, int walls) {
+  const int smask = sym_x | sym_y << 1, wmask = walls;
   std::vector<int> sym{0}, wal{0};
   for (int m = 1; m < 4; ++m) {
     if ((m & smask) == m) sym.push_back(m);
@@ -919,23 +991,19 @@ static void set_reflections(HbgSession* s) {
   std::vector<int> masks;
   for (int w : wal)
     for (int g : sym) masks.push_back(w | g);
-  s->G = (int)masks.size();
-  s->Gs = (int)sym.size();
-  s->refl.clear();
-  for (int m : masks) s->refl.push_back({(m & 1) ? -1.0 : 1.0, (m & 2) ? -1.0 : 1.0});
+  Group grp;
+  grp.G = (int)masks.size();
+  grp.Gs = (int)sym.size();
+  grp.refl.clear();
+  for (int m : masks) grp.refl.push_back({(m & 1) ? -1.0 : 1.0, (m & 2) ? -1.0 : 1.0});
   auto chi = [](int c, int g) { return __builtin_popcount(c & g) & 1 ? -1.0 : 1.0; };
-  s->chi.assign((size_t)s->Gs * s->G, 0.0);
-  s->chisym.assign((size_t)s->Gs * s->Gs, 0.0);
-  for (int r = 0; r < s->Gs; ++r) {
-    for (int g = 0; g < s->G; ++g) s->chi[(size_t)r * s->G + g] = chi(sym[r], masks[g]);
-    for (int g = 0; g < s->Gs; ++g) s->chisym[(size_t)r * s->Gs + g] = chi(sym[r], masks[g]);
+  grp.chi.assign((size_t)grp.Gs * grp.G, 0.0);
+  grp.chisym.assign((size_t)grp.Gs * grp.Gs, 0.0);
+  for (int r = 0; r < grp.Gs; ++r) {
+    for (int g = 0; g < grp.G; ++g) grp.chi[(size_t)r * grp.G + g] = chi(sym[r], masks[g]);
+    for (int g = 0; g < grp.Gs; ++g) grp.chisym[(size_t)r * grp.Gs + g] = chi(sym[r], masks[g]);
   }
-  use_device(s);
-  s->d_chi.ensure(s->chi.size());
-  HBG_HIP(hipMemcpy(s->d_chi.p, s->chi.data(), s->chi.size() * sizeof(double), hipMemcpyHostToDevice));
-  s->d_chisym.ensure(s->chisym.size());
-  HBG_HIP(hipMemcpy(s->d_chisym.p, s->chisym.data(), s->chisym.size() * sizeof(double), hipMemcpyHostToDevice));
-  if (s->N > 0) build_geometry(s);
+  return grp;
 }
 
 HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64_t n, double* out) {
@@ -1215,6 +1283,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
   s->d_heads.ensure(heads.size());
   HBG_HIP(hipMemcpy(s->d_heads.p, heads.data(), heads.size() * sizeof(double), hipMemcpyHostToDevice));
+  // finite depth: every frequency's roots and coefficients on the device before the first launch,
+  // so that no pageable copy (which waits for the stream) sits between the batches' launches
+  if (std::isfinite(s->depth) && n_omegas > 0) {
+    std::vector<FinFreq> ffs(n_omegas);
+    for (int i = 0; i < n_omegas; ++i) ffs[i] = make_fin_freq(s->depth, omegas[i] * omegas[i] / s->grav);
+    s->d_ffs.ensure(ffs.size());
+    HBG_HIP(hipMemcpy(s->d_ffs.p, ffs.data(), ffs.size() * sizeof(FinFreq), hipMemcpyHostToDevice));
+  }
   const size_t nAB = (size_t)D * nrad, nX = (size_t)H * D, nHK = (size_t)nrad * H;
   const size_t nout = 2 * nAB + 6 * nX + 2 * nHK;
   // frequencies in flight: each holds its [A | B] (N x (N + R)) and incident-wave values until
@@ -1250,7 +1326,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     s->d_M32.ensure(mstride * BG * halves);
     s->d_D32.ensure((size_t)N * R * BG);
     s->d_X64.ensure((size_t)N * R * BG);
-    s->d_rpart.ensure((size_t)residual_blocks(N) * 4 * BG);
+    s->d_rpart.ensure(residual_max_elems(N) * BG);
     s->d_rpart2.ensure(residual_part_elems(N, R) * BG);
     s->d_rscale.ensure((size_t)R * BG);
     s->d_active.ensure((size_t)BG);
@@ -1277,7 +1353,6 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       s->d_pff.ensure((size_t)NS + s->np);
       s->d_pcsf.ensure(((size_t)NS + s->np) * FIN_ROOTS);
       s->d_fworkf.ensure(fin_tables_work());
-      s->d_fff.ensure(1);
     }
     hfield.resize(fstride * B);
     s->field_vals.assign(fstride * (size_t)n_omegas, make_double2(0.0, 0.0));
@@ -1302,7 +1377,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     double2* Mb = s->d_M.p + mstride * slot(h, b) * G;
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], sa));
     const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0,
-                                         mixed ? s->d_M32.p + mstride * slot(h, b) * G : nullptr, N, mstride);
+                                         mixed ? s->d_M32.p + mstride * slot(h, b) * G : nullptr, N, mstride,
+                                         std::isfinite(s->depth) ? s->d_ffs.p + fi : nullptr);
     launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
                     s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b), s->d_u.p, sa, s->walls);
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], sa));
@@ -1443,16 +1519,16 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
           // the frequency's tables for the panels and the points, on the LU stream
           const FinFreq f = make_fin_freq(s->depth, K);
           kw = f.k0;
-          HBG_HIP(hipMemcpyAsync(s->d_fff.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice, sb));
           launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1f.p, s->d_phi2f.p, s->fitm, s->d_fcomb.p,
                             NS + s->np, s->d_pff.p, s->d_pcsf.p, s->d_fworkf.p, sb);
           Ff = FinTables{s->d_phi1f.p, s->d_phi2f.p, s->ktab, s->d_pff.p, s->d_pcsf.p};
         }
+        // the tables of the next frequency overwrite these on the same stream, after this launch
         launch_field_values(s->d_fpts.p, s->d_panels.p, s->d_body.p, s->d_nj.p, s->np, NS, NF, nrad, H, w, K, kw,
-                            std::isfinite(s->depth) ? s->depth : 0.0, s->d_fff.p, Ff, logc, s->T, s->grav,
+                            std::isfinite(s->depth) ? s->depth : 0.0,
+                            std::isfinite(s->depth) ? s->d_ffs.p + f0 + b : nullptr, Ff, logc, s->T, s->grav,
                             s->d_heads.p, s->walls, s->d_stf.p, s->d_Sx.p, s->d_Ax.p, solb,
                             s->d_phiI.p + hstride * slot(h, b), s->d_fout.p + fstride * slot(h, b), sb);
-        if (std::isfinite(s->depth)) HBG_HIP(hipStreamSynchronize(sb));  // f (host) and the tables are reused
       }
     }
     if (prof) HBG_HIP(hipEventRecord(s->bev[e1], sb));
@@ -1580,6 +1656,12 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     ensure_statics(si, o.logarithmic ? 1 : 0, o.log_tolerance);
     si->d_heads.ensure(heads.size());
     HBG_HIP(hipMemcpy(si->d_heads.p, heads.data(), heads.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (std::isfinite(si->depth) && n_omegas > 0) {  // the frequencies' roots, uploaded once
+      std::vector<FinFreq> ffs(n_omegas);
+      for (int f = 0; f < n_omegas; ++f) ffs[f] = make_fin_freq(si->depth, omegas[f] * omegas[f] / si->grav);
+      si->d_ffs.ensure(ffs.size());
+      HBG_HIP(hipMemcpy(si->d_ffs.p, ffs.data(), ffs.size() * sizeof(FinFreq), hipMemcpyHostToDevice));
+    }
   }
   for (int i = 0; i < count; ++i) sessions[i]->stream = lead->stream;
   hipStream_t st = lead->stream;
@@ -1599,7 +1681,7 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     lead->d_M32.ensure(mstride_max * B);
     lead->d_D32.ensure((size_t)nmax * Rmax * B);
     lead->d_X64.ensure((size_t)nmax * Rmax * B);
-    lead->d_rpart.ensure((size_t)residual_blocks(nmax) * 4 * B);
+    lead->d_rpart.ensure(residual_max_elems(nmax) * B);
     lead->d_rpart2.ensure(residual_part_elems(nmax, Rmax) * B);
     lead->d_rscale.ensure((size_t)Rmax * B);
     lead->d_active.ensure((size_t)B);
@@ -1628,7 +1710,8 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     const double depth = std::isfinite(si->depth) ? si->depth : 0.0;
     double2* Mb = lead->d_M.p + mstride * b;
     float2* M32b = mixed ? lead->d_M32.p + mstride * b : nullptr;
-    const double kw = assemble_frequency(si, K, logc, Mb, o.surrogate != 0, M32b, n);
+    const double kw = assemble_frequency(si, K, logc, Mb, o.surrogate != 0, M32b, n, 0,
+                                         std::isfinite(si->depth) ? si->d_ffs.p + fi : nullptr);
     launch_boundary(si->d_panels.p, si->d_body.p, si->d_nj.p, N, nrad[i], H, w, kw, si->grav, depth, si->d_heads.p,
                     si->d_phiI.p + hstride[i] * b, si->d_dphiI.p + hstride[i] * b, si->d_u.p, st);
     launch_rhs(si->d_S.p, N, n, si->d_u.p, Rs[i], Mb + (size_t)(n - N) * n, n, si->d_part.p, st);
@@ -1716,6 +1799,23 @@ HbgStatus hbg_motions(HbgSession* s, int32_t n_omegas, const double* omegas, int
   if (n_dof > motions_max_dof() || n_headings > motions_max_headings())
     return fail(HBG_ERR_UNSUPPORTED, "more than 64 degrees of freedom or 32 headings", "hbg_motions");
   if (n_omegas == 0 || n_headings == 0) return HBG_OK;
+  {
+    // every input finite before anything runs (the elimination would carry a NaN into its pivots)
+    const size_t F = n_omegas, D = n_dof, H = n_headings;
+    auto finite = [](const double* a, size_t n) {
+      for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) return false;
+      return true;
+    };
+    if (!finite(omegas, F)) return fail(HBG_ERR_VALUE, "non-finite omega", "hbg_motions");
+    if (!finite(mass, D * D)) return fail(HBG_ERR_VALUE, "non-finite mass matrix entry", "hbg_motions");
+    if (!finite(stiffness, D * D)) return fail(HBG_ERR_VALUE, "non-finite stiffness matrix entry", "hbg_motions");
+    if (external_damping && !finite(external_damping, D * D))
+      return fail(HBG_ERR_VALUE, "non-finite external damping entry", "hbg_motions");
+    if (!finite(added_mass, F * D * D)) return fail(HBG_ERR_VALUE, "non-finite added mass entry", "hbg_motions");
+    if (!finite(damping, F * D * D)) return fail(HBG_ERR_VALUE, "non-finite damping entry", "hbg_motions");
+    if (!finite(excitation, 2 * F * H * D)) return fail(HBG_ERR_VALUE, "non-finite excitation entry", "hbg_motions");
+  }
   GUARD_BEGIN
   use_device(s);
   const size_t F = n_omegas, D = n_dof, H = n_headings;

@@ -147,6 +147,12 @@ __global__ void __launch_bounds__(RES_THREADS) residual_part_kernel(const double
   o[(size_t)R * n] = double2(rowsum, 0.0);
 }
 
+// a maximum that keeps a NaN operand as +inf (fmax drops it), so that a non-finite residual,
+// solution or right-hand side can never pass the tolerance
+__device__ __forceinline__ double max_keep_nan(double acc, double v) { return v != v ? INFINITY : fmax(acc, v); }
+
+// Partial maxima per workgroup: for each right-hand side c, max |r_ic|, max |x_ic|, max |b_ic|
+// over its rows (|re| + |im|), then max_i sum_j |a_ij|, into part[z][block][3 R + 1].
 __global__ void __launch_bounds__(RES_THREADS) residual_sum_kernel(const double2* __restrict__ M, size_t mstride,
                                                                    int n, const double2* __restrict__ X,
                                                                    size_t xstride, int R,
@@ -155,17 +161,24 @@ __global__ void __launch_bounds__(RES_THREADS) residual_sum_kernel(const double2
                                                                    double* __restrict__ part,
                                                                    const int* __restrict__ active) {
   if (active[blockIdx.z] == 0) return;
-  __shared__ double red[4][RES_THREADS / 64];
+  __shared__ double red[3 * RES_MAXR + 1][RES_THREADS / 64];
   M += blockIdx.z * mstride;
   X += blockIdx.z * xstride;
   D += blockIdx.z * dstride;
   const int tid = threadIdx.x;
   const int i = blockIdx.x * RES_THREADS + tid;
-  double rmax = 0.0, rowsum = 0.0, xmax = 0.0, bmax = 0.0;
-  if (i < n) {
-    const double2* p = part2 + ((size_t)blockIdx.z * RES_SPLIT * (R + 1)) * n + i;
+  const bool live = i < n;
+  const double2* p = part2 + ((size_t)blockIdx.z * RES_SPLIT * (R + 1)) * n + i;
+  double rowsum = 0.0;
+  if (live)
     for (int cy = 0; cy < RES_SPLIT; ++cy) rowsum += p[((size_t)cy * (R + 1) + R) * n].x;
-    for (int c = 0; c < R; ++c) {
+  auto reduce = [&](double v, int slot) {
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((tid & 63) == 0) red[slot][tid >> 6] = v;
+  };
+  for (int c = 0; c < R; ++c) {
+    double rm = 0.0, xm = 0.0, bm = 0.0;
+    if (live) {
       double2 sacc = double2(0.0, 0.0);
       for (int cy = 0; cy < RES_SPLIT; ++cy) {
         const double2 v = p[((size_t)cy * (R + 1) + c) * n];
@@ -176,24 +189,47 @@ __global__ void __launch_bounds__(RES_THREADS) residual_sum_kernel(const double2
       const double2 x = X[(size_t)c * n + i];
       const double rx = b.x - sacc.x, ry = b.y - sacc.y;
       D[(size_t)c * n + i] = float2((float)rx, (float)ry);
-      rmax = fmax(rmax, fabs(rx) + fabs(ry));
-      xmax = fmax(xmax, fabs(x.x) + fabs(x.y));
-      bmax = fmax(bmax, fabs(b.x) + fabs(b.y));
+      rm = max_keep_nan(0.0, fabs(rx) + fabs(ry));
+      xm = max_keep_nan(0.0, fabs(x.x) + fabs(x.y));
+      bm = max_keep_nan(0.0, fabs(b.x) + fabs(b.y));
     }
+    reduce(rm, 3 * c);
+    reduce(xm, 3 * c + 1);
+    reduce(bm, 3 * c + 2);
   }
-  double v[4] = {rmax, rowsum, xmax, bmax};
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    for (int off = 32; off >= 1; off >>= 1) v[q] = fmax(v[q], __shfl_xor(v[q], off));
-  if ((tid & 63) == 0)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) red[q][tid >> 6] = v[q];
+  reduce(max_keep_nan(0.0, rowsum), 3 * R);
   __syncthreads();
-  if (tid < 4) {
+  for (int q = tid; q <= 3 * R; q += RES_THREADS) {
     double m = 0.0;
-    for (int w = 0; w < RES_THREADS / 64; ++w) m = fmax(m, red[tid][w]);
-    part[((size_t)blockIdx.z * gridDim.x + blockIdx.x) * 4 + tid] = m;
+    for (int w = 0; w < RES_THREADS / 64; ++w) m = fmax(m, red[q][w]);
+    part[((size_t)blockIdx.z * gridDim.x + blockIdx.x) * (3 * RES_MAXR + 1) + q] = m;
   }
+}
+
+// The backward error of every system still refined, column by column (a right-hand side of small
+// magnitude is held to the tolerance on its own scale):
+//   eta = max_c |r_c| / (|A| |x_c| + |b_c|)  (infinity norms, |re| + |im|),
+// 0 for a zero residual, +inf for a non-finite one. One wavefront per system.
+__global__ void __launch_bounds__(64) eta_kernel(const double* __restrict__ part, int nblk, int R,
+                                                 const int* __restrict__ active, double* __restrict__ eta) {
+  const int z = blockIdx.x, tid = threadIdx.x;
+  if (active[z] == 0) return;
+  const double* p = part + (size_t)z * nblk * (3 * RES_MAXR + 1);
+  double rowsum = 0.0;
+  for (int k = tid; k < nblk; k += 64) rowsum = fmax(rowsum, p[(size_t)k * (3 * RES_MAXR + 1) + 3 * R]);
+  for (int off = 32; off >= 1; off >>= 1) rowsum = fmax(rowsum, __shfl_xor(rowsum, off));
+  double e = 0.0;
+  for (int c = 0; c < R; ++c) {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int k = tid; k < nblk; k += 64)
+      for (int q = 0; q < 3; ++q) v[q] = fmax(v[q], p[(size_t)k * (3 * RES_MAXR + 1) + 3 * c + q]);
+    for (int q = 0; q < 3; ++q)
+      for (int off = 32; off >= 1; off >>= 1) v[q] = fmax(v[q], __shfl_xor(v[q], off));
+    const double den = rowsum * v[1] + v[2];
+    const double ec = v[0] == 0.0 ? 0.0 : (den > 0.0 ? v[0] / den : INFINITY);
+    e = max_keep_nan(e, ec);
+  }
+  if (tid == 0) eta[z] = e;
 }
 
 // X += D (the correction, complex64) for the systems still refined
@@ -231,12 +267,14 @@ void lu_resolve32(const float2* M, size_t mstride, int batch, int n, int lda, fl
                   int nrhs, hipStream_t stream, const int* active);
 
 int residual_blocks(int n) { return (n + RES_THREADS - 1) / RES_THREADS; }
+// doubles of the per-workgroup maxima of the residual, per system (the eta of the system last)
+size_t residual_max_elems(int n) { return (size_t)residual_blocks(n) * (3 * RES_MAXR + 1) + 1; }
 // the chunk partials of the residual, per system
 size_t residual_part_elems(int n, int R) { return (size_t)RES_SPLIT * (R + 1) * n; }
 
 // Solve the `batch` FP64 systems M64 (n x (n + R), ld n, stride n (n + R)) in mixed precision:
 // M32 (same shape, complex64; its first n columns already written by the assembly when
-// a32_written) and D (n x R complex64), X (n x R complex128), part (batch x residual_blocks(n) x 4),
+// a32_written) and D (n x R complex64), X (n x R complex128), part (batch x residual_max_elems(n)),
 // part2 (batch x residual_part_elems(n, R)) and active (batch ints) are workspaces; the systems
 // that meet the tolerance take no further step (their solution is the one checked). On return the
 // solution is in the columns
@@ -258,7 +296,9 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
   }
   lu_solve_batched32(M32, ms, batch, n, n, R, stream, stats, status, nsys);
   widen_kernel<<<dim3(grid_for(xs), 1, batch), 256, 0, stream>>>(M32, ms, n, X, xs, R, rscale);
-  std::vector<double> hp((size_t)batch * nblk * 4);
+  // the per-workgroup maxima first, then one eta per system after them
+  double* const deta = part + (size_t)batch * nblk * (3 * RES_MAXR + 1);
+  std::vector<double> he(batch);
   for (int b = 0; b < batch; ++b) {
     iters[b] = 0;
     eta[b] = INFINITY;
@@ -288,16 +328,13 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
       residual_part_kernel<RES_MAXR><<<rg, RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, active);
     residual_sum_kernel<<<dim3(nblk, 1, batch), RES_THREADS, 0, stream>>>(M64, ms, n, X, xs, R, part2, D, xs, part,
                                                                           active);
-    HBG_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    eta_kernel<<<batch, 64, 0, stream>>>(part, nblk, R, active, deta);
+    HBG_HIP(hipMemcpyAsync(he.data(), deta, batch * sizeof(double), hipMemcpyDeviceToHost, stream));
     HBG_HIP(hipStreamSynchronize(stream));
     open = false;
     for (int b = 0; b < batch; ++b) {
       if (!act[b]) continue;
-      double v[4] = {0, 0, 0, 0};
-      for (int k = 0; k < nblk; ++k)
-        for (int q = 0; q < 4; ++q) v[q] = std::max(v[q], hp[((size_t)b * nblk + k) * 4 + q]);
-      const double den = v[1] * v[2] + v[3];
-      eta[b] = den > 0.0 ? v[0] / den : 0.0;
+      eta[b] = he[b];
       if (!(eta[b] <= tol) && iters[b] < max_iter)
         open = true;
       else

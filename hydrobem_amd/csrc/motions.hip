@@ -36,11 +36,13 @@ __global__ void __launch_bounds__(64) motions_kernel(int D, int H, const double*
   __syncthreads();
   int zp = 0;
   for (int j = 0; j < D; ++j) {
-    // pivot: the largest |re| + |im| in column j from row j down, the first on ties
+    // pivot: the largest |re| + |im| in column j from row j down, the first on ties; a NaN
+    // magnitude counts as the largest, so that every lane agrees on a row in [j, D)
     double v = -1.0;
     int r = 0x7fffffff;
     if (i >= j && i < D) {
       v = fabs(Z[i][j].x) + fabs(Z[i][j].y);
+      if (v != v) v = INFINITY;
       r = i;
     }
     for (int off = 32; off >= 1; off >>= 1) {
@@ -51,7 +53,7 @@ __global__ void __launch_bounds__(64) motions_kernel(int D, int H, const double*
         r = r2;
       }
     }
-    if (v == 0.0) {
+    if (v == 0.0 || r < j || r >= D) {  // no usable pivot (r outside the rows is never taken)
       if (zp == 0) zp = j + 1;
       continue;
     }

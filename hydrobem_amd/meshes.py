@@ -1,110 +1,130 @@
 """Procedural low-order meshes of the solve's workloads (normals out of the body into the fluid).
 
-The generators give the panel layout of the reference's test meshes vertex for vertex
-(tests/python/meshes.py: orient :10-13, face_panels :16-24, box_panels :27-46,
-hemisphere_panels :49-71, _cosine :74-78, cylinder_panels :81-112): the same vertex order,
-the same orientation flips, triangles as a repeated last vertex. Arrays are (P, 4, 3) float64.
+Arrays are (P, 4, 3) float64, one quadrilateral per panel, a triangle written as a quadrilateral
+whose last vertex repeats the third. The panel layout (vertex order, orientation, panel order and
+the coordinates to the last bit) is that of the reference's test meshes (tests/python/meshes.py:
+box_panels, hemisphere_panels, cylinder_panels), which the solve's workloads name; the fixtures of
+tests/golden/meshes_layout.npz hold it and tests/test_meshes.py requires it bit for bit.
+
+The construction here is array-wise: each surface is a grid of vertices (or, for the flat faces of
+a box, a grid of panel origins with the edge vectors added), the quadrilaterals are read off the
+grid as (i, j), (i, j+1), (i+1, j+1), (i+1, j), and one vectorised test against the surface's
+outward direction reverses the ones whose diagonal cross product points into the body.
 """
 
 from __future__ import annotations
 
 import numpy as np
 
+__all__ = ["box_panels", "hemisphere_panels", "cylinder_panels", "expand_symmetry", "place"]
 
-def orient(quad, normal):
-    """The quad as given if its diagonal cross product points along `normal`, else reversed."""
-    q = np.asarray(quad, dtype=float)
-    n = np.cross(q[2] - q[0], q[3] - q[1])
-    return q if np.dot(n, normal) > 0 else q[[0, 3, 2, 1]]
+# the order of a quadrilateral with its orientation reversed (the first vertex kept)
+_REVERSED = [0, 3, 2, 1]
 
 
-def face_panels(origin, du, dv, nu, nv, normal):
-    origin, du, dv = (np.asarray(v, dtype=float) for v in (origin, du, dv))
-    out = []
+def _outward(quads, direction):
+    """Each quadrilateral whose diagonal cross product (v2 - v0) x (v3 - v1) has a positive
+    component along its outward `direction` ([P, 3] or [3]) as it is, the others reversed."""
+    d1 = quads[:, 2] - quads[:, 0]
+    d2 = quads[:, 3] - quads[:, 1]
+    n = np.stack([d1[:, 1] * d2[:, 2] - d1[:, 2] * d2[:, 1],
+                  d1[:, 2] * d2[:, 0] - d1[:, 0] * d2[:, 2],
+                  d1[:, 0] * d2[:, 1] - d1[:, 1] * d2[:, 0]], axis=1)
+    keep = np.einsum("pk,pk->p", n, np.broadcast_to(direction, n.shape)) > 0
+    return np.where(keep[:, None, None], quads, quads[:, _REVERSED])
+
+
+def _grid_quads(V):
+    """Quadrilaterals of a vertex grid V [I + 1][J + 1][3], row-major over (i, j)."""
+    q = np.stack([V[:-1, :-1], V[:-1, 1:], V[1:, 1:], V[1:, :-1]], axis=2)
+    return q.reshape(-1, 4, 3)
+
+
+def _close_last_ring(quads, per_ring):
+    """The last ring's quadrilaterals (a collapsed edge at the pole or the axis) as triangles:
+    the fourth vertex replaced by the third."""
+    quads[-per_ring:, 3] = quads[-per_ring:, 2]
+    return quads
+
+
+def _flat_face(origin, du, dv, nu, nv, direction):
+    """nu x nv panels of the parallelogram origin + s du + t dv, panel (i, j) at
+    origin + (du/nu) i + (dv/nv) j with the corners p, p + a, p + a + b, p + b (i-major)."""
+    origin, du, dv = (np.asarray(x, dtype=float) for x in (origin, du, dv))
     a, b = du / nu, dv / nv
-    for i in range(nu):
-        for j in range(nv):
-            p = origin + a * i + b * j
-            out.append(orient([p, p + a, p + a + b, p + b], normal))
-    return out
+    i = np.arange(nu)[:, None, None]
+    j = np.arange(nv)[None, :, None]
+    p = (origin + a * i + b * j).reshape(-1, 3)
+    pa = p + a
+    q = np.stack([p, pa, pa + b, p + b], axis=1)
+    return _outward(q, np.asarray(direction, dtype=float))
 
 
-def box_panels(a=1.0, b=0.5, d=0.5, n=(4, 2, 2), quarter=False):
-    """Surface-piercing box |x| <= a, |y| <= b, -d <= z <= 0: bottom nx*ny, x faces ny*nz,
-    y faces nx*nz panels (quarter: the part x >= 0, y >= 0)."""
+def box_panels(a=1.0, b=0.5, d=0.5, n=(4, 2, 2), quarter=False, lid=False):
+    """Surface-piercing box |x| <= a, |y| <= b, -d <= z <= 0: the bottom (nx ny panels), the faces
+    x = a (ny nz), y = b (nx nz) and, for the whole box, x = -a and y = -b; quarter: the part
+    x >= 0, y >= 0 (for the symmetry planes x = 0 and y = 0); lid: the interior water plane z = 0
+    (nx ny panels, normals up) after the hull."""
     nx, ny, nz = n
     x0, y0 = (0.0, 0.0) if quarter else (-a, -b)
     lx, ly = a - x0, b - y0
-    panels = []
-    panels += face_panels((x0, y0, -d), (lx, 0, 0), (0, ly, 0), nx, ny, (0, 0, -1))
-    panels += face_panels((a, y0, -d), (0, ly, 0), (0, 0, d), ny, nz, (1, 0, 0))
-    panels += face_panels((x0, b, -d), (lx, 0, 0), (0, 0, d), nx, nz, (0, 1, 0))
+    faces = [((x0, y0, -d), (lx, 0, 0), (0, ly, 0), nx, ny, (0, 0, -1)),
+             ((a, y0, -d), (0, ly, 0), (0, 0, d), ny, nz, (1, 0, 0)),
+             ((x0, b, -d), (lx, 0, 0), (0, 0, d), nx, nz, (0, 1, 0))]
     if not quarter:
-        panels += face_panels((-a, y0, -d), (0, ly, 0), (0, 0, d), ny, nz, (-1, 0, 0))
-        panels += face_panels((x0, -b, -d), (lx, 0, 0), (0, 0, d), nx, nz, (0, -1, 0))
-    return np.array(panels)
+        faces += [((-a, y0, -d), (0, ly, 0), (0, 0, d), ny, nz, (-1, 0, 0)),
+                  ((x0, -b, -d), (lx, 0, 0), (0, 0, d), nx, nz, (0, -1, 0))]
+    if lid:
+        faces.append(((x0, y0, 0.0), (lx, 0, 0), (0, ly, 0), nx, ny, (0, 0, 1)))
+    return np.concatenate([_flat_face(*f) for f in faces])
 
 
-def hemisphere_panels(radius=1.0, n_theta=8, n_phi=8, quarter=True):
-    """Floating hemisphere z <= 0: n_theta polar divisions from the waterline to the pole,
-    n_phi azimuthal divisions over the part; the bottom ring is triangles."""
-    phis = np.linspace(0.0, np.pi / 2 if quarter else 2 * np.pi, n_phi + 1)
+def hemisphere_panels(radius=1.0, n_theta=8, n_phi=8, quarter=True, graded=False):
+    """Floating hemisphere z <= 0 with its vertices on the sphere: n_theta rings in polar angle
+    from the waterline to the bottom pole (graded: refined towards the waterline, s -> 1 - cos(pi s
+    / 2)), n_phi divisions in azimuth over the whole circle or, quarter, over x >= 0, y >= 0; the
+    polar ring holds triangles."""
+    phi = np.linspace(0.0, np.pi / 2 if quarter else 2 * np.pi, n_phi + 1)
     s = np.linspace(0.0, 1.0, n_theta + 1)
-    ths = np.pi / 2 + 0.5 * np.pi * s
-
-    def pt(t, p):
-        return radius * np.array([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)])
-
-    out = []
-    for i in range(n_theta):
-        for j in range(n_phi):
-            a, b = pt(ths[i], phis[j]), pt(ths[i], phis[j + 1])
-            c, d = pt(ths[i + 1], phis[j + 1]), pt(ths[i + 1], phis[j])
-            quad = [a, b, c, d] if i + 1 < n_theta else [a, b, c, c]
-            mid = np.mean(quad, axis=0)
-            out.append(orient(quad, mid))
-    return np.array(out)
+    if graded:
+        s = 1.0 - np.cos(0.5 * np.pi * s)
+    theta = np.pi / 2 + 0.5 * np.pi * s  # polar angle from +z
+    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+    cp, sp = np.cos(phi)[None, :], np.sin(phi)[None, :]
+    V = radius * np.stack([st * cp, st * sp, np.broadcast_to(ct, st.shape[:1] + cp.shape[1:])], axis=2)
+    q = _close_last_ring(_grid_quads(V), n_phi)
+    return _outward(q, q.mean(axis=1))  # outward: away from the centre
 
 
-def _cosine(n):
-    """n intervals on [0, 1], refined at both ends."""
+def _cosine_spacing(n):
+    """n intervals on [0, 1] refined at both ends: (1 - cos(pi k / n)) / 2, k = 0..n."""
     if n == 0:
         return np.zeros(1)
     return 0.5 * (1.0 - np.cos(np.pi * np.arange(n + 1) / n))
 
 
 def cylinder_panels(radius=1.0, draft=0.5, n_phi=8, n_z=2, n_r=2, quarter=True, cosine=False):
-    """Floating vertical circular cylinder (polygonal), side then bottom; cosine spacing of the
-    vertical and radial rings refines them towards the edges."""
-    phis = np.linspace(0.0, np.pi / 2 if quarter else 2 * np.pi, n_phi + 1)
-    zs = -draft * (_cosine(n_z) if cosine else np.linspace(0.0, 1.0, n_z + 1))
-    rs = radius * (1.0 - (_cosine(n_r) if cosine else np.linspace(0.0, 1.0, n_r + 1)))
-    out = []
-    for i in range(n_z):
-        for j in range(n_phi):
-            p0, p1 = phis[j], phis[j + 1]
-            q = [
-                [radius * np.cos(p0), radius * np.sin(p0), zs[i]],
-                [radius * np.cos(p1), radius * np.sin(p1), zs[i]],
-                [radius * np.cos(p1), radius * np.sin(p1), zs[i + 1]],
-                [radius * np.cos(p0), radius * np.sin(p0), zs[i + 1]],
-            ]
-            mid = np.mean(q, axis=0)
-            out.append(orient(q, [mid[0], mid[1], 0.0]))
-    for i in range(n_r):
-        for j in range(n_phi):
-            p0, p1 = phis[j], phis[j + 1]
-            r0, r1 = rs[i], rs[i + 1]
-            q = [
-                [r0 * np.cos(p0), r0 * np.sin(p0), -draft],
-                [r0 * np.cos(p1), r0 * np.sin(p1), -draft],
-                [r1 * np.cos(p1), r1 * np.sin(p1), -draft],
-                [r1 * np.cos(p0), r1 * np.sin(p0), -draft],
-            ]
-            if i + 1 == n_r:
-                q = q[:3] + [q[2]]
-            out.append(orient(q, [0.0, 0.0, -1.0]))
-    return np.array(out)
+    """Floating vertical circular cylinder (a polygon of n_phi sides over the whole circle or,
+    quarter, over x >= 0, y >= 0): the side (n_z rings from the waterline down), then the bottom
+    (n_r rings from the rim to the axis, the innermost of triangles); cosine: the side and bottom
+    rings cosine-spaced, refined towards both ends."""
+    phi = np.linspace(0.0, np.pi / 2 if quarter else 2 * np.pi, n_phi + 1)
+    spacing = _cosine_spacing if cosine else (lambda m: np.linspace(0.0, 1.0, m + 1))
+    z = -draft * spacing(n_z)
+    r = radius * (1.0 - spacing(n_r))
+    cp, sp = np.cos(phi), np.sin(phi)
+    # side: ring i at depth z[i], vertex j at angle phi[j]
+    side = np.stack([np.broadcast_to(radius * cp, (n_z + 1, n_phi + 1)),
+                     np.broadcast_to(radius * sp, (n_z + 1, n_phi + 1)),
+                     np.broadcast_to(z[:, None], (n_z + 1, n_phi + 1))], axis=2)
+    qs = _grid_quads(side)
+    mid = qs.mean(axis=1)
+    qs = _outward(qs, np.stack([mid[:, 0], mid[:, 1], np.zeros(len(mid))], axis=1))  # outward: radial
+    # bottom: ring i at radius r[i]
+    bottom = np.stack([r[:, None] * cp[None, :], r[:, None] * sp[None, :],
+                       np.full((n_r + 1, n_phi + 1), -draft)], axis=2)
+    qb = _outward(_close_last_ring(_grid_quads(bottom), n_phi), np.array([0.0, 0.0, -1.0]))
+    return np.concatenate([qs, qb])
 
 
 def expand_symmetry(quadrant):
@@ -115,7 +135,7 @@ def expand_symmetry(quadrant):
     out = [q]
     for sx, sy in ((-1.0, 1.0), (1.0, -1.0), (-1.0, -1.0)):
         r = q * np.array([sx, sy, 1.0])
-        out.append(r[:, [0, 3, 2, 1]] if sx * sy < 0 else r)
+        out.append(r[:, _REVERSED] if sx * sy < 0 else r)
     return np.concatenate(out)
 
 

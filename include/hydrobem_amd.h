@@ -131,9 +131,15 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
 HbgStatus hbg_set_field_points(HbgSession* s, const double* points, int32_t n);
 
 /* The field values of the last hbg_solve with field points: frad [F][n][6 nbodies], finc, fsca,
- * ftot [F][n][H], complex (re, im) doubles; any output may be NULL. HBG_ERR_STATE before such a
- * solve. */
+ * ftot [F][n][H], complex (re, im) doubles; any output may be NULL. The sizes are those of that
+ * solve (hbg_field_shape); the caller's buffers must hold them. HBG_ERR_STATE before such a solve
+ * and after a new geometry or new field points. */
 HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fsca, double* ftot);
+
+/* The sizes of hbg_field_values's arrays (no counterpart in the reference, whose results carry
+ * their shapes): F, n, 6 nbodies and H of the last solve with field points; HBG_ERR_STATE as
+ * hbg_field_values. */
+HbgStatus hbg_field_shape(HbgSession* s, int32_t* n_omegas, int32_t* n_points, int32_t* n_rad, int32_t* n_headings);
 
 /* Kochin functions (lo_frequency.rs:1279-1372, `Output` kr/krt/kd/kdt): n_angles angles
  * theta_t = 2 pi t / n_angles (0: none). Every later hbg_solve (infinite or finite depth, with
@@ -144,9 +150,14 @@ HbgStatus hbg_field_values(HbgSession* s, double* frad, double* finc, double* fs
 HbgStatus hbg_set_kochin(HbgSession* s, int32_t n_angles);
 
 /* The Kochin functions of the last hbg_solve with angles: kr, krt [F][n_angles][6 nbodies], kd,
- * kdt [F][n_angles][H], complex (re, im) doubles; any output may be NULL. HBG_ERR_STATE before
- * such a solve. */
+ * kdt [F][n_angles][H], complex (re, im) doubles; any output may be NULL. The sizes are those of
+ * that solve (hbg_kochin_shape). HBG_ERR_STATE before such a solve and after a new geometry or a
+ * new angle count. */
 HbgStatus hbg_kochin_values(HbgSession* s, double* kr, double* krt, double* kd, double* kdt);
+
+/* The sizes of hbg_kochin_values's arrays: F, n_angles, 6 nbodies and H of the last solve with
+ * angles (no counterpart in the reference); HBG_ERR_STATE as hbg_kochin_values. */
+HbgStatus hbg_kochin_shape(HbgSession* s, int32_t* n_omegas, int32_t* n_angles, int32_t* n_rad, int32_t* n_headings);
 
 /* The solve of n_omegas frequencies (rad/s) and n_headings headings (degrees), radiation of the
  * six rigid modes of every body plus diffraction (solver::run, lo_frequency.rs:1565-1635).

@@ -1,9 +1,70 @@
-"""The workload meshes: panel counts and orientation of the configurations' bodies."""
+"""The workload meshes: the panel layout of the reference's test meshes, bit for bit (the fixtures
+of tests/golden/meshes_layout.npz, written by tests/golden/write_meshes.py from the reference's
+generators), and the counts and orientation of the configurations' bodies."""
+
+import hashlib
 
 import numpy as np
 import pytest
 
+from conftest import GOLDEN
 from hydrobem_amd import meshes
+
+FIXTURES = GOLDEN / "meshes_layout.npz"
+
+# the same cases as tests/golden/write_meshes.py, generated by this package's meshes
+CASES = {
+    "hemi_16x32": lambda: meshes.hemisphere_panels(1.0, 16, 32, quarter=False),
+    "hemi_6x24": lambda: meshes.hemisphere_panels(1.0, 6, 24, quarter=False),
+    "hemi_4x8": lambda: meshes.hemisphere_panels(1.0, 4, 8, quarter=False),
+    "hemi_q_6x7": lambda: meshes.hemisphere_panels(1.0, 6, 7, quarter=True),
+    "hemi_q_3x3": lambda: meshes.hemisphere_panels(1.0, 3, 3, quarter=True),
+    "hemi_q_graded": lambda: meshes.hemisphere_panels(0.8, 5, 6, quarter=True, graded=True),
+    "cyl_drift": lambda: meshes.cylinder_panels(1.0, 0.5, n_phi=32, n_z=4, n_r=4, quarter=False),
+    "cyl_quarter": lambda: meshes.cylinder_panels(1.0, 0.5, n_phi=8, n_z=4, n_r=4, quarter=True),
+    "cyl_q_odd": lambda: meshes.cylinder_panels(1.0, 0.8, 9, 4, 3, quarter=True),
+    "cyl_cos_16": lambda: meshes.cylinder_panels(1.0, 1.0, 16, 8, 8, quarter=False, cosine=True),
+    "cyl_cos_q": lambda: meshes.cylinder_panels(1.0, 1.0, 16, 12, 8, quarter=True, cosine=True),
+    "box_q": lambda: meshes.box_panels(1.0, 0.5, 0.5, n=(6, 3, 3), quarter=True),
+    "box_whole": lambda: meshes.box_panels(1.0, 0.6, 0.5, (6, 4, 3), quarter=False),
+    "box_lid": lambda: meshes.box_panels(1.0, 0.5, 0.5, (4, 2, 2), quarter=False, lid=True),
+    "cyl_1536": lambda: meshes.cylinder_panels(1.0, 1.0, 48, 16, 16, quarter=False, cosine=True),
+    "cyl_2816": lambda: meshes.cylinder_panels(0.75, 1.0, 64, 22, 22, quarter=False, cosine=True),
+    "cyl_4096": lambda: meshes.cylinder_panels(1.0, 1.0, 64, 32, 32, quarter=False, cosine=True),
+    "cyl_6144": lambda: meshes.cylinder_panels(1.5, 1.0, 96, 32, 32, quarter=False, cosine=True),
+    "cyl_8192": lambda: meshes.cylinder_panels(1.0, 1.0, 128, 32, 32, quarter=False, cosine=True),
+    "box_12288": lambda: meshes.box_panels(40, 10, 5, n=(168, 43, 12)),
+}
+
+
+@pytest.fixture(scope="module")
+def fixtures():
+    return np.load(FIXTURES, allow_pickle=False)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_layout_bit_for_bit(fixtures, name):
+    v = np.asarray(CASES[name](), dtype=np.float64)
+    if name in fixtures.files:
+        ref = fixtures[name]
+        assert v.shape == ref.shape
+        assert v.tobytes() == ref.tobytes(), f"{name}: first differing panel " \
+            f"{int(np.argwhere((v != ref).any(axis=(1, 2)))[0, 0])}"
+    else:
+        assert tuple(v.shape) == tuple(fixtures[name + "__shape"])
+        assert hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest() == str(fixtures[name + "__sha256"])
+
+
+def test_fixtures_cover_every_case(fixtures):
+    stored = {k.split("__")[0] for k in fixtures.files}
+    assert stored == set(CASES)
+
+
+def test_a_misplaced_vertex_fails_the_layout_check(fixtures):
+    # the check is sensitive to orientation: the same panels with one reversed differ
+    v = CASES["hemi_4x8"]()
+    v[5] = v[5][[0, 3, 2, 1]]
+    assert v.tobytes() != fixtures["hemi_4x8"].tobytes()
 
 
 def normals(v):
@@ -25,6 +86,13 @@ def test_counts_and_outward_normals(make, count):
     # every panel's normal points away from the body axis or down (into the fluid)
     out = n[:, 0] * c[:, 0] + n[:, 1] * c[:, 1] - n[:, 2] * 1e-3 * np.sign(c[:, 2])
     assert np.all((out > -1e-12) | (n[:, 2] < -0.5))
+
+
+def test_box_lid_is_the_water_plane_with_upward_normals():
+    v = meshes.box_panels(1.0, 0.5, 0.5, (4, 2, 2), quarter=False, lid=True)
+    lid = v[-8:]
+    assert np.all(lid[:, :, 2] == 0.0)
+    assert np.all(normals(lid)[:, 2] > 0.99)
 
 
 def test_expand_symmetry_keeps_orientation():
