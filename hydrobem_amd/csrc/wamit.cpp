@@ -24,6 +24,8 @@
 
 namespace hbg {
 
+void set_error(int status, const std::string& msg, const char* where);
+
 namespace {
 
 const double PI_W = 3.14159265358979323846;
@@ -225,3 +227,45 @@ int write_wamit(const char* stem, int overwrite, double rho, double g, double L,
 }
 
 }  // namespace hbg
+
+// ---- the number fields of the files at the C ABI (include/hydrobem_amd.h hbg_fortran_e /
+// hbg_fortran_i), for the reference's formatter table (crates/hydrobem/tests/io.rs:37-65)
+#include "../../include/hydrobem_amd.h"
+
+namespace {
+
+HbgStatus field_out(bool ok, const std::string& s, char* out, int32_t capacity, const char* what) {
+  if (!ok) {
+    hbg::set_error(HBG_ERR_VALUE, std::string(what) + ": the number does not fit the field", what);
+    return HBG_ERR_VALUE;
+  }
+  if ((int64_t)s.size() + 1 > capacity) {
+    hbg::set_error(HBG_ERR_ARGUMENT, std::string(what) + ": output capacity too small", what);
+    return HBG_ERR_ARGUMENT;
+  }
+  std::memcpy(out, s.c_str(), s.size() + 1);
+  return HBG_OK;
+}
+
+}  // namespace
+
+extern "C" HbgStatus hbg_fortran_e(double x, int32_t one_p, int32_t width, int32_t digits, char* out,
+                                   int32_t capacity) {
+  if (!out || width < 1 || width > 64 || digits < 1 || digits > 30) {
+    hbg::set_error(HBG_ERR_ARGUMENT, "null output or width / digits out of range", "hbg_fortran_e");
+    return HBG_ERR_ARGUMENT;
+  }
+  std::string s;
+  const bool ok = hbg::fortran_e(x, one_p != 0, width, digits, s);
+  return field_out(ok, s, out, capacity, "hbg_fortran_e");
+}
+
+extern "C" HbgStatus hbg_fortran_i(int64_t n, int32_t width, char* out, int32_t capacity) {
+  if (!out || width < 1 || width > 64) {
+    hbg::set_error(HBG_ERR_ARGUMENT, "null output or width out of range", "hbg_fortran_i");
+    return HBG_ERR_ARGUMENT;
+  }
+  std::string s;
+  const bool ok = hbg::fortran_i((long)n, width, s);
+  return field_out(ok, s, out, capacity, "hbg_fortran_i");
+}

@@ -231,6 +231,13 @@ HbgStatus hbg_statics(HbgSession* s, int32_t logarithmic, double log_tolerance, 
  * complex [N][N] row-major (S[k][n] = influence of panel n at field point k). */
 HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, double* S, double* A);
 
+/* The number fields of the WAMIT files (wamit_files.rs:1718-1757 fortran_e, fortran_i): x as
+ * Fortran E<width>.<digits> (one_p 0: 0.ddddddE+xx, 1: 1P form d.ddddddE+xx), n as I<width>,
+ * right-aligned, into out (NUL-terminated, capacity bytes). HBG_ERR_VALUE when the value does not
+ * fit the field (a non-finite x, a three-digit exponent, too many digits). Host only. */
+HbgStatus hbg_fortran_e(double x, int32_t one_p, int32_t width, int32_t digits, char* out, int32_t capacity);
+HbgStatus hbg_fortran_i(int64_t n, int32_t width, char* out, int32_t capacity);
+
 /* A x = b on the GPU in FP64 (linalg.rs:321-328 lu_solve_c128): a [n][n] and b [nrhs][n]
  * complex, column-major (a[j][i] = A(i, j)), overwritten by the factors and the solution. */
 HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b);

@@ -155,3 +155,47 @@ def test_wamit_force_parts_and_motions_on_host(tmp_path):
     assert rows(".3sc")[q] == pytest.approx(sc / scale(4), rel=1e-5)
     assert rows(".4")[q] == pytest.approx(rao[0, 1, 4] * L, rel=1e-5)
     assert rows(".4")[2] == pytest.approx(rao[0, 0, 2], rel=1e-5)
+
+
+# the reference's formatter table (crates/hydrobem/tests/io.rs:37-65): value, 1P, field text
+FORTRAN_E_TABLE = [
+    (1.0, False, "  0.100000E+01"),
+    (-123.4567, False, " -0.123457E+03"),
+    (0.0, False, "  0.000000E+00"),
+    (-0.0, False, "  0.000000E+00"),
+    (6.2831853, False, "  0.628319E+01"),
+    (0.99999951, False, "  0.100000E+01"),
+    (1e-5, False, "  0.100000E-04"),
+    (9.0e98, False, "  0.900000E+99"),
+    (1.0, True, "  1.000000E+00"),
+    (-1.0, True, " -1.000000E+00"),
+    (9.9999996, True, "  1.000000E+01"),
+    (0.0, True, "  0.000000E+00"),
+    (123456.78, True, "  1.234568E+05"),
+    (-2.5e-99, True, " -2.500000E-99"),
+    (float(__import__("numpy").float32(0.1)), True, "  1.000000E-01"),
+]
+
+
+def fortran_e(x, one_p, width=14, digits=6):
+    buf = ctypes.create_string_buffer(80)
+    st = hb.library().hbg_fortran_e(float(x), int(one_p), width, digits, buf, 80)
+    return st, buf.value.decode()
+
+
+@pytest.mark.parametrize("x,one_p,text", FORTRAN_E_TABLE)
+def test_fortran_e_matches_the_reference_table(x, one_p, text):
+    st, got = fortran_e(x, one_p)
+    assert st == 0 and got == text, (x, one_p, got)
+
+
+def test_fortran_fields_width_integers_and_refusals():
+    assert fortran_e(1.0, True, 15) == (0, "   1.000000E+00")
+    buf = ctypes.create_string_buffer(16)
+    assert hb.library().hbg_fortran_i(12, 6, buf, 16) == 0 and buf.value == b"    12"
+    # a three-digit exponent does not fit, nor a non-finite value or a long integer
+    for x in (1e100, -3e101, 1e-100, float("nan"), float("inf")):
+        assert fortran_e(x, True)[0] == 1, x
+    assert fortran_e(1e99, False)[0] == 1
+    assert hb.library().hbg_fortran_i(1234567, 6, buf, 16) == 1
+    assert hb.library().hbg_fortran_e(1.0, 1, 14, 6, buf, 4) == 7  # capacity too small
