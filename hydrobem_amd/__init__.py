@@ -83,6 +83,7 @@ SIGNATURES = {
     "hbg_statics": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp]),
     "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
+    "hbg_lu_solve_noalloc": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
     "hbg_fortran_e": (ctypes.c_int32, [ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_char_p, ctypes.c_int32]),
     "hbg_fortran_i": (ctypes.c_int32, [ctypes.c_int64, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32]),

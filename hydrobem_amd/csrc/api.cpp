@@ -73,6 +73,11 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
 void lu_solve_batched(double2* M, size_t mstride, int batch, int n, int lda, int nrhs, hipStream_t stream,
                       LuStats* stats, int* status, const int* nsys = nullptr);
 int lu_solve_pivoted(double2* M, int n, int lda, int nrhs, hipStream_t stream, LuStats* stats);
+size_t lu_region_workspace_bytes(int n);
+int lu_solve_region(double2* M, int n, int lda, int nrhs, void* region, size_t region_bytes, int pivoted,
+                    hipStream_t stream);
+size_t noalloc_scratch_bytes();
+void* noalloc_scratch_setup(int device);
 void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
                          double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
                          hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta,
@@ -398,6 +403,7 @@ struct HbgSession {
   bool profiling = false;
   double prof[HBG_PROFILE_COUNTERS] = {};
   hipEvent_t ev[8] = {};
+  void* scratch = nullptr;  // this device's address of the allocation-free entry point's array
 };
 
 namespace {
@@ -643,6 +649,7 @@ HbgStatus hbg_session_create(int32_t device, HbgSession** out) {
   HBG_HIP(hipStreamCreate(&s->lu_stream));
   for (auto& e : s->assembled) HBG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (auto& e : s->ev) HBG_HIP(hipEventCreate(&e));
+  s->scratch = noalloc_scratch_setup(device);
   TablesOnDevice& t = tables_for(device);
   s->T.t2d = t.t2d;
   s->T.t1d = t.t1d;
@@ -1186,6 +1193,53 @@ HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double
     if (!std::isfinite(b[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
   return HBG_OK;
   GUARD_END("hbg_lu_solve")
+}
+
+// The bytes hbg_lu_solve_noalloc needs in the scratch array: [A | B] (aligned), then the workspace.
+static size_t noalloc_bytes(int n, int nrhs, size_t* matrix_bytes) {
+  const size_t m = ((size_t)n * (n + nrhs) * sizeof(double2) + 255) / 256 * 256;
+  if (matrix_bytes) *matrix_bytes = m;
+  return m + lu_region_workspace_bytes(n);
+}
+
+HbgStatus hbg_lu_solve_noalloc(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b) {
+  if (!s || !a || (nrhs > 0 && !b)) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_lu_solve_noalloc");
+  if (n <= 0 || nrhs < 0) return fail(HBG_ERR_ARGUMENT, "n must be > 0 and nrhs >= 0", "hbg_lu_solve_noalloc");
+  if (n > 1 << 16 || nrhs > 1 << 16 || noalloc_bytes(n, nrhs, nullptr) > noalloc_scratch_bytes())
+    return fail(HBG_ERR_VALUE,
+                "order " + std::to_string(n) + " with " + std::to_string(nrhs) +
+                    " right-hand sides exceeds the allocation-free maximum (32 MiB of system and workspace)",
+                "hbg_lu_solve_noalloc");
+  for (size_t i = 0; i < 2 * (size_t)n * n; ++i)
+    if (!std::isfinite(a[i])) return fail(HBG_ERR_NUMERICAL, "non-finite system matrix", "hbg_lu_solve_noalloc");
+  GUARD_BEGIN
+  use_device(s);
+  size_t mb = 0;
+  noalloc_bytes(n, nrhs, &mb);
+  char* base = (char*)s->scratch;  // the device address taken at session creation
+  double2* M = (double2*)base;
+  auto upload = [&]() {
+    HBG_HIP(hipMemcpyAsync(M, a, (size_t)n * n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    if (nrhs > 0)
+      HBG_HIP(hipMemcpyAsync(M + (size_t)n * n, b, (size_t)n * nrhs * sizeof(double2), hipMemcpyHostToDevice,
+                             s->stream));
+  };
+  upload();
+  int zp = lu_solve_region(M, n, n, nrhs, base + mb, noalloc_scratch_bytes() - mb, 0, s->stream);
+  if (zp < 0) {  // a panel needs interchanges: the system again from the caller's arrays, pivoted
+    upload();
+    zp = lu_solve_region(M, n, n, nrhs, base + mb, noalloc_scratch_bytes() - mb, 1, s->stream);
+  }
+  HBG_HIP(hipMemcpyAsync(a, M, (size_t)n * n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+  if (nrhs > 0)
+    HBG_HIP(hipMemcpyAsync(b, M + (size_t)n * n, (size_t)n * nrhs * sizeof(double2), hipMemcpyDeviceToHost,
+                           s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(zp) + ")"};
+  for (size_t i = 0; i < 2 * (size_t)n * nrhs; ++i)
+    if (!std::isfinite(b[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
+  return HBG_OK;
+  GUARD_END("hbg_lu_solve_noalloc")
 }
 
 namespace {

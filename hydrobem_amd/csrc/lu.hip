@@ -1078,9 +1078,12 @@ void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int
 // a panel, the earlier panels of its block update its columns (rows from the panel down) and its
 // rows (columns right of it); after the block one trailing update with K = the block's width, so
 // the trailing matrix is read and written once per block rather than once per panel.
+// one_stream: the panel rows on the calling stream too (no second stream, no events: the
+// allocation-free solve, which creates nothing)
 template <typename T>
 void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int batch, BatchStrides bs,
-                       const int* nsys, hipStream_t stream, LuWorkspace<T>& ws, GemmTimer& gt) {
+                       const int* nsys, hipStream_t stream, LuWorkspace<T>& ws, GemmTimer& gt,
+                       bool one_stream = false) {
   const int Q = std::max(1, g_lu_depth_[prec_index<T>()]);
   const GemmBatch gb{bs.m, batch, nsys, n};
   // the split-FP16 GEMM reads its operands from the split buffers, filled panel by panel
@@ -1095,7 +1098,10 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int kb = min(NB, n - k0);
       const int m = n - k0;
       AuxStream* ax = nullptr;
-      if (k0 > K0) {
+      if (k0 > K0 && one_stream) {
+        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, gb, sop, stream, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, gb, sop, stream, gt);
+      } else if (k0 > K0) {
         gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, gb, sop, stream, gt);
         // the panel's rows (the columns right of it) are needed only by apply_kernel: they are
         // updated on the second stream beside the diagonal block (disjoint parts of M)
@@ -1469,6 +1475,110 @@ int lu_solve_device(double2* M, int n, int lda, int nrhs, hipStream_t stream, Lu
   if (st >= 0) return st;
   HBG_HIP(hipMemcpyAsync(M, ws.backup, whole * sizeof(double2), hipMemcpyDeviceToDevice, stream));
   return lu_solve_pivoted(M, n, lda, nrhs, stream, stats);
+}
+
+// ---- one FP64 system in a region the caller owns (the allocation-free solve): the workspace of
+// a batch of one carved out of the region, the launches eager on one stream.
+namespace {
+constexpr size_t REGION_ALIGN = 256;
+size_t region_take(size_t& off, size_t bytes) {
+  const size_t o = (off + REGION_ALIGN - 1) / REGION_ALIGN * REGION_ALIGN;
+  off = o + bytes;
+  return o;
+}
+// the workspace's pointers into base (when given) for order n; returns the bytes it spans
+size_t carve_workspace(LuWorkspace<double>* ws, char* base, int n) {
+  const int panels = (n + NB - 1) / NB;
+  int leaves = (n + LEAF - 1) / LEAF, levels = 1, nodes = leaves;
+  while (nodes > 1) {
+    nodes = (nodes + FAN - 1) / FAN;
+    ++levels;
+  }
+  const size_t slots = (size_t)levels * leaves;
+  size_t off = 0;
+  const size_t o_cand = region_take(off, slots * NB * sizeof(int));
+  const size_t o_ccount = region_take(off, slots * sizeof(int));
+  const size_t o_ctr = region_take(off, slots * sizeof(unsigned));
+  const size_t o_top = region_take(off, (size_t)NB * NB * sizeof(double2));
+  const size_t o_win = region_take(off, NB * sizeof(int));
+  const size_t o_zp = region_take(off, sizeof(int));
+  const size_t o_save = region_take(off, (size_t)n * NB * sizeof(double2));
+  const size_t o_ipiv = region_take(off, (size_t)n * sizeof(int));
+  const size_t o_nsys = region_take(off, sizeof(int));
+  const size_t o_flags = region_take(off, (size_t)panels * sizeof(int));
+  const size_t o_zpf = region_take(off, (size_t)panels * sizeof(int));
+  const size_t o_uinv = region_take(off, (size_t)panels * NB * NB * sizeof(double2));
+  const size_t o_linv = region_take(off, (size_t)panels * NB * NB * sizeof(double2));
+  const size_t o_udiag = region_take(off, (size_t)NB * sizeof(double2));
+  const size_t o_subw = region_take(off, (size_t)MAXRHS_LU * n * sizeof(double2));
+  if (ws && base) {
+    ws->n = n;
+    ws->batch = 1;
+    ws->w.levels = levels;
+    ws->w.max_nodes = leaves;
+    ws->w.cand = (int*)(base + o_cand);
+    ws->w.ccount = (int*)(base + o_ccount);
+    ws->w.ctr = (unsigned*)(base + o_ctr);
+    ws->w.top = (double2*)(base + o_top);
+    ws->w.winners = (int*)(base + o_win);
+    ws->w.zero_pivot = (int*)(base + o_zp);
+    ws->save = (double2*)(base + o_save);
+    ws->ipiv = (int*)(base + o_ipiv);
+    ws->nsys = (int*)(base + o_nsys);
+    ws->flags = (int*)(base + o_flags);
+    ws->zp_fast = (int*)(base + o_zpf);
+    ws->uinv = (double2*)(base + o_uinv);
+    ws->linv = (double2*)(base + o_linv);
+    ws->udiag = (double2*)(base + o_udiag);
+    ws->subw = (double2*)(base + o_subw);
+  }
+  return off;
+}
+}  // namespace
+
+size_t lu_region_workspace_bytes(int n) { return carve_workspace(nullptr, nullptr, n); }
+
+// M (n x (n + nrhs), leading dimension lda) solved with the workspace in `region` (at least
+// lu_region_workspace_bytes(n) bytes, device memory): pivoted 0 the attempt without interchanges
+// (returns 0, the first zero pivot k > 0, or -1: a panel needs interchanges, M overwritten),
+// pivoted 1 the partially pivoted factorization (returns 0 or the first zero pivot). No
+// allocation, no graph, no second stream; every flag and counter of the workspace is filled by a
+// kernel before it is read, so the region's earlier contents do not matter.
+int lu_solve_region(double2* M, int n, int lda, int nrhs, void* region, size_t region_bytes, int pivoted,
+                    hipStream_t stream) {
+  if (lu_region_workspace_bytes(n) > region_bytes) throw Error{6, "LU workspace beyond its region"};
+  LuWorkspace<double> ws;  // views into the region: never ensured or released
+  carve_workspace(&ws, (char*)region, n);
+  const BatchStrides bs = ws.strides();
+  EventPool ep;
+  GemmTimer gt{&ep, nullptr, {}};
+  const int ncols = n + nrhs;
+  const int panels = (n + NB - 1) / NB;
+  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.flags, panels, 0);
+  fill_i32<<<(panels + 255) / 256, 256, 0, stream>>>(ws.zp_fast, panels, 0x7fffffff);
+  if (pivoted) {
+    fill_i32<<<1, 64, 0, stream>>>(ws.w.zero_pivot, 1, 0x7fffffff);
+    factor_pivoted(M, n, lda, ncols, stream, ws, gt);
+  } else {
+    BatchStrides b1 = bs;
+    b1.m = (size_t)lda * ncols;
+    factor_optimistic<double>(M, n, lda, ncols, 1, b1, nullptr, stream, ws, gt, true);
+  }
+  if (nrhs > 0) substitute<double>(M, n, lda, 0, M + (size_t)n * lda, lda, 0, nrhs, 1, true, stream, ws);
+  HBG_HIP(hipGetLastError());
+  std::vector<int> flags(panels), zps(panels);
+  int zp = 0x7fffffff;
+  HBG_HIP(hipMemcpyAsync(flags.data(), ws.flags, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HBG_HIP(hipMemcpyAsync(zps.data(), ws.zp_fast, panels * sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (pivoted) HBG_HIP(hipMemcpyAsync(&zp, ws.w.zero_pivot, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HBG_HIP(hipStreamSynchronize(stream));
+  if (pivoted) return zp == 0x7fffffff ? 0 : zp;
+  bool any = false;
+  for (int p = 0; p < panels; ++p) {
+    any = any || flags[p] != 0;
+    zp = std::min(zp, zps[p]);
+  }
+  return any ? -1 : (zp == 0x7fffffff ? 0 : zp);
 }
 
 void lu_release() {

@@ -242,6 +242,16 @@ HbgStatus hbg_fortran_i(int64_t n, int32_t width, char* out, int32_t capacity);
  * complex, column-major (a[j][i] = A(i, j)), overwritten by the factors and the solution. */
 HbgStatus hbg_lu_solve(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b);
 
+/* hbg_lu_solve without any device allocation, for callers that cannot allocate (serving loops,
+ * harnesses that forbid hipMalloc; no counterpart in the reference). The staged [A | b] and the
+ * LU's workspace live in a fixed per-GPU array of the library (32 MiB, held by each GPU the
+ * library runs on until the process ends): n (n + nrhs) 16 bytes plus the workspace of order n
+ * must fit (n <= 1300 with 32 right-hand sides), else HBG_ERR_VALUE. Calls no hipMalloc, hipFree
+ * or allocating library routine; the launches are eager on the session's stream. Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU race. Results and errors as
+ * hbg_lu_solve. */
+HbgStatus hbg_lu_solve_noalloc(HbgSession* s, int32_t n, double* a, int32_t nrhs, double* b);
+
 /* Frequencies in flight in hbg_solve (no counterpart in the reference, whose frequency pools run
  * one CPU solve per worker, parallel.rs:63-89): each holds its N x (N + R) system on the device
  * and their LUs advance panel by panel in the same launches. 0 (the default): as many as half the
