@@ -36,7 +36,8 @@ PEAK_FP16 = 2.5e15    # MI355X dense FP16 matrix, FLOP/s
 # LU GEMM variants: tiles, kernel, arithmetic of the products
 VARIANTS = {1: ("64x64", "gemm_mfma_kernel<T, 64, 64>", None), 2: ("128x64", "gemm_mfma_kernel<T, 128, 64>", None),
             3: ("64x64 split-fp16", "gemm_f16s_kernel<64, 64>", "f16x3"),
-            4: ("128x64 split-fp16", "gemm_f16s_kernel<128, 64>", "f16x3")}
+            4: ("128x64 split-fp16", "gemm_f16s_kernel<128, 64>", "f16x3"),
+            5: ("128x128 split-fp16", "gemm_f16s_kernel<128, 128, 2, 4>", "f16x3")}
 # HBM bytes per launch of the path's GEMM kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
 # calibrated by a device copy of known size (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md):
 # (workload, path, GEMM tiles, panels per trailing update, batch) -> bytes
@@ -49,9 +50,14 @@ ASM_TRAFFIC = {"cylinder8192fd": 8.97e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
                       "operands split into hi + lo, autotuned), FP64 iterative refinement to backward error 1e-12",
+             "mixed32": "mixed32: FP32 far-field assembly (FP32 tables and statics, self and adjacent pairs in FP64), "
+                        "complex64 LU on the matrix cores, FP64 iterative refinement against that matrix to "
+                        "backward error 1e-10",
              "mixed_nn": "mixed+NN: finite-depth evanescent remainder from the BF16-MFMA surrogate (FP64 evaluator for "
                          "R/h < 0.1), complex64 LU as the mixed path, FP64 iterative refinement to backward error 1e-12"}
-DTYPE = {"fp64": "f64", "mixed": "f32+f64", "mixed_nn": "bf16+f32+f64"}
+DTYPE = {"fp64": "f64", "mixed": "f32+f64", "mixed32": "f32+f64", "mixed_nn": "bf16+f32+f64"}
+# the library precision of each path
+PRECISION = {"fp64": "fp64", "mixed": "mixed", "mixed32": "mixed32", "mixed_nn": "mixed"}
 PEAK_HBM = 8.0e12    # MI355X HBM3E, B/s
 
 
@@ -220,7 +226,7 @@ def run_path(s, w, path, steps, warmup, label):
     """Warm-up steps, `steps` timed steps (stage profiling off, so the LU runs from its captured
     hipGraphs), then one profiled step for the stage times and the GEMM's live roofline."""
     v, omegas, heads = w["vertices"], w["omegas"], w["headings"]
-    precision = "fp64" if path == "fp64" else "mixed"
+    precision = PRECISION[path]
     surrogate = path == "mixed_nn"
 
     def step():
@@ -329,7 +335,7 @@ def main():
         s.set_tuning(s.TUNE_LU_DEPTH, a.depth)
     print(f"[bench] workload {w['name']}: {N} panels, {F} frequencies", file=sys.stderr, flush=True)
     paths = [p for p in a.paths.split(",") if p and not (a.symmetry and p == "mixed_nn")]
-    peaks = {"fp64": PEAK_FP64, "mixed": PEAK_FP32, "mixed_nn": PEAK_FP32}
+    peaks = {"fp64": PEAK_FP64, "mixed": PEAK_FP32, "mixed32": PEAK_FP32, "mixed_nn": PEAK_FP32}
     res = {}
     if "mixed_nn" in paths:
         from hydrobem_amd import train_surrogate
@@ -346,6 +352,9 @@ def main():
             prec = "fp64" if p == "fp64" else "mixed"
             if prec not in tuning:
                 rows = s.autotune(N, 6 + len(heads), prec)
+                print(f"[bench] autotune {prec} candidates (variant, depth, batch, ms per system): " +
+                      "; ".join(f"{int(r[0])},{int(r[1])},{int(r[2])}: {r[3]:.3f}" for r in rows),
+                      file=sys.stderr, flush=True)
                 best = min(rows, key=lambda r: r[3])
                 tuning[prec] = {"gemm_variant": int(best[0]), "gemm_tiles": VARIANTS[int(best[0])][0],
                                 "panels_per_update": int(best[1]),

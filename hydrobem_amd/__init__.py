@@ -80,6 +80,9 @@ SIGNATURES = {
     "hbg_finite_kernel": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _dp,
                                            _dp, _dp, _dp]),
     "hbg_dispersion_roots": (ctypes.c_int32, [ctypes.c_double, ctypes.c_int32, _dp]),
+    "hbg_wave_kernel32": (ctypes.c_int32, [ctypes.c_void_p, _dp, _dp, ctypes.c_int32, _dp]),
+    "hbg_finite_kernel32": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _dp,
+                                             _dp, _dp, _dp]),
     "hbg_statics": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, _dp]),
     "hbg_assemble": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(Options), _dp, _dp]),
     "hbg_lu_solve": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32, _dp]),
@@ -138,7 +141,7 @@ def _p(a):
     return a.ctypes.data_as(_dp) if a is not None else None
 
 
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed32": 2}
 
 
 def _options(logarithmic=True, log_tolerance=1e-4, precision=0, surrogate=False) -> Options:
@@ -241,7 +244,8 @@ class Session:
     def solve(self, omegas, headings_deg=(), logarithmic=True, log_tolerance=1e-4, progress=None, precision="fp64",
               surrogate=False):
         """The quantities of one solve (the reference's result names and axes). precision "fp64"
-        (the plain path) or "mixed" (complex64 LU on the matrix cores + FP64 refinement);
+        (the plain path), "mixed" (complex64 LU on the matrix cores + FP64 refinement) or "mixed32"
+        (the same with the FP32 far-field assembly);
         surrogate: the finite-depth evanescent remainder from the trained network."""
         w = np.ascontiguousarray(omegas, dtype=np.float64).reshape(-1)
         h = np.ascontiguousarray(headings_deg, dtype=np.float64).reshape(-1)
@@ -335,6 +339,7 @@ class Session:
     TUNE_LU_GRAPHS = 2
     TUNE_LU_GEMM_MIXED = 3
     TUNE_LU_DEPTH_MIXED = 4
+    TUNE_MIXED32_TOL = 5
 
     def autotune(self, n, nrhs, precision="fp64") -> list:
         """Time the batched LU variants on synthetic systems and keep the fastest (hbg_autotune);
@@ -377,16 +382,34 @@ class Session:
         _check(library().hbg_finite_kernel(self._h, float(h), float(K), len(R), _p(R), _p(z), _p(zeta), _p(out)))
         return out
 
+    def wave_kernel32(self, X, Y):
+        """(Ft, Ft_X, Gi, Gi_X) of the FP32 evaluator (the precision path's) at 0 <= X, Y < 48."""
+        x = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        y = np.ascontiguousarray(Y, dtype=np.float64).reshape(-1)
+        out = np.zeros((len(x), 4))
+        _check(library().hbg_wave_kernel32(self._h, _p(x), _p(y), len(x), _p(out)))
+        return out
+
+    def finite_kernel32(self, h, K, R, z, zeta):
+        """finite_kernel's eight values from the FP32 evaluator."""
+        R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
+        z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+        zeta = np.ascontiguousarray(zeta, dtype=np.float64).reshape(-1)
+        out = np.zeros((len(R), 8))
+        _check(library().hbg_finite_kernel32(self._h, float(h), float(K), len(R), _p(R), _p(z), _p(zeta), _p(out)))
+        return out
+
     def statics(self, logarithmic=True, log_tolerance=1e-4):
         out = np.zeros((self.n_panels, self.n_panels, 4))
         _check(library().hbg_statics(self._h, int(bool(logarithmic)), float(log_tolerance), _p(out)))
         return out
 
-    def assemble(self, omega, logarithmic=True, log_tolerance=1e-4):
+    def assemble(self, omega, logarithmic=True, log_tolerance=1e-4, precision="fp64"):
+        """S and A of one frequency: the FP64 assembly, or (precision "mixed32") the FP32 one."""
         N = self.n_panels
         S = np.zeros((N, N), np.complex128)
         A = np.zeros((N, N), np.complex128)
-        opt = _options(logarithmic, log_tolerance)
+        opt = _options(logarithmic, log_tolerance, PRECISIONS[precision])
         _check(library().hbg_assemble(self._h, float(omega), ctypes.byref(opt), _p(S), _p(A)))
         return S, A
 

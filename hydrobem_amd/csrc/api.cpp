@@ -78,6 +78,46 @@ int lu_solve_region(double2* M, int n, int lda, int nrhs, void* region, size_t r
                     hipStream_t stream);
 size_t noalloc_scratch_bytes();
 void* noalloc_scratch_setup(int device);
+// the FP32 far-field assembly (assembly32.hip)
+void build_tables32(const WaveTables& T64, const double* ktab64, float* t2d32, float* t1d32, float* ktab32,
+                    hipStream_t stream);
+size_t tables32_t2d_elems();
+size_t tables32_t1d_elems();
+size_t tables32_ktab_elems();
+size_t fin32_phi1_elems();
+size_t fin32_phi2_elems();
+size_t panel32_bytes();
+size_t finfreq32_bytes();
+double near_ratio();
+std::vector<double> fin32_truncation();
+void launch_fin_tables32(const double* phi1, const double* phi2, const double* dP, float* phi1_32, float* phi2_32,
+                         const double4* pf, const double2* pcs, int N, float4* pf32, float2* pcs32,
+                         hipStream_t stream);
+void launch_panels32(const PanelD* P, int N, void* out, hipStream_t stream);
+void launch_statics32(const StaticD* st, size_t count, float4* out, hipStream_t stream);
+void launch_assemble32(const void* P32, int N, double K, double logc, const float* t2d32, const float* t1d32,
+                       const float4* st32, float2* S, float2* A, float2* A2, int lda, double free_term,
+                       hipStream_t stream);
+void launch_assemble_fin32(const void* P32, int N, const void* f32, const float* phi1_32, const float* phi2_32,
+                           const float* ktab32, const float4* pf32, const float2* pcs32, const float* t2d32,
+                           const float* t1d32, double logc, const float4* st32, float2* S, float2* A, float2* A2,
+                           int lda, double free_term, hipStream_t stream);
+void launch_near64(const PanelD* P, int N, const int2* ent, int nnz, bool finite, double K, const FinFreq* f,
+                   FinTables F, WaveTables T, double logc, const StaticD* st, float2* S, float2* A, float2* A2,
+                   int lda, double free_term, double2* dS, double2* dA, hipStream_t stream);
+void launch_rhs32(const float2* S, int N, const double2* u, int R, double2* B, double2* part, const int* rowptr,
+                  const int2* ent, const double2* dS, hipStream_t stream);
+void launch_wave_points32(const float* t2d32, const float* t1d32, const double* X, const double* Y, int n,
+                          double* out, hipStream_t stream);
+void launch_fin_points32(const void* f32, const float* phi1_32, const float* phi2_32, const float* ktab32,
+                         const float4* pf32, const float2* pcs32, const float* t2d32, const float* t1d32,
+                         const double* R, const double* z, const double* zeta, int n, double* out,
+                         hipStream_t stream);
+void* make_finfreq32(const FinFreq& f, void* host_out);
+void mixed32_solve_batched(float2* M32, const float2* A32r, double2* B, const int* rowptr, const int2* ent,
+                           const double2* dA, size_t dstride, double2* X, float2* D, double* part, double2* part2,
+                           double* rscale, int batch, int n, int R, double tol, int max_iter, hipStream_t stream,
+                           LuStats* stats, int* status, int* iters, double* eta, int* active);
 void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, double* part, double2* part2,
                          double* rscale, int batch, int n, int R, bool a32_written, double tol, int max_iter,
                          hipStream_t stream, LuStats* stats, int* status, int* iters, double* eta,
@@ -287,6 +327,11 @@ struct TablesOnDevice {
   double* t1d = nullptr;
   double* ktab = nullptr;  // K0, K1 tiles of the finite-depth series
   double* fitm = nullptr;  // node values -> monomial coefficients of the Phi tiles
+  // the FP32 tables of the precision path, truncated from the FP64 ones (built on first use)
+  float* t2d32 = nullptr;
+  float* t1d32 = nullptr;
+  float* ktab32 = nullptr;
+  double* fin32P = nullptr;  // the Phi tiles' truncation matrix
 };
 TablesOnDevice& tables_for(int device) {
   static TablesOnDevice tabs[16];
@@ -304,6 +349,29 @@ TablesOnDevice& tables_for(int device) {
   HBG_HIP(hipMemcpy(t.ktab, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
   HBG_HIP(hipMemcpy(t.fitm, fm.data(), fm.size() * sizeof(double), hipMemcpyHostToDevice));
   t.device = device;
+  return t;
+}
+
+// The FP32 tables of the device (the tables of tables_for(device) truncated), once per device.
+TablesOnDevice& tables32_for(int device, hipStream_t stream) {
+  TablesOnDevice& t = tables_for(device);
+  if (t.t2d32) return t;
+  float *a = nullptr, *b = nullptr, *c = nullptr;
+  double* d = nullptr;
+  HBG_HIP(hipMalloc(&a, tables32_t2d_elems() * sizeof(float)));
+  HBG_HIP(hipMalloc(&b, tables32_t1d_elems() * sizeof(float)));
+  HBG_HIP(hipMalloc(&c, tables32_ktab_elems() * sizeof(float)));
+  const std::vector<double> P = fin32_truncation();
+  HBG_HIP(hipMalloc(&d, P.size() * sizeof(double)));
+  HBG_HIP(hipMemcpy(d, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice));
+  WaveTables T;
+  T.t2d = t.t2d;
+  T.t1d = t.t1d;
+  build_tables32(T, t.ktab, a, b, c, stream);
+  t.t2d32 = a;
+  t.t1d32 = b;
+  t.ktab32 = c;
+  t.fin32P = d;
   return t;
 }
 
@@ -404,6 +472,24 @@ struct HbgSession {
   double prof[HBG_PROFILE_COUNTERS] = {};
   hipEvent_t ev[8] = {};
   void* scratch = nullptr;  // this device's address of the allocation-free entry point's array
+  // the FP32 far-field assembly (HBG_PRECISION_MIXED32): per geometry the FP32 panel records,
+  // statics and the list of near entries (rows in order: rowptr [N + 1], (row, column) entries);
+  // per frequency the FP32 tables and factors; the complex64 S, the FP64 right-hand sides and
+  // solutions [system][N R], the near entries' FP64 differences (dS shared, dA per system)
+  DevBuf<unsigned char> d_panels32;
+  DevBuf<float4> d_st32;
+  bool st32_valid = false;
+  bool near_valid = false;
+  int near_nnz = 0;
+  std::vector<int2> near_ent;  // host copy of the near entries (row, column)
+  DevBuf<int> d_near_rowptr;
+  DevBuf<int2> d_near_ent;
+  DevBuf<float> d_phi1_32, d_phi2_32;
+  DevBuf<float4> d_pf32;
+  DevBuf<float2> d_pcs32;
+  DevBuf<unsigned char> d_ffs32;
+  DevBuf<float2> d_S32, d_A32r;  // S; the unfactored copy of A per system (the residuals)
+  DevBuf<double2> d_B64, d_dS, d_dA;
 };
 
 namespace {
@@ -456,6 +542,7 @@ void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false
         s->prof[11] += 1.0;
       }
       s->statics_valid = true;
+      s->st32_valid = false;
       s->stinf_valid = want_inf;
       s->st_logint = logint;
       s->st_tol = tol;
@@ -583,6 +670,126 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
 
 constexpr double MIXED_TOL = 1e-12;  // normwise backward error that ends the refinement
 constexpr int MIXED_MAX_ITER = 10;
+// the refinement of the FP32 assembly ends at 10^-g_mixed32_tol_exp (HBG_TUNE_MIXED32_TOL)
+int g_mixed32_tol_exp = 10;
+
+// ---- the FP32 far-field assembly (HBG_PRECISION_MIXED32)
+
+// The FP32 panel records and statics of the geometry and its near entries (G = 1; the statics
+// current): the pairs whose centroids are closer than near_ratio() (r_k + r_n), found on a uniform
+// grid of the centroids, every row k's columns in increasing order, the diagonal included.
+void ensure_fp32_geometry(HbgSession* s) {
+  const int N = s->N;
+  if (!s->st32_valid) {
+    s->d_panels32.ensure((size_t)N * panel32_bytes());
+    launch_panels32(s->d_panels.p, N, s->d_panels32.p, s->stream);
+    s->d_st32.ensure((size_t)N * N);
+    launch_statics32(s->d_st.p, (size_t)N * N, s->d_st32.p, s->stream);
+    HBG_HIP(hipStreamSynchronize(s->stream));
+    s->st32_valid = true;
+  }
+  if (s->near_valid) return;
+  const double ratio = near_ratio();
+  double rmax = 0.0, lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int j = 0; j < N; ++j) {
+    rmax = std::max(rmax, s->panels[j].radius);
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], s->panels[j].c[d]);
+      hi[d] = std::max(hi[d], s->panels[j].c[d]);
+    }
+  }
+  const double cell = std::max(2.0 * ratio * rmax, 1e-12);
+  int dims[3];
+  for (int d = 0; d < 3; ++d) dims[d] = std::max(1, std::min(1024, (int)((hi[d] - lo[d]) / cell) + 1));
+  auto cell_of = [&](int j, int d) {
+    return std::min(dims[d] - 1, std::max(0, (int)((s->panels[j].c[d] - lo[d]) / cell)));
+  };
+  std::vector<std::vector<int>> bins((size_t)dims[0] * dims[1] * dims[2]);
+  for (int j = 0; j < N; ++j)
+    bins[((size_t)cell_of(j, 2) * dims[1] + cell_of(j, 1)) * dims[0] + cell_of(j, 0)].push_back(j);
+  std::vector<int> rowptr(N + 1, 0);
+  std::vector<int2> ent;
+  for (int k = 0; k < N; ++k) {
+    const PanelD& pk = s->panels[k];
+    std::vector<int> cols;
+    const int ck[3] = {cell_of(k, 0), cell_of(k, 1), cell_of(k, 2)};
+    for (int z = std::max(0, ck[2] - 1); z <= std::min(dims[2] - 1, ck[2] + 1); ++z)
+      for (int y = std::max(0, ck[1] - 1); y <= std::min(dims[1] - 1, ck[1] + 1); ++y)
+        for (int x = std::max(0, ck[0] - 1); x <= std::min(dims[0] - 1, ck[0] + 1); ++x)
+          for (int n : bins[((size_t)z * dims[1] + y) * dims[0] + x]) {
+            const PanelD& pn = s->panels[n];
+            const double dx = pk.c[0] - pn.c[0], dy = pk.c[1] - pn.c[1], dz = pk.c[2] - pn.c[2];
+            if (std::sqrt(dx * dx + dy * dy + dz * dz) < ratio * (pk.radius + pn.radius)) cols.push_back(n);
+          }
+    std::sort(cols.begin(), cols.end());
+    for (int n : cols) ent.push_back(make_int2(k, n));
+    rowptr[k + 1] = (int)ent.size();
+  }
+  s->near_nnz = (int)ent.size();
+  s->near_ent = ent;
+  s->d_near_rowptr.ensure(rowptr.size());
+  s->d_near_ent.ensure(std::max<size_t>(ent.size(), 1));
+  HBG_HIP(hipMemcpy(s->d_near_rowptr.p, rowptr.data(), rowptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!ent.empty())
+    HBG_HIP(hipMemcpy(s->d_near_ent.p, ent.data(), ent.size() * sizeof(int2), hipMemcpyHostToDevice));
+  s->near_valid = true;
+}
+
+// Whether every pair of a frequency K lies inside the FP32 tables (the FP32 kernel has no
+// asymptotic forms): infinite depth X = K R < 48 and Y = K |z + zeta| < 48 over the body's extent;
+// finite depth the same for the Newman form (R < h / 2) and k0 R < 48 for the series.
+bool fp32_in_range(HbgSession* s, double K) {
+  const int N = s->N;
+  double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY}, zmin = 0.0;
+  for (int j = 0; j < N; ++j) {
+    for (int d = 0; d < 2; ++d) {
+      lo[d] = std::min(lo[d], s->panels[j].c[d]);
+      hi[d] = std::max(hi[d], s->panels[j].c[d]);
+    }
+    zmin = std::min(zmin, s->panels[j].c[2]);
+  }
+  const double Rmax = std::hypot(hi[0] - lo[0], hi[1] - lo[1]);
+  const double lim = 0.999 * XT;
+  if (!std::isfinite(s->depth)) return K * Rmax < lim && K * 2.0 * -zmin < lim;
+  const FinFreq f = make_fin_freq(s->depth, K);
+  return K * std::min(Rmax, 0.5 * s->depth) < lim && K * 2.0 * -zmin < lim && f.k0 * Rmax < lim;
+}
+
+// S (complex64, s->d_S32) and A (complex64, ld N) of the frequency K by the FP32 kernels, the near
+// entries in FP64 (their differences into s->d_dS and dA); fi: the frequency's index in the
+// solve's preloaded FinFreq arrays (finite depth). Returns the incident wave's wave number.
+// A32r: the copy of A (ld N) that stays unfactored for the refinement's residuals.
+double assemble_frequency32(HbgSession* s, double K, double logc, int fi, float2* A32, float2* A32r, double2* dA) {
+  const int N = s->N;
+  TablesOnDevice& t = tables32_for(s->device, s->stream);
+  if (!std::isfinite(s->depth)) {
+    launch_assemble32(s->d_panels32.p, N, K, logc, t.t2d32, t.t1d32, s->d_st32.p, s->d_S32.p, A32, A32r, N, 2.0 * PI,
+                      s->stream);
+    launch_near64(s->d_panels.p, N, s->d_near_ent.p, s->near_nnz, false, K, nullptr, FinTables{}, s->T, logc,
+                  s->d_st.p, s->d_S32.p, A32, A32r, N, 2.0 * PI, s->d_dS.p, dA, s->stream);
+    return K;
+  }
+  const FinFreq f = make_fin_freq(s->depth, K);
+  s->d_phi1.ensure((size_t)FTS * FTW * FTILE);
+  s->d_phi2.ensure((size_t)FTS * FTV * FTILE);
+  s->d_pf.ensure(N);
+  s->d_pcs.ensure((size_t)N * FIN_ROOTS);
+  s->d_fwork.ensure(fin_tables_work());
+  launch_fin_tables(f, s->T, s->d_gl.p + 16 * 32, s->d_phi1.p, s->d_phi2.p, s->fitm, s->d_panels.p, N, s->d_pf.p,
+                    s->d_pcs.p, s->d_fwork.p, s->stream);
+  s->d_phi1_32.ensure(fin32_phi1_elems());
+  s->d_phi2_32.ensure(fin32_phi2_elems());
+  s->d_pf32.ensure(N);
+  s->d_pcs32.ensure((size_t)N * FIN_ROOTS);
+  launch_fin_tables32(s->d_phi1.p, s->d_phi2.p, t.fin32P, s->d_phi1_32.p, s->d_phi2_32.p, s->d_pf.p, s->d_pcs.p, N,
+                      s->d_pf32.p, s->d_pcs32.p, s->stream);
+  launch_assemble_fin32(s->d_panels32.p, N, s->d_ffs32.p + (size_t)fi * finfreq32_bytes(), s->d_phi1_32.p,
+                        s->d_phi2_32.p, t.ktab32, s->d_pf32.p, s->d_pcs32.p, t.t2d32, t.t1d32, logc, s->d_st32.p,
+                        s->d_S32.p, A32, A32r, N, 2.0 * PI, s->stream);
+  launch_near64(s->d_panels.p, N, s->d_near_ent.p, s->near_nnz, true, K, s->d_ffs.p + fi, fin_tables(s), s->T, logc,
+                s->d_st.p, s->d_S32.p, A32, A32r, N, 2.0 * PI, s->d_dS.p, dA, s->stream);
+  return f.k0;
+}
 
 // Device bytes per frequency in flight: its N x (N + R) system (and its complex64 copy), the
 // LU's per-panel inverses, the refinement's vectors.
@@ -606,7 +813,8 @@ int batch_size(HbgSession* s, int N, int R, int F, bool mixed, int copies = 1) {
 }
 
 void check_options(const HbgOptions* o) {
-  if (o && o->precision != HBG_PRECISION_FP64 && o->precision != HBG_PRECISION_MIXED)
+  if (o && o->precision != HBG_PRECISION_FP64 && o->precision != HBG_PRECISION_MIXED &&
+      o->precision != HBG_PRECISION_MIXED32)
     throw Error{HBG_ERR_UNSUPPORTED, "precision not supported"};
   if (o && !(o->log_tolerance >= 0.0)) throw Error{HBG_ERR_VALUE, "log_tolerance must be >= 0"};
   if (o && o->surrogate != 0 && o->surrogate != 1) throw Error{HBG_ERR_VALUE, "surrogate must be 0 or 1"};
@@ -817,6 +1025,8 @@ static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, 
   s->statics_valid = false;
   s->stinf_valid = false;
   s->stf_valid = false;
+  s->st32_valid = false;
+  s->near_valid = false;
   s->field_F = -1;
   s->kochin_F = -1;
   s->N = 0;
@@ -1106,6 +1316,93 @@ HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const 
   GUARD_END("hbg_finite_kernel")
 }
 
+HbgStatus hbg_wave_kernel32(HbgSession* s, const double* X, const double* Y, int32_t n, double* out) {
+  if (!s || !X || !Y || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_wave_kernel32");
+  if (n <= 0) return HBG_OK;
+  for (int i = 0; i < n; ++i)
+    if (!(X[i] >= 0.0) || !(Y[i] > 0.0) || !(X[i] < XT) || !(Y[i] < YT))
+      return fail(HBG_ERR_VALUE, "point " + std::to_string(i) + " outside 0 <= X < 48, 0 < Y < 48",
+                  "hbg_wave_kernel32");
+  GUARD_BEGIN
+  use_device(s);
+  TablesOnDevice& t = tables32_for(s->device, s->stream);
+  DevBuf<double> dx, dy, dout;
+  dx.ensure(n);
+  dy.ensure(n);
+  dout.ensure(4 * (size_t)n);
+  HBG_HIP(hipMemcpy(dx.p, X, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dy.p, Y, n * sizeof(double), hipMemcpyHostToDevice));
+  launch_wave_points32(t.t2d32, t.t1d32, dx.p, dy.p, n, dout.p, s->stream);
+  HBG_HIP(hipMemcpyAsync(out, dout.p, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  return HBG_OK;
+  GUARD_END("hbg_wave_kernel32")
+}
+
+HbgStatus hbg_finite_kernel32(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
+                              const double* zeta, double* out) {
+  if (!s || !R || !z || !zeta || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_finite_kernel32");
+  if (!(h > 0.0) || !std::isfinite(h) || !(K > 0.0) || !std::isfinite(K) || n < 0)
+    return fail(HBG_ERR_VALUE, "h and K must be finite and > 0", "hbg_finite_kernel32");
+  const FinFreq f0 = make_fin_freq(h, K);
+  for (int i = 0; i < n; ++i) {
+    const bool nearf = R[i] < 0.5 * h;
+    if (!(R[i] >= 0.0) || !(z[i] < 0.0) || !(zeta[i] < 0.0) || z[i] < -h || zeta[i] < -h ||
+        (nearf && (K * R[i] >= XT || -K * (z[i] + zeta[i]) >= YT)) || f0.k0 * R[i] >= XT)
+      return fail(HBG_ERR_VALUE, "point " + std::to_string(i) + " outside the FP32 tables or the water",
+                  "hbg_finite_kernel32");
+  }
+  if (n == 0) return HBG_OK;
+  GUARD_BEGIN
+  use_device(s);
+  TablesOnDevice& t = tables32_for(s->device, s->stream);
+  std::vector<PanelD> pts(2 * (size_t)n);
+  std::memset(pts.data(), 0, pts.size() * sizeof(PanelD));
+  for (int i = 0; i < n; ++i) {
+    pts[i].c[2] = z[i];
+    pts[n + i].c[2] = zeta[i];
+  }
+  DevBuf<PanelD> dp;
+  DevBuf<double> dphi1, dphi2, dR, dz, dzeta, dout;
+  DevBuf<double4> dpf;
+  DevBuf<double2> dpcs;
+  DevBuf<float> p1, p2;
+  DevBuf<float4> pf32;
+  DevBuf<float2> pcs32;
+  DevBuf<unsigned char> f32;
+  dp.ensure(pts.size());
+  dphi1.ensure((size_t)FTS * FTW * FTILE);
+  dphi2.ensure((size_t)FTS * FTV * FTILE);
+  dpf.ensure(pts.size());
+  dpcs.ensure(pts.size() * FIN_ROOTS);
+  p1.ensure(fin32_phi1_elems());
+  p2.ensure(fin32_phi2_elems());
+  pf32.ensure(pts.size());
+  pcs32.ensure(pts.size() * FIN_ROOTS);
+  f32.ensure(finfreq32_bytes());
+  dR.ensure(n);
+  dz.ensure(n);
+  dzeta.ensure(n);
+  dout.ensure(8 * (size_t)n);
+  std::vector<unsigned char> hf(finfreq32_bytes());
+  make_finfreq32(f0, hf.data());
+  HBG_HIP(hipMemcpy(f32.p, hf.data(), hf.size(), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dp.p, pts.data(), pts.size() * sizeof(PanelD), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dR.p, R, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dz.p, z, n * sizeof(double), hipMemcpyHostToDevice));
+  HBG_HIP(hipMemcpy(dzeta.p, zeta, n * sizeof(double), hipMemcpyHostToDevice));
+  s->d_fwork.ensure(fin_tables_work());
+  launch_fin_tables(f0, s->T, s->d_gl.p + 16 * 32, dphi1.p, dphi2.p, s->fitm, dp.p, 2 * n, dpf.p, dpcs.p,
+                    s->d_fwork.p, s->stream);
+  launch_fin_tables32(dphi1.p, dphi2.p, t.fin32P, p1.p, p2.p, dpf.p, dpcs.p, 2 * n, pf32.p, pcs32.p, s->stream);
+  launch_fin_points32(f32.p, p1.p, p2.p, t.ktab32, pf32.p, pcs32.p, t.t2d32, t.t1d32, dR.p, dz.p, dzeta.p, n, dout.p,
+                      s->stream);
+  HBG_HIP(hipMemcpyAsync(out, dout.p, 8 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  return HBG_OK;
+  GUARD_END("hbg_finite_kernel32")
+}
+
 HbgStatus hbg_dispersion_roots(double H, int32_t n, double* out) {
   if (!out || n < 0) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_dispersion_roots");
   if (!(H >= 0.0)) return fail(HBG_ERR_VALUE, "H must be >= 0", "hbg_dispersion_roots");
@@ -1151,6 +1448,54 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   s->d_M.ensure((size_t)N * N);
   const double K = omega * omega / s->grav;
   check_depth(s);
+  if (o.precision == HBG_PRECISION_MIXED32) {
+    // the matrices of the FP32 assembly: its complex64 entries, the FP64 values at the near entries
+    if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with the surrogate"};
+    if (!fp32_in_range(s, K)) throw Error{HBG_ERR_UNSUPPORTED, "pairs beyond the FP32 tables at this frequency"};
+    ensure_fp32_geometry(s);
+    if (std::isfinite(s->depth)) {
+      const FinFreq f = make_fin_freq(s->depth, K);
+      std::vector<unsigned char> f32(finfreq32_bytes());
+      make_finfreq32(f, f32.data());
+      s->d_ffs.ensure(1);
+      s->d_ffs32.ensure(f32.size());
+      HBG_HIP(hipMemcpy(s->d_ffs.p, &f, sizeof(FinFreq), hipMemcpyHostToDevice));
+      HBG_HIP(hipMemcpy(s->d_ffs32.p, f32.data(), f32.size(), hipMemcpyHostToDevice));
+    }
+    const size_t nnz = (size_t)std::max(s->near_nnz, 1);
+    s->d_S32.ensure((size_t)N * N);
+    s->d_dS.ensure(nnz);
+    DevBuf<float2> a32;
+    DevBuf<double2> dA;
+    a32.ensure((size_t)N * N);
+    dA.ensure(nnz);
+    DevBuf<float2> a32r;
+    a32r.ensure((size_t)N * N);
+    assemble_frequency32(s, K, o.logarithmic ? 1.0 : 0.0, 0, a32.p, a32r.p, dA.p);
+    std::vector<float2> hs((size_t)N * N), ha((size_t)N * N);
+    std::vector<double2> hds(nnz), hda(nnz);
+    HBG_HIP(hipMemcpyAsync(hs.data(), s->d_S32.p, hs.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipMemcpyAsync(ha.data(), a32.p, ha.size() * sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipMemcpyAsync(hds.data(), s->d_dS.p, nnz * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipMemcpyAsync(hda.data(), dA.p, nnz * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < N; ++k)
+      for (int n = 0; n < N; ++n) {
+        const size_t src = (size_t)n * N + k, dst = 2 * ((size_t)k * N + n);
+        S[dst] = hs[src].x;
+        S[dst + 1] = hs[src].y;
+        A[dst] = ha[src].x;
+        A[dst + 1] = ha[src].y;
+      }
+    for (int e = 0; e < s->near_nnz; ++e) {
+      const size_t dst = 2 * ((size_t)s->near_ent[e].x * N + s->near_ent[e].y);
+      S[dst] += hds[e].x;
+      S[dst + 1] += hds[e].y;
+      A[dst] += hda[e].x;
+      A[dst + 1] += hda[e].y;
+    }
+    return HBG_OK;
+  }
   assemble_frequency(s, K, o.logarithmic ? 1.0 : 0.0, s->d_M.p, o.surrogate != 0);
   std::vector<double2> hs((size_t)N * N), ha((size_t)N * N);
   HBG_HIP(hipMemcpyAsync(hs.data(), s->d_S.p, hs.size() * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
@@ -1351,14 +1696,31 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // its forces are taken; S, the right-hand-side partials and the finite-depth tables are shared
   const size_t mstride = (size_t)N * (N + R);
   const size_t hstride = (size_t)std::max(H, 1) * NF;
-  const bool mixed = o.precision == HBG_PRECISION_MIXED;
+  // the FP32 far-field assembly: one geometry without images, no surrogate, every frequency's
+  // pairs inside the FP32 tables; otherwise HBG_PRECISION_MIXED
+  bool m32 = o.precision == HBG_PRECISION_MIXED32 && s->G == 1 && !o.surrogate;
+  for (int i = 0; m32 && i < n_omegas; ++i) m32 = fp32_in_range(s, omegas[i] * omegas[i] / s->grav);
+  const bool mixed = o.precision == HBG_PRECISION_MIXED || (o.precision == HBG_PRECISION_MIXED32 && !m32);
   const bool prof = s->profiling;
+  const double m32_tol = std::pow(10.0, -(double)g_mixed32_tol_exp);
+  if (m32) {
+    ensure_fp32_geometry(s);
+    if (std::isfinite(s->depth) && n_omegas > 0) {
+      std::vector<unsigned char> f32((size_t)n_omegas * finfreq32_bytes());
+      for (int i = 0; i < n_omegas; ++i)
+        make_finfreq32(make_fin_freq(s->depth, omegas[i] * omegas[i] / s->grav), f32.data() + i * finfreq32_bytes());
+      s->d_ffs32.ensure(f32.size());
+      HBG_HIP(hipMemcpy(s->d_ffs32.p, f32.data(), f32.size(), hipMemcpyHostToDevice));
+    }
+  }
+  const size_t nnzc = (size_t)std::max(s->near_nnz, 1);  // near entries per system (FP32 assembly)
+  const size_t bstride = (size_t)N * R;                   // FP64 right-hand sides per system (FP32 assembly)
   // two halves of the batch buffers: the assembly of batch t + 1 (session stream) runs beside the
   // LU, refinement and forces of batch t (LU stream). One half (and one stream) when one batch
   // takes every frequency, or with stage timing. The batch size is balanced over the batches.
-  int B = batch_size(s, N, R, n_omegas, mixed, G);
+  int B = batch_size(s, N, R, n_omegas, mixed || m32, G);
   const int halves = (prof || B >= n_omegas) ? 1 : 2;
-  if (halves == 2) B = batch_size(s, N, R, n_omegas, mixed, 2 * G);
+  if (halves == 2) B = batch_size(s, N, R, n_omegas, mixed || m32, 2 * G);
   if (B > 0 && n_omegas > 0) {
     const int nbt = (n_omegas + B - 1) / B;
     B = (n_omegas + nbt - 1) / nbt;
@@ -1366,7 +1728,17 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   hipStream_t sa = s->stream, sb = halves == 1 ? s->stream : s->lu_stream;
   const int BG = B * G;  // systems in flight per half
   s->d_S.ensure((size_t)N * N * G);
-  s->d_M.ensure(mstride * BG * halves);
+  // the FP32 assembly keeps no FP64 matrices in flight: one FP64 system for a frequency that
+  // needs the FP64 path
+  s->d_M.ensure(m32 ? mstride : mstride * BG * halves);
+  if (m32) {
+    s->d_M32.ensure(mstride * BG * halves);
+    s->d_B64.ensure(bstride * BG * halves);
+    s->d_S32.ensure((size_t)N * N);
+    s->d_A32r.ensure((size_t)N * N * BG * halves);
+    s->d_dS.ensure(nnzc);
+    s->d_dA.ensure(nnzc * BG * halves);
+  }
   s->d_phiI.ensure(hstride * B * halves);
   s->d_dphiI.ensure(hstride * B * halves);
   s->d_u.ensure((size_t)R * NF);
@@ -1376,7 +1748,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     s->d_uc.ensure((size_t)R * N * G);
     s->d_phif.ensure((size_t)R * NF);
   }
-  if (mixed) {
+  if (mixed || m32) {
     s->d_M32.ensure(mstride * BG * halves);
     s->d_D32.ensure((size_t)N * R * BG);
     s->d_X64.ensure((size_t)N * R * BG);
@@ -1430,6 +1802,19 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     const double w = omegas[fi], K = w * w / s->grav;
     double2* Mb = s->d_M.p + mstride * slot(h, b) * G;
     if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b], sa));
+    if (m32) {
+      // the FP32 assembly into the slot's complex64 system, the right-hand sides in FP64
+      double2* Bb = s->d_B64.p + bstride * slot(h, b);
+      const double kw = assemble_frequency32(s, K, logc, fi, s->d_M32.p + mstride * slot(h, b),
+                                             s->d_A32r.p + (size_t)N * N * slot(h, b), s->d_dA.p + nnzc * slot(h, b));
+      launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
+                      s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b), s->d_u.p, sa,
+                      s->walls);
+      if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 1], sa));
+      launch_rhs32(s->d_S32.p, N, s->d_u.p, R, Bb, s->d_part.p, s->d_near_rowptr.p, s->d_near_ent.p, s->d_dS.p, sa);
+      if (prof) HBG_HIP(hipEventRecord(s->bev[3 * b + 2], sa));
+      return;
+    }
     const double kw = assemble_frequency(s, K, logc, Mb, o.surrogate != 0,
                                          mixed ? s->d_M32.p + mstride * slot(h, b) * G : nullptr, N, mstride,
                                          std::isfinite(s->depth) ? s->d_ffs.p + fi : nullptr);
@@ -1453,6 +1838,25 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       HBG_HIP(hipEventRecord(s->assembled[h], sa));
       HBG_HIP(hipStreamWaitEvent(sb, s->assembled[h], 0));
     }
+  };
+  // a system of the FP32 assembly that needs the FP64 path: FP64 assembly into the one FP64
+  // system, its LU (pivoted where needed), the solution into the slot's right-hand sides
+  auto fallback64 = [&](int fi, int b, int h) {
+    const double w = omegas[fi], K = w * w / s->grav;
+    const double kw = assemble_frequency(s, K, logc, s->d_M.p, false, nullptr, N, mstride,
+                                         std::isfinite(s->depth) ? s->d_ffs.p + fi : nullptr);
+    launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, w, kw, s->grav, depth, s->d_heads.p,
+                    s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b), s->d_u.p, sa, s->walls);
+    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->d_part.p, sa);
+    if (sb != sa) {
+      HBG_HIP(hipEventRecord(s->assembled[h], sa));
+      HBG_HIP(hipStreamWaitEvent(sb, s->assembled[h], 0));
+    }
+    const int zp = lu_solve_device(s->d_M.p, N, N, R, sb, nullptr);
+    if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(zp) + ")"};
+    HBG_HIP(hipMemcpyAsync(s->d_B64.p + bstride * slot(h, b), s->d_M.p + (size_t)N * N, bstride * sizeof(double2),
+                           hipMemcpyDeviceToDevice, sb));
+    if (prof) s->prof[15] += 1.0;
   };
   const int nbatches = (n_omegas + B - 1) / B;
   auto enqueue_batch = [&](int t) {
@@ -1517,6 +1921,24 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
         if (mixed && prof) s->prof[15] += 1.0;
       }
       std::fill(status.begin(), status.begin() + nbg, 0);
+    } else if (m32) {
+      mixed32_solve_batched(s->d_M32.p + mstride * slot(h, 0), s->d_A32r.p + (size_t)N * N * slot(h, 0),
+                            s->d_B64.p + bstride * slot(h, 0),
+                            s->d_near_rowptr.p, s->d_near_ent.p, s->d_dA.p + nnzc * slot(h, 0), nnzc, s->d_X64.p,
+                            s->d_D32.p, s->d_rpart.p, s->d_rpart2.p, s->d_rscale.p, nb, N, R, m32_tol,
+                            MIXED_MAX_ITER, sb, prof ? &lus : nullptr, status.data(), iters.data(), eta.data(),
+                            s->d_active.p);
+      for (int b = 0; b < nb; ++b) {
+        if (status[b] > 0)
+          throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(status[b]) + ")"};
+        if (status[b] < 0) {
+          fallback64(f0 + b, b, h);
+        } else if (prof) {
+          s->prof[13] += iters[b];
+          s->prof[14] = std::max(s->prof[14], eta[b]);
+        }
+        status[b] = 0;
+      }
     } else if (mixed) {
       mixed_solve_batched(Mh, s->d_M32.p + mstride * slot(h, 0), s->d_X64.p, s->d_D32.p, s->d_rpart.p, s->d_rpart2.p,
                           s->d_rscale.p, nb, N, R, true, MIXED_TOL, MIXED_MAX_ITER, sb, prof ? &lus : nullptr,
@@ -1545,7 +1967,11 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
     const int e0 = 3 * B, e1 = 3 * B + 1;
     if (prof) HBG_HIP(hipEventRecord(s->bev[e0], sb));
     for (int b = 0; b < nb; ++b) {
-      if (G == 1) {
+      if (m32) {
+        launch_forces_sol(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, omegas[f0 + b], s->rho,
+                          s->d_B64.p + bstride * slot(h, b), N, s->d_phiI.p + hstride * slot(h, b),
+                          s->d_dphiI.p + hstride * slot(h, b), s->d_out.p + nout * slot(h, b), sb);
+      } else if (G == 1) {
         launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, H, omegas[f0 + b], s->rho, Mh + mstride * b, N,
                       s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
                       s->d_out.p + nout * slot(h, b), sb);
@@ -1558,7 +1984,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       }
       // Kochin functions and field values from the potential on all NF panels (a symmetric body:
       // recombined just above, before the next frequency reuses the buffer)
-      const double2* solb = G == 1 ? Mh + mstride * b + (size_t)N * N : s->d_phif.p;
+      const double2* solb = m32 ? s->d_B64.p + bstride * slot(h, b)
+                                : (G == 1 ? Mh + mstride * b + (size_t)N * N : s->d_phif.p);
       const double w = omegas[f0 + b], K = w * w / s->grav;
       if (s->nk > 0) {
         const double kw = std::isfinite(s->depth) ? make_fin_freq(s->depth, K).k0 : K;
@@ -1604,7 +2031,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       s->prof[7] += stage_ms(s->bev[e0], s->bev[e1]);
       s->prof[8] += nb;
       s->prof[9] += nbg * 8.0 / 3.0 * (double)N * N * N;
-      s->prof[10] += nbg * (double)N * N * (sizeof(StaticD) + 2 * sizeof(double2));
+      s->prof[10] += nbg * (double)N * N * (m32 ? 4 * sizeof(float) + 2 * sizeof(float2)
+                                                : sizeof(StaticD) + 2 * sizeof(double2));
       s->prof[12] += (double)lus.fallback_panels;
       s->prof[16] += lus.refine_ms;
     }
@@ -2053,7 +2481,7 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_tuning");
   switch (key) {
     case HBG_TUNE_LU_GEMM:
-      if (value < 1 || value > 4) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..4", "hbg_set_tuning");
+      if (value < 1 || value > 5) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..5", "hbg_set_tuning");
       lu_set_gemm_variant(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH:
@@ -2065,12 +2493,17 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       lu_set_graphs(value);
       return HBG_OK;
     case HBG_TUNE_LU_GEMM_MIXED:
-      if (value < 1 || value > 4) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..4", "hbg_set_tuning");
+      if (value < 1 || value > 5) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..5", "hbg_set_tuning");
       lu_set_gemm_variant32(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH_MIXED:
       if (value < 1 || value > 16) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..16", "hbg_set_tuning");
       lu_set_depth32(value);
+      return HBG_OK;
+    case HBG_TUNE_MIXED32_TOL:
+      if (value < 6 || value > 14) return fail(HBG_ERR_VALUE, "refinement tolerance exponent must be in 6..14",
+                                               "hbg_set_tuning");
+      g_mixed32_tol_exp = value;
       return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_set_tuning");
@@ -2094,6 +2527,9 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value) {
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH_MIXED:
       *value = lu_depth32();
+      return HBG_OK;
+    case HBG_TUNE_MIXED32_TOL:
+      *value = g_mixed32_tol_exp;
       return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_get_tuning");

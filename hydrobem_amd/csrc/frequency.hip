@@ -5,8 +5,13 @@
 #include "green_fin_dev.h"
 #include "green_inf_dev.h"
 #include "hbg_internal.h"
+#include "pair_sort.h"
 
 namespace hbg {
+
+void launch_near_rows(const int* rowptr, const int2* ent, const double2* d, size_t dstride, const double2* v,
+                      size_t vstride, int N, int R, double sign, double2* out, size_t ostride, int ldo, int accumulate,
+                      int nsys, hipStream_t stream);
 
 // The matrices of an ordinary frequency: for the field point k (centroid of panel k) and panel n,
 //   S(k, n) = rs + A_n Wr - 2 K logc dl + i A_n Wi
@@ -31,71 +36,6 @@ __device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __re
   const double2 v = make_double2((row == col ? free_term : 0.0) - dre, -dim);
   A[(size_t)col * lda + row] = v;
   if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
-}
-
-constexpr int ASM_TILE = 64;
-
-// The tables make the kernels' cost: a pair reads hundreds of coefficients (the V tile and the 1D
-// tile in infinite depth; the two Phi tiles and the V tile near the axis, one K0/K1 tile per root
-// in the series of finite depth), and read lane by lane they bound the kernel by the vector
-// memory pipeline. So a workgroup first sorts its tile's pairs by table tile (a counting sort in
-// LDS), and each wavefront then evaluates 64 consecutive sorted pairs with tile reads that are
-// wavefront-uniform: scalar loads, made once per distinct tile of the wavefront.
-constexpr int PS_KEYS = 4096;                              // keys of the sort
-constexpr int PS_PER_THREAD = ASM_TILE * ASM_TILE / 256;   // pairs per thread of the tile
-struct PairSort {
-  unsigned cnt[PS_KEYS];
-  unsigned short order[ASM_TILE * ASM_TILE];  // pair p = k_local + 64 n_local, in key order
-  unsigned wsum[4];
-};
-
-// Sorts the pairs (k, n) of the tile (bi, bj) (k <= n on the diagonal tile, both < N) by
-// key_of(k, n) in [0, PS_KEYS); returns their number. Order within a key is arbitrary: every
-// pair is evaluated on its own.
-template <class KeyOf>
-__device__ __forceinline__ unsigned sort_tile_pairs(PairSort& L, int bi, int bj, int N, KeyOf key_of) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int i = tid; i < PS_KEYS; i += 256) L.cnt[i] = 0;
-  __syncthreads();
-  unsigned kr[PS_PER_THREAD];  // key << 13 | rank within the key, or ~0 for no pair
-#pragma unroll
-  for (int j = 0; j < PS_PER_THREAD; ++j) {
-    const int p = tid + 256 * j, k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
-    kr[j] = 0xffffffffu;
-    if (k < N && n < N && !(bi == bj && k > n)) {
-      const int kf = key_of(k, n);
-      const unsigned key = kf < 0 ? 0u : (kf >= PS_KEYS ? (unsigned)PS_KEYS - 1 : (unsigned)kf);
-      kr[j] = (key << 13) | atomicAdd(&L.cnt[key], 1u);
-    }
-  }
-  __syncthreads();
-  // exclusive scan of the counts: thread t owns cnt[16 t .. 16 t + 15]
-  constexpr int OWN = PS_KEYS / 256;
-  unsigned loc[OWN], sum = 0;
-#pragma unroll
-  for (int i = 0; i < OWN; ++i) {
-    loc[i] = sum;
-    sum += L.cnt[OWN * tid + i];
-  }
-  unsigned incl = sum;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned o = __shfl_up(incl, off);
-    if (lane >= off) incl += o;
-  }
-  if (lane == 63) L.wsum[wave] = incl;
-  __syncthreads();
-  unsigned base = incl - sum;
-  for (int w = 0; w < wave; ++w) base += L.wsum[w];
-  const unsigned total = L.wsum[0] + L.wsum[1] + L.wsum[2] + L.wsum[3];
-#pragma unroll
-  for (int i = 0; i < OWN; ++i) L.cnt[OWN * tid + i] = base + loc[i];
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < PS_PER_THREAD; ++j)
-    if (kr[j] != 0xffffffffu) L.order[L.cnt[kr[j] >> 13] + (kr[j] & 8191u)] = (unsigned short)(tid + 256 * j);
-  __syncthreads();
-  return total;
 }
 
 // Infinite depth: the pairs sorted by the V tile of (X, Y) = (K R, -K (z + zeta)) (then also
@@ -330,8 +270,12 @@ int rhs_chunk_len(int N) { return N <= 4096 ? 128 : 256; }
 // loads are in flight per lane (the kernel streams S once; its rate is that of the loads in flight),
 // and the wavefront-uniform u of 64 columns at a time is staged in LDS and read as broadcasts.
 constexpr int RHS_SUB = 64;
-template <int RM>
-__global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restrict__ S, size_t sstride, int N,
+__device__ __forceinline__ double2 to_double2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_double2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+
+// ST: the element type of S (complex128, or complex64 from the FP32 assembly; the sums in FP64)
+template <int RM, class ST = double2>
+__global__ void __launch_bounds__(256) rhs_partial_kernel(const ST* __restrict__ S, size_t sstride, int N,
                                                           int lds, const double2* __restrict__ u, size_t ustride,
                                                           int R, int chunk_len, double2* __restrict__ part,
                                                           size_t pstride) {
@@ -367,13 +311,13 @@ __global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restr
     __syncthreads();
     int j = 0;
     for (; j + 8 <= ns; j += 8) {
-      double2 sv[8];
+      ST sv[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) sv[q] = S[(size_t)(s0 + j + q) * lds + kk];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) column(sv[q], j + q);
+      for (int q = 0; q < 8; ++q) column(to_double2(sv[q]), j + q);
     }
-    for (; j < ns; ++j) column(S[(size_t)(s0 + j) * lds + kk], j);
+    for (; j < ns; ++j) column(to_double2(S[(size_t)(s0 + j) * lds + kk]), j);
   }
   if (row)
 #pragma unroll
@@ -381,8 +325,9 @@ __global__ void __launch_bounds__(256) rhs_partial_kernel(const double2* __restr
       if (c < R) part[((size_t)chunk * R + c) * N + k] = acc[c];
 }
 
+// the sums into column col0 + c of M (leading dimension ldm)
 __global__ void rhs_reduce_kernel(const double2* __restrict__ part, size_t pstride, int N, int R, int chunks,
-                                  double2* __restrict__ M, size_t mstride, int ldm) {
+                                  double2* __restrict__ M, size_t mstride, int ldm, int col0) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)R * N) return;
   part += blockIdx.y * pstride;
@@ -394,7 +339,7 @@ __global__ void rhs_reduce_kernel(const double2* __restrict__ part, size_t pstri
     s.x += v.x;
     s.y += v.y;
   }
-  M[(size_t)(N + c) * ldm + k] = s;
+  M[(size_t)(col0 + c) * ldm + k] = s;
 }
 
 // A system of order nz stored at order n (leading dimension n) among systems of uneven order:
@@ -901,8 +846,28 @@ void launch_rhs_batched(const double2* S, size_t sstride, int N, int lds, const 
   else
     rhs_partial_kernel<MAXRHS><<<g, 256, 0, stream>>>(S, sstride, N, lds, u, ustride, R, len, part, pstride);
   rhs_reduce_kernel<<<dim3((unsigned)(((size_t)R * N + 255) / 256), nsys), 256, 0, stream>>>(part, pstride, N, R,
-                                                                                           chunks, M, mstride, ldm);
+                                                                                           chunks, M, mstride, ldm, N);
   HBG_HIP(hipGetLastError());
+}
+
+// B (N x R, ld N) = S (-u) with S complex64 (the FP32 assembly's) and, at the near entries
+// (rowptr, ent), S + dS; the products summed in FP64
+void launch_rhs32(const float2* S, int N, const double2* u, int R, double2* B, double2* part, const int* rowptr,
+                  const int2* ent, const double2* dS, hipStream_t stream) {
+  const int chunks = rhs_chunks(N), len = rhs_chunk_len(N);
+  const size_t pstride = (size_t)chunks * R * N;
+  const dim3 g((N + 255) / 256, chunks, 1);
+  if (R <= 8)
+    rhs_partial_kernel<8, float2><<<g, 256, 0, stream>>>(S, 0, N, N, u, 0, R, len, part, pstride);
+  else if (R <= 16)
+    rhs_partial_kernel<16, float2><<<g, 256, 0, stream>>>(S, 0, N, N, u, 0, R, len, part, pstride);
+  else
+    rhs_partial_kernel<MAXRHS, float2><<<g, 256, 0, stream>>>(S, 0, N, N, u, 0, R, len, part, pstride);
+  rhs_reduce_kernel<<<dim3((unsigned)(((size_t)R * N + 255) / 256), 1), 256, 0, stream>>>(part, pstride, N, R, chunks,
+                                                                                       B, 0, N, 0);
+  HBG_HIP(hipGetLastError());
+  // the near entries' dS (-u), row by row
+  if (rowptr) launch_near_rows(rowptr, ent, dS, 0, u, 0, N, R, -1.0, B, 0, N, 1, 1, stream);
 }
 
 void launch_rhs(const double2* S, int N, int lds, const double2* u, int R, double2* M, int ldm, double2* part,

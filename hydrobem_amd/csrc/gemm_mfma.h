@@ -245,11 +245,18 @@ __device__ __forceinline__ void split8(const float2* v, h8* part) {
 // consecutive k (8 (l / 16) ..) of row or column l % 16; the next chunk of k is loaded into
 // registers while the current one is multiplied. Grid gemm_grid(ceil(nrows / TM),
 // ceil(ncolsC / TN), batch), 256 threads.
-template <int TM, int TN>
-__global__ void __launch_bounds__(256) gemm_f16s_kernel(float2* __restrict__ M, int lda, int r0, int nrows, int c0,
-                                                        int ncolsC, SplitOps so, int kg0, int kb, GemmBatch gb) {
-  constexpr int BR = TM / 32, BC = TN / 32;   // 16 x 16 blocks per wave: rows of C, columns of C
-  constexpr int UA = TM / 64, UB = TN / 64;   // (row or column, k group) units per thread and chunk
+//
+// WR x WC waves (64 WR WC threads) share the TM x TN tile, each (TM / WR) x (TN / WC): the 128 x 128
+// tile of 8 waves (2 x 4, 64 x 32 each) does twice the MFMAs of the 64 x 64 one per barrier and
+// reads a quarter less LDS and half the L2 data per MFMA.
+template <int TM, int TN, int WR = 2, int WC = 2>
+__global__ void __launch_bounds__(64 * WR * WC) gemm_f16s_kernel(float2* __restrict__ M, int lda, int r0, int nrows,
+                                                                 int c0, int ncolsC, SplitOps so, int kg0, int kb,
+                                                                 GemmBatch gb) {
+  constexpr int NT = 64 * WR * WC;
+  constexpr int BR = TM / (16 * WR), BC = TN / (16 * WC);  // 16 x 16 blocks per wave: rows of C, columns of C
+  constexpr int UA = 4 * TM / NT, UB = 4 * TN / NT;        // (row or column, k group) units per thread and chunk
+  static_assert(UA * NT == 4 * TM && UB * NT == 4 * TN, "tile and threads");
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * TM][H16_PAD];  // [part][row of C][k]
   __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * TN][H16_PAD];  // [part][column of C][k]
   const GemmTile tile = gemm_tile((nrows + TM - 1) / TM, (ncolsC + TN - 1) / TN, gb.batch);
@@ -259,21 +266,21 @@ __global__ void __launch_bounds__(256) gemm_f16s_kernel(float2* __restrict__ M, 
   const h8* __restrict__ Us = so.us + tile.b * so.ustride;
   const size_t pad = so.pad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave & 1) * (TM / 2), wc = (wave >> 1) * (TN / 2);
+  const int wr = (wave % WR) * (TM / WR), wc = (wave / WR) * (TN / WC);
   const int tr = tile.row * TM, tc = tile.col * TN;
   h8 ra[UA][4], rb[UB][4];
-  // chunk kc (4 k groups) into registers: unit u = tid + 256 s: row (column) u % T, k group u / T
+  // chunk kc (4 k groups) into registers: unit u = tid + NT s: row (column) u % T, k group u / T
   auto load = [&](int kc) {
 #pragma unroll
     for (int s = 0; s < UA; ++s) {
-      const int u = tid + 256 * s;
+      const int u = tid + NT * s;
       const h8* src = Ls + (size_t)(kg0 + kc / 8 + u / TM) * 4 * pad + r0 + tr + u % TM;
 #pragma unroll
       for (int p = 0; p < 4; ++p) ra[s][p] = src[p * pad];
     }
 #pragma unroll
     for (int s = 0; s < UB; ++s) {
-      const int u = tid + 256 * s;
+      const int u = tid + NT * s;
       const h8* src = Us + (size_t)(kg0 + kc / 8 + u / TN) * 4 * pad + c0 + tc + u % TN;
 #pragma unroll
       for (int p = 0; p < 4; ++p) rb[s][p] = src[p * pad];
@@ -291,13 +298,13 @@ __global__ void __launch_bounds__(256) gemm_f16s_kernel(float2* __restrict__ M, 
   for (int kc = 0; kc < kb; kc += H16_KC) {
 #pragma unroll
     for (int s = 0; s < UA; ++s) {
-      const int u = tid + 256 * s;
+      const int u = tid + NT * s;
 #pragma unroll
       for (int p = 0; p < 4; ++p) *(h8*)&As[p * TM + u % TM][8 * (u / TM)] = ra[s][p];
     }
 #pragma unroll
     for (int s = 0; s < UB; ++s) {
-      const int u = tid + 256 * s;
+      const int u = tid + NT * s;
 #pragma unroll
       for (int p = 0; p < 4; ++p) *(h8*)&Bs[p * TN + u % TN][8 * (u / TN)] = rb[s][p];
     }

@@ -825,7 +825,7 @@ struct LuWorkspace {
   C2* backup = nullptr;      // [A | B] of a single solve, restored for the pivoted path
   C2* subw = nullptr;        // [batch][MAXRHS_LU][n]: work copy of the right-hand sides (substitutions)
   size_t backup_elems = 0;
-  // the split FP16 operands of a block's panels (complex64 LU, GEMM variants 3 and 4)
+  // the split FP16 operands of a block's panels (complex64 LU, GEMM variants 3, 4 and 5)
   h8* ls = nullptr;
   h8* us = nullptr;
   int split_pad = 0, split_kmax = 0, split_batch = 0;
@@ -834,11 +834,11 @@ struct LuWorkspace {
   // buffers for systems of order up to n, blocks of up to kmax columns, `batch` systems; zeroed
   // once (rows and columns past the matrix are read as zeros)
   void ensure_split(int n_, int batch_, int kmax) {
-    if (ls && split_pad >= n_ + 128 && split_kmax >= kmax && split_batch >= batch_) return;
+    if (ls && split_pad >= n_ + 256 && split_kmax >= kmax && split_batch >= batch_) return;
     (void)hipFree(ls);
     (void)hipFree(us);
     ls = us = nullptr;
-    split_pad = std::max(split_pad, n_ + 128);
+    split_pad = std::max(split_pad, n_ + 256);  // the zero rows and columns that 128-wide tiles read past n + R
     split_kmax = std::max(split_kmax, kmax);
     split_batch = std::max(split_batch, batch_);
     const size_t bytes = split_stride() * split_batch * sizeof(h8);
@@ -920,7 +920,7 @@ constexpr int prec_index() {
   return std::is_same<T, double>::value ? 0 : 1;
 }
 void lu_set_gemm_variant(int v) {
-  g_gemm_variant_[0] = (v == 2 || v == 4) ? 2 : 1;  // complex128 has the FP64 kernels only
+  g_gemm_variant_[0] = (v == 2 || v == 4 || v == 5) ? 2 : 1;  // complex128 has the FP64 kernels only
   g_gemm_variant_[1] = v;
 }
 int lu_gemm_variant() { return g_gemm_variant_[0]; }
@@ -1023,8 +1023,8 @@ struct GemmTimer {
 };
 
 // One launch of the trailing-update GEMM in the given tile variant: 1 64 x 64, 2 128 x 64 (FP32 or
-// FP64 matrix cores); complex64 only: 3 64 x 64, 4 128 x 64 on the FP16 matrix cores with split
-// operands (gemm_f16s_kernel).
+// FP64 matrix cores); complex64 only: 3 64 x 64, 4 128 x 64, 5 128 x 128 (8 waves) on the FP16
+// matrix cores with split operands (gemm_f16s_kernel).
 template <typename T>
 void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf,
                  int kk, const GemmBatch& gb, const SplitOps* so, hipStream_t stream) {
@@ -1038,8 +1038,11 @@ void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nr
       const int kg0 = (kf - so->K0) / 8;
       if (variant == 3)
         gemm_f16s_kernel<64, 64><<<g64, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
-      else
+      else if (variant == 4)
         gemm_f16s_kernel<128, 64><<<g128, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
+      else
+        gemm_f16s_kernel<128, 128, 2, 4><<<gemm_grid((nrows + 127) / 128, (ncolsC + 127) / 128, batch), 512, 0,
+                                           stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
       return;
     }
   }
@@ -1361,7 +1364,7 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   *rows = 0;
   std::vector<int> status(batches.back());
   try {
-    const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4}
+    const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4, 5}
                                                                       : std::vector<int>{1, 2};
     std::vector<std::array<int, 3>> configs;  // variant, depth, batch
     for (int v : variants)

@@ -57,7 +57,17 @@ typedef int32_t (*HbgProgressFn)(void* user, int32_t stage, int32_t done, int32_
  * accumulation), the solution refined in FP64 against the FP64 matrix until the normwise backward
  * error |b - A x| / (|A| |x| + |b|) (infinity norms) is at most 1e-12; a frequency whose complex64
  * factorization needs row interchanges or does not converge in 10 steps is solved in FP64. */
-enum { HBG_PRECISION_FP64 = 0, HBG_PRECISION_MIXED = 1 };
+enum { HBG_PRECISION_FP64 = 0, HBG_PRECISION_MIXED = 1, HBG_PRECISION_MIXED32 = 2 };
+/* HBG_PRECISION_MIXED32 (the precision addition; no counterpart in the reference, whose
+ * production mode, env_hooks.rs:51-56, assembles with FP32 Rankine tiers and factors in complex64
+ * without refinement): the matrices assembled in FP32 (the wave kernel from FP32 tables of lower
+ * degree, FP32 statics records), the self and adjacent pairs (centroids closer than twice the sum
+ * of the panels' radii) in FP64 (statics, wave kernel and the log correction), the complex64
+ * system factored on the matrix cores and the solution refined in FP64 against that matrix (its
+ * complex64 entries with the FP64 near entries) to the backward error 10^-10 (HBG_TUNE_MIXED32_TOL);
+ * a frequency that fails is solved in FP64 as in HBG_PRECISION_MIXED. Symmetry planes, walls, the
+ * surrogate, and frequencies with pairs beyond the FP32 tables (K R or K |z + zeta| >= 48) take
+ * HBG_PRECISION_MIXED for the whole solve. */
 
 typedef struct HbgOptions {
   int32_t precision;     /* HBG_PRECISION_FP64 or HBG_PRECISION_MIXED */
@@ -219,6 +229,14 @@ HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64
 HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
                             const double* zeta, double* out);
 
+/* The FP32 evaluators of HBG_PRECISION_MIXED32 at points, for the parity tests against
+ * hbg_wave_kernel and hbg_finite_kernel (no counterpart in the reference): infinite depth
+ * out [n][4] = Ft, Ft_X, Gi, Gi_X at 0 <= X < 48, 0 < Y < 48; finite depth out [n][8] as
+ * hbg_finite_kernel, at points inside the FP32 tables (else HBG_ERR_VALUE). */
+HbgStatus hbg_wave_kernel32(HbgSession* s, const double* X, const double* Y, int32_t n, double* out);
+HbgStatus hbg_finite_kernel32(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
+                              const double* zeta, double* out);
+
 /* Dispersion roots (dispersion.rs:19-95): out[0] = x0 (x0 tanh x0 = H), out[i] = x_i
  * (x_i tan x_i = -H, (i - 1/2) pi < x_i < i pi) for i = 1..n. */
 HbgStatus hbg_dispersion_roots(double H, int32_t n, double* out);
@@ -228,7 +246,9 @@ HbgStatus hbg_dispersion_roots(double H, int32_t n, double* out);
 HbgStatus hbg_statics(HbgSession* s, int32_t logarithmic, double log_tolerance, double* out);
 
 /* The influence matrices of one frequency (lo_frequency.rs:399-537, G = 1): S and A = 2πI - D,
- * complex [N][N] row-major (S[k][n] = influence of panel n at field point k). */
+ * complex [N][N] row-major (S[k][n] = influence of panel n at field point k). With the precision
+ * HBG_PRECISION_MIXED32, the matrices of the FP32 assembly as its solve uses them (the complex64
+ * entries, the FP64 values at the near entries). */
 HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, double* S, double* A);
 
 /* The number fields of the WAMIT files (wamit_files.rs:1718-1757 fortran_e, fortran_i): x as
@@ -296,9 +316,9 @@ HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t bat
 
 /* Kernel variants (no counterpart in the reference), per process: HBG_TUNE_LU_GEMM selects the
  * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64, and for
- * the complex64 LU of the mixed precision also 3: 64 x 64, 4: 128 x 64 on the FP16 matrix cores
- * with each FP32 operand split into two FP16 numbers (three FP16 products per FP32 product; the
- * complex128 LU takes 1 for 3 and 2 for 4); HBG_TUNE_LU_DEPTH the number of 64-column panels per
+ * the complex64 LU of the mixed precision also 3: 64 x 64, 4: 128 x 64, 5: 128 x 128 (8 waves) on
+ * the FP16 matrix cores with each FP32 operand split into two FP16 numbers (three FP16 products
+ * per FP32 product; the complex128 LU takes 1 for 3 and 2 for 4 and 5); HBG_TUNE_LU_DEPTH the number of 64-column panels per
  * trailing update (1..16, default 4). Both set the complex128 and the complex64 LU and read back
  * the complex128 one; HBG_TUNE_LU_GEMM_MIXED and HBG_TUNE_LU_DEPTH_MIXED set and read the
  * complex64 LU's (the mixed precision) alone. HBG_TUNE_LU_GRAPHS 1 (default): the batched LU's
@@ -309,12 +329,13 @@ enum {
   HBG_TUNE_LU_DEPTH = 1,
   HBG_TUNE_LU_GRAPHS = 2,
   HBG_TUNE_LU_GEMM_MIXED = 3,
-  HBG_TUNE_LU_DEPTH_MIXED = 4
+  HBG_TUNE_LU_DEPTH_MIXED = 4,
+  HBG_TUNE_MIXED32_TOL = 5 /* HBG_PRECISION_MIXED32 refines to backward error 10^-value (6..14, default 10) */
 };
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
  * systems of order n with nrhs right-hand sides in the given precision, first for every GEMM tile
- * variant (1, 2; mixed precision also 3, 4) and block depth (2, 4, 8, 12, 16) at the largest batch
+ * variant (1, 2; mixed precision also 3, 4, 5) and block depth (2, 4, 8, 12, 16) at the largest batch
  * of 4, 8, 16, 32, 48 that fits in a third of the free memory, then for the smaller batches at the
  * fastest variant and depth; keeps the fastest variant and depth (HBG_TUNE_LU_GEMM,
  * HBG_TUNE_LU_DEPTH; for the mixed precision HBG_TUNE_LU_GEMM_MIXED, HBG_TUNE_LU_DEPTH_MIXED) and

@@ -67,7 +67,7 @@ def test_mixed_matches_plain_cylinder(session):
         assert e < 1e-9
 
 
-@pytest.mark.parametrize("variant", [3, 4])
+@pytest.mark.parametrize("variant", [3, 4, 5])
 def test_mixed_split_fp16_gemm(session, variant):
     """The complex64 LU's GEMM on the FP16 matrix cores with split operands: the refined results
     stay those of the plain path, in at most two refinement steps per frequency and without an
