@@ -94,7 +94,12 @@ void launch_fin_tables32(const double* phi1, const double* phi2, const double* d
                          const double4* pf, const double2* pcs, int N, float4* pf32, float2* pcs32,
                          hipStream_t stream);
 void launch_panels32(const PanelD* P, int N, void* out, hipStream_t stream);
-void launch_statics32(const StaticD* st, size_t count, float4* out, hipStream_t stream);
+void launch_statics_list(const PanelD* P, int N, const int2* ent, int nnz, int logint, double tol, double depth,
+                         const double* gl, StaticD* st, int2* queue, unsigned* qcount, unsigned qcap,
+                         hipStream_t stream);
+size_t tier_rules_elems(int N);
+void launch_statics_far32(const PanelD* P, int N, int logint, double tol, double depth, const double* gl,
+                          float4* rules, float4* st32, hipStream_t stream);
 void launch_assemble32(const void* P32, int N, double K, double logc, const float* t2d32, const float* t1d32,
                        const float4* st32, float2* S, float2* A, float2* A2, int lda, double free_term,
                        hipStream_t stream);
@@ -477,8 +482,10 @@ struct HbgSession {
   // per frequency the FP32 tables and factors; the complex64 S, the FP64 right-hand sides and
   // solutions [system][N R], the near entries' FP64 differences (dS shared, dA per system)
   DevBuf<unsigned char> d_panels32;
-  DevBuf<float4> d_st32;
-  bool st32_valid = false;
+  DevBuf<float4> d_st32, d_rules;
+  bool st32_valid = false;  // d_st32 and the FP64 records of the near entries, for st32_logint, st32_tol
+  int st32_logint = -1;
+  double st32_tol = -1.0;
   bool near_valid = false;
   int near_nnz = 0;
   std::vector<int2> near_ent;  // host copy of the near entries (row, column)
@@ -542,7 +549,6 @@ void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false
         s->prof[11] += 1.0;
       }
       s->statics_valid = true;
-      s->st32_valid = false;
       s->stinf_valid = want_inf;
       s->st_logint = logint;
       s->st_tol = tol;
@@ -675,19 +681,55 @@ int g_mixed32_tol_exp = 10;
 
 // ---- the FP32 far-field assembly (HBG_PRECISION_MIXED32)
 
-// The FP32 panel records and statics of the geometry and its near entries (G = 1; the statics
-// current): the pairs whose centroids are closer than near_ratio() (r_k + r_n), found on a uniform
-// grid of the centroids, every row k's columns in increasing order, the diagonal included.
-void ensure_fp32_geometry(HbgSession* s) {
+void ensure_near_list(HbgSession* s);
+
+// The FP32 panel records and statics of the geometry (G = 1) and its near entries (the pairs
+// whose centroids are closer than near_ratio() (r_k + r_n)): FP32 records of every pair, by the
+// Gauss tiers beyond two panel radii (statics_far32_kernel; 0 nearer), and the FP64 records of
+// the near entries (all pairs' FP64 records when the FP64 statics are current already), which
+// the near entries of the assembly take instead. Counted as the statics in the stage timing.
+void ensure_fp32_geometry(HbgSession* s, int logint, double tol) {
   const int N = s->N;
-  if (!s->st32_valid) {
-    s->d_panels32.ensure((size_t)N * panel32_bytes());
-    launch_panels32(s->d_panels.p, N, s->d_panels32.p, s->stream);
-    s->d_st32.ensure((size_t)N * N);
-    launch_statics32(s->d_st.p, (size_t)N * N, s->d_st32.p, s->stream);
+  ensure_near_list(s);
+  if (s->st32_valid && s->st32_logint == logint && s->st32_tol == tol) return;
+  if (s->profiling) HBG_HIP(hipEventRecord(s->ev[0], s->stream));
+  s->d_panels32.ensure((size_t)N * panel32_bytes());
+  launch_panels32(s->d_panels.p, N, s->d_panels32.p, s->stream);
+  const double depth = std::isfinite(s->depth) ? s->depth : 0.0;
+  if (!(s->statics_valid && s->st_logint == logint && s->st_tol == tol)) {
+    s->d_st.ensure((size_t)N * N);
+    const unsigned qcap = (unsigned)std::max(s->near_nnz, 1);
+    s->d_queue.ensure(qcap);
+    s->d_qcount.ensure(1);
+    HBG_HIP(hipMemsetAsync(s->d_qcount.p, 0, sizeof(unsigned), s->stream));
+    launch_statics_list(s->d_panels.p, N, s->d_near_ent.p, s->near_nnz, logint, tol, depth, s->d_gl.p, s->d_st.p,
+                        s->d_queue.p, s->d_qcount.p, qcap, s->stream);
+    unsigned count = 0;
+    HBG_HIP(hipMemcpyAsync(&count, s->d_qcount.p, sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
     HBG_HIP(hipStreamSynchronize(s->stream));
-    s->st32_valid = true;
+    if (count > qcap) throw Error{HBG_ERR_INTERNAL, "statics: adaptive queue overflow at the near entries"};
+    launch_adaptive(s->d_panels.p, s->d_panels.p, N, s->d_gl.p, s->d_st.p, s->d_queue.p, count, s->stream);
   }
+  s->d_rules.ensure(tier_rules_elems(N));
+  s->d_st32.ensure((size_t)N * N);
+  launch_statics_far32(s->d_panels.p, N, logint, tol, depth, s->d_gl.p, s->d_rules.p, s->d_st32.p, s->stream);
+  if (s->profiling) HBG_HIP(hipEventRecord(s->ev[1], s->stream));
+  HBG_HIP(hipStreamSynchronize(s->stream));
+  if (s->profiling) {
+    float ms = 0.0f;
+    HBG_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+    s->prof[0] += ms;
+    s->prof[11] += 1.0;
+  }
+  s->st32_valid = true;
+  s->st32_logint = logint;
+  s->st32_tol = tol;
+}
+
+// The near entries of the geometry, found on a uniform grid of the centroids, every row k's
+// columns in increasing order, the diagonal included.
+void ensure_near_list(HbgSession* s) {
+  const int N = s->N;
   if (s->near_valid) return;
   const double ratio = near_ratio();
   double rmax = 0.0, lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -916,7 +958,7 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
   if (!(density > 0.0) || !std::isfinite(density)) return fail(HBG_ERR_VALUE, "density must be > 0", "hbg_set_environment");
   if (!(gravity > 0.0) || !std::isfinite(gravity)) return fail(HBG_ERR_VALUE, "gravity must be > 0", "hbg_set_environment");
   if (!(depth > 0.0)) return fail(HBG_ERR_VALUE, "depth must be > 0", "hbg_set_environment");
-  if (depth != s->depth) s->statics_valid = s->stf_valid = false;
+  if (depth != s->depth) s->statics_valid = s->stf_valid = s->st32_valid = false;
   s->rho = density;
   s->grav = gravity;
   s->depth = depth;
@@ -1442,17 +1484,18 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   check_options(options);
   HbgOptions o = defaults(options);
   use_device(s);
-  ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
+  const bool m32 = o.precision == HBG_PRECISION_MIXED32;
+  if (!m32) ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
   const int N = s->N;
   s->d_S.ensure((size_t)N * N);
   s->d_M.ensure((size_t)N * N);
   const double K = omega * omega / s->grav;
   check_depth(s);
-  if (o.precision == HBG_PRECISION_MIXED32) {
+  if (m32) {
     // the matrices of the FP32 assembly: its complex64 entries, the FP64 values at the near entries
     if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with the surrogate"};
     if (!fp32_in_range(s, K)) throw Error{HBG_ERR_UNSUPPORTED, "pairs beyond the FP32 tables at this frequency"};
-    ensure_fp32_geometry(s);
+    ensure_fp32_geometry(s, o.logarithmic ? 1 : 0, o.log_tolerance);
     if (std::isfinite(s->depth)) {
       const FinFreq f = make_fin_freq(s->depth, K);
       std::vector<unsigned char> f32(finfreq32_bytes());
@@ -1676,7 +1719,13 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   omegas = om.data();
   n_omegas = (int32_t)om.size();
   if (progress && progress(user, HBG_STAGE_STATICS, 0, 1) == 0) throw Error{HBG_ERR_CANCELLED, "cancelled"};
-  ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance, want_inf);
+  // the FP32 far-field assembly: one geometry without images, no surrogate, every frequency's
+  // pairs inside the FP32 tables; otherwise HBG_PRECISION_MIXED. It takes its own statics (FP32
+  // records, FP64 at the near entries); the FP64 statics of every pair only for the limits.
+  bool m32 = o.precision == HBG_PRECISION_MIXED32 && s->G == 1 && !o.surrogate;
+  for (int i = 0; m32 && i < n_omegas; ++i) m32 = fp32_in_range(s, omegas[i] * omegas[i] / s->grav);
+  if (!m32 || !il.empty()) ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance, want_inf);
+  if (m32) ensure_fp32_geometry(s, o.logarithmic ? 1 : 0, o.log_tolerance);
   if (s->np > 0) ensure_field_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
   std::vector<double> heads(std::max(H, 1), 0.0);
   for (int h = 0; h < H; ++h) heads[h] = headings_deg[h] * PI / 180.0;
@@ -1696,15 +1745,10 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // its forces are taken; S, the right-hand-side partials and the finite-depth tables are shared
   const size_t mstride = (size_t)N * (N + R);
   const size_t hstride = (size_t)std::max(H, 1) * NF;
-  // the FP32 far-field assembly: one geometry without images, no surrogate, every frequency's
-  // pairs inside the FP32 tables; otherwise HBG_PRECISION_MIXED
-  bool m32 = o.precision == HBG_PRECISION_MIXED32 && s->G == 1 && !o.surrogate;
-  for (int i = 0; m32 && i < n_omegas; ++i) m32 = fp32_in_range(s, omegas[i] * omegas[i] / s->grav);
   const bool mixed = o.precision == HBG_PRECISION_MIXED || (o.precision == HBG_PRECISION_MIXED32 && !m32);
   const bool prof = s->profiling;
   const double m32_tol = std::pow(10.0, -(double)g_mixed32_tol_exp);
   if (m32) {
-    ensure_fp32_geometry(s);
     if (std::isfinite(s->depth) && n_omegas > 0) {
       std::vector<unsigned char> f32((size_t)n_omegas * finfreq32_bytes());
       for (int i = 0; i < n_omegas; ++i)
@@ -1843,6 +1887,9 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // system, its LU (pivoted where needed), the solution into the slot's right-hand sides
   auto fallback64 = [&](int fi, int b, int h) {
     const double w = omegas[fi], K = w * w / s->grav;
+    // the FP64 statics of every pair (the FP32 assembly computed them at its near entries only),
+    // on the session stream after the assembly work queued there
+    ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance);
     const double kw = assemble_frequency(s, K, logc, s->d_M.p, false, nullptr, N, mstride,
                                          std::isfinite(s->depth) ? s->d_ffs.p + fi : nullptr);
     launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, w, kw, s->grav, depth, s->d_heads.p,

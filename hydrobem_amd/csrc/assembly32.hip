@@ -150,13 +150,6 @@ __global__ void panels32_kernel(const PanelD* __restrict__ P, int N, PanelF* __r
   out[j] = q;
 }
 
-__global__ void statics32_kernel(const StaticD* __restrict__ st, size_t count, float4* __restrict__ out) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
-    const StaticD e = st[i];
-    out[i] = make_float4((float)e.rs, (float)e.rd, (float)e.dl, (float)e.dd);
-  }
-}
-
 __global__ void factors32_kernel(const double4* __restrict__ pf, const double2* __restrict__ pcs, int N,
                                  float4* __restrict__ pf32, float2* __restrict__ pcs32) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -720,11 +713,6 @@ void launch_fin_tables32(const double* phi1, const double* phi2, const double* d
 
 void launch_panels32(const PanelD* P, int N, void* out, hipStream_t stream) {
   panels32_kernel<<<(N + 255) / 256, 256, 0, stream>>>(P, N, (PanelF*)out);
-  HBG_HIP(hipGetLastError());
-}
-
-void launch_statics32(const StaticD* st, size_t count, float4* out, hipStream_t stream) {
-  statics32_kernel<<<(unsigned)std::min<size_t>((count + 255) / 256, 8192), 256, 0, stream>>>(st, count, out);
   HBG_HIP(hipGetLastError());
 }
 

@@ -135,30 +135,20 @@ __device__ __forceinline__ bool log_pair_d(const PanelD* __restrict__ p, P3 x, d
   return d * d * tol < 0.6 * p->radius * p->radius;
 }
 
-}  // namespace
-
-// gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]. inf (optional, infinite depth):
-// the omega = infinity row of the pair, the integrals of 1/r - 1/r' (lo_static.rs:89-96), at
-// inf[j * N + k] as (source, dipole).
-__global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, const PanelD* __restrict__ Pf,
-                                                      int N, int Nsrc, int logint,
-                                                      double tol, double depth, const double* __restrict__ gl,
-                                                      StaticD* __restrict__ st, int2* __restrict__ queue,
-                                                      unsigned* __restrict__ qcount, unsigned qcap,
-                                                      double2* __restrict__ inf) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
-  if (j >= Nsrc || k >= N) return;
-  const PanelD* p = P + j;
-  const P3 x = ld3(Pf[k].c);
+// The record of the pair (field point x, panel p): returns whether its log corrections need the
+// adaptive rule (then dl, dd are left 0 for adaptive_kernel). sinf, dinf: the integrals of
+// 1/r - 1/r' (the omega = infinity row, infinite depth).
+__device__ __forceinline__ bool pair_statics(const PanelD* __restrict__ p, P3 x, int logint, double tol, double depth,
+                                             const double* __restrict__ gl, StaticD& e, double& sinf,
+                                             double& dinf) {
   double s0, d0, sb, db;
   rankine_d(p, x, gl, s0, d0);
   const P3 xb = mk(x.x, x.y, -x.z);
   rankine_d(p, xb, gl, sb, db);
-  StaticD e;
   e.rs = s0 + sb;
   e.rd = d0 + db;
-  if (inf) inf[(size_t)j * N + k] = make_double2(s0 - sb, d0 - db);
+  sinf = s0 - sb;
+  dinf = d0 - db;
   if (depth > 0.0) {
     // finite depth: the bottom images of lo_static.rs:29-38 (x, y, -z - 2h), (x, y, z + 2h),
     // (x, y, z - 2h), (x, y, -z - 4h), summed in the reference's order
@@ -196,11 +186,279 @@ __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__
       e.dl = L - p->area * psi;
       e.dd = Ld - p->area * dpsi;
     } else {
-      const unsigned slot = atomicAdd(qcount, 1u);
-      if (slot < qcap) queue[slot] = make_int2(k, j);
+      return true;
     }
   }
+  return false;
+}
+
+__device__ __forceinline__ void queue_pair(int2* __restrict__ queue, unsigned* __restrict__ qcount, unsigned qcap,
+                                           int k, int j) {
+  const unsigned slot = atomicAdd(qcount, 1u);
+  if (slot < qcap) queue[slot] = make_int2(k, j);
+}
+
+}  // namespace
+
+// gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]. inf (optional, infinite depth):
+// the omega = infinity row of the pair, the integrals of 1/r - 1/r' (lo_static.rs:89-96), at
+// inf[j * N + k] as (source, dipole).
+__global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, const PanelD* __restrict__ Pf,
+                                                      int N, int Nsrc, int logint,
+                                                      double tol, double depth, const double* __restrict__ gl,
+                                                      StaticD* __restrict__ st, int2* __restrict__ queue,
+                                                      unsigned* __restrict__ qcount, unsigned qcap,
+                                                      double2* __restrict__ inf) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  if (j >= Nsrc || k >= N) return;
+  StaticD e;
+  double sinf, dinf;
+  if (pair_statics(P + j, ld3(Pf[k].c), logint, tol, depth, gl, e, sinf, dinf)) queue_pair(queue, qcount, qcap, k, j);
+  if (inf) inf[(size_t)j * N + k] = make_double2(sinf, dinf);
   st[(size_t)j * N + k] = e;
+}
+
+// The records of the listed pairs (field point ent[e].x, panel ent[e].y) only, at their places
+// st[j * N + k]: the near entries of the FP32 assembly, whose other pairs take FP32 records.
+__global__ void __launch_bounds__(256) statics_list_kernel(const PanelD* __restrict__ P, int N,
+                                                           const int2* __restrict__ ent, int nnz, int logint,
+                                                           double tol, double depth, const double* __restrict__ gl,
+                                                           StaticD* __restrict__ st, int2* __restrict__ queue,
+                                                           unsigned* __restrict__ qcount, unsigned qcap) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const int k = ent[e].x, j = ent[e].y;
+  StaticD r;
+  double sinf, dinf;
+  if (pair_statics(P + j, ld3(P[k].c), logint, tol, depth, gl, r, sinf, dinf)) queue_pair(queue, qcount, qcap, k, j);
+  st[(size_t)j * N + k] = r;
+}
+
+// ------------------------------------------------------------------ FP32 statics
+// The FP32 records (rs, rd, dl, dd) of the precision path with the FP32 far-field assembly, for
+// the pairs whose field point lies at least two radii from the panel (every image point then lies
+// farther still: |z + zeta| >= |z - zeta| for z, zeta <= 0, and likewise about the sea floor):
+// - the Rankine integrals of the field point and each image by Gauss rules on the bilinear map of
+//   the corners, the float32 tiers of the reference's production path (panel_integrals.rs:288-306:
+//   orders 6, 5, 4, 3 from 2, 3, 4, 8 radii; 1e-6 relative at each tier's inner boundary), the
+//   points taken relative to the centroid in FP64 first and the weights rescaled to the area
+//   (panel_integrals.rs:185-214), so that no FP32 difference cancels;
+// - the log corrections by the Gauss rules of log_order on the map of the distinct vertices.
+// A wavefront takes 64 field points of one panel and evaluates every lane with the highest order
+// any of its lanes needs (an order above a pair's tier only adds accuracy), so that the rule
+// points are wavefront-uniform scalar loads. The pairs nearer than two radii are written 0: they
+// are near entries of the assembly, whose FP64 records replace them.
+constexpr int TIER_RANKINE = 9 + 16 + 25 + 36;  // orders 3, 4, 5, 6
+constexpr int TIER_LOG = 16 + 25 + 36;          // orders 4, 5, 6
+constexpr int TIER_STRIDE = TIER_RANKINE + TIER_LOG + 1;  // float4 per panel (padded to 32 bytes)
+__host__ __device__ constexpr int tier_rankine_off(int order) {
+  return order == 3 ? 0 : order == 4 ? 9 : order == 5 ? 25 : 50;
+}
+__host__ __device__ constexpr int tier_log_off(int order) {
+  return TIER_RANKINE + (order == 4 ? 0 : order == 5 ? 16 : 41);
+}
+
+// rules[j][TIER_STRIDE]: (x - c, y - c, z - c, w) of every rule point of panel j; one workgroup
+// per panel, one thread per point
+__global__ void __launch_bounds__(192) tier_rules_kernel(const PanelD* __restrict__ P, int N,
+                                                         const double* __restrict__ gl, float4* __restrict__ rules) {
+  const int j = blockIdx.x, t = threadIdx.x;
+  if (j >= N || t >= TIER_STRIDE) return;
+  const PanelD* p = P + j;
+  float4* out = rules + (size_t)j * TIER_STRIDE;
+  if (t == TIER_STRIDE - 1) {
+    out[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const P3 c = ld3(p->c);
+  const bool rank = t < TIER_RANKINE;
+  int order, q;
+  if (rank) {
+    order = t < 9 ? 3 : t < 25 ? 4 : t < 50 ? 5 : 6;
+    q = t - tier_rankine_off(order);
+  } else {
+    const int u = t - TIER_RANKINE;
+    order = u < 16 ? 4 : u < 41 ? 5 : 6;
+    q = t - tier_log_off(order);
+  }
+  const double* xs = gl + order * 32;
+  const double* ws = xs + 16;
+  const double(*map)[3] = rank ? p->q4 : p->v;
+  P3 x;
+  double jac;
+  bilinear(map, xs[q / order], xs[q % order], x, jac);
+  double w = ws[q / order] * ws[q % order] * jac;
+  if (rank) {
+    double sum = 0.0;
+    for (int a = 0; a < order; ++a)
+      for (int b = 0; b < order; ++b) {
+        P3 y;
+        double jb;
+        bilinear(map, xs[a], xs[b], y, jb);
+        sum += ws[a] * ws[b] * jb;
+      }
+    w *= p->area / sum;
+  }
+  out[t] = make_float4((float)(x.x - c.x), (float)(x.y - c.y), (float)(x.z - c.z), (float)w);
+}
+
+namespace {
+
+// the highest of the lanes' orders (0 where no lane is active)
+__device__ __forceinline__ int wave_max_order(int o) {
+  int m = 0;
+  for (int v = 3; v <= 6; ++v)
+    if (__ballot(o == v) != 0) m = v;
+  return m;
+}
+
+// Rankine source and dipole of the tier rule of ORDER of a panel at d = x - c (unrolled, so that
+// the rule's scalar loads go several at a time)
+template <int ORDER>
+__device__ __forceinline__ void rankine_tier32_o(const float4* __restrict__ R, float dx, float dy, float dz, float nx,
+                                                 float ny, float nz, float& S, float& D) {
+  const float4* g = R + tier_rankine_off(ORDER);
+  float s = 0.0f, d = 0.0f;
+#pragma unroll
+  for (int q = 0; q < ORDER * ORDER; ++q) {
+    const float4 a = g[q];
+    const float rx = dx - a.x, ry = dy - a.y, rz = dz - a.z;
+    const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float nr = fmaf(nx, rx, fmaf(ny, ry, nz * rz));
+    s = fmaf(a.w, ir, s);
+    d = fmaf(a.w * nr, ir * ir * ir, d);
+  }
+  S = s;
+  D = d;
+}
+
+__device__ __forceinline__ void rankine_tier32(const float4* __restrict__ R, int order, float dx, float dy, float dz,
+                                               float nx, float ny, float nz, float& S, float& D) {
+  switch (order) {
+    case 3: rankine_tier32_o<3>(R, dx, dy, dz, nx, ny, nz, S, D); break;
+    case 4: rankine_tier32_o<4>(R, dx, dy, dz, nx, ny, nz, S, D); break;
+    case 5: rankine_tier32_o<5>(R, dx, dy, dz, nx, ny, nz, S, D); break;
+    default: rankine_tier32_o<6>(R, dx, dy, dz, nx, ny, nz, S, D); break;
+  }
+}
+
+// The log corrections dl = int psi dS - A psi(c) and dd = int d psi / dn dS - A dpsi(c) by the
+// log rule of ORDER of a panel, the image at e = image - c, in difference form: the sums of
+// w (psi_i - psi_c) and w (dpsi_i - dpsi_c) (the rule's weights sum to the area), each difference
+// computed without cancellation from the rule point a = xi - c:
+//   R_i - R_c = (a.a - 2 a.e) / (R_i + R_c),  den_i - den_c = (R_i - R_c) - a_z  (den = R - d_z),
+//   psi_i - psi_c = log1p((den_i - den_c) / den_c),
+//   dpsi = g / den with g = d.n / R - n_z:  g_i - g_c = a.n / R_i + (e.n) (R_i - R_c) / (R_i R_c),
+//   dpsi_i - dpsi_c = ((g_i - g_c) den_c - g_c (den_i - den_c)) / (den_i den_c),
+// so that FP32 keeps the corrections to their own precision (the direct FP32 difference of the two
+// terms loses it to cancellation where the corrections are small beside A psi(c)).
+template <int ORDER>
+__device__ __forceinline__ void log_tier32_o(const float4* __restrict__ R, float ex, float ey, float ez, float nx,
+                                             float ny, float nz, float& dl, float& dd) {
+  const float4* g = R + tier_log_off(ORDER);
+  const float Rc = __builtin_amdgcn_sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez)));
+  const float denc = Rc + ez;  // at the centroid d = -e, den = R_c + e_z
+  const float idenc = __builtin_amdgcn_rcpf(denc), iRc = __builtin_amdgcn_rcpf(Rc);
+  const float en = fmaf(nx, ex, fmaf(ny, ey, nz * ez));
+  const float gc = -en * iRc - nz;
+  float l = 0.0f, ld = 0.0f;
+#pragma unroll
+  for (int q = 0; q < ORDER * ORDER; ++q) {
+    const float4 a = g[q];
+    const float rx = a.x - ex, ry = a.y - ey, rz = a.z - ez;  // xi - image, rz <= 0
+    const float Ri = __builtin_amdgcn_sqrtf(fmaf(rx, rx, fmaf(ry, ry, rz * rz)));
+    const float iRi = __builtin_amdgcn_rcpf(Ri);
+    const float den = Ri - rz;
+    const float aa = fmaf(a.x, a.x - 2.0f * ex, fmaf(a.y, a.y - 2.0f * ey, a.z * (a.z - 2.0f * ez)));
+    const float dR = aa * __builtin_amdgcn_rcpf(Ri + Rc);
+    const float delta = dR - a.z;
+    l = fmaf(a.w, log1pf(delta * idenc), l);
+    const float gg = fmaf(fmaf(nx, a.x, fmaf(ny, a.y, nz * a.z)), iRi, en * dR * iRi * iRc);
+    ld = fmaf(a.w, fmaf(gg, denc, -gc * delta) * __builtin_amdgcn_rcpf(den * denc), ld);
+  }
+  dl = l;
+  dd = ld;
+}
+
+__device__ __forceinline__ int rankine_tier_order(float r2) {  // r2: (distance / radius)^2
+  return r2 >= 64.0f ? 3 : r2 >= 16.0f ? 4 : r2 >= 9.0f ? 5 : 6;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) statics_far32_kernel(const PanelD* __restrict__ P, int N, int logint,
+                                                            double tol, double depth,
+                                                            const float4* __restrict__ rules,
+                                                            float4* __restrict__ st32) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + threadIdx.y);
+  if (j >= N) return;  // wavefront-uniform: every lane below takes part in the ballots
+  const bool live = k < N;
+  const PanelD* p = P + j;
+  const P3 x = ld3(P[live ? k : 0].c);
+  const P3 c = ld3(p->c);
+  const double rad = p->radius;
+  const P3 d0 = sub(x, c);
+  const bool act = live && dot3(d0, d0) >= 4.0 * rad * rad;
+  const float4* R = rules + (size_t)j * TIER_STRIDE;
+  const float nx = (float)p->n[0], ny = (float)p->n[1], nz = (float)p->n[2];
+  const float dx = (float)d0.x, dy = (float)d0.y;
+  const float irad2 = (float)(1.0 / (rad * rad));
+  // the image points' z - c_z: the field point, its free-surface image and, in finite depth, the
+  // bottom images of lo_static.rs:29-38 in the reference's order
+  const int nim = depth > 0.0 ? 6 : 2;
+  // sums in the reference's order: (s0 + sb) + (((s2 + s4) + s5) + s6)
+  float sa = 0.0f, da = 0.0f, sbt = 0.0f, dbt = 0.0f;
+#pragma unroll 1
+  for (int m = 0; m < nim; ++m) {
+    const double h = depth;
+    const double zi = m == 0 ? x.z : m == 1 ? -x.z : m == 2 ? -x.z - 2.0 * h : m == 3 ? x.z + 2.0 * h
+                    : m == 4 ? x.z - 2.0 * h : -x.z - 4.0 * h;
+    const float dz = (float)(zi - c.z);
+    const float r2 = (fmaf(dx, dx, fmaf(dy, dy, dz * dz))) * irad2;
+    const int order = wave_max_order(act ? rankine_tier_order(r2) : 0);
+    float si = 0.0f, di = 0.0f;
+    if (order > 0) rankine_tier32(R, order, dx, dy, dz, nx, ny, nz, si, di);
+    if (m < 2) {
+      sa += si;
+      da += di;
+    } else {
+      sbt += si;
+      dbt += di;
+    }
+  }
+  float4 e;
+  e.x = nim == 6 ? sa + sbt : sa;
+  e.y = nim == 6 ? da + dbt : da;
+  e.z = 0.0f;
+  e.w = 0.0f;
+  // the log corrections dl = int psi dS - A psi(c), dd likewise for d psi / dn, psi at the image
+  // (x, y, -z); beyond two radii log_order takes 6, 5 or 4 points a side (6 from two radii up)
+  const P3 xb = mk(x.x, x.y, -x.z);
+  const bool lg = act && logint && log_pair_d(p, x, tol);
+  int lo = 0;
+  if (lg) {
+    const double r = nrm(sub(xb, c)) / rad;
+    lo = r >= 6.0 ? 4 : r >= 3.0 ? 5 : 6;
+  }
+  const int order = wave_max_order(lo);
+  if (order > 0) {
+    const float ex = dx, ey = dy, ez = (float)(xb.z - c.z);  // the image relative to the centroid
+    float dl = 0.0f, dd = 0.0f;
+    switch (order) {
+      case 4: log_tier32_o<4>(R, ex, ey, ez, nx, ny, nz, dl, dd); break;
+      case 5: log_tier32_o<5>(R, ex, ey, ez, nx, ny, nz, dl, dd); break;
+      default: log_tier32_o<6>(R, ex, ey, ez, nx, ny, nz, dl, dd); break;
+    }
+    if (lg) {
+      e.z = dl;
+      e.w = dd;
+    }
+  }
+  if (!act) e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (live) st32[(size_t)j * N + k] = e;
 }
 
 // One wavefront per queued pair: the adaptive 4 x 4 Gauss rule of panel_integrals.rs:69-140 /
@@ -327,6 +585,27 @@ void launch_adaptive(const PanelD* P, const PanelD* Pf, int N, const double* gl,
                      unsigned count, hipStream_t stream) {
   if (count == 0) return;
   adaptive_kernel<<<count, 64, 0, stream>>>(P, Pf, N, gl, st, queue, count);
+  HBG_HIP(hipGetLastError());
+}
+
+void launch_statics_list(const PanelD* P, int N, const int2* ent, int nnz, int logint, double tol, double depth,
+                         const double* gl, StaticD* st, int2* queue, unsigned* qcount, unsigned qcap,
+                         hipStream_t stream) {
+  if (nnz <= 0) return;
+  statics_list_kernel<<<(nnz + 255) / 256, 256, 0, stream>>>(P, N, ent, nnz, logint, tol, depth, gl, st, queue, qcount,
+                                                             qcap);
+  HBG_HIP(hipGetLastError());
+}
+
+size_t tier_rules_elems(int N) { return (size_t)N * TIER_STRIDE; }
+
+// rules: tier_rules_elems(N) float4 of workspace; st32[j * N + k] for every pair
+void launch_statics_far32(const PanelD* P, int N, int logint, double tol, double depth, const double* gl,
+                          float4* rules, float4* st32, hipStream_t stream) {
+  tier_rules_kernel<<<N, 192, 0, stream>>>(P, N, gl, rules);
+  HBG_HIP(hipGetLastError());
+  statics_far32_kernel<<<dim3((N + 63) / 64, (N + 3) / 4), dim3(64, 4), 0, stream>>>(P, N, logint, tol, depth, rules,
+                                                                                     st32);
   HBG_HIP(hipGetLastError());
 }
 
