@@ -37,7 +37,9 @@ PEAK_FP16 = 2.5e15    # MI355X dense FP16 matrix, FLOP/s
 VARIANTS = {1: ("64x64", "gemm_mfma_kernel<T, 64, 64>", None), 2: ("128x64", "gemm_mfma_kernel<T, 128, 64>", None),
             3: ("64x64 split-fp16", "gemm_f16s_kernel<64, 64>", "f16x3"),
             4: ("128x64 split-fp16", "gemm_f16s_kernel<128, 64>", "f16x3"),
-            5: ("128x128 split-fp16", "gemm_f16s_kernel<128, 128, 2, 4>", "f16x3")}
+            5: ("128x128 split-fp16", "gemm_f16s_kernel<128, 128, 2, 4>", "f16x3"),
+            6: ("128x128 split-fp16 direct-LDS", "gemm_f16s_big_kernel<false>", "f16x3"),
+            7: ("128x128 split-fp16 direct-LDS double-buffered", "gemm_f16s_big_kernel<true>", "f16x3")}
 # HBM bytes per launch of the path's GEMM kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
 # calibrated by a device copy of known size (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md):
 # (workload, path, GEMM tiles, panels per trailing update, batch) -> bytes

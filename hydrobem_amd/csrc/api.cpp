@@ -2082,6 +2082,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                                                 : sizeof(StaticD) + 2 * sizeof(double2));
       s->prof[12] += (double)lus.fallback_panels;
       s->prof[16] += lus.refine_ms;
+      s->prof[17] = std::max(s->prof[17], lus.first_eta);
     }
     for (size_t i = 0; i < nb * nout; ++i)
       if (!std::isfinite(hout[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
@@ -2528,7 +2529,7 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
   if (!s) return fail(HBG_ERR_ARGUMENT, "session is null", "hbg_set_tuning");
   switch (key) {
     case HBG_TUNE_LU_GEMM:
-      if (value < 1 || value > 5) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..5", "hbg_set_tuning");
+      if (value < 1 || value > 7) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..7", "hbg_set_tuning");
       lu_set_gemm_variant(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH:
@@ -2540,7 +2541,7 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       lu_set_graphs(value);
       return HBG_OK;
     case HBG_TUNE_LU_GEMM_MIXED:
-      if (value < 1 || value > 5) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..5", "hbg_set_tuning");
+      if (value < 1 || value > 7) return fail(HBG_ERR_VALUE, "LU GEMM variant must be 1..7", "hbg_set_tuning");
       lu_set_gemm_variant32(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH_MIXED:

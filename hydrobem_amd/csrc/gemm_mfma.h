@@ -359,4 +359,143 @@ __global__ void __launch_bounds__(64 * WR * WC) gemm_f16s_kernel(float2* __restr
       }
 }
 
+// The same update on 128 x 128 tiles, each of the 4 waves (2 x 2) 64 x 64 (4 x 4 blocks of
+// 16 x 16): every operand fragment read from LDS feeds four blocks' MFMAs, half the LDS bytes per
+// MFMA of the 32 x 32 wave tiles above, and each tile reads half the operand bytes per flop from
+// L2. The operands go from the split buffers into LDS by the direct loads of gfx950
+// (global_load_lds_dwordx4: lane l's 16 bytes land at the wavefront-uniform LDS base + 16 l), with
+// no staging through registers, in the split buffers' own order: LDS As[part][k group][row of the
+// tile] (and Bs by columns), each (part, k group) a contiguous run of 128 h8 filled by two
+// wavefront loads. An MFMA operand read is then 16 lanes x 16 B contiguous per k group, distinct
+// banks. DB false: one buffer (64 KB for 128 x 128), two workgroups per CU alternate loading and
+// multiplying; DB true: two buffers (one workgroup per CU), the next chunk loading while the
+// current one is multiplied, one barrier per chunk. The 128 x 64 and 64 x 128 tiles (waves
+// 64 x 32, 32 x 64) take the LU panels' 64-wide strips. Rows and columns past the matrix read the
+// zero padding of the split buffers (pad >= 128 past the last row or column a tile reaches). Grid
+// gemm_grid(ceil(nrows / TM), ceil(ncolsC / TN), batch), 256 threads; kb a multiple of 32.
+constexpr int BIG_T = 128;
+
+// the (TM + TN) / 4 wavefront loads of chunk kc into LDS (A at As: TM rows, B at Bs: TN columns),
+// wave w issuing (TM + TN) / 16 of them: load i, A for i < 16 TM / 64: (part, k group) i / (TM / 64),
+// rows 64 (i % (TM / 64)) ..; B likewise after it
+template <int TM, int TN>
+__device__ __forceinline__ void big_load(const h8* __restrict__ Ls, const h8* __restrict__ Us, size_t pad, int kg,
+                                         int rowA, int colB, h8* As, h8* Bs, int wave, int lane) {
+  constexpr int LA = 16 * TM / 64, PER = (16 * (TM + TN) / 64) / 4;
+#pragma unroll
+  for (int s = 0; s < PER; ++s) {
+    const int i = PER * wave + s;
+    const bool opb = i >= LA;
+    const int j = opb ? i - LA : i, per = opb ? TN / 64 : TM / 64;
+    const int pk = j / per, part64 = j % per;
+    const int p = pk >> 2, g = pk & 3;
+    const h8* src = (opb ? Us : Ls) + ((size_t)(kg + g) * 4 + p) * pad + (opb ? colB : rowA) + 64 * part64 + lane;
+    h8* dst = (opb ? Bs + (p * 4 + g) * TN : As + (p * 4 + g) * TM) + 64 * part64;
+    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+  }
+}
+
+// the chunk's products from LDS into the wave's BR x BC blocks of 16 x 16
+template <int TM, int TN, int BR, int BC>
+__device__ __forceinline__ void big_mma(const h8* As, const h8* Bs, int wr, int wc, int kq, int l16,
+                                        f4v (&are)[BC][BR], f4v (&aim)[BC][BR]) {
+  h8 a[BR][4];  // [block row][part]
+#pragma unroll
+  for (int r = 0; r < BR; ++r)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) a[r][p] = As[(p * 4 + kq) * TM + wr + 16 * r + l16];
+#pragma unroll
+  for (int c = 0; c < BC; ++c) {
+    h8 b[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) b[p] = Bs[(p * 4 + kq) * TN + wc + 16 * c + l16];
+    // re += Br Ar - Bi Ai, im += Br Ai + Bi Ar; each real product hi hi' + hi lo' + lo hi'
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+      are[c][r] = mma16(b[0], a[r][0], are[c][r]);
+      aim[c][r] = mma16(b[0], a[r][2], aim[c][r]);
+      are[c][r] = mma16(b[0], a[r][1], are[c][r]);
+      aim[c][r] = mma16(b[0], a[r][3], aim[c][r]);
+      are[c][r] = mma16(b[1], a[r][0], are[c][r]);
+      aim[c][r] = mma16(b[1], a[r][2], aim[c][r]);
+      are[c][r] = mma16(-b[2], a[r][2], are[c][r]);
+      aim[c][r] = mma16(b[2], a[r][0], aim[c][r]);
+      are[c][r] = mma16(-b[2], a[r][3], are[c][r]);
+      aim[c][r] = mma16(b[2], a[r][1], aim[c][r]);
+      are[c][r] = mma16(-b[3], a[r][2], are[c][r]);
+      aim[c][r] = mma16(b[3], a[r][0], aim[c][r]);
+    }
+  }
+}
+
+// waves per SIMD the tiles' registers leave room for
+template <int TM, int TN>
+constexpr int big_waves() { return TM * TN >= 64 * 128 ? 2 : 4; }
+
+template <int TM, int TN, bool DB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DB ? 1 : big_waves<TM, TN>())))
+gemm_f16s_big_kernel(float2* __restrict__ M, int lda, int r0, int nrows, int c0, int ncolsC, SplitOps so, int kg0,
+                     int kb, GemmBatch gb) {
+  constexpr int NBUF = DB ? 2 : 1, BR = TM / 32, BC = TN / 32;
+  constexpr int CA = 16 * TM, CB = 16 * TN;  // h8 per chunk: [part][k group][row or column]
+  __shared__ __attribute__((aligned(16))) h8 As[NBUF * CA];
+  __shared__ __attribute__((aligned(16))) h8 Bs[NBUF * CB];
+  const GemmTile tile = gemm_tile((nrows + TM - 1) / TM, (ncolsC + TN - 1) / TN, gb.batch);
+  if (!tile.ok || gemm_tile_padding(gb, tile.b, r0 + tile.row * TM, c0 + tile.col * TN, TN, so.K0 + 8 * kg0)) return;
+  M += tile.b * gb.m;
+  const h8* __restrict__ Ls = so.ls + tile.b * so.lstride;
+  const h8* __restrict__ Us = so.us + tile.b * so.ustride;
+  const size_t pad = so.pad;
+  // the wave index as a wavefront-uniform value: the loads' operand, part and k group are then
+  // scalar arithmetic
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = (wave & 1) * (TM / 2), wc = (wave >> 1) * (TN / 2);
+  const int tr = tile.row * TM, tc = tile.col * TN;
+  const int rowA = r0 + tr, colB = c0 + tc;
+  f4v are[BC][BR], aim[BC][BR];  // [block column][block row]
+#pragma unroll
+  for (int c = 0; c < BC; ++c)
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+      are[c][r] = f4v{0.f, 0.f, 0.f, 0.f};
+      aim[c][r] = f4v{0.f, 0.f, 0.f, 0.f};
+    }
+  const int kq = lane >> 4, l16 = lane & 15;
+  if constexpr (DB) {
+    big_load<TM, TN>(Ls, Us, pad, kg0, rowA, colB, As, Bs, wave, lane);
+    for (int kc = 0, buf = 0; kc < kb; kc += H16_KC, buf ^= 1) {
+      __builtin_amdgcn_s_waitcnt(0);  // this chunk's loads (issued by every wave)
+      __syncthreads();                // ... visible, and the other buffer no longer read
+      if (kc + H16_KC < kb)
+        big_load<TM, TN>(Ls, Us, pad, kg0 + (kc + H16_KC) / 8, rowA, colB, As + (buf ^ 1) * CA, Bs + (buf ^ 1) * CB,
+                         wave, lane);
+      big_mma<TM, TN, BR, BC>(As + buf * CA, Bs + buf * CB, wr, wc, kq, l16, are, aim);
+    }
+  } else {
+    for (int kc = 0; kc < kb; kc += H16_KC) {
+      big_load<TM, TN>(Ls, Us, pad, kg0 + kc / 8, rowA, colB, As, Bs, wave, lane);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      big_mma<TM, TN, BR, BC>(As, Bs, wr, wc, kq, l16, are, aim);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < BC; ++c)
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = tc + wc + c * 16 + 4 * kq + q;
+        const int row = tr + wr + r * 16 + l16;
+        if (col < ncolsC && row < nrows) {
+          float2* p = M + (size_t)(c0 + col) * lda + r0 + row;
+          float2 v = *p;
+          v.x -= are[c][r][q];
+          v.y -= aim[c][r][q];
+          *p = v;
+        }
+      }
+}
+
 }  // namespace hbg

@@ -61,6 +61,7 @@ struct WaveTables {
 struct LuStats {
   double total_ms = 0.0, gemm_ms = 0.0, gemm_flops = 0.0;
   double refine_ms = 0.0;  // mixed precision: residuals, corrections and the FP64 store
+  double first_eta = 0.0;  // mixed precision: the largest backward error of a first solution
   long gemm_launches = 0;
   long fallback_panels = 0;
 };

@@ -1024,7 +1024,9 @@ struct GemmTimer {
 
 // One launch of the trailing-update GEMM in the given tile variant: 1 64 x 64, 2 128 x 64 (FP32 or
 // FP64 matrix cores); complex64 only: 3 64 x 64, 4 128 x 64, 5 128 x 128 (8 waves) on the FP16
-// matrix cores with split operands (gemm_f16s_kernel).
+// matrix cores with split operands (gemm_f16s_kernel), 6 the 128 x 128 tiles of
+// gemm_f16s_big_kernel where both sides exceed 64 (the trailing updates) and 64 x 64 for the
+// panels' 64-wide strips, 7 the same with the double-buffered big kernel.
 template <typename T>
 void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int ncolsC, int kf,
                  int kk, const GemmBatch& gb, const SplitOps* so, hipStream_t stream) {
@@ -1040,6 +1042,23 @@ void gemm_launch(int variant, typename Cplx<T>::type* M, int lda, int r0, int nr
         gemm_f16s_kernel<64, 64><<<g64, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
       else if (variant == 4)
         gemm_f16s_kernel<128, 64><<<g128, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
+      else if (variant >= 6) {
+        // the panels' 64-wide strips on 128 x 64 or 64 x 128 tiles, the trailing updates on 128 x 128
+        const bool db = variant == 7;
+        const int tm = nrows <= 64 ? 64 : 128, tn = ncolsC <= 64 ? 64 : 128;
+        const unsigned g = gemm_grid((nrows + tm - 1) / tm, (ncolsC + tn - 1) / tn, batch);
+        if (tm == 64 && tn == 64)
+          gemm_f16s_kernel<64, 64><<<g64, 256, 0, stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
+        else if (tm == 64)
+          (db ? gemm_f16s_big_kernel<64, 128, true> : gemm_f16s_big_kernel<64, 128, false>)<<<g, 256, 0, stream>>>(
+              M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
+        else if (tn == 64)
+          (db ? gemm_f16s_big_kernel<128, 64, true> : gemm_f16s_big_kernel<128, 64, false>)<<<g, 256, 0, stream>>>(
+              M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
+        else
+          (db ? gemm_f16s_big_kernel<128, 128, true> : gemm_f16s_big_kernel<128, 128, false>)<<<g, 256, 0, stream>>>(
+              M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
+      }
       else
         gemm_f16s_kernel<128, 128, 2, 4><<<gemm_grid((nrows + 127) / 128, (ncolsC + 127) / 128, batch), 512, 0,
                                            stream>>>(M, lda, r0, nrows, c0, ncolsC, *so, kg0, kk, gb);
@@ -1364,7 +1383,7 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   *rows = 0;
   std::vector<int> status(batches.back());
   try {
-    const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4, 5}
+    const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4, 5, 6, 7}
                                                                       : std::vector<int>{1, 2};
     std::vector<std::array<int, 3>> configs;  // variant, depth, batch
     for (int v : variants)

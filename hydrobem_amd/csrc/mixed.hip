@@ -367,6 +367,7 @@ void mixed_core(const MT* Amat, size_t astride, double2* B, size_t bstride, Near
     for (int b = 0; b < batch; ++b) {
       if (!act[b]) continue;
       eta[b] = he[b];
+      if (stats && iters[b] == 0) stats->first_eta = std::max(stats->first_eta, eta[b] == eta[b] ? eta[b] : INFINITY);
       if (!(eta[b] <= tol) && iters[b] < max_iter)
         open = true;
       else

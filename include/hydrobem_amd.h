@@ -316,9 +316,12 @@ HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t bat
 
 /* Kernel variants (no counterpart in the reference), per process: HBG_TUNE_LU_GEMM selects the
  * tiles of the LU's complex GEMM on the matrix cores, 1: 64 x 64 (default), 2: 128 x 64, and for
- * the complex64 LU of the mixed precision also 3: 64 x 64, 4: 128 x 64, 5: 128 x 128 (8 waves) on
- * the FP16 matrix cores with each FP32 operand split into two FP16 numbers (three FP16 products
- * per FP32 product; the complex128 LU takes 1 for 3 and 2 for 4 and 5); HBG_TUNE_LU_DEPTH the number of 64-column panels per
+ * the complex64 LU of the mixed precision also 3: 64 x 64, 4: 128 x 64, 5: 128 x 128 (8 waves),
+ * 6: 128 x 128 (4 waves of 64 x 64, operands loaded straight into LDS) for the trailing updates
+ * and 64 x 64 for the panels' strips, 7: as 6 with two LDS buffers (the next chunk of k loading
+ * while the current one is multiplied), on the FP16 matrix cores with each FP32 operand split
+ * into two FP16 numbers (three FP16 products per FP32 product; the complex128 LU takes 1 for 3, 6
+ * and 7 and 2 for 4 and 5); HBG_TUNE_LU_DEPTH the number of 64-column panels per
  * trailing update (1..16, default 4). Both set the complex128 and the complex64 LU and read back
  * the complex128 one; HBG_TUNE_LU_GEMM_MIXED and HBG_TUNE_LU_DEPTH_MIXED set and read the
  * complex64 LU's (the mixed precision) alone. HBG_TUNE_LU_GRAPHS 1 (default): the batched LU's
@@ -335,7 +338,7 @@ enum {
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
  * systems of order n with nrhs right-hand sides in the given precision, first for every GEMM tile
- * variant (1, 2; mixed precision also 3, 4, 5) and block depth (2, 4, 8, 12, 16) at the largest batch
+ * variant (1, 2; mixed precision also 3 to 7) and block depth (2, 4, 8, 12, 16) at the largest batch
  * of 4, 8, 16, 32, 48 that fits in a third of the free memory, then for the smaller batches at the
  * fastest variant and depth; keeps the fastest variant and depth (HBG_TUNE_LU_GEMM,
  * HBG_TUNE_LU_DEPTH; for the mixed precision HBG_TUNE_LU_GEMM_MIXED, HBG_TUNE_LU_DEPTH_MIXED) and
@@ -355,8 +358,10 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);
  *   12 LU panels that needed row interchanges (the pivoting fallback), 13 refinement steps of the
  *   mixed-precision solves, 14 their largest final backward error, 15 frequencies of a
  *   mixed-precision solve solved in FP64 instead, 16 the mixed-precision refinement ms (FP64
- *   residuals, corrections with the complex64 factors, the FP64 solution stored). */
-#define HBG_PROFILE_COUNTERS 17
+ *   residuals, corrections with the complex64 factors, the FP64 solution stored), 17 the largest
+ *   backward error of a mixed-precision solve's first solution (that of the complex64 LU, before
+ *   any refinement step). */
+#define HBG_PROFILE_COUNTERS 18
 HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable);
 HbgStatus hbg_profile_read(HbgSession* s, double* out, int32_t n);
 

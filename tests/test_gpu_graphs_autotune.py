@@ -58,7 +58,7 @@ def test_autotune_sweep_and_solve(oracle):
     assert s.get_tuning(s.TUNE_LU_GEMM) == int(best[0]) and s.get_tuning(s.TUNE_LU_DEPTH) == int(best[1])
     rows32 = s.autotune(1024, 7, "mixed")
     say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} b{int(r[2])} {r[3]:.3f} ms" for r in rows32))
-    assert len(rows32) == 5 * 5 + 4 and all(r[3] > 0 for r in rows32)
+    assert len(rows32) == 7 * 5 + 4 and all(r[3] > 0 for r in rows32)
     best32 = min(rows32, key=lambda r: r[3])
     assert s.get_tuning(s.TUNE_LU_GEMM_MIXED) == int(best32[0])
     assert s.get_tuning(s.TUNE_LU_DEPTH_MIXED) == int(best32[1])
