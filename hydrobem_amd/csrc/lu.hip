@@ -294,7 +294,9 @@ __device__ __forceinline__ C2 crecip(C2 v) {
 // that no entry below the diagonal exceeds the pivot in |re| + |im| (else the panel's flag is
 // set), writes L11 \ U11 back and the identity interchanges; mode 1 (gated) takes the block as the
 // tournament wrote it. Both then invert the triangles: Uinv = U11^-1 (kept per panel for the back
-// substitution), Linv = L11^-1, and keep the diagonal of U. Outside kb the block is the identity.
+// substitution), Linv = L11^-1 (stored column-major: entry (r, c) at c NB + r, so that the
+// column slices the MFMA operands take are contiguous), and keep the diagonal of U. Outside kb the
+// block is the identity.
 //
 // The block lives in registers: thread (ti, tj) of the 16 x 16 grid holds the entries
 // (ti + 16 a, tj + 16 b), a, b < 4. At step j the owners of row j and column j publish them in
@@ -458,7 +460,7 @@ __global__ void __launch_bounds__(256) diag_kernel(typename Cplx<T>::type* __res
     for (int b = 0; b < 4; ++b) {
       const int r = ti + 16 * a, c = tj + 16 * b;
       Uinv[r * NB + c] = xu[a][b];
-      Linv[r * NB + c] = xl[a][b];
+      Linv[c * NB + r] = xl[a][b];  // L11^-1 column-major (its transpose row-major)
     }
 }
 
@@ -477,22 +479,30 @@ constexpr int AB = 32;
 // The complex64 block products of apply_kernel on the FP32 matrix cores (v_mfma_f32_16x16x4f32,
 // the lane layout of gemm_mfma_kernel: operand lane l holds column (row) l % 16 at k = l / 16 of
 // the k-step, output item q of lane l is row l % 16, column 4 (l / 16) + q), in place in Xs:
-//   right: Xs[c][i] = sum_k Xs[k][i] Ts[k][c]  (L21 = A21 Uinv; i < AB, c < NB; wave w: columns
+//   right: Xs[c][i] = sum_k Xs[k][i] Uinv[k][c]  (L21 = A21 Uinv; i < AB, c < NB; wave w: columns
 //          16 w .., both 16-row blocks),
-//   left:  Xs[r][c] = sum_k Ts[r][k] Xs[k][c]  (U12 = Linv A12; r < NB, c < AB; wave w: rows
+//   left:  Xs[r][c] = sum_k Linv[r][k] Xs[k][c]  (U12 = Linv A12; r < NB, c < AB; wave w: rows
 //          16 w .., both 16-column blocks).
-// Every thread of the workgroup calls it (it synchronizes).
-__device__ __forceinline__ void apply_mfma(const float2 (&Ts)[NB][NB + 1], float2 (&Xs)[NB][AB + 1], bool right) {
+// The triangle's operands come in registers, tf[s] its entry (k = 4 s + l / 16, 16 w + l % 16) of
+// Uinv (row-major) or (16 w + l % 16, k) of Linv (column-major): a contiguous 16-lane run either
+// way, loaded by apply_fragments. Every thread of the workgroup calls it (it synchronizes).
+__device__ __forceinline__ void apply_fragments(const float2* __restrict__ Tm, float2 (&tf)[NB / 4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < NB / 4; ++s) tf[s] = Tm[(4 * s + (lane >> 4)) * NB + 16 * wave + (lane & 15)];
+}
+
+__device__ __forceinline__ void apply_mfma(const float2 (&tf)[NB / 4], float2 (&Xs)[NB][AB + 1], bool right) {
   typedef Mfma<float> F;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
   F::v4 re[2], im[2];
 #pragma unroll
   for (int b = 0; b < 2; ++b) re[b] = im[b] = F::v4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
+#pragma unroll
   for (int s = 0; s < NB / 4; ++s) {
     const int k = 4 * s + kq;
     if (right) {
-      const float2 u = Ts[k][16 * wave + l16];
+      const float2 u = tf[s];
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         const float2 x = Xs[k][16 * b + l16];
@@ -502,7 +512,7 @@ __device__ __forceinline__ void apply_mfma(const float2 (&Ts)[NB][NB + 1], float
         im[b] = F::mma(u.x, x.y, im[b]);
       }
     } else {
-      const float2 l = Ts[16 * wave + l16][k];
+      const float2 l = tf[s];
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         const float2 x = Xs[k][16 * b + l16];
@@ -547,20 +557,29 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     if (flag) flag += z * bs.flag;
   }
   if (gate && *gate == 0) return;
-  __shared__ C2 Ts[NB][NB + 1];
+  constexpr bool F32 = std::is_same<T, float>::value;
+  // complex128: the triangle staged in LDS, Ts[r][k] its entry (r, k); complex64: in registers
+  __shared__ C2 Ts[F32 ? 1 : NB][NB + 1];
   __shared__ C2 Xs[NB][AB + 1];  // right: [k][row]; left: [k][col]
   __shared__ int s_fail;
   const int tid = threadIdx.x;
   const bool right = (int)blockIdx.x < rblocks;
   const C2* __restrict__ Tm = right ? Uinv : Linv;
-  {
+  float2 tf[NB / 4];
+  if constexpr (F32) {
+    apply_fragments(Tm, tf);
+  } else {
     C2 t[NB * NB / 256];
 #pragma unroll
     for (int j = 0; j < NB * NB / 256; ++j) t[j] = Tm[tid + 256 * j];
+    // Uinv row-major, Linv column-major
 #pragma unroll
     for (int j = 0; j < NB * NB / 256; ++j) {
       const int e = tid + 256 * j;
-      Ts[e >> 6][e & 63] = t[j];
+      if (right)
+        Ts[e >> 6][e & 63] = t[j];
+      else
+        Ts[e & 63][e >> 6] = t[j];
     }
   }
   constexpr int VR = NB * AB / 256;  // outputs per thread
@@ -585,7 +604,7 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     const int i = tid % AB, cq = tid / AB;  // row of the block, columns VR cq .. VR cq + 7
     C2 acc[VR];
     if constexpr (std::is_same<T, float>::value) {
-      apply_mfma(Ts, Xs, true);
+      apply_mfma(tf, Xs, true);
 #pragma unroll
       for (int v = 0; v < VR; ++v) acc[v] = Xs[VR * cq + v][i];
     } else {
@@ -644,7 +663,7 @@ __global__ void __launch_bounds__(256) apply_kernel(typename Cplx<T>::type* __re
     const int cl = tid % AB, rg = tid / AB;  // column of the block, rows VR rg .. VR rg + 7
     C2 acc[VR];
     if constexpr (std::is_same<T, float>::value) {
-      apply_mfma(Ts, Xs, false);
+      apply_mfma(tf, Xs, false);
 #pragma unroll
       for (int v = 0; v < VR; ++v) acc[v] = Xs[VR * rg + v][cl];
     } else {
@@ -713,10 +732,14 @@ __global__ void __launch_bounds__(256) subst_panel_kernel(const typename Cplx<T>
     C2 t[NB * NB / 256];
 #pragma unroll
     for (int j = 0; j < NB * NB / 256; ++j) t[j] = Tinv[tid + 256 * j];
+    // U[r][k]: Uinv row-major, Linv column-major
 #pragma unroll
     for (int j = 0; j < NB * NB / 256; ++j) {
       const int e = tid + 256 * j;
-      U[e >> 6][e & 63] = t[j];
+      if (upper)
+        U[e >> 6][e & 63] = t[j];
+      else
+        U[e & 63][e >> 6] = t[j];
     }
   }
   for (int e = tid; e < NB * nrhs; e += 256) {
