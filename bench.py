@@ -45,10 +45,13 @@ VARIANTS = {1: ("64x64", "gemm_mfma_kernel<T, 64, 64>", None), 2: ("128x64", "ge
 # (workload, path, GEMM tiles, panels per trailing update, batch) -> bytes
 TRAFFIC = {("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 16): 8.496e8,
            ("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 48): 2.548e9,
-           ("cylinder8192fd", "mixed", "64x64 split-fp16", 16, 48): 2.852e9}
-# HBM bytes per launch (one frequency) of the FP64 assembly kernel, from the same calibrated runs:
-# workload -> bytes (profiles/assembly_hbm_traffic_cylinder8192_finite_depth.md)
-ASM_TRAFFIC = {"cylinder8192fd": 8.97e9}
+           ("cylinder8192fd", "mixed", "64x64 split-fp16", 16, 48): 2.852e9,
+           # the 128 x 128 kernel, averaged over the trailing updates of a solve
+           ("cylinder8192fd", "mixed32", "128x128 split-fp16 direct-LDS", 16, 48): 5.637e10}
+# HBM bytes per launch (one frequency) of the assembly kernel, from the same calibrated runs:
+# (workload, path) -> bytes (profiles/assembly_hbm_traffic_cylinder8192_finite_depth.md)
+ASM_TRAFFIC = {("cylinder8192fd", "fp64"): 8.97e9, ("cylinder8192fd", "mixed"): 8.97e9,
+               ("cylinder8192fd", "mixed32"): 5.22e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
                       "operands split into hi + lo, autotuned), FP64 iterative refinement to backward error 1e-12",
@@ -270,6 +273,14 @@ def summary(prof, dt, steps, F, peak):
         "gemm_peak_tflops": peak / 1e12,
         "gemm_per_launch": {"flops": prof["lu_gemm_flops"] / max(prof["lu_gemm_launches"], 1.0),
                             "ms": prof["lu_gemm_ms"] / max(prof["lu_gemm_launches"], 1.0)},
+        # variants 6 and 7: the trailing updates on the 128 x 128 tiles alone (the panels' strips
+        # run on other tile shapes)
+        "gemm_big": {"tflops": round(prof["lu_big_gemm_flops"] / (prof["lu_big_gemm_ms"] * 1e-3) / 1e12, 3)
+                     if prof["lu_big_gemm_ms"] > 0 else 0.0,
+                     "launches": int(prof["lu_big_gemm_launches"]),
+                     "per_launch": {"flops": prof["lu_big_gemm_flops"] / max(prof["lu_big_gemm_launches"], 1.0),
+                                    "ms": prof["lu_big_gemm_ms"] / max(prof["lu_big_gemm_launches"], 1.0)}},
+        "first_solution_backward_error": prof["mixed_max_first_backward_error"],
         "lu_pivoting_fallback_panels": int(prof["lu_fallback_panels"]),
         "assembly_achieved_gbs": round(asm / 1e9, 1),
         "refinement_steps_per_frequency": round(prof["mixed_refinement_steps"] / nf, 3),
@@ -287,7 +298,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="frequencies in flight (0: automatic)")
     ap.add_argument("--gemm", type=int, default=-1, help="LU GEMM variant (-1: the library's default)")
     ap.add_argument("--depth", type=int, default=-1, help="LU panels per trailing update (-1: the library's default)")
-    ap.add_argument("--paths", default="fp64,mixed,mixed_nn", help="paths timed, the last one gives `value`")
+    ap.add_argument("--paths", default="fp64,mixed,mixed_nn,mixed32",
+                    help="paths timed; `value` is the fastest whose A, B, X are within 1e-5 of the FP64 path's")
     ap.add_argument("--no-autotune", action="store_true", help="skip the LU autotuner before each path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--symmetry", action="store_true",
@@ -400,7 +412,10 @@ def main():
             config[p]["rel_err_vs_fp64_gpu"]["rao"] = rel_err(s2.motions(res[p][0], M, C), rao_ref)
             config[p]["speedup_vs_fp64_gpu"] = round(res[p][1]["freq_per_s"] / res["fp64"][1]["freq_per_s"], 3)
         s2.close()
-    gemm_tf = summ["gemm_achieved_tflops"]
+    gemm_tf, gemm_launch = summ["gemm_achieved_tflops"], summ["gemm_per_launch"]
+    if summ.get("gemm_big", {}).get("launches", 0) > 0:
+        # variants 6 and 7: the dominant kernel is the 128 x 128 one of the trailing updates
+        gemm_tf, gemm_launch = summ["gemm_big"]["tflops"], summ["gemm_big"]["per_launch"]
     # the LU settings the value path ran with: autotuned, or given on the command line, or the defaults
     gv = summ["gemm_variant"]
     tun = tuning.get("fp64" if main_path == "fp64" else "mixed",
@@ -410,6 +425,8 @@ def main():
     traffic = TRAFFIC.get((wkey, main_path, tun["gemm_tiles"], tun["panels_per_update"], tun["batch"]))
     tiles, kernel, split = VARIANTS[tun["gemm_variant"]]
     kernel = kernel.replace("<T,", "<double," if main_path == "fp64" else "<float,")
+    if tun["gemm_variant"] >= 6:
+        kernel = kernel.replace("gemm_f16s_big_kernel<", "gemm_f16s_big_kernel<128, 128, ")
     # the split-FP16 GEMM is priced against the FP16 matrix peak (each FP32 product is three FP16
     # products, so its FP32-equivalent ceiling is a third of that)
     gpeak = PEAK_FP16 if split else peaks[main_path]
@@ -431,15 +448,18 @@ def main():
                                                 + (", FP32 operands split into FP16 hi + lo)" if split else ")"),
                      "achieved": gemm_tf, "peak": gpeak / 1e12, "unit": "TFLOP/s",
                      "frac": round(gemm_tf * 1e12 / gpeak, 4), "traffic": traffic,
-                     "per_launch": summ["gemm_per_launch"]},
-        # the other large kernel of the step: the FP64 assembly (one launch per frequency), bound by
-        # FP64 vector issue, not HBM; its algorithmic bytes are N^2 (32 + 2 x 16) (statics read,
-        # S and A written), against the stage time (boundary values, tables and matrices)
-        "roofline_assembly": {"bound": "hbm", "kernel": "assemble_fin_kernel" if np.isfinite(w.get("depth", np.inf))
-                              else "assemble_kernel", "achieved": summ["assembly_achieved_gbs"],
+                     "per_launch": gemm_launch},
+        # the other large kernel of the step: the assembly (one launch per frequency), bound by
+        # vector issue (FP64, or FP32 for mixed32), not HBM; its algorithmic bytes are N^2 (32 + 2 x 16)
+        # (statics read, S and A written), for mixed32 N^2 (16 + 3 x 8) (FP32 statics read, S, A and
+        # A's unfactored copy written), against the stage time (boundary values, tables, matrices)
+        "roofline_assembly": {"bound": "hbm", "kernel": ("assemble_fin32_kernel" if main_path == "mixed32" else
+                                                         "assemble_fin_kernel") if np.isfinite(w.get("depth", np.inf))
+                              else ("assemble32_kernel" if main_path == "mixed32" else "assemble_kernel"),
+                              "achieved": summ["assembly_achieved_gbs"],
                               "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                               "frac": round(summ["assembly_achieved_gbs"] * 1e9 / PEAK_HBM, 4),
-                              "traffic": ASM_TRAFFIC.get(wkey),
+                              "traffic": ASM_TRAFFIC.get((wkey, main_path)),
                               "ms_per_launch": summ["stage_ms_per_frequency"]["assembly_ms"]},
     }
     if not a.no_cpu_baseline:

@@ -267,7 +267,8 @@ class Session:
     PROFILE_KEYS = ("statics_ms", "assembly_ms", "rhs_ms", "lu_ms", "lu_gemm_ms", "lu_gemm_flops",
                     "lu_gemm_launches", "forces_ms", "frequencies", "lu_flops", "assembly_bytes", "statics_runs",
                     "lu_fallback_panels", "mixed_refinement_steps", "mixed_max_backward_error",
-                    "mixed_fp64_fallbacks", "mixed_refinement_ms", "mixed_max_first_backward_error")
+                    "mixed_fp64_fallbacks", "mixed_refinement_ms", "mixed_max_first_backward_error",
+                    "lu_big_gemm_ms", "lu_big_gemm_flops", "lu_big_gemm_launches")
 
     def set_batch(self, max_batch: int):
         """Frequencies in flight in solve (0: as many as fit, at most 48)."""

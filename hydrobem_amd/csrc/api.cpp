@@ -2078,11 +2078,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       s->prof[7] += stage_ms(s->bev[e0], s->bev[e1]);
       s->prof[8] += nb;
       s->prof[9] += nbg * 8.0 / 3.0 * (double)N * N * N;
-      s->prof[10] += nbg * (double)N * N * (m32 ? 4 * sizeof(float) + 2 * sizeof(float2)
+      s->prof[10] += nbg * (double)N * N * (m32 ? 4 * sizeof(float) + 3 * sizeof(float2)
                                                 : sizeof(StaticD) + 2 * sizeof(double2));
       s->prof[12] += (double)lus.fallback_panels;
       s->prof[16] += lus.refine_ms;
       s->prof[17] = std::max(s->prof[17], lus.first_eta);
+      s->prof[18] += lus.big_ms;
+      s->prof[19] += lus.big_flops;
+      s->prof[20] += (double)lus.big_launches;
     }
     for (size_t i = 0; i < nb * nout; ++i)
       if (!std::isfinite(hout[i])) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};

@@ -33,6 +33,15 @@ constexpr int FDF = 7, MFF = FDF + 1, FTILEF = 3 * MFF * MFF;  // Phi: [value, d
 constexpr int KDF = 11, KTILEF = 2 * (KDF + 1);                 // K0, K1: [f][power]
 constexpr double NEAR_RATIO = 2.0;  // pairs in FP64: |c_k - c_n| < NEAR_RATIO (radius_k + radius_n)
 
+// The FP32 evaluation's square roots, reciprocals and logarithms by the hardware's instructions
+// (v_sqrt_f32, v_rcp_f32, v_log_f32, about 1 ulp) instead of the correctly rounded library
+// sequences: the FP32 path's accuracy is that of its tables and of FP32 arithmetic (about 1e-7
+// relative), not half an ulp here. The exponential stays the library's (its argument reaches 48,
+// where the instruction's argument rounding would cost digits).
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+
 struct WaveTables32 {
   const float* t2d = nullptr;  // [YT][XT][TILE2F]
   const float* t1d = nullptr;  // [XT][TILE1F]
@@ -259,11 +268,12 @@ struct WaveK32 {
 __device__ __forceinline__ WaveK32 wave_tab32(float X, float Y, float j0, float j1, float yt, float dyt, float V,
                                               float VX) {
   WaveK32 w;
-  const float R0 = sqrtf(X * X + Y * Y);
+  const float R0 = fsqrt(X * X + Y * Y);
   const float eY = expf(-Y);
-  const float lg = logf(Y + R0) + (GAMMA_F - LOG2_F);
+  const float lg = flog(Y + R0) + (GAMMA_F - LOG2_F);
+  const float iR0 = frcp(R0);
   w.F = -2.0f * eY * j0 * lg - PI_F * eY * yt - 2.0f * R0 * V;
-  w.FX = 2.0f * eY * j1 * lg - 2.0f * eY * j0 * X / (R0 * (Y + R0)) - PI_F * eY * dyt - 2.0f * (X / R0) * V -
+  w.FX = 2.0f * eY * j1 * lg - 2.0f * eY * j0 * X * iR0 * frcp(Y + R0) - PI_F * eY * dyt - 2.0f * (X * iR0) * V -
          2.0f * R0 * VX;
   w.I = -2.0f * PI_F * eY * j0;
   w.IX = 2.0f * PI_F * eY * j1;
@@ -343,9 +353,9 @@ __device__ __forceinline__ FinK32 fin_kernel32_u(const FinFreq32* __restrict__ f
   if (__ballot(series)) {
     float y0 = 0.0f, y1 = 0.0f;
     if (series) {
-      const float L = logf(0.5f * x) + GAMMA_F;
+      const float L = flog(0.5f * x) + GAMMA_F;
       y0 = yt + 2.0f / PI_F * L * j0;
-      y1 = -dyt - 2.0f / PI_F * (j0 / x - L * j1);
+      y1 = -dyt - 2.0f / PI_F * (j0 * frcp(x) - L * j1);
     }
     float re = -A0 * E * y0, reR = A0 * E * k0 * y1, reZ = -A0 * Ez * y0, rez = -A0 * Ef * y0;
     const float2* cz = F.pcs + (size_t)k * FIN_ROOTS;
@@ -393,7 +403,7 @@ __device__ __forceinline__ FinK32 fin_kernel32_u(const FinFreq32* __restrict__ f
     }
   }
   if (__ballot(near)) {
-    const float ih = 1.0f / h, ih2 = ih * ih;
+    const float ih = frcp(h), ih2 = ih * ih;
     float f1[3] = {0.0f, 0.0f, 0.0f}, f2[3] = {0.0f, 0.0f, 0.0f};
     phi32_u(F.phi1, FTW, 1.0f, near, R * ih, v1 * ih, f1);
     phi32_u(F.phi2, FTV, 2.0f, near, R * ih, v2 * ih, f2);
@@ -403,7 +413,7 @@ __device__ __forceinline__ FinK32 fin_kernel32_u(const FinFreq32* __restrict__ f
     table2d32_u(T.t2d, near, X, Y, V, VX);
     if (near) {
       const WaveK32 w = wave_tab32(X, Y, wj0, wj1, wyt, wdyt, V, VX);
-      const float FY = -w.F - 2.0f / sqrtf(X * X + Y * Y);
+      const float FY = -w.F - 2.0f * __builtin_amdgcn_rsqf(X * X + Y * Y);
       Wr = ih * (f1[0] + f2[0]) + K * w.F;
       WrR = ih2 * (f1[1] + f2[1]) + K * K * w.FX;
       const float A1z = -sg * ih2 * f1[2];
@@ -451,7 +461,7 @@ __global__ void __launch_bounds__(256) assemble32_kernel(const PanelF* __restric
     const float4 ck = P[k].ca, cn = P[n].ca;
     const float dx = ck.x - cn.x, dy = ck.y - cn.y;
     float tx, ty;
-    return tile2d32(K * sqrtf(dx * dx + dy * dy), -K * (ck.z + cn.z), tx, ty);
+    return tile2d32(K * fsqrt(dx * dx + dy * dy), -K * (ck.z + cn.z), tx, ty);
   });
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float c2 = 2.0f * K * logc, K2 = K * K;
@@ -463,19 +473,19 @@ __global__ void __launch_bounds__(256) assemble32_kernel(const PanelF* __restric
     const int k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
     const PanelF pk = P[k], pn = P[n];
     const float dx = pk.ca.x - pn.ca.x, dy = pk.ca.y - pn.ca.y;
-    const float R = sqrtf(dx * dx + dy * dy);
+    const float R = fsqrt(dx * dx + dy * dy);
     const float X = K * R, Y = -K * (pk.ca.z + pn.ca.z);
     const WaveK32 w = wave_kernel32_u(T, valid, X, Y);
     if (valid) {
-      const float R0 = sqrtf(X * X + Y * Y);
-      const float FY = -w.F - 2.0f / R0, IY = -w.I;
+      const float R0 = fsqrt(X * X + Y * Y);
+      const float FY = -w.F - 2.0f * frcp(R0), IY = -w.I;
       const float Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
       const float Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
-      const float an = R > 0.0f ? (dx * pn.nr.x + dy * pn.nr.y) / R : 0.0f;
+      const float an = R > 0.0f ? (dx * pn.nr.x + dy * pn.nr.y) * frcp(R) : 0.0f;
       put_entry32(S, A, A2, N, lda, k, n, st[(size_t)n * N + k], pn.ca.w, pn.nr.z, an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ,
                   free_term);
       if (k != n) {
-        const float ak = R > 0.0f ? -(dx * pk.nr.x + dy * pk.nr.y) / R : 0.0f;
+        const float ak = R > 0.0f ? -(dx * pk.nr.x + dy * pk.nr.y) * frcp(R) : 0.0f;
         put_entry32(S, A, A2, N, lda, n, k, st[(size_t)k * N + n], pk.ca.w, pk.nr.z, ak, c2, Wr, WrR, WrZ, Wi, WiR, WiZ,
                     free_term);
       }
@@ -496,11 +506,11 @@ __global__ void __launch_bounds__(256) assemble_fin32_kernel(const PanelF* __res
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
   __shared__ PairSort L;
-  const float h = f->h, ih = 1.0f / h;
+  const float h = f->h, ih = frcp(h);
   const unsigned total = sort_tile_pairs(L, bi, bj, N, [&](int k, int n) {
     const float4 ck = P[k].ca, cn = P[n].ca;
     const float dx = ck.x - cn.x, dy = ck.y - cn.y;
-    const float R = sqrtf(dx * dx + dy * dy);
+    const float R = fsqrt(dx * dx + dy * dy);
     if (R >= 0.5f * h) {
       const float b = (R * ih - 0.5f) * 32.0f;
       return FS_NEAR32 + (b < 1023.0f ? (int)b : 1023);
@@ -520,14 +530,14 @@ __global__ void __launch_bounds__(256) assemble_fin32_kernel(const PanelF* __res
     const int k = bi * ASM_TILE + (p & 63), n = bj * ASM_TILE + (p >> 6);
     const PanelF pk = P[k], pn = P[n];
     const float dx = pk.ca.x - pn.ca.x, dy = pk.ca.y - pn.ca.y;
-    const float R = sqrtf(dx * dx + dy * dy);
+    const float R = fsqrt(dx * dx + dy * dy);
     const FinK32 w = fin_kernel32_u(f, F, T, valid, k, n, R, pk.ca.z, pn.ca.z);
     if (valid) {
-      const float an = R > 0.0f ? (dx * pn.nr.x + dy * pn.nr.y) / R : 0.0f;
+      const float an = R > 0.0f ? (dx * pn.nr.x + dy * pn.nr.y) * frcp(R) : 0.0f;
       put_entry32(S, A, A2, N, lda, k, n, st[(size_t)n * N + k], pn.ca.w, pn.nr.z, an, c2, w.Wr, w.WrR, w.WrZ, w.Wi,
                   w.WiR, w.WiZ, free_term);
       if (k != n) {
-        const float ak = R > 0.0f ? -(dx * pk.nr.x + dy * pk.nr.y) / R : 0.0f;
+        const float ak = R > 0.0f ? -(dx * pk.nr.x + dy * pk.nr.y) * frcp(R) : 0.0f;
         put_entry32(S, A, A2, N, lda, n, k, st[(size_t)k * N + n], pk.ca.w, pk.nr.z, ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi,
                     w.WiR, w.Wiz, free_term);
       }

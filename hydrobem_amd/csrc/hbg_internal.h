@@ -62,6 +62,8 @@ struct LuStats {
   double total_ms = 0.0, gemm_ms = 0.0, gemm_flops = 0.0;
   double refine_ms = 0.0;  // mixed precision: residuals, corrections and the FP64 store
   double first_eta = 0.0;  // mixed precision: the largest backward error of a first solution
+  double big_ms = 0.0, big_flops = 0.0;  // the launches of the 128 x 128 split-FP16 GEMM tiles
+  long big_launches = 0;
   long gemm_launches = 0;
   long fallback_panels = 0;
 };

@@ -1043,6 +1043,7 @@ struct GemmTimer {
   EventPool* ep;
   LuStats* stats;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::vector<char> big;  // per launch: on the 128 x 128 tiles of gemm_f16s_big_kernel
 };
 
 // One launch of the trailing-update GEMM in the given tile variant: 1 64 x 64, 2 128 x 64 (FP32 or
@@ -1112,8 +1113,16 @@ void gemm_sub(typename Cplx<T>::type* M, int lda, int r0, int nrows, int c0, int
     hipEvent_t g1 = gt.ep->next();
     HBG_HIP(hipEventRecord(g1, stream));
     gt.ev.push_back({g0, g1});
-    gt.stats->gemm_flops += 8.0 * batch * (double)nrows * (double)ncolsC * (double)kk;
+    const double fl = 8.0 * batch * (double)nrows * (double)ncolsC * (double)kk;
+    gt.stats->gemm_flops += fl;
     gt.stats->gemm_launches += 1;
+    const bool big = std::is_same<T, float>::value && g_gemm_variant_[prec_index<T>()] >= 6 && nrows > 64 &&
+                     ncolsC > 64;
+    gt.big.push_back(big ? 1 : 0);
+    if (big) {
+      gt.stats->big_flops += fl;
+      gt.stats->big_launches += 1;
+    }
   }
 }
 
@@ -1246,11 +1255,13 @@ void finish_stats(LuStats* stats, EventPool& ep, hipEvent_t t0, GemmTimer& gt, h
   float ms = 0.0f;
   HBG_HIP(hipEventElapsedTime(&ms, t0, t1));
   stats->total_ms += ms;
-  for (auto& g : gt.ev) {
-    HBG_HIP(hipEventElapsedTime(&ms, g.first, g.second));
+  for (size_t i = 0; i < gt.ev.size(); ++i) {
+    HBG_HIP(hipEventElapsedTime(&ms, gt.ev[i].first, gt.ev[i].second));
     stats->gemm_ms += ms;
+    if (i < gt.big.size() && gt.big[i]) stats->big_ms += ms;
   }
   gt.ev.clear();
+  gt.big.clear();
 }
 
 template <typename T>

@@ -360,8 +360,10 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);
  *   mixed-precision solve solved in FP64 instead, 16 the mixed-precision refinement ms (FP64
  *   residuals, corrections with the complex64 factors, the FP64 solution stored), 17 the largest
  *   backward error of a mixed-precision solve's first solution (that of the complex64 LU, before
- *   any refinement step). */
-#define HBG_PROFILE_COUNTERS 18
+ *   any refinement step), 18-20 ms, algorithmic flops and launches of the GEMM launches on the
+ *   128 x 128 tiles of tuning variants 6 and 7 (part of 4-6). Counter 10 counts N^2 (16 + 3 x 8)
+ *   bytes per system of HBG_PRECISION_MIXED32 (FP32 statics read, S, A and A's copy written). */
+#define HBG_PROFILE_COUNTERS 21
 HbgStatus hbg_set_profiling(HbgSession* s, int32_t enable);
 HbgStatus hbg_profile_read(HbgSession* s, double* out, int32_t n);
 
