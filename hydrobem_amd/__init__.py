@@ -63,6 +63,7 @@ SIGNATURES = {
     "hbg_session_free": (ctypes.c_int32, [ctypes.c_void_p]),
     "hbg_set_environment": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "hbg_set_panels": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip, ctypes.c_int32, _dp]),
+    "hbg_set_interior": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip]),
     "hbg_set_symmetry": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_set_walls": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_set_field_points": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
@@ -193,6 +194,17 @@ class Session:
         _check(library().hbg_set_panels(self._h, _p(v), n, b.ctypes.data_as(_ip) if b is not None else None, nb,
                                         _p(rc)))
         self.n_panels, self.n_bodies = n, nb
+        self.n_interior = 0
+
+    def set_interior(self, vertices=None, body=None):
+        """The interior free surface (panels in z = 0 inside the waterlines, in the stored symmetry
+        form of the body panels; meshes.interior_graded makes them): extra unknowns that remove the
+        irregular frequencies (None: none)."""
+        v = np.zeros((0, 4, 3)) if vertices is None else np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 4, 3)
+        b = None if body is None else np.ascontiguousarray(body, dtype=np.int32)
+        _check(library().hbg_set_interior(self._h, _p(v) if len(v) else None, len(v),
+                                          b.ctypes.data_as(_ip) if b is not None else None))
+        self.n_interior = len(v)
 
     def set_symmetry(self, sym_x=False, sym_y=False):
         """Symmetry planes x = 0 (sym_x) and y = 0 (sym_y): the panels set are one sector of the

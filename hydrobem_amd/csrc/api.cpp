@@ -410,6 +410,8 @@ struct HbgSession {
   std::vector<double> chi{1.0}, chisym{1.0};  // [Gs][G], [Gs][Gs]
   std::vector<double> raw_vertices, raw_centres;  // the panels as set, to rebuild the images
   std::vector<int> raw_body;
+  std::vector<char> raw_interior;  // per panel: on the interior free surface (after the body panels)
+  int NI = 0;                      // panels of the interior free surface among the N (all images' sector)
   std::vector<PanelD> panels;
   std::vector<int> body;
   std::vector<double> nj;
@@ -969,7 +971,8 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
 struct RawPanels {
   std::vector<double> vertices, centres;
   std::vector<int> body;
-  int N = 0, nb = 0;
+  std::vector<char> interior;  // per panel: a panel of the interior free surface (after the body's)
+  int N = 0, nb = 0, NI = 0;
 };
 
 // The reflection group of the symmetry planes and walls and its solved characters.
@@ -986,8 +989,10 @@ static RawPanels raw_of(const HbgSession* s) {
   r.vertices = s->raw_vertices;
   r.centres = s->raw_centres;
   r.body = s->raw_body;
+  r.interior = s->raw_interior;
   r.N = s->N;
   r.nb = s->nb;
+  r.NI = s->NI;
   return r;
 }
 
@@ -1002,6 +1007,7 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
   for (int j = 0; j < n_panels; ++j) raw.body[j] = body ? body[j] : 0;
   raw.centres.assign(3 * (size_t)n_bodies, 0.0);
   if (rotation_centres) raw.centres.assign(rotation_centres, rotation_centres + 3 * (size_t)n_bodies);
+  raw.interior.assign(n_panels, 0);
   raw.N = n_panels;
   raw.nb = n_bodies;
   // a refused panel leaves the session's geometry (and its statics) as they were
@@ -1030,23 +1036,34 @@ static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, 
             if ((walls >> a & 1) && raw.vertices[12 * (size_t)j + 3 * q + a] < 0.0)
               throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": behind the wall " +
                                              (a ? "y" : "x") + " = 0"};
+      const bool inner = !raw.interior.empty() && raw.interior[j];
       double v[12];
       for (int q = 0; q < 4; ++q) {
         const int src = (sx * sy < 0.0 && q > 0) ? 4 - q : q;  // 0, 3, 2, 1 where the orientation flips
         v[3 * q] = sx * raw.vertices[12 * (size_t)j + 3 * src];
         v[3 * q + 1] = sy * raw.vertices[12 * (size_t)j + 3 * src + 1];
-        v[3 * q + 2] = raw.vertices[12 * (size_t)j + 3 * src + 2];
+        v[3 * q + 2] = inner ? 0.0 : raw.vertices[12 * (size_t)j + 3 * src + 2];
       }
       V3 proj[4], poly[4], normal, centroid;
       int m;
       double area;
       std::string why = make_panel(v, proj, poly, m, normal, centroid, area);
+      if (why.empty() && inner && normal[2] < 0.0) {
+        // the interior free surface lies in z = 0 with its normal up (lo_geometry.rs:707-718):
+        // a panel given clockwise from above is taken counter-clockwise
+        for (int d = 0; d < 3; ++d) std::swap(v[3 + d], v[9 + d]);
+        why = make_panel(v, proj, poly, m, normal, centroid, area);
+      }
       if (!why.empty()) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": " + why};
-      if (!(centroid[2] < 0.0))
+      if (!inner && !(centroid[2] < 0.0))
         throw Error{HBG_ERR_UNSUPPORTED,
                     "panel " + std::to_string(j + 1) + ": centroid not below z = 0 (surfaces in z = 0 are not supported)"};
       const size_t o = (size_t)g * n_panels + j;
       panels[o] = panel_record(proj, normal, centroid, area);
+      if (inner) {
+        panels[o].c[2] = 0.0;
+        panels[o].interior = 1.0;
+      }
       const int bi = raw.body[j];
       if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
       b[o] = bi;
@@ -1057,9 +1074,9 @@ static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, 
       }
       V3 rc = v3(raw.centres[3 * bi], raw.centres[3 * bi + 1], raw.centres[3 * bi + 2]);
       V3 mom = cross(centroid - rc, normal);
-      for (int d = 0; d < 3; ++d) {
-        nj[6 * o + d] = normal[d];
-        nj[6 * o + 3 + d] = mom[d];
+      for (int d = 0; d < 3; ++d) {  // the rigid modes do not move the interior free surface
+        nj[6 * o + d] = inner ? 0.0 : normal[d];
+        nj[6 * o + 3 + d] = inner ? 0.0 : mom[d];
       }
     }
   use_device(s);
@@ -1099,8 +1116,38 @@ static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, 
   s->raw_vertices = raw.vertices;
   s->raw_centres = raw.centres;
   s->raw_body = raw.body;
+  s->raw_interior = raw.interior;
   s->N = n_panels;
   s->nb = n_bodies;
+  s->NI = raw.NI;
+}
+
+HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body) {
+  if (!s || (n_panels > 0 && !vertices)) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_set_interior");
+  if (n_panels < 0) return fail(HBG_ERR_ARGUMENT, "negative count", "hbg_set_interior");
+  if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_set_interior");
+  if (s->walls) return fail(HBG_ERR_UNSUPPORTED, "the interior free surface with walls", "hbg_set_interior");
+  GUARD_BEGIN
+  // the body panels as set, then the new interior panels (z = 0 in body-local coordinates)
+  RawPanels raw = raw_of(s);
+  const int nbody = raw.N - raw.NI;
+  raw.vertices.resize(12 * (size_t)nbody);
+  raw.body.resize(nbody);
+  raw.interior.assign(nbody, 0);
+  for (int j = 0; j < n_panels; ++j) {
+    const int bi = body ? body[j] : 0;
+    for (int q = 0; q < 4; ++q)
+      if (std::fabs(vertices[12 * (size_t)j + 3 * q + 2]) > 1e-9)
+        throw Error{HBG_ERR_VALUE, "interior panel " + std::to_string(j + 1) + ": a vertex off z = 0"};
+    raw.vertices.insert(raw.vertices.end(), vertices + 12 * (size_t)j, vertices + 12 * (size_t)j + 12);
+    raw.body.push_back(bi);
+    raw.interior.push_back(1);
+  }
+  raw.N = nbody + n_panels;
+  raw.NI = n_panels;
+  set_geometry(s, make_group(s->sym_x, s->sym_y, s->walls), s->sym_x, s->sym_y, s->walls, raw);
+  return HBG_OK;
+  GUARD_END("hbg_set_interior")
 }
 
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
@@ -1224,6 +1271,8 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
     return fail(HBG_ERR_VALUE, "wall_x and wall_y must be 0 or 1", "hbg_set_walls");
   if ((wall_x && s->sym_x) || (wall_y && s->sym_y))
     return fail(HBG_ERR_VALUE, "a wall where a symmetry plane stands", "hbg_set_walls");
+  if ((wall_x || wall_y) && s->NI > 0)
+    return fail(HBG_ERR_UNSUPPORTED, "walls with an interior free surface", "hbg_set_walls");
   GUARD_BEGIN
   // panels behind a new wall leave the walls and the geometry as they were
   const int walls = wall_x | wall_y << 1;
@@ -1491,9 +1540,12 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   s->d_M.ensure((size_t)N * N);
   const double K = omega * omega / s->grav;
   check_depth(s);
+  if (s->NI > 0 && std::isfinite(s->depth))
+    throw Error{HBG_ERR_UNSUPPORTED, "the interior free surface in finite depth"};
   if (m32) {
     // the matrices of the FP32 assembly: its complex64 entries, the FP64 values at the near entries
     if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with the surrogate"};
+    if (s->NI > 0) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with an interior free surface"};
     if (!fp32_in_range(s, K)) throw Error{HBG_ERR_UNSUPPORTED, "pairs beyond the FP32 tables at this frequency"};
     ensure_fp32_geometry(s, o.logarithmic ? 1 : 0, o.log_tolerance);
     if (std::isfinite(s->depth)) {
@@ -1701,6 +1753,14 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   }
   if (!il.empty() && std::isfinite(s->depth))
     throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
+  if (s->NI > 0) {
+    // the interior free surface: infinite depth, ordinary frequencies, the forces
+    if (std::isfinite(s->depth)) throw Error{HBG_ERR_UNSUPPORTED, "the interior free surface in finite depth"};
+    if (!il.empty()) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with an interior free surface"};
+    if (s->np > 0 || s->nk > 0)
+      throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions with an interior free surface"};
+    if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
+  }
   if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
   if ((s->nk > 0 || s->np > 0) && !il.empty())
@@ -1722,7 +1782,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   // the FP32 far-field assembly: one geometry without images, no surrogate, every frequency's
   // pairs inside the FP32 tables; otherwise HBG_PRECISION_MIXED. It takes its own statics (FP32
   // records, FP64 at the near entries); the FP64 statics of every pair only for the limits.
-  bool m32 = o.precision == HBG_PRECISION_MIXED32 && s->G == 1 && !o.surrogate;
+  bool m32 = o.precision == HBG_PRECISION_MIXED32 && s->G == 1 && !o.surrogate && s->NI == 0;
   for (int i = 0; m32 && i < n_omegas; ++i) m32 = fp32_in_range(s, omegas[i] * omegas[i] / s->grav);
   if (!m32 || !il.empty()) ensure_statics(s, o.logarithmic ? 1 : 0, o.log_tolerance, want_inf);
   if (m32) ensure_fp32_geometry(s, o.logarithmic ? 1 : 0, o.log_tolerance);
@@ -2149,6 +2209,8 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
     if (sessions[i]->N == 0) return fail(HBG_ERR_STATE, "a session has no panels", "hbg_solve_group");
     if (sessions[i]->G > 1)
       return fail(HBG_ERR_UNSUPPORTED, "a session with symmetry planes in a group", "hbg_solve_group");
+    if (sessions[i]->NI > 0)
+      return fail(HBG_ERR_UNSUPPORTED, "a session with an interior free surface in a group", "hbg_solve_group");
     if (sessions[i]->device != sessions[0]->device)
       return fail(HBG_ERR_VALUE, "the sessions of a group must share their device", "hbg_solve_group");
     for (int j = 0; j < i; ++j)

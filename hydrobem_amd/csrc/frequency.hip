@@ -27,13 +27,22 @@ void launch_near_rows(const int* rowptr, const int2* ent, const double2* d, size
 // workgroup takes a 64 x 64 tile (k-block i <= n-block j; the diagonal tiles the pairs k <= n),
 // the 64 lanes of a wavefront along k and 16 n per wavefront, and writes (k, n) (coalesced) and
 // (n, k) (one column per lane; the wavefront's consecutive n fill the same cache lines).
+// rint, cint: the row's and the column's panel lie on the interior free surface (kind 1): the row's
+// free term is -4 pi (-2 free_term; Lee, Newman & Zhu 1996 eq. 25, lo_frequency.rs:85-97) and
+// the column's "dipole" is K S (dG/dzeta = K G on zeta = 0, low_order.rs:771-785)
 __device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __restrict__ A, float2* __restrict__ A32,
                                           int lda, int row, int col, const StaticD& e, double Ar, double nz, double a,
                                           double c2, double Wr, double WrR, double WrZ, double Wi, double WiR,
-                                          double WiZ, double free_term) {
-  S[(size_t)col * lda + row] = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
-  const double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
-  const double2 v = make_double2((row == col ? free_term : 0.0) - dre, -dim);
+                                          double WiZ, double free_term, bool rint = false, bool cint = false,
+                                          double K = 0.0) {
+  const double2 sv = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
+  S[(size_t)col * lda + row] = sv;
+  double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
+  if (cint) {
+    dre = K * sv.x;
+    dim = K * sv.y;
+  }
+  const double2 v = make_double2((row == col ? (rint ? -2.0 * free_term : free_term) : 0.0) - dre, -dim);
   A[(size_t)col * lda + row] = v;
   if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
 }
@@ -84,15 +93,25 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
     if (valid) {
       const double R0 = sqrt(X * X + Y * Y);
       const double FY = -w.F - 2.0 / R0, IY = -w.I;
-      const double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
-      const double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
+      double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
+      double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
+      const bool ik = pk->interior != 0.0, in = pn->interior != 0.0;  // on the interior free surface
+      if (R == 0.0 && pk->c[2] == 0.0 && pn->c[2] == 0.0) {
+        // coincident points on z = 0 (an interior panel and its own centroid): the regular part
+        // K (Ft + 2 psi), psi = log(r' + |z + zeta|), tends to -2 K (gamma - log 2 + log K)
+        // (low_order.rs:634-643); the log part is integrated in the statics, the columns of such
+        // panels are K S (the derivatives are not used)
+        Wr = -2.0 * K * (GAMMA - LOG2 + log(K));
+        Wi = -2.0 * PI * K;
+        WrR = WrZ = WiR = WiZ = 0.0;
+      }
       const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
       put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ,
-                free_term);
+                free_term, ik, in, K);
       if (k != n) {
         const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
         put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR,
-                  WiZ, free_term);
+                  WiZ, free_term, in, ik, K);
       }
     }
   }
@@ -250,8 +269,9 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
       g2.x += e.x * Zz;
       g2.y += e.y * Zz;
     }
-    const double2 dn = make_double2(g0.x * p->n[0] + g1.x * p->n[1] + g2.x * p->n[2],
-                                    g0.y * p->n[0] + g1.y * p->n[1] + g2.y * p->n[2]);
+    double2 dn = make_double2(g0.x * p->n[0] + g1.x * p->n[1] + g2.x * p->n[2],
+                              g0.y * p->n[0] + g1.y * p->n[1] + g2.y * p->n[2]);
+    if (p->interior != 0.0) dn = make_double2(0.0, 0.0);  // the interior free surface carries no velocity
     phiI[(size_t)h * N + j] = phi;
     dphiI[(size_t)h * N + j] = dn;
     u[(size_t)(nrad + h) * N + j] = make_double2(-dn.x, -dn.y);

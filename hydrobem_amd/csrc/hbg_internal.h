@@ -29,7 +29,8 @@ struct PanelD {
   double q4[4][3];  // the four projected corners as given (bilinear map of the 3 x 3 far rule)
   double nv;        // 3 or 4 distinct vertices
   double top;       // largest vertex z
-  double pad[6];
+  double interior;  // 1: a panel of the interior free surface (in z = 0, normal +z), 0: a body panel
+  double pad[5];
 };
 static_assert(sizeof(PanelD) == 56 * sizeof(double), "PanelD layout");
 

@@ -135,6 +135,31 @@ __device__ __forceinline__ bool log_pair_d(const PanelD* __restrict__ p, P3 x, d
   return d * d * tol < 0.6 * p->radius * p->radius;
 }
 
+// int log|xi - x| dS over the plane polygon of p for x in its plane: the polygon as the signed
+// triangles (x, edge) (panel_integrals.rs:621-648), each in polar coordinates about x: with h the
+// distance of x from the edge's line (> 0 inside) and t the coordinate along the edge from the
+// foot of the perpendicular, r = sqrt(h^2 + t^2) on the edge, and
+//   int int log r r dr dtheta = (h / 4) int (log(h^2 + t^2) - 1) dt = (h / 2) [F(t2) - F(t1)],
+//   F(t) = (t log(h^2 + t^2) - 2 t + 2 h atan(t / h)) / 2 - t / 2.
+__device__ __forceinline__ double log_in_plane(const PanelD* __restrict__ p, P3 x) {
+  const int nv = (int)p->nv;
+  double sum = 0.0;
+  for (int i = 0; i < nv; ++i) {
+    const P3 a = ld3(p->v[i]), b = ld3(p->v[(i + 1) % nv]);
+    const double h = dot3(sub(a, x), ld3(p->em[i]));
+    if (h == 0.0) continue;
+    const double il = 1.0 / p->elen[i];
+    const P3 e = mk(il * (b.x - a.x), il * (b.y - a.y), il * (b.z - a.z));
+    const double t1 = dot3(sub(a, x), e), t2 = dot3(sub(b, x), e);
+    auto F = [h](double t) {
+      const double q = h * h + t * t;
+      return 0.5 * (t * log(q) - 2.0 * t + 2.0 * h * atan(t / h)) - 0.5 * t;
+    };
+    sum += 0.5 * h * (F(t2) - F(t1));
+  }
+  return sum;
+}
+
 // The record of the pair (field point x, panel p): returns whether its log corrections need the
 // adaptive rule (then dl, dd are left 0 for adaptive_kernel). sinf, dinf: the integrals of
 // 1/r - 1/r' (the omega = infinity row, infinite depth).
@@ -163,7 +188,14 @@ __device__ __forceinline__ bool pair_statics(const PanelD* __restrict__ p, P3 x,
   }
   e.dl = 0.0;
   e.dd = 0.0;
-  if (logint && log_pair_d(p, x, tol)) {
+  if (logint && log_pair_d(p, x, tol) && p->interior != 0.0 && x.z == 0.0) {
+    // both in z = 0 (lo_static.rs:104-113): psi = log |xi - x| integrated exactly in the plane;
+    // at the panel's own centroid the centroid value is part of the kernel's limit; the columns
+    // of interior panels are K times their sources, so dd is not used
+    const double L = log_in_plane(p, x);
+    const double Rr = nrm(sub(ld3(p->c), x));
+    e.dl = Rr > 0.0 ? L - p->area * log(Rr) : L;
+  } else if (logint && log_pair_d(p, x, tol)) {
     const int order = log_order(p, xb);
     if (order > 0) {
       const double* xs = gl + order * 32;

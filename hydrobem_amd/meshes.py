@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["box_panels", "hemisphere_panels", "cylinder_panels", "expand_symmetry", "place"]
+__all__ = ["box_panels", "hemisphere_panels", "cylinder_panels", "expand_symmetry", "place", "interior_graded"]
 
 # the order of a quadrilateral with its orientation reversed (the first vertex kept)
 _REVERSED = [0, 3, 2, 1]
@@ -142,3 +142,101 @@ def expand_symmetry(quadrant):
 def place(vertices, position=(0.0, 0.0, 0.0)):
     """The panels translated to a body position."""
     return np.asarray(vertices, dtype=float) + np.asarray(position, dtype=float)
+
+
+def _waterline(vertices, z0, tol):
+    """The waterline segments of the wetted panels: the edges in z = z0 of the non-horizontal
+    panels, each oriented with the body on its left seen from above (the normals point out of the
+    body), as [(p, q)] of 2D points."""
+    segs = []
+    for quad in np.asarray(vertices, dtype=float):
+        n = np.cross(quad[2] - quad[0], quad[3] - quad[1])
+        if np.hypot(n[0], n[1]) <= 1e-6 * np.linalg.norm(n):
+            continue  # horizontal: no waterline edge
+        for i in range(4):
+            a, b = quad[i], quad[(i + 1) % 4]
+            if abs(a[2] - z0) > tol or abs(b[2] - z0) > tol or np.hypot(*(b - a)[:2]) <= tol:
+                continue
+            p, q = a[:2].copy(), b[:2].copy()
+            d = q - p
+            if -d[1] * -n[0] + d[0] * -n[1] < 0.0:  # the body (-n) on the left of p -> q
+                p, q = q, p
+            segs.append((p, q))
+    return segs
+
+
+def interior_graded(vertices, sym_x=False, sym_y=False, z0=0.0):
+    """The graded interior free surface of a body (interior.rs:437-515, theory §5.1): the
+    waterplane region inside the waterline of the wetted panels `vertices` [P][4][3] (the stored
+    sector: x >= 0 where sym_x, y >= 0 where sym_y), closed along those symmetry planes, is fanned
+    from its area centroid (moved onto each symmetry plane the region touches) to every waterline
+    segment; the rays are divided at t_k = sin(pi k / (2 M)), k = 0..M, denser towards the
+    waterline, with M = ceil(mean ray length / mean segment length); each segment gives one
+    triangle at the centroid and M - 1 quadrilaterals, which together cover the polygonal
+    waterplane exactly. Returns the panels [m][4][3] in z = z0, counter-clockwise seen from above
+    (normals up; a triangle repeats its third vertex), segment by segment along the waterline. One
+    region without openings, star-shaped from its centroid; otherwise ValueError."""
+    v = np.asarray(vertices, dtype=float).reshape(-1, 4, 3)
+    extent = max(1.0, float(np.abs(v).max()))
+    tol = 1e-9 * extent
+    segs = _waterline(v, z0, tol)
+    if not segs:
+        raise ValueError("interior surface 'graded': no waterline")
+    # chain the segments (end of one = start of the next); an open chain of a sector runs from one
+    # symmetry plane to another
+    key = lambda p: (round(p[0] / tol), round(p[1] / tol))
+    start = {key(p): i for i, (p, q) in enumerate(segs)}
+    ends = {key(q) for p, q in segs}
+    heads = [i for i, (p, q) in enumerate(segs) if key(p) not in ends]
+    if len(heads) > 1:
+        raise ValueError("interior surface 'graded': the waterline is not one chain")
+    i = heads[0] if heads else 0
+    order, seen = [], set()
+    while i is not None and i not in seen:
+        seen.add(i)
+        order.append(i)
+        i = start.get(key(segs[i][1]))
+    if len(order) != len(segs):
+        raise ValueError("interior surface 'graded': the waterline is not one chain")
+    chain = [segs[i] for i in order]
+    poly = [p for p, q in chain] + [chain[-1][1]]
+    closed = np.linalg.norm(poly[0] - poly[-1]) <= tol
+    if closed:
+        poly = poly[:-1]
+    else:
+        a, b = poly[0], poly[-1]
+        on = lambda p, ax: (ax == 0 and sym_x and abs(p[0]) <= tol) or (ax == 1 and sym_y and abs(p[1]) <= tol)
+        if not any(on(a, ax) for ax in (0, 1)) or not any(on(b, ax) for ax in (0, 1)):
+            raise ValueError("interior surface 'graded': the waterline does not close")
+        if not any(on(a, ax) and on(b, ax) for ax in (0, 1)):
+            poly.append(np.zeros(2))  # along both symmetry planes through the origin
+    P = np.array(poly)
+    Q = np.roll(P, -1, axis=0)
+    cr = P[:, 0] * Q[:, 1] - P[:, 1] * Q[:, 0]
+    area = 0.5 * cr.sum()
+    c = np.array([((P[:, 0] + Q[:, 0]) * cr).sum(), ((P[:, 1] + Q[:, 1]) * cr).sum()]) / (6.0 * area)
+    for p, q in zip(P, Q):  # the closing edges on the symmetry planes
+        if sym_x and abs(p[0]) <= tol and abs(q[0]) <= tol:
+            c[0] = 0.0
+        if sym_y and abs(p[1]) <= tol and abs(q[1]) <= tol:
+            c[1] = 0.0
+    fan, ray, seg = [], 0.0, 0.0
+    for p, q in chain:
+        s = (p[0] - c[0]) * (q[1] - c[1]) - (p[1] - c[1]) * (q[0] - c[0])
+        if abs(s) <= tol * np.linalg.norm(q - p):
+            continue
+        if s < 0.0:
+            raise ValueError("interior surface 'graded': a ray from the centroid meets the waterline twice")
+        fan.append((p, q))
+        ray += 0.5 * (np.linalg.norm(c - p) + np.linalg.norm(c - q))
+        seg += np.linalg.norm(q - p)
+    M = max(1, int(np.ceil(ray / seg - 1e-9)))
+    t = np.array([np.sin(0.5 * np.pi * k / M) for k in range(M)] + [1.0])
+    at = lambda p, s: np.array([c[0] + s * (p[0] - c[0]), c[1] + s * (p[1] - c[1]), z0])
+    c3 = np.array([c[0], c[1], z0])
+    out = []
+    for p, q in fan:
+        out.append([c3, at(p, t[1]), at(q, t[1]), at(q, t[1])])
+        for k in range(1, M):
+            out.append([at(p, t[k]), at(p, t[k + 1]), at(q, t[k + 1]), at(q, t[k])])
+    return np.array(out)

@@ -105,6 +105,18 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
 HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
                          int32_t n_bodies, const double* rotation_centres);
 
+/* The artificial interior free surface of the bodies (irregular-frequency removal; the
+ * SurfaceKind::Interior panels of lo_geometry.rs:598-615, interior.rs:519-570): panels in z = 0
+ * inside the waterlines, vertices [n_panels][4][3] in the stored symmetry form of the body panels
+ * (z within 1e-9 of 0, set to 0; normals taken upward), body [n_panels] (NULL: body 0); they follow
+ * the body panels as extra unknowns: free term -4 pi, columns K S (lo_frequency.rs:85-97,
+ * low_order.rs:771-785), no velocity, no forces. n_panels 0 removes them; hbg_set_panels removes
+ * them too. Infinite depth and ordinary frequencies only (the solve refuses finite depth, the
+ * frequency limits, field points and Kochin functions, the surrogate, walls and group solves with
+ * them with HBG_ERR_UNSUPPORTED; HBG_PRECISION_MIXED32 solves them as HBG_PRECISION_MIXED).
+ * Invalidates the statics. */
+HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body);
+
 /* Symmetry planes of the wetted surface (the mesh file's ISX, ISY; mesh.rs symmetry "yz", "xz",
  * "xz_and_yz"): sym_x 1 reflects the panels about x = 0, sym_y 1 about y = 0 (each 0 or 1). The
  * panels given to hbg_set_panels are then one sector; the solve adds their images (geometry.rs:

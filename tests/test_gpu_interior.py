@@ -1,0 +1,126 @@
+"""The interior free surface (irregular-frequency removal; GPU):
+- the reference's regression baseline lo_cylinder_interior.npz (tests/data/regression of the
+  reference, copied into tests/golden/, loaded with allow_pickle=False): the quadrant cylinder of
+  tests/python/regression_problems.py (radius 1, draft 0.5, 8 x 4 side and 8 x 4 bottom panels,
+  symmetry planes x = 0 and y = 0) with its graded interior surface, rho 1000, g 9.80665,
+  omega = sqrt(g), sqrt(2.5 g), headings 0 and 30 degrees: A, B and the excitations at the rules
+  of the other regression baselines (1e-5 of each array's peak, 5e-5 of the per-entry scale;
+  test_gpu_regression.py); its field-point velocities are not an output of this path;
+- the quadrant with its interior surface and the two symmetry planes against the whole body with
+  the whole interior surface (FP64 and mixed);
+- the irregular frequency itself: at the first one of the cylinder (K a = j01 coth(j01 d / a),
+  about 2.88) the added mass without the interior surface departs from the smooth curve through
+  its neighbours, with it it does not.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import hydrobem_amd as hb
+from conftest import GOLDEN
+from hydrobem_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+RHO, G = 1000.0, 9.80665
+PEAK_TOL, ENTRY_TOL = 1e-5, 5e-5
+KEYS = ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_froude_krylov",
+        "excitation_scattering", "excitation_haskind")
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def errors(a, b):
+    """(max |a - b| / max |b|, max |a - b| / per-entry scale) (test_regression.py:25-75)."""
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b).astype(np.complex128)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    peak = np.abs(b).max()
+    scale = np.maximum(np.abs(b).max(axis=0, keepdims=True), 1e-3 * peak)
+    d = np.abs(a - b)
+    return float(d.max() / peak), float((d / scale).max())
+
+
+def quadrant_cylinder():
+    return meshes.cylinder_panels(1.0, 0.5, n_phi=8, n_z=4, n_r=4, quarter=True)
+
+
+def test_graded_interior_of_the_quadrant_cylinder_layout():
+    # 8 waterline segments, rays of mean length 1 over segments of 2 sin(pi / 32): M = 6 cells a ray
+    q = quadrant_cylinder()
+    inner = meshes.interior_graded(q, True, True)
+    assert inner.shape == (48, 4, 3) and np.all(inner[:, :, 2] == 0.0)
+    n = np.cross(inner[:, 2] - inner[:, 0], inner[:, 3] - inner[:, 1])
+    assert np.all(n[:, 2] > 0.0)
+    area = 0.5 * n[:, 2].sum()
+    assert abs(area - 4.0 * np.sin(np.pi / 16)) < 1e-12  # the polygonal quarter waterplane
+
+
+def test_interior_cylinder_against_the_reference_baseline():
+    ref = np.load(GOLDEN / "lo_cylinder_interior.npz", allow_pickle=False)
+    q = quadrant_cylinder()
+    inner = meshes.interior_graded(q, True, True)
+    say(f"case: lo_cylinder_interior: quadrant {len(q)} + {len(inner)} interior panels, both planes, "
+        f"omega {ref['omegas']}, headings {ref['headings']}")
+    with hb.Session(0) as s:
+        s.set_environment(RHO, G)
+        s.set_symmetry(True, True)
+        s.set_panels(q)
+        s.set_interior(inner)
+        res = s.solve(ref["omegas"], ref["headings"])
+    for key in KEYS:
+        pk, pe = errors(res[key], ref[key])
+        say(f"  {key}: {pk:.2e} of the peak, {pe:.2e} of the per-entry scale")
+        assert pk <= PEAK_TOL and pe <= ENTRY_TOL, (key, pk, pe)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_interior_quadrant_against_the_whole(precision):
+    q = quadrant_cylinder()
+    qi = meshes.interior_graded(q, True, True)
+    w_body = meshes.expand_symmetry(q)
+    w_inner = meshes.interior_graded(w_body)
+    assert len(w_inner) == 4 * len(qi)
+    w = np.sqrt(G * np.array([1.0, 2.5, 2.88]))
+    heads = [0.0, 30.0]
+    say(f"case: interior cylinder, quadrant with planes vs whole ({len(w_body)} + {len(w_inner)} panels), {precision}")
+    with hb.Session(0) as a, hb.Session(0) as b:
+        for s in (a, b):
+            s.set_environment(RHO, G)
+        a.set_symmetry(True, True)
+        a.set_panels(q)
+        a.set_interior(qi)
+        b.set_panels(w_body)
+        b.set_interior(w_inner)
+        got, ref = a.solve(w, heads, precision=precision), b.solve(w, heads, precision=precision)
+    for key in KEYS:
+        pk, pe = errors(got[key], ref[key])
+        say(f"  {key}: quadrant vs whole {pe:.2e}")
+        assert pe < 1e-9
+
+
+def test_interior_surface_removes_the_irregular_frequency():
+    # first irregular frequency of the truncated cylinder (radius 1, draft 0.5): the interior
+    # Dirichlet eigenvalue K = j01 coth(j01 d) with j01 the first zero of J0
+    j01 = 2.404825557695773
+    K_irr = j01 / np.tanh(j01 * 0.5)
+    body = meshes.cylinder_panels(1.0, 0.5, n_phi=32, n_z=8, n_r=8, quarter=False, cosine=True)
+    inner = meshes.interior_graded(body)
+    Ks = K_irr + np.array([-0.3, 0.0, 0.3])
+    w = np.sqrt(G * Ks)
+    say(f"case: cylinder {len(body)} panels, added mass at K = {Ks} (irregular {K_irr:.4f}), without and "
+        f"with {len(inner)} interior panels")
+    with hb.Session(0) as s:
+        s.set_environment(RHO, G)
+        s.set_panels(body)
+        plain = s.solve(w, [0.0])["added_mass"][:, 2, 2]
+        s.set_interior(inner)
+        removed = s.solve(w, [0.0])["added_mass"][:, 2, 2]
+    # the departure of the middle value from the chord through its neighbours, of the values' size
+    dev = lambda a: abs(a[1] - 0.5 * (a[0] + a[2])) / np.abs(a).max()
+    say(f"  heave added mass without {plain}, with {removed}: departures {dev(plain):.2e}, {dev(removed):.2e}")
+    assert dev(removed) < 0.02 and dev(plain) > 5.0 * dev(removed)
