@@ -1540,8 +1540,6 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   s->d_M.ensure((size_t)N * N);
   const double K = omega * omega / s->grav;
   check_depth(s);
-  if (s->NI > 0 && std::isfinite(s->depth))
-    throw Error{HBG_ERR_UNSUPPORTED, "the interior free surface in finite depth"};
   if (m32) {
     // the matrices of the FP32 assembly: its complex64 entries, the FP64 values at the near entries
     if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with the surrogate"};
@@ -1754,8 +1752,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   if (!il.empty() && std::isfinite(s->depth))
     throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
   if (s->NI > 0) {
-    // the interior free surface: infinite depth, ordinary frequencies, the forces
-    if (std::isfinite(s->depth)) throw Error{HBG_ERR_UNSUPPORTED, "the interior free surface in finite depth"};
+    // the interior free surface: ordinary frequencies, the forces
     if (!il.empty()) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with an interior free surface"};
     if (s->np > 0 || s->nk > 0)
       throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions with an interior free surface"};

@@ -173,13 +173,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
     const double R = sqrt(dx * dx + dy * dy);
     const FinK w = fin_kernel_u(f, F, T, valid, k, n, R, pk->c[2], pn->c[2]);
     if (valid) {
+      const bool ik = pk->interior != 0.0, in = pn->interior != 0.0;  // on the interior free surface
       const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
       put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi,
-                w.WiR, w.WiZ, free_term);
+                w.WiR, w.WiZ, free_term, ik, in, f->K);
       if (k != n) {
         const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
         put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi,
-                  w.WiR, w.Wiz, free_term);
+                  w.WiR, w.Wiz, free_term, in, ik, f->K);
       }
     }
   }

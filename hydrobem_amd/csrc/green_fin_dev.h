@@ -224,9 +224,14 @@ __device__ __forceinline__ FinK fin_kernel(const FinFreq* __restrict__ fp, const
     double f1[3], f2[3];
     phi_tile(F.phi1, FTW, 1.0, R * ih, v1 * ih, f1);
     phi_tile(F.phi2, FTV, 2.0, R * ih, v2 * ih, f2);
-    const WaveK w = wave_kernel(T, K * R, K * v3);
+    WaveK w = wave_kernel(T, K * R, K * v3);
     const double X = K * R, Y = K * v3;
-    const double FY = -w.F - 2.0 / sqrt(X * X + Y * Y);
+    double FY = -w.F - 2.0 / sqrt(X * X + Y * Y);
+    if (R == 0.0 && v3 == 0.0) {  // coincident points on z = 0: the limit of Ft + 2 psi (green_fin.rs:816-822)
+      w.F = -2.0 * (GAMMA - LOG2 + log(K));
+      w.FX = 0.0;
+      FY = 0.0;
+    }
     o.Wr = ih * (f1[0] + f2[0]) + K * w.F;
     o.WrR = ih2 * (f1[1] + f2[1]) + K * K * w.FX;
     const double A1z = -sg * ih2 * f1[2];
@@ -356,8 +361,15 @@ __device__ __forceinline__ FinK fin_kernel_u(const FinFreq* __restrict__ fp, con
       })
     }
     if (near) {
-      const WaveK w = wave_kernel_tab(X, Y, wj0, wj1, wyt, wdyt, V, VX);
-      const double FY = -w.F - 2.0 / sqrt(X * X + Y * Y);
+      WaveK w = wave_kernel_tab(X, Y, wj0, wj1, wyt, wdyt, V, VX);
+      double FY = -w.F - 2.0 / sqrt(X * X + Y * Y);
+      if (R == 0.0 && v3 == 0.0) {
+        // coincident points on z = 0 (an interior panel and its own centroid): the limit of
+        // Ft + 2 psi, psi = log((Y + R0) / K), and no derivatives (green_fin.rs:816-822)
+        w.F = -2.0 * (GAMMA - LOG2 + log(K));
+        w.FX = 0.0;
+        FY = 0.0;
+      }
       Wr = ih * (f1[0] + f2[0]) + K * w.F;
       WrR = ih2 * (f1[1] + f2[1]) + K * K * w.FX;
       const double A1z = -sg * ih2 * f1[2];

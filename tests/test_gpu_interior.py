@@ -10,7 +10,10 @@
   the whole interior surface (FP64 and mixed);
 - the irregular frequency itself: at the first one of the cylinder (K a = j01 coth(j01 d / a),
   about 2.88) the added mass without the interior surface departs from the smooth curve through
-  its neighbours, with it it does not.
+  its neighbours, with it it does not; the same for the 8192-panel workload's body in depth 3
+  (nu h about 7.3), coarser;
+- finite depth: the interior surface on water 60 radii deep against the infinite-depth solve that
+  the baseline pins.
 """
 
 import sys
@@ -121,6 +124,52 @@ def test_interior_surface_removes_the_irregular_frequency():
         s.set_interior(inner)
         removed = s.solve(w, [0.0])["added_mass"][:, 2, 2]
     # the departure of the middle value from the chord through its neighbours, of the values' size
+    dev = lambda a: abs(a[1] - 0.5 * (a[0] + a[2])) / np.abs(a).max()
+    say(f"  heave added mass without {plain}, with {removed}: departures {dev(plain):.2e}, {dev(removed):.2e}")
+    assert dev(removed) < 0.02 and dev(plain) > 5.0 * dev(removed)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_interior_in_deep_finite_depth_against_infinite_depth(precision):
+    # the finite-depth kernel (Newman form near the axis, with its own coincident-point limit) on
+    # water 60 radii deep against the infinite-depth solve pinned by the reference baseline: the
+    # bottom's effect there is e^{-2 K h} (below 1e-50)
+    q = quadrant_cylinder()
+    inner = meshes.interior_graded(q, True, True)
+    w = np.sqrt(G * np.array([1.0, 2.5]))
+    heads = [0.0, 30.0]
+    say(f"case: interior cylinder (quadrant, planes), depth 60 vs infinite depth, {precision}")
+    out = []
+    for depth in (60.0, np.inf):
+        with hb.Session(0) as s:
+            s.set_environment(RHO, G, depth)
+            s.set_symmetry(True, True)
+            s.set_panels(q)
+            s.set_interior(inner)
+            out.append(s.solve(w, heads, precision=precision))
+    for key in KEYS:
+        pk, pe = errors(out[0][key], out[1][key])
+        say(f"  {key}: depth 60 vs infinite {pe:.2e}")
+        assert pe < 1e-6
+
+
+def test_interior_removes_the_irregular_frequency_in_finite_depth():
+    # the 8192-panel workload's body (radius 1, draft 1) in depth 3, coarser: its first irregular
+    # frequency, K = j01 coth(j01 d) (nu h about 7.3), with and without the interior surface
+    j01 = 2.404825557695773
+    K_irr = j01 / np.tanh(j01 * 1.0)
+    body = meshes.cylinder_panels(1.0, 1.0, 32, 8, 8, quarter=False, cosine=True)
+    inner = meshes.interior_graded(body)
+    Ks = K_irr + np.array([-0.3, 0.0, 0.3])
+    w = np.sqrt(G * Ks)
+    say(f"case: cylinder {len(body)} panels, depth 3, K = {Ks} (irregular {K_irr:.4f}, nu h {3 * K_irr:.2f}), without "
+        f"and with {len(inner)} interior panels")
+    with hb.Session(0) as s:
+        s.set_environment(RHO, G, 3.0)
+        s.set_panels(body)
+        plain = s.solve(w, [0.0])["added_mass"][:, 2, 2]
+        s.set_interior(inner)
+        removed = s.solve(w, [0.0])["added_mass"][:, 2, 2]
     dev = lambda a: abs(a[1] - 0.5 * (a[0] + a[2])) / np.abs(a).max()
     say(f"  heave added mass without {plain}, with {removed}: departures {dev(plain):.2e}, {dev(removed):.2e}")
     assert dev(removed) < 0.02 and dev(plain) > 5.0 * dev(removed)
