@@ -302,6 +302,10 @@ def main():
                     help="paths timed; `value` is the fastest whose A, B, X are within 1e-5 of the FP64 path's")
     ap.add_argument("--no-autotune", action="store_true", help="skip the LU autotuner before each path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nn-rmin", type=float, default=0.1,
+                    help="the surrogate takes the pairs with nn-rmin <= R/h <= 20 (mixed_nn path)")
+    ap.add_argument("--nn-sweep", default="",
+                    help="comma-separated lower R/h bounds: the mixed_nn path once at each, errors against fp64")
     ap.add_argument("--symmetry", action="store_true",
                     help="solve the body as its quadrant with the symmetry planes x = 0 and y = 0 (not the "
                          "headline workload; the surrogate path is not available there)")
@@ -347,6 +351,7 @@ def main():
         s.set_tuning(s.TUNE_LU_GEMM, a.gemm)
     if a.depth >= 0:
         s.set_tuning(s.TUNE_LU_DEPTH, a.depth)
+    s.set_tuning(s.TUNE_NN_RMIN, int(round(100 * a.nn_rmin)))
     print(f"[bench] workload {w['name']}: {N} panels, {F} frequencies", file=sys.stderr, flush=True)
     paths = [p for p in a.paths.split(",") if p and not (a.symmetry and p == "mixed_nn")]
     peaks = {"fp64": PEAK_FP64, "mixed": PEAK_FP32, "mixed32": PEAK_FP32, "mixed_nn": PEAK_FP32}
@@ -380,6 +385,20 @@ def main():
         gvar = s.get_tuning(s.TUNE_LU_GEMM if p == "fp64" else s.TUNE_LU_GEMM_MIXED)
         res[p] = (out, summary(prof, dt, a.steps, F, PEAK_FP16 if VARIANTS[gvar][2] else peaks[p]))
         res[p][1]["gemm_variant"] = gvar
+    sweep = []
+    if a.nn_sweep and "fp64" in res and "mixed_nn" in paths:
+        # where the network's pairs end (its lower R/h bound) against what it costs and saves: one
+        # timed step at each bound, the errors against the plain FP64 results
+        for r in [float(x) for x in a.nn_sweep.split(",") if x]:
+            s.set_tuning(s.TUNE_NN_RMIN, int(round(100 * r)))
+            o_r, dt_r, prof_r = run_path(s, w, "mixed_nn", 1, 0, f"mixed_nn R/h >= {r}")
+            sm = summary(prof_r, dt_r, 1, F, PEAK_FP32)
+            sweep.append({"nn_rmin": r, "freq_per_s": sm["freq_per_s"],
+                          "assembly_ms": sm["stage_ms_per_frequency"]["assembly_ms"],
+                          "rel_err_vs_fp64_gpu": {k: rel_err(o_r[k], res["fp64"][0][k]) for k in
+                                                  ("added_mass", "radiation_damping", "excitation_diffraction")}})
+            print(f"[bench] nn sweep: {sweep[-1]}", file=sys.stderr, flush=True)
+        s.set_tuning(s.TUNE_NN_RMIN, int(round(100 * a.nn_rmin)))
     s.close()
     # `value`: the fastest path whose A, B and X stay within 1e-5 of the plain FP64 results (which
     # match the FP64 oracle to 1e-10); the others are reported beside it with their errors
@@ -399,6 +418,9 @@ def main():
         config[p] = res[p][1]
     if "mixed_nn" in paths:
         config["surrogate"] = config_nn
+        config["surrogate"]["nn_rmin"] = a.nn_rmin
+    if sweep:
+        config["mixed_nn_rmin_sweep"] = sweep
     if tuning:
         config["autotuned_lu"] = tuning
     if "fp64" in res and len(paths) > 1:

@@ -353,6 +353,7 @@ class Session:
     TUNE_LU_GEMM_MIXED = 3
     TUNE_LU_DEPTH_MIXED = 4
     TUNE_MIXED32_TOL = 5
+    TUNE_NN_RMIN = 6
 
     def autotune(self, n, nrhs, precision="fp64") -> list:
         """Time the batched LU variants on synthetic systems and keep the fastest (hbg_autotune);

@@ -132,7 +132,7 @@ size_t residual_max_elems(int n);
 size_t residual_part_elems(int n, int R);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                             const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
-                            int2* queue, unsigned* qcount, hipStream_t stream);
+                            int2* queue, unsigned* qcount, double rmin, hipStream_t stream);
 void launch_remainder(const FinFreq* f, FinTables F, WaveTables T, int n, const double* R, const double* z,
                       const double* zeta, double* out, hipStream_t stream);
 void nn_eval(const NnDevice& nn, const float4* X, int n, float4* Y, hipStream_t stream);
@@ -431,6 +431,7 @@ struct HbgSession {
   DevBuf<int> d_active;  // systems still refined (mixed precision)
   DevBuf<double2> d_rpart2;
   NnDevice nn;          // the learned surrogate of the finite-depth evanescent remainder
+  double nn_rmin = NN_RMIN;  // the network takes the pairs with nn_rmin <= R/h <= 20 (HBG_TUNE_NN_RMIN)
   DevBuf<int2> d_nnq;   // pairs left to the FP64 evaluator by the surrogate's assembly
   DevBuf<unsigned> d_nnc;
   int max_batch = 0;  // frequencies in flight (0: as many as fit, at most 48)
@@ -667,7 +668,7 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
     s->d_nnq.ensure((size_t)N * N);
     s->d_nnc.ensure(1);
     launch_assemble_fin_nn(s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, lda,
-                           s->nn, s->d_nnq.p, s->d_nnc.p, s->stream);
+                           s->nn, s->d_nnq.p, s->d_nnc.p, s->nn_rmin, s->stream);
   } else
     blocks([&](double2* S, double2* Ab, float2* a32, int images) {
       launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32, lda,
@@ -2615,6 +2616,11 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
                                                "hbg_set_tuning");
       g_mixed32_tol_exp = value;
       return HBG_OK;
+    case HBG_TUNE_NN_RMIN:
+      if (value < 10 || value > 2000)
+        return fail(HBG_ERR_VALUE, "the surrogate's lower R/h bound must be in 10..2000 hundredths", "hbg_set_tuning");
+      s->nn_rmin = value / 100.0;
+      return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_set_tuning");
   }
@@ -2640,6 +2646,9 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value) {
       return HBG_OK;
     case HBG_TUNE_MIXED32_TOL:
       *value = g_mixed32_tol_exp;
+      return HBG_OK;
+    case HBG_TUNE_NN_RMIN:
+      *value = (int32_t)std::lround(s->nn_rmin * 100.0);
       return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_get_tuning");

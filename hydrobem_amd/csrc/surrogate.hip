@@ -372,7 +372,8 @@ __global__ void __launch_bounds__(256) nn_eval_kernel(const v8bf* __restrict__ g
 }
 
 // The finite-depth S and A of one frequency (as assemble_fin_kernel in frequency.hip) with the
-// evanescent remainder from the network where NN_RMIN <= R/h <= 20; the other pairs are queued
+// evanescent remainder from the network where rmin <= R/h <= 20 (rmin >= NN_RMIN, the training
+// domain's lower end, 0.1 by default); the other pairs are queued
 // (wave-aggregated appends) for fallback_kernel, which evaluates them in FP64. As in frequency.hip
 // each unordered pair is taken once (64 x 64 tiles with k-block <= n-block, the diagonal tiles
 // the pairs k <= n) and both entries are written: the network's third output is the remainder's
@@ -412,7 +413,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __restrict__ f, FinTables F, WaveTables T,
                        double logc, const StaticD* __restrict__ st, double2* __restrict__ S, double2* __restrict__ A,
                        float2* __restrict__ A32, int lda, const v8bf* __restrict__ gpk, const float* __restrict__ gbias,
-                       int2* __restrict__ queue, unsigned* __restrict__ qcount) {
+                       int2* __restrict__ queue, unsigned* __restrict__ qcount, double rmin) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;  // workgroup-uniform
   __shared__ v8bf pk[NN_PK_V8];
@@ -432,7 +433,7 @@ assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __res
     const PanelD* pn = P + n;
     const double dx = xk - pn->c[0], dy = yk - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy), r = R * ih;
-    const bool nn = r >= NN_RMIN && r <= 20.0;
+    const bool nn = r >= rmin && r <= 20.0;
     const float4 u = nn_input(nn ? r : 1.0, zk * ih, pn->c[2] * ih, nuh);
     const float4 y = nn_wave(pk, bias, u);  // every lane takes part
     const unsigned long long fb = __ballot(live && !nn);
@@ -883,12 +884,12 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
 // queue: N (N + 1) / 2 pairs of room (every pair may fall back), qcount: one counter
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                             const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
-                            int2* queue, unsigned* qcount, hipStream_t stream) {
+                            int2* queue, unsigned* qcount, double rmin, hipStream_t stream) {
   zero_u32_kernel<<<1, 64, 0, stream>>>(qcount);
   const unsigned nt = (unsigned)((N + 63) / 64);
   dim3 grid(nt, nt);
   assemble_fin_nn_kernel<<<grid, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda, (const v8bf*)nn.pk,
-                                                   nn.bias, queue, qcount);
+                                                   nn.bias, queue, qcount, std::max(rmin, NN_RMIN));
   fallback_kernel<<<2048, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda, queue, qcount);
   HBG_HIP(hipGetLastError());
 }

@@ -345,7 +345,9 @@ enum {
   HBG_TUNE_LU_GRAPHS = 2,
   HBG_TUNE_LU_GEMM_MIXED = 3,
   HBG_TUNE_LU_DEPTH_MIXED = 4,
-  HBG_TUNE_MIXED32_TOL = 5 /* HBG_PRECISION_MIXED32 refines to backward error 10^-value (6..14, default 10) */
+  HBG_TUNE_MIXED32_TOL = 5, /* HBG_PRECISION_MIXED32 refines to backward error 10^-value (6..14, default 10) */
+  HBG_TUNE_NN_RMIN = 6 /* per session: the surrogate takes the pairs with value / 100 <= R/h <= 20, the
+                          table/series evaluator the others (10..2000, default 10: the training domain) */
 };
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic

@@ -11,7 +11,7 @@ import pytest
 from scipy import special
 
 import hydrobem_amd as hb
-from hydrobem_amd import meshes
+from hydrobem_amd import meshes, train_surrogate
 from oracle import mlp
 
 pytestmark = pytest.mark.gpu
@@ -119,5 +119,36 @@ def test_short_training_learns(session):
     ref = session.finite_remainder(h, nuh / h, R, z, zeta)
     got = session.surrogate_eval(np.column_stack([R / h, z / h, zeta / h, np.full(n, nuh)]))
     err = np.abs(got - ref).max(0) / (np.abs(ref).max(0) + 1e-12)
-    say(f"  relative max error per output at nu h {nuh}: {err}")
-    assert np.all(np.isfinite(got))
+    trained = session.surrogate_get().copy()
+    session.surrogate_set(mlp.init(7).astype(np.float32))
+    got0 = session.surrogate_eval(np.column_stack([R / h, z / h, zeta / h, np.full(n, nuh)]))
+    session.surrogate_set(trained)
+    err0 = np.abs(got0 - ref).max(0) / (np.abs(ref).max(0) + 1e-12)
+    rms = np.sqrt(((got - ref) ** 2).mean(0) / (ref ** 2).mean(0))
+    rms0 = np.sqrt(((got0 - ref) ** 2).mean(0) / (ref ** 2).mean(0))
+    say(f"  relative max error per output at nu h {nuh}: {err}; an untrained network: {err0}")
+    say(f"  relative RMS error per output: {rms}; an untrained network: {rms0}")
+    # 291 steps: the outputs together far closer to the evaluator than an untrained network's, in
+    # the RMS the loss measures (measured 0.14-0.61 per output against 0.37-3.7 untrained)
+    assert np.sum(rms ** 2) < 0.25 * np.sum(rms0 ** 2)
+    assert rms.max() < 0.8
+
+
+@pytest.mark.skipif(not train_surrogate.WEIGHTS.exists(), reason="no trained surrogate shipped")
+def test_shipped_network_held_out(session):
+    """The shipped network against the FP64 evaluator at fresh points of its domain: the held-out
+    judge of the training, at three depths and frequencies."""
+    say("case: shipped surrogate, held-out points against the FP64 evaluator")
+    train_surrogate.load(session)
+    rng = np.random.default_rng(21)
+    for h, nuh in ((1.0, 1.3), (3.0, 0.1), (3.0, 10.0)):
+        n = 2000
+        R = h * np.exp(rng.uniform(np.log(0.1), np.log(20), n))
+        z, zeta = -h * rng.uniform(0, 1, n), -h * rng.uniform(0, 1, n)
+        ref = session.finite_remainder(h, nuh / h, R, z, zeta)
+        got = session.surrogate_eval(np.column_stack([R / h, z / h, zeta / h, np.full(n, nuh)]))
+        err = np.abs(got - ref).max(0) / np.abs(ref).max(0)
+        say(f"  h {h}, nu h {nuh}: max error / max |output| {err}")
+        # measured on MI355X: 0.5-7 % per output (BF16 weights and activations), the floor that
+        # keeps the surrogate's A, B, X about a per cent off the evaluator's
+        assert np.all(err < 0.1)

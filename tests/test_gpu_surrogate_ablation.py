@@ -1,16 +1,20 @@
 """What the trained surrogate gains over the method without it: the same finite-depth solve with the
-FP64 evaluator and with the network (GPU). Uses the trained parameters shipped with the package."""
+FP64 evaluator and with the network (GPU), the network confined to R/h ranges of its training
+domain, and networks that are deliberately wrong. Uses the trained parameters shipped with the
+package."""
 
 import sys
-import time
 
 import numpy as np
 import pytest
 
 import hydrobem_amd as hb
 from hydrobem_amd import meshes, train_surrogate
+from oracle import mlp
 
 pytestmark = pytest.mark.gpu
+
+KEYS = ("added_mass", "radiation_damping", "excitation_diffraction")
 
 
 def say(*a):
@@ -24,8 +28,8 @@ def rel_per_entry(a, b):
     return float((np.abs(np.asarray(a) - b) / scale).max())
 
 
-@pytest.mark.skipif(not train_surrogate.WEIGHTS.exists(), reason="no trained surrogate shipped")
-def test_surrogate_ablation_cylinder_finite_depth():
+@pytest.fixture(scope="module")
+def case():
     say("case: ablation, cylinder 4096 panels, depth 3 m, 8 frequencies nu h 0.05..20")
     s = hb.Session(0)
     rep = train_surrogate.load(s)
@@ -34,28 +38,82 @@ def test_surrogate_ablation_cylinder_finite_depth():
     s.set_environment(1025.0, 9.81, 3.0)
     s.set_panels(v)
     w = np.sqrt(9.81 * np.logspace(np.log10(0.05), np.log10(20.0), 8) / 3.0)
-    times = {}
-    outs = {}
-    asm = {}
-    for label, kw in (("fp64", {}), ("fp64+nn", {"surrogate": True})):
+
+    def run(**kw):
         s.solve(w[:1], [0.0], **kw)
         s.set_profiling(True)
-        t0 = time.perf_counter()
-        outs[label] = s.solve(w, [0.0], **kw)
-        times[label] = time.perf_counter() - t0
+        out = s.solve(w, [0.0], **kw)
         prof = s.profile()
         s.set_profiling(False)
-        asm[label] = prof["assembly_ms"] / prof["frequencies"]
-        say(f"  {label}: {times[label] * 1e3:.1f} ms, assembly {asm[label]:.2f} ms/frequency")
-    errs = {k: rel_per_entry(outs["fp64+nn"][k], outs["fp64"][k])
-            for k in ("added_mass", "radiation_damping", "excitation_diffraction")}
-    say("  errors of the surrogate solve: " + ", ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+        return out, prof["assembly_ms"] / prof["frequencies"]
+
+    ref, asm_ref = run()
+    say(f"  fp64 evaluator: assembly {asm_ref:.2f} ms/frequency")
+    yield s, run, ref, asm_ref
     s.close()
-    say(f"  assembly time with the network / without: {asm['fp64+nn'] / asm['fp64']:.2f}")
-    # the network (BF16, held-out RMS ~0.5 % of the outputs' spread) costs an error of about a per
-    # cent in A, B and X; the FP64 evaluator stays the accurate path. Which assembly is faster is
-    # measured and reported, not assumed: the evaluator's table reads are wavefront-uniform
-    # (sorted pairs, scalar loads) and the network's evaluation is bound by the same pair count.
-    assert all(np.isfinite(e) for e in errs.values())
-    assert max(errs.values()) < 0.1
-    assert asm["fp64+nn"] < 2.0 * asm["fp64"]
+
+
+@pytest.mark.skipif(not train_surrogate.WEIGHTS.exists(), reason="no trained surrogate shipped")
+def test_surrogate_ablation_cylinder_finite_depth(case):
+    s, run, ref, asm_ref = case
+    s.set_tuning(s.TUNE_NN_RMIN, 10)
+    out, asm = run(surrogate=True)
+    errs = {k: rel_per_entry(out[k], ref[k]) for k in KEYS}
+    say("  errors of the surrogate solve: " + ", ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    say(f"  assembly time with the network / without: {asm / asm_ref:.2f}")
+    # measured on MI355X: 0.8-1.8 % (the BF16 network's held-out maximum error is 1-7 % of each
+    # output's range), far from the 1e-5 of the FP64 evaluator's own path
+    assert max(errs.values()) < 0.03
+    assert min(errs.values()) > 1e-4  # the network did take the pairs of its domain
+    # measured 1.6: the FP64 evaluator's sorted, scalar-load tables beat the network's ~68 kflop
+    # per pair (and its SiLU on the vector units) at this pair count
+    assert 1.0 < asm / asm_ref < 2.5
+
+
+@pytest.mark.skipif(not train_surrogate.WEIGHTS.exists(), reason="no trained surrogate shipped")
+def test_wrong_networks_are_caught(case):
+    """The ablation's error measure separates the trained network from wrong ones: random weights,
+    the output layer scaled by 1.1, its biases shifted by 0.05 (standardized units)."""
+    s, run, ref, _ = case
+    s.set_tuning(s.TUNE_NN_RMIN, 10)
+    p0 = s.surrogate_get().copy()
+    out, _ = run(surrogate=True)
+    good = max(rel_per_entry(out[k], ref[k]) for k in KEYS)
+    off = sum(int(np.prod(sh)) for _, sh in mlp.SHAPES[:-2])
+    n_w4 = int(np.prod(mlp.SHAPES[-2][1]))
+    scaled, shifted = p0.copy(), p0.copy()
+    scaled[off:off + n_w4] *= 1.1
+    shifted[off + n_w4:] += 0.05
+    try:
+        for label, p in (("random", mlp.init(3).astype(np.float32)), ("output layer x 1.1", scaled),
+                         ("output biases + 0.05", shifted)):
+            s.surrogate_set(p)
+            out, _ = run(surrogate=True)
+            bad = max(rel_per_entry(out[k], ref[k]) for k in KEYS)
+            say(f"  {label}: largest error {bad:.3g} against the trained network's {good:.3g}")
+            assert bad > 5.0 * good and bad > 0.05, label
+    finally:
+        s.surrogate_set(p0)
+
+
+@pytest.mark.skipif(not train_surrogate.WEIGHTS.exists(), reason="no trained surrogate shipped")
+def test_network_confined_to_larger_r_over_h(case):
+    """The lower R/h bound of the network's pairs: below it the FP64 evaluator. On this body (R/h <=
+    0.67 in depth 3) the error falls as the network takes fewer pairs and vanishes at the bound 1
+    (every pair then goes through the evaluator's fallback queue), while the assembly slows."""
+    s, run, ref, asm_ref = case
+    errs, times = [], []
+    for r in (0.1, 0.5, 1.0):
+        s.set_tuning(s.TUNE_NN_RMIN, int(round(100 * r)))
+        assert s.get_tuning(s.TUNE_NN_RMIN) == int(round(100 * r))
+        out, asm = run(surrogate=True)
+        errs.append(max(rel_per_entry(out[k], ref[k]) for k in KEYS))
+        times.append(asm)
+        say(f"  network for R/h >= {r}: largest error {errs[-1]:.3g}, assembly {asm:.2f} ms "
+            f"({asm / asm_ref:.2f} of the evaluator's)")
+    s.set_tuning(s.TUNE_NN_RMIN, 10)
+    assert errs[1] < errs[0]
+    assert errs[2] < 1e-10  # no pair left to the network: the FP64 evaluator's result
+    assert times[2] > times[0]
+    with pytest.raises(hb.HbgError):
+        s.set_tuning(s.TUNE_NN_RMIN, 5)  # below the training domain
