@@ -11,10 +11,12 @@ C++/OpenMP restatement of the reference's algorithm) timed on this host on a bou
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cylinder8192fd] [--paths fp64,mixed]
                     [--batch B] [--no-cpu-baseline]
     python bench.py --workload batching4    (uneven geometries through one group solve)
+    python bench.py --workload variants8    (eight hull variants, four frequencies each, one group solve)
 
-By default three paths run on the 8192-panel finite-depth cylinder (the workload the metric is
-quoted on): plain FP64, mixed (complex64 LU + FP64 refinement) and mixed+NN (the learned surrogate
-in the assembly). `value` is the throughput of the fastest path whose A, B and X stay within 1e-5
+By default four paths run on the 8192-panel finite-depth cylinder (the workload the metric is
+quoted on): plain FP64, mixed (complex64 LU + FP64 refinement), mixed+NN (the learned surrogate in
+the assembly) and mixed32 (the FP32 far-field assembly with FP64 near entries, complex64 LU, FP64
+refinement). `value` is the throughput of the fastest path whose A, B and X stay within 1e-5
 of the plain FP64 GPU results (which match the FP64 oracle to 1e-10 in the tests); every path's
 throughput, stage times, speed-up and errors are in `config`.
 """
@@ -115,13 +117,24 @@ def bench_group(a):
     import hydrobem_amd as hb
     from hydrobem_amd import meshes
 
-    bodies = [meshes.cylinder_panels(R, 1.0, *d, quarter=False, cosine=True)
-              for R, d in ((1.0, (48, 16, 16)), (0.75, (64, 22, 22)), (1.5, (96, 32, 32)))]
-    shifts = [np.array([4.0 * i, 0.0, 0.0]) for i in range(3)]
-    array = np.concatenate([b + sft for b, sft in zip(bodies, shifts)])
-    body = np.concatenate([np.full(len(b), i, np.int32) for i, b in enumerate(bodies)])
-    geoms = [(b, None, None) for b in bodies] + [(array, body, np.array(shifts))]
-    omegas = np.sqrt(9.81 * np.linspace(0.05, 4.0, 64))
+    if a.workload == "variants8":
+        # a design sweep: eight hull variants (truncated cylinders R = 0.8 .. 1.5 m, draft 1 m, 1536
+        # panels each), four frequencies each: separately, every solve is a batch of four systems
+        geoms = [(meshes.cylinder_panels(R, 1.0, 48, 16, 16, quarter=False, cosine=True), None, None)
+                 for R in np.linspace(0.8, 1.5, 8)]
+        omegas = np.sqrt(9.81 * np.array([0.5, 1.0, 1.5, 2.0]))
+        desc = ("8 truncated cylinders R=0.8..1.5 draft 1 (1536 panels each), infinite depth, 4 kR 0.5-2.0, "
+                "heading 0, mixed precision")
+    else:
+        bodies = [meshes.cylinder_panels(R, 1.0, *d, quarter=False, cosine=True)
+                  for R, d in ((1.0, (48, 16, 16)), (0.75, (64, 22, 22)), (1.5, (96, 32, 32)))]
+        shifts = [np.array([4.0 * i, 0.0, 0.0]) for i in range(3)]
+        array = np.concatenate([b + sft for b, sft in zip(bodies, shifts)])
+        body = np.concatenate([np.full(len(b), i, np.int32) for i, b in enumerate(bodies)])
+        geoms = [(b, None, None) for b in bodies] + [(array, body, np.array(shifts))]
+        omegas = np.sqrt(9.81 * np.linspace(0.05, 4.0, 64))
+        desc = ("cylinders R=1.0/0.75/1.5 draft 1 (1536/2816/6144 panels) + their array (10496 panels, 3 bodies), "
+                "infinite depth, 64 kR 0.05-4.0, heading 0, mixed precision")
     heads = [0.0]
     sessions = []
     for v, bd, rc in geoms:
@@ -135,26 +148,43 @@ def bench_group(a):
     s0.set_tuning(s0.TUNE_LU_DEPTH_MIXED, a.depth if a.depth > 0 else 8)
     print(f"[bench] batching: panels {[s.n_panels for s in sessions]}, {len(omegas)} frequencies each",
           file=sys.stderr, flush=True)
-    hb.solve_group(sessions, omegas[:2], heads, precision="mixed")  # warm-up: statics, buffers, graphs
-    t0 = time.perf_counter()
-    group = hb.solve_group(sessions, omegas, heads, precision="mixed")
-    t_group = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sep = [s.solve(omegas, heads, precision="mixed") for s in sessions]
-    t_sep = time.perf_counter() - t0
+    # warm-up: statics, buffers and the LU graphs of both forms (--group-form: one form alone, for
+    # a profiler's kernel statistics of that form)
+    do_group, do_sep = a.group_form in ("both", "group"), a.group_form in ("both", "separate")
+    group = sep = None
+    if do_group:
+        group = hb.solve_group(sessions, omegas, heads, precision="mixed")
+    if do_sep:
+        sep = [s.solve(omegas, heads, precision="mixed") for s in sessions]
+    t_group, t_sep = [float("nan")], [float("nan")]
+    for _ in range(max(a.steps, 1)):
+        if do_group:
+            t0 = time.perf_counter()
+            group = hb.solve_group(sessions, omegas, heads, precision="mixed")
+            t_group.append(time.perf_counter() - t0)
+        if do_sep:
+            t0 = time.perf_counter()
+            sep = [s.solve(omegas, heads, precision="mixed") for s in sessions]
+            t_sep.append(time.perf_counter() - t0)
+        print(f"[bench] group {t_group[-1] * 1e3:.1f} ms, separate {t_sep[-1] * 1e3:.1f} ms", file=sys.stderr,
+              flush=True)
+    t_group = min(t_group[1:]) if do_group else float("nan")
+    t_sep = min(t_sep[1:]) if do_sep else float("nan")
     err = max(rel_err(g[k], r[k]) for g, r in zip(group, sep)
-              for k in ("added_mass", "radiation_damping", "excitation_diffraction"))
+              for k in ("added_mass", "radiation_damping", "excitation_diffraction")) if group and sep else None
     systems = len(omegas) * len(sessions)
     line = {
-        "metric": "(N, omega) systems solved/s, 4 geometries of uneven panel counts in one group solve",
-        "value": round(systems / t_group, 3), "unit": "systems/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": round(1e3 * t_group, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": f"(N, omega) systems solved/s, {len(sessions)} geometries in one group solve",
+        "value": round(systems / (t_group if do_group else t_sep), 3), "unit": "systems/s", "n_gpus": 1,
+        "steps": max(a.steps, 1),
+        "warmup": 1,
+        "ms_per_step": round(1e3 * (t_group if do_group else t_sep), 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f16x3+f32+f64", "data": "synthetic: procedural meshes (tests/python/meshes.py layout)",
-        "config": {"workload": "cylinders R=1.0/0.75/1.5 draft 1 (1536/2816/6144 panels) + their array (10496 "
-                               "panels, 3 bodies), infinite depth, 64 kR 0.05-4.0, heading 0, mixed precision",
+        "config": {"workload": desc, "timing": "fastest of the steps, group and separate solves alternating",
                    "panels": [s.n_panels for s in sessions], "systems": systems,
-                   "group_s": round(t_group, 3), "separate_s": round(t_sep, 3),
-                   "group_speedup_vs_separate": round(t_sep / t_group, 3),
+                   "group_s": round(t_group, 4) if do_group else None,
+                   "separate_s": round(t_sep, 4) if do_sep else None,
+                   "group_speedup_vs_separate": round(t_sep / t_group, 3) if do_group and do_sep else None,
                    "rel_err_group_vs_separate": err},
     }
     for s in sessions:
@@ -302,6 +332,8 @@ def main():
                     help="paths timed; `value` is the fastest whose A, B, X are within 1e-5 of the FP64 path's")
     ap.add_argument("--no-autotune", action="store_true", help="skip the LU autotuner before each path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--group-form", default="both", choices=("both", "group", "separate"),
+                    help="batching4 / variants8: time the group solve, the separate solves or both")
     ap.add_argument("--nn-rmin", type=float, default=0.1,
                     help="the surrogate takes the pairs with nn-rmin <= R/h <= 20 (mixed_nn path)")
     ap.add_argument("--nn-sweep", default="",
@@ -327,7 +359,7 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the frequency sweep "
               f"would shard across GPUs with no exchange, see the design notes)", file=sys.stderr)
         return 2
-    if a.workload == "batching4":
+    if a.workload in ("batching4", "variants8"):
         return bench_group(a)
     import hydrobem_amd as hb
 

@@ -677,6 +677,9 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   return f.k0;
 }
 
+// a group solve's batch takes the next system while its order is at least this share of the
+// batch's first (largest) order
+constexpr double GROUP_ORDER_RATIO = 0.8;
 constexpr double MIXED_TOL = 1e-12;  // normwise backward error that ends the refinement
 constexpr int MIXED_MAX_ITER = 10;
 // the refinement of the FP32 assembly ends at 10^-g_mixed32_tol_exp (HBG_TUNE_MIXED32_TOL)
@@ -2312,8 +2315,15 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   };
   std::vector<int> status(B), iters(B), nsys(B);
   std::vector<double> eta(B), hout;
-  for (int q0 = 0; q0 < systems; q0 += B) {
-    const int nb = std::min(B, systems - q0);
+  std::vector<size_t> hoff(B + 1);
+  for (int q0 = 0, nb = 0; q0 < systems; q0 += nb) {
+    // the batch: the next systems (in decreasing order of N) while there is room and their order
+    // is at least GROUP_ORDER_RATIO of the first one's, so that no system is padded far beyond its
+    // own order (the panel and substitution kernels run over the batch's order)
+    nb = 1;
+    while (nb < B && q0 + nb < systems &&
+           Ns[order[q0 + nb].first] >= GROUP_ORDER_RATIO * Ns[order[q0].first])
+      ++nb;
     int n = 0, R = 0;
     for (int b = 0; b < nb; ++b) {
       n = std::max(n, Ns[order[q0 + b].first]);
@@ -2355,16 +2365,22 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
                     lead->d_M.p + mstride * b + (size_t)(n - Ns[i]) * n, n, si->d_phiI.p + hstride[i] * b,
                     si->d_dphiI.p + hstride[i] * b, si->d_out.p + nout[i] * b, st);
     }
+    // every system's outputs in one host buffer, one wait for the batch
+    hoff[0] = 0;
+    for (int b = 0; b < nb; ++b) hoff[b + 1] = hoff[b] + nout[order[q0 + b].first];
+    hout.resize(hoff[nb]);
+    for (int b = 0; b < nb; ++b) {
+      const int i = order[q0 + b].first;
+      HBG_HIP(hipMemcpyAsync(hout.data() + hoff[b], sessions[i]->d_out.p + nout[i] * b, nout[i] * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    }
     HBG_HIP(hipStreamSynchronize(st));
+    for (double v : hout)
+      if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
     for (int b = 0; b < nb; ++b) {
       const int i = order[q0 + b].first, fi = order[q0 + b].second;
-      HbgSession* si = sessions[i];
-      hout.resize(nout[i]);
-      HBG_HIP(hipMemcpy(hout.data(), si->d_out.p + nout[i] * b, nout[i] * sizeof(double), hipMemcpyDeviceToHost));
-      for (double v : hout)
-        if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
       const size_t D = nrad[i], nAB = D * nrad[i], nX = (size_t)H * D, nHK = (size_t)nrad[i] * H;
-      const double* hA = hout.data();
+      const double* hA = hout.data() + hoff[b];
       const double* hB = hA + nAB;
       const double* hX = hB + nAB;
       const double* hFK = hX + 2 * nX;

@@ -46,7 +46,9 @@ def test_group_solve_matches_separate_solves(precision):
     assert len(set(orders)) == len(orders) and any(n % 64 for n in orders)
     w = np.array([0.6, 1.3, 2.2])
     heads = [0.0, 30.0]
-    sessions[0].set_batch(4)  # 9 systems in batches of 4: every batch mixes the orders
+    # 9 systems in batches of at most 4, sorted by order: 308 and 280 share a batch (the ratio
+    # 0.91 is above the group solve's 0.8), 200 starts its own
+    sessions[0].set_batch(4)
     group = hb.solve_group(sessions, w, heads, precision=precision)
     tol = 1e-11 if precision == "fp64" else 1e-9
     for s, g in zip(sessions, group):
@@ -56,5 +58,27 @@ def test_group_solve_matches_separate_solves(precision):
             e = rel(g[key], ref[key])
             say(f"  N {s.n_panels} {key}: {e:.2e}")
             assert e < tol
+    for s in sessions:
+        s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_of_hull_variants_is_bit_identical(precision):
+    """A design sweep: six hull variants of one panel count, two frequencies each. Alone, each is a
+    batch of two systems; together, one batch of twelve. A system's arithmetic does not depend on
+    the batch it is in, so the results are the same bits."""
+    sessions = []
+    for R in np.linspace(0.8, 1.3, 6):
+        s = hb.Session(0)
+        s.set_environment(1025.0, 9.81)
+        s.set_panels(meshes.cylinder_panels(R, 1.0, 22, 7, 7, quarter=False, cosine=True))
+        sessions.append(s)
+    say(f"case: group of 6 hull variants, {sessions[0].n_panels} panels each, {precision}")
+    w = np.array([0.9, 1.7])
+    group = hb.solve_group(sessions, w, [0.0], precision=precision)
+    for s, g in zip(sessions, group):
+        ref = s.solve(w, [0.0], precision=precision)
+        for key in ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_haskind"):
+            assert np.array_equal(g[key], ref[key]), (key, rel(g[key], ref[key]))
     for s in sessions:
         s.close()
