@@ -130,6 +130,10 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
 int residual_blocks(int n);
 size_t residual_max_elems(int n);
 size_t residual_part_elems(int n, int R);
+void launch_limit_fin_matrices(const PanelD* P, int N, double h, bool zero, const double* ktab, const StaticD* st,
+                               const double2* inf, double2* S, double2* A, hipStream_t stream);
+void launch_limit_points(double h, bool zero, const double* ktab, int n, const double* R, const double* z,
+                         const double* zeta, double* out, hipStream_t stream);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                             const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
                             int2* queue, unsigned* qcount, double rmin, hipStream_t stream);
@@ -1347,8 +1351,8 @@ HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64
 HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
                             const double* zeta, double* out) {
   if (!s || !R || !z || !zeta || !out) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_finite_kernel");
-  if (!(h > 0.0) || !std::isfinite(h) || !(K > 0.0) || !std::isfinite(K) || n < 0)
-    return fail(HBG_ERR_VALUE, "h and K must be finite and > 0", "hbg_finite_kernel");
+  if (!(h > 0.0) || !std::isfinite(h) || !(K >= 0.0) || n < 0)
+    return fail(HBG_ERR_VALUE, "h must be finite and > 0, K >= 0 (0 and +infinity: the limits)", "hbg_finite_kernel");
   for (int i = 0; i < n; ++i)
     if (!(R[i] >= 0.0) || !(z[i] <= 0.0) || !(zeta[i] <= 0.0) || z[i] < -h || zeta[i] < -h ||
         (R[i] == 0.0 && z[i] + zeta[i] == 0.0))
@@ -1357,6 +1361,22 @@ HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const 
   if (n == 0) return HBG_OK;
   GUARD_BEGIN
   use_device(s);
+  if (K == 0.0 || std::isinf(K)) {
+    // the frequency limits omega = 0 (K = 0) and omega = infinity (green_fin.rs FreqMode::Zero /
+    // Infinite)
+    DevBuf<double> dR, dz, dzeta, dout;
+    dR.ensure(n);
+    dz.ensure(n);
+    dzeta.ensure(n);
+    dout.ensure(8 * (size_t)n);
+    HBG_HIP(hipMemcpy(dR.p, R, n * sizeof(double), hipMemcpyHostToDevice));
+    HBG_HIP(hipMemcpy(dz.p, z, n * sizeof(double), hipMemcpyHostToDevice));
+    HBG_HIP(hipMemcpy(dzeta.p, zeta, n * sizeof(double), hipMemcpyHostToDevice));
+    launch_limit_points(h, K == 0.0, s->ktab, n, dR.p, dz.p, dzeta.p, dout.p, s->stream);
+    HBG_HIP(hipMemcpyAsync(out, dout.p, 8 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HBG_HIP(hipStreamSynchronize(s->stream));
+    return HBG_OK;
+  }
   const FinFreq f = make_fin_freq(h, K);
   // the point depths as panels: z[i] at i, zeta[i] at n + i
   std::vector<PanelD> pts(2 * (size_t)n);
@@ -1686,11 +1706,12 @@ HbgStatus hbg_lu_solve_noalloc(HbgSession* s, int32_t n, double* a, int32_t nrhs
 
 namespace {
 
-// A frequency limit of the radiation problem in infinite depth (lo_frequency.rs with
-// FreqMode::Zero / Infinite; no diffraction there): the matrices of the statics alone
-// (launch_limit_matrices), S U for the radiation columns, the FP64 LU and the forces with omega = 0
-// in the force kernel (B = 0). Writes the added mass [D][K] into A (host). The statics (and at
-// infinity their omega = infinity row) must be current.
+// A frequency limit of the radiation problem (lo_frequency.rs with FreqMode::Zero / Infinite; no
+// diffraction there): the matrices of the statics alone in infinite depth (launch_limit_matrices),
+// in finite depth the statics and the limit's regular part (launch_limit_fin_matrices), S U for
+// the radiation columns, the FP64 LU and the forces with omega = 0 in the force kernel (B = 0).
+// Writes the added mass [D][K] into A (host). The statics (and at infinity their omega = infinity
+// row) must be current.
 void solve_limit(HbgSession* s, bool at_infinity, double* A) {
   const int N = s->N, nrad = 6 * s->nb, R = nrad;
   const size_t nAB = (size_t)nrad * nrad;
@@ -1701,7 +1722,11 @@ void solve_limit(HbgSession* s, bool at_infinity, double* A) {
   s->d_out.ensure(2 * nAB);
   s->d_phiI.ensure((size_t)N);
   s->d_dphiI.ensure((size_t)N);
-  launch_limit_matrices(s->d_st.p, at_infinity ? s->d_stinf.p : nullptr, N, s->d_S.p, s->d_M.p, s->stream);
+  if (std::isfinite(s->depth))
+    launch_limit_fin_matrices(s->d_panels.p, N, s->depth, !at_infinity, s->ktab, s->d_st.p, s->d_stinf.p, s->d_S.p,
+                              s->d_M.p, s->stream);
+  else
+    launch_limit_matrices(s->d_st.p, at_infinity ? s->d_stinf.p : nullptr, N, s->d_S.p, s->d_M.p, s->stream);
   launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, 0, 1.0, 1.0, s->grav, 0.0, s->d_heads.p,
                   s->d_phiI.p, s->d_dphiI.p, s->d_u.p, s->stream);
   launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->d_part.p, s->stream);
@@ -1716,6 +1741,26 @@ void solve_limit(HbgSession* s, bool at_infinity, double* A) {
   for (double v : h)
     if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution at a frequency limit"};
   std::memcpy(A, h.data(), nAB * sizeof(double));
+  if (std::isfinite(s->depth) && !at_infinity) {
+    // omega = 0 in finite depth: a mode with a net volume flux through the body (heave of a
+    // surface-piercing body) has an added mass that diverges as omega -> 0 and is not defined
+    // (the reference refuses such a mode, rules/coordinates.rs:15-75, at the relative flux 1e-6):
+    // its row and column are NaN. A mode whose normals vanish to round-off (yaw of a body of
+    // revolution) carries no flux whatever its round-off sum.
+    std::vector<double> flux(nrad, 0.0), scale(nrad, 0.0), body_scale(s->nb, 0.0);
+    for (int j = 0; j < N; ++j) {
+      const PanelD& pj = s->panels[j];
+      body_scale[s->body[j]] += pj.area * (1.0 + std::sqrt(pj.c[0] * pj.c[0] + pj.c[1] * pj.c[1] + pj.c[2] * pj.c[2]));
+      for (int m = 0; m < 6; ++m) {
+        const double f = s->nj[6 * (size_t)j + m] * pj.area;
+        flux[6 * s->body[j] + m] += f;
+        scale[6 * s->body[j] + m] += std::fabs(f);
+      }
+    }
+    for (int c = 0; c < nrad; ++c)
+      if (std::fabs(flux[c]) > 1e-6 * scale[c] && scale[c] > 1e-9 * body_scale[c / 6])
+        for (int i = 0; i < nrad; ++i) A[(size_t)c * nrad + i] = A[(size_t)i * nrad + c] = NAN;
+  }
 }
 
 }  // namespace
@@ -1753,8 +1798,6 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       want_inf = want_inf || std::isinf(omegas[i]);
     }
   }
-  if (!il.empty() && std::isfinite(s->depth))
-    throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits (omega = 0, infinity) in finite depth"};
   if (s->NI > 0) {
     // the interior free surface: ordinary frequencies, the forces
     if (!il.empty()) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with an interior free surface"};

@@ -848,6 +848,58 @@ void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2
   HBG_HIP(hipGetLastError());
 }
 
+// The matrices at a frequency limit in finite depth (lo_frequency.rs with FreqMode::Zero /
+// Infinite, apply_kernel low_order.rs:738-769 with K = 0: no log terms): entry (k, n), field point
+// the centroid of k, source panel n, S = rs + A_n Wr and A = 2 pi delta_kn - (rd + A_n (-WrR a +
+// WrZ n_z)) with the limit's kernel (fin_limit_kernel) and rs, rd the statics (omega = 0) or their
+// omega = infinity row (inf). One thread per entry: this runs once per solve.
+__global__ void __launch_bounds__(256) limit_fin_matrices_kernel(const PanelD* __restrict__ P, int N, double h,
+                                                                 int zero, const double* __restrict__ ktab,
+                                                                 const StaticD* __restrict__ st,
+                                                                 const double2* __restrict__ inf,
+                                                                 double2* __restrict__ S, double2* __restrict__ A) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N) return;
+  const int k = (int)(e % N), n = (int)(e / N);
+  const PanelD* pk = P + k;
+  const PanelD* pn = P + n;
+  const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
+  const double R = sqrt(dx * dx + dy * dy);
+  const FinK w = fin_limit_kernel(zero != 0, h, R, pk->c[2], pn->c[2], ktab);
+  const double a = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
+  const double rs = zero ? st[e].rs : inf[e].x, rd = zero ? st[e].rd : inf[e].y;
+  S[e] = make_double2(rs + pn->area * w.Wr, 0.0);
+  A[e] = make_double2((k == n ? 2.0 * PI : 0.0) - (rd + pn->area * (-w.WrR * a + w.WrZ * pn->n[2])), 0.0);
+}
+
+void launch_limit_fin_matrices(const PanelD* P, int N, double h, bool zero, const double* ktab, const StaticD* st,
+                               const double2* inf, double2* S, double2* A, hipStream_t stream) {
+  limit_fin_matrices_kernel<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, stream>>>(P, N, h, zero ? 1 : 0, ktab,
+                                                                                        st, inf, S, A);
+  HBG_HIP(hipGetLastError());
+}
+
+// The limit kernels at n point pairs (hbg_finite_kernel at K = 0 and K = infinity): out [n][8]
+__global__ void limit_points_kernel(double h, int zero, const double* __restrict__ ktab, int n,
+                                    const double* __restrict__ R, const double* __restrict__ z,
+                                    const double* __restrict__ zeta, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const FinK w = fin_limit_kernel(zero != 0, h, R[i], z[i], zeta[i], ktab);
+  double* o = out + 8 * (size_t)i;
+  o[0] = w.Wr;
+  o[1] = w.WrR;
+  o[2] = w.WrZ;
+  o[3] = w.Wrz;
+  o[4] = o[5] = o[6] = o[7] = 0.0;
+}
+
+void launch_limit_points(double h, bool zero, const double* ktab, int n, const double* R, const double* z,
+                         const double* zeta, double* out, hipStream_t stream) {
+  limit_points_kernel<<<(n + 127) / 128, 128, 0, stream>>>(h, zero ? 1 : 0, ktab, n, R, z, zeta, out);
+  HBG_HIP(hipGetLastError());
+}
+
 void launch_pad_system(double2* M, float2* M32, int n, int nz, int R, int rz, hipStream_t stream) {
   if (nz >= n && rz >= R) return;
   pad_system_kernel<<<n + R, 256, 0, stream>>>(M, M32, n, nz, R, rz);

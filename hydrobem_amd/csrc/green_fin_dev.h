@@ -411,3 +411,148 @@ __device__ __forceinline__ FinProp fin_prop(const FinFreq* __restrict__ fp, cons
 }
 
 }  // namespace hbg
+
+namespace hbg {
+
+// ---- the frequency limits in finite depth (green_fin.rs FreqMode::Zero / Infinite): Wi = 0 and
+// the regular part Wr = G - sum_p s_p / r_p of the Green function that the limit's boundary
+// conditions on z = 0 and z = -h fix:
+//   omega = 0 (zero): dG/dz = 0 on both, s_p = +1 for all six Rankine terms,
+//     G = -(2/h) (log(R/2h) + gamma) + (4/h) sum_n cos k_n(z+h) cos k_n(zeta+h) K0(k_n R), k_n = n pi / h;
+//   omega = inf: G = 0 on z = 0, dG/dz = 0 on z = -h, s_p = (+, +, -, -, -, -),
+//     G = (4/h) sum_n cos k_n(z+h) cos k_n(zeta+h) K0(k_n R), k_n = (n - 1/2) pi / h.
+// R >= h/2: that series, 24 roots, stopped at k_n R >= 18, as green_fin.rs:256-322 evaluates it.
+// R < h/2 (where the reference reads generated tables of the smooth remainder): the same G as
+// the sum of its images, the source's reflections in z = 0 and z = -h,
+//   G = sum_m e_m [1/rho(d1 + 2 m h) + t_m / rho(s + 2 m h)],  d1 = z - zeta, s = z + zeta,
+//   rho(x) = sqrt(R^2 + x^2), e_m = 1, t_m = 1 (omega = 0) or e_m = (-1)^m, t_m = -1 (omega = inf),
+// the six Rankine terms being the images m = -1, 0, 1 of the first family and m = 0, 1, 2 of the
+// second. The terms |m| <= LIM_M are summed; the four runs beyond go in closed form: at omega = 0
+// by Euler-Maclaurin about m = M + 1/2 (the integral of 1/rho in asinh, the log divergence of the
+// runs cancelling against the regularization that the series' constant -(2/h) gamma fixes), at
+// omega = inf by Boole's summation of the alternating runs about m = M + 1; with the third
+// derivative terms kept the truncation is below 1e-11 of the kernel. Checked against the
+// mpmath values of the reference's tests/data/green_fin_reference.txt (modes 1 and 2).
+constexpr int LIM_M = 32;
+constexpr double EULER_GAMMA = 0.57721566490153286061;
+
+// 1/rho and its x derivatives up to the fourth, and the R derivatives of f, f', f''' at x
+struct RhoD {
+  double f0, f1, f2, f3, f4, f0R, f1R, f3R;
+};
+__device__ __forceinline__ RhoD rho_derivs(double R, double x) {
+  const double r2 = R * R + x * x, ir = 1.0 / sqrt(r2), ir2 = 1.0 / r2;
+  const double ir3 = ir * ir2, ir5 = ir3 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2;
+  const double x2 = x * x;
+  RhoD d;
+  d.f0 = ir;
+  d.f1 = -x * ir3;
+  d.f2 = -ir3 + 3.0 * x2 * ir5;
+  d.f3 = 9.0 * x * ir5 - 15.0 * x2 * x * ir7;
+  d.f4 = 9.0 * ir5 - 90.0 * x2 * ir7 + 105.0 * x2 * x2 * ir9;
+  d.f0R = -R * ir3;
+  d.f1R = 3.0 * x * R * ir5;
+  d.f3R = -45.0 * x * R * ir7 + 105.0 * x2 * x * R * ir9;
+  return d;
+}
+
+__device__ __noinline__ FinK fin_limit_kernel(bool zero, double h, double R, double z, double zeta,
+                                              const double* __restrict__ ktab) {
+  const double d1 = z - zeta, s = z + zeta;
+  const double sg = d1 > 0.0 ? 1.0 : (d1 < 0.0 ? -1.0 : 0.0);
+  FinK o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (R >= 0.5 * h) {
+    const double v[6] = {fabs(d1), s + 2.0 * h, -s, d1 + 2.0 * h, -d1 + 2.0 * h, s + 4.0 * h};
+    const double sp[6] = {1.0, 1.0, zero ? 1.0 : -1.0, zero ? 1.0 : -1.0, zero ? 1.0 : -1.0, zero ? 1.0 : -1.0};
+    const double dvz[6] = {-sg, 1.0, -1.0, -1.0, 1.0, 1.0};  // d v_p / d zeta
+    const double dvf[6] = {sg, 1.0, -1.0, 1.0, -1.0, 1.0};   // d v_p / d z
+    double re = 0.0, reR = 0.0, reZ = 0.0, rez = 0.0;
+    const double c = 4.0 / h;
+    for (int n = 1; n <= FIN_ROOTS; ++n) {
+      const double k = ((double)n - (zero ? 0.0 : 0.5)) * PI / h, x = k * R;
+      if (x >= SERIES_STOP) break;
+      double K0, K1, sz, cz, ss, cs;
+      k01(ktab, x, K0, K1);
+      sincos(k * (z + h), &sz, &cz);
+      sincos(k * (zeta + h), &ss, &cs);
+      re += c * cz * cs * K0;
+      reR -= c * cz * cs * k * K1;
+      reZ -= c * cz * k * ss * K0;
+      rez -= c * k * sz * cs * K0;
+    }
+    if (zero) {
+      re += -2.0 / h * (log(R / (2.0 * h)) + EULER_GAMMA);
+      reR += -2.0 / (h * R);
+    }
+    for (int p = 0; p < 6; ++p) {
+      const double r2 = R * R + v[p] * v[p], r = sqrt(r2), ir3 = sp[p] / (r2 * r);
+      re -= sp[p] / r;
+      reR += R * ir3;
+      reZ += v[p] * dvz[p] * ir3;
+      rez += v[p] * dvf[p] * ir3;
+    }
+    o.Wr = re;
+    o.WrR = reR;
+    o.WrZ = reZ;
+    o.Wrz = rez;
+    return o;
+  }
+  // the images: family a (offset d1: dx/dzeta = -1, dx/dz = 1), family b (offset s: both 1)
+  double re = 0.0, reR = 0.0, reZ = 0.0, rez = 0.0;
+  for (int m = -LIM_M; m <= LIM_M; ++m) {
+    const double e = zero ? 1.0 : ((m & 1) ? -1.0 : 1.0);
+    if (m < -1 || m > 1) {
+      const double x = d1 + 2.0 * m * h, r2 = R * R + x * x, ir = 1.0 / sqrt(r2), ir3 = ir / r2;
+      re += e * ir;
+      reR -= e * R * ir3;
+      reZ += e * x * ir3;
+      rez -= e * x * ir3;
+    }
+    if (m < 0 || m > 2) {
+      const double t = zero ? e : -e;
+      const double x = s + 2.0 * m * h, r2 = R * R + x * x, ir = 1.0 / sqrt(r2), ir3 = ir / r2;
+      re += t * ir;
+      reR -= t * R * ir3;
+      reZ -= t * x * ir3;
+      rez -= t * x * ir3;
+    }
+  }
+  // the four runs beyond |m| = LIM_M, each sum_{m > M} w_m / rho(c + 2 m h): offset c, its
+  // derivatives in zeta and z, and the weight of the run's first term at omega = inf
+  const double cs4[4] = {d1, -d1, s, -s};
+  const double dcz[4] = {-1.0, 1.0, 1.0, -1.0};
+  const double dcf[4] = {1.0, -1.0, 1.0, -1.0};
+  const double h2 = 2.0 * h, h23 = h2 * h2 * h2;
+  for (int i = 0; i < 4; ++i) {
+    const double c = cs4[i];
+    double T, TR, Tc;
+    if (zero) {
+      // Euler-Maclaurin about M + 1/2: the integral (1/2h)[log(2h/R) - asinh(u/R)] (the
+      // log(2h/R) of the regularization), + f'/24 - 7 f'''/5760 (in m)
+      const double u = c + (2.0 * LIM_M + 1.0) * h, q = sqrt(u * u + R * R);
+      const RhoD d = rho_derivs(R, u);
+      T = log(h2 / (u + q)) / h2 + h2 * d.f1 / 24.0 - 7.0 * h23 * d.f3 / 5760.0;
+      TR = -(R / q) / (u + q) / h2 + h2 * d.f1R / 24.0 - 7.0 * h23 * d.f3R / 5760.0;
+      Tc = -1.0 / (q * h2) + h2 * d.f2 / 24.0 - 7.0 * h23 * d.f4 / 5760.0;
+    } else {
+      // Boole about m = M + 1: f/2 - f'/4 + f'''/48 (in m), the run's first sign
+      const double w = (((LIM_M + 1) & 1) ? -1.0 : 1.0) * (i < 2 ? 1.0 : -1.0);
+      const RhoD d = rho_derivs(R, c + 2.0 * (LIM_M + 1) * h);
+      T = w * (0.5 * d.f0 - 0.25 * h2 * d.f1 + h23 * d.f3 / 48.0);
+      TR = w * (0.5 * d.f0R - 0.25 * h2 * d.f1R + h23 * d.f3R / 48.0);
+      Tc = w * (0.5 * d.f1 - 0.25 * h2 * d.f2 + h23 * d.f4 / 48.0);
+    }
+    re += T;
+    reR += TR;
+    reZ += Tc * dcz[i];
+    rez += Tc * dcf[i];
+  }
+  if (zero) re -= 2.0 / h * EULER_GAMMA;
+  o.Wr = re;
+  o.WrR = reR;
+  o.WrZ = reZ;
+  o.Wrz = rez;
+  return o;
+}
+
+}  // namespace hbg

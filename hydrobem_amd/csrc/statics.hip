@@ -161,8 +161,9 @@ __device__ __forceinline__ double log_in_plane(const PanelD* __restrict__ p, P3 
 }
 
 // The record of the pair (field point x, panel p): returns whether its log corrections need the
-// adaptive rule (then dl, dd are left 0 for adaptive_kernel). sinf, dinf: the integrals of
-// 1/r - 1/r' (the omega = infinity row, infinite depth).
+// adaptive rule (then dl, dd are left 0 for adaptive_kernel). sinf, dinf: the omega = infinity
+// row, the integrals of 1/r - 1/r' (infinite depth) or of 1/r1 + 1/r2 - (1/r3 + ... + 1/r6)
+// (finite depth, the six image points).
 __device__ __forceinline__ bool pair_statics(const PanelD* __restrict__ p, P3 x, int logint, double tol, double depth,
                                              const double* __restrict__ gl, StaticD& e, double& sinf,
                                              double& dinf) {
@@ -185,6 +186,9 @@ __device__ __forceinline__ bool pair_statics(const PanelD* __restrict__ p, P3 x,
     rankine_d(p, mk(x.x, x.y, -x.z - 4.0 * h), gl, s6, d6);
     e.rs += (s2 + s4) + s5 + s6;
     e.rd += (d2 + d4) + d5 + d6;
+    // the omega = infinity row in finite depth (lo_static.rs:84-95): 1/r1 + 1/r2 - (1/r3 + ... + 1/r6)
+    sinf = s0 + s2 - sb - s4 - s5 - s6;
+    dinf = d0 + d2 - db - d4 - d5 - d6;
   }
   e.dl = 0.0;
   e.dd = 0.0;
@@ -232,9 +236,9 @@ __device__ __forceinline__ void queue_pair(int2* __restrict__ queue, unsigned* _
 
 }  // namespace
 
-// gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]. inf (optional, infinite depth):
-// the omega = infinity row of the pair, the integrals of 1/r - 1/r' (lo_static.rs:89-96), at
-// inf[j * N + k] as (source, dipole).
+// gl: Gauss-Legendre nodes and weights [order 0..16][x 16 | w 16]. inf (optional):
+// the omega = infinity row of the pair (lo_static.rs:84-96, pair_statics), at inf[j * N + k] as
+// (source, dipole).
 __global__ void __launch_bounds__(256) statics_kernel(const PanelD* __restrict__ P, const PanelD* __restrict__ Pf,
                                                       int N, int Nsrc, int logint,
                                                       double tol, double depth, const double* __restrict__ gl,
@@ -606,7 +610,6 @@ __global__ void __launch_bounds__(64) adaptive_kernel(const PanelD* __restrict__
 void launch_statics(const PanelD* P, const PanelD* Pf, int N, int Nsrc, int logint, double tol, double depth,
                     const double* gl, StaticD* st, int2* queue, unsigned* qcount, unsigned qcap, double2* inf,
                     hipStream_t stream) {
-  if (inf && depth > 0.0) throw Error{HBG_ERR_UNSUPPORTED, "the omega = infinity statics in finite depth"};
   dim3 block(64, 4);
   dim3 grid((N + 63) / 64, (Nsrc + 3) / 4);
   statics_kernel<<<grid, block, 0, stream>>>(P, Pf, N, Nsrc, logint, tol, depth, gl, st, queue, qcount, qcap, inf);

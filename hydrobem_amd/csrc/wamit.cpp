@@ -140,6 +140,9 @@ int write_wamit(const char* stem, int overwrite, double rho, double g, double L,
         for (int k = 0; k < K; ++k) {
           const double s = rho * std::pow(L, 3 + r(i) + r(k));
           const size_t o = ((size_t)f * D + i) * K + k;
+          // a limit's entry that is not defined (NaN: at omega = 0 in finite depth the modes with a
+          // net volume flux) has no row, as a mode the reference refuses there
+          if (limit(f) && std::isnan(A[o])) continue;
           std::string l = per[f];
           if (!fortran_i(i + 1, 6, l) || !fortran_i(k + 1, 6, l) || !fortran_e(A[o] / s, true, 14, 6, l) ||
               (!limit(f) && !fortran_e(B[o] / (omegas[f] * s), true, 14, 6, l))) {

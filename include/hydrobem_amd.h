@@ -188,11 +188,15 @@ HbgStatus hbg_kochin_shape(HbgSession* s, int32_t* n_omegas, int32_t* n_angles, 
  *   excitation, froude_krylov,
  *   scattering                  [F][H][D] complex              (X_i = FK + SC)
  *   haskind                     [F][D][H] complex              (Haskind exciting force, Output.HK)
- * An omega of 0 or +infinity (infinite depth only; HBG_ERR_UNSUPPORTED in finite depth) requests
- * that radiation limit (lo_static.rs:89-96, low_order.rs:635-637, 757-758: the statics alone, with
- * 1/r - 1/r' at infinity), solved in FP64 whatever the precision option: its added mass row is
- * written and its damping and diffraction rows are zero (the reference's
- * added_mass_zero_frequency / added_mass_infinite_frequency).
+ * An omega of 0 or +infinity requests that radiation limit (lo_static.rs:84-96,
+ * low_order.rs:635-637, 757-758, green_fin.rs FreqMode::Zero / Infinite: in infinite depth the
+ * statics alone, with 1/r - 1/r' at infinity; in finite depth the statics with their six images
+ * and the limit's regular part), solved in FP64 whatever the precision option: its added mass row
+ * is written and its damping and diffraction rows are zero (the reference's
+ * added_mass_zero_frequency / added_mass_infinite_frequency). At omega = 0 in finite depth the
+ * added mass of a mode with a net volume flux through the body (heave of a surface-piercing body)
+ * diverges and is not defined: its row and column are NaN (the reference refuses such a mode,
+ * rules/coordinates.rs:15-75, at the same relative flux 1e-6).
  * HBG_ERR_NUMERICAL on a zero pivot ("singular system matrix (zero pivot k)") or a non-finite
  * matrix or solution (linalg.rs:221-252); HBG_ERR_CANCELLED when the callback returns 0. */
 HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const double* headings_deg,
@@ -237,7 +241,8 @@ HbgStatus hbg_wave_kernel(HbgSession* s, const double* X, const double* Y, int64
 /* The finite-depth kernel at n point pairs (green_fin.rs:1007-1095 finite_kernel, ordinary
  * frequency K = w^2/g, depth h): horizontal distance R[i], field-point depth z[i], source depth
  * zeta[i]; out [n][8] = Wr, dWr/dR, dWr/dzeta, dWr/dz, Wi, dWi/dR, dWi/dzeta, dWi/dz of the regular
- * part Wr = Re G - sum_p 1/r_p, Wi = Im G. */
+ * part Wr = Re G - sum_p 1/r_p, Wi = Im G. K = 0 and K = +infinity: the limits omega = 0 and
+ * omega = infinity (FreqMode::Zero / Infinite, Rankine signs + and (+, +, -, -, -, -), Wi = 0). */
 HbgStatus hbg_finite_kernel(HbgSession* s, double h, double K, int32_t n, const double* R, const double* z,
                             const double* zeta, double* out);
 

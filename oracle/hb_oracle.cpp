@@ -368,12 +368,19 @@ bool log_pair(const PanelGeom& p, V3 x, double tol) {
 // lo_static.rs:59-119, wetted panels (no surface panels in z = 0); finite depth h adds the bottom
 // images of image_points (:29-38): (x, y, -z - 2h) and its image (x, y, z + 2h), (x, y, z - 2h),
 // (x, y, -z - 4h)
-void static_entry(const PanelGeom& p, V3 x, bool logint, double tol, double depth, double* e) {
+// inf (optional): the omega = infinity row (:84-95), the integrals of 1/r1 + 1/r2 - (1/r3 + ... +
+// 1/r6) in finite depth, of 1/r - 1/r' in infinite depth
+void static_entry(const PanelGeom& p, V3 x, bool logint, double tol, double depth, double* e,
+                  double* inf = nullptr) {
   double s0, d0, sb, db;
   rankine(p, x, s0, d0);
   rankine(p, v3(x[0], x[1], -x[2]), sb, db);
   e[0] = s0 + sb;
   e[1] = d0 + db;
+  if (inf) {
+    inf[0] = s0 - sb;
+    inf[1] = d0 - db;
+  }
   if (std::isfinite(depth)) {
     const double h = depth;
     double s2, d2, s4, d4, s5, d5, s6, d6;
@@ -383,6 +390,10 @@ void static_entry(const PanelGeom& p, V3 x, bool logint, double tol, double dept
     rankine(p, v3(x[0], x[1], -x[2] - 4.0 * h), s6, d6);
     e[0] += (s2 + s4) + s5 + s6;
     e[1] += (d2 + d4) + d5 + d6;
+    if (inf) {
+      inf[0] = s0 + s2 - sb - s4 - s5 - s6;
+      inf[1] = d0 + d2 - db - d4 - d5 - d6;
+    }
   }
   e[2] = 0.0;
   e[3] = 0.0;
@@ -905,7 +916,116 @@ void finite_kernel(const FinDepth& fd, double R, double z, double zeta, double* 
   o[7] = -fd.A0 * Ef * j0v;
 }
 
+// The regular part at the frequency limits in finite depth (green_fin.rs FreqMode::Zero with
+// zero true, FreqMode::Infinite otherwise; Wi = 0): R >= h/2 the eigenfunction series of
+// series_form (:256-322) with k_n = n pi / h (zero) or (n - 1/2) pi / h, c_n = 4/h
+// (make_finite_depth :874-902), at omega = 0 the term -(2/h)(log(R/2h) + gamma), the Rankine
+// signs +1 (zero) or (+, +, -, -, -, -); R < h/2, where the reference interpolates its generated
+// tables of the smooth remainder (L0 and the last region's Q), the same Green function as the sum
+// of the source's images in z = 0 and z = -h (family a at d1 + 2mh, family b at s + 2mh, signs
+// 1, 1 at omega = 0 and (-1)^m, -(-1)^m at omega = inf), |m| <= 64 summed and the four runs
+// beyond in closed form (omega = 0: Euler-Maclaurin about m = 64.5, the asinh integral whose log
+// divergence the -(2/h) gamma of the series' regularization absorbs; omega = inf: Boole's sum of
+// the alternating runs about m = 65), minus the six Rankine terms. Pinned by the mpmath rows of
+// modes 1 and 2 in the reference's tests/data/green_fin_reference.txt.
+void finite_limit_kernel(bool zero, double h, double R, double z, double zeta, double* o) {
+  const double d1 = z - zeta, s = z + zeta;
+  const double sg = d1 > 0.0 ? 1.0 : d1 < 0.0 ? -1.0 : 0.0;
+  double re = 0.0, reR = 0.0, reZ = 0.0, rez = 0.0;
+  if (R >= 0.5 * h) {
+    const double v[6] = {std::fabs(d1), s + 2.0 * h, -s, d1 + 2.0 * h, -d1 + 2.0 * h, s + 4.0 * h};
+    const double c = 4.0 / h;
+    for (int n = 1; n <= FIN_ROOTS; ++n) {
+      const double k = (n - (zero ? 0.0 : 0.5)) * PI / h, xn = k * R;
+      if (xn >= SERIES_STOP) break;
+      double K0, K1;
+      bessel_k01(xn, K0, K1);
+      const double cz = std::cos(k * (z + h)), sz = std::sin(k * (z + h));
+      const double cs = std::cos(k * (zeta + h)), ss = std::sin(k * (zeta + h));
+      re += c * cz * cs * K0;
+      reR -= c * cz * cs * k * K1;
+      reZ -= c * cz * k * ss * K0;
+      rez -= c * k * sz * cs * K0;
+    }
+    if (zero) {
+      re += -2.0 / h * (std::log(R / (2.0 * h)) + GAMMA);
+      reR += -2.0 / (h * R);
+    }
+    const double dvz[6] = {-sg, 1.0, -1.0, -1.0, 1.0, 1.0}, dvf[6] = {sg, 1.0, -1.0, 1.0, -1.0, 1.0};
+    for (int p = 0; p < 6; ++p) {
+      const double sp = (zero || p < 2) ? 1.0 : -1.0;
+      const double r2 = R * R + v[p] * v[p], r = std::sqrt(r2), ir3 = sp / (r2 * r);
+      re -= sp / r;
+      reR += R * ir3;
+      reZ += v[p] * dvz[p] * ir3;
+      rez += v[p] * dvf[p] * ir3;
+    }
+  } else {
+    const int M = 64;
+    for (int m = -M; m <= M; ++m) {
+      const double e = zero ? 1.0 : ((m % 2) ? -1.0 : 1.0);
+      if (m < -1 || m > 1) {  // family a: dx/dzeta = -1, dx/dz = 1
+        const double x = d1 + 2.0 * m * h, r2 = R * R + x * x, r = std::sqrt(r2), ir3 = 1.0 / (r2 * r);
+        re += e / r;
+        reR -= e * R * ir3;
+        reZ += e * x * ir3;
+        rez -= e * x * ir3;
+      }
+      if (m < 0 || m > 2) {  // family b: dx/dzeta = dx/dz = 1
+        const double t = zero ? e : -e;
+        const double x = s + 2.0 * m * h, r2 = R * R + x * x, r = std::sqrt(r2), ir3 = 1.0 / (r2 * r);
+        re += t / r;
+        reR -= t * R * ir3;
+        reZ -= t * x * ir3;
+        rez -= t * x * ir3;
+      }
+    }
+    // the runs m > M of 1/rho(c + 2 m h): c = d1, -d1 (family a), s, -s (family b)
+    const double cs4[4] = {d1, -d1, s, -s}, dcz[4] = {-1.0, 1.0, 1.0, -1.0}, dcf[4] = {1.0, -1.0, 1.0, -1.0};
+    const double h2 = 2.0 * h, h23 = h2 * h2 * h2;
+    for (int i = 0; i < 4; ++i) {
+      // f = 1/rho(x) and its derivatives at the run's start x0
+      const double c = cs4[i];
+      const double x0 = zero ? c + (2.0 * M + 1.0) * h : c + 2.0 * (M + 1) * h;
+      const double r2 = R * R + x0 * x0, r = std::sqrt(r2);
+      const double ir3 = 1.0 / (r2 * r), ir5 = ir3 / r2, ir7 = ir5 / r2, ir9 = ir7 / r2, x2 = x0 * x0;
+      const double f0 = 1.0 / r, f1 = -x0 * ir3, f2 = -ir3 + 3.0 * x2 * ir5, f3 = 9.0 * x0 * ir5 - 15.0 * x2 * x0 * ir7;
+      const double f4 = 9.0 * ir5 - 90.0 * x2 * ir7 + 105.0 * x2 * x2 * ir9;
+      const double f0R = -R * ir3, f1R = 3.0 * x0 * R * ir5, f3R = -45.0 * x0 * R * ir7 + 105.0 * x2 * x0 * R * ir9;
+      double T, TR, Tc;
+      if (zero) {
+        T = std::log(h2 / (x0 + r)) / h2 + h2 * f1 / 24.0 - 7.0 * h23 * f3 / 5760.0;
+        TR = -(R / r) / (x0 + r) / h2 + h2 * f1R / 24.0 - 7.0 * h23 * f3R / 5760.0;
+        Tc = -1.0 / (r * h2) + h2 * f2 / 24.0 - 7.0 * h23 * f4 / 5760.0;
+      } else {
+        const double w = (((M + 1) % 2) ? -1.0 : 1.0) * (i < 2 ? 1.0 : -1.0);
+        T = w * (0.5 * f0 - 0.25 * h2 * f1 + h23 * f3 / 48.0);
+        TR = w * (0.5 * f0R - 0.25 * h2 * f1R + h23 * f3R / 48.0);
+        Tc = w * (0.5 * f1 - 0.25 * h2 * f2 + h23 * f4 / 48.0);
+      }
+      re += T;
+      reR += TR;
+      reZ += Tc * dcz[i];
+      rez += Tc * dcf[i];
+    }
+    if (zero) re -= 2.0 / h * GAMMA;
+  }
+  o[0] = re;
+  o[1] = reR;
+  o[2] = reZ;
+  o[3] = rez;
+  for (int i = 4; i < 8; ++i) o[i] = 0.0;
+}
+
 }  // namespace
+
+// mode 1: omega = 0, mode 2: omega = infinity (the reference's green_fin_reference.txt modes)
+extern "C" int orc_finite_limit_kernel(int mode, double h, double R, double z, double zeta, double* out8) {
+  if ((mode != 1 && mode != 2) || !(h > 0.0) || !(R >= 0.0) || z > 0.0 || zeta > 0.0 || z < -h || zeta < -h)
+    return ORC_ERR_ARGUMENT;
+  finite_limit_kernel(mode == 1, h, R, z, zeta, out8);
+  return ORC_OK;
+}
 
 extern "C" int orc_finite_kernel(double h, double K, double R, double z, double zeta, double* out8) {
   if (!(h > 0.0) || !(K > 0.0) || !(R >= 0.0) || z > 0.0 || zeta > 0.0 || z < -h || zeta < -h)
@@ -929,7 +1049,8 @@ struct OrcProblem {
   double rho = 1025.0, grav = 9.81, depth = INFINITY;
   bool logint = true;
   double logtol = 1e-4;
-  std::vector<double> st;  // statics [k][j][4]: rs, rd, dl, dd
+  std::vector<double> st;     // statics [k][j][4]: rs, rd, dl, dd
+  std::vector<double> stinf;  // the omega = infinity rows [k][j][2]
   bool have_statics = false;
 };
 
@@ -976,11 +1097,13 @@ static void ensure_statics(OrcProblem* p) {
   if (p->have_statics) return;
   int N = p->N;
   p->st.assign((size_t)N * N * 4, 0.0);
+  p->stinf.assign((size_t)N * N * 2, 0.0);
 #pragma omp parallel for schedule(dynamic, 1)
   for (int k = 0; k < N; ++k) {
     V3 x = p->pg[k].c;
     for (int j = 0; j < N; ++j)
-      static_entry(p->pg[j], x, p->logint, p->logtol, p->depth, &p->st[((size_t)k * N + j) * 4]);
+      static_entry(p->pg[j], x, p->logint, p->logtol, p->depth, &p->st[((size_t)k * N + j) * 4],
+                   &p->stinf[((size_t)k * N + j) * 2]);
   }
   p->have_statics = true;
 }
@@ -1023,29 +1146,36 @@ Kern pair_kernel(double K, V3 x, V3 xi, double& R) {
 }
 
 // low_order.rs:738-769 (wetted panels: the dipole column is D)
+// at omega = infinity (at_inf) the statics' omega = infinity row (:752-753)
 void apply_kernel(const OrcProblem* p, int row_k, int j, V3 x, double K, double logc, const Kern& k, double R,
-                  bool swapped, cd& S, cd& D) {
+                  bool swapped, cd& S, cd& D, bool at_inf = false) {
   const PanelGeom& pj = p->pg[j];
   double a = R > 0.0 ? ((x[0] - pj.c[0]) * pj.n[0] + (x[1] - pj.c[1]) * pj.n[1]) / R : 0.0;
-  const double* e = &p->st[((size_t)row_k * p->N + j) * 4];
+  const size_t o = (size_t)row_k * p->N + j;
+  const double* e = &p->st[o * 4];
+  const double rs = at_inf ? p->stinf[o * 2] : e[0], rd = at_inf ? p->stinf[o * 2 + 1] : e[1];
   double A = pj.area;
   double rz = swapped ? k.Wrz : k.WrZ;
   double iz = swapped ? k.Wiz : k.WiZ;
-  S = cd(e[0] + A * k.Wr - 2.0 * K * logc * e[2], A * k.Wi);
-  D = cd(e[1] + A * (-k.WrR * a + rz * pj.n[2]) - 2.0 * K * logc * e[3], A * (-k.WiR * a + iz * pj.n[2]));
+  S = cd(rs + A * k.Wr - 2.0 * K * logc * e[2], A * k.Wi);
+  D = cd(rd + A * (-k.WrR * a + rz * pj.n[2]) - 2.0 * K * logc * e[3], A * (-k.WiR * a + iz * pj.n[2]));
 }
 
 // lo_frequency.rs:399-537 for G = 1: S(k, n), A(k, n) = 2 pi delta_kn - D(k, n), column-major;
 // finite depth: the kernel of finite_kernel (green_fin.rs:1007-1095), the (n, k) entry with the
 // derivative in its first point (Wrz, lo_frequency.rs:494-497)
+// omega = 0 and omega = infinity: the limits (solve_common.rs:44-71: K = 0, no log terms; the
+// kernel of finite_limit_kernel in finite depth, none in infinite depth: 1/r +- 1/r' only,
+// low_order.rs:635-637)
 void assemble(OrcProblem* p, double omega, std::vector<cd>& S, std::vector<cd>& Am) {
   ensure_statics(p);
   int N = p->N;
-  double K = omega * omega / p->grav;
-  double logc = p->logint ? 1.0 : 0.0;
+  const bool limit = omega == 0.0 || std::isinf(omega), at_inf = std::isinf(omega);
+  double K = limit ? 0.0 : omega * omega / p->grav;
+  double logc = p->logint && !limit ? 1.0 : 0.0;
   const bool finite = std::isfinite(p->depth);
   FinDepth fd;
-  if (finite) fd = make_fd(p->depth, K);
+  if (finite && !limit) fd = make_fd(p->depth, K);
   S.assign((size_t)N * N, cd(0, 0));
   Am.assign((size_t)N * N, cd(0, 0));
 #pragma omp parallel for schedule(dynamic, 4)
@@ -1059,7 +1189,10 @@ void assemble(OrcProblem* p, double omega, std::vector<cd>& S, std::vector<cd>& 
         double dx = xk[0] - xn[0], dy = xk[1] - xn[1];
         R = std::sqrt(dx * dx + dy * dy);
         double o[8];
-        finite_kernel(fd, R, xk[2], xn[2], o);
+        if (limit)
+          finite_limit_kernel(!at_inf, p->depth, R, xk[2], xn[2], o);
+        else
+          finite_kernel(fd, R, xk[2], xn[2], o);
         w.Wr = o[0];
         w.WrR = o[1];
         w.WrZ = o[2];
@@ -1068,15 +1201,18 @@ void assemble(OrcProblem* p, double omega, std::vector<cd>& S, std::vector<cd>& 
         w.WiR = o[5];
         w.WiZ = o[6];
         w.Wiz = o[7];
+      } else if (limit) {
+        double dx = xk[0] - xn[0], dy = xk[1] - xn[1];
+        R = std::sqrt(dx * dx + dy * dy);
       } else {
         w = pair_kernel(K, xk, xn, R);
       }
       cd S1, D1, S2, D2;
-      apply_kernel(p, k, n, xk, K, logc, w, R, false, S1, D1);
+      apply_kernel(p, k, n, xk, K, logc, w, R, false, S1, D1, at_inf);
       S[k + (size_t)n * N] = S1;
       Am[k + (size_t)n * N] = (k == n ? cd(2.0 * PI, 0.0) : cd(0, 0)) - D1;
       if (n != k) {
-        apply_kernel(p, n, k, xn, K, logc, w, R, true, S2, D2);
+        apply_kernel(p, n, k, xn, K, logc, w, R, true, S2, D2, at_inf);
         S[n + (size_t)k * N] = S2;
         Am[n + (size_t)k * N] = -D2;
       }
@@ -1140,7 +1276,7 @@ int lu_solve(int n, cd* a, int nrhs, cd* b) {
 
 extern "C" int orc_assemble(void* h, double omega, double* S_out, double* A_out) {
   OrcProblem* p = (OrcProblem*)h;
-  if (!(omega > 0.0)) return ORC_ERR_ARGUMENT;
+  if (!(omega >= 0.0)) return ORC_ERR_ARGUMENT;
   std::vector<cd> S, A;
   assemble(p, omega, S, A);
   std::memcpy(S_out, S.data(), S.size() * sizeof(cd));
@@ -1163,14 +1299,17 @@ extern "C" int orc_solve(void* h, const double* omegas, int nf, const double* he
   std::vector<cd> S, Am;
   for (int fi = 0; fi < nf; ++fi) {
     double w = omegas[fi];
-    if (!(w > 0.0) || !std::isfinite(w)) return ORC_ERR_ARGUMENT;
-    double K = w * w / p->grav, k = K;
+    if (!(w >= 0.0)) return ORC_ERR_ARGUMENT;
+    // omega = 0 and infinity: the radiation limits (no incident wave; B, X, FK, SC, HK are zero)
+    const bool limit = w == 0.0 || std::isinf(w);
+    const int nhw = limit ? 0 : nh;
+    double K = limit ? 0.0 : w * w / p->grav, k = K;
     const bool finite = std::isfinite(p->depth);
-    if (finite) k = orc_dispersion_x0(K * p->depth) / p->depth;
+    if (finite && !limit) k = orc_dispersion_x0(K * p->depth) / p->depth;
     assemble(p, w, S, Am);
     // boundary values (lo_frequency.rs:214-273; low_order.rs:1458-1468 incident_one, frame at the origin)
     std::vector<cd> phiI((size_t)nh * N), dphiI((size_t)nh * N), u((size_t)nrhs * N, cd(0, 0));
-    for (int hh = 0; hh < nh; ++hh) {
+    for (int hh = 0; hh < nhw; ++hh) {
       double beta = headings_deg[hh] * PI / 180.0;
       double cb = std::cos(beta), sb = std::sin(beta);
       for (int j = 0; j < N; ++j) {
@@ -1212,7 +1351,7 @@ extern "C" int orc_solve(void* h, const double* omegas, int nf, const double* he
     int st = lu_solve(N, Am.data(), nrhs, rhs.data());
     if (st) return st;
     const std::vector<cd>& phi = rhs;  // [nrhs][N]
-    cd iwr(0.0, w * p->rho);
+    cd iwr(0.0, limit ? 0.0 : w * p->rho);
     for (int i = 0; i < D; ++i) {
       int bi = i / 6, mi = i % 6;
       for (int c = 0; c < nrad; ++c) {
@@ -1222,7 +1361,7 @@ extern "C" int orc_solve(void* h, const double* omegas, int nf, const double* he
         z *= -p->rho;
         size_t o = ((size_t)fi * D + i) * nrad + c;
         Aout[o] = z.real();
-        Bout[o] = -w * z.imag();
+        Bout[o] = limit ? 0.0 : -w * z.imag();
       }
       for (int hh = 0; hh < nh; ++hh) {
         cd x(0, 0), fk(0, 0);

@@ -12,6 +12,7 @@ double orc_dispersion_x0(double H);
 double orc_dispersion_xn(double H, int n);
 int orc_finite_kernel(double h, double K, double R, double z, double zeta, double* out8);
 int orc_finite_phi(double H, int which, double xb, double vb, double* out3);
+int orc_finite_limit_kernel(int mode, double h, double R, double z, double zeta, double* out8);
 int orc_v_vx(double X, double Y, double* out2);
 int orc_yt(double X, double* out2);
 int orc_struve01(double x, double* out2);

@@ -51,6 +51,7 @@ def lib():
             ("orc_lu_solve", [ctypes.c_int, _dp, ctypes.c_int, _dp]),
             ("orc_finite_kernel", [ctypes.c_double] * 5 + [_dp]),
             ("orc_finite_phi", [ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]),
+            ("orc_finite_limit_kernel", [ctypes.c_int] + [ctypes.c_double] * 4 + [_dp]),
             ("orc_solve", [ctypes.c_void_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         ]:
             f = getattr(L, name)
@@ -117,6 +118,15 @@ def finite_kernel(h, K, R, z, zeta) -> np.ndarray:
     """(Wr, WrR, WrZ, Wrz, Wi, WiR, WiZ, Wiz) of the finite-depth regular part (green_fin.rs:1007)."""
     out = np.zeros(8)
     _check(lib().orc_finite_kernel(float(h), float(K), float(R), float(z), float(zeta), _p(out)), "finite_kernel")
+    return out
+
+
+def finite_limit_kernel(mode, h, R, z, zeta) -> np.ndarray:
+    """The finite-depth regular part at a frequency limit (mode 1: omega = 0, 2: omega = infinity,
+    green_fin.rs FreqMode::Zero / Infinite); the eight values of finite_kernel, Wi = 0."""
+    out = np.zeros(8)
+    _check(lib().orc_finite_limit_kernel(int(mode), float(h), float(R), float(z), float(zeta), _p(out)),
+           "finite_limit_kernel")
     return out
 
 

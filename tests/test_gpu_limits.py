@@ -1,7 +1,8 @@
 """The radiation limits omega = 0 and omega = infinity in infinite depth (GPU): the floating
 hemisphere's surge and heave added mass against the exact sphere series (tests/sphere_limits.py)
 on two meshes (discretization error O(h^2): it must fall with the mesh), the limit approached by
-small frequencies, limits interleaved with ordinary frequencies, and finite depth refused."""
+small frequencies, limits interleaved with ordinary frequencies. Finite depth:
+tests/test_gpu_limits_finite.py."""
 
 import math
 import sys
@@ -73,15 +74,3 @@ def test_limits_interleaved_with_ordinary_frequencies(session):
     assert np.array_equal(both["added_mass"][1], limits["added_mass"][1])
     assert np.array_equal(both["added_mass"][2], limits["added_mass"][0])
     assert np.all(both["excitation_diffraction"][[1, 2]] == 0.0)
-
-
-def test_limits_in_finite_depth_are_refused(session):
-    v = meshes.hemisphere_panels(1.0, 6, 12, quarter=False)
-    session.set_environment(RHO, G, 3.0)
-    session.set_panels(v)
-    try:
-        with pytest.raises(hb.HbgError) as e:
-            session.solve([0.0], [])
-        assert e.value.status == 3
-    finally:
-        session.set_environment(RHO, G)

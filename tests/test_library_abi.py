@@ -119,6 +119,26 @@ def test_wamit_limit_rows_on_host(tmp_path):
     assert len(t3) == 1 * H * D and all(float(r[:14]) == pytest.approx(2 * np.pi / 0.8, rel=5e-6) for r in t3)
 
 
+def test_wamit_limit_rows_skip_undefined_entries(tmp_path):
+    # finite depth, omega = 0: the heave row and column are NaN (a net volume flux): no rows for
+    # them, every other entry written as usual
+    import math
+
+    import numpy as np
+
+    D = 6
+    A = np.arange(1.0, 1.0 + 3 * D * D).reshape(3, D, D) * 10.0
+    A[0, 2, :] = A[0, :, 2] = math.nan
+    res = {"omegas": np.array([0.0, 0.8, math.inf]), "headings": np.array([0.0]), "added_mass": A,
+           "radiation_damping": np.zeros((3, D, D)), "excitation_diffraction": np.ones((3, 1, D), complex),
+           "excitation_haskind": np.ones((3, 1, D), complex)}
+    hb.wamit(tmp_path / "fd", res, density=1025.0, gravity=9.81, stamp="2026-01-01 00:00:00")
+    lines = (tmp_path / "fd.1").read_bytes().decode().split("\r\n")[1:-1]
+    zero = [r for r in lines if float(r[:14]) == -1.0]
+    assert len(zero) == D * D - (2 * D - 1) and len(lines) == 3 * D * D - (2 * D - 1)
+    assert all(int(r[14:20]) != 3 and int(r[20:26]) != 3 for r in zero)
+
+
 def test_wamit_force_parts_and_motions_on_host(tmp_path):
     # .2fk / .2sc: the Froude-Krylov force by solved column and Haskind - Froude-Krylov; .3fk / .3sc:
     # the Froude-Krylov and scattering parts; .4: the motions times L^ri (rotations), not divided by rho g

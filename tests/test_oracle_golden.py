@@ -225,6 +225,68 @@ def test_finite_kernel_against_mpmath_vectors(oracle):
     assert worst_im < 1e-13, worst_im
 
 
+def finite_limit_rows():
+    rows = []
+    for line in (GOLDEN / "green_fin_reference.txt").read_text().splitlines():
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.split()
+        if f[0] in ("1", "2"):  # the limits omega = 0 (1) and omega = inf (2)
+            rows.append([float(x) for x in f])
+    return np.array(rows)
+
+
+def test_finite_limit_kernels_against_mpmath_vectors(oracle):
+    # the mpmath rows of modes 1 and 2 (82 each, R/h 0 to 5, both depths in [-h, 0]): the image
+    # sums below R = h/2 to 1e-11 of the scale, the series above with the reference's stop at
+    # k_n R >= 18 (about 1e-7 in the derivatives), Im G = 0
+    d = finite_limit_rows()
+    assert len(d) == 164
+    worst = {(m, near): 0.0 for m in (1, 2) for near in (True, False)}
+    for mode, h, K, R, z, ze, *r in d:
+        ref, sc, scD = np.array(r[:8]), r[8], r[9]
+        g = oracle.finite_limit_kernel(int(mode), h, R, z, ze)
+        e = max(abs(g[i] - ref[i]) / (sc if i == 0 else scD) for i in range(4))
+        key = (int(mode), R < 0.5 * h)
+        worst[key] = max(worst[key], e)
+        assert np.all(g[4:] == 0.0)
+    for (mode, near), e in worst.items():
+        assert e < (1e-11 if near else 2e-6), (mode, near, e)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_finite_limit_kernel_forms_agree_at_the_switch(oracle, mode):
+    # the image sums (R < h/2) and the series (R >= h/2) are the same function: across the switch
+    # they differ by the series' truncation only
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        h = rng.uniform(1.0, 10.0)
+        z, ze = -h * rng.uniform(0, 1), -h * rng.uniform(0, 1)
+        a = oracle.finite_limit_kernel(mode, h, 0.5 * h * (1 - 1e-13), z, ze)
+        b = oracle.finite_limit_kernel(mode, h, 0.5 * h, z, ze)
+        assert np.abs(a[0] - b[0]) < 5e-8 / h
+        assert np.abs(a[1:4] - b[1:4]).max() < 2e-6 / h ** 2
+
+
+def test_finite_limits_tend_to_infinite_depth(oracle):
+    # h -> inf: omega = 0 is the rigid lid, 1/r + 1/r' (the finite-depth regular part tends to
+    # the bottom images' 1/r_p less the log term's constant, (2/h) log 2h vanishing); omega = inf
+    # is 1/r - 1/r': the remainder decays with the images
+    for R, z, ze in [(0.3, -0.2, -0.5), (1.5, -0.1, -0.1), (3.0, -1.0, -0.3)]:
+        d0, dinf = [], []
+        for h in (50.0, 100.0, 200.0):
+            g0 = oracle.finite_limit_kernel(1, h, R, z, ze)
+            gi = oracle.finite_limit_kernel(2, h, R, z, ze)
+            v = [z + ze + 2 * h, z - ze + 2 * h, -(z - ze) + 2 * h, z + ze + 4 * h]
+            rank0 = sum(1.0 / math.hypot(R, vv) for vv in v)
+            ranki = 1.0 / math.hypot(R, v[0]) - sum(1.0 / math.hypot(R, vv) for vv in v[1:])
+            d0.append(abs(g0[0] + rank0))
+            dinf.append(abs(gi[0] + ranki))
+        # omega = 0: G minus the near terms is -(2/h)(log(R/2h) + gamma) + O(1/h): O(log h / h)
+        assert d0[2] < d0[1] < d0[0] < 0.2, d0
+        assert dinf[2] < dinf[1] < dinf[0] < 1e-3, dinf
+
+
 def test_finite_depth_tends_to_infinite_depth(oracle):
     # h -> inf: the regular part plus the bottom images tends to the infinite-depth kernel
     # K Ft(X, V3) (the bottom's wave response decays algebraically, about h^-3); Im G tends to
