@@ -42,6 +42,36 @@ def test_refused_panels_on_a_fresh_session_leave_no_geometry():
         assert e.value.status == 4  # STATE: no panels set, nothing launched
 
 
+def panel_cases():
+    """Quads that the panel rule (geometry.rs:120-199) refuses, with the words of its message."""
+    sq = np.array([[0.0, 0.0, -1.0], [1.0, 0.0, -1.0], [1.0, 1.0, -1.0], [0.0, 1.0, -1.0]])
+    nan = sq.copy()
+    nan[2, 1] = np.nan
+    inf = sq.copy()
+    inf[0, 2] = -np.inf
+    point = np.repeat(sq[:1], 4, axis=0)
+    line = np.array([[0.0, 0.0, -1.0], [1.0, 0.0, -1.0], [2.0, 0.0, -1.0], [3.0, 0.0, -1.0]])
+    dart = np.array([[0.0, 0.0, -1.0], [2.0, 0.0, -1.0], [0.5, 0.5, -1.0], [0.0, 2.0, -1.0]])
+    bowtie = np.array([[0.0, 0.0, -1.0], [1.0, 1.0, -1.0], [1.0, 0.0, -1.0], [0.0, 1.0, -1.0]])
+    return [("nan", nan, "non-finite"), ("inf", inf, "non-finite"), ("point", point, "distinct vertices"),
+            ("line", line, "zero area"), ("dart", dart, "convex"), ("bow tie", bowtie, "")]
+
+
+@pytest.mark.parametrize("label,quad,words", panel_cases(), ids=[c[0] for c in panel_cases()])
+def test_panel_rule_refusals_name_the_panel(label, quad, words):
+    # failure injection at the ABI: the panel rule's refusals are HBG_ERR_VALUE naming the panel
+    # (1-based, as the reference's messages), before anything is built or launched
+    say(f"case: set_panels refuses a {label} quad as panel 5")
+    with hb.Session(0) as s:
+        v = np.array(meshes.hemisphere_panels(1.0, 4, 8, quarter=False), copy=True)
+        v[4] = quad
+        with pytest.raises(hb.HbgError) as e:
+            s.set_panels(v)
+        assert e.value.status == 1 and "panel 5" in e.value.message, e.value.message
+        assert words in e.value.message, e.value.message
+        assert s.n_panels == 0
+
+
 def test_refused_panels_keep_the_previous_geometry():
     say("case: valid geometry, refused set_panels / set_walls, the old geometry solves to the same bits")
     with hb.Session(0) as s:
