@@ -114,6 +114,10 @@ SIGNATURES = {
     "hbg_write_wamit_files": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
                                                ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_char_p]),
+    "hbg_write_wamit_all": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, _dp, ctypes.c_int32, _dp, ctypes.c_int32,
+                                             ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int32, _dp,
+                                             _dp, _dp, ctypes.c_char_p]),
 }
 
 
@@ -469,10 +473,12 @@ def dispersion_roots(H, n):
     return out
 
 
-def wamit(stem, result, density, gravity, reference_length=1.0, overwrite=True, stamp="", n_bodies=1, rao=None):
-    """The WAMIT files of a solve (the C ABI's hbg_write_wamit_files): .1, .2, .3, and .2fk, .2sc,
+def wamit(stem, result, density, gravity, reference_length=1.0, overwrite=True, stamp="", n_bodies=1, rao=None,
+          field_points=None, field=None):
+    """The WAMIT files of a solve (the C ABI's hbg_write_wamit_all): .1, .2, .3, and .2fk, .2sc,
     .3fk, .3sc when the result holds the Froude-Krylov and scattering forces, .4 with rao (the
-    motions [F][H][D] of Session.motions)."""
+    motions [F][H][D] of Session.motions), .fpt and .6p with field_points [P][3] and field (the
+    dict of Session.field_values)."""
     w = np.ascontiguousarray(result["omegas"], dtype=np.float64)
     h = np.ascontiguousarray(result["headings"], dtype=np.float64)
     A = np.ascontiguousarray(result["added_mass"], dtype=np.float64)
@@ -485,7 +491,11 @@ def wamit(stem, result, density, gravity, reference_length=1.0, overwrite=True, 
 
     FK, SC, RAO = opt(result.get("excitation_froude_krylov")), opt(result.get("excitation_scattering")), opt(rao)
     ptr = (lambda a: None if a is None else a.ctypes.data_as(_dp))
-    _check(library().hbg_write_wamit_files(str(stem).encode(), int(bool(overwrite)), float(density), float(gravity),
-                                           float(reference_length), _p(w), len(w), _p(h), len(h), int(n_bodies),
-                                           _p(A), _p(B), HK.ctypes.data_as(_dp), X.ctypes.data_as(_dp), ptr(FK),
-                                           ptr(SC), ptr(RAO), stamp.encode()))
+    pts = None if field_points is None else np.ascontiguousarray(field_points, dtype=np.float64).reshape(-1, 3)
+    frad = opt(field["frad"]) if field is not None else None
+    fdif = opt(field["ftot"]) if field is not None else None
+    _check(library().hbg_write_wamit_all(str(stem).encode(), int(bool(overwrite)), float(density), float(gravity),
+                                         float(reference_length), _p(w), len(w), _p(h), len(h), int(n_bodies),
+                                         _p(A), _p(B), HK.ctypes.data_as(_dp), X.ctypes.data_as(_dp), ptr(FK),
+                                         ptr(SC), ptr(RAO), 0 if pts is None else len(pts), ptr(pts), ptr(frad),
+                                         ptr(fdif), stamp.encode()))

@@ -165,7 +165,8 @@ int lu_depth();
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
                 const double* X, const double* FK, const double* SC, const double* RAO, const char* stamp,
-                std::string& err);
+                std::string& err, int P = 0, const double* pts = nullptr,
+                const double* frad = nullptr, const double* fdif = nullptr);
 
 namespace {
 
@@ -2786,6 +2787,26 @@ HbgStatus hbg_write_wamit_files(const char* stem, int32_t overwrite, double dens
   if (st) throw Error{st, err};
   return HBG_OK;
   GUARD_END("hbg_write_wamit_files")
+}
+
+HbgStatus hbg_write_wamit_all(const char* stem, int32_t overwrite, double density, double gravity,
+                              double reference_length, const double* omegas, int32_t n_omegas,
+                              const double* headings_deg, int32_t n_headings, int32_t n_bodies,
+                              const double* added_mass, const double* damping, const double* haskind,
+                              const double* excitation, const double* froude_krylov, const double* scattering,
+                              const double* rao, int32_t n_points, const double* points,
+                              const double* field_radiation, const double* field_diffraction, const char* stamp) {
+  if (!stem || !omegas || !added_mass || !damping || (n_headings > 0 && (!headings_deg || !haskind || !excitation)) ||
+      n_points < 0 || (n_points > 0 && !points))
+    return fail(HBG_ERR_ARGUMENT, "null argument or negative point count", "hbg_write_wamit_all");
+  GUARD_BEGIN
+  std::string err;
+  int st = write_wamit(stem, overwrite, density, gravity, reference_length, omegas, n_omegas, headings_deg,
+                       n_headings, n_bodies, added_mass, damping, haskind, excitation, froude_krylov, scattering, rao,
+                       stamp, err, n_points, points, field_radiation, n_headings > 0 ? field_diffraction : nullptr);
+  if (st) throw Error{st, err};
+  return HBG_OK;
+  GUARD_END("hbg_write_wamit_all")
 }
 
 }  // extern "C"

@@ -8,6 +8,8 @@
 //   .2fk, .2sc  the same for its Froude-Krylov part and Haskind - Froude-Krylov (:836-853)
 //   .3  the same columns for the pressure-integrated force; .3fk, .3sc its Froude-Krylov and
 //       scattering parts; .4 the motions (RAO) xi L^ri (:855-866)
+//   .6p field-point elevations (radiation block, then diffraction block per frequency), .fpt the
+//       field points (:938-984)
 // (each written when its arrays are given; the limits have no rows in them)
 // Period 2 pi / w in E14.6 0P form, values E14.6 1P, integers I6, CRLF line ends, a header line
 // " HydroBEM <version>   <file>   <stamp>"; the files are written to temporary names and renamed
@@ -86,11 +88,14 @@ bool complex_cols(std::complex<double> z, std::string& out) {
 
 }  // namespace
 
-// A, B [F][D][D]; HK [F][D][H] complex (Output.HK order); X [F][H][D] complex.
+// A, B [F][D][D]; HK [F][D][H] complex (Output.HK order); X [F][H][D] complex. Field points
+// (P > 0): pts [P][3] global, frad [F][P][D] and fdif [F][P][H] complex, the elevations (p / (rho g)
+// below z = 0) per unit displacement of the radiation columns and of the total diffraction wave.
 int write_wamit(const char* stem, int overwrite, double rho, double g, double L, const double* omegas, int F,
                 const double* heads, int H, int nb, const double* A, const double* B, const double* HK,
                 const double* X, const double* FK, const double* SC, const double* RAO, const char* stamp,
-                std::string& err) {
+                std::string& err, int P = 0, const double* pts = nullptr, const double* frad = nullptr,
+                const double* fdif = nullptr) {
   const int D = 6 * nb, K = D;
   if (!(rho > 0.0) || !(g > 0.0) || !(L > 0.0)) {
     err = "density, gravity and reference_length must be > 0";
@@ -187,6 +192,52 @@ int write_wamit(const char* stem, int overwrite, double rho, double g, double L,
     }
     if (SC && !force_file(".3sc", fhd(SC), force_scale)) return 1;
     if (RAO && !force_file(".4", fhd(RAO), rao_scale)) return 1;
+  }
+  if (P > 0 && pts) {
+    // .6p (wamit_files.rs:938-974, file-formats.md section 2): per ordinary frequency the
+    // radiation block (the points in order, (Re, Im) of eta / L^r_k per column on each row), then
+    // the diffraction block (per heading the points, Re, Im of eta / A); .fpt the points once
+    auto e1p = [&](double x, std::string& l) { return fortran_e(x, true, 14, 6, l); };
+    if (frad || fdif) {
+      std::vector<std::string> lines;
+      for (int f = 0; f < F; ++f) {
+        if (limit(f)) continue;
+        for (int q = 0; frad && K > 0 && q < P; ++q) {
+          std::string l = per[f];
+          bool ok = fortran_i(q + 1, 10, l);
+          for (int k = 0; k < K && ok; ++k) {
+            const C z = at(frad, ((size_t)f * P + q) * K + k) * std::pow(L, -r(k));
+            ok = e1p(z.real(), l) && e1p(z.imag(), l);
+          }
+          if (!ok) {
+            err = "field value does not fit E14.6";
+            return 1;
+          }
+          lines.push_back(l);
+        }
+        for (int h = 0; fdif && h < H; ++h)
+          for (int q = 0; q < P; ++q) {
+            std::string l = per[f] + bcol[h];
+            const C z = at(fdif, ((size_t)f * P + q) * H + h);
+            if (!fortran_i(q + 1, 10, l) || !e1p(z.real(), l) || !e1p(z.imag(), l)) {
+              err = "field value does not fit E14.6";
+              return 1;
+            }
+            lines.push_back(l);
+          }
+      }
+      files.push_back({".6p", lines});
+    }
+    std::vector<std::string> lines;
+    for (int q = 0; q < P; ++q) {
+      std::string l;
+      if (!fortran_i(q + 1, 10, l) || !e1p(pts[3 * q], l) || !e1p(pts[3 * q + 1], l) || !e1p(pts[3 * q + 2], l)) {
+        err = "field point does not fit E14.6";
+        return 1;
+      }
+      lines.push_back(l);
+    }
+    files.push_back({".fpt", lines});
   }
   const std::string sstem(stem);
   if (!overwrite)

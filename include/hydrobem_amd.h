@@ -406,6 +406,19 @@ HbgStatus hbg_write_wamit_files(const char* stem, int32_t overwrite, double dens
                                 const double* added_mass, const double* damping, const double* haskind,
                                 const double* excitation, const double* froude_krylov, const double* scattering,
                                 const double* rao, const char* stamp);
+/* The same and, with n_points field points (points [P][3] global, as hbg_set_field_points), .fpt
+ * (point, x, y, z) and .6p (wamit_files.rs:938-984; file-formats.md section 2: per ordinary
+ * frequency the radiation block, the points in order with (Re, Im) of eta / L^r_k for every
+ * column, then the diffraction block, per heading the points with Re, Im of eta / A):
+ * field_radiation [F][P][D] and field_diffraction [F][P][H] complex, hbg_field_values's frad and
+ * ftot (either NULL: its block is left out; both NULL: no .6p). */
+HbgStatus hbg_write_wamit_all(const char* stem, int32_t overwrite, double density, double gravity,
+                              double reference_length, const double* omegas, int32_t n_omegas,
+                              const double* headings_deg, int32_t n_headings, int32_t n_bodies,
+                              const double* added_mass, const double* damping, const double* haskind,
+                              const double* excitation, const double* froude_krylov, const double* scattering,
+                              const double* rao, int32_t n_points, const double* points,
+                              const double* field_radiation, const double* field_diffraction, const char* stamp);
 
 #ifdef __cplusplus
 }

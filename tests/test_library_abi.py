@@ -139,6 +139,50 @@ def test_wamit_limit_rows_skip_undefined_entries(tmp_path):
     assert all(int(r[14:20]) != 3 and int(r[20:26]) != 3 for r in zero)
 
 
+def test_wamit_field_point_files_on_host(tmp_path):
+    # .fpt: the points once; .6p: per ordinary frequency the radiation block (points in order, Re,
+    # Im of eta / L^r per column) then the diffraction block (per heading the points, Re, Im)
+    import math
+
+    import numpy as np
+
+    F, H, D, P, L = 2, 2, 6, 3, 2.0
+    rng = np.random.default_rng(3)
+    c = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)  # noqa: E731
+    res = {"omegas": np.array([0.7, 1.4]), "headings": np.array([0.0, 45.0]), "added_mass": rng.normal(size=(F, D, D)),
+           "radiation_damping": rng.normal(size=(F, D, D)), "excitation_diffraction": c(F, H, D),
+           "excitation_haskind": c(F, D, H)}
+    pts = np.array([[3.0, 0.0, 0.0], [0.0, 4.5, -1.0], [-2.0, -2.0, -0.25]])
+    field = {"frad": c(F, P, D), "ftot": c(F, P, H)}
+    hb.wamit(tmp_path / "fp", res, density=1025.0, gravity=9.81, reference_length=L, stamp="2026-01-01 00:00:00",
+             field_points=pts, field=field)
+    fpt = (tmp_path / "fp.fpt").read_bytes().decode().split("\r\n")[1:-1]
+    assert len(fpt) == P and all(len(r) == 10 + 3 * 14 for r in fpt)
+    assert [int(r[:10]) for r in fpt] == [1, 2, 3]
+    assert np.allclose([[float(r[10 + 14 * j:24 + 14 * j]) for j in range(3)] for r in fpt], pts, rtol=1e-6)
+    rows = (tmp_path / "fp.6p").read_bytes().decode().split("\r\n")[1:-1]
+    assert len(rows) == F * (P + H * P)
+    for f in range(F):
+        block = rows[f * (P + H * P):(f + 1) * (P + H * P)]
+        per = 2 * math.pi / res["omegas"][f]
+        for q in range(P):
+            r = block[q]
+            assert len(r) == 14 + 10 + 2 * 14 * D and int(r[14:24]) == q + 1
+            assert float(r[:14]) == pytest.approx(per, rel=5e-6)
+            for k in range(D):
+                z = field["frad"][f, q, k] / L ** (1 if k >= 3 else 0)
+                assert float(r[24 + 28 * k:38 + 28 * k]) == pytest.approx(z.real, rel=1e-6, abs=1e-12)
+                assert float(r[38 + 28 * k:52 + 28 * k]) == pytest.approx(z.imag, rel=1e-6, abs=1e-12)
+        for h in range(H):
+            for q in range(P):
+                r = block[P + h * P + q]
+                assert len(r) == 14 + 14 + 10 + 28 and int(r[28:38]) == q + 1
+                assert float(r[14:28]) == res["headings"][h]
+                z = field["ftot"][f, q, h]
+                assert float(r[38:52]) == pytest.approx(z.real, rel=1e-6) and float(r[52:66]) == pytest.approx(
+                    z.imag, rel=1e-6)
+
+
 def test_wamit_force_parts_and_motions_on_host(tmp_path):
     # .2fk / .2sc: the Froude-Krylov force by solved column and Haskind - Froude-Krylov; .3fk / .3sc:
     # the Froude-Krylov and scattering parts; .4: the motions times L^ri (rotations), not divided by rho g
