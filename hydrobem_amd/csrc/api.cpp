@@ -154,6 +154,10 @@ int lu_autotune(int precision64, int n, int nrhs, int max_batch, size_t mem_byte
                 int max_rows, int* rows);
 void lu_set_graphs(int on);
 int lu_graphs();
+void lu_set_sub(int q);
+void lu_set_sub32(int q);
+int lu_sub();
+int lu_sub32();
 void lu_set_gemm_variant(int v);
 int lu_gemm_variant();
 void lu_set_gemm_variant32(int v);
@@ -2656,7 +2660,7 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       lu_set_gemm_variant(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH:
-      if (value < 1 || value > 16) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..16", "hbg_set_tuning");
+      if (value < 1 || value > 32) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..32", "hbg_set_tuning");
       lu_set_depth(value);
       return HBG_OK;
     case HBG_TUNE_LU_GRAPHS:
@@ -2668,13 +2672,21 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       lu_set_gemm_variant32(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH_MIXED:
-      if (value < 1 || value > 16) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..16", "hbg_set_tuning");
+      if (value < 1 || value > 32) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..32", "hbg_set_tuning");
       lu_set_depth32(value);
       return HBG_OK;
     case HBG_TUNE_MIXED32_TOL:
       if (value < 6 || value > 14) return fail(HBG_ERR_VALUE, "refinement tolerance exponent must be in 6..14",
                                                "hbg_set_tuning");
       g_mixed32_tol_exp = value;
+      return HBG_OK;
+    case HBG_TUNE_LU_SUB:
+    case HBG_TUNE_LU_SUB_MIXED:
+      if (value < 0 || value > 32) return fail(HBG_ERR_VALUE, "LU sub-block: 0..32 panels", "hbg_set_tuning");
+      if (key == HBG_TUNE_LU_SUB)
+        lu_set_sub(value);
+      else
+        lu_set_sub32(value);
       return HBG_OK;
     case HBG_TUNE_NN_RMIN:
       if (value < 10 || value > 2000)
@@ -2709,6 +2721,12 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value) {
       return HBG_OK;
     case HBG_TUNE_NN_RMIN:
       *value = (int32_t)std::lround(s->nn_rmin * 100.0);
+      return HBG_OK;
+    case HBG_TUNE_LU_SUB:
+      *value = lu_sub();
+      return HBG_OK;
+    case HBG_TUNE_LU_SUB_MIXED:
+      *value = lu_sub32();
       return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_get_tuning");

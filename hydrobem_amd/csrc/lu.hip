@@ -938,6 +938,7 @@ struct LuWorkspace {
 // tuned settings per arithmetic: [0] complex128, [1] complex64
 static int g_gemm_variant_[2] = {1, 1};
 static int g_lu_depth_[2] = {4, 4};
+static int g_lu_sub_[2] = {0, 0};  // panels per sub-block of a block (0: none)
 template <typename T>
 constexpr int prec_index() {
   return std::is_same<T, double>::value ? 0 : 1;
@@ -953,6 +954,10 @@ void lu_set_depth32(int q) { g_lu_depth_[1] = q; }
 int lu_depth32() { return g_lu_depth_[1]; }
 void lu_set_depth(int q) { g_lu_depth_[0] = g_lu_depth_[1] = q; }
 int lu_depth() { return g_lu_depth_[0]; }
+void lu_set_sub(int q) { g_lu_sub_[0] = g_lu_sub_[1] = q; }
+void lu_set_sub32(int q) { g_lu_sub_[1] = q; }
+int lu_sub() { return g_lu_sub_[0]; }
+int lu_sub32() { return g_lu_sub_[1]; }
 
 template <typename T>
 static LuWorkspace<T>& workspace() {
@@ -969,12 +974,12 @@ namespace {
 struct GraphKey {
   const void* M;
   size_t mstride;
-  int batch, n, lda, nrhs, variant, depth, real_bytes, uneven;
+  int batch, n, lda, nrhs, variant, depth, real_bytes, uneven, sub;
   unsigned long generation;
   bool operator==(const GraphKey& o) const {
     return M == o.M && mstride == o.mstride && batch == o.batch && n == o.n && lda == o.lda && nrhs == o.nrhs &&
            variant == o.variant && depth == o.depth && real_bytes == o.real_bytes && uneven == o.uneven &&
-           generation == o.generation;
+           sub == o.sub && generation == o.generation;
   }
 };
 struct GraphCache {
@@ -1144,6 +1149,13 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
   const bool split = std::is_same<T, float>::value && g_gemm_variant_[prec_index<T>()] >= 3;
   SplitOps so{ws.ls, ws.us, ws.split_pad, ws.split_stride(), ws.split_stride(), 0};
   const SplitOps* sop = split ? &so : nullptr;
+  // sub-blocks of S panels inside a block: a panel's columns and rows are updated by the earlier
+  // panels of its sub-block only; at the end of a sub-block the block's later columns (all rows
+  // below) and later rows (the columns past the block) take the sub-block's panels in two wide
+  // products (an L-shaped update), so that most of the block's in-block work runs as large GEMMs
+  // instead of thin strips whose depth grows panel by panel. The trailing matrix past the block
+  // still takes the whole block at once.
+  const int S = g_lu_sub_[prec_index<T>()] > 0 && g_lu_sub_[prec_index<T>()] < Q ? g_lu_sub_[prec_index<T>()] : Q;
   for (int K0 = 0; K0 < n; K0 += Q * NB) {
     const int qend = std::min(n, K0 + Q * NB);
     so.K0 = K0;
@@ -1151,18 +1163,19 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int p = k0 / NB;
       const int kb = min(NB, n - k0);
       const int m = n - k0;
+      const int Ks = K0 + (k0 - K0) / (S * NB) * (S * NB);  // the panel's sub-block
       AuxStream* ax = nullptr;
-      if (k0 > K0 && one_stream) {
-        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, gb, sop, stream, gt);
-        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, gb, sop, stream, gt);
-      } else if (k0 > K0) {
-        gemm_sub<T>(M, lda, k0, m, k0, kb, K0, k0 - K0, gb, sop, stream, gt);
+      if (k0 > Ks && one_stream) {
+        gemm_sub<T>(M, lda, k0, m, k0, kb, Ks, k0 - Ks, gb, sop, stream, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, Ks, k0 - Ks, gb, sop, stream, gt);
+      } else if (k0 > Ks) {
+        gemm_sub<T>(M, lda, k0, m, k0, kb, Ks, k0 - Ks, gb, sop, stream, gt);
         // the panel's rows (the columns right of it) are needed only by apply_kernel: they are
         // updated on the second stream beside the diagonal block (disjoint parts of M)
         ax = &aux_stream();
         HBG_HIP(hipEventRecord(ax->fork, stream));
         HBG_HIP(hipStreamWaitEvent(ax->s, ax->fork, 0));
-        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, K0, k0 - K0, gb, sop, ax->s, gt);
+        gemm_sub<T>(M, lda, k0, kb, k0 + kb, ncols - k0 - kb, Ks, k0 - Ks, gb, sop, ax->s, gt);
         HBG_HIP(hipEventRecord(ax->join, ax->s));
       }
       const size_t po = (size_t)p * NB * NB;
@@ -1174,6 +1187,11 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
         apply_kernel<T><<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, ws.uinv + po,
                                                                      ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs,
                                                                      so, split && k0 + NB < n ? 1 : 0);
+      const int se = k0 + kb;  // the end of a sub-block before the block's end: the L-shaped update
+      if (S < Q && se < qend && (se - K0) % (S * NB) == 0) {
+        gemm_sub<T>(M, lda, se, n - se, se, qend - se, Ks, se - Ks, gb, sop, stream, gt);
+        gemm_sub<T>(M, lda, se, qend - se, qend, ncols - qend, Ks, se - Ks, gb, sop, stream, gt);
+      }
     }
     gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, gb, sop, stream, gt);
   }
@@ -1297,7 +1315,8 @@ void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, 
     // the launch sequence depends only on these; replayed from a captured hipGraph
     const GraphKey key{(const void*)M, mstride, batch, n, lda, nrhs, g_gemm_variant_[prec_index<T>()],
                        g_lu_depth_[prec_index<T>()],
-                       std::is_same<T, double>::value ? 8 : 4, nsys ? 1 : 0, ws.generation};
+                       std::is_same<T, double>::value ? 8 : 4, nsys ? 1 : 0, g_lu_sub_[prec_index<T>()],
+                       ws.generation};
     GraphCache& gc = graphs();
     hipGraphExec_t exec = gc.find(key);
     if (!exec) {
@@ -1388,12 +1407,13 @@ __global__ void synth_kernel(typename Cplx<T>::type* M, size_t count, int n, int
 
 }  // namespace
 
-// Time the batched LU on synthetic systems (n, nrhs) in two stages: every GEMM tile variant (1, 2;
-// complex64 also 3, 4, the split-FP16 kernels) and block depth (2, 4, 8, 12, 16) at the largest
-// batch of 4, 8, 16, 32, 48 that is at most max_batch and fits in mem_bytes, then the smaller
-// batches at the fastest variant and depth; keeps the fastest variant and depth and returns the
-// fastest batch (by time per system). rows of report: variant, depth, batch, ms per system (up to
-// max_rows rows).
+// Time the batched LU on synthetic systems (n, nrhs) in three stages: every GEMM tile variant (1,
+// 2; complex64 also 3 to 7, the split-FP16 kernels) and block depth (2, 4, 8, 12, 16) at the
+// largest batch of 4, 8, 16, 32, 48 that is at most max_batch and fits in mem_bytes; deeper blocks
+// with sub-blocks (depth, sub) = (16, 8), (24, 8), (24, 12), (32, 8), (32, 16) at the fastest
+// variant; the smaller batches at the fastest of all. Keeps the fastest variant, depth and
+// sub-block and returns the fastest batch (by time per system). rows of report: variant, depth,
+// sub-block, batch, ms per system (up to max_rows rows).
 template <typename T>
 int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t stream, double* report, int max_rows,
                 int* rows) {
@@ -1411,22 +1431,25 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   HBG_HIP(hipEventCreate(&e1));
   int& gv = g_gemm_variant_[prec_index<T>()];
   int& gd = g_lu_depth_[prec_index<T>()];
-  const int old_v = gv, old_d = gd;
+  int& gs = g_lu_sub_[prec_index<T>()];
+  const int old_v = gv, old_d = gd, old_s = gs;
   double best = 1e300;
-  int bv = old_v, bd = old_d, bb = batches.back();
+  int bv = old_v, bd = old_d, bs = old_s, bb = batches.back();
   *rows = 0;
   std::vector<int> status(batches.back());
   try {
     const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4, 5, 6, 7}
                                                                       : std::vector<int>{1, 2};
-    std::vector<std::array<int, 3>> configs;  // variant, depth, batch
+    std::vector<std::array<int, 4>> configs;  // variant, depth, sub-block, batch
     for (int v : variants)
-      for (int d : {2, 4, 8, 12, 16}) configs.push_back({v, d, batches.back()});
+      for (int d : {2, 4, 8, 12, 16}) configs.push_back({v, d, 0, batches.back()});
+    const size_t stage1 = configs.size(), stage2 = stage1 + 5;
     for (size_t c = 0; c < configs.size(); ++c) {
-      const int v = configs[c][0], d = configs[c][1], b = configs[c][2];
+      const int v = configs[c][0], d = configs[c][1], sb = configs[c][2], b = configs[c][3];
       {
           gv = v;
           gd = d;
+          gs = sb;
           double ms_sys = 0.0;
           for (int rep = 0; rep < 2; ++rep) {  // the first captures the graph
             synth_kernel<T><<<4096, 256, 0, stream>>>(M, ms * b, n, n, 1234567ull + rep);
@@ -1439,26 +1462,34 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
             ms_sys = t / b;
           }
           if (*rows < max_rows && report) {
-            double* r = report + 4 * (*rows);
+            double* r = report + 5 * (*rows);
             r[0] = v;
             r[1] = d;
-            r[2] = b;
-            r[3] = ms_sys;
+            r[2] = sb;
+            r[3] = b;
+            r[4] = ms_sys;
             ++*rows;
           }
           if (ms_sys < best) {
             best = ms_sys;
             bv = v;
             bd = d;
+            bs = sb;
             bb = b;
           }
       }
-      if (c + 1 == variants.size() * 5)  // second stage: the smaller batches at the fastest variant and depth
-        for (size_t i = 0; i + 1 < batches.size(); ++i) configs.push_back({bv, bd, batches[i]});
+      // second stage: deeper blocks with sub-blocks at the fastest variant; third: the smaller
+      // batches at the fastest of all
+      if (c + 1 == stage1)
+        for (const auto& ds : {std::array<int, 2>{16, 8}, {24, 8}, {24, 12}, {32, 8}, {32, 16}})
+          configs.push_back({bv, ds[0], ds[1], batches.back()});
+      if (c + 1 == stage2)
+        for (size_t i = 0; i + 1 < batches.size(); ++i) configs.push_back({bv, bd, bs, batches[i]});
     }
   } catch (...) {
     gv = old_v;
     gd = old_d;
+    gs = old_s;
     (void)hipFree(M);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1466,6 +1497,7 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   }
   gv = bv;
   gd = bd;
+  gs = bs;
   graphs().clear();  // the captures hold the tuning buffer
   (void)hipFree(M);
   HBG_HIP(hipEventDestroy(e0));

@@ -106,3 +106,46 @@ def test_mixed_split_fp16_gemm(session, variant):
         session.set_profiling(False)
         session.set_tuning(session.TUNE_LU_GRAPHS, 1)
         session.set_tuning(session.TUNE_LU_GEMM_MIXED, old)
+
+
+@pytest.mark.parametrize("depth,sub", [(4, 2), (8, 3), (32, 8)])
+def test_lu_sub_blocks(session, depth, sub):
+    """Sub-blocks inside the LU's blocks (the in-block updates from the panel's sub-block, an
+    L-shaped update at each sub-block's end): FP64 to round-off of the plain blocked LU, and the
+    complex64 LU (split-FP16 GEMM) FP32-accurate before its refinement."""
+    say(f"case: LU depth {depth}, sub-blocks of {sub} panels, cylinder 2304 panels, finite depth, 3 frequencies")
+    v = meshes.cylinder_panels(1.0, 1.0, 48, 24, 24, quarter=False, cosine=True)
+    w = np.sqrt(9.81 * np.array([0.05, 1.0, 6.0]) / 3.0)
+    session.set_environment(1025.0, 9.81, 3.0)
+    session.set_panels(v)
+    keys = ("added_mass", "radiation_damping", "excitation_diffraction")
+    old = [session.get_tuning(k) for k in (session.TUNE_LU_DEPTH, session.TUNE_LU_DEPTH_MIXED,
+                                            session.TUNE_LU_GEMM_MIXED)]
+    plain = session.solve(w, [0.0])
+    try:
+        session.set_tuning(session.TUNE_LU_DEPTH, depth)
+        session.set_tuning(session.TUNE_LU_SUB, sub)
+        assert session.get_tuning(session.TUNE_LU_SUB_MIXED) == sub
+        sb = session.solve(w, [0.0])
+        for key in keys:
+            e = rel_per_entry(sb[key], plain[key])
+            say(f"  fp64 {key}: {e:.2e}")
+            assert e < 1e-11
+        session.set_tuning(session.TUNE_LU_GEMM_MIXED, 6)
+        session.set_profiling(True)
+        mixed = session.solve(w, [0.0], precision="mixed32")
+        prof = session.profile()
+        say(f"  mixed32: first-solution backward error {prof['mixed_max_first_backward_error']:.2e}, "
+            f"refinement steps {prof['mixed_refinement_steps']:.0f}")
+        assert 0.0 < prof["mixed_max_first_backward_error"] < 1e-5
+        assert prof["mixed_fp64_fallbacks"] == 0
+        for key in keys:
+            e = rel_per_entry(mixed[key], plain[key])
+            say(f"  mixed32 {key}: {e:.2e}")
+            assert e < 1e-5
+    finally:
+        session.set_profiling(False)
+        session.set_tuning(session.TUNE_LU_SUB, 0)
+        session.set_tuning(session.TUNE_LU_DEPTH, old[0])
+        session.set_tuning(session.TUNE_LU_DEPTH_MIXED, old[1])
+        session.set_tuning(session.TUNE_LU_GEMM_MIXED, old[2])
