@@ -329,6 +329,7 @@ def main():
     ap.add_argument("--gemm", type=int, default=-1, help="LU GEMM variant (-1: the library's default)")
     ap.add_argument("--depth", type=int, default=-1, help="LU panels per trailing update (-1: the library's default)")
     ap.add_argument("--sub", type=int, default=-1, help="LU panels per sub-block of a block (-1: the library's default)")
+    ap.add_argument("--sub2", type=int, default=-1, help="LU panels per sub-block of a sub-block (-1: the default)")
     ap.add_argument("--paths", default="fp64,mixed,mixed_nn,mixed32",
                     help="paths timed; `value` is the fastest whose A, B, X are within 1e-5 of the FP64 path's")
     ap.add_argument("--no-autotune", action="store_true", help="skip the LU autotuner before each path")
@@ -386,6 +387,8 @@ def main():
         s.set_tuning(s.TUNE_LU_DEPTH, a.depth)
     if a.sub >= 0:
         s.set_tuning(s.TUNE_LU_SUB, a.sub)
+    if a.sub2 >= 0:
+        s.set_tuning(s.TUNE_LU_SUB2, a.sub2)
     s.set_tuning(s.TUNE_NN_RMIN, int(round(100 * a.nn_rmin)))
     print(f"[bench] workload {w['name']}: {N} panels, {F} frequencies", file=sys.stderr, flush=True)
     paths = [p for p in a.paths.split(",") if p and not (a.symmetry and p == "mixed_nn")]
@@ -406,13 +409,13 @@ def main():
             prec = "fp64" if p == "fp64" else "mixed"
             if prec not in tuning:
                 rows = s.autotune(N, 6 + len(heads), prec)
-                print(f"[bench] autotune {prec} candidates (variant, depth, sub-block, batch, ms per system): " +
-                      "; ".join(f"{int(r[0])},{int(r[1])},{int(r[2])},{int(r[3])}: {r[4]:.3f}" for r in rows),
-                      file=sys.stderr, flush=True)
-                best = min(rows, key=lambda r: r[4])
+                print(f"[bench] autotune {prec} candidates (variant, depth, sub-blocks, batch, ms per system): " +
+                      "; ".join(f"{int(r[0])},{int(r[1])},{int(r[2])}/{int(r[3])},{int(r[4])}: {r[5]:.3f}"
+                                for r in rows), file=sys.stderr, flush=True)
+                best = min(rows, key=lambda r: r[5])
                 tuning[prec] = {"gemm_variant": int(best[0]), "gemm_tiles": VARIANTS[int(best[0])][0],
-                                "panels_per_update": int(best[1]), "panels_per_sub_block": int(best[2]),
-                                "batch": int(best[3]), "ms_per_system": round(best[4], 3), "candidates": len(rows)}
+                                "panels_per_update": int(best[1]), "panels_per_sub_block": [int(best[2]), int(best[3])],
+                                "batch": int(best[4]), "ms_per_system": round(best[5], 3), "candidates": len(rows)}
                 print(f"[bench] autotuned {prec}: {tuning[prec]}", file=sys.stderr, flush=True)
             if a.batch == 0:
                 s.set_batch(tuning[prec]["batch"])

@@ -360,11 +360,13 @@ class Session:
     TUNE_NN_RMIN = 6
     TUNE_LU_SUB = 7
     TUNE_LU_SUB_MIXED = 8
+    TUNE_LU_SUB2 = 9
+    TUNE_LU_SUB2_MIXED = 10
 
     def autotune(self, n, nrhs, precision="fp64") -> list:
         """Time the batched LU variants on synthetic systems and keep the fastest (hbg_autotune);
-        returns the rows (variant, depth, sub-block, batch, ms per system)."""
-        rep = np.zeros((128, 5))
+        returns the rows (variant, depth, sub-block, its sub-block, batch, ms per system)."""
+        rep = np.zeros((128, 6))
         rows = ctypes.c_int32(0)
         _check(library().hbg_autotune(self._h, int(n), int(nrhs), PRECISIONS[precision], _p(rep), len(rep),
                                       ctypes.byref(rows)))

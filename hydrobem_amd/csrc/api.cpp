@@ -154,6 +154,10 @@ int lu_autotune(int precision64, int n, int nrhs, int max_batch, size_t mem_byte
                 int max_rows, int* rows);
 void lu_set_graphs(int on);
 int lu_graphs();
+void lu_set_sub2(int q);
+void lu_set_sub2_32(int q);
+int lu_sub2();
+int lu_sub2_32();
 void lu_set_sub(int q);
 void lu_set_sub32(int q);
 int lu_sub();
@@ -2660,7 +2664,7 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       lu_set_gemm_variant(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH:
-      if (value < 1 || value > 32) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..32", "hbg_set_tuning");
+      if (value < 1 || value > 64) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..64", "hbg_set_tuning");
       lu_set_depth(value);
       return HBG_OK;
     case HBG_TUNE_LU_GRAPHS:
@@ -2672,7 +2676,7 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       lu_set_gemm_variant32(value);
       return HBG_OK;
     case HBG_TUNE_LU_DEPTH_MIXED:
-      if (value < 1 || value > 32) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..32", "hbg_set_tuning");
+      if (value < 1 || value > 64) return fail(HBG_ERR_VALUE, "LU block depth must be in 1..64", "hbg_set_tuning");
       lu_set_depth32(value);
       return HBG_OK;
     case HBG_TUNE_MIXED32_TOL:
@@ -2682,11 +2686,17 @@ HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value) {
       return HBG_OK;
     case HBG_TUNE_LU_SUB:
     case HBG_TUNE_LU_SUB_MIXED:
-      if (value < 0 || value > 32) return fail(HBG_ERR_VALUE, "LU sub-block: 0..32 panels", "hbg_set_tuning");
+    case HBG_TUNE_LU_SUB2:
+    case HBG_TUNE_LU_SUB2_MIXED:
+      if (value < 0 || value > 64) return fail(HBG_ERR_VALUE, "LU sub-block: 0..64 panels", "hbg_set_tuning");
       if (key == HBG_TUNE_LU_SUB)
         lu_set_sub(value);
-      else
+      else if (key == HBG_TUNE_LU_SUB_MIXED)
         lu_set_sub32(value);
+      else if (key == HBG_TUNE_LU_SUB2)
+        lu_set_sub2(value);
+      else
+        lu_set_sub2_32(value);
       return HBG_OK;
     case HBG_TUNE_NN_RMIN:
       if (value < 10 || value > 2000)
@@ -2727,6 +2737,12 @@ HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value) {
       return HBG_OK;
     case HBG_TUNE_LU_SUB_MIXED:
       *value = lu_sub32();
+      return HBG_OK;
+    case HBG_TUNE_LU_SUB2:
+      *value = lu_sub2();
+      return HBG_OK;
+    case HBG_TUNE_LU_SUB2_MIXED:
+      *value = lu_sub2_32();
       return HBG_OK;
     default:
       return fail(HBG_ERR_VALUE, "unknown tuning key " + std::to_string(key), "hbg_get_tuning");

@@ -938,7 +938,8 @@ struct LuWorkspace {
 // tuned settings per arithmetic: [0] complex128, [1] complex64
 static int g_gemm_variant_[2] = {1, 1};
 static int g_lu_depth_[2] = {4, 4};
-static int g_lu_sub_[2] = {0, 0};  // panels per sub-block of a block (0: none)
+static int g_lu_sub_[2] = {0, 0};   // panels per sub-block of a block (0: none)
+static int g_lu_sub2_[2] = {0, 0};  // panels per sub-block of a sub-block (0: none)
 template <typename T>
 constexpr int prec_index() {
   return std::is_same<T, double>::value ? 0 : 1;
@@ -954,6 +955,10 @@ void lu_set_depth32(int q) { g_lu_depth_[1] = q; }
 int lu_depth32() { return g_lu_depth_[1]; }
 void lu_set_depth(int q) { g_lu_depth_[0] = g_lu_depth_[1] = q; }
 int lu_depth() { return g_lu_depth_[0]; }
+void lu_set_sub2(int q) { g_lu_sub2_[0] = g_lu_sub2_[1] = q; }
+void lu_set_sub2_32(int q) { g_lu_sub2_[1] = q; }
+int lu_sub2() { return g_lu_sub2_[0]; }
+int lu_sub2_32() { return g_lu_sub2_[1]; }
 void lu_set_sub(int q) { g_lu_sub_[0] = g_lu_sub_[1] = q; }
 void lu_set_sub32(int q) { g_lu_sub_[1] = q; }
 int lu_sub() { return g_lu_sub_[0]; }
@@ -1155,7 +1160,12 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
   // products (an L-shaped update), so that most of the block's in-block work runs as large GEMMs
   // instead of thin strips whose depth grows panel by panel. The trailing matrix past the block
   // still takes the whole block at once.
-  const int S = g_lu_sub_[prec_index<T>()] > 0 && g_lu_sub_[prec_index<T>()] < Q ? g_lu_sub_[prec_index<T>()] : Q;
+  // Levels nest: sub-blocks of S1 panels in a block, of S2 panels in a sub-block (either 0: none);
+  // at the end of a sub-block that is not its parent's end, the L-shaped update over the parent's
+  // later columns and rows, with the sub-block's panels.
+  int sz[3] = {Q, 0, 0}, L = 0;
+  for (int v : {g_lu_sub_[prec_index<T>()], g_lu_sub2_[prec_index<T>()]})
+    if (v > 0 && v < sz[L]) sz[++L] = v;
   for (int K0 = 0; K0 < n; K0 += Q * NB) {
     const int qend = std::min(n, K0 + Q * NB);
     so.K0 = K0;
@@ -1163,7 +1173,13 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
       const int p = k0 / NB;
       const int kb = min(NB, n - k0);
       const int m = n - k0;
-      const int Ks = K0 + (k0 - K0) / (S * NB) * (S * NB);  // the panel's sub-block
+      // the panel's sub-blocks, level by level: start and end
+      int st[3] = {K0, 0, 0}, en[3] = {qend, 0, 0};
+      for (int l = 1; l <= L; ++l) {
+        st[l] = st[l - 1] + (k0 - st[l - 1]) / (sz[l] * NB) * (sz[l] * NB);
+        en[l] = std::min(st[l] + sz[l] * NB, en[l - 1]);
+      }
+      const int Ks = st[L];  // the panel's finest sub-block
       AuxStream* ax = nullptr;
       if (k0 > Ks && one_stream) {
         gemm_sub<T>(M, lda, k0, m, k0, kb, Ks, k0 - Ks, gb, sop, stream, gt);
@@ -1187,11 +1203,16 @@ void factor_optimistic(typename Cplx<T>::type* M, int n, int lda, int ncols, int
         apply_kernel<T><<<dim3(rb + cb, 1, batch), 256, 0, stream>>>(M, lda, n, ncols, k0, kb, rb, ws.uinv + po,
                                                                      ws.linv + po, ws.udiag, ws.flags + p, nullptr, bs,
                                                                      so, split && k0 + NB < n ? 1 : 0);
-      const int se = k0 + kb;  // the end of a sub-block before the block's end: the L-shaped update
-      if (S < Q && se < qend && (se - K0) % (S * NB) == 0) {
-        gemm_sub<T>(M, lda, se, n - se, se, qend - se, Ks, se - Ks, gb, sop, stream, gt);
-        gemm_sub<T>(M, lda, se, qend - se, qend, ncols - qend, Ks, se - Ks, gb, sop, stream, gt);
-      }
+      // the coarsest sub-block that ends here before its parent does: the L-shaped update of the
+      // parent's later columns (all rows below) and later rows (the columns past the parent)
+      const int se = k0 + kb;
+      for (int l = 1; l <= L; ++l)
+        if (se == en[l] && se < en[l - 1]) {
+          const int pe = en[l - 1];
+          gemm_sub<T>(M, lda, se, n - se, se, pe - se, st[l], se - st[l], gb, sop, stream, gt);
+          gemm_sub<T>(M, lda, se, pe - se, pe, ncols - pe, st[l], se - st[l], gb, sop, stream, gt);
+          break;
+        }
     }
     gemm_sub<T>(M, lda, qend, n - qend, qend, ncols - qend, K0, qend - K0, gb, sop, stream, gt);
   }
@@ -1315,8 +1336,8 @@ void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, 
     // the launch sequence depends only on these; replayed from a captured hipGraph
     const GraphKey key{(const void*)M, mstride, batch, n, lda, nrhs, g_gemm_variant_[prec_index<T>()],
                        g_lu_depth_[prec_index<T>()],
-                       std::is_same<T, double>::value ? 8 : 4, nsys ? 1 : 0, g_lu_sub_[prec_index<T>()],
-                       ws.generation};
+                       std::is_same<T, double>::value ? 8 : 4, nsys ? 1 : 0,
+                       g_lu_sub_[prec_index<T>()] + 100 * g_lu_sub2_[prec_index<T>()], ws.generation};
     GraphCache& gc = graphs();
     hipGraphExec_t exec = gc.find(key);
     if (!exec) {
@@ -1410,10 +1431,11 @@ __global__ void synth_kernel(typename Cplx<T>::type* M, size_t count, int n, int
 // Time the batched LU on synthetic systems (n, nrhs) in three stages: every GEMM tile variant (1,
 // 2; complex64 also 3 to 7, the split-FP16 kernels) and block depth (2, 4, 8, 12, 16) at the
 // largest batch of 4, 8, 16, 32, 48 that is at most max_batch and fits in mem_bytes; deeper blocks
-// with sub-blocks (depth, sub) = (16, 8), (24, 8), (24, 12), (32, 8), (32, 16) at the fastest
-// variant; the smaller batches at the fastest of all. Keeps the fastest variant, depth and
-// sub-block and returns the fastest batch (by time per system). rows of report: variant, depth,
-// sub-block, batch, ms per system (up to max_rows rows).
+// with nested sub-blocks (depth, sub, sub of sub) = (16, 8, 0), (24, 8, 0), (24, 12, 0),
+// (32, 16, 0), (32, 16, 8), (64, 32, 16) at the fastest variant; the smaller batches at the
+// fastest of all. Keeps the fastest variant, depth and sub-blocks and returns the fastest batch
+// (by time per system). rows of report: variant, depth, sub-block, its sub-block, batch, ms per
+// system (up to max_rows rows).
 template <typename T>
 int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t stream, double* report, int max_rows,
                 int* rows) {
@@ -1432,24 +1454,27 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   int& gv = g_gemm_variant_[prec_index<T>()];
   int& gd = g_lu_depth_[prec_index<T>()];
   int& gs = g_lu_sub_[prec_index<T>()];
-  const int old_v = gv, old_d = gd, old_s = gs;
+  int& gs2 = g_lu_sub2_[prec_index<T>()];
+  const int old_v = gv, old_d = gd, old_s = gs, old_s2 = gs2;
   double best = 1e300;
-  int bv = old_v, bd = old_d, bs = old_s, bb = batches.back();
+  int bv = old_v, bd = old_d, bs = old_s, bs2 = old_s2, bb = batches.back();
   *rows = 0;
   std::vector<int> status(batches.back());
   try {
     const std::vector<int> variants = std::is_same<T, float>::value ? std::vector<int>{1, 2, 3, 4, 5, 6, 7}
                                                                       : std::vector<int>{1, 2};
-    std::vector<std::array<int, 4>> configs;  // variant, depth, sub-block, batch
+    std::vector<std::array<int, 5>> configs;  // variant, depth, sub-block, its sub-block, batch
     for (int v : variants)
-      for (int d : {2, 4, 8, 12, 16}) configs.push_back({v, d, 0, batches.back()});
-    const size_t stage1 = configs.size(), stage2 = stage1 + 5;
+      for (int d : {2, 4, 8, 12, 16}) configs.push_back({v, d, 0, 0, batches.back()});
+    const std::array<int, 3> nested[] = {{16, 8, 0}, {24, 8, 0}, {24, 12, 0}, {32, 16, 0}, {32, 16, 8}, {64, 32, 16}};
+    const size_t stage1 = configs.size(), stage2 = stage1 + sizeof(nested) / sizeof(nested[0]);
     for (size_t c = 0; c < configs.size(); ++c) {
-      const int v = configs[c][0], d = configs[c][1], sb = configs[c][2], b = configs[c][3];
+      const int v = configs[c][0], d = configs[c][1], sb = configs[c][2], sb2 = configs[c][3], b = configs[c][4];
       {
           gv = v;
           gd = d;
           gs = sb;
+          gs2 = sb2;
           double ms_sys = 0.0;
           for (int rep = 0; rep < 2; ++rep) {  // the first captures the graph
             synth_kernel<T><<<4096, 256, 0, stream>>>(M, ms * b, n, n, 1234567ull + rep);
@@ -1462,12 +1487,13 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
             ms_sys = t / b;
           }
           if (*rows < max_rows && report) {
-            double* r = report + 5 * (*rows);
+            double* r = report + 6 * (*rows);
             r[0] = v;
             r[1] = d;
             r[2] = sb;
-            r[3] = b;
-            r[4] = ms_sys;
+            r[3] = sb2;
+            r[4] = b;
+            r[5] = ms_sys;
             ++*rows;
           }
           if (ms_sys < best) {
@@ -1475,21 +1501,22 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
             bv = v;
             bd = d;
             bs = sb;
+            bs2 = sb2;
             bb = b;
           }
       }
-      // second stage: deeper blocks with sub-blocks at the fastest variant; third: the smaller
-      // batches at the fastest of all
+      // second stage: deeper blocks with nested sub-blocks at the fastest variant; third: the
+      // smaller batches at the fastest of all
       if (c + 1 == stage1)
-        for (const auto& ds : {std::array<int, 2>{16, 8}, {24, 8}, {24, 12}, {32, 8}, {32, 16}})
-          configs.push_back({bv, ds[0], ds[1], batches.back()});
+        for (const auto& ds : nested) configs.push_back({bv, ds[0], ds[1], ds[2], batches.back()});
       if (c + 1 == stage2)
-        for (size_t i = 0; i + 1 < batches.size(); ++i) configs.push_back({bv, bd, bs, batches[i]});
+        for (size_t i = 0; i + 1 < batches.size(); ++i) configs.push_back({bv, bd, bs, bs2, batches[i]});
     }
   } catch (...) {
     gv = old_v;
     gd = old_d;
     gs = old_s;
+    gs2 = old_s2;
     (void)hipFree(M);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1498,6 +1525,7 @@ int autotune_lu(int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t st
   gv = bv;
   gd = bd;
   gs = bs;
+  gs2 = bs2;
   graphs().clear();  // the captures hold the tuning buffer
   (void)hipFree(M);
   HBG_HIP(hipEventDestroy(e0));

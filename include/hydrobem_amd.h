@@ -339,7 +339,7 @@ HbgStatus hbg_surrogate_gradient(HbgSession* s, const float* params, int32_t bat
  * while the current one is multiplied), on the FP16 matrix cores with each FP32 operand split
  * into two FP16 numbers (three FP16 products per FP32 product; the complex128 LU takes 1 for 3, 6
  * and 7 and 2 for 4 and 5); HBG_TUNE_LU_DEPTH the number of 64-column panels per
- * trailing update (1..32, default 4). Both set the complex128 and the complex64 LU and read back
+ * trailing update (1..64, default 4). Both set the complex128 and the complex64 LU and read back
  * the complex128 one; HBG_TUNE_LU_GEMM_MIXED and HBG_TUNE_LU_DEPTH_MIXED set and read the
  * complex64 LU's (the mixed precision) alone. HBG_TUNE_LU_GRAPHS 1 (default): the batched LU's
  * launches are captured once per system shape and buffer into a hipGraph and replayed (not while
@@ -355,18 +355,21 @@ enum {
                            table/series evaluator the others (10..2000, default 10: the training domain) */
   HBG_TUNE_LU_SUB = 7,      /* panels per sub-block of an LU block (0: none, default): the in-block updates
                                take the sub-block's panels, its end an L-shaped update (both arithmetics) */
-  HBG_TUNE_LU_SUB_MIXED = 8 /* the same for the complex64 LU alone */
+  HBG_TUNE_LU_SUB_MIXED = 8, /* the same for the complex64 LU alone */
+  HBG_TUNE_LU_SUB2 = 9,       /* panels per sub-block of a sub-block (0: none, default), both arithmetics */
+  HBG_TUNE_LU_SUB2_MIXED = 10 /* the same for the complex64 LU alone */
 };
 HbgStatus hbg_set_tuning(HbgSession* s, int32_t key, int32_t value);
 /* The project's autotuner (no counterpart in the reference): times the batched LU on synthetic
  * systems of order n with nrhs right-hand sides in the given precision, first for every GEMM tile
  * variant (1, 2; mixed precision also 3 to 7) and block depth (2, 4, 8, 12, 16) at the largest batch
- * of 4, 8, 16, 32, 48 that fits in a third of the free memory, then deeper blocks with sub-blocks
- * ((depth, sub) = (16, 8), (24, 8), (24, 12), (32, 8), (32, 16)) at the fastest variant, then the
- * smaller batches at the fastest of all; keeps the fastest variant, depth and sub-block
- * (HBG_TUNE_LU_GEMM, HBG_TUNE_LU_DEPTH, HBG_TUNE_LU_SUB; for the mixed precision their _MIXED
- * keys) and sets the session's batch (the fastest per system). report: up to max_rows rows of
- * (variant, depth, sub-block, batch, ms per system), rows the number written. */
+ * of 4, 8, 16, 32, 48 that fits in a third of the free memory, then deeper blocks with nested
+ * sub-blocks ((depth, sub, sub of sub) = (16, 8, 0), (24, 8, 0), (24, 12, 0), (32, 16, 0),
+ * (32, 16, 8), (64, 32, 16)) at the fastest variant, then the smaller batches at the fastest of
+ * all; keeps the fastest variant, depth and sub-blocks (HBG_TUNE_LU_GEMM, HBG_TUNE_LU_DEPTH,
+ * HBG_TUNE_LU_SUB, HBG_TUNE_LU_SUB2; for the mixed precision their _MIXED keys) and sets the
+ * session's batch (the fastest per system). report: up to max_rows rows of (variant, depth,
+ * sub-block, its sub-block, batch, ms per system), rows the number written. */
 HbgStatus hbg_autotune(HbgSession* s, int32_t n, int32_t nrhs, int32_t precision, double* report, int32_t max_rows,
                        int32_t* rows);
 HbgStatus hbg_get_tuning(HbgSession* s, int32_t key, int32_t* value);

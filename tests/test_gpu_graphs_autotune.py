@@ -50,25 +50,28 @@ def test_autotune_sweep_and_solve(oracle):
     say("case: autotune n = 1024, 7 rhs, FP64 and mixed; solve afterwards vs oracle")
     s = hb.Session(0)
     rows = s.autotune(1024, 7, "fp64")
-    say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} s{int(r[2])} b{int(r[3])} {r[4]:.3f} ms" for r in rows))
-    # two variants x five depths at batch 48, five (depth, sub-block) pairs at the fastest variant,
-    # then batches 4..32 at the fastest of them
-    assert len(rows) == 2 * 5 + 5 + 4 and all(r[4] > 0 for r in rows)
-    assert {int(r[1]) for r in rows} == {2, 4, 8, 12, 16, 24, 32}
-    assert {int(r[2]) for r in rows} == {0, 8, 12, 16}
-    best = min(rows, key=lambda r: r[4])
+    say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} s{int(r[2])}/{int(r[3])} b{int(r[4])} {r[5]:.3f} ms"
+                         for r in rows))
+    # two variants x five depths at batch 48, six nested (depth, sub-blocks) at the fastest
+    # variant, then batches 4..32 at the fastest of them
+    assert len(rows) == 2 * 5 + 6 + 4 and all(r[5] > 0 for r in rows)
+    assert {int(r[1]) for r in rows} == {2, 4, 8, 12, 16, 24, 32, 64}
+    assert {int(r[2]) for r in rows} == {0, 8, 12, 16, 32} and {int(r[3]) for r in rows} == {0, 8, 16}
+    best = min(rows, key=lambda r: r[5])
     assert s.get_tuning(s.TUNE_LU_GEMM) == int(best[0]) and s.get_tuning(s.TUNE_LU_DEPTH) == int(best[1])
-    assert s.get_tuning(s.TUNE_LU_SUB) == int(best[2])
+    assert s.get_tuning(s.TUNE_LU_SUB) == int(best[2]) and s.get_tuning(s.TUNE_LU_SUB2) == int(best[3])
     rows32 = s.autotune(1024, 7, "mixed")
-    say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} s{int(r[2])} b{int(r[3])} {r[4]:.3f} ms" for r in rows32))
-    assert len(rows32) == 7 * 5 + 5 + 4 and all(r[4] > 0 for r in rows32)
-    best32 = min(rows32, key=lambda r: r[4])
+    say("  " + "; ".join(f"v{int(r[0])} d{int(r[1])} s{int(r[2])}/{int(r[3])} b{int(r[4])} {r[5]:.3f} ms"
+                         for r in rows32))
+    assert len(rows32) == 7 * 5 + 6 + 4 and all(r[5] > 0 for r in rows32)
+    best32 = min(rows32, key=lambda r: r[5])
     assert s.get_tuning(s.TUNE_LU_GEMM_MIXED) == int(best32[0])
     assert s.get_tuning(s.TUNE_LU_DEPTH_MIXED) == int(best32[1])
     assert s.get_tuning(s.TUNE_LU_SUB_MIXED) == int(best32[2])
+    assert s.get_tuning(s.TUNE_LU_SUB2_MIXED) == int(best32[3])
     # the complex128 settings stay those of its own sweep
     assert s.get_tuning(s.TUNE_LU_GEMM) == int(best[0]) and s.get_tuning(s.TUNE_LU_DEPTH) == int(best[1])
-    assert s.get_tuning(s.TUNE_LU_SUB) == int(best[2])
+    assert s.get_tuning(s.TUNE_LU_SUB) == int(best[2]) and s.get_tuning(s.TUNE_LU_SUB2) == int(best[3])
     v = meshes.cylinder_panels(1.0, 1.0, 32, 16, 16, quarter=False, cosine=True)
     s.set_environment(1025.0, 9.81)
     s.set_panels(v)
@@ -82,4 +85,5 @@ def test_autotune_sweep_and_solve(oracle):
     s.set_tuning(s.TUNE_LU_GEMM, 1)
     s.set_tuning(s.TUNE_LU_DEPTH, 4)
     s.set_tuning(s.TUNE_LU_SUB, 0)
+    s.set_tuning(s.TUNE_LU_SUB2, 0)
     s.close()

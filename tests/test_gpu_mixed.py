@@ -108,12 +108,13 @@ def test_mixed_split_fp16_gemm(session, variant):
         session.set_tuning(session.TUNE_LU_GEMM_MIXED, old)
 
 
-@pytest.mark.parametrize("depth,sub", [(4, 2), (8, 3), (32, 8)])
-def test_lu_sub_blocks(session, depth, sub):
+@pytest.mark.parametrize("depth,sub,sub2", [(4, 2, 0), (8, 3, 0), (32, 8, 0), (16, 8, 3), (36, 16, 4)])
+def test_lu_sub_blocks(session, depth, sub, sub2):
     """Sub-blocks inside the LU's blocks (the in-block updates from the panel's sub-block, an
     L-shaped update at each sub-block's end): FP64 to round-off of the plain blocked LU, and the
     complex64 LU (split-FP16 GEMM) FP32-accurate before its refinement."""
-    say(f"case: LU depth {depth}, sub-blocks of {sub} panels, cylinder 2304 panels, finite depth, 3 frequencies")
+    say(f"case: LU depth {depth}, sub-blocks of {sub} and {sub2} panels, cylinder 2304 panels, finite depth, "
+        "3 frequencies")
     v = meshes.cylinder_panels(1.0, 1.0, 48, 24, 24, quarter=False, cosine=True)
     w = np.sqrt(9.81 * np.array([0.05, 1.0, 6.0]) / 3.0)
     session.set_environment(1025.0, 9.81, 3.0)
@@ -125,7 +126,9 @@ def test_lu_sub_blocks(session, depth, sub):
     try:
         session.set_tuning(session.TUNE_LU_DEPTH, depth)
         session.set_tuning(session.TUNE_LU_SUB, sub)
+        session.set_tuning(session.TUNE_LU_SUB2, sub2)
         assert session.get_tuning(session.TUNE_LU_SUB_MIXED) == sub
+        assert session.get_tuning(session.TUNE_LU_SUB2_MIXED) == sub2
         sb = session.solve(w, [0.0])
         for key in keys:
             e = rel_per_entry(sb[key], plain[key])
@@ -146,6 +149,7 @@ def test_lu_sub_blocks(session, depth, sub):
     finally:
         session.set_profiling(False)
         session.set_tuning(session.TUNE_LU_SUB, 0)
+        session.set_tuning(session.TUNE_LU_SUB2, 0)
         session.set_tuning(session.TUNE_LU_DEPTH, old[0])
         session.set_tuning(session.TUNE_LU_DEPTH_MIXED, old[1])
         session.set_tuning(session.TUNE_LU_GEMM_MIXED, old[2])
