@@ -45,15 +45,17 @@ VARIANTS = {1: ("64x64", "gemm_mfma_kernel<T, 64, 64>", None), 2: ("128x64", "ge
 # HBM bytes per launch of the path's GEMM kernel, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
 # calibrated by a device copy of known size (profiles/gemm_hbm_traffic_cylinder8192_finite_depth.md):
 # (workload, path, GEMM tiles, panels per trailing update, batch) -> bytes
-TRAFFIC = {("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 16): 8.496e8,
-           ("cylinder8192fd", "mixed", "64x64 split-fp16", 8, 48): 2.548e9,
-           ("cylinder8192fd", "mixed", "64x64 split-fp16", 16, 48): 2.852e9,
+TRAFFIC = {("cylinder8192fd", "mixed", "64x64 split-fp16", 8, (0, 0), 16): 8.496e8,
+           ("cylinder8192fd", "mixed", "64x64 split-fp16", 8, (0, 0), 48): 2.548e9,
+           ("cylinder8192fd", "mixed", "64x64 split-fp16", 16, (0, 0), 48): 2.852e9,
            # the 128 x 128 kernel, averaged over the trailing updates of a solve
-           ("cylinder8192fd", "mixed32", "128x128 split-fp16 direct-LDS", 16, 48): 5.637e10}
+           ("cylinder8192fd", "mixed32", "128x128 split-fp16 direct-LDS", 16, (0, 0), 48): 5.637e10,
+           # ... and, with sub-blocks, over its trailing and L-shaped updates
+           ("cylinder8192fd", "mixed32", "128x128 split-fp16 direct-LDS", 32, (16, 8), 48): 1.320e10}
 # HBM bytes per launch (one frequency) of the assembly kernel, from the same calibrated runs:
 # (workload, path) -> bytes (profiles/assembly_hbm_traffic_cylinder8192_finite_depth.md)
 ASM_TRAFFIC = {("cylinder8192fd", "fp64"): 8.97e9, ("cylinder8192fd", "mixed"): 8.97e9,
-               ("cylinder8192fd", "mixed32"): 5.22e9}
+               ("cylinder8192fd", "mixed32"): 5.28e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
                       "operands split into hi + lo, autotuned), FP64 iterative refinement to backward error 1e-12",
@@ -481,8 +483,10 @@ def main():
     tun = tuning.get("fp64" if main_path == "fp64" else "mixed",
                      {"gemm_variant": gv, "gemm_tiles": VARIANTS[gv][0],
                       "panels_per_update": a.depth if a.depth >= 0 else 4,
-                      "batch": a.batch or None})
-    traffic = TRAFFIC.get((wkey, main_path, tun["gemm_tiles"], tun["panels_per_update"], tun["batch"]))
+                      "panels_per_sub_block": [max(a.sub, 0), max(a.sub2, 0)],
+                      "batch": a.batch or 48})
+    traffic = TRAFFIC.get((wkey, main_path, tun["gemm_tiles"], tun["panels_per_update"],
+                           tuple(tun.get("panels_per_sub_block", (0, 0))), tun["batch"]))
     tiles, kernel, split = VARIANTS[tun["gemm_variant"]]
     kernel = kernel.replace("<T,", "<double," if main_path == "fp64" else "<float,")
     if tun["gemm_variant"] >= 6:
@@ -504,7 +508,7 @@ def main():
         "dtype": ("f16x3+" if split else "") + DTYPE[main_path],
         "data": "synthetic: procedural mesh of the workload (tests/python/meshes.py layout)",
         "config": config,
-        "roofline": {"bound": "mfma", "kernel": kernel + " (LU trailing updates, complex GEMM on MFMA"
+        "roofline": {"bound": "mfma", "kernel": kernel + " (LU trailing and sub-block updates, complex GEMM on MFMA"
                                                 + (", FP32 operands split into FP16 hi + lo)" if split else ")"),
                      "achieved": gemm_tf, "peak": gpeak / 1e12, "unit": "TFLOP/s",
                      "frac": round(gemm_tf * 1e12 / gpeak, 4), "traffic": traffic,
