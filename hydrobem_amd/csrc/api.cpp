@@ -30,7 +30,8 @@ void launch_symmetry_project(const double2* u, int N, int G, int R, const double
                              hipStream_t stream);
 void launch_symmetry_recombine(const double2* M, size_t mstride, int N, int G, int R, const double* chi, double2* phi,
                                hipStream_t stream);
-void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2* S, double2* A, hipStream_t stream);
+void launch_limit_matrices(const StaticD* st, const double2* inf, int N, int Nsrc, double2* S, double2* A,
+                           hipStream_t stream);
 void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
                          double logc, const StaticD* st, double2* S, double2* A, float2* A32, int lda,
                          double free_term, hipStream_t stream, int images = 1);
@@ -130,8 +131,8 @@ void mixed_solve_batched(double2* M64, float2* M32, double2* X, float2* D, doubl
 int residual_blocks(int n);
 size_t residual_max_elems(int n);
 size_t residual_part_elems(int n, int R);
-void launch_limit_fin_matrices(const PanelD* P, int N, double h, bool zero, const double* ktab, const StaticD* st,
-                               const double2* inf, double2* S, double2* A, hipStream_t stream);
+void launch_limit_fin_matrices(const PanelD* P, int N, int Nsrc, double h, bool zero, const double* ktab,
+                               const StaticD* st, const double2* inf, double2* S, double2* A, hipStream_t stream);
 void launch_limit_points(double h, bool zero, const double* ktab, int n, const double* R, const double* z,
                          const double* zeta, double* out, hipStream_t stream);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
@@ -540,9 +541,8 @@ void use_device(HbgSession* s) { HBG_HIP(hipSetDevice(s->device)); }
 void ensure_statics(HbgSession* s, int logint, double tol, bool want_inf = false) {
   if (s->statics_valid && s->st_logint == logint && s->st_tol == tol && (!want_inf || s->stinf_valid)) return;
   const int N = s->N, NF = s->G * N;
-  if (want_inf && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   s->d_st.ensure((size_t)N * NF);
-  if (want_inf) s->d_stinf.ensure((size_t)N * N);
+  if (want_inf) s->d_stinf.ensure((size_t)N * NF);
   unsigned qcap = (unsigned)std::min<size_t>((size_t)N * 64, (size_t)1 << 28);
   if (s->profiling) HBG_HIP(hipEventRecord(s->ev[0], s->stream));
   for (int attempt = 0; attempt < 2; ++attempt) {
@@ -1719,31 +1719,63 @@ namespace {
 // diffraction there): the matrices of the statics alone in infinite depth (launch_limit_matrices),
 // in finite depth the statics and the limit's regular part (launch_limit_fin_matrices), S U for
 // the radiation columns, the FP64 LU and the forces with omega = 0 in the force kernel (B = 0).
+// A symmetric body or walls as at a frequency: the image blocks combined into the character
+// systems, the right-hand sides projected on the characters, the potential recombined on the
+// whole body (walls: the one even character, the forces on the real panels).
 // Writes the added mass [D][K] into A (host). The statics (and at infinity their omega = infinity
 // row) must be current.
 void solve_limit(HbgSession* s, bool at_infinity, double* A) {
-  const int N = s->N, nrad = 6 * s->nb, R = nrad;
-  const size_t nAB = (size_t)nrad * nrad;
-  s->d_S.ensure((size_t)N * N);
-  s->d_M.ensure((size_t)N * (N + R));
-  s->d_u.ensure((size_t)R * N);
-  s->d_part.ensure((size_t)rhs_chunks(N) * R * N);
+  const int N = s->N, nrad = 6 * s->nb, R = nrad, G = s->G, Gs = s->Gs, NF = Gs * N;
+  const size_t nAB = (size_t)nrad * nrad, mstride = (size_t)N * (N + R);
+  s->d_S.ensure((size_t)N * N * Gs);
+  s->d_M.ensure(mstride * Gs);
+  s->d_u.ensure((size_t)R * NF);
+  s->d_part.ensure((size_t)rhs_chunks(N) * R * N * Gs);
   s->d_out.ensure(2 * nAB);
-  s->d_phiI.ensure((size_t)N);
-  s->d_dphiI.ensure((size_t)N);
+  s->d_phiI.ensure((size_t)NF);
+  s->d_dphiI.ensure((size_t)NF);
+  // the matrices: straight into S and M (G = 1), or the G image blocks
+  double2* Sb = s->d_S.p;
+  double2* Ab = s->d_M.p;
+  if (G > 1) {
+    s->d_Sg.ensure((size_t)N * N * G);
+    s->d_Ag.ensure((size_t)N * N * G);
+    Sb = s->d_Sg.p;
+    Ab = s->d_Ag.p;
+  }
   if (std::isfinite(s->depth))
-    launch_limit_fin_matrices(s->d_panels.p, N, s->depth, !at_infinity, s->ktab, s->d_st.p, s->d_stinf.p, s->d_S.p,
-                              s->d_M.p, s->stream);
+    launch_limit_fin_matrices(s->d_panels.p, N, G * N, s->depth, !at_infinity, s->ktab, s->d_st.p, s->d_stinf.p, Sb,
+                              Ab, s->stream);
   else
-    launch_limit_matrices(s->d_st.p, at_infinity ? s->d_stinf.p : nullptr, N, s->d_S.p, s->d_M.p, s->stream);
-  launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, 0, 1.0, 1.0, s->grav, 0.0, s->d_heads.p,
-                  s->d_phiI.p, s->d_dphiI.p, s->d_u.p, s->stream);
-  launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->d_part.p, s->stream);
-  const int zp = lu_solve_device(s->d_M.p, N, N, R, s->stream, nullptr);
-  if (zp > 0) throw Error{HBG_ERR_NUMERICAL, "singular system matrix at a frequency limit (zero pivot " +
-                                                 std::to_string(zp) + ")"};
-  launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, 0, 0.0, s->rho, s->d_M.p, N, s->d_phiI.p,
-                s->d_dphiI.p, s->d_out.p, s->stream);
+    launch_limit_matrices(s->d_st.p, at_infinity ? s->d_stinf.p : nullptr, N, G * N, Sb, Ab, s->stream);
+  if (G > 1)
+    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, Gs, s->d_S.p, s->d_M.p, mstride, nullptr,
+                            s->stream);
+  launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, 0, 1.0, 1.0, s->grav, 0.0, s->d_heads.p,
+                  s->d_phiI.p, s->d_dphiI.p, s->d_u.p, s->stream, s->walls);
+  if (Gs == 1) {
+    launch_rhs(s->d_S.p, N, N, s->d_u.p, R, s->d_M.p, N, s->d_part.p, s->stream);
+  } else {
+    s->d_uc.ensure((size_t)N * R * Gs);
+    launch_symmetry_project(s->d_u.p, N, Gs, R, s->d_chisym.p, s->d_uc.p, s->stream);
+    launch_rhs_batched(s->d_S.p, (size_t)N * N, N, N, s->d_uc.p, (size_t)R * N, R, s->d_M.p, mstride, N, s->d_part.p,
+                       Gs, s->stream);
+  }
+  for (int c = 0; c < Gs; ++c) {
+    const int zp = lu_solve_device(s->d_M.p + mstride * c, N, N, R, s->stream, nullptr);
+    if (zp > 0)
+      throw Error{HBG_ERR_NUMERICAL, "singular system matrix at a frequency limit (zero pivot " + std::to_string(zp) +
+                                         (Gs > 1 ? ", character " + std::to_string(c) : std::string()) + ")"};
+  }
+  if (Gs == 1) {
+    launch_forces(s->d_panels.p, s->d_body.p, s->d_nj.p, N, nrad, 0, 0.0, s->rho, s->d_M.p, N, s->d_phiI.p,
+                  s->d_dphiI.p, s->d_out.p, s->stream);
+  } else {
+    s->d_phif.ensure((size_t)NF * R);
+    launch_symmetry_recombine(s->d_M.p, mstride, N, Gs, R, s->d_chisym.p, s->d_phif.p, s->stream);
+    launch_forces_sol(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, 0, 0.0, s->rho, s->d_phif.p, NF, s->d_phiI.p,
+                      s->d_dphiI.p, s->d_out.p, s->stream);
+  }
   std::vector<double> h(2 * nAB);
   HBG_HIP(hipMemcpyAsync(h.data(), s->d_out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HBG_HIP(hipStreamSynchronize(s->stream));
@@ -1757,7 +1789,7 @@ void solve_limit(HbgSession* s, bool at_infinity, double* A) {
     // its row and column are NaN. A mode whose normals vanish to round-off (yaw of a body of
     // revolution) carries no flux whatever its round-off sum.
     std::vector<double> flux(nrad, 0.0), scale(nrad, 0.0), body_scale(s->nb, 0.0);
-    for (int j = 0; j < N; ++j) {
+    for (int j = 0; j < NF; ++j) {  // the whole body (its symmetric images; not a wall's)
       const PanelD& pj = s->panels[j];
       body_scale[s->body[j]] += pj.area * (1.0 + std::sqrt(pj.c[0] * pj.c[0] + pj.c[1] * pj.c[1] + pj.c[2] * pj.c[2]));
       for (int m = 0; m < 6; ++m) {
@@ -1814,7 +1846,6 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions with an interior free surface"};
     if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
   }
-  if (!il.empty() && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with symmetry planes"};
   if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
   if ((s->nk > 0 || s->np > 0) && !il.empty())
     throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions at the frequency limits"};

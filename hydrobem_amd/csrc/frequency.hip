@@ -832,19 +832,22 @@ int rhs_chunks(int N) { return (N + rhs_chunk_len(N) - 1) / rhs_chunk_len(N); }
 // The matrices of a frequency limit in infinite depth (low_order.rs:635-637, 757-758: the wave
 // terms and the log terms vanish, K = 0): S = rs, D = rd (1/r + 1/r') at omega = 0, and with
 // phi = 0 on z = 0 the integrals of 1/r - 1/r' (inf) at omega = infinity; A = 2 pi I - D.
-// Column-major, leading dimension N (the layout of the statics).
+// Column-major, leading dimension N (the layout of the statics): entry (k, n) for the N field
+// panels k and the Nsrc sources n (a symmetric body or walls: the panels and their images, the
+// image g's block at g N N), the 2 pi only on the diagonal of the first block.
 __global__ void limit_matrices_kernel(const StaticD* __restrict__ st, const double2* __restrict__ inf, int N,
-                                      double2* __restrict__ S, double2* __restrict__ A) {
+                                      int Nsrc, double2* __restrict__ S, double2* __restrict__ A) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * N) return;
+  if (e >= (size_t)N * Nsrc) return;
   const int k = (int)(e % N), n = (int)(e / N);
   const double s = inf ? inf[e].x : st[e].rs, d = inf ? inf[e].y : st[e].rd;
   S[e] = make_double2(s, 0.0);
   A[e] = make_double2((k == n ? 2.0 * PI : 0.0) - d, 0.0);
 }
 
-void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2* S, double2* A, hipStream_t stream) {
-  limit_matrices_kernel<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, stream>>>(st, inf, N, S, A);
+void launch_limit_matrices(const StaticD* st, const double2* inf, int N, int Nsrc, double2* S, double2* A,
+                           hipStream_t stream) {
+  limit_matrices_kernel<<<(unsigned)(((size_t)N * Nsrc + 255) / 256), 256, 0, stream>>>(st, inf, N, Nsrc, S, A);
   HBG_HIP(hipGetLastError());
 }
 
@@ -852,14 +855,15 @@ void launch_limit_matrices(const StaticD* st, const double2* inf, int N, double2
 // Infinite, apply_kernel low_order.rs:738-769 with K = 0: no log terms): entry (k, n), field point
 // the centroid of k, source panel n, S = rs + A_n Wr and A = 2 pi delta_kn - (rd + A_n (-WrR a +
 // WrZ n_z)) with the limit's kernel (fin_limit_kernel) and rs, rd the statics (omega = 0) or their
-// omega = infinity row (inf). One thread per entry: this runs once per solve.
-__global__ void __launch_bounds__(256) limit_fin_matrices_kernel(const PanelD* __restrict__ P, int N, double h,
-                                                                 int zero, const double* __restrict__ ktab,
+// omega = infinity row (inf). Sources n < Nsrc as in limit_matrices_kernel (the images' records
+// follow the panels in P). One thread per entry: this runs once per solve.
+__global__ void __launch_bounds__(256) limit_fin_matrices_kernel(const PanelD* __restrict__ P, int N, int Nsrc,
+                                                                 double h, int zero, const double* __restrict__ ktab,
                                                                  const StaticD* __restrict__ st,
                                                                  const double2* __restrict__ inf,
                                                                  double2* __restrict__ S, double2* __restrict__ A) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * N) return;
+  if (e >= (size_t)N * Nsrc) return;
   const int k = (int)(e % N), n = (int)(e / N);
   const PanelD* pk = P + k;
   const PanelD* pn = P + n;
@@ -872,10 +876,10 @@ __global__ void __launch_bounds__(256) limit_fin_matrices_kernel(const PanelD* _
   A[e] = make_double2((k == n ? 2.0 * PI : 0.0) - (rd + pn->area * (-w.WrR * a + w.WrZ * pn->n[2])), 0.0);
 }
 
-void launch_limit_fin_matrices(const PanelD* P, int N, double h, bool zero, const double* ktab, const StaticD* st,
-                               const double2* inf, double2* S, double2* A, hipStream_t stream) {
-  limit_fin_matrices_kernel<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, stream>>>(P, N, h, zero ? 1 : 0, ktab,
-                                                                                        st, inf, S, A);
+void launch_limit_fin_matrices(const PanelD* P, int N, int Nsrc, double h, bool zero, const double* ktab,
+                               const StaticD* st, const double2* inf, double2* S, double2* A, hipStream_t stream) {
+  limit_fin_matrices_kernel<<<(unsigned)(((size_t)N * Nsrc + 255) / 256), 256, 0, stream>>>(
+      P, N, Nsrc, h, zero ? 1 : 0, ktab, st, inf, S, A);
   HBG_HIP(hipGetLastError());
 }
 

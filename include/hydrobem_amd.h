@@ -125,8 +125,10 @@ HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_pane
  * the reflection group (lo_geometry.rs:25-35, lo_frequency.rs:860-943); the results are those of
  * the whole body. Kept across hbg_set_panels; set before or after it. With walls on the other
  * axis the group of both is used (only the characters even under the walls are solved); a plane
- * on the axis of a wall is HBG_ERR_VALUE. With G > 1, hbg_statics, hbg_assemble, hbg_solve_group,
- * the surrogate and the frequency limits are HBG_ERR_UNSUPPORTED. */
+ * on the axis of a wall is HBG_ERR_VALUE. The frequency limits omega = 0 and infinity are solved the
+ * same way (the limit matrices' image blocks combined into the characters; at omega = 0 in finite
+ * depth the net-flux rule takes the whole body). With G > 1, hbg_statics, hbg_assemble,
+ * hbg_solve_group and the surrogate are HBG_ERR_UNSUPPORTED. */
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
 
 /* Walls (the domain x_wall, y_wall, orthogonal_walls of model_codes.rs:15-31): wall_x 1 a
@@ -138,8 +140,9 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
  * (low_order.rs:1470-1486) and returns the quantities of the real bodies. Kept across
  * hbg_set_panels. With a symmetry plane on the other axis the panels are the body's half and the
  * group of both is used (lo_geometry.rs:863-945); a wall on the axis of a symmetry plane is
- * HBG_ERR_VALUE. With walls hbg_statics, hbg_assemble, hbg_solve_group, the surrogate and the
- * frequency limits are HBG_ERR_UNSUPPORTED. */
+ * HBG_ERR_VALUE. The frequency limits omega = 0 and infinity are solved with the walls too (at
+ * omega = 0 in finite depth the net-flux rule takes the real bodies). With walls hbg_statics,
+ * hbg_assemble, hbg_solve_group and the surrogate are HBG_ERR_UNSUPPORTED. */
 HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
 
 /* Field points (lo_frequency.rs:1392-1495, `Output` frad/finc/fsca/ftot, low_order.rs:1490-1500):

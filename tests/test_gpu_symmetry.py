@@ -141,11 +141,10 @@ def test_unsupported_uses_with_symmetry_are_refused(sessions):
     s.set_environment(1025.0, 9.81, float("inf"))
     s.set_symmetry(True, False)
     s.set_panels(meshes.hemisphere_panels(1.0, 3, 3, quarter=True))
-    say("case: symmetry planes with the statics, the assembly, the limits, the surrogate")
+    say("case: symmetry planes with the statics, the assembly, the surrogate")
     try:
         for call in (lambda: s.statics(),
                      lambda: s.assemble(1.0),
-                     lambda: s.solve([0.0], []),
                      lambda: s.solve([1.0], [], surrogate=True)):
             with pytest.raises(hb.HbgError) as e:
                 call()
@@ -154,6 +153,51 @@ def test_unsupported_uses_with_symmetry_are_refused(sessions):
             hb._check(hb.library().hbg_set_symmetry(s._h, 2, 0))
     finally:
         s.set_symmetry(False, False)
+
+
+LIMIT_CASES = [
+    ("hemisphere", float("inf"), 1, 1),
+    ("hemisphere", float("inf"), 0, 1),
+    ("cylinder", 3.0, 1, 1),
+    ("box", 2.0, 1, 0),
+]
+
+
+@pytest.mark.parametrize("shape,depth,sym_x,sym_y", LIMIT_CASES)
+def test_limits_of_a_sector_equal_the_whole_mesh(sessions, shape, depth, sym_x, sym_y):
+    """The radiation limits omega = 0 and infinity (and a frequency between) of a sector against the
+    whole mesh. In finite depth at omega = 0 the modes with a net volume flux through the whole
+    body (heave, and roll and pitch about a centre off the axis) are NaN on both sides."""
+    quadrant = {
+        "hemisphere": lambda: meshes.hemisphere_panels(1.0, 6, 7, quarter=True),
+        "cylinder": lambda: meshes.cylinder_panels(1.0, 0.8, 9, 4, 3, quarter=True),
+        "box": lambda: meshes.box_panels(1.0, 0.6, 0.5, (5, 3, 3), quarter=True),
+    }[shape]()
+    sector, whole = sector_and_whole(quadrant, sym_x, sym_y)
+    centre = np.array([[0.3, -0.2, -0.1]])
+    w = np.array([0.0, 1.1, np.inf])
+    say(f"case: limits of a {shape} sector {len(sector)} of {len(whole)} panels, planes x {sym_x} y {sym_y}, "
+        f"depth {depth}")
+    s, f = sessions
+    for x in (s, f):
+        x.set_environment(1025.0, 9.81, depth)
+    s.set_symmetry(sym_x, sym_y)
+    s.set_panels(sector, rotation_centres=centre)
+    f.set_symmetry(False, False)
+    f.set_panels(whole, rotation_centres=centre)
+    got = s.solve(w, [0.0])["added_mass"]
+    ref = f.solve(w, [0.0])["added_mass"]
+    s.set_symmetry(False, False)
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(got), nan)
+    if np.isfinite(depth):
+        assert nan[0, 2, :].all() and nan[0, 3, :].all() and not nan[0, 0, 0] and not nan[1:].any()
+    else:
+        assert not nan.any()
+    for i, label in enumerate(("omega 0", "omega 1.1", "omega infinity")):
+        e = rel_per_entry(np.nan_to_num(got[i]), np.nan_to_num(ref[i]))
+        say(f"  {label}: sector vs whole {e:.2e}")
+        assert e < 1e-9
 
 
 @pytest.mark.parametrize("depth", [float("inf"), 3.0])
