@@ -25,11 +25,11 @@ void launch_forces_sol(const PanelD* P, const int* body, const double* nj, int N
                        double rho, const double2* sol, int lds, const double2* phiI, const double2* dphiI, double* out,
                        hipStream_t stream);
 void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, int nchar, double2* S,
-                             double2* M, size_t mstride, float2* A32, hipStream_t stream);
+                             double2* M, size_t mstride, int ldm, float2* A32, hipStream_t stream);
 void launch_symmetry_project(const double2* u, int N, int G, int R, const double* chi, double2* uc,
                              hipStream_t stream);
-void launch_symmetry_recombine(const double2* M, size_t mstride, int N, int G, int R, const double* chi, double2* phi,
-                               hipStream_t stream);
+void launch_symmetry_recombine(const double2* sol, size_t mstride, int lds, int N, int G, int R, const double* chi,
+                               double2* phi, hipStream_t stream);
 void launch_limit_matrices(const StaticD* st, const double2* inf, int N, int Nsrc, double2* S, double2* A,
                            hipStream_t stream);
 void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
@@ -633,32 +633,32 @@ FinTables fin_tables(HbgSession* s) {
 // S and A (and A32) with leading dimension lda (at least N; S in s->d_S, lda x N)
 // The matrices of one frequency: S (d_S) and A into Mout (leading dimension lda), A32 the
 // complex64 copy (nullable). A symmetric body (G > 1): the G character systems at Mout + c mstride
-// (leading dimension N) and S_c at d_S + c N N, from its image blocks (scratch d_Sg, d_Ag).
+// (leading dimension lda) and S_c at d_S + c N N, from its image blocks (scratch d_Sg, d_Ag).
 // dff: the frequency's FinFreq already on the device (finite depth; nullptr: uploaded here, which
 // waits for the stream, the host copy being pageable).
 double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, bool surrogate = false,
                           float2* A32 = nullptr, int lda = 0, size_t mstride = 0, const FinFreq* dff = nullptr) {
   const int N = s->N, G = s->G;
   if (lda <= 0) lda = N;
-  if (G > 1 && (surrogate || lda != N || !mstride))
-    throw Error{HBG_ERR_UNSUPPORTED, "symmetry planes with the surrogate or a padded system"};
+  if (G > 1 && (surrogate || !mstride)) throw Error{HBG_ERR_UNSUPPORTED, "symmetry planes with the surrogate"};
   if (G > 1) {
     s->d_Sg.ensure((size_t)N * N * G);
     s->d_Ag.ensure((size_t)N * N * G);
   }
   auto blocks = [&](auto one) {
     if (G == 1) {
-      one(s->d_S.p, Mout, A32, 1);
+      one(s->d_S.p, Mout, A32, 1, lda);
       return;
     }
-    // the G image blocks in one launch, each into its own buffer, then the G characters in one pass
-    one(s->d_Sg.p, s->d_Ag.p, (float2*)nullptr, G);
-    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->Gs, s->d_S.p, Mout, mstride, A32,
+    // the G image blocks (leading dimension N) in one launch, each into its own buffer, then the
+    // G characters in one pass
+    one(s->d_Sg.p, s->d_Ag.p, (float2*)nullptr, G, N);
+    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, s->Gs, s->d_S.p, Mout, mstride, lda, A32,
                             s->stream);
   };
   if (!std::isfinite(s->depth)) {
-    blocks([&](double2* S, double2* Ab, float2* a32, int images) {
-      launch_assemble(s->d_panels.p, s->d_panels.p, N, K, logc, s->T, s->d_st.p, S, Ab, a32, lda, 2.0 * PI, s->stream,
+    blocks([&](double2* S, double2* Ab, float2* a32, int images, int ld) {
+      launch_assemble(s->d_panels.p, s->d_panels.p, N, K, logc, s->T, s->d_st.p, S, Ab, a32, ld, 2.0 * PI, s->stream,
                       images);
     });
     return K;
@@ -683,8 +683,8 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
     launch_assemble_fin_nn(s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, lda,
                            s->nn, s->d_nnq.p, s->d_nnc.p, s->nn_rmin, s->stream);
   } else
-    blocks([&](double2* S, double2* Ab, float2* a32, int images) {
-      launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32, lda,
+    blocks([&](double2* S, double2* Ab, float2* a32, int images, int ld) {
+      launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32, ld,
                           2.0 * PI, s->stream, images);
     });
   return f.k0;
@@ -1749,7 +1749,7 @@ void solve_limit(HbgSession* s, bool at_infinity, double* A) {
   else
     launch_limit_matrices(s->d_st.p, at_infinity ? s->d_stinf.p : nullptr, N, G * N, Sb, Ab, s->stream);
   if (G > 1)
-    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, Gs, s->d_S.p, s->d_M.p, mstride, nullptr,
+    launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, Gs, s->d_S.p, s->d_M.p, mstride, N, nullptr,
                             s->stream);
   launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, 0, 1.0, 1.0, s->grav, 0.0, s->d_heads.p,
                   s->d_phiI.p, s->d_dphiI.p, s->d_u.p, s->stream, s->walls);
@@ -1772,7 +1772,7 @@ void solve_limit(HbgSession* s, bool at_infinity, double* A) {
                   s->d_dphiI.p, s->d_out.p, s->stream);
   } else {
     s->d_phif.ensure((size_t)NF * R);
-    launch_symmetry_recombine(s->d_M.p, mstride, N, Gs, R, s->d_chisym.p, s->d_phif.p, s->stream);
+    launch_symmetry_recombine(s->d_M.p + (size_t)N * N, mstride, N, N, Gs, R, s->d_chisym.p, s->d_phif.p, s->stream);
     launch_forces_sol(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, 0, 0.0, s->rho, s->d_phif.p, NF, s->d_phiI.p,
                       s->d_dphiI.p, s->d_out.p, s->stream);
   }
@@ -2168,7 +2168,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
                       s->d_out.p + nout * slot(h, b), sb);
       } else {
         // the potential on the whole body from the characters (lo_frequency.rs:931-943)
-        launch_symmetry_recombine(Mh + mstride * b * G, mstride, N, G, R, s->d_chisym.p, s->d_phif.p, sb);
+        launch_symmetry_recombine(Mh + mstride * b * G + (size_t)N * N, mstride, N, N, G, R, s->d_chisym.p,
+                                  s->d_phif.p, sb);
         launch_forces_sol(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, H, omegas[f0 + b], s->rho, s->d_phif.p,
                           NF, s->d_phiI.p + hstride * slot(h, b), s->d_dphiI.p + hstride * slot(h, b),
                           s->d_out.p + nout * slot(h, b), sb);
@@ -2291,8 +2292,6 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   for (int i = 0; i < count; ++i) {
     if (!sessions[i]) return fail(HBG_ERR_ARGUMENT, "null session", "hbg_solve_group");
     if (sessions[i]->N == 0) return fail(HBG_ERR_STATE, "a session has no panels", "hbg_solve_group");
-    if (sessions[i]->G > 1)
-      return fail(HBG_ERR_UNSUPPORTED, "a session with symmetry planes in a group", "hbg_solve_group");
     if (sessions[i]->NI > 0)
       return fail(HBG_ERR_UNSUPPORTED, "a session with an interior free surface in a group", "hbg_solve_group");
     if (sessions[i]->device != sessions[0]->device)
@@ -2317,6 +2316,8 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   GUARD_BEGIN
   check_options(options);
   HbgOptions o = defaults(options);
+  for (int i = 0; i < count; ++i)
+    if (o.surrogate && sessions[i]->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
   use_device(lead);
   const int H = n_headings;
   int nmax = 0, Rmax = 0;
@@ -2345,16 +2346,23 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   for (int i = 0; i < count; ++i) sessions[i]->stream = lead->stream;
   hipStream_t st = lead->stream;
   const bool mixed = o.precision == HBG_PRECISION_MIXED;
-  // the systems (session, frequency) in decreasing order of N (stable): a batch is padded to its
-  // own largest order only, so systems of very different sizes rarely share a batch
+  // the units (session, frequency) in decreasing order of N (stable): a batch is padded to its own
+  // largest order only, so systems of very different sizes rarely share a batch. A unit is Gs
+  // systems in consecutive slots of one batch: the characters of a symmetric body (one with walls
+  // or without images); the boundary values and the forces take its NF = Gs N panels.
   std::vector<std::pair<int, int>> order;
   for (int fi = 0; fi < n_omegas; ++fi)
     for (int i = 0; i < count; ++i) order.push_back({i, fi});
   std::stable_sort(order.begin(), order.end(),
                    [&](const std::pair<int, int>& x, const std::pair<int, int>& y) { return Ns[x.first] > Ns[y.first]; });
-  const int systems = (int)order.size();
+  const int units = (int)order.size();
+  int systems = 0, gmax = 1;
+  for (const auto& u : order) {
+    systems += sessions[u.first]->Gs;
+    gmax = std::max(gmax, sessions[u.first]->Gs);
+  }
   const size_t mstride_max = (size_t)nmax * (nmax + Rmax);
-  const int B = batch_size(lead, nmax, Rmax, systems, mixed);
+  const int B = std::max(batch_size(lead, nmax, Rmax, systems, mixed), gmax);
   lead->d_M.ensure(mstride_max * B);
   if (mixed) {
     lead->d_M32.ensure(mstride_max * B);
@@ -2368,102 +2376,146 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   std::vector<size_t> hstride(count), nout(count);
   for (int i = 0; i < count; ++i) {
     HbgSession* si = sessions[i];
+    const int Gs = si->Gs, NF = Gs * Ns[i];
     const size_t D = nrad[i], nAB = D * nrad[i], nX = (size_t)H * D, nHK = (size_t)nrad[i] * H;
-    hstride[i] = (size_t)std::max(H, 1) * Ns[i];
+    hstride[i] = (size_t)std::max(H, 1) * NF;
     nout[i] = 2 * nAB + 6 * nX + 2 * nHK;
-    si->d_S.ensure((size_t)nmax * Ns[i]);  // leading dimension up to nmax, as A
+    // S: leading dimension up to nmax, as A (the characters' S_c: leading dimension N)
+    si->d_S.ensure(si->G > 1 ? (size_t)Ns[i] * Ns[i] * Gs : (size_t)nmax * Ns[i]);
     si->d_phiI.ensure(hstride[i] * B);
     si->d_dphiI.ensure(hstride[i] * B);
-    si->d_u.ensure((size_t)Rs[i] * Ns[i]);
-    si->d_part.ensure((size_t)rhs_chunks(Ns[i]) * Rs[i] * Ns[i]);
+    si->d_u.ensure((size_t)Rs[i] * NF);
+    si->d_part.ensure((size_t)rhs_chunks(Ns[i]) * Rs[i] * Ns[i] * Gs);
     si->d_out.ensure(nout[i] * B);
+    if (Gs > 1) {
+      si->d_uc.ensure((size_t)Ns[i] * Rs[i] * Gs);
+      si->d_phif.ensure((size_t)NF * Rs[i]);
+    }
   }
   const double logc = o.logarithmic ? 1.0 : 0.0;
-  // system q into slot b of a batch of order n (leading dimension n) with R right-hand sides: its
-  // matrices in the leading N x N block, its right-hand sides at columns n + c, the rest padding
+  // unit q into slots b .. b + Gs - 1 of a batch of order n (leading dimension n) with R
+  // right-hand sides: each system's matrices in its leading N x N block, its right-hand sides at
+  // columns n + c, the rest padding
   auto build = [&](int q, int b, int n, int R, size_t mstride) {
     const int i = order[q].first, fi = order[q].second;
     HbgSession* si = sessions[i];
-    const int N = Ns[i];
+    const int N = Ns[i], Gs = si->Gs, NF = Gs * N;
     const double w = omegas[fi], K = w * w / si->grav;
     const double depth = std::isfinite(si->depth) ? si->depth : 0.0;
     double2* Mb = lead->d_M.p + mstride * b;
     float2* M32b = mixed ? lead->d_M32.p + mstride * b : nullptr;
-    const double kw = assemble_frequency(si, K, logc, Mb, o.surrogate != 0, M32b, n, 0,
+    const double kw = assemble_frequency(si, K, logc, Mb, o.surrogate != 0, M32b, n, mstride,
                                          std::isfinite(si->depth) ? si->d_ffs.p + fi : nullptr);
-    launch_boundary(si->d_panels.p, si->d_body.p, si->d_nj.p, N, nrad[i], H, w, kw, si->grav, depth, si->d_heads.p,
-                    si->d_phiI.p + hstride[i] * b, si->d_dphiI.p + hstride[i] * b, si->d_u.p, st);
-    launch_rhs(si->d_S.p, N, n, si->d_u.p, Rs[i], Mb + (size_t)(n - N) * n, n, si->d_part.p, st);
-    launch_pad_system(Mb, M32b, n, N, R, Rs[i], st);
+    launch_boundary(si->d_panels.p, si->d_body.p, si->d_nj.p, NF, nrad[i], H, w, kw, si->grav, depth, si->d_heads.p,
+                    si->d_phiI.p + hstride[i] * b, si->d_dphiI.p + hstride[i] * b, si->d_u.p, st, si->walls);
+    if (Gs == 1) {
+      launch_rhs(si->d_S.p, N, si->G > 1 ? N : n, si->d_u.p, Rs[i], Mb + (size_t)(n - N) * n, n, si->d_part.p, st);
+    } else {
+      launch_symmetry_project(si->d_u.p, N, Gs, Rs[i], si->d_chisym.p, si->d_uc.p, st);
+      launch_rhs_batched(si->d_S.p, (size_t)N * N, N, N, si->d_uc.p, (size_t)Rs[i] * N, Rs[i],
+                         Mb + (size_t)(n - N) * n, mstride, n, si->d_part.p, Gs, st);
+    }
+    for (int c = 0; c < Gs; ++c)
+      launch_pad_system(Mb + mstride * c, M32b ? M32b + mstride * c : nullptr, n, N, R, Rs[i], st);
   };
-  std::vector<int> status(B), iters(B), nsys(B);
+  std::vector<int> status(B), iters(B), nsys(B), first(B + 1);
   std::vector<double> eta(B), hout;
   std::vector<size_t> hoff(B + 1);
-  for (int q0 = 0, nb = 0; q0 < systems; q0 += nb) {
-    // the batch: the next systems (in decreasing order of N) while there is room and their order
-    // is at least GROUP_ORDER_RATIO of the first one's, so that no system is padded far beyond its
-    // own order (the panel and substitution kernels run over the batch's order)
-    nb = 1;
-    while (nb < B && q0 + nb < systems &&
-           Ns[order[q0 + nb].first] >= GROUP_ORDER_RATIO * Ns[order[q0].first])
-      ++nb;
+  for (int q0 = 0, nu = 0; q0 < units; q0 += nu) {
+    // the batch: the next units (in decreasing order of N) while their systems fit and their
+    // order is at least GROUP_ORDER_RATIO of the first one's, so that no system is padded far
+    // beyond its own order (the panel and substitution kernels run over the batch's order)
+    nu = 0;
+    int nb = 0;
+    while (q0 + nu < units && nb + sessions[order[q0 + nu].first]->Gs <= B &&
+           (nu == 0 || Ns[order[q0 + nu].first] >= GROUP_ORDER_RATIO * Ns[order[q0].first])) {
+      first[nu] = nb;
+      nb += sessions[order[q0 + nu].first]->Gs;
+      ++nu;
+    }
+    first[nu] = nb;
     int n = 0, R = 0;
-    for (int b = 0; b < nb; ++b) {
-      n = std::max(n, Ns[order[q0 + b].first]);
-      R = std::max(R, Rs[order[q0 + b].first]);
+    for (int u = 0; u < nu; ++u) {
+      n = std::max(n, Ns[order[q0 + u].first]);
+      R = std::max(R, Rs[order[q0 + u].first]);
     }
     const size_t mstride = (size_t)n * (n + R);
-    for (int b = 0; b < nb; ++b) {
-      build(q0 + b, b, n, R, mstride);
-      nsys[b] = Ns[order[q0 + b].first];
+    for (int u = 0; u < nu; ++u) {
+      build(q0 + u, first[u], n, R, mstride);
+      for (int b = first[u]; b < first[u + 1]; ++b) nsys[b] = Ns[order[q0 + u].first];
     }
     bool uneven = false;
     for (int b = 0; b < nb; ++b) uneven = uneven || nsys[b] != n;
     const int* ns = uneven ? nsys.data() : nullptr;
+    // the unit of a failing system is rebuilt whole (its characters are assembled together), and
+    // each of its systems solved again by the plain FP64 path
+    auto unit_of = [&](int b) {
+      int u = 0;
+      while (first[u + 1] <= b) ++u;
+      return u;
+    };
     if (mixed) {
       mixed_solve_batched(lead->d_M.p, lead->d_M32.p, lead->d_X64.p, lead->d_D32.p, lead->d_rpart.p, lead->d_rpart2.p,
                           lead->d_rscale.p, nb, n, R, true, MIXED_TOL, MIXED_MAX_ITER, st, nullptr, status.data(),
                           iters.data(), eta.data(), ns, lead->d_active.p);
-      for (int b = 0; b < nb; ++b)
-        if (status[b] < 0) {  // the plain FP64 path for this system
-          build(q0 + b, b, n, R, mstride);
+      for (int u = 0; u < nu; ++u) {
+        bool fail_u = false;
+        for (int b = first[u]; b < first[u + 1]; ++b) fail_u = fail_u || status[b] < 0;
+        if (!fail_u) continue;
+        build(q0 + u, first[u], n, R, mstride);
+        for (int b = first[u]; b < first[u + 1]; ++b)
           lu_solve_batched(lead->d_M.p + mstride * b, mstride, 1, n, n, R, st, nullptr, &status[b],
                            uneven ? &nsys[b] : nullptr);
-        }
+      }
     } else {
       lu_solve_batched(lead->d_M.p, mstride, nb, n, n, R, st, nullptr, status.data(), ns);
     }
     for (int b = 0; b < nb; ++b) {
-      if (status[b] < 0) {  // a panel needs interchanges: rebuild the system, pivoted path
-        build(q0 + b, b, n, R, mstride);
-        status[b] = lu_solve_pivoted(lead->d_M.p + mstride * b, n, n, R, st, nullptr);
+      if (status[b] < 0) {  // a panel needs interchanges: rebuild the unit, pivoted path for its systems
+        const int u = unit_of(b);
+        build(q0 + u, first[u], n, R, mstride);
+        for (int c = first[u]; c < first[u + 1]; ++c) {
+          const int sc = lu_solve_pivoted(lead->d_M.p + mstride * c, n, n, R, st, nullptr);
+          if (c >= b) status[c] = sc;
+          else if (sc != 0) status[b] = sc;
+        }
       }
       if (status[b] > 0)
         throw Error{HBG_ERR_NUMERICAL, "singular system matrix (zero pivot " + std::to_string(status[b]) + ")"};
     }
-    for (int b = 0; b < nb; ++b) {
-      const int i = order[q0 + b].first, fi = order[q0 + b].second;
+    for (int u = 0; u < nu; ++u) {
+      const int i = order[q0 + u].first, fi = order[q0 + u].second, b = first[u];
       HbgSession* si = sessions[i];
-      launch_forces(si->d_panels.p, si->d_body.p, si->d_nj.p, Ns[i], nrad[i], H, omegas[fi], si->rho,
-                    lead->d_M.p + mstride * b + (size_t)(n - Ns[i]) * n, n, si->d_phiI.p + hstride[i] * b,
-                    si->d_dphiI.p + hstride[i] * b, si->d_out.p + nout[i] * b, st);
+      const int N = Ns[i], Gs = si->Gs;
+      const double2* sol = lead->d_M.p + mstride * b + (size_t)n * n;  // the solution columns, leading dimension n
+      if (Gs == 1) {
+        launch_forces(si->d_panels.p, si->d_body.p, si->d_nj.p, N, nrad[i], H, omegas[fi], si->rho,
+                      lead->d_M.p + mstride * b + (size_t)(n - N) * n, n, si->d_phiI.p + hstride[i] * b,
+                      si->d_dphiI.p + hstride[i] * b, si->d_out.p + nout[i] * b, st);
+      } else {
+        // the potential on the whole body from the characters (lo_frequency.rs:931-943)
+        launch_symmetry_recombine(sol, mstride, n, N, Gs, Rs[i], si->d_chisym.p, si->d_phif.p, st);
+        launch_forces_sol(si->d_panels.p, si->d_body.p, si->d_nj.p, Gs * N, nrad[i], H, omegas[fi], si->rho,
+                          si->d_phif.p, Gs * N, si->d_phiI.p + hstride[i] * b, si->d_dphiI.p + hstride[i] * b,
+                          si->d_out.p + nout[i] * b, st);
+      }
     }
-    // every system's outputs in one host buffer, one wait for the batch
+    // every unit's outputs in one host buffer, one wait for the batch
     hoff[0] = 0;
-    for (int b = 0; b < nb; ++b) hoff[b + 1] = hoff[b] + nout[order[q0 + b].first];
-    hout.resize(hoff[nb]);
-    for (int b = 0; b < nb; ++b) {
-      const int i = order[q0 + b].first;
-      HBG_HIP(hipMemcpyAsync(hout.data() + hoff[b], sessions[i]->d_out.p + nout[i] * b, nout[i] * sizeof(double),
-                             hipMemcpyDeviceToHost, st));
+    for (int u = 0; u < nu; ++u) hoff[u + 1] = hoff[u] + nout[order[q0 + u].first];
+    hout.resize(hoff[nu]);
+    for (int u = 0; u < nu; ++u) {
+      const int i = order[q0 + u].first;
+      HBG_HIP(hipMemcpyAsync(hout.data() + hoff[u], sessions[i]->d_out.p + nout[i] * first[u],
+                             nout[i] * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     HBG_HIP(hipStreamSynchronize(st));
     for (double v : hout)
       if (!std::isfinite(v)) throw Error{HBG_ERR_NUMERICAL, "non-finite solution"};
-    for (int b = 0; b < nb; ++b) {
-      const int i = order[q0 + b].first, fi = order[q0 + b].second;
+    for (int u = 0; u < nu; ++u) {
+      const int i = order[q0 + u].first, fi = order[q0 + u].second;
       const size_t D = nrad[i], nAB = D * nrad[i], nX = (size_t)H * D, nHK = (size_t)nrad[i] * H;
-      const double* hA = hout.data() + hoff[b];
+      const double* hA = hout.data() + hoff[u];
       const double* hB = hA + nAB;
       const double* hX = hB + nAB;
       const double* hFK = hX + 2 * nX;

@@ -712,15 +712,17 @@ void launch_kochin(const PanelD* P, const int* body, const double* nj, int N, in
 // S_c = sum_g chi_c(g) S_g and the matrix part of M_c = sum_g chi_c(g) A_g for every character c,
 // with A32 (nullable) the complex64 copy of M_c, for the first nchar characters (all of them for a
 // symmetric body; with walls, the first only: the one even under every wall reflection). One pass:
-// each entry of the G blocks is read once and each character entry written once. chi: [G][G] row c, column g; S_g, A_g at Sg + g N N,
-// Ag + g N N; M_c at M + c mstride (leading dimension N), S_c at S + c N N.
+// each entry of the G blocks is read once and each character entry written once. chi: [G][G] row
+// c, column g; S_g, A_g at Sg + g N N, Ag + g N N (leading dimension N); M_c at M + c mstride with
+// leading dimension ldm (>= N: a system padded to a batch's larger order), S_c at S + c N N.
 template <int G>
 __global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const double2* __restrict__ Ag, int N,
                                         const double* __restrict__ chi, int nchar, double2* __restrict__ S,
-                                        double2* __restrict__ M, size_t mstride, float2* __restrict__ A32) {
+                                        double2* __restrict__ M, size_t mstride, int ldm, float2* __restrict__ A32) {
   const size_t NN = (size_t)N * N;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= NN) return;
+  const size_t em = (e / N) * ldm + e % N;
   double2 sg[G], ag[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -740,19 +742,20 @@ __global__ void symmetry_combine_kernel(const double2* __restrict__ Sg, const do
       mv.y += x * ag[g].y;
     }
     S[c * NN + e] = sv;
-    M[c * mstride + e] = mv;
-    if (A32) A32[c * mstride + e] = float2((float)mv.x, (float)mv.y);
+    M[c * mstride + em] = mv;
+    if (A32) A32[c * mstride + em] = float2((float)mv.x, (float)mv.y);
   }
 }
 
 void launch_symmetry_combine(const double2* Sg, const double2* Ag, int N, int G, const double* chi, int nchar,
-                             double2* S, double2* M, size_t mstride, float2* A32, hipStream_t stream) {
+                             double2* S, double2* M, size_t mstride, int ldm, float2* A32, hipStream_t stream) {
   const unsigned blocks = (unsigned)(((size_t)N * N + 255) / 256);
   if (nchar < 1 || nchar > G) throw Error{6, "characters out of range"};
+  if (ldm < N) throw Error{6, "leading dimension below the order"};
   if (G == 2)
-    symmetry_combine_kernel<2><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, nchar, S, M, mstride, A32);
+    symmetry_combine_kernel<2><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, nchar, S, M, mstride, ldm, A32);
   else if (G == 4)
-    symmetry_combine_kernel<4><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, nchar, S, M, mstride, A32);
+    symmetry_combine_kernel<4><<<blocks, 256, 0, stream>>>(Sg, Ag, N, chi, nchar, S, M, mstride, ldm, A32);
   else
     throw Error{6, "symmetry groups have 2 or 4 elements"};
   HBG_HIP(hipGetLastError());
@@ -783,9 +786,9 @@ void launch_symmetry_project(const double2* u, int N, int G, int R, const double
 }
 
 // The potential on the whole body from the character solutions (lo_frequency.rs:931-943):
-// phi[col][g N + n] = sum_c chi_c(g) phi_c[col][n], phi_c the solution columns of M_c
-// (M + c mstride + N N, leading dimension N).
-__global__ void symmetry_recombine_kernel(const double2* __restrict__ M, size_t mstride, int N, int G, int R,
+// phi[col][g N + n] = sum_c chi_c(g) phi_c[col][n], phi_c the solution columns of character c at
+// sol + c mstride (leading dimension lds).
+__global__ void symmetry_recombine_kernel(const double2* __restrict__ sol, size_t mstride, int lds, int N, int G, int R,
                                           const double* __restrict__ chi, double2* __restrict__ phi) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)G * N * R) return;
@@ -794,17 +797,17 @@ __global__ void symmetry_recombine_kernel(const double2* __restrict__ M, size_t 
   double2 v = make_double2(0.0, 0.0);
   for (int c = 0; c < G; ++c) {
     const double x = chi[c * G + g];
-    const double2 a = M[(size_t)c * mstride + (size_t)N * N + (size_t)col * N + n];
+    const double2 a = sol[(size_t)c * mstride + (size_t)col * lds + n];
     v.x += x * a.x;
     v.y += x * a.y;
   }
   phi[e] = v;
 }
 
-void launch_symmetry_recombine(const double2* M, size_t mstride, int N, int G, int R, const double* chi, double2* phi,
-                               hipStream_t stream) {
-  symmetry_recombine_kernel<<<(unsigned)(((size_t)G * N * R + 255) / 256), 256, 0, stream>>>(M, mstride, N, G, R,
-                                                                                            chi, phi);
+void launch_symmetry_recombine(const double2* sol, size_t mstride, int lds, int N, int G, int R, const double* chi,
+                               double2* phi, hipStream_t stream) {
+  symmetry_recombine_kernel<<<(unsigned)(((size_t)G * N * R + 255) / 256), 256, 0, stream>>>(sol, mstride, lds, N, G,
+                                                                                            R, chi, phi);
   HBG_HIP(hipGetLastError());
 }
 

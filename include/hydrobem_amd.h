@@ -127,8 +127,8 @@ HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_pane
  * axis the group of both is used (only the characters even under the walls are solved); a plane
  * on the axis of a wall is HBG_ERR_VALUE. The frequency limits omega = 0 and infinity are solved the
  * same way (the limit matrices' image blocks combined into the characters; at omega = 0 in finite
- * depth the net-flux rule takes the whole body). With G > 1, hbg_statics, hbg_assemble,
- * hbg_solve_group and the surrogate are HBG_ERR_UNSUPPORTED. */
+ * depth the net-flux rule takes the whole body); hbg_solve_group takes such sessions too. With
+ * G > 1, hbg_statics, hbg_assemble and the surrogate are HBG_ERR_UNSUPPORTED. */
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
 
 /* Walls (the domain x_wall, y_wall, orthogonal_walls of model_codes.rs:15-31): wall_x 1 a
@@ -141,8 +141,8 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
  * hbg_set_panels. With a symmetry plane on the other axis the panels are the body's half and the
  * group of both is used (lo_geometry.rs:863-945); a wall on the axis of a symmetry plane is
  * HBG_ERR_VALUE. The frequency limits omega = 0 and infinity are solved with the walls too (at
- * omega = 0 in finite depth the net-flux rule takes the real bodies). With walls hbg_statics,
- * hbg_assemble, hbg_solve_group and the surrogate are HBG_ERR_UNSUPPORTED. */
+ * omega = 0 in finite depth the net-flux rule takes the real bodies); hbg_solve_group takes such
+ * sessions too. With walls hbg_statics, hbg_assemble and the surrogate are HBG_ERR_UNSUPPORTED. */
 HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
 
 /* Field points (lo_frequency.rs:1392-1495, `Output` frad/finc/fsca/ftot, low_order.rs:1490-1500):
@@ -214,9 +214,12 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
  * hbg_solve's). The systems are taken in decreasing order of N, in batches; a system of N panels
  * goes to the LU at the largest order of its batch, with the identity on the rows and columns
  * past N (its LU is block-diagonal, its solution unchanged) and the GEMM tiles of that identity
- * block skipped. Outputs: arrays of count pointers (arrays or pointers may be NULL), entry i laid
- * out as hbg_solve's for session i. All work runs on the first session's stream; no session may
- * be in use elsewhere during the call. */
+ * block skipped. A session with symmetry planes or walls brings its character systems (G or one
+ * per frequency, of the sector's order), kept in one batch, and its results are those of its own
+ * hbg_solve; ordinary frequencies only, no interior free surface, no surrogate with symmetry
+ * planes or walls (else HBG_ERR_UNSUPPORTED). Outputs: arrays of count pointers (arrays or
+ * pointers may be NULL), entry i laid out as hbg_solve's for session i. All work runs on the
+ * first session's stream; no session may be in use elsewhere during the call. */
 HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const double* omegas, int32_t n_omegas,
                           const double* headings_deg, int32_t n_headings, const HbgOptions* options,
                           double* const* added_mass, double* const* damping, double* const* excitation,

@@ -82,3 +82,47 @@ def test_group_of_hull_variants_is_bit_identical(precision):
             assert np.array_equal(g[key], ref[key]), (key, rel(g[key], ref[key]))
     for s in sessions:
         s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_with_symmetry_planes_and_walls(precision):
+    """Sessions with symmetry planes (their character systems kept in one batch) and with walls
+    (one even character) beside a plain geometry: each gives the results of its own solve."""
+    quadrant = meshes.cylinder_panels(1.0, 0.8, 12, 5, 4, quarter=True)
+    q = meshes.hemisphere_panels(1.0, 8, 6, quarter=True)
+    # the half x >= 0 of the symmetry plane x = 0: the quadrant and its mirror image in y = 0 (the
+    # vertex order reversed, so the normals stay in the fluid)
+    half = np.concatenate([q, (q * np.array([1.0, -1.0, 1.0]))[:, [0, 3, 2, 1]]])
+    box = meshes.place(meshes.box_panels(1.0, 0.5, 0.5, (6, 3, 3), quarter=False), (2.0, 1.5, 0.0))
+    plain = meshes.hemisphere_panels(0.9, 8, 14, quarter=False)
+    specs = [  # (vertices, symmetry, walls, depth, rotation centre)
+        (quadrant, (1, 1), (0, 0), 3.0, [0.0, 0.0, -0.1]),
+        (half, (1, 0), (0, 0), np.inf, [0.0, 0.0, -0.2]),
+        (box, (0, 0), (1, 1), 3.0, [2.0, 1.5, -0.1]),
+        (plain, (0, 0), (0, 0), np.inf, [0.0, 0.0, 0.0]),
+    ]
+    sessions = []
+    for v, sym, walls, depth, c in specs:
+        s = hb.Session(0)
+        s.set_environment(1025.0, 9.81, depth)
+        s.set_symmetry(*sym)
+        s.set_panels(v, rotation_centres=np.array([c]))
+        s.set_walls(*walls)
+        sessions.append(s)
+    say(f"case: group of a quadrant (both planes), a half (plane x), a box by two walls and a plain "
+        f"hemisphere, panels {[s.n_panels for s in sessions]}, {precision}")
+    w = np.array([0.7, 1.6])
+    heads = [0.0, 35.0]
+    sessions[0].set_batch(6)  # a batch of six slots: the quadrant's four characters never split
+    group = hb.solve_group(sessions, w, heads, precision=precision)
+    tol = 1e-11 if precision == "fp64" else 1e-9
+    for s, g in zip(sessions, group):
+        ref = s.solve(w, heads, precision=precision)
+        for key in ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_haskind"):
+            e = rel(g[key], ref[key])
+            say(f"  N {s.n_panels} {key}: {e:.2e}")
+            assert e < tol
+    for s in sessions:
+        s.set_walls(False, False)
+        s.set_symmetry(False, False)
+        s.close()
