@@ -137,7 +137,7 @@ void launch_limit_points(double h, bool zero, const double* ktab, int n, const d
                          const double* zeta, double* out, hipStream_t stream);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                             const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
-                            int2* queue, unsigned* qcount, double rmin, hipStream_t stream);
+                            int2* queue, unsigned* qcount, double rmin, hipStream_t stream, int images);
 void launch_remainder(const FinFreq* f, FinTables F, WaveTables T, int n, const double* R, const double* z,
                       const double* zeta, double* out, hipStream_t stream);
 void nn_eval(const NnDevice& nn, const float4* X, int n, float4* Y, hipStream_t stream);
@@ -154,6 +154,7 @@ int motions_max_headings();
 int lu_autotune(int precision64, int n, int nrhs, int max_batch, size_t mem_bytes, hipStream_t stream, double* report,
                 int max_rows, int* rows);
 void lu_set_graphs(int on);
+void lu_graphs_note_free();  // device memory was freed: the LU's captured graphs are not replayed
 int lu_graphs();
 void lu_set_sub2(int q);
 void lu_set_sub2_32(int q);
@@ -327,13 +328,19 @@ struct DevBuf {
   ~DevBuf() { release(); }
   void ensure(size_t count) {
     if (count <= n && p) return;
-    if (p) HBG_HIP(hipFree(p));
+    if (p) {
+      HBG_HIP(hipFree(p));
+      lu_graphs_note_free();
+    }
     p = nullptr;
     HBG_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
     n = count;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      lu_graphs_note_free();
+    }
     p = nullptr;
     n = 0;
   }
@@ -640,7 +647,7 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
                           float2* A32 = nullptr, int lda = 0, size_t mstride = 0, const FinFreq* dff = nullptr) {
   const int N = s->N, G = s->G;
   if (lda <= 0) lda = N;
-  if (G > 1 && (surrogate || !mstride)) throw Error{HBG_ERR_UNSUPPORTED, "symmetry planes with the surrogate"};
+  if (G > 1 && !mstride) throw Error{HBG_ERR_INTERNAL, "symmetry planes without the stride of the character systems"};
   if (G > 1) {
     s->d_Sg.ensure((size_t)N * N * G);
     s->d_Ag.ensure((size_t)N * N * G);
@@ -678,10 +685,12 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   }
   const double nuh = K * s->depth;
   if (surrogate && s->nn.ready && nuh >= std::exp(NN_LNNU_LO) && nuh <= std::exp(NN_LNNU_HI)) {
-    s->d_nnq.ensure((size_t)N * N);
+    s->d_nnq.ensure((size_t)N * (N + 1) / 2 * G);  // every pair of every image block may fall back
     s->d_nnc.ensure(1);
-    launch_assemble_fin_nn(s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, s->d_S.p, Mout, A32, lda,
-                           s->nn, s->d_nnq.p, s->d_nnc.p, s->nn_rmin, s->stream);
+    blocks([&](double2* S, double2* Ab, float2* a32, int images, int ld) {
+      launch_assemble_fin_nn(s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32, ld, s->nn,
+                             s->d_nnq.p, s->d_nnc.p, s->nn_rmin, s->stream, images);
+    });
   } else
     blocks([&](double2* S, double2* Ab, float2* a32, int images, int ld) {
       launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32, ld,
@@ -1846,7 +1855,6 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions with an interior free surface"};
     if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
   }
-  if (o.surrogate && s->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
   if ((s->nk > 0 || s->np > 0) && !il.empty())
     throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions at the frequency limits"};
   if (s->np > 0 && std::isfinite(s->depth))
@@ -2316,8 +2324,6 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   GUARD_BEGIN
   check_options(options);
   HbgOptions o = defaults(options);
-  for (int i = 0; i < count; ++i)
-    if (o.surrogate && sessions[i]->G > 1) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with symmetry planes"};
   use_device(lead);
   const int H = n_headings;
   int nmax = 0, Rmax = 0;
@@ -2345,7 +2351,9 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   }
   for (int i = 0; i < count; ++i) sessions[i]->stream = lead->stream;
   hipStream_t st = lead->stream;
-  const bool mixed = o.precision == HBG_PRECISION_MIXED;
+  // the FP32 assembly keeps one geometry's matrices per system; a group runs it as
+  // HBG_PRECISION_MIXED (FP64 assembly, complex64 LU, FP64 refinement)
+  const bool mixed = o.precision == HBG_PRECISION_MIXED || o.precision == HBG_PRECISION_MIXED32;
   // the units (session, frequency) in decreasing order of N (stable): a batch is padded to its own
   // largest order only, so systems of very different sizes rarely share a batch. A unit is Gs
   // systems in consecutive slots of one batch: the characters of a symmetric body (one with walls

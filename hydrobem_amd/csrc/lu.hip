@@ -21,6 +21,7 @@
 // more it is tournament pivoting (Grigori, Demmel & Xiang 2011).
 // Then U12 = L11^-1 A12 and the trailing update A22 -= L21 U12 (gemm_mfma_kernel, complex FP64
 // on the matrix cores), once per block of panels in the pass without interchanges.
+#include <atomic>
 #include <algorithm>
 #include <array>
 #include <type_traits>
@@ -974,17 +975,24 @@ static LuWorkspace<T>& workspace() {
 static int g_lu_graphs = 1;
 void lu_set_graphs(int on) { g_lu_graphs = on; }
 int lu_graphs() { return g_lu_graphs; }
+// Bumped whenever the library frees device memory that a solve's matrices may live in: a
+// captured graph replayed after its matrices' memory was freed and handed out again at the same
+// address gave wrong factors (measured: a session freed, a new one with the same buffer addresses,
+// its replayed LU 1e5 off the eager one), so a graph is replayed only within the epoch it was
+// captured in.
+static std::atomic<unsigned long> g_free_epoch{0};
+void lu_graphs_note_free() { g_free_epoch.fetch_add(1, std::memory_order_relaxed); }
 
 namespace {
 struct GraphKey {
   const void* M;
   size_t mstride;
   int batch, n, lda, nrhs, variant, depth, real_bytes, uneven, sub;
-  unsigned long generation;
+  unsigned long generation, epoch;
   bool operator==(const GraphKey& o) const {
     return M == o.M && mstride == o.mstride && batch == o.batch && n == o.n && lda == o.lda && nrhs == o.nrhs &&
            variant == o.variant && depth == o.depth && real_bytes == o.real_bytes && uneven == o.uneven &&
-           sub == o.sub && generation == o.generation;
+           sub == o.sub && generation == o.generation && epoch == o.epoch;
   }
 };
 struct GraphCache {
@@ -1337,7 +1345,8 @@ void solve_batched(typename Cplx<T>::type* M, size_t mstride, int batch, int n, 
     const GraphKey key{(const void*)M, mstride, batch, n, lda, nrhs, g_gemm_variant_[prec_index<T>()],
                        g_lu_depth_[prec_index<T>()],
                        std::is_same<T, double>::value ? 8 : 4, nsys ? 1 : 0,
-                       g_lu_sub_[prec_index<T>()] + 100 * g_lu_sub2_[prec_index<T>()], ws.generation};
+                       g_lu_sub_[prec_index<T>()] + 100 * g_lu_sub2_[prec_index<T>()], ws.generation,
+                       g_free_epoch.load(std::memory_order_relaxed)};
     GraphCache& gc = graphs();
     hipGraphExec_t exec = gc.find(key);
     if (!exec) {

@@ -383,29 +383,33 @@ constexpr int NN_PER_WAVE = 16;
 __device__ __forceinline__ void write_entry(double2* __restrict__ S, double2* __restrict__ A,
                                             float2* __restrict__ A32, int lda, int row, int col, const StaticD& e,
                                             double Ar, double nz, double a, double c2, double Wr, double WrR,
-                                            double WrZ, double Wi, double WiR, double WiZ) {
+                                            double WrZ, double Wi, double WiR, double WiZ, double free_term) {
   S[(size_t)col * lda + row] = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
   const double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
-  const double2 v = make_double2((row == col ? 2.0 * PI : 0.0) - dre, -dim);
+  const double2 v = make_double2((row == col ? free_term : 0.0) - dre, -dim);
   A[(size_t)col * lda + row] = v;
   if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
 }
 
-// both entries of the pair (k, n), k <= n, from the pair's kernel values
-__device__ __forceinline__ void write_pair(const PanelD* __restrict__ P, int N, const StaticD* __restrict__ st,
-                                           const FinK& w, double c2, int k, int n, int lda, double2* __restrict__ S,
-                                           double2* __restrict__ A, float2* __restrict__ A32) {
+// both entries of the pair (k, n), k <= n, from the pair's kernel values: field point k, source
+// Psrc + n, and field point n, source Psrc + k (the panels themselves, or their images in one
+// reflection g: |c_k - g c_n| = |c_n - g c_k|, and the mirrored entry's normal term is that of
+// panel k, g being orthogonal), st, S, A those of the block, free_term on its diagonal
+__device__ __forceinline__ void write_pair(const PanelD* __restrict__ P, const PanelD* __restrict__ Psrc, int N,
+                                           const StaticD* __restrict__ st, const FinK& w, double c2, int k, int n,
+                                           int lda, double2* __restrict__ S, double2* __restrict__ A,
+                                           float2* __restrict__ A32, double free_term) {
   const PanelD* pk = P + k;
-  const PanelD* pn = P + n;
+  const PanelD* pn = Psrc + n;
   const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
   const double R = sqrt(dx * dx + dy * dy);
   const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
   write_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi, w.WiR,
-              w.WiZ);
+              w.WiZ, free_term);
   if (k != n) {
     const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
     write_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi, w.WiR,
-                w.Wiz);
+                w.Wiz, free_term);
   }
 }
 
@@ -414,8 +418,14 @@ assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __res
                        double logc, const StaticD* __restrict__ st, double2* __restrict__ S, double2* __restrict__ A,
                        float2* __restrict__ A32, int lda, const v8bf* __restrict__ gpk, const float* __restrict__ gbias,
                        int2* __restrict__ queue, unsigned* __restrict__ qcount, double rmin) {
-  const int bi = blockIdx.x, bj = blockIdx.y;
+  const int bi = blockIdx.x, bj = blockIdx.y, g = blockIdx.z;
   if (bi > bj) return;  // workgroup-uniform
+  // image block g of a symmetric body or walls: sources g n, no free term (A32 is then null)
+  const PanelD* Psrc = P + (size_t)g * N;
+  st += (size_t)g * N * N;
+  S += (size_t)g * N * N;
+  A += (size_t)g * N * lda;
+  const double free_term = g ? 0.0 : 2.0 * PI;
   __shared__ v8bf pk[NN_PK_V8];
   __shared__ float bias[NN_PK_BIAS];
   nn_stage(gpk, gbias, pk, bias);
@@ -430,7 +440,7 @@ assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __res
     const int n = bj * 64 + wave * NN_PER_WAVE + t;
     if (n >= N) break;  // wave-uniform
     const bool live = k < N && !(bi == bj && k > n);
-    const PanelD* pn = P + n;
+    const PanelD* pn = Psrc + n;
     const double dx = xk - pn->c[0], dy = yk - pn->c[1];
     const double R = sqrt(dx * dx + dy * dy), r = R * ih;
     const bool nn = r >= rmin && r <= 20.0;
@@ -441,7 +451,7 @@ assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __res
       unsigned base = 0;
       if (lane == 0) base = atomicAdd(qcount, (unsigned)__popcll(fb));
       base = __shfl(base, 0);
-      if (live && !nn) queue[base + (unsigned)__popcll(fb & ((1ull << lane) - 1ull))] = make_int2(k, n);
+      if (live && !nn) queue[base + (unsigned)__popcll(fb & ((1ull << lane) - 1ull))] = make_int2(k, g * N + n);
     }
     if (!live || !nn) continue;
     const FinProp pp = fin_prop(f, F, T, kk, n, R);
@@ -457,11 +467,12 @@ assemble_fin_nn_kernel(const PanelD* __restrict__ P, int N, const FinFreq* __res
     w.WiR = pp.a * k0 * pp.j1;
     w.WiZ = -pp.az * pp.j0;
     w.Wiz = -pp.af * pp.j0;
-    write_pair(P, N, st, w, c2, k, n, lda, S, A, A32);
+    write_pair(P, Psrc, N, st, w, c2, k, n, lda, S, A, A32, free_term);
   }
 }
 
-// the queued pairs by the FP64 evaluator (grid-stride over the count the first pass left)
+// the queued pairs by the FP64 evaluator (grid-stride over the count the first pass left); a
+// queued pair (k, g N + n) is the pair (k, n) of the image block g
 __global__ void __launch_bounds__(256) fallback_kernel(const PanelD* __restrict__ P, int N,
                                                        const FinFreq* __restrict__ f, FinTables F, WaveTables T,
                                                        double logc, const StaticD* __restrict__ st,
@@ -473,10 +484,12 @@ __global__ void __launch_bounds__(256) fallback_kernel(const PanelD* __restrict_
   const double c2 = 2.0 * f->K * logc;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const int2 q = queue[i];
-    const int k = q.x, n = q.y;
-    const double dx = P[k].c[0] - P[n].c[0], dy = P[k].c[1] - P[n].c[1];
-    const FinK w = fin_kernel(f, F, T, k, n, sqrt(dx * dx + dy * dy), P[k].c[2], P[n].c[2]);
-    write_pair(P, N, st, w, c2, k, n, lda, S, A, A32);
+    const int k = q.x, g = q.y / N, n = q.y % N;
+    const PanelD* Psrc = P + (size_t)g * N;
+    const double dx = P[k].c[0] - Psrc[n].c[0], dy = P[k].c[1] - Psrc[n].c[1];
+    const FinK w = fin_kernel(f, F, T, k, n, sqrt(dx * dx + dy * dy), P[k].c[2], Psrc[n].c[2]);
+    write_pair(P, Psrc, N, st + (size_t)g * N * N, w, c2, k, n, lda, S + (size_t)g * N * N, A + (size_t)g * N * lda,
+               A32, g ? 0.0 : 2.0 * PI);
   }
 }
 
@@ -881,13 +894,16 @@ void nn_train(unsigned long long seed, int n_train, int n_hold, int epochs, int 
   report[8] = (double)step;
 }
 
-// queue: N (N + 1) / 2 pairs of room (every pair may fall back), qcount: one counter
+// queue: images N (N + 1) / 2 pairs of room (every pair may fall back), qcount: one counter.
+// images > 1 (a symmetric body or walls): grid z = g assembles the image block g (sources the
+// records P + g N, S, A and the statics at g N N, leading dimension N), as assemble_fin_kernel.
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
                             const StaticD* st, double2* S, double2* A, float2* A32, int lda, const NnDevice& nn,
-                            int2* queue, unsigned* qcount, double rmin, hipStream_t stream) {
+                            int2* queue, unsigned* qcount, double rmin, hipStream_t stream, int images) {
+  if (images > 1 && (A32 || lda != N)) throw Error{6, "image blocks without a complex64 copy, at leading dimension N"};
   zero_u32_kernel<<<1, 64, 0, stream>>>(qcount);
   const unsigned nt = (unsigned)((N + 63) / 64);
-  dim3 grid(nt, nt);
+  dim3 grid(nt, nt, images);
   assemble_fin_nn_kernel<<<grid, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda, (const v8bf*)nn.pk,
                                                    nn.bias, queue, qcount, std::max(rmin, NN_RMIN));
   fallback_kernel<<<2048, 256, 0, stream>>>(P, N, f, F, T, logc, st, S, A, A32, lda, queue, qcount);

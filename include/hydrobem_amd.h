@@ -217,9 +217,10 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
  * block skipped. A session with symmetry planes or walls brings its character systems (G or one
  * per frequency, of the sector's order), kept in one batch, and its results are those of its own
  * hbg_solve; ordinary frequencies only, no interior free surface, no surrogate with symmetry
- * planes or walls (else HBG_ERR_UNSUPPORTED). Outputs: arrays of count pointers (arrays or
- * pointers may be NULL), entry i laid out as hbg_solve's for session i. All work runs on the
- * first session's stream; no session may be in use elsewhere during the call. */
+ * planes or walls (else HBG_ERR_UNSUPPORTED); HBG_PRECISION_MIXED32 runs as HBG_PRECISION_MIXED.
+ * Outputs: arrays of count pointers (arrays or pointers may be NULL), entry i laid out as
+ * hbg_solve's for session i. All work runs on the first session's stream; no session may be in
+ * use elsewhere during the call. */
 HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const double* omegas, int32_t n_omegas,
                           const double* headings_deg, int32_t n_headings, const HbgOptions* options,
                           double* const* added_mass, double* const* damping, double* const* excitation,

@@ -87,3 +87,40 @@ def test_autotune_sweep_and_solve(oracle):
     s.set_tuning(s.TUNE_LU_SUB, 0)
     s.set_tuning(s.TUNE_LU_SUB2, 0)
     s.close()
+
+
+def test_graph_replays_after_sessions_are_freed():
+    """Sessions freed and made again get their buffers at the same addresses, so a new session's
+    LU meets graphs captured for the old one's matrices: the cache replays a graph only within the
+    epoch of device frees it was captured in, and the results equal the eager launches'."""
+    say("case: three sessions solved, freed and made again (same sizes), graphs vs eager launches")
+    geoms = [meshes.hemisphere_panels(1.0, 10, 20, quarter=False),
+             meshes.cylinder_panels(1.0, 1.0, 22, 7, 7, quarter=False, cosine=True),
+             meshes.hemisphere_panels(0.8, 8, 16, quarter=False)]
+    w, heads = np.array([0.6, 1.3, 2.2]), [0.0, 30.0]
+
+    def make():
+        out = []
+        for v in geoms:
+            s = hb.Session(0)
+            s.set_environment(1025.0, 9.81)
+            s.set_panels(v)
+            out.append(s)
+        return out
+
+    for _ in range(2):
+        ss = make()
+        for s in ss:
+            s.solve(w, heads)
+        for s in ss:
+            s.close()
+    ss = make()
+    graphs = [s.solve(w, heads) for s in ss]
+    ss[0].set_tuning(ss[0].TUNE_LU_GRAPHS, 0)
+    eager = [s.solve(w, heads) for s in ss]
+    ss[0].set_tuning(ss[0].TUNE_LU_GRAPHS, 1)
+    for g, e in zip(graphs, eager):
+        for key in ("added_mass", "radiation_damping", "excitation_diffraction"):
+            assert np.array_equal(g[key], e[key]), key
+    for s in ss:
+        s.close()

@@ -58,6 +58,11 @@ def test_group_solve_matches_separate_solves(precision):
             e = rel(g[key], ref[key])
             say(f"  N {s.n_panels} {key}: {e:.2e}")
             assert e < tol
+    if precision == "mixed":  # the FP32 assembly's precision runs as the mixed one in a group
+        again = hb.solve_group(sessions, w, heads, precision="mixed32")
+        for g, a in zip(group, again):
+            for key in ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_haskind"):
+                assert np.array_equal(g[key], a[key]), key
     for s in sessions:
         s.close()
 

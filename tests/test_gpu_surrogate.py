@@ -152,3 +152,54 @@ def test_shipped_network_held_out(session):
         # measured on MI355X: 0.5-7 % per output (BF16 weights and activations), the floor that
         # keeps the surrogate's A, B, X about a per cent off the evaluator's
         assert np.all(err < 0.1)
+
+
+def _reflect(v, sx, sy):
+    r = np.asarray(v, dtype=float) * np.array([sx, sy, 1.0])
+    return r[:, [0, 3, 2, 1]] if sx * sy < 0 else r
+
+
+@pytest.mark.skipif(not train_surrogate.WEIGHTS.exists(), reason="no trained surrogate shipped")
+@pytest.mark.parametrize("sym_x,sym_y", [(1, 1), (0, 1)])
+def test_surrogate_on_a_sector_against_the_whole_mesh(sym_x, sym_y):
+    """The surrogate over the image blocks of a symmetric body (the network and its fallback queue
+    in every block) against the whole mesh. With the network's lower R/h bound above every pair
+    all pairs take the fallback queue, and the sector gives the evaluator's results of the whole
+    mesh to the solver's rounding (the queue's image blocks, statics and free term). With the
+    network on, a pair (k, g n) of the sector is the entry (k, m) of the whole mesh, which the
+    whole mesh may take as the mirror of its pair (m, k): the network at (z_m, z_k) instead of
+    (z_k, z_m), equal only to the network's own error, so the two differ by less than either
+    differs from the evaluator."""
+    q = meshes.cylinder_panels(1.0, 0.8, 12, 5, 4, quarter=True)
+    if sym_x and sym_y:
+        sector, whole = q, meshes.expand_symmetry(q)
+    else:  # the plane y = 0: the half y >= 0, the quadrant and its mirror in x = 0
+        sector = np.concatenate([q, _reflect(q, -1.0, 1.0)])
+        whole = np.concatenate([sector, _reflect(sector, 1.0, -1.0)])
+    say(f"case: surrogate on a sector of {len(sector)} of {len(whole)} panels, planes x {sym_x} y {sym_y}, depth 3")
+    a, b = hb.Session(0), hb.Session(0)
+    try:
+        for s in (a, b):
+            train_surrogate.load(s)
+            s.set_environment(1025.0, 9.81, 3.0)
+        a.set_symmetry(sym_x, sym_y)
+        a.set_panels(sector)
+        b.set_panels(whole)
+        w, heads = np.array([1.0, 2.0]), [0.0, 30.0]
+        plain = b.solve(w, heads)
+        a.set_tuning(a.TUNE_NN_RMIN, 100)  # R/h >= 1: no pair of this body (R/h <= 0.67) to the network
+        fb = a.solve(w, heads, surrogate=True)
+        a.set_tuning(a.TUNE_NN_RMIN, 10)
+        got = a.solve(w, heads, surrogate=True)
+        ref = b.solve(w, heads, surrogate=True)
+        for key in ("added_mass", "radiation_damping", "excitation_diffraction"):
+            def rel(x, y):
+                return np.abs(x[key] - y[key]).max() / np.abs(y[key]).max()
+            e0, e, d, ds = rel(fb, plain), rel(got, ref), rel(ref, plain), rel(got, plain)
+            say(f"  {key}: fallback only vs evaluator {e0:.2e}; with the network sector vs whole {e:.2e}, "
+                f"whole vs evaluator {d:.2e}, sector vs evaluator {ds:.2e}")
+            assert e0 < 1e-9
+            assert e < 0.5 * min(d, ds) and d > 1e-5 and ds > 1e-5
+    finally:
+        a.close()
+        b.close()

@@ -141,11 +141,10 @@ def test_unsupported_uses_with_symmetry_are_refused(sessions):
     s.set_environment(1025.0, 9.81, float("inf"))
     s.set_symmetry(True, False)
     s.set_panels(meshes.hemisphere_panels(1.0, 3, 3, quarter=True))
-    say("case: symmetry planes with the statics, the assembly, the surrogate")
+    say("case: symmetry planes with the statics, the assembly")
     try:
         for call in (lambda: s.statics(),
-                     lambda: s.assemble(1.0),
-                     lambda: s.solve([1.0], [], surrogate=True)):
+                     lambda: s.assemble(1.0)):
             with pytest.raises(hb.HbgError) as e:
                 call()
             assert e.value.status == 3
