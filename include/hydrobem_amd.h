@@ -128,7 +128,8 @@ HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_pane
  * on the axis of a wall is HBG_ERR_VALUE. The frequency limits omega = 0 and infinity are solved the
  * same way (the limit matrices' image blocks combined into the characters; at omega = 0 in finite
  * depth the net-flux rule takes the whole body); hbg_solve_group takes such sessions too. With
- * G > 1, hbg_statics, hbg_assemble and the surrogate are HBG_ERR_UNSUPPORTED. */
+ * G > 1, hbg_statics and hbg_assemble are HBG_ERR_UNSUPPORTED (the surrogate takes the image
+ * blocks as the evaluator does). */
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
 
 /* Walls (the domain x_wall, y_wall, orthogonal_walls of model_codes.rs:15-31): wall_x 1 a
@@ -142,7 +143,7 @@ HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y);
  * group of both is used (lo_geometry.rs:863-945); a wall on the axis of a symmetry plane is
  * HBG_ERR_VALUE. The frequency limits omega = 0 and infinity are solved with the walls too (at
  * omega = 0 in finite depth the net-flux rule takes the real bodies); hbg_solve_group takes such
- * sessions too. With walls hbg_statics, hbg_assemble and the surrogate are HBG_ERR_UNSUPPORTED. */
+ * sessions too. With walls hbg_statics and hbg_assemble are HBG_ERR_UNSUPPORTED. */
 HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y);
 
 /* Field points (lo_frequency.rs:1392-1495, `Output` frad/finc/fsca/ftot, low_order.rs:1490-1500):
@@ -216,8 +217,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
  * past N (its LU is block-diagonal, its solution unchanged) and the GEMM tiles of that identity
  * block skipped. A session with symmetry planes or walls brings its character systems (G or one
  * per frequency, of the sector's order), kept in one batch, and its results are those of its own
- * hbg_solve; ordinary frequencies only, no interior free surface, no surrogate with symmetry
- * planes or walls (else HBG_ERR_UNSUPPORTED); HBG_PRECISION_MIXED32 runs as HBG_PRECISION_MIXED.
+ * hbg_solve; ordinary frequencies only, no interior free surface (else HBG_ERR_UNSUPPORTED);
+ * HBG_PRECISION_MIXED32 runs as HBG_PRECISION_MIXED.
  * Outputs: arrays of count pointers (arrays or pointers may be NULL), entry i laid out as
  * hbg_solve's for session i. All work runs on the first session's stream; no session may be in
  * use elsewhere during the call. */
