@@ -197,10 +197,12 @@ def bench_group(a):
 
 def cpu_baseline(w):
     """The FP64 oracle (C++/OpenMP restatement of the reference's algorithm) on this host's cores, on a
-    bounded sample: the workload's body meshed with 1024 panels (same depth, the middle frequency of
-    the sweep), statics, assembly and LU solve timed separately and scaled to the workload's panel
-    count (statics and assembly by N^2, the LU by N^3); frequencies/s of the whole sweep with the
-    statics amortized over it."""
+    bounded sample: the workload's body meshed with 512 and with 1024 panels (same depth, the middle
+    frequency of the sweep), statics, assembly and LU solve timed separately; the 1024-panel times
+    scaled to the workload's panel count (statics and assembly by N^2, the LU by N^3, the two sample
+    sizes showing the N^2 of the assembly); frequencies/s of the whole sweep with the statics
+    amortized over it. Beside it, measured at the workload's full order: the complex64 LU solve of
+    the reference's production path as this host's LAPACK (numpy's) runs it on the same cores."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from hydrobem_amd import meshes
     from oracle import oracle as orc
@@ -208,30 +210,51 @@ def cpu_baseline(w):
     orc.lib()
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     N = w["vertices"].shape[0]
-    vs = w.get("sample_mesh") or meshes.cylinder_panels(1.0, 1.0, 32, 16, 16, quarter=False, cosine=True)
-    Ns = vs.shape[0]
     depth = w.get("depth", float("inf"))
-    P = orc.Problem(vs, rho=w["rho"], grav=w["g"], depth=depth)
     mid = float(w["omegas"][len(w["omegas"]) // 2])
-    t0 = time.perf_counter()
-    P.statics()
-    t1 = time.perf_counter()
-    S, A = P.assemble(mid)
-    t2 = time.perf_counter()
-    rhs = S @ np.ones((Ns, 6 + len(w["headings"])), complex)
-    orc.lu_solve(A, rhs)
-    t3 = time.perf_counter()
+    R = 6 + len(w["headings"])
+    times = {}
+    for nz in (8, 16):
+        vs = meshes.cylinder_panels(1.0, 1.0, 32, nz, nz, quarter=False, cosine=True)
+        Ns = vs.shape[0]
+        P = orc.Problem(vs, rho=w["rho"], grav=w["g"], depth=depth)
+        t0 = time.perf_counter()
+        P.statics()
+        t1 = time.perf_counter()
+        S, A = P.assemble(mid)
+        t2 = time.perf_counter()
+        orc.lu_solve(A, S @ np.ones((Ns, R), complex))
+        t3 = time.perf_counter()
+        times[Ns] = (t1 - t0, t2 - t1, t3 - t2)
+        print(f"[bench] cpu baseline: oracle at {Ns} panels: statics {t1 - t0:.2f} s, assembly {t2 - t1:.2f} s, "
+              f"LU {t3 - t2:.2f} s", file=sys.stderr, flush=True)
+    (Na, ta), (Ns, (st, asm, lu)) = sorted(times.items())[0], sorted(times.items())[1]
+    growth = asm / ta[1]
     s2, s3 = (N / Ns) ** 2, (N / Ns) ** 3
-    t_st, t_asm, t_lu = (t1 - t0) * s2, (t2 - t1) * s2, (t3 - t2) * s3
+    t_st, t_asm, t_lu = st * s2, asm * s2, lu * s3
     F = len(w["omegas"])
+    # the production path's LU at the full order (complex64 LAPACK on the same cores)
+    rng = np.random.default_rng(0)
+    M = (rng.standard_normal((N, N), dtype=np.float32) + 1j * rng.standard_normal((N, N), dtype=np.float32))
+    M = M.astype(np.complex64) / np.float32(np.sqrt(N))
+    M[np.diag_indices(N)] += np.complex64(2.0 * np.pi)
+    B = np.ones((N, R), np.complex64)
+    t4 = time.perf_counter()
+    np.linalg.solve(M, B)
+    t_lu64 = time.perf_counter() - t4
+    del M
+    print(f"[bench] cpu baseline: complex64 LAPACK LU solve at {N}: {t_lu64:.2f} s", file=sys.stderr, flush=True)
     return {"value": round(F / (t_st + F * (t_asm + t_lu)), 6), "unit": "freq solves/s", "cores": cores,
             "kind": "port",
-            "sample": f"FP64 oracle on {cores} OpenMP threads, the body meshed with {Ns} panels (depth {depth}, "
-                      f"omega {mid:.3f}): statics {t1 - t0:.2f} s, assembly {t2 - t1:.2f} s, LU solve "
-                      f"{t3 - t2:.2f} s, scaled to {N} panels by N^2 (statics, assembly) and N^3 (LU): "
-                      f"{t_st:.0f} s statics + {t_asm + t_lu:.0f} s per frequency; the reference publishes "
+            "sample": f"FP64 oracle on {cores} OpenMP threads, the body meshed with {Na} and {Ns} panels (depth "
+                      f"{depth}, omega {mid:.3f}): assembly {ta[1]:.2f} s and {asm:.2f} s ({growth:.2f}x for "
+                      f"{Ns / Na:.0f}x the panels), at {Ns}: statics {st:.2f} s, LU solve {lu:.2f} s; scaled to "
+                      f"{N} panels by N^2 (statics, assembly) and N^3 (LU): {t_st:.0f} s statics + "
+                      f"{t_asm + t_lu:.0f} s per frequency. Measured at {N}: the production path's complex64 LU "
+                      f"solve by this host's LAPACK on the same cores, {t_lu64:.2f} s. The reference publishes "
                       f"16.4 s (1 thread) / 4.2 s (6 threads) per frequency at 8192 panels for its complex64 "
-                      f"production path (docs/theory/13-linear-algebra-and-threads.md:141-144)"}
+                      f"production path (docs/theory/13-linear-algebra-and-threads.md:141-144)",
+            "assembly_growth_2x_panels": round(growth, 3), "lu_complex64_full_order_s": round(t_lu64, 3)}
 
 
 def motion_matrices(w):
