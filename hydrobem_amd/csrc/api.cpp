@@ -833,10 +833,12 @@ double assemble_frequency32(HbgSession* s, double K, double logc, int fi, float2
   const int N = s->N;
   TablesOnDevice& t = tables32_for(s->device, s->stream);
   if (!std::isfinite(s->depth)) {
-    launch_assemble32(s->d_panels32.p, N, K, logc, t.t2d32, t.t1d32, s->d_st32.p, s->d_S32.p, A32, A32r, N, 2.0 * PI,
-                      s->stream);
+    launch_assemble32(s->d_panels32.p, N, K, logc, t.t2d32, t.t1d32, s->d_st32.p, s->d_S32.p, A32, nullptr, N,
+                      2.0 * PI, s->stream);
     launch_near64(s->d_panels.p, N, s->d_near_ent.p, s->near_nnz, false, K, nullptr, FinTables{}, s->T, logc,
-                  s->d_st.p, s->d_S32.p, A32, A32r, N, 2.0 * PI, s->d_dS.p, dA, s->stream);
+                  s->d_st.p, s->d_S32.p, A32, nullptr, N, 2.0 * PI, s->d_dS.p, dA, s->stream);
+    // the refinement's copy of A (the LU factors A in place), in one coalesced pass
+    HBG_HIP(hipMemcpyAsync(A32r, A32, (size_t)N * N * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
     return K;
   }
   const FinFreq f = make_fin_freq(s->depth, K);
@@ -855,9 +857,11 @@ double assemble_frequency32(HbgSession* s, double K, double logc, int fi, float2
                       s->d_pf32.p, s->d_pcs32.p, s->stream);
   launch_assemble_fin32(s->d_panels32.p, N, s->d_ffs32.p + (size_t)fi * finfreq32_bytes(), s->d_phi1_32.p,
                         s->d_phi2_32.p, t.ktab32, s->d_pf32.p, s->d_pcs32.p, t.t2d32, t.t1d32, logc, s->d_st32.p,
-                        s->d_S32.p, A32, A32r, N, 2.0 * PI, s->stream);
+                        s->d_S32.p, A32, nullptr, N, 2.0 * PI, s->stream);
   launch_near64(s->d_panels.p, N, s->d_near_ent.p, s->near_nnz, true, K, s->d_ffs.p + fi, fin_tables(s), s->T, logc,
-                s->d_st.p, s->d_S32.p, A32, A32r, N, 2.0 * PI, s->d_dS.p, dA, s->stream);
+                s->d_st.p, s->d_S32.p, A32, nullptr, N, 2.0 * PI, s->d_dS.p, dA, s->stream);
+  // the refinement's copy of A (the LU factors A in place), in one coalesced pass
+  HBG_HIP(hipMemcpyAsync(A32r, A32, (size_t)N * N * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
   return f.k0;
 }
 

@@ -435,8 +435,9 @@ __device__ __forceinline__ FinK32 fin_kernel32_u(const FinFreq32* __restrict__ f
 }
 
 // one entry (row = field point, col = source panel) of S and A in complex64
-// (A2: a second copy of A, ld lds: the one the refinement's residuals read after the LU has
-// factored A in place)
+// (A2, nullable: a second copy of A, ld lds: the one the refinement's residuals read after the LU
+// has factored A in place; the solve copies A there after the assembly in one coalesced pass,
+// which costs less than a third stream of scattered writes here)
 __device__ __forceinline__ void put_entry32(float2* __restrict__ S, float2* __restrict__ A, float2* __restrict__ A2,
                                             int lds, int lda, int row, int col, float4 e, float Ar, float nz, float a,
                                             float c2, float Wr, float WrR, float WrZ, float Wi, float WiR, float WiZ,
@@ -445,7 +446,7 @@ __device__ __forceinline__ void put_entry32(float2* __restrict__ S, float2* __re
   const float dre = e.y + Ar * (-WrR * a + WrZ * nz) - c2 * e.w, dim = Ar * (-WiR * a + WiZ * nz);
   const float2 v = make_float2((row == col ? free_term : 0.0f) - dre, -dim);
   A[(size_t)col * lda + row] = v;
-  A2[(size_t)col * lds + row] = v;
+  if (A2) A2[(size_t)col * lds + row] = v;
 }
 
 // Infinite depth, FP32: every unordered pair once (64 x 64 tiles above the diagonal), the pairs
@@ -595,7 +596,7 @@ __global__ void __launch_bounds__(256) near64_kernel(const PanelD* __restrict__ 
   const float2 s32 = make_float2((float)sv.x, (float)sv.y), a32 = make_float2((float)av.x, (float)av.y);
   S[(size_t)n * N + k] = s32;
   A[(size_t)n * lda + k] = a32;
-  A2[(size_t)n * N + k] = a32;
+  if (A2) A2[(size_t)n * N + k] = a32;  // nullable: the solve copies A whole after the assembly
   dS[e] = make_double2(sv.x - (double)s32.x, sv.y - (double)s32.y);
   dA[e] = make_double2(av.x - (double)a32.x, av.y - (double)a32.y);
 }
