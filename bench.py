@@ -54,8 +54,9 @@ TRAFFIC = {("cylinder8192fd", "mixed", "64x64 split-fp16", 8, (0, 0), 16): 8.496
            ("cylinder8192fd", "mixed32", "128x128 split-fp16 direct-LDS", 32, (16, 8), 48): 1.320e10}
 # HBM bytes per launch (one frequency) of the assembly kernel, from the same calibrated runs:
 # (workload, path) -> bytes (profiles/assembly_hbm_traffic_cylinder8192_finite_depth.md)
+# (mixed32: the assembly kernel and the copy of A after it, profiles/assemble_fin32_counters_*.md)
 ASM_TRAFFIC = {("cylinder8192fd", "fp64"): 8.97e9, ("cylinder8192fd", "mixed"): 8.97e9,
-               ("cylinder8192fd", "mixed32"): 5.28e9}
+               ("cylinder8192fd", "mixed32"): 5.08e9}
 PATH_DESC = {"fp64": "plain FP64: complex128 assembly + LU (MFMA FP64), partial-pivoting check with tournament fallback",
              "mixed": "mixed: FP64 assembly, complex64 LU on the matrix cores (FP32 MFMA, or FP16 MFMA with FP32 "
                       "operands split into hi + lo, autotuned), FP64 iterative refinement to backward error 1e-12",
@@ -539,7 +540,8 @@ def main():
         # the other large kernel of the step: the assembly (one launch per frequency), bound by
         # vector issue (FP64, or FP32 for mixed32), not HBM; its algorithmic bytes are N^2 (32 + 2 x 16)
         # (statics read, S and A written), for mixed32 N^2 (16 + 3 x 8) (FP32 statics read, S, A and
-        # A's unfactored copy written), against the stage time (boundary values, tables, matrices)
+        # A's unfactored copy written, the copy by a copy after the kernel), against the stage time
+        # (boundary values, tables, matrices, the copy)
         "roofline_assembly": {"bound": "hbm", "kernel": ("assemble_fin32_kernel" if main_path == "mixed32" else
                                                          "assemble_fin_kernel") if np.isfinite(w.get("depth", np.inf))
                               else ("assemble32_kernel" if main_path == "mixed32" else "assemble_kernel"),
