@@ -133,6 +133,8 @@ size_t residual_max_elems(int n);
 size_t residual_part_elems(int n, int R);
 void launch_limit_fin_matrices(const PanelD* P, int N, int Nsrc, double h, bool zero, const double* ktab,
                                const StaticD* st, const double2* inf, double2* S, double2* A, hipStream_t stream);
+void launch_interior_identity(const PanelD* P, int N, int nchar, double2* S, double2* M, size_t mstride,
+                              hipStream_t stream);
 void launch_limit_points(double h, bool zero, const double* ktab, int n, const double* R, const double* z,
                          const double* zeta, double* out, hipStream_t stream);
 void launch_assemble_fin_nn(const PanelD* P, int N, const FinFreq* f, FinTables F, WaveTables T, double logc,
@@ -1764,6 +1766,7 @@ void solve_limit(HbgSession* s, bool at_infinity, double* A) {
   if (G > 1)
     launch_symmetry_combine(s->d_Sg.p, s->d_Ag.p, N, G, s->d_chi.p, Gs, s->d_S.p, s->d_M.p, mstride, N, nullptr,
                             s->stream);
+  if (s->NI > 0) launch_interior_identity(s->d_panels.p, N, Gs, s->d_S.p, s->d_M.p, mstride, s->stream);
   launch_boundary(s->d_panels.p, s->d_body.p, s->d_nj.p, NF, nrad, 0, 1.0, 1.0, s->grav, 0.0, s->d_heads.p,
                   s->d_phiI.p, s->d_dphiI.p, s->d_u.p, s->stream, s->walls);
   if (Gs == 1) {
@@ -1852,13 +1855,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       want_inf = want_inf || std::isinf(omegas[i]);
     }
   }
-  if (s->NI > 0) {
-    // the interior free surface: ordinary frequencies, the forces
-    if (!il.empty()) throw Error{HBG_ERR_UNSUPPORTED, "the frequency limits with an interior free surface"};
-    if (s->np > 0 || s->nk > 0)
-      throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions with an interior free surface"};
-    if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
-  }
+  if (s->NI > 0 && o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
   if ((s->nk > 0 || s->np > 0) && !il.empty())
     throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions at the frequency limits"};
   if (s->np > 0 && std::isfinite(s->depth))

@@ -519,6 +519,10 @@ __global__ void __launch_bounds__(256) field_influence_kernel(const PanelD* __re
   const bool valid = k < Np;
   const PanelD* pk = Pf + (valid ? k : 0);
   const PanelD* pn = P + j;
+  if (pn->interior != 0.0) {  // wavefront-uniform: the interior free surface is not in the sums
+    if (valid) Sx[(size_t)j * Np + k] = Ax[(size_t)j * Np + k] = make_double2(0.0, 0.0);
+    return;
+  }
   const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
   const double R = sqrt(dx * dx + dy * dy);
   const double X = K * R, Y = -K * (pk->c[2] + pn->c[2]);
@@ -549,6 +553,10 @@ __global__ void __launch_bounds__(256) field_influence_fin_kernel(const PanelD* 
   const int kk = valid ? k : 0;
   const PanelD* pk = Pf + kk;
   const PanelD* pn = P + j;
+  if (pn->interior != 0.0) {  // wavefront-uniform: the interior free surface is not in the sums
+    if (valid) Sx[(size_t)j * Np + k] = Ax[(size_t)j * Np + k] = make_double2(0.0, 0.0);
+    return;
+  }
   const double dx = pk->c[0] - pn->c[0], dy = pk->c[1] - pn->c[1];
   const double R = sqrt(dx * dx + dy * dy);
   const FinK w = fin_kernel_u(f, F, T, valid, N + kk, j, R, pk->c[2], pn->c[2]);
@@ -559,7 +567,8 @@ __global__ void __launch_bounds__(256) field_influence_fin_kernel(const PanelD* 
 }
 
 // The field values of one frequency at every point k (lo_frequency.rs:1461-1491): radiation column
-// c: frad = (w^2 / g) (1 / 4 pi) sum_j (phi_j D_x - S_x u_j) with u the body's n_j; heading h
+// c: frad = (w^2 / g) (1 / 4 pi) sum_j (phi_j D_x - S_x u_j) with u the body's n_j, over the body
+// panels (an interior free surface's panels have S_x = D_x = 0, lo_frequency.rs:1443-1458); heading h
 // (the scattered formulation): phi_S = (1 / 4 pi) sum_j (phi_S + phi_I)_j D_x, and the elevations
 // eta = -(i w / g) phi of the incident wave at the point, the scattered wave and their sum.
 // sol: the nsol x R solution columns (leading dimension nsol); phiI: [H][nsol] on the panels; N
@@ -669,6 +678,7 @@ __global__ void __launch_bounds__(256) kochin_kernel(const PanelD* __restrict__ 
   double hr = 0.0, hi = 0.0, tr = 0.0, ti = 0.0;
   for (int j = 0; j < N; ++j) {
     const PanelD* p = P + j;
+    if (p->interior != 0.0) continue;  // the body panels only (lo_frequency.rs:1311-1313)
     const double x = p->c[0], y = p->c[1], z = p->c[2];
     double Z = exp(k * z), Zz = k * Z;
     if (depth > 0.0) {
@@ -883,6 +893,28 @@ void launch_limit_fin_matrices(const PanelD* P, int N, int Nsrc, double h, bool 
                                const StaticD* st, const double2* inf, double2* S, double2* A, hipStream_t stream) {
   limit_fin_matrices_kernel<<<(unsigned)(((size_t)N * Nsrc + 255) / 256), 256, 0, stream>>>(
       P, N, Nsrc, h, zero ? 1 : 0, ktab, st, inf, S, A);
+  HBG_HIP(hipGetLastError());
+}
+
+// The frequency limits with an interior free surface (lo_frequency.rs:99-104): its unknowns are
+// not used there, their rows and columns the identity (S zero), so the body panels' system is
+// solved as without the surface. The nchar systems: S_c at S + c N N, A_c at M + c mstride (both
+// leading dimension N).
+__global__ void interior_identity_kernel(const PanelD* __restrict__ P, int N, int nchar, double2* __restrict__ S,
+                                         double2* __restrict__ M, size_t mstride) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N) return;
+  const int k = (int)(e % N), n = (int)(e / N);
+  if (P[k].interior == 0.0 && P[n].interior == 0.0) return;
+  for (int c = 0; c < nchar; ++c) {
+    S[(size_t)c * N * N + e] = make_double2(0.0, 0.0);
+    M[(size_t)c * mstride + e] = make_double2(k == n ? 1.0 : 0.0, 0.0);
+  }
+}
+
+void launch_interior_identity(const PanelD* P, int N, int nchar, double2* S, double2* M, size_t mstride,
+                              hipStream_t stream) {
+  interior_identity_kernel<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, stream>>>(P, N, nchar, S, M, mstride);
   HBG_HIP(hipGetLastError());
 }
 

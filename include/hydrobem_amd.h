@@ -111,9 +111,10 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
  * (z within 1e-9 of 0, set to 0; normals taken upward), body [n_panels] (NULL: body 0); they follow
  * the body panels as extra unknowns: free term -4 pi, columns K S (lo_frequency.rs:85-97,
  * low_order.rs:771-785), no velocity, no forces. n_panels 0 removes them; hbg_set_panels removes
- * them too. Ordinary frequencies only (the solve refuses the frequency limits, field points and
- * Kochin functions, the surrogate, walls and group solves with them with HBG_ERR_UNSUPPORTED;
- * HBG_PRECISION_MIXED32 solves them as HBG_PRECISION_MIXED).
+ * them too. Field points and Kochin functions leave them out of their sums (lo_frequency.rs:
+ * 1311-1313, 1443-1458); at the frequency limits their unknowns are not used (identity rows,
+ * lo_frequency.rs:99-104). The surrogate, walls and group solves with them are
+ * HBG_ERR_UNSUPPORTED; HBG_PRECISION_MIXED32 solves them as HBG_PRECISION_MIXED.
  * Invalidates the statics. */
 HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body);
 

@@ -173,3 +173,84 @@ def test_interior_removes_the_irregular_frequency_in_finite_depth():
     dev = lambda a: abs(a[1] - 0.5 * (a[0] + a[2])) / np.abs(a).max()
     say(f"  heave added mass without {plain}, with {removed}: departures {dev(plain):.2e}, {dev(removed):.2e}")
     assert dev(removed) < 0.02 and dev(plain) > 5.0 * dev(removed)
+
+
+@pytest.mark.parametrize("depth", [float("inf"), 3.0])
+def test_limits_with_the_interior_surface(depth):
+    """At omega = 0 and infinity the interior surface's unknowns are not used (identity rows,
+    lo_frequency.rs:99-104): the added mass is the body's alone, as without the surface; the sector
+    with both planes gives the same. An ordinary frequency between them differs (the surface
+    takes part)."""
+    q = quadrant_cylinder()
+    qi = meshes.interior_graded(q, True, True)
+    body = meshes.expand_symmetry(q)
+    inner = meshes.interior_graded(body)
+    w = np.array([0.0, 2.0, np.inf])
+    say(f"case: interior cylinder {len(body)} + {len(inner)} panels, depth {depth}, limits with and without the "
+        f"surface, and as the quadrant")
+    with hb.Session(0) as a, hb.Session(0) as b:
+        for s in (a, b):
+            s.set_environment(RHO, G, depth)
+        a.set_panels(body)
+        plain = a.solve(w, [])["added_mass"]
+        a.set_interior(inner)
+        with_surface = a.solve(w, [])["added_mass"]
+        b.set_symmetry(True, True)
+        b.set_panels(q)
+        b.set_interior(qi)
+        sector = b.solve(w, [])["added_mass"]
+    for i, label in ((0, "omega 0"), (2, "omega infinity")):
+        e = np.nanmax(np.abs(with_surface[i] - plain[i])) / np.nanmax(np.abs(plain[i]))
+        es = np.nanmax(np.abs(sector[i] - plain[i])) / np.nanmax(np.abs(plain[i]))
+        say(f"  {label}: with the surface vs without {e:.2e}, quadrant vs without {es:.2e}")
+        assert np.array_equal(np.isnan(with_surface[i]), np.isnan(plain[i]))
+        assert e < 1e-12 and es < 1e-9
+    d = np.abs(with_surface[1] - plain[1]).max() / np.abs(plain[1]).max()
+    say(f"  omega 2: with the surface vs without {d:.2e}")
+    assert d > 1e-6
+
+
+@pytest.mark.parametrize("depth", [float("inf"), 3.0])
+def test_kochin_and_field_points_with_the_interior_surface(depth):
+    """Kochin functions and field values with the interior surface, whose panels are left out of
+    their sums (lo_frequency.rs:1311-1313, 1443-1458): the Haskind excitation equals rho g H(beta +
+    pi) of the radiation Kochin functions to rounding (the same body panels and values), and the
+    field values on a far circle are C H(theta) with the one factor of the far field."""
+    body = meshes.cylinder_panels(1.0, 0.5, n_phi=32, n_z=8, n_r=8, quarter=False, cosine=True)
+    inner = meshes.interior_graded(body)
+    om = 2.0
+    k = om * om / G if not np.isfinite(depth) else None
+    NK = 72
+    heads = [0.0, 45.0]
+    say(f"case: cylinder {len(body)} + {len(inner)} interior panels, depth {depth}: Kochin functions and far-field "
+        f"points")
+    with hb.Session(0) as s:
+        s.set_environment(RHO, G, depth)
+        s.set_panels(body)
+        s.set_interior(inner)
+        s.set_kochin(NK)
+        if k is not None:
+            r = 200.0 / k
+            th = 2.0 * np.pi * np.arange(NK) / NK
+            s.set_field_points(np.stack([r * np.cos(th), r * np.sin(th), np.zeros(NK)], axis=1))
+        out = s.solve([om], heads)
+        kv = s.kochin_values(1, len(heads))
+        t = [int(round((h + 180.0) / 5.0)) % NK for h in heads]
+        hk = RHO * G * kv["kr"][:, t, :]
+        ref = out["excitation_haskind"]
+        e = np.abs(hk - ref).max() / np.abs(ref).max()
+        say(f"  |rho g H(beta + pi) - Haskind excitation| / peak = {e:.2e}")
+        assert e < 1e-12
+        if k is not None:
+            fv = s.field_values(1, len(heads))
+            phi = np.concatenate([fv["frad"][0] / k, fv["fsca"][0] / (-1j * om / G)], axis=1)
+            H = np.concatenate([kv["kr"][0], kv["kd"][0]], axis=1)
+            big = np.abs(H) > 0.05 * np.abs(H).max(axis=0, keepdims=True)
+            ratio = phi[big] / H[big]
+            C = np.median(ratio.real) + 1j * np.median(ratio.imag)
+            spread = np.abs(ratio / C - 1.0).max()
+            c_theory = 0.5 * k * np.sqrt(2.0 / (np.pi * k * r))
+            say(f"  far circle k r = 200: phi / H = C with spread {spread:.2e}, |C| / theory {abs(C) / c_theory:.6f}")
+            assert spread < 2e-2 and abs(abs(C) / c_theory - 1.0) < 1e-2
+        s.set_kochin(0)
+        s.set_field_points(None)
