@@ -2301,8 +2301,6 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   for (int i = 0; i < count; ++i) {
     if (!sessions[i]) return fail(HBG_ERR_ARGUMENT, "null session", "hbg_solve_group");
     if (sessions[i]->N == 0) return fail(HBG_ERR_STATE, "a session has no panels", "hbg_solve_group");
-    if (sessions[i]->NI > 0)
-      return fail(HBG_ERR_UNSUPPORTED, "a session with an interior free surface in a group", "hbg_solve_group");
     if (sessions[i]->device != sessions[0]->device)
       return fail(HBG_ERR_VALUE, "the sessions of a group must share their device", "hbg_solve_group");
     for (int j = 0; j < i; ++j)
@@ -2325,6 +2323,9 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   GUARD_BEGIN
   check_options(options);
   HbgOptions o = defaults(options);
+  for (int i = 0; i < count; ++i)
+    if (o.surrogate && sessions[i]->NI > 0)
+      throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
   use_device(lead);
   const int H = n_headings;
   int nmax = 0, Rmax = 0;

@@ -113,8 +113,8 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
  * low_order.rs:771-785), no velocity, no forces. n_panels 0 removes them; hbg_set_panels removes
  * them too. Field points and Kochin functions leave them out of their sums (lo_frequency.rs:
  * 1311-1313, 1443-1458); at the frequency limits their unknowns are not used (identity rows,
- * lo_frequency.rs:99-104). The surrogate, walls and group solves with them are
- * HBG_ERR_UNSUPPORTED; HBG_PRECISION_MIXED32 solves them as HBG_PRECISION_MIXED.
+ * lo_frequency.rs:99-104); hbg_solve_group takes such sessions too. The surrogate and walls with
+ * them are HBG_ERR_UNSUPPORTED; HBG_PRECISION_MIXED32 solves them as HBG_PRECISION_MIXED.
  * Invalidates the statics. */
 HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body);
 
@@ -218,8 +218,8 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
  * past N (its LU is block-diagonal, its solution unchanged) and the GEMM tiles of that identity
  * block skipped. A session with symmetry planes or walls brings its character systems (G or one
  * per frequency, of the sector's order), kept in one batch, and its results are those of its own
- * hbg_solve; ordinary frequencies only, no interior free surface (else HBG_ERR_UNSUPPORTED);
- * HBG_PRECISION_MIXED32 runs as HBG_PRECISION_MIXED.
+ * hbg_solve, with an interior free surface too; ordinary frequencies only (else
+ * HBG_ERR_UNSUPPORTED); HBG_PRECISION_MIXED32 runs as HBG_PRECISION_MIXED.
  * Outputs: arrays of count pointers (arrays or pointers may be NULL), entry i laid out as
  * hbg_solve's for session i. All work runs on the first session's stream; no session may be in
  * use elsewhere during the call. */

@@ -131,3 +131,35 @@ def test_group_with_symmetry_planes_and_walls(precision):
         s.set_walls(False, False)
         s.set_symmetry(False, False)
         s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_with_an_interior_free_surface(precision):
+    """A session with an interior free surface (extra unknowns, free term -4 pi, K S columns) in a
+    group beside plain geometries: each gives the results of its own solve, the frequency
+    K = 2.88 (the cylinder's first irregular frequency) among them."""
+    body = meshes.cylinder_panels(1.0, 0.5, n_phi=16, n_z=4, n_r=4, quarter=False, cosine=True)
+    inner = meshes.interior_graded(body)
+    sessions = []
+    for v, interior in ((body, inner), (body, None), (meshes.hemisphere_panels(1.0, 8, 16, quarter=False), None)):
+        s = hb.Session(0)
+        s.set_environment(1025.0, 9.81)
+        s.set_panels(v)
+        if interior is not None:
+            s.set_interior(interior)
+        sessions.append(s)
+    say(f"case: group with an interior surface ({len(body)} + {len(inner)} panels) beside two plain geometries, "
+        f"{precision}")
+    w = np.sqrt(9.81 * np.array([1.0, 2.88]))
+    heads = [0.0, 30.0]
+    group = hb.solve_group(sessions, w, heads, precision=precision)
+    tol = 1e-11 if precision == "fp64" else 1e-9
+    for s, g in zip(sessions, group):
+        ref = s.solve(w, heads, precision=precision)
+        for key in ("added_mass", "radiation_damping", "excitation_diffraction", "excitation_haskind"):
+            e = rel(g[key], ref[key])
+            say(f"  N {s.n_panels} {key}: {e:.2e}")
+            assert e < tol
+    assert rel(group[0]["added_mass"], group[1]["added_mass"]) > 1e-4  # the surface took part
+    for s in sessions:
+        s.close()
