@@ -1158,7 +1158,6 @@ HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_pane
   if (!s || (n_panels > 0 && !vertices)) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_set_interior");
   if (n_panels < 0) return fail(HBG_ERR_ARGUMENT, "negative count", "hbg_set_interior");
   if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_set_interior");
-  if (s->walls) return fail(HBG_ERR_UNSUPPORTED, "the interior free surface with walls", "hbg_set_interior");
   GUARD_BEGIN
   // the body panels as set, then the new interior panels (z = 0 in body-local coordinates)
   RawPanels raw = raw_of(s);
@@ -1303,8 +1302,6 @@ HbgStatus hbg_set_walls(HbgSession* s, int32_t wall_x, int32_t wall_y) {
     return fail(HBG_ERR_VALUE, "wall_x and wall_y must be 0 or 1", "hbg_set_walls");
   if ((wall_x && s->sym_x) || (wall_y && s->sym_y))
     return fail(HBG_ERR_VALUE, "a wall where a symmetry plane stands", "hbg_set_walls");
-  if ((wall_x || wall_y) && s->NI > 0)
-    return fail(HBG_ERR_UNSUPPORTED, "walls with an interior free surface", "hbg_set_walls");
   GUARD_BEGIN
   // panels behind a new wall leave the walls and the geometry as they were
   const int walls = wall_x | wall_y << 1;

@@ -113,8 +113,9 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
  * low_order.rs:771-785), no velocity, no forces. n_panels 0 removes them; hbg_set_panels removes
  * them too. Field points and Kochin functions leave them out of their sums (lo_frequency.rs:
  * 1311-1313, 1443-1458); at the frequency limits their unknowns are not used (identity rows,
- * lo_frequency.rs:99-104); hbg_solve_group takes such sessions too. The surrogate and walls with
- * them are HBG_ERR_UNSUPPORTED; HBG_PRECISION_MIXED32 solves them as HBG_PRECISION_MIXED.
+ * lo_frequency.rs:99-104); walls (their images mirror the surface) and hbg_solve_group take them
+ * too. The surrogate with them is HBG_ERR_UNSUPPORTED; HBG_PRECISION_MIXED32 solves them as
+ * HBG_PRECISION_MIXED.
  * Invalidates the statics. */
 HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body);
 

@@ -321,3 +321,45 @@ def test_walls_with_a_symmetry_plane_against_the_whole_body(session, wx, depth, 
         say(f"  {key}: half with the symmetry plane vs whole body {e:.2e}")
         # two mixed-precision solves each stop at a backward error of 1e-12
         assert e < (1e-9 if precision == "fp64" else 1e-8)
+
+
+@pytest.mark.parametrize("depth,wx,wy", [(float("inf"), 1, 0), (3.0, 1, 1)])
+def test_walls_with_an_interior_free_surface(session, depth, wx, wy):
+    """A body with its interior free surface by the walls against the open sea with the mirror
+    bodies, each with its own interior surface (the image of the real one): A, B, X at an ordinary
+    frequency and at the body's first irregular frequency, and the limits."""
+    body = meshes.place(meshes.cylinder_panels(1.0, 0.5, n_phi=16, n_z=4, n_r=4, quarter=False, cosine=True),
+                        (2.0, 1.5, 0.0))
+    inner = meshes.place(meshes.interior_graded(meshes.cylinder_panels(1.0, 0.5, n_phi=16, n_z=4, n_r=4,
+                                                                       quarter=False, cosine=True)), (2.0, 1.5, 0.0))
+    centre = np.array([2.0, 1.5, -0.1])
+    refl, verts, bodies, centres = open_sea_equivalent(body, centre, wx, wy)
+    inners = np.concatenate([reflect(inner, sx, sy) for sx, sy in refl])
+    ibody = np.repeat(np.arange(len(refl), dtype=np.int32), len(inner))
+    heads = [0.0, 40.0]
+    w = np.sqrt(9.81 * np.array([1.0, 2.88]))
+    say(f"case: cylinder {len(body)} + {len(inner)} interior panels by walls x {wx} y {wy}, depth {depth}; open sea "
+        f"with {len(refl)} bodies and their interior surfaces")
+    s = session
+    s.set_environment(1025.0, 9.81, depth)
+    s.set_walls(False, False)
+    s.set_panels(verts, body=bodies, rotation_centres=centres)
+    s.set_interior(inners, ibody)
+    o = s.solve(w, headings_reflected(heads, refl))
+    ref = wall_from_open(o, refl, len(heads))
+    lim_o = wall_from_open(s.solve([0.0, np.inf], []), refl, 0)["added_mass"]
+    s.set_panels(body, rotation_centres=centre[None])
+    s.set_walls(wx, wy)
+    s.set_interior(inner)
+    got = s.solve(w, heads)
+    lim = s.solve([0.0, np.inf], [])["added_mass"]
+    s.set_interior(None)
+    s.set_walls(False, False)
+    for key, r in ref.items():
+        e = rel_per_entry(got[key], r)
+        say(f"  {key}: walls vs open sea with images {e:.2e}")
+        assert e < 1e-9
+    assert np.array_equal(np.isnan(lim), np.isnan(lim_o))
+    e = rel_per_entry(np.nan_to_num(lim.reshape(-1, 6)), np.nan_to_num(lim_o.reshape(-1, 6)))
+    say(f"  limits: walls vs open sea with images {e:.2e}")
+    assert e < 1e-9
