@@ -1836,7 +1836,40 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   HbgOptions o = defaults(options);
   use_device(s);
   const int N = s->N, H = n_headings, nrad = 6 * s->nb, D = nrad, R = nrad + H;
-  if (R > 32) throw Error{HBG_ERR_UNSUPPORTED, "more than 32 right-hand sides"};
+  if (R > 32) {
+    // more headings than the 32 right-hand sides of one system: the headings in groups, each
+    // solved with the radiation columns (the Haskind excitation of a heading takes them), the
+    // outputs of each group into the caller's arrays (the matrices are assembled and factored
+    // once per group)
+    const int hmax = 32 - nrad;
+    if (hmax < 1) throw Error{HBG_ERR_UNSUPPORTED, "more than 31 radiation columns (6 per body)"};
+    if (s->np > 0 || s->nk > 0)
+      throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions with more than 32 right-hand sides"};
+    const size_t F = (size_t)n_omegas;
+    std::vector<double> x, fk, sc, hk;
+    for (int h0 = 0; h0 < H; h0 += hmax) {
+      const int hc = std::min(hmax, H - h0);
+      x.assign(F * hc * D * 2, 0.0);
+      fk.assign(x.size(), 0.0);
+      sc.assign(x.size(), 0.0);
+      hk.assign(x.size(), 0.0);
+      const HbgStatus st = hbg_solve(s, omegas, n_omegas, headings_deg + h0, hc, options, h0 ? nullptr : added_mass,
+                                     h0 ? nullptr : damping, x.data(), fk.data(), sc.data(), hk.data(), progress, user);
+      if (st != HBG_OK) return st;
+      for (size_t f = 0; f < F; ++f)
+        for (int hh = 0; hh < hc; ++hh)
+          for (int d = 0; d < D; ++d)
+            for (int c = 0; c < 2; ++c) {
+              const size_t src = ((f * hc + hh) * D + d) * 2 + c, dst = ((f * H + h0 + hh) * D + d) * 2 + c;
+              if (excitation) excitation[dst] = x[src];
+              if (froude_krylov) froude_krylov[dst] = fk[src];
+              if (scattering) scattering[dst] = sc[src];
+              // the Haskind excitation [F][D][H]
+              if (haskind) haskind[((f * D + d) * H + h0 + hh) * 2 + c] = hk[((f * D + d) * hc + hh) * 2 + c];
+            }
+    }
+    return HBG_OK;
+  }
   check_depth(s);
   // the ordinary frequencies go through the batched path (om[i] is the caller's omegas[io[i]]);
   // the limits omega = 0 and infinity, infinite depth only, are solved after them (solve_limit)

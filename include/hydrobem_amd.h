@@ -203,6 +203,10 @@ HbgStatus hbg_kochin_shape(HbgSession* s, int32_t* n_omegas, int32_t* n_angles, 
  * added mass of a mode with a net volume flux through the body (heave of a surface-piercing body)
  * diverges and is not defined: its row and column are NaN (the reference refuses such a mode,
  * rules/coordinates.rs:15-75, at the same relative flux 1e-6).
+ * One system holds at most 32 right-hand sides (D radiation columns and H headings): with more,
+ * the headings are solved in groups of 32 - D, each with the radiation columns (the matrices
+ * assembled and factored once per group), and the outputs are the same as for one solve; not with
+ * field points or Kochin functions, and D > 31 (more than five bodies), HBG_ERR_UNSUPPORTED.
  * HBG_ERR_NUMERICAL on a zero pivot ("singular system matrix (zero pivot k)") or a non-finite
  * matrix or solution (linalg.rs:221-252); HBG_ERR_CANCELLED when the callback returns 0. */
 HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const double* headings_deg,
