@@ -2580,9 +2580,29 @@ HbgStatus hbg_motions(HbgSession* s, int32_t n_omegas, const double* omegas, int
     return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_motions");
   if (n_omegas < 0 || n_headings < 0 || n_dof < 1)
     return fail(HBG_ERR_VALUE, "negative count or no degree of freedom", "hbg_motions");
-  if (n_dof > motions_max_dof() || n_headings > motions_max_headings())
-    return fail(HBG_ERR_UNSUPPORTED, "more than 64 degrees of freedom or 32 headings", "hbg_motions");
+  if (n_dof > motions_max_dof())
+    return fail(HBG_ERR_UNSUPPORTED, "more than 64 degrees of freedom", "hbg_motions");
   if (n_omegas == 0 || n_headings == 0) return HBG_OK;
+  if (n_headings > motions_max_headings()) {
+    // the headings in groups of the kernel's right-hand sides (the excitation's rows of each group
+    // gathered, its motions scattered back)
+    const int hm = motions_max_headings();
+    const size_t F = n_omegas, D = n_dof, H = n_headings;
+    std::vector<double> x, r;
+    for (int h0 = 0; h0 < n_headings; h0 += hm) {
+      const size_t hc = std::min<size_t>(hm, H - h0);
+      x.assign(2 * F * hc * D, 0.0);
+      r.assign(x.size(), 0.0);
+      for (size_t f = 0; f < F; ++f)
+        std::memcpy(x.data() + 2 * f * hc * D, excitation + 2 * (f * H + h0) * D, 2 * hc * D * sizeof(double));
+      const HbgStatus st = hbg_motions(s, n_omegas, omegas, (int32_t)hc, n_dof, mass, stiffness, external_damping,
+                                       added_mass, damping, x.data(), r.data());
+      if (st != HBG_OK) return st;
+      for (size_t f = 0; f < F; ++f)
+        std::memcpy(rao + 2 * (f * H + h0) * D, r.data() + 2 * f * hc * D, 2 * hc * D * sizeof(double));
+    }
+    return HBG_OK;
+  }
   {
     // every input finite before anything runs (the elimination would carry a NaN into its pivots)
     const size_t F = n_omegas, D = n_dof, H = n_headings;

@@ -239,7 +239,7 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
  * rotation centres (the reference's motion_matrices, motion.rs:85-115, are the caller's);
  * added_mass, damping [F][D][D] and excitation [F][H][D] complex as hbg_solve gives them (a body
  * whose excitation is the Haskind force passes those columns instead). rao [F][H][D] complex.
- * D <= 64, H <= 32. HBG_ERR_NUMERICAL "motion equation at omega = w: singular system matrix
+ * D <= 64 (more headings than 32 are taken in groups of 32). HBG_ERR_NUMERICAL "motion equation at omega = w: singular system matrix
  * (zero pivot k)". */
 HbgStatus hbg_motions(HbgSession* s, int32_t n_omegas, const double* omegas, int32_t n_headings, int32_t n_dof,
                       const double* mass, const double* stiffness, const double* external_damping,

@@ -65,3 +65,24 @@ def test_motions_random_systems_and_zero_pivot():
         s.motions(res0, np.zeros((D, D)), np.zeros((D, D)))
     assert e.value.status == 8 and "motion equation at omega" in e.value.message and "zero pivot 1" in e.value.message
     s.close()
+
+
+def test_motions_with_more_headings_than_one_launch():
+    say("case: RAOs of random 12-mode systems, 4 frequencies, 70 headings (groups of 32)")
+    s = hb.Session(0)
+    rng = np.random.default_rng(11)
+    F, H, D = 4, 70, 12
+    res = {"omegas": rng.uniform(0.3, 2.0, F), "added_mass": rng.normal(size=(F, D, D)),
+           "radiation_damping": rng.normal(size=(F, D, D)),
+           "excitation_diffraction": rng.normal(size=(F, H, D)) + 1j * rng.normal(size=(F, H, D))}
+    M, C, Bx = rng.normal(size=(D, D)), rng.normal(size=(D, D)) + 5 * np.eye(D), rng.normal(size=(D, D))
+    rao = s.motions(res, M, C, Bx)
+    ref = reference(res, M, C, Bx)
+    err = np.abs(rao - ref).max() / np.abs(ref).max()
+    say(f"  max error / max |xi| {err:.2e}")
+    assert err < 1e-10
+    # each heading's motions do not depend on the other headings of its launch
+    part = {k: (v[:, 33:34] if k == "excitation_diffraction" else v) for k, v in res.items()}
+    one = s.motions(part, M, C, Bx)
+    assert np.array_equal(one[:, 0], rao[:, 33])
+    s.close()
