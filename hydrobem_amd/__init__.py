@@ -64,6 +64,7 @@ SIGNATURES = {
     "hbg_set_environment": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "hbg_set_panels": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip, ctypes.c_int32, _dp]),
     "hbg_set_interior": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip]),
+    "hbg_set_lids": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32, _ip, _dp]),
     "hbg_set_symmetry": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_set_walls": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "hbg_set_field_points": (ctypes.c_int32, [ctypes.c_void_p, _dp, ctypes.c_int32]),
@@ -199,6 +200,7 @@ class Session:
                                         _p(rc)))
         self.n_panels, self.n_bodies = n, nb
         self.n_interior = 0
+        self.n_lids = 0
 
     def set_interior(self, vertices=None, body=None):
         """The interior free surface (panels in z = 0 inside the waterlines, in the stored symmetry
@@ -209,6 +211,16 @@ class Session:
         _check(library().hbg_set_interior(self._h, _p(v) if len(v) else None, len(v),
                                           b.ctypes.data_as(_ip) if b is not None else None))
         self.n_interior = len(v)
+
+    def set_lids(self, vertices=None, damping=0.0, body=None):
+        """Damping lids (hbg_set_lids): panels in z = 0 on the outer free surface with the damping
+        factor eps (a scalar or one per panel; kappa = w^2 / (g + i w eps)); None: none."""
+        v = np.zeros((0, 4, 3)) if vertices is None else np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 4, 3)
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(damping, dtype=np.float64), (len(v),)))
+        b = None if body is None else np.ascontiguousarray(body, dtype=np.int32)
+        _check(library().hbg_set_lids(self._h, _p(v) if len(v) else None, len(v),
+                                      b.ctypes.data_as(_ip) if b is not None else None, _p(e) if len(v) else None))
+        self.n_lids = len(v)
 
     def set_symmetry(self, sym_x=False, sym_y=False):
         """Symmetry planes x = 0 (sym_x) and y = 0 (sym_y): the panels set are one sector of the

@@ -34,7 +34,7 @@ void launch_limit_matrices(const StaticD* st, const double2* inf, int N, int Nsr
                            hipStream_t stream);
 void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
                          double logc, const StaticD* st, double2* S, double2* A, float2* A32, int lda,
-                         double free_term, hipStream_t stream, int images = 1);
+                         double free_term, hipStream_t stream, int images = 1, double wg = 0.0);
 void launch_fin_points_eval(const FinFreq* f, FinTables F, WaveTables T, const double* R, const double* z,
                             const double* zeta, int n, double* out, hipStream_t stream);
 size_t fin_tables_work();
@@ -56,7 +56,7 @@ void launch_adaptive(const PanelD* P, const PanelD* Pf, int N, const double* gl,
                      unsigned count, hipStream_t stream);
 void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
                      const StaticD* st, double2* S, double2* A, float2* A32, int lda, double free_term,
-                     hipStream_t stream, int images = 1);
+                     hipStream_t stream, int images = 1, double wg = 0.0);
 void launch_wave_points(WaveTables T, const double* X, const double* Y, long n, double* out, hipStream_t stream);
 void launch_boundary(const PanelD* P, const int* body, const double* nj, int N, int nrad, int H, double w, double k,
                      double grav, double depth, const double* heads, double2* phiI, double2* dphiI, double2* u,
@@ -433,7 +433,9 @@ struct HbgSession {
   std::vector<double> chi{1.0}, chisym{1.0};  // [Gs][G], [Gs][Gs]
   std::vector<double> raw_vertices, raw_centres;  // the panels as set, to rebuild the images
   std::vector<int> raw_body;
-  std::vector<char> raw_interior;  // per panel: on the interior free surface (after the body panels)
+  std::vector<char> raw_interior;  // per panel: 1 interior free surface, 2 lid (after the body panels)
+  std::vector<double> raw_eps;     // per panel: a lid panel's damping factor (empty: none)
+  int NL = 0;                      // lid panels among the N
   int NI = 0;                      // panels of the interior free surface among the N (all images' sector)
   std::vector<PanelD> panels;
   std::vector<int> body;
@@ -668,7 +670,7 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   if (!std::isfinite(s->depth)) {
     blocks([&](double2* S, double2* Ab, float2* a32, int images, int ld) {
       launch_assemble(s->d_panels.p, s->d_panels.p, N, K, logc, s->T, s->d_st.p, S, Ab, a32, ld, 2.0 * PI, s->stream,
-                      images);
+                      images, std::sqrt(K / s->grav));
     });
     return K;
   }
@@ -696,7 +698,7 @@ double assemble_frequency(HbgSession* s, double K, double logc, double2* Mout, b
   } else
     blocks([&](double2* S, double2* Ab, float2* a32, int images, int ld) {
       launch_assemble_fin(s->d_panels.p, s->d_panels.p, N, dff, fin_tables(s), s->T, logc, s->d_st.p, S, Ab, a32, ld,
-                          2.0 * PI, s->stream, images);
+                          2.0 * PI, s->stream, images, std::sqrt(K / s->grav));
     });
   return f.k0;
 }
@@ -1003,7 +1005,8 @@ HbgStatus hbg_set_environment(HbgSession* s, double density, double gravity, dou
 struct RawPanels {
   std::vector<double> vertices, centres;
   std::vector<int> body;
-  std::vector<char> interior;  // per panel: a panel of the interior free surface (after the body's)
+  std::vector<char> interior;  // per panel: 1 interior free surface, 2 lid (after the body's), 0 body
+  std::vector<double> eps;     // per panel: a lid panel's damping factor (empty: none)
   int N = 0, nb = 0, NI = 0;
 };
 
@@ -1022,6 +1025,7 @@ static RawPanels raw_of(const HbgSession* s) {
   r.centres = s->raw_centres;
   r.body = s->raw_body;
   r.interior = s->raw_interior;
+  r.eps = s->raw_eps;
   r.N = s->N;
   r.nb = s->nb;
   r.NI = s->NI;
@@ -1094,7 +1098,8 @@ static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, 
       panels[o] = panel_record(proj, normal, centroid, area);
       if (inner) {
         panels[o].c[2] = 0.0;
-        panels[o].interior = 1.0;
+        panels[o].interior = (double)raw.interior[j];
+        panels[o].lid_eps = raw.eps.empty() ? 0.0 : raw.eps[j];
       }
       const int bi = raw.body[j];
       if (bi < 0 || bi >= n_bodies) throw Error{HBG_ERR_VALUE, "panel " + std::to_string(j + 1) + ": body out of range"};
@@ -1149,36 +1154,63 @@ static void set_geometry(HbgSession* s, const Group& grp, int sym_x, int sym_y, 
   s->raw_centres = raw.centres;
   s->raw_body = raw.body;
   s->raw_interior = raw.interior;
+  s->raw_eps = raw.eps;
+  s->NL = (int)std::count(raw.interior.begin(), raw.interior.end(), (char)2);
   s->N = n_panels;
   s->nb = n_bodies;
   s->NI = raw.NI;
 }
 
-HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body) {
-  if (!s || (n_panels > 0 && !vertices)) return fail(HBG_ERR_ARGUMENT, "null argument", "hbg_set_interior");
-  if (n_panels < 0) return fail(HBG_ERR_ARGUMENT, "negative count", "hbg_set_interior");
-  if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", "hbg_set_interior");
+// The panels in z = 0 after the body's: those of one kind (1 interior free surface, 2 lids) replaced
+// by the given ones, those of the other kind kept. eps: the lids' damping factors (NULL: 0).
+static HbgStatus set_sheet(HbgSession* s, char kind, const double* vertices, int32_t n_panels, const int32_t* body,
+                           const double* eps, const char* where) {
+  if (!s || (n_panels > 0 && !vertices)) return fail(HBG_ERR_ARGUMENT, "null argument", where);
+  if (n_panels < 0) return fail(HBG_ERR_ARGUMENT, "negative count", where);
+  if (s->N == 0) return fail(HBG_ERR_STATE, "no panels set", where);
   GUARD_BEGIN
-  // the body panels as set, then the new interior panels (z = 0 in body-local coordinates)
-  RawPanels raw = raw_of(s);
+  const char* what = kind == 1 ? "interior panel " : "lid panel ";
+  const RawPanels raw = raw_of(s);
   const int nbody = raw.N - raw.NI;
-  raw.vertices.resize(12 * (size_t)nbody);
-  raw.body.resize(nbody);
-  raw.interior.assign(nbody, 0);
+  RawPanels out = raw;
+  out.vertices.resize(12 * (size_t)nbody);
+  out.body.resize(nbody);
+  out.interior.assign(nbody, 0);
+  out.eps.assign(nbody, 0.0);
+  for (int j = nbody; j < raw.N; ++j)
+    if (raw.interior[j] != kind) {
+      out.vertices.insert(out.vertices.end(), raw.vertices.begin() + 12 * (size_t)j,
+                          raw.vertices.begin() + 12 * (size_t)j + 12);
+      out.body.push_back(raw.body[j]);
+      out.interior.push_back(raw.interior[j]);
+      out.eps.push_back(raw.eps.empty() ? 0.0 : raw.eps[j]);
+    }
   for (int j = 0; j < n_panels; ++j) {
-    const int bi = body ? body[j] : 0;
     for (int q = 0; q < 4; ++q)
       if (std::fabs(vertices[12 * (size_t)j + 3 * q + 2]) > 1e-9)
-        throw Error{HBG_ERR_VALUE, "interior panel " + std::to_string(j + 1) + ": a vertex off z = 0"};
-    raw.vertices.insert(raw.vertices.end(), vertices + 12 * (size_t)j, vertices + 12 * (size_t)j + 12);
-    raw.body.push_back(bi);
-    raw.interior.push_back(1);
+        throw Error{HBG_ERR_VALUE, what + std::to_string(j + 1) + ": a vertex off z = 0"};
+    const double e = eps ? eps[j] : 0.0;
+    if (!(e >= 0.0) || !std::isfinite(e))
+      throw Error{HBG_ERR_VALUE, what + std::to_string(j + 1) + ": damping factor must be finite and >= 0"};
+    out.vertices.insert(out.vertices.end(), vertices + 12 * (size_t)j, vertices + 12 * (size_t)j + 12);
+    out.body.push_back(body ? body[j] : 0);
+    out.interior.push_back(kind);
+    out.eps.push_back(e);
   }
-  raw.N = nbody + n_panels;
-  raw.NI = n_panels;
-  set_geometry(s, make_group(s->sym_x, s->sym_y, s->walls), s->sym_x, s->sym_y, s->walls, raw);
+  out.N = (int)out.body.size();
+  out.NI = out.N - nbody;
+  set_geometry(s, make_group(s->sym_x, s->sym_y, s->walls), s->sym_x, s->sym_y, s->walls, out);
   return HBG_OK;
-  GUARD_END("hbg_set_interior")
+  GUARD_END(where)
+}
+
+HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body) {
+  return set_sheet(s, 1, vertices, n_panels, body, nullptr, "hbg_set_interior");
+}
+
+HbgStatus hbg_set_lids(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
+                       const double* damping_factor) {
+  return set_sheet(s, 2, vertices, n_panels, body, damping_factor, "hbg_set_lids");
 }
 
 HbgStatus hbg_set_symmetry(HbgSession* s, int32_t sym_x, int32_t sym_y) {
@@ -1888,6 +1920,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
   if (s->NI > 0 && o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
   if ((s->nk > 0 || s->np > 0) && !il.empty())
     throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions at the frequency limits"};
+  if (s->np > 0 && s->NL > 0) throw Error{HBG_ERR_UNSUPPORTED, "field points with lids"};
   if (s->np > 0 && std::isfinite(s->depth))
     for (int i = 0; i < s->np; ++i)
       if (s->fpts[3 * i + 2] < -s->depth)

@@ -27,22 +27,35 @@ void launch_near_rows(const int* rowptr, const int2* ent, const double2* d, size
 // workgroup takes a 64 x 64 tile (k-block i <= n-block j; the diagonal tiles the pairs k <= n),
 // the 64 lanes of a wavefront along k and 16 n per wavefront, and writes (k, n) (coalesced) and
 // (n, k) (one column per lane; the wavefront's consecutive n fill the same cache lines).
-// rint, cint: the row's and the column's panel lie on the interior free surface (kind 1): the row's
-// free term is -4 pi (-2 free_term; Lee, Newman & Zhu 1996 eq. 25, lo_frequency.rs:85-97) and
-// the column's "dipole" is K S (dG/dzeta = K G on zeta = 0, low_order.rs:771-785)
+// rint, cint: the kind of the row's and of the column's panel (0 body, 1 interior free surface,
+// 2 lid). The row's free term is -4 pi on the interior free surface (-2 free_term; Lee, Newman &
+// Zhu 1996 eq. 25) and 4 pi on a lid (lo_frequency.rs:85-97); the column's "dipole" is K S on the
+// interior free surface (dG/dzeta = K G on zeta = 0, low_order.rs:771-785) and -(K - kappa) S on a
+// lid, kappa = w^2 / (g + i w eps) = K / (1 + i eps w / g) with the column panel's eps (ceps) and
+// wg = w / g (low_order.rs:811-819, solve_common.rs:52-54); eps = 0 gives kappa = K, a column 0.
+__device__ __forceinline__ double2 lid_column_coef(double K, double ceps, double wg) {
+  const double e = ceps * wg, d = 1.0 / (1.0 + e * e);
+  return make_double2(K * d - K, -K * e * d);  // kappa - K, kappa = K (1 - i e) / (1 + e^2)
+}
+
 __device__ __forceinline__ void put_entry(double2* __restrict__ S, double2* __restrict__ A, float2* __restrict__ A32,
                                           int lda, int row, int col, const StaticD& e, double Ar, double nz, double a,
                                           double c2, double Wr, double WrR, double WrZ, double Wi, double WiR,
-                                          double WiZ, double free_term, bool rint = false, bool cint = false,
-                                          double K = 0.0) {
+                                          double WiZ, double free_term, double rint = 0.0, double cint = 0.0,
+                                          double K = 0.0, double ceps = 0.0, double wg = 0.0) {
   const double2 sv = make_double2(e.rs + Ar * Wr - c2 * e.dl, Ar * Wi);
   S[(size_t)col * lda + row] = sv;
   double dre = e.rd + Ar * (-WrR * a + WrZ * nz) - c2 * e.dd, dim = Ar * (-WiR * a + WiZ * nz);
-  if (cint) {
+  if (cint == 1.0) {
     dre = K * sv.x;
     dim = K * sv.y;
+  } else if (cint != 0.0) {
+    const double2 c = lid_column_coef(K, ceps, wg);
+    dre = c.x * sv.x - c.y * sv.y;
+    dim = c.x * sv.y + c.y * sv.x;
   }
-  const double2 v = make_double2((row == col ? (rint ? -2.0 * free_term : free_term) : 0.0) - dre, -dim);
+  const double ft = rint == 0.0 ? free_term : (rint == 1.0 ? -2.0 * free_term : 2.0 * free_term);
+  const double2 v = make_double2((row == col ? ft : 0.0) - dre, -dim);
   A[(size_t)col * lda + row] = v;
   if (A32) A32[(size_t)col * lda + row] = float2((float)v.x, (float)v.y);
 }
@@ -53,7 +66,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
                                                        int N, double K, double logc, WaveTables T,
                                                        const StaticD* __restrict__ st, double2* __restrict__ S,
                                                        double2* __restrict__ A, float2* __restrict__ A32, int lda,
-                                                       double free_term) {
+                                                       double free_term, double wg) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
   if (blockIdx.z) {  // image block g of a symmetric body: sources g n, no free term
@@ -95,9 +108,8 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
       const double FY = -w.F - 2.0 / R0, IY = -w.I;
       double Wr = K * w.F, WrR = K2 * w.FX, WrZ = -K2 * FY;
       double Wi = K * w.I, WiR = K2 * w.IX, WiZ = -K2 * IY;
-      const bool ik = pk->interior != 0.0, in = pn->interior != 0.0;  // on the interior free surface
       if (R == 0.0 && pk->c[2] == 0.0 && pn->c[2] == 0.0) {
-        // coincident points on z = 0 (an interior panel and its own centroid): the regular part
+        // coincident points on z = 0 (a panel in z = 0 and its own centroid): the regular part
         // K (Ft + 2 psi), psi = log(r' + |z + zeta|), tends to -2 K (gamma - log 2 + log K)
         // (low_order.rs:634-643); the log part is integrated in the statics, the columns of such
         // panels are K S (the derivatives are not used)
@@ -107,11 +119,11 @@ __global__ void __launch_bounds__(256) assemble_kernel(const PanelD* __restrict_
       }
       const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
       put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, Wr, WrR, WrZ, Wi, WiR, WiZ,
-                free_term, ik, in, K);
+                free_term, pk->interior, pn->interior, K, pn->lid_eps, wg);
       if (k != n) {
         const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
         put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, Wr, WrR, WrZ, Wi, WiR,
-                  WiZ, free_term, in, ik, K);
+                  WiZ, free_term, pn->interior, pk->interior, K, pk->lid_eps, wg);
       }
     }
   }
@@ -131,7 +143,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
                                                            FinTables F, WaveTables T, double logc,
                                                            const StaticD* __restrict__ st, double2* __restrict__ S,
                                                            double2* __restrict__ A, float2* __restrict__ A32,
-                                                           int lda, double free_term) {
+                                                           int lda, double free_term, double wg) {
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
   if (blockIdx.z) {  // image block g of a symmetric body: sources g n, no free term
@@ -173,14 +185,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) a
     const double R = sqrt(dx * dx + dy * dy);
     const FinK w = fin_kernel_u(f, F, T, valid, k, n, R, pk->c[2], pn->c[2]);
     if (valid) {
-      const bool ik = pk->interior != 0.0, in = pn->interior != 0.0;  // on the interior free surface
       const double an = R > 0.0 ? (dx * pn->n[0] + dy * pn->n[1]) / R : 0.0;
       put_entry(S, A, A32, lda, k, n, st[(size_t)n * N + k], pn->area, pn->n[2], an, c2, w.Wr, w.WrR, w.WrZ, w.Wi,
-                w.WiR, w.WiZ, free_term, ik, in, f->K);
+                w.WiR, w.WiZ, free_term, pk->interior, pn->interior, f->K, pn->lid_eps, wg);
       if (k != n) {
         const double ak = R > 0.0 ? -(dx * pk->n[0] + dy * pk->n[1]) / R : 0.0;
         put_entry(S, A, A32, lda, n, k, st[(size_t)k * N + n], pk->area, pk->n[2], ak, c2, w.Wr, w.WrR, w.Wrz, w.Wi,
-                  w.WiR, w.Wiz, free_term, in, ik, f->K);
+                  w.WiR, w.Wiz, free_term, pn->interior, pk->interior, f->K, pk->lid_eps, wg);
       }
     }
   }
@@ -272,7 +283,16 @@ __global__ void boundary_kernel(const PanelD* __restrict__ P, const int* __restr
     }
     double2 dn = make_double2(g0.x * p->n[0] + g1.x * p->n[1] + g2.x * p->n[2],
                               g0.y * p->n[0] + g1.y * p->n[1] + g2.y * p->n[2]);
-    if (p->interior != 0.0) dn = make_double2(0.0, 0.0);  // the interior free surface carries no velocity
+    if (p->interior == 1.0) dn = make_double2(0.0, 0.0);  // the interior free surface carries no velocity
+    if (p->interior == 2.0) {
+      // a lid: the condition phi_z - kappa phi = 0 of the total potential leaves the boundary value
+      // u = (K - kappa) phi_I of the scattered one (lo_frequency.rs:263-268), carried here as
+      // dphi_I/dn = -(K - kappa) phi_I (then u = -dphi_I/dn, and the Haskind sum's -phi_c dphi_I/dn
+      // is the lid term (K - kappa) phi_c phi_I of lo_frequency.rs:1150-1156)
+      const double K = w * w / grav, e = p->lid_eps * w / grav, d = 1.0 / (1.0 + e * e);
+      const double cx = K * d - K, cy = -K * e * d;  // kappa - K
+      dn = make_double2(cx * phi.x - cy * phi.y, cx * phi.y + cy * phi.x);
+    }
     phiI[(size_t)h * N + j] = phi;
     dphiI[(size_t)h * N + j] = dn;
     u[(size_t)(nrad + h) * N + j] = make_double2(-dn.x, -dn.y);
@@ -486,20 +506,21 @@ __global__ void __launch_bounds__(256) forces_kernel(const PanelD* __restrict__ 
 // one block alone would not (528 workgroups at order 2048).
 void launch_assemble(const PanelD* P, const PanelD* Psrc, int N, double K, double logc, WaveTables T,
                      const StaticD* st, double2* S, double2* A, float2* A32, int lda, double free_term,
-                     hipStream_t stream, int images) {
+                     hipStream_t stream, int images, double wg) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
   if (images > 1 && A32) throw Error{6, "image blocks without a complex64 copy"};
-  assemble_kernel<<<dim3(nt, nt, images), 256, 0, stream>>>(P, Psrc, N, K, logc, T, st, S, A, A32, lda, free_term);
+  assemble_kernel<<<dim3(nt, nt, images), 256, 0, stream>>>(P, Psrc, N, K, logc, T, st, S, A, A32, lda, free_term,
+                                                            wg);
   HBG_HIP(hipGetLastError());
 }
 
 void launch_assemble_fin(const PanelD* P, const PanelD* Psrc, int N, const FinFreq* f, FinTables F, WaveTables T,
                          double logc, const StaticD* st, double2* S, double2* A, float2* A32, int lda,
-                         double free_term, hipStream_t stream, int images) {
+                         double free_term, hipStream_t stream, int images, double wg) {
   const unsigned nt = (unsigned)((N + ASM_TILE - 1) / ASM_TILE);
   if (images > 1 && A32) throw Error{6, "image blocks without a complex64 copy"};
   assemble_fin_kernel<<<dim3(nt, nt, images), 256, 0, stream>>>(P, Psrc, N, f, F, T, logc, st, S, A, A32, lda,
-                                                                free_term);
+                                                                free_term, wg);
   HBG_HIP(hipGetLastError());
 }
 

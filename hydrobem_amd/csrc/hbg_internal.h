@@ -29,8 +29,10 @@ struct PanelD {
   double q4[4][3];  // the four projected corners as given (bilinear map of the 3 x 3 far rule)
   double nv;        // 3 or 4 distinct vertices
   double top;       // largest vertex z
-  double interior;  // 1: a panel of the interior free surface (in z = 0, normal +z), 0: a body panel
-  double pad[5];
+  double interior;  // 1: a panel of the interior free surface, 2: a lid panel (both in z = 0, normal +z),
+                    // 0: a body panel
+  double lid_eps;   // a lid panel's damping factor eps (kappa = w^2 / (g + i w eps)), else 0
+  double pad[4];
 };
 static_assert(sizeof(PanelD) == 56 * sizeof(double), "PanelD layout");
 

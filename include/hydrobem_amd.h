@@ -119,6 +119,23 @@ HbgStatus hbg_set_panels(HbgSession* s, const double* vertices, int32_t n_panels
  * Invalidates the statics. */
 HbgStatus hbg_set_interior(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body);
 
+/* Damping lids (the reference's lids, SurfaceKind::Lid with a damping factor; lo_frequency.rs:85-97,
+ * 263-268, 1150-1156; low_order.rs:811-819; solve_common.rs:52-54): panels in z = 0 on the outer
+ * free surface (a moonpool, the gap between two hulls), vertices [n_panels][4][3] as for
+ * hbg_set_interior, body [n_panels] (NULL: body 0), damping_factor [n_panels] (eps >= 0, NULL: 0).
+ * Extra unknowns with free term 4 pi and columns -(K - kappa) S, kappa = w^2 / (g + i w eps); the
+ * diffraction boundary value (K - kappa) phi_I; the Haskind sum takes (K - kappa) phi_i phi_I over
+ * them. eps = 0 is the free surface itself (kappa = K: the lid decouples, the body's results are
+ * those without it). No velocity, no forces; Kochin functions leave them out (lo_frequency.rs:
+ * 1279-1372 takes no lid); at the frequency limits their unknowns are not used (identity rows,
+ * lo_frequency.rs:99-104). They replace the lids set before and keep the interior free surface
+ * (and hbg_set_interior keeps the lids); n_panels 0 removes them, hbg_set_panels too. Field points
+ * with lids and the surrogate with them are HBG_ERR_UNSUPPORTED; HBG_PRECISION_MIXED32 solves them
+ * as HBG_PRECISION_MIXED. HBG_ERR_VALUE "lid panel i: a vertex off z = 0" / "damping factor must be
+ * finite and >= 0". Invalidates the statics. */
+HbgStatus hbg_set_lids(HbgSession* s, const double* vertices, int32_t n_panels, const int32_t* body,
+                       const double* damping_factor);
+
 /* Symmetry planes of the wetted surface (the mesh file's ISX, ISY; mesh.rs symmetry "yz", "xz",
  * "xz_and_yz"): sym_x 1 reflects the panels about x = 0, sym_y 1 about y = 0 (each 0 or 1). The
  * panels given to hbg_set_panels are then one sector; the solve adds their images (geometry.rs:
