@@ -1620,7 +1620,7 @@ HbgStatus hbg_assemble(HbgSession* s, double omega, const HbgOptions* options, d
   if (m32) {
     // the matrices of the FP32 assembly: its complex64 entries, the FP64 values at the near entries
     if (o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with the surrogate"};
-    if (s->NI > 0) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with an interior free surface"};
+    if (s->NI > 0) throw Error{HBG_ERR_UNSUPPORTED, "the FP32 assembly with panels in z = 0 (an interior free surface or lids)"};
     if (!fp32_in_range(s, K)) throw Error{HBG_ERR_UNSUPPORTED, "pairs beyond the FP32 tables at this frequency"};
     ensure_fp32_geometry(s, o.logarithmic ? 1 : 0, o.log_tolerance);
     if (std::isfinite(s->depth)) {
@@ -1917,7 +1917,7 @@ HbgStatus hbg_solve(HbgSession* s, const double* omegas, int32_t n_omegas, const
       want_inf = want_inf || std::isinf(omegas[i]);
     }
   }
-  if (s->NI > 0 && o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
+  if (s->NI > 0 && o.surrogate) throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with panels in z = 0 (an interior free surface or lids)"};
   if ((s->nk > 0 || s->np > 0) && !il.empty())
     throw Error{HBG_ERR_UNSUPPORTED, "field points or Kochin functions at the frequency limits"};
   if (s->np > 0 && s->NL > 0) throw Error{HBG_ERR_UNSUPPORTED, "field points with lids"};
@@ -2388,7 +2388,7 @@ HbgStatus hbg_solve_group(HbgSession* const* sessions, int32_t count, const doub
   HbgOptions o = defaults(options);
   for (int i = 0; i < count; ++i)
     if (o.surrogate && sessions[i]->NI > 0)
-      throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with an interior free surface"};
+      throw Error{HBG_ERR_UNSUPPORTED, "the surrogate with panels in z = 0 (an interior free surface or lids)"};
   use_device(lead);
   const int H = n_headings;
   int nmax = 0, Rmax = 0;
